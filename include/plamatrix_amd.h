@@ -1,0 +1,217 @@
+/*
+ * plamatrix_amd.h -- C ABI of the MI355X (gfx950) sparse solve library.
+ *
+ * This is the drop-in boundary under PlaMatrix's C++ API (COOMatrix /
+ * CSRMatrix, cooToCsr, spmv, cg / pcg / blockPcg, the fixed-iteration async
+ * solves and the damped Schur-complement solve).  The C++ headers in
+ * include/plamatrix/ implement those templates on top of these entry points;
+ * any other host language binds the same symbols.
+ *
+ * Conventions
+ *   - All array arguments are device pointers unless stated otherwise.
+ *   - Every entry returns a pm_status.  On failure pm_last_error() holds the
+ *     message for the calling thread.  Codes map onto the reference's
+ *     exception classes (std::invalid_argument, std::out_of_range, ...).
+ *   - Work is enqueued on `stream` (NULL = the legacy default stream) and is
+ *     stream ordered; entries that must read results on the host say so.
+ *   - A pm_workspace is not thread-safe and binds to the first stream it is
+ *     used on, as SparseOpsWorkspace / IterativeSolverWorkspace do
+ *     (sparse_ops.h:13-16, iterative_solver.h:29-33).
+ */
+#ifndef PLAMATRIX_AMD_H
+#define PLAMATRIX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pm_status {
+    PM_OK = 0,
+    PM_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+    PM_OUT_OF_RANGE = 2,     /* std::out_of_range */
+    PM_OVERFLOW = 3,         /* std::overflow_error */
+    PM_LOGIC_ERROR = 4,      /* std::logic_error (stream / ownership misuse, pending work) */
+    PM_RUNTIME_ERROR = 5,    /* std::runtime_error (breakdown, non-convergence, non-SPD) */
+    PM_DEVICE_ERROR = 6      /* HIP runtime error (std::runtime_error in C++) */
+} pm_status;
+
+typedef enum pm_dtype { PM_F32 = 0, PM_F64 = 1 } pm_dtype;
+
+/* Message of the last failing call on this thread ("" when none). */
+const char* pm_last_error(void);
+int pm_version(void);
+
+/* ---- CSR descriptor ---------------------------------------------------------
+ * Mirrors CSRMatrix<Scalar, Device::GPU> (sparse/csr_matrix.h:22-825): int64
+ * row offsets and column indices, values of `dtype`.  The optional 32-bit
+ * mirrors (filled by pm_csr_compact_indices) are what the kernels stream when
+ * present: 4 instead of 8 bytes per index on the bandwidth-bound path. */
+typedef struct pm_csr {
+    int64_t rows;
+    int64_t cols;
+    int64_t nnz;
+    const int64_t* row_offsets;
+    const int64_t* col_indices;
+    const void* values;
+    int32_t dtype;
+    const int32_t* row_offsets32;
+    const int32_t* col_indices32;
+} pm_csr;
+
+/* Workspace: device scratch, reduction partials, solver state, sort buffers
+ * and cached launch plans.  Replaces SparseOpsWorkspace (sparse_ops.h:13-77)
+ * and IterativeSolverWorkspace<S> (iterative_solver.h:29-62). */
+typedef struct pm_workspace pm_workspace;
+int pm_workspace_create(pm_workspace** ws);
+/* Synchronises the bound stream, then frees everything. */
+int pm_workspace_destroy(pm_workspace* ws);
+/* closeAsyncAllocation(): releases storage; PM_LOGIC_ERROR while work that
+ * uses it is still pending on the bound stream. */
+int pm_workspace_close(pm_workspace* ws);
+size_t pm_workspace_capacity_bytes(const pm_workspace* ws);
+/* Vector capacity (rows) of the iterative-solver storage (capacitySize()). */
+int64_t pm_workspace_capacity_rows(const pm_workspace* ws);
+
+/* Structural validation on the device (CSRMatrix::validateStructure,
+ * csr_matrix.h:197-237, 633-667): offsets start at 0, end at nnz and are
+ * monotone, columns in [0, cols), values finite.  Synchronises `stream`;
+ * PM_INVALID_ARGUMENT names the first offending row or position. */
+int pm_csr_validate(const pm_csr* a, pm_workspace* ws, hipStream_t stream);
+
+/* Fill 32-bit index mirrors (rows+1 and nnz entries).  PM_OVERFLOW when
+ * nnz or cols do not fit in int32. */
+int pm_csr_compact_indices(const pm_csr* a, int32_t* row_offsets32, int32_t* col_indices32,
+                           hipStream_t stream);
+
+/* ---- COO -> CSR ---------------------------------------------------------------
+ * Replaces cooToCsr / cooToCsrAsync for device triplets (sparse_ops.h:94-120,
+ * coo_to_csr.cu:430-576).  Stable (row, col) order and left fold of
+ * duplicates in input order starting from the first value, bit-identical to
+ * the CPU cooToCsr (sparse_ops_cpu.cpp:24-134).  Triplets are not modified.
+ * Limits: count < 2^31, ceil(log2 rows) + ceil(log2 cols) <= 64. */
+enum { PM_COO_CHECK_FINITE = 1 };
+
+/* Validate, sort and count (asynchronous). */
+int pm_coo_to_csr_prepare(int64_t rows, int64_t cols, int64_t count, const int64_t* row_indices,
+                          const int64_t* col_indices, const void* values, int32_t dtype, int32_t flags,
+                          pm_workspace* ws, hipStream_t stream);
+/* Synchronise and return the combined nnz of the prepared conversion; raises
+ * the deferred coordinate / value errors (lowest triplet position first). */
+int pm_coo_to_csr_nnz(pm_workspace* ws, int64_t* nnz);
+/* Fold duplicates and write CSR arrays of exactly expected_nnz entries
+ * (asynchronous; a mismatch is reported by pm_workspace_check_status). */
+int pm_coo_to_csr_finish(int64_t* row_offsets, int64_t* col_indices, void* values, int64_t expected_nnz,
+                         pm_workspace* ws, hipStream_t stream);
+/* prepare + finish in one asynchronous call (cooToCsrAsync). */
+int pm_coo_to_csr_async(int64_t rows, int64_t cols, int64_t count, const int64_t* row_indices,
+                        const int64_t* col_indices, const void* values, int32_t dtype, int32_t flags,
+                        int64_t* row_offsets, int64_t* out_col_indices, void* out_values, int64_t expected_nnz,
+                        pm_workspace* ws, hipStream_t stream);
+/* SparseOpsWorkspace::checkStatus (sparse_ops.h:27): PM_LOGIC_ERROR while the
+ * conversion is still running, else the deferred error, else PM_OK. */
+int pm_workspace_check_status(pm_workspace* ws);
+
+/* ---- SpMV -------------------------------------------------------------------
+ * y = A x (spmvAsync, sparse_ops.h:158-166; sparse_ops.cu:687-740).  x and y
+ * have a->dtype; accumulation in that precision.  x and y must not alias. */
+int pm_spmv(const pm_csr* a, const void* x, void* y, pm_workspace* ws, hipStream_t stream);
+/* Same with an explicit lanes-per-row choice (1..64, power of two); used by
+ * the autotuner. */
+int pm_spmv_lanes(const pm_csr* a, const void* x, void* y, int32_t lanes_per_row, pm_workspace* ws,
+                  hipStream_t stream);
+/* Autotuner (no reference counterpart): times every lanes-per-row variant of
+ * y = A x on this matrix (median of `repeats` launches) and records the
+ * fastest for its shape (rows, nnz, dtype, index width) in memory and in the
+ * cache file named by PLAMATRIX_AMD_TUNE_CACHE.  Synchronises `stream`. */
+int pm_autotune_spmv(const pm_csr* a, const void* x, void* y, int32_t repeats, int32_t* best_lanes,
+                     float* best_ms, hipStream_t stream);
+/* The lanes-per-row the kernels will use for this matrix (-1 on error). */
+int pm_tuned_lanes(const pm_csr* a);
+/* Record a lanes-per-row choice (1..64, power of two) for this matrix shape in
+ * the same table the autotuner fills. */
+int pm_tune_record(const pm_csr* a, int32_t lanes_per_row);
+/* FP64 true relative residual ||b - A x|| / ||b|| (||b - A x|| if b = 0).
+ * vec_dtype is the dtype of b and x (PM_F64 allowed with a float matrix).
+ * Synchronises `stream`. */
+int pm_true_relres(const pm_csr* a, const void* b, const void* x, int32_t vec_dtype, double* relres,
+                   pm_workspace* ws, hipStream_t stream);
+
+/* ---- CG / PCG -----------------------------------------------------------------
+ * Mirrors IterativeSolverOptions / IterativeSolverReport
+ * (iterative_solver.h:10-26). */
+typedef struct pm_solver_options {
+    int32_t max_iterations;
+    double relative_tolerance;
+    double absolute_tolerance;
+    int32_t use_jacobi;
+    int32_t require_convergence;
+} pm_solver_options;
+
+typedef struct pm_solver_report {
+    int32_t converged;
+    int32_t iterations;
+    double initial_residual;
+    double final_residual;
+} pm_solver_report;
+
+/* Execution policy (no counterpart in the reference). Zero-initialised means
+ * the defaults. */
+typedef struct pm_solve_policy {
+    int32_t chunk_iterations;  /* iterations launched per device-side chunk (0: 16) */
+    int32_t use_graph;         /* 1: replay chunks as HIP graphs, 0: direct launches, -1: default */
+    int32_t refresh_interval;  /* recompute r = b - A x (FP64 accumulation) every k iterations; 0: never */
+    int32_t lanes_per_row;     /* SpMV lanes per row; 0: autotuned / heuristic */
+} pm_solve_policy;
+
+/* Adaptive solves (cg / pcg / blockPcg on the GPU, iterative_solver.h:110-141;
+ * iterative_solver.cu:262-402).  Convergence is decided on the device; the
+ * host polls one small status record per chunk instead of every iteration.
+ * x holds the initial guess and receives the solution.  Arithmetic: vectors
+ * and SpMV in a->dtype, every dot product and alpha/beta in FP64. */
+int pm_cg(const pm_csr* a, const void* rhs, void* x, const pm_solver_options* options,
+          const pm_solve_policy* policy, pm_solver_report* report, pm_workspace* ws, hipStream_t stream);
+int pm_pcg(const pm_csr* a, const void* rhs, void* x, const pm_solver_options* options,
+           const pm_solve_policy* policy, pm_solver_report* report, pm_workspace* ws, hipStream_t stream);
+/* inverse_blocks: rows/block_size row-major block_size^2 blocks (rows*block_size
+ * entries), block_size <= 8. */
+int pm_block_pcg(const pm_csr* a, const void* rhs, void* x, const void* inverse_blocks, int64_t block_size,
+                 const pm_solver_options* options, const pm_solve_policy* policy, pm_solver_report* report,
+                 pm_workspace* ws, hipStream_t stream);
+
+/* Fixed-iteration asynchronous solves (cgFixedIterationsAsync /
+ * pcgFixedIterationsAsync / finalizeIterativeSolverReport,
+ * iterative_solver.h:143-171).  Exactly `iterations` steps are enqueued with
+ * no host synchronisation; the state records the initial and final squared
+ * residuals on the device and an event. */
+typedef struct pm_fixed_state pm_fixed_state;
+int pm_solve_fixed_async(const pm_csr* a, const void* rhs, void* x, int32_t iterations, int32_t preconditioned,
+                         pm_fixed_state** state, pm_workspace* ws, hipStream_t stream);
+/* PM_LOGIC_ERROR if the stream has not completed the state's work. */
+int pm_fixed_finalize(const pm_fixed_state* state, const pm_solver_options* options, pm_solver_report* report);
+int pm_fixed_state_destroy(pm_fixed_state* state);
+
+/* Mixed precision: FP32 matrix and inner Jacobi-PCG (FP64 dots, periodic
+ * FP64-accumulated residual refresh), FP64 solution and outer iterative
+ * refinement until the FP64 true relative residual reaches target_relres.
+ * rhs and x are FP64 device vectors. */
+typedef struct pm_refine_report {
+    int32_t converged;
+    int32_t outer_iterations;
+    int32_t inner_iterations;   /* total over all outer steps */
+    double initial_relres;
+    double final_relres;        /* FP64 ||b - A x|| / ||b|| */
+} pm_refine_report;
+int pm_pcg_refine(const pm_csr* a, const double* rhs, double* x, const pm_solver_options* inner,
+                  double target_relres, int32_t max_outer, const pm_solve_policy* policy,
+                  pm_refine_report* report, pm_workspace* ws, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

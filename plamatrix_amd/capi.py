@@ -1,0 +1,336 @@
+"""ctypes binding of the C ABI in ``include/plamatrix_amd.h``.
+
+Device memory comes from torch tensors (``tensor.data_ptr()``) and work is
+enqueued on torch's current HIP stream.  Every call goes through
+``libplamatrix_amd.so``; if the library is missing, ``lib()`` raises instead of
+falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libplamatrix_amd.so")
+
+PM_F32, PM_F64 = 0, 1
+PM_COO_CHECK_FINITE = 1
+
+_i32, _i64, _f64, _p = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+KINDS = {1: "invalid_argument", 2: "out_of_range", 3: "overflow_error", 4: "logic_error",
+         5: "runtime_error", 6: "device_error"}
+
+
+class PlamatrixError(Exception):
+    def __init__(self, code: int, message: str):
+        self.code = code
+        self.kind = KINDS.get(code, "error")
+        super().__init__(f"{self.kind}: {message}")
+
+
+class pm_csr(ctypes.Structure):
+    _fields_ = [("rows", _i64), ("cols", _i64), ("nnz", _i64), ("row_offsets", _p), ("col_indices", _p),
+                ("values", _p), ("dtype", _i32), ("row_offsets32", _p), ("col_indices32", _p)]
+
+
+class pm_solver_options(ctypes.Structure):
+    _fields_ = [("max_iterations", _i32), ("relative_tolerance", _f64), ("absolute_tolerance", _f64),
+                ("use_jacobi", _i32), ("require_convergence", _i32)]
+
+
+class pm_solver_report(ctypes.Structure):
+    _fields_ = [("converged", _i32), ("iterations", _i32), ("initial_residual", _f64), ("final_residual", _f64)]
+
+
+class pm_solve_policy(ctypes.Structure):
+    _fields_ = [("chunk_iterations", _i32), ("use_graph", _i32), ("refresh_interval", _i32),
+                ("lanes_per_row", _i32)]
+
+
+class pm_refine_report(ctypes.Structure):
+    _fields_ = [("converged", _i32), ("outer_iterations", _i32), ("inner_iterations", _i32),
+                ("initial_relres", _f64), ("final_relres", _f64)]
+
+
+# Every symbol declared in include/plamatrix_amd.h (checked by the CPU tests).
+EXPORTS = [
+    "pm_last_error", "pm_version", "pm_workspace_create", "pm_workspace_destroy", "pm_workspace_close",
+    "pm_workspace_capacity_bytes", "pm_workspace_capacity_rows", "pm_csr_validate", "pm_csr_compact_indices",
+    "pm_coo_to_csr_prepare", "pm_coo_to_csr_nnz", "pm_coo_to_csr_finish", "pm_coo_to_csr_async",
+    "pm_workspace_check_status", "pm_spmv", "pm_spmv_lanes", "pm_autotune_spmv", "pm_tuned_lanes", "pm_tune_record", "pm_true_relres", "pm_cg", "pm_pcg",
+    "pm_block_pcg", "pm_solve_fixed_async", "pm_fixed_finalize", "pm_fixed_state_destroy", "pm_pcg_refine",
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise RuntimeError(f"{LIBRARY} is not built; run __graft_entry__.build() (HIP library required)")
+        _lib = ctypes.CDLL(LIBRARY)
+        _lib.pm_last_error.restype = ctypes.c_char_p
+        _lib.pm_workspace_capacity_bytes.restype = ctypes.c_size_t
+        _lib.pm_workspace_capacity_bytes.argtypes = [_p]
+        _lib.pm_workspace_capacity_rows.restype = _i64
+        _lib.pm_workspace_capacity_rows.argtypes = [_p]
+        _lib.pm_workspace_destroy.argtypes = [_p]
+        _lib.pm_workspace_close.argtypes = [_p]
+        _lib.pm_workspace_check_status.argtypes = [_p]
+        _lib.pm_fixed_state_destroy.argtypes = [_p]
+    return _lib
+
+
+def check(code: int):
+    if code != 0:
+        raise PlamatrixError(code, lib().pm_last_error().decode())
+
+
+def current_stream():
+    import torch
+
+    return _p(torch.cuda.current_stream().cuda_stream)
+
+
+class Workspace:
+    """Owns a pm_workspace (device scratch, solver state, cached launch plans)."""
+
+    def __init__(self):
+        h = _p()
+        check(lib().pm_workspace_create(ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        check(lib().pm_workspace_close(self.handle))
+
+    def check_status(self):
+        check(lib().pm_workspace_check_status(self.handle))
+
+    @property
+    def capacity_bytes(self) -> int:
+        return int(lib().pm_workspace_capacity_bytes(self.handle))
+
+    @property
+    def capacity_rows(self) -> int:
+        return int(lib().pm_workspace_capacity_rows(self.handle))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.pm_workspace_destroy(h)
+            self.handle = None
+
+
+def dtype_code(t) -> int:
+    import torch
+
+    if t.dtype == torch.float32:
+        return PM_F32
+    if t.dtype == torch.float64:
+        return PM_F64
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+@dataclass
+class DeviceCSR:
+    """CSR arrays on the device (torch tensors) plus optional int32 mirrors."""
+
+    rows: int
+    cols: int
+    row_offsets: object
+    col_indices: object
+    values: object
+    row_offsets32: object = None
+    col_indices32: object = None
+
+    @property
+    def nnz(self) -> int:
+        return int(self.values.numel())
+
+    def descriptor(self) -> pm_csr:
+        def ptr(t):
+            return None if t is None else t.data_ptr()
+
+        return pm_csr(self.rows, self.cols, self.nnz, ptr(self.row_offsets), ptr(self.col_indices),
+                      ptr(self.values), dtype_code(self.values), ptr(self.row_offsets32), ptr(self.col_indices32))
+
+    def compact(self):
+        """Build the int32 index mirrors the kernels stream when present."""
+        import torch
+
+        ro = torch.empty(self.rows + 1, dtype=torch.int32, device=self.values.device)
+        ci = torch.empty(max(self.nnz, 1), dtype=torch.int32, device=self.values.device)
+        d = self.descriptor()
+        check(lib().pm_csr_compact_indices(ctypes.byref(d), _p(ro.data_ptr()), _p(ci.data_ptr()), current_stream()))
+        self.row_offsets32, self.col_indices32 = ro, ci
+        return self
+
+    def without_compact(self) -> "DeviceCSR":
+        return DeviceCSR(self.rows, self.cols, self.row_offsets, self.col_indices, self.values)
+
+
+def to_device_csr(host, device="cuda", dtype=None, compact=False) -> DeviceCSR:
+    import numpy as np
+    import torch
+
+    vals = host.values if dtype is None else host.values.astype(dtype)
+    csr = DeviceCSR(host.rows, host.cols,
+                    torch.from_numpy(np.ascontiguousarray(host.row_offsets, np.int64)).to(device),
+                    torch.from_numpy(np.ascontiguousarray(host.col_indices, np.int64)).to(device),
+                    torch.from_numpy(np.ascontiguousarray(vals)).to(device))
+    return csr.compact() if compact else csr
+
+
+def options(max_iterations=1000, relative_tolerance=1e-6, absolute_tolerance=0.0, use_jacobi=True,
+            require_convergence=False) -> pm_solver_options:
+    return pm_solver_options(int(max_iterations), float(relative_tolerance), float(absolute_tolerance),
+                             int(bool(use_jacobi)), int(bool(require_convergence)))
+
+
+def policy(chunk_iterations=0, use_graph=-1, refresh_interval=0, lanes_per_row=0) -> pm_solve_policy:
+    return pm_solve_policy(int(chunk_iterations), int(use_graph), int(refresh_interval), int(lanes_per_row))
+
+
+def _ptr(t):
+    return _p(t.data_ptr()) if t is not None else None
+
+
+def spmv(a: DeviceCSR, x, y, ws: Workspace | None = None, lanes: int = 0):
+    d = a.descriptor()
+    h = ws.handle if ws else None
+    if lanes:
+        check(lib().pm_spmv_lanes(ctypes.byref(d), _ptr(x), _ptr(y), _i32(lanes), h, current_stream()))
+    else:
+        check(lib().pm_spmv(ctypes.byref(d), _ptr(x), _ptr(y), h, current_stream()))
+    return y
+
+
+def validate(a: DeviceCSR, ws: Workspace):
+    d = a.descriptor()
+    check(lib().pm_csr_validate(ctypes.byref(d), ws.handle, current_stream()))
+
+
+def true_relres(a: DeviceCSR, b, x, ws: Workspace) -> float:
+    d = a.descriptor()
+    out = _f64()
+    check(lib().pm_true_relres(ctypes.byref(d), _ptr(b), _ptr(x), _i32(dtype_code(x)), ctypes.byref(out), ws.handle,
+                               current_stream()))
+    return out.value
+
+
+@dataclass
+class Report:
+    converged: bool
+    iterations: int
+    initial_residual: float
+    final_residual: float
+
+
+def _report(r: pm_solver_report) -> Report:
+    return Report(bool(r.converged), r.iterations, r.initial_residual, r.final_residual)
+
+
+def _solve(fn, a, b, x, ws, pol, extra=(), **opts):
+    d = a.descriptor()
+    o = options(**opts)
+    rep = pm_solver_report()
+    p = pol if pol is not None else policy()
+    check(fn(ctypes.byref(d), _ptr(b), _ptr(x), *extra, ctypes.byref(o), ctypes.byref(p), ctypes.byref(rep),
+             ws.handle, current_stream()))
+    return _report(rep)
+
+
+def cg(a, b, x, ws, pol=None, **opts) -> Report:
+    return _solve(lib().pm_cg, a, b, x, ws, pol, **opts)
+
+
+def pcg(a, b, x, ws, pol=None, **opts) -> Report:
+    return _solve(lib().pm_pcg, a, b, x, ws, pol, **opts)
+
+
+def block_pcg(a, b, x, inverse_blocks, block_size, ws, pol=None, **opts) -> Report:
+    return _solve(lib().pm_block_pcg, a, b, x, ws, pol, (_ptr(inverse_blocks), _i64(block_size)), **opts)
+
+
+class FixedState:
+    def __init__(self, handle):
+        self.handle = handle
+
+    def finalize(self, **opts) -> Report:
+        o = options(**opts)
+        rep = pm_solver_report()
+        check(lib().pm_fixed_finalize(self.handle, ctypes.byref(o), ctypes.byref(rep)))
+        return _report(rep)
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None and _lib is not None:
+            _lib.pm_fixed_state_destroy(self.handle)
+            self.handle = None
+
+
+def solve_fixed_async(a, b, x, iterations, preconditioned, ws) -> FixedState:
+    d = a.descriptor()
+    h = _p()
+    check(lib().pm_solve_fixed_async(ctypes.byref(d), _ptr(b), _ptr(x), _i32(iterations), _i32(int(preconditioned)),
+                                     ctypes.byref(h), ws.handle, current_stream()))
+    return FixedState(h)
+
+
+@dataclass
+class RefineReport:
+    converged: bool
+    outer_iterations: int
+    inner_iterations: int
+    initial_relres: float
+    final_relres: float
+
+
+def pcg_refine(a, b64, x64, ws, target=1e-8, max_outer=10, pol=None, **inner) -> RefineReport:
+    d = a.descriptor()
+    o = options(**inner)
+    p = pol if pol is not None else policy()
+    rep = pm_refine_report()
+    check(lib().pm_pcg_refine(ctypes.byref(d), _ptr(b64), _ptr(x64), ctypes.byref(o), _f64(target), _i32(max_outer),
+                              ctypes.byref(p), ctypes.byref(rep), ws.handle, current_stream()))
+    return RefineReport(bool(rep.converged), rep.outer_iterations, rep.inner_iterations, rep.initial_relres,
+                        rep.final_relres)
+
+
+def coo_to_csr(rows, cols, r, c, v, ws: Workspace, check_finite=True) -> DeviceCSR:
+    """Synchronous device conversion (prepare, read nnz, allocate, finish)."""
+    import torch
+
+    flags = PM_COO_CHECK_FINITE if check_finite else 0
+    stream = current_stream()
+    count = int(v.numel())
+    check(lib().pm_coo_to_csr_prepare(_i64(rows), _i64(cols), _i64(count), _ptr(r), _ptr(c), _ptr(v),
+                                      _i32(dtype_code(v)), _i32(flags), ws.handle, stream))
+    nnz = _i64()
+    check(lib().pm_coo_to_csr_nnz(ws.handle, ctypes.byref(nnz)))
+    dev = v.device
+    ro = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+    ci = torch.empty(max(nnz.value, 1), dtype=torch.int64, device=dev)
+    vals = torch.empty(max(nnz.value, 1), dtype=v.dtype, device=dev)
+    check(lib().pm_coo_to_csr_finish(_ptr(ro), _ptr(ci), _ptr(vals), _i64(nnz.value), ws.handle, stream))
+    torch.cuda.current_stream().synchronize()
+    ws.check_status()
+    return DeviceCSR(rows, cols, ro, ci[:nnz.value], vals[:nnz.value])
+
+
+def coo_to_csr_async(rows, cols, r, c, v, out: DeviceCSR, ws: Workspace, check_finite=True):
+    flags = PM_COO_CHECK_FINITE if check_finite else 0
+    check(lib().pm_coo_to_csr_async(_i64(rows), _i64(cols), _i64(int(v.numel())), _ptr(r), _ptr(c), _ptr(v),
+                                    _i32(dtype_code(v)), _i32(flags), _ptr(out.row_offsets), _ptr(out.col_indices),
+                                    _ptr(out.values), _i64(out.nnz), ws.handle, current_stream()))
+
+
+def autotune_spmv(a: DeviceCSR, x, y, repeats=10):
+    d = a.descriptor()
+    best = _i32()
+    ms = ctypes.c_float()
+    check(lib().pm_autotune_spmv(ctypes.byref(d), _ptr(x), _ptr(y), _i32(repeats), ctypes.byref(best),
+                                 ctypes.byref(ms), current_stream()))
+    return best.value, ms.value

@@ -1,0 +1,876 @@
+// Conjugate gradients on gfx950: CG, Jacobi-PCG, block-Jacobi PCG, the
+// fixed-iteration asynchronous variant and FP32/FP64 iterative refinement.
+//
+// One iteration is two kernels:
+//   k_direction_spmv  p = z + beta p (ping-pong), q = A p, p.q -> alpha
+//   k_update          x += alpha p, r -= alpha q, z = M^-1 r, r.r and r.z
+//                     -> convergence flag, iteration count, beta
+// Every dot product is accumulated in FP64 with a fixed reduction order, and
+// alpha, beta and the stopping test are computed on the device by the last
+// workgroup to finish, so the host never waits inside an iteration.  The host
+// enqueues chunks of iterations (replayed as HIP graphs), keeps one chunk of
+// lookahead in flight and reads one small status record per chunk; once the
+// device marks the solve finished the remaining kernels return immediately.
+//
+// The iteration follows the reference's adaptive solver
+// (iterative_solver.cu:87-163, 262-349; alphaKernel / betaKernel in
+// iterative_solver_kernels.cuh:169-238), with the stopping rule
+// ||r|| <= max(abs, rel * ||r0||).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <string>
+
+#include "plan_cache.hpp"
+#include "spmv_kernels.hpp"
+#include "tuning.hpp"
+
+namespace pm
+{
+
+enum Pre
+{
+    kPreNone = 0,
+    kPreJacobi = 1,
+    kPreBlock = 2,
+};
+
+enum Mode
+{
+    kModeInit = 0,
+    kModeStep = 1,
+    kModeRefresh = 2,
+};
+
+constexpr int kMaxBlock = 8;
+
+__global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel_tol, double abs_tol)
+{
+    st->rho = 0.0;
+    st->alpha = 0.0;
+    st->beta = 0.0;
+    st->rr = 0.0;
+    st->rr0 = 0.0;
+    st->tol = 0.0;
+    st->rel_tol = rel_tol;
+    st->abs_tol = abs_tol;
+    st->iteration = 0;
+    st->max_iterations = max_iterations;
+    st->done = kRunning;
+    st->bad_row = ~0ull;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) st->counters[i] = 0u;
+}
+
+// Last-workgroup epilogue of k_update: stopping test and beta
+// (betaKernel, iterative_solver_kernels.cuh:204-238).
+template <typename S>
+__device__ void finish_update(SolverState* st, int mode, double rr, double rz)
+{
+    if (mode == kModeInit)
+    {
+        st->rr0 = rr;
+        st->rr = rr;
+        const double initial = sqrt(rr);
+        st->tol = fmax(st->abs_tol, st->rel_tol * initial);
+        st->rho = rz;
+        st->beta = 0.0;
+        if (!isfinite(rr)) st->done = kNonFinite;
+        else if (initial <= st->tol) st->done = kConverged;
+        else if (st->max_iterations <= 0) st->done = kIterationLimit;
+        return;
+    }
+    st->rr = rr;
+    st->iteration += 1;
+    if (!isfinite(rr) || rr < 0.0)
+    {
+        st->done = kNonFinite;
+        return;
+    }
+    if (sqrt(rr) <= st->tol)
+    {
+        st->done = kConverged;
+        return;
+    }
+    if (st->iteration >= st->max_iterations)
+    {
+        st->done = kIterationLimit;
+        return;
+    }
+    const double rho = st->rho;
+    double beta = 0.0;
+    bool breakdown = false;
+    if (rho == 0.0 && rz == 0.0)
+    {
+        beta = 0.0;
+    }
+    else if (isfinite(rho) && isfinite(rz) && rho > 0.0 && rz > 0.0)
+    {
+        beta = rz / rho;
+        if (!isfinite(beta) || !isfinite(static_cast<S>(beta)))
+        {
+            breakdown = true;
+            beta = 0.0;
+        }
+    }
+    else
+    {
+        breakdown = true;
+    }
+    st->beta = beta;
+    st->rho = rz;
+    if (breakdown) st->done = kBreakdown;
+}
+
+// x += alpha p, r -= alpha q (STEP), z = M^-1 r, partial r.r and r.z.  A unit
+// is one row (None / Jacobi) or one diagonal block of bsize rows (Block).
+template <typename S, int PRE, int MODE>
+__global__ __launch_bounds__(kBlock) void k_update(int64_t units, int64_t bsize, S* __restrict__ x,
+                                                   S* __restrict__ r, S* __restrict__ z, const S* __restrict__ p,
+                                                   const S* __restrict__ q, const S* __restrict__ inv,
+                                                   SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
+    const S alpha = static_cast<S>(st->alpha);
+    int64_t begin, end;
+    block_rows(units, kBlock, begin, end);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t u = begin + threadIdx.x; u < end; u += kBlock)
+    {
+        if (PRE == kPreBlock)
+        {
+            S ri[kMaxBlock];
+            const int64_t base = u * bsize;
+            for (int j = 0; j < bsize; ++j)
+            {
+                const int64_t i = base + j;
+                if (MODE == kModeStep)
+                {
+                    x[i] = fma_s(alpha, p[i], x[i]);
+                    ri[j] = fma_s(-alpha, q[i], r[i]);
+                    r[i] = ri[j];
+                }
+                else
+                {
+                    ri[j] = r[i];
+                }
+            }
+            const S* m = inv + u * bsize * bsize;
+            for (int j = 0; j < bsize; ++j)
+            {
+                S zj = S(0);
+                for (int c = 0; c < bsize; ++c) zj = fma_s(m[j * bsize + c], ri[c], zj);
+                z[base + j] = zj;
+                rr = __builtin_fma(static_cast<double>(ri[j]), static_cast<double>(ri[j]), rr);
+                rz = __builtin_fma(static_cast<double>(ri[j]), static_cast<double>(zj), rz);
+            }
+        }
+        else
+        {
+            S ri;
+            if (MODE == kModeStep)
+            {
+                x[u] = fma_s(alpha, p[u], x[u]);
+                ri = fma_s(-alpha, q[u], r[u]);
+                r[u] = ri;
+            }
+            else
+            {
+                ri = r[u];
+            }
+            S zi = ri;
+            if (PRE == kPreJacobi)
+            {
+                zi = inv[u] * ri;
+                z[u] = zi;
+            }
+            rr = __builtin_fma(static_cast<double>(ri), static_cast<double>(ri), rr);
+            rz = __builtin_fma(static_cast<double>(ri), static_cast<double>(zi), rz);
+        }
+    }
+    double part[2] = {rr, rz};
+    double total[2];
+    if (last_block_total<2>(part, partials, &st->counters[1], total) && threadIdx.x == 0)
+        finish_update<S>(st, MODE, total[0], total[1]);
+}
+
+// x += alpha p (first half of a refresh iteration).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_axpy_x(int64_t rows, S* __restrict__ x, const S* __restrict__ p,
+                                                   const SolverState* st)
+{
+    if (st->done != kRunning) return;
+    const S alpha = static_cast<S>(st->alpha);
+    int64_t begin, end;
+    block_rows(rows, kBlock, begin, end);
+    for (int64_t i = begin + threadIdx.x; i < end; i += kBlock) x[i] = fma_s(alpha, p[i], x[i]);
+}
+
+// Copies the squared residuals into a fixed-solve state (toDoubleKernel /
+// toDoubleCheckedKernel, iterative_solver_kernels.cuh:240-252).
+__global__ void k_fixed_out(const SolverState* st, double* out)
+{
+    out[0] = st->rr0;
+    out[1] = (st->done == kBreakdown || st->done == kNonFinite) ? NAN : st->rr;
+}
+
+// ---- host side --------------------------------------------------------------
+
+struct Problem
+{
+    const pm_csr* a;
+    const void* rhs;
+    void* x;
+    const void* inv_blocks;  // block preconditioner (device), or null
+    int64_t bsize;
+    int pre;                 // Pre
+    int lanes;
+    // workspace vectors
+    void *r, *z, *q, *p0, *p1, *inv;
+    SolverState* st;
+    double* partials;
+};
+
+template <typename S, typename I, int L>
+struct Calls
+{
+    static const I* ro(const pm_csr* a)
+    {
+        if constexpr (sizeof(I) == 4) return a->row_offsets32;
+        else return a->row_offsets;
+    }
+    static const I* ci(const pm_csr* a)
+    {
+        if constexpr (sizeof(I) == 4) return a->col_indices32;
+        else return a->col_indices;
+    }
+
+    static int grid_units(int64_t units)
+    {
+        int64_t g = ceil_div(units > 0 ? units : 1, kBlock);
+        return static_cast<int>(g < kMaxPartials ? g : kMaxPartials);
+    }
+
+    template <int PRE, int MODE>
+    static KernelCall update(const Problem& pb)
+    {
+        const int64_t units = PRE == kPreBlock ? pb.a->rows / pb.bsize : pb.a->rows;
+        S* zz = PRE == kPreNone ? static_cast<S*>(pb.r) : static_cast<S*>(pb.z);
+        const S* inv = PRE == kPreBlock ? static_cast<const S*>(pb.inv_blocks) : static_cast<const S*>(pb.inv);
+        return make_call(k_update<S, PRE, MODE>, dim3(grid_units(units)), dim3(kBlock), units, pb.bsize,
+                         static_cast<S*>(pb.x), static_cast<S*>(pb.r), zz, static_cast<const S*>(pb.p0),
+                         static_cast<const S*>(pb.q), inv, pb.st, pb.partials);
+    }
+
+    template <int PRE, int MODE>
+    static KernelCall update_p(const Problem& pb, const void* p)
+    {
+        const int64_t units = PRE == kPreBlock ? pb.a->rows / pb.bsize : pb.a->rows;
+        S* zz = PRE == kPreNone ? static_cast<S*>(pb.r) : static_cast<S*>(pb.z);
+        const S* inv = PRE == kPreBlock ? static_cast<const S*>(pb.inv_blocks) : static_cast<const S*>(pb.inv);
+        return make_call(k_update<S, PRE, MODE>, dim3(grid_units(units)), dim3(kBlock), units, pb.bsize,
+                         static_cast<S*>(pb.x), static_cast<S*>(pb.r), zz, static_cast<const S*>(p),
+                         static_cast<const S*>(pb.q), inv, pb.st, pb.partials);
+    }
+
+    template <int MODE>
+    static KernelCall update_any(const Problem& pb, const void* p)
+    {
+        switch (pb.pre)
+        {
+        case kPreNone: return update_p<kPreNone, MODE>(pb, p);
+        case kPreJacobi: return update_p<kPreJacobi, MODE>(pb, p);
+        default: return update_p<kPreBlock, MODE>(pb, p);
+        }
+    }
+
+    static void init(const Problem& pb, std::vector<KernelCall>& out)
+    {
+        const int grid = grid_for_rows(pb.a->rows, L);
+        const S* z_for_init = nullptr;
+        (void)z_for_init;
+        if (pb.pre == kPreJacobi)
+            out.push_back(make_call(k_residual<S, I, L, true, true>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
+                                    ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
+                                    static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(pb.inv),
+                                    static_cast<S*>(pb.p0), pb.st));
+        else
+            out.push_back(make_call(k_residual<S, I, L, false, true>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
+                                    ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
+                                    static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(nullptr),
+                                    static_cast<S*>(pb.p0), pb.st));
+        out.push_back(update_any<kModeInit>(pb, pb.p0));
+    }
+
+    // One iteration: t is the global iteration index (parity picks p buffers).
+    static void iteration(const Problem& pb, int64_t t, bool refresh, std::vector<KernelCall>& out)
+    {
+        const int grid = grid_for_rows(pb.a->rows, L);
+        const void* p_old = (t % 2 == 0) ? pb.p0 : pb.p1;
+        void* p_new = (t % 2 == 0) ? pb.p1 : pb.p0;
+        const S* zsrc = pb.pre == kPreNone ? static_cast<const S*>(pb.r) : static_cast<const S*>(pb.z);
+        out.push_back(make_call(k_direction_spmv<S, I, L>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a), ci(pb.a),
+                                static_cast<const S*>(pb.a->values), zsrc, static_cast<const S*>(p_old),
+                                static_cast<S*>(p_new), static_cast<S*>(pb.q), pb.st, pb.partials));
+        if (!refresh)
+        {
+            out.push_back(update_any<kModeStep>(pb, p_new));
+            return;
+        }
+        out.push_back(make_call(k_axpy_x<S>, dim3(grid_units(pb.a->rows)), dim3(kBlock), pb.a->rows,
+                                static_cast<S*>(pb.x), static_cast<const S*>(p_new), pb.st));
+        out.push_back(make_call(k_residual<S, I, L, false, false>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
+                                ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
+                                static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(nullptr),
+                                static_cast<S*>(nullptr), pb.st));
+        out.push_back(update_any<kModeRefresh>(pb, p_new));
+    }
+};
+
+template <typename S, typename I>
+struct BuildCalls
+{
+    template <int L>
+    struct F
+    {
+        static int run(const Problem& pb, int what, int64_t first, int64_t count, int refresh_interval,
+                       std::vector<KernelCall>& out)
+        {
+            if (what == 0)
+            {
+                Calls<S, I, L>::init(pb, out);
+                return PM_OK;
+            }
+            for (int64_t t = first; t < first + count; ++t)
+            {
+                const bool refresh = refresh_interval > 0 && ((t + 1) % refresh_interval) == 0;
+                Calls<S, I, L>::iteration(pb, t, refresh, out);
+            }
+            return PM_OK;
+        }
+    };
+};
+
+template <typename S>
+int build_calls(const Problem& pb, int what, int64_t first, int64_t count, int refresh, std::vector<KernelCall>& out)
+{
+    if (use_compact(pb.a))
+        return dispatch_lanes<BuildCalls<S, int32_t>::template F>(pb.lanes, pb, what, first, count, refresh, out);
+    return dispatch_lanes<BuildCalls<S, int64_t>::template F>(pb.lanes, pb, what, first, count, refresh, out);
+}
+
+int validate_options(const pm_solver_options* o)
+{
+    if (!o) return fail(PM_INVALID_ARGUMENT, "null solver options");
+    if (o->max_iterations < 0 || !std::isfinite(o->relative_tolerance) || o->relative_tolerance < 0.0 ||
+        o->relative_tolerance > 1.0 || !std::isfinite(o->absolute_tolerance) || o->absolute_tolerance < 0.0)
+        return fail(PM_INVALID_ARGUMENT, "invalid iterative solver options");
+    return PM_OK;
+}
+
+int check_system(const pm_csr* a, const void* rhs, const void* x, const char* op)
+{
+    PM_TRY(check_csr(a, op));
+    if (a->rows != a->cols) return fail(PM_INVALID_ARGUMENT, "%s requires square A and Nx1 rhs/solution", op);
+    if (a->rows > 0 && (!rhs || !x)) return fail(PM_INVALID_ARGUMENT, "%s: null rhs or solution", op);
+    if (rhs && rhs == x) return fail(PM_INVALID_ARGUMENT, "%s: rhs and solution must not alias", op);
+    return PM_OK;
+}
+
+size_t elem_size(int32_t dtype) { return dtype == PM_F64 ? 8 : 4; }
+
+// Reserve workspace storage and fill the workspace part of a Problem.
+int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
+{
+    PM_TRY(ws->bind(stream));
+    PM_TRY(ws->ensure_events());
+    const int64_t n = pb.a->rows;
+    const size_t vec = ((elem_size(pb.a->dtype) * static_cast<size_t>(n > 0 ? n : 1)) + 255) & ~size_t(255);
+    PM_TRY(ws->vectors.reserve(vec * 6));
+    if (n > ws->capacity_rows) ws->capacity_rows = n;
+    PM_TRY(ws->partials.reserve(sizeof(double) * kMaxPartials * 8));
+    PM_TRY(ws->state.reserve(4096));
+    char* base = ws->vectors.as<char>();
+    pb.r = base;
+    pb.z = base + vec;
+    pb.q = base + 2 * vec;
+    pb.p0 = base + 3 * vec;
+    pb.p1 = base + 4 * vec;
+    pb.inv = base + 5 * vec;
+    pb.st = ws->state.as<SolverState>();
+    pb.partials = ws->partials.as<double>();
+    return PM_OK;
+}
+
+std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t count, int refresh)
+{
+    // workspace addresses are part of the key: a grown workspace must never
+    // replay a plan that still points at its old buffers
+    char buf[640];
+    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%d|%lld|%lld|%d|%lld", kind,
+                  pb.r, static_cast<void*>(pb.st), static_cast<void*>(pb.partials),
+                  (const void*)pb.a->row_offsets, (const void*)pb.a->row_offsets32, (const void*)pb.a->col_indices,
+                  (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
+                  static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
+                  pb.lanes, static_cast<long long>(first), static_cast<long long>(count), refresh,
+                  static_cast<long long>(pb.bsize));
+    return buf;
+}
+
+template <typename S>
+int get_plan(pm_workspace* ws, const Problem& pb, int what, int64_t first, int64_t count, int refresh, Plan** out)
+{
+    const std::string key = plan_key(pb, what == 0 ? "init" : "chunk", first, count, refresh);
+    Plan* plan = ws->plans->find(key);
+    if (!plan)
+    {
+        auto fresh = std::make_unique<Plan>();
+        PM_TRY(build_calls<S>(pb, what, first, count, refresh, fresh->calls));
+        plan = ws->plans->insert(key, std::move(fresh));
+    }
+    *out = plan;
+    return PM_OK;
+}
+
+struct Policy
+{
+    int chunk = 16;
+    bool graph = true;
+    int refresh = 0;
+    int lanes = 0;
+};
+
+Policy resolve_policy(const pm_solve_policy* p)
+{
+    Policy out;
+    if (p)
+    {
+        if (p->chunk_iterations > 0) out.chunk = p->chunk_iterations;
+        if (p->use_graph >= 0) out.graph = p->use_graph != 0;
+        out.refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
+        out.lanes = p->lanes_per_row;
+    }
+    if (out.chunk % 2) out.chunk += 1;
+    if (out.refresh > 0)
+    {
+        if (out.refresh % 2) out.refresh += 1;
+        // a chunk must divide the refresh interval so chunk plans repeat
+        int k = std::min(out.chunk, out.refresh);
+        while (k > 2 && (out.refresh % k != 0 || k % 2 != 0)) --k;
+        out.chunk = std::max(k, 2);
+    }
+    return out;
+}
+
+template <typename S>
+int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_blocks, int64_t bsize, int pre,
+                   const pm_solver_options* options, const pm_solve_policy* policy_in, pm_solver_report* report,
+                   pm_workspace* ws, hipStream_t stream)
+{
+    const Policy policy = resolve_policy(policy_in);
+    Problem pb{};
+    pb.a = a;
+    pb.rhs = rhs;
+    pb.x = x;
+    pb.inv_blocks = inv_blocks;
+    pb.bsize = pre == kPreBlock ? bsize : 1;
+    pb.pre = pre;
+    pb.lanes = policy.lanes > 0 ? policy.lanes : tuned_lanes(a);
+    PM_TRY(prepare(ws, pb, stream));
+
+    pm_solver_report rep{};
+    if (a->rows == 0)
+    {
+        rep.converged = 1;
+        if (report) *report = rep;
+        return PM_OK;
+    }
+    hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, stream, pb.st, options->max_iterations,
+                       options->relative_tolerance, options->absolute_tolerance);
+    PM_HIP(hipGetLastError());
+    Plan* init = nullptr;
+    PM_TRY(get_plan<S>(ws, pb, 0, 0, 0, 0, &init));
+    PM_TRY(init->run(stream, false));
+    SolverState* hs = ws->host_state;
+    PM_HIP(hipMemcpyAsync(hs, pb.st, sizeof(SolverState), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    SolverState fin = *hs;
+    rep.initial_residual = std::sqrt(fin.rr0);
+    rep.final_residual = rep.initial_residual;
+    if (!std::isfinite(fin.rr0))
+        return fail(PM_RUNTIME_ERROR, "iterative solver initial residual is not finite");
+    if (fin.done == kConverged)
+    {
+        rep.converged = 1;
+        if (report) *report = rep;
+        return PM_OK;
+    }
+    if (pre == kPreJacobi && fin.bad_row != ~0ull)
+        return fail(PM_RUNTIME_ERROR, "pcg Jacobi diagonal is missing, non-positive, or near zero at row %llu",
+                    fin.bad_row);
+    if (fin.done == kRunning)
+    {
+        const int64_t K = policy.chunk;
+        std::deque<int64_t> inflight;
+        int64_t launched = 0;
+        int64_t chunk_id = 0;
+        auto issue = [&]() -> int {
+            Plan* plan = nullptr;
+            const int64_t first = launched;
+            // chunk plans depend only on (first mod refresh interval) and parity; K is
+            // even and divides the interval, so key on the phase.
+            const int64_t phase = policy.refresh > 0 ? first % policy.refresh : 0;
+            PM_TRY(get_plan<S>(ws, pb, 1, phase, K, policy.refresh, &plan));
+            PM_TRY(plan->run(stream, policy.graph));
+            const int slot = static_cast<int>(chunk_id % 2);
+            PM_HIP(hipMemcpyAsync(hs + slot, pb.st, sizeof(SolverState), hipMemcpyDeviceToHost, stream));
+            PM_HIP(hipEventRecord(ws->events[slot], stream));
+            inflight.push_back(chunk_id);
+            ++chunk_id;
+            launched += K;
+            return PM_OK;
+        };
+        PM_TRY(issue());
+        while (true)
+        {
+            if (launched < options->max_iterations && inflight.size() < 2) PM_TRY(issue());
+            const int64_t oldest = inflight.front();
+            inflight.pop_front();
+            const int slot = static_cast<int>(oldest % 2);
+            PM_HIP(hipEventSynchronize(ws->events[slot]));
+            fin = hs[slot];
+            if (fin.done != kRunning) break;
+            if (inflight.empty() && launched >= options->max_iterations) break;
+        }
+    }
+    PM_HIP(hipEventRecord(ws->done_event, stream));
+    rep.iterations = fin.iteration;
+    rep.final_residual = std::sqrt(fin.rr);
+    if (fin.done == kBreakdown || fin.done == kNonFinite)
+        return fail(PM_RUNTIME_ERROR, "iterative solver breakdown: matrix is not numerically SPD");
+    rep.converged = fin.done == kConverged ? 1 : 0;
+    if (report) *report = rep;
+    if (!rep.converged && options->require_convergence)
+        return fail(PM_RUNTIME_ERROR, "iterative solver did not converge in %d iterations; final residual=%g",
+                    rep.iterations, rep.final_residual);
+    return PM_OK;
+}
+
+int solve_dispatch(const pm_csr* a, const void* rhs, void* x, const void* inv_blocks, int64_t bsize, int pre,
+                   const pm_solver_options* options, const pm_solve_policy* policy, pm_solver_report* report,
+                   pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(validate_options(options));
+    PM_TRY(check_system(a, rhs, x, "iterative solver"));
+    if (!ws) return fail(PM_INVALID_ARGUMENT, "iterative solver: null workspace");
+    if (a->dtype == PM_F32)
+        return adaptive_solve<float>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream);
+    return adaptive_solve<double>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream);
+}
+
+} // namespace pm
+
+struct pm_fixed_state
+{
+    double* device = nullptr;  // [initial rr, final rr]
+    hipEvent_t event = nullptr;
+    hipStream_t stream = nullptr;
+    int32_t submitted = 0;
+};
+
+using namespace pm;
+
+namespace pm
+{
+template <typename S>
+int fixed_solve(const pm_csr* a, const void* rhs, void* x, int32_t iterations, int pre, pm_fixed_state* state,
+                pm_workspace* ws, hipStream_t stream)
+{
+    Problem pb{};
+    pb.a = a;
+    pb.rhs = rhs;
+    pb.x = x;
+    pb.bsize = 1;
+    pb.pre = pre;
+    pb.lanes = tuned_lanes(a);
+    PM_TRY(prepare(ws, pb, stream));
+    if (a->rows == 0)
+    {
+        PM_HIP(hipMemsetAsync(state->device, 0, 2 * sizeof(double), stream));
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, stream, pb.st, iterations, 0.0, 0.0);
+        PM_HIP(hipGetLastError());
+        std::vector<KernelCall> calls;
+        PM_TRY(build_calls<S>(pb, 0, 0, 0, 0, calls));
+        PM_TRY(build_calls<S>(pb, 1, 0, iterations, 0, calls));
+        for (const KernelCall& k : calls) PM_TRY(launch(k, stream));
+        hipLaunchKernelGGL(k_fixed_out, dim3(1), dim3(1), 0, stream, pb.st, state->device);
+        PM_HIP(hipGetLastError());
+    }
+    PM_HIP(hipEventRecord(state->event, stream));
+    PM_HIP(hipEventRecord(ws->done_event, stream));
+    return PM_OK;
+}
+} // namespace pm
+
+extern "C" {
+
+int pm_cg(const pm_csr* a, const void* rhs, void* x, const pm_solver_options* options,
+          const pm_solve_policy* policy, pm_solver_report* report, pm_workspace* ws, hipStream_t stream)
+{
+    return solve_dispatch(a, rhs, x, nullptr, 1, kPreNone, options, policy, report, ws, stream);
+}
+
+int pm_pcg(const pm_csr* a, const void* rhs, void* x, const pm_solver_options* options,
+           const pm_solve_policy* policy, pm_solver_report* report, pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(validate_options(options));
+    return solve_dispatch(a, rhs, x, nullptr, 1, options->use_jacobi ? kPreJacobi : kPreNone, options, policy, report,
+                          ws, stream);
+}
+
+int pm_block_pcg(const pm_csr* a, const void* rhs, void* x, const void* inverse_blocks, int64_t block_size,
+                 const pm_solver_options* options, const pm_solve_policy* policy, pm_solver_report* report,
+                 pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(check_csr(a, "block PCG"));
+    if (block_size <= 0 || block_size > kMaxBlock || a->rows % block_size != 0 || (a->rows > 0 && !inverse_blocks))
+        return fail(PM_INVALID_ARGUMENT, "block PCG requires complete row-major inverse diagonal blocks "
+                                         "(block size 1..%d dividing the row count)", kMaxBlock);
+    if (inverse_blocks && (inverse_blocks == rhs || inverse_blocks == x))
+        return fail(PM_INVALID_ARGUMENT, "block PCG inverse blocks must not alias rhs or solution");
+    return solve_dispatch(a, rhs, x, inverse_blocks, block_size, kPreBlock, options, policy, report, ws, stream);
+}
+
+int pm_solve_fixed_async(const pm_csr* a, const void* rhs, void* x, int32_t iterations, int32_t preconditioned,
+                         pm_fixed_state** state, pm_workspace* ws, hipStream_t stream)
+{
+    if (!state) return fail(PM_INVALID_ARGUMENT, "fixed solve: null state output");
+    *state = nullptr;
+    if (iterations < 0) return fail(PM_INVALID_ARGUMENT, "fixed iterative solver iterations must be non-negative");
+    PM_TRY(check_system(a, rhs, x, "fixed iterative solver"));
+    if (!ws) return fail(PM_INVALID_ARGUMENT, "fixed iterative solver: null workspace");
+    auto* s = new pm_fixed_state();
+    s->stream = stream;
+    s->submitted = iterations;
+    hipError_t err = hipMalloc(reinterpret_cast<void**>(&s->device), 2 * sizeof(double));
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&s->event, hipEventDisableTiming);
+    if (err != hipSuccess)
+    {
+        pm_fixed_state_destroy(s);
+        return check_hip(err, "fixed state allocation", __FILE__, __LINE__);
+    }
+    const int pre = preconditioned ? kPreJacobi : kPreNone;
+    const int status = a->dtype == PM_F32 ? fixed_solve<float>(a, rhs, x, iterations, pre, s, ws, stream)
+                                          : fixed_solve<double>(a, rhs, x, iterations, pre, s, ws, stream);
+    if (status != PM_OK)
+    {
+        pm_fixed_state_destroy(s);
+        return status;
+    }
+    *state = s;
+    return PM_OK;
+}
+
+int pm_fixed_finalize(const pm_fixed_state* state, const pm_solver_options* options, pm_solver_report* report)
+{
+    PM_TRY(validate_options(options));
+    if (!state || !state->event) return fail(PM_LOGIC_ERROR, "fixed iterative solver state is empty or closed");
+    const hipError_t q = hipEventQuery(state->event);
+    if (q == hipErrorNotReady)
+        return fail(PM_LOGIC_ERROR, "finalizeIterativeSolverReport requires caller stream synchronization");
+    if (q != hipSuccess) return check_hip(q, "hipEventQuery", __FILE__, __LINE__);
+    double host[2] = {0.0, 0.0};
+    PM_HIP(hipMemcpy(host, state->device, sizeof(host), hipMemcpyDeviceToHost));
+    if (!std::isfinite(host[0]) || !std::isfinite(host[1]) || host[0] < 0.0 || host[1] < 0.0)
+        return fail(PM_RUNTIME_ERROR, "fixed iterative solver breakdown: residual is not finite");
+    pm_solver_report rep{};
+    rep.iterations = state->submitted;
+    rep.initial_residual = std::sqrt(host[0]);
+    rep.final_residual = std::sqrt(host[1]);
+    const double tol = std::max(options->absolute_tolerance, options->relative_tolerance * rep.initial_residual);
+    rep.converged = rep.final_residual <= tol ? 1 : 0;
+    if (report) *report = rep;
+    if (!rep.converged && options->require_convergence)
+        return fail(PM_RUNTIME_ERROR, "fixed iterative solver did not converge in %d iterations; final residual=%g",
+                    rep.iterations, rep.final_residual);
+    return PM_OK;
+}
+
+int pm_fixed_state_destroy(pm_fixed_state* state)
+{
+    if (!state) return PM_OK;
+    if (state->event)
+    {
+        (void)hipEventSynchronize(state->event);
+        (void)hipEventDestroy(state->event);
+    }
+    if (state->device) (void)hipFree(state->device);
+    delete state;
+    return PM_OK;
+}
+
+} // extern "C"
+
+// ---- FP32 inner solve, FP64 outer refinement --------------------------------
+namespace pm
+{
+
+// r = b - A x in FP64 (FP32 matrix, FP64 vectors) plus partial ||r||^2, ||b||^2.
+template <typename I, int L>
+__global__ __launch_bounds__(kBlock) void k_residual64(int64_t rows, const I* __restrict__ ro,
+                                                       const I* __restrict__ ci, const float* __restrict__ v,
+                                                       const double* __restrict__ b, const double* __restrict__ x,
+                                                       double* __restrict__ r, double* partials, unsigned* counter,
+                                                       double* out)
+{
+    constexpr int64_t gpb = kBlock / L;
+    const int group = threadIdx.x / L;
+    const int lane = threadIdx.x % L;
+    int64_t begin, end;
+    block_rows(rows, gpb, begin, end);
+    double rr = 0.0, bb = 0.0;
+    for (int64_t row = begin + group; row < end; row += gpb)
+    {
+        const int64_t s = ro[row], e = ro[row + 1];
+        double acc = 0.0;
+        for (int64_t k = s + lane; k < e; k += L) acc = __builtin_fma(static_cast<double>(v[k]), x[ci[k]], acc);
+        acc = group_sum<L>(acc);
+        if (lane == 0)
+        {
+            const double res = b[row] - acc;
+            r[row] = res;
+            rr = __builtin_fma(res, res, rr);
+            bb = __builtin_fma(b[row], b[row], bb);
+        }
+    }
+    double part[2] = {rr, bb};
+    double total[2];
+    if (last_block_total<2>(part, partials, counter, total) && threadIdx.x == 0)
+    {
+        out[0] = total[0];
+        out[1] = total[1];
+    }
+}
+
+__global__ void k_scale_to_f32(int64_t n, const double* __restrict__ r, double scale, float* __restrict__ rf,
+                               float* __restrict__ d)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        rf[i] = static_cast<float>(r[i] * scale);
+        d[i] = 0.0f;
+    }
+}
+
+__global__ void k_accumulate(int64_t n, double* __restrict__ x, const float* __restrict__ d, double scale)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+        x[i] = __builtin_fma(scale, static_cast<double>(d[i]), x[i]);
+}
+
+__global__ void k_zero_u32(unsigned* p) { *p = 0u; }
+
+template <typename I>
+struct Residual64
+{
+    template <int L>
+    struct F
+    {
+        static int run(const pm_csr* a, const I* ro, const I* ci, const double* b, const double* x, double* r,
+                       double* partials, unsigned* counter, double* out, hipStream_t stream)
+        {
+            hipLaunchKernelGGL((k_residual64<I, L>), dim3(grid_for_rows(a->rows, L)), dim3(kBlock), 0, stream,
+                               a->rows, ro, ci, static_cast<const float*>(a->values), b, x, r, partials, counter,
+                               out);
+            PM_HIP(hipGetLastError());
+            return PM_OK;
+        }
+    };
+};
+
+int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options* inner, double target,
+           int32_t max_outer, const pm_solve_policy* policy, pm_refine_report* report, pm_workspace* ws,
+           hipStream_t stream)
+{
+    const int64_t n = a->rows;
+    const size_t r_bytes = (sizeof(double) * static_cast<size_t>(n) + 255) & ~size_t(255);
+    const size_t f_bytes = (sizeof(float) * static_cast<size_t>(n) + 255) & ~size_t(255);
+    PM_TRY(ws->refine.reserve(r_bytes + 2 * f_bytes + 4096));
+    PM_TRY(ws->partials.reserve(sizeof(double) * kMaxPartials * 8));
+    char* base = ws->refine.as<char>();
+    double* r64 = reinterpret_cast<double*>(base);
+    float* rf = reinterpret_cast<float*>(base + r_bytes);
+    float* d = reinterpret_cast<float*>(base + r_bytes + f_bytes);
+    unsigned* counter = reinterpret_cast<unsigned*>(base + r_bytes + 2 * f_bytes);
+    double* out = reinterpret_cast<double*>(base + r_bytes + 2 * f_bytes + 256);
+    const int lanes = policy && policy->lanes_per_row > 0 ? policy->lanes_per_row : tuned_lanes(a);
+    const int grid = static_cast<int>(std::min<int64_t>(ceil_div(n > 0 ? n : 1, kBlock), 8192));
+
+    pm_solver_options in = *inner;
+    in.require_convergence = 0;
+    in.absolute_tolerance = 0.0;
+    pm_refine_report rep{};
+    for (int outer = 0;; ++outer)
+    {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, counter);
+        if (use_compact(a))
+            PM_TRY(dispatch_lanes<Residual64<int32_t>::F>(lanes, a, a->row_offsets32, a->col_indices32, b, x, r64,
+                                                          ws->partials.as<double>(), counter, out, stream));
+        else
+            PM_TRY(dispatch_lanes<Residual64<int64_t>::F>(lanes, a, a->row_offsets, a->col_indices, b, x, r64,
+                                                          ws->partials.as<double>(), counter, out, stream));
+        double host[2];
+        PM_HIP(hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, stream));
+        PM_HIP(hipStreamSynchronize(stream));
+        if (!std::isfinite(host[0])) return fail(PM_RUNTIME_ERROR, "refinement residual is not finite");
+        const double relres = host[1] > 0.0 ? std::sqrt(host[0] / host[1]) : std::sqrt(host[0]);
+        if (outer == 0) rep.initial_relres = relres;
+        rep.final_relres = relres;
+        rep.outer_iterations = outer;
+        if (relres <= target || host[0] == 0.0)
+        {
+            rep.converged = 1;
+            break;
+        }
+        if (outer >= max_outer) break;
+        const double norm = std::sqrt(host[0]);
+        hipLaunchKernelGGL(k_scale_to_f32, dim3(grid), dim3(kBlock), 0, stream, n, r64, 1.0 / norm, rf, d);
+        PM_HIP(hipGetLastError());
+        pm_solver_report inner_rep{};
+        PM_TRY(adaptive_solve<float>(a, rf, d, nullptr, 1, in.use_jacobi ? kPreJacobi : kPreNone, &in, policy,
+                                     &inner_rep, ws, stream));
+        rep.inner_iterations += inner_rep.iterations;
+        hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, stream, n, x, d, norm);
+        PM_HIP(hipGetLastError());
+    }
+    PM_HIP(hipEventRecord(ws->done_event, stream));
+    if (report) *report = rep;
+    return PM_OK;
+}
+
+} // namespace pm
+
+extern "C" int pm_pcg_refine(const pm_csr* a, const double* rhs, double* x, const pm_solver_options* inner,
+                             double target_relres, int32_t max_outer, const pm_solve_policy* policy,
+                             pm_refine_report* report, pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(pm::validate_options(inner));
+    PM_TRY(pm::check_system(a, rhs, x, "refinement"));
+    if (a->dtype != PM_F32) return pm::fail(PM_INVALID_ARGUMENT, "refinement expects an FP32 matrix");
+    if (!ws) return pm::fail(PM_INVALID_ARGUMENT, "refinement: null workspace");
+    if (!(target_relres >= 0.0) || max_outer < 0) return pm::fail(PM_INVALID_ARGUMENT, "refinement: invalid target");
+    PM_TRY(ws->bind(stream));
+    PM_TRY(ws->ensure_events());
+    if (a->rows == 0)
+    {
+        if (report) *report = pm_refine_report{1, 0, 0, 0.0, 0.0};
+        return PM_OK;
+    }
+    return pm::refine(a, rhs, x, inner, target_relres, max_outer, policy, report, ws, stream);
+}

@@ -1,0 +1,139 @@
+// Workspace layout and the kernel-call records used to replay solver chunks
+// either as direct launches or as HIP graphs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "pm_common.hpp"
+
+namespace pm
+{
+
+// Device-resident solver state.  One record per solve; written only by the
+// single last-arriving workgroup of each reduction, read by every kernel.
+struct SolverState
+{
+    double rho;        // r . z of the current direction
+    double alpha;
+    double beta;
+    double rr;         // latest ||r||^2
+    double rr0;        // initial ||r||^2
+    double tol;        // absolute tolerance on ||r||
+    double rel_tol;
+    double abs_tol;
+    int32_t iteration; // completed iterations
+    int32_t max_iterations;
+    int32_t done;      // see Done below
+    int32_t pad0;
+    unsigned long long bad_row;  // first row with an unusable Jacobi diagonal (~0 if none)
+    unsigned counters[8];
+    double pad1[4];
+};
+
+enum Done : int32_t
+{
+    kRunning = 0,
+    kConverged = 1,
+    kIterationLimit = 2,
+    kBreakdown = 3,
+    kNonFinite = 4,
+};
+
+// COO conversion status (deferred, lowest triplet position first), as the
+// reference's CooStatus (coo_to_csr.cu:116-126).
+struct CooStatus
+{
+    unsigned long long invalid_source;  // first triplet with an out-of-range coordinate
+    unsigned long long invalid_value;   // first triplet (or run) with a non-finite value / sum
+    long long nnz;                      // combined nnz
+    long long expected_nnz;             // caller's output nnz (-1 when unchecked)
+};
+
+// One recorded kernel launch: the function, its shape and a private copy of
+// its arguments, so that it can be launched now or added to a graph later.
+struct KernelCall
+{
+    const void* func = nullptr;
+    dim3 grid;
+    dim3 block;
+    std::shared_ptr<void> storage;
+    std::vector<void*> args;
+};
+
+template <typename Tuple, size_t... I>
+std::vector<void*> tuple_addresses(Tuple& t, std::index_sequence<I...>)
+{
+    return {static_cast<void*>(&std::get<I>(t))...};
+}
+
+template <typename... P, typename... A>
+KernelCall make_call(void (*f)(P...), dim3 grid, dim3 block, A... a)
+{
+    static_assert(sizeof...(P) == sizeof...(A), "argument count mismatch");
+    auto tup = std::make_shared<std::tuple<P...>>(static_cast<P>(a)...);
+    KernelCall k;
+    k.func = reinterpret_cast<const void*>(f);
+    k.grid = grid;
+    k.block = block;
+    k.args = tuple_addresses(*tup, std::index_sequence_for<P...>{});
+    k.storage = tup;
+    return k;
+}
+
+int launch(const KernelCall& k, hipStream_t stream);
+
+// A chunk of recorded calls, optionally instantiated as one HIP graph.
+struct Plan
+{
+    std::vector<KernelCall> calls;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int instantiate();
+    int run(hipStream_t stream, bool use_graph);
+    void reset();
+    ~Plan() { reset(); }
+};
+
+} // namespace pm
+
+struct pm_workspace
+{
+    hipStream_t stream = nullptr;
+    bool bound = false;
+    int device = -1;
+
+    pm::DeviceBuffer vectors;   // solver vectors
+    pm::DeviceBuffer partials;  // kMaxPartials * 8 doubles
+    pm::DeviceBuffer state;     // SolverState + small scalars
+    pm::DeviceBuffer scratch;   // miscellaneous (relres, validation)
+    pm::DeviceBuffer sort;      // COO conversion buffers
+    pm::DeviceBuffer refine;    // FP64 residual / FP32 correction of the refinement loop
+    int64_t capacity_rows = 0;
+
+    pm::SolverState* host_state = nullptr;  // pinned, 2 slots
+    hipEvent_t events[2] = {nullptr, nullptr};
+    hipEvent_t done_event = nullptr;        // completion of the last enqueued work
+
+    // COO conversion in flight
+    int64_t coo_rows = 0, coo_cols = 0, coo_count = 0;
+    int32_t coo_dtype = 0, coo_flags = 0;
+    int coo_key_bits = 0;
+    const void* coo_values = nullptr;
+    bool coo_prepared = false;
+    bool coo_pending_status = false;
+    pm::CooStatus* host_coo = nullptr;      // pinned
+
+    // cached solver chunk plans
+    struct PlanCache;
+    std::unique_ptr<PlanCache> plans;
+
+    int bind(hipStream_t s);
+    int ensure_events();
+    ~pm_workspace();
+};
