@@ -53,7 +53,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--max-inner", type=int, default=100000)
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for profilers: one solve capped at 2000 inner iterations, no CPU leg")
     args = ap.parse_args()
+    if args.profile:
+        args.steps, args.warmup, args.no_cpu, args.max_inner, args.spmv_rounds = 1, 0, True, 2000, 20
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this path has no multi-GPU mode (systems shard without "
               "communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
@@ -95,7 +100,7 @@ def main() -> int:
         torch.cuda.synchronize()
         return start.elapsed_time(stop) / (rounds * len(copies))
 
-    candidates = {lanes: time_spmv(lanes, 5) for lanes in (2, 4, 8, 16)}
+    candidates = {lanes: time_spmv(lanes, 5) for lanes in (0, 4, 8)}
     lanes = min(candidates, key=candidates.get)
     capi.lib().pm_tune_record(capi.ctypes.byref(a.descriptor()), capi.ctypes.c_int32(lanes))
     spmv_ms = time_spmv(lanes, args.spmv_rounds)
@@ -106,7 +111,7 @@ def main() -> int:
     # ---- time-to-solve ----------------------------------------------------------
     pol = capi.policy(refresh_interval=50, use_graph=args.graph, chunk_iterations=args.chunk)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
-    inner = dict(relative_tolerance=1e-6, max_iterations=20000)
+    inner = dict(relative_tolerance=1e-6, max_iterations=args.max_inner)
 
     def solve():
         x.zero_()
@@ -132,7 +137,7 @@ def main() -> int:
     x.zero_()
     torch.cuda.synchronize()
     t8 = time.perf_counter()
-    rep8 = capi.pcg_refine(a, b, x, ws, target=1e-8, max_outer=20, pol=pol, **inner)
+    rep8 = capi.pcg_refine(a, b, x, ws, target=1e-8, max_outer=1 if args.profile else 20, pol=pol, **inner)
     torch.cuda.synchronize()
     ms_1e8 = (time.perf_counter() - t8) * 1e3
 
@@ -173,7 +178,7 @@ def main() -> int:
         },
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"k_spmv<float,int,{lanes}>", "bytes_per_launch": spmv_b,
+                     "kernel": "k_spmv<float,int>" if lanes == 0 else f"k_spmv_lanes<float,int,{lanes}>", "bytes_per_launch": spmv_b,
                      "ms_per_launch": round(spmv_ms, 5)},
         "cpu_baseline": cpu,
         "solve": {"converged": rep.converged, "outer": rep.outer_iterations, "inner_iterations": iterations,

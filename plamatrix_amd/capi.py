@@ -198,10 +198,12 @@ def _ptr(t):
     return _p(t.data_ptr()) if t is not None else None
 
 
-def spmv(a: DeviceCSR, x, y, ws: Workspace | None = None, lanes: int = 0):
+def spmv(a: DeviceCSR, x, y, ws: Workspace | None = None, lanes: int | None = None):
+    """y = A x.  lanes=None uses the tuned variant; 0 forces the wave-stream
+    kernel, 1..64 the lanes-per-row kernel."""
     d = a.descriptor()
     h = ws.handle if ws else None
-    if lanes:
+    if lanes is not None:
         check(lib().pm_spmv_lanes(ctypes.byref(d), _ptr(x), _ptr(y), _i32(lanes), h, current_stream()))
     else:
         check(lib().pm_spmv(ctypes.byref(d), _ptr(x), _ptr(y), h, current_stream()))

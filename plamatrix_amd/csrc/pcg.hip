@@ -2,20 +2,23 @@
 // fixed-iteration asynchronous variant and FP32/FP64 iterative refinement.
 //
 // One iteration is two kernels:
-//   k_direction_spmv  p = z + beta p (ping-pong), q = A p, p.q -> alpha
-//   k_update          x += alpha p, r -= alpha q, z = M^-1 r, r.r and r.z
-//                     -> convergence flag, iteration count, beta
-// Every dot product is accumulated in FP64 with a fixed reduction order, and
-// alpha, beta and the stopping test are computed on the device by the last
-// workgroup to finish, so the host never waits inside an iteration.  The host
-// enqueues chunks of iterations (replayed as HIP graphs), keeps one chunk of
-// lookahead in flight and reads one small status record per chunk; once the
-// device marks the solve finished the remaining kernels return immediately.
+//   k_direction_spmv  reduce the previous update's r.r, r.z  -> stop test, beta
+//                     p = z + beta p (ping-pong), q = A p, publish p.q
+//   k_update          reduce p.q -> alpha; x += alpha p, r -= alpha q,
+//                     z = M^-1 r, publish r.r and r.z
+// Every dot product is accumulated in FP64 in a fixed order; every workgroup
+// re-reduces the previous kernel's partial sums itself, so alpha, beta and the
+// stopping test are derived identically everywhere on the device without
+// atomics, fences or host round trips.  The host enqueues chunks of
+// iterations (replayed as HIP graphs), keeps one chunk of lookahead in flight
+// and reads one small status record per chunk; once the device marks the solve
+// finished the remaining kernels return immediately.
 //
 // The iteration follows the reference's adaptive solver
 // (iterative_solver.cu:87-163, 262-349; alphaKernel / betaKernel in
 // iterative_solver_kernels.cuh:169-238), with the stopping rule
-// ||r|| <= max(abs, rel * ||r0||).
+// ||r|| <= max(abs, rel * ||r0||) and the Jacobi diagonal rules of
+// iterative_solver_cpu.cpp:127-160.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -44,17 +47,24 @@ enum Mode
 };
 
 constexpr int kMaxBlock = 8;
+constexpr int kDirBlock = 1024;                 // threads of the direction / SpMV kernel
+constexpr int kDirWaves = kDirBlock / kWave;
+constexpr int kDirGrid = 512;                   // at most 512 x 16 waves: one full-occupancy round
+constexpr int kPartialsDen = 0;                 // p.q partials (1 value)
+constexpr int kPartialsRes = 2 * kMaxPartials;  // r.r, r.z partials (2 values)
 
 __global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel_tol, double abs_tol)
 {
-    st->rho = 0.0;
+    st->rho[0] = 0.0;
+    st->rho[1] = 0.0;
     st->alpha = 0.0;
-    st->beta = 0.0;
     st->rr = 0.0;
     st->rr0 = 0.0;
     st->tol = 0.0;
     st->rel_tol = rel_tol;
     st->abs_tol = abs_tol;
+    st->iter[0] = -1;
+    st->iter[1] = -1;
     st->iteration = 0;
     st->max_iterations = max_iterations;
     st->done = kRunning;
@@ -63,82 +73,146 @@ __global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel
     for (int i = 0; i < 8; ++i) st->counters[i] = 0u;
 }
 
-// Last-workgroup epilogue of k_update: stopping test and beta
-// (betaKernel, iterative_solver_kernels.cuh:204-238).
+// alpha from rho and p.q (alphaKernel, iterative_solver_kernels.cuh:169-202);
+// the step is applied in S, so it must also be finite in S.
 template <typename S>
-__device__ void finish_update(SolverState* st, int mode, double rr, double rz)
+__device__ __forceinline__ bool step_length(double rho, double den, double& alpha)
 {
-    if (mode == kModeInit)
+    alpha = 0.0;
+    if (rho == 0.0 && den == 0.0) return true;
+    if (isfinite(rho) && isfinite(den) && rho > 0.0 && den > 0.0)
     {
-        st->rr0 = rr;
-        st->rr = rr;
-        const double initial = sqrt(rr);
-        st->tol = fmax(st->abs_tol, st->rel_tol * initial);
-        st->rho = rz;
-        st->beta = 0.0;
-        if (!isfinite(rr)) st->done = kNonFinite;
-        else if (initial <= st->tol) st->done = kConverged;
-        else if (st->max_iterations <= 0) st->done = kIterationLimit;
-        return;
+        alpha = rho / den;
+        if (isfinite(alpha) && isfinite(static_cast<S>(alpha))) return true;
     }
-    st->rr = rr;
-    st->iteration += 1;
-    if (!isfinite(rr) || rr < 0.0)
-    {
-        st->done = kNonFinite;
-        return;
-    }
-    if (sqrt(rr) <= st->tol)
-    {
-        st->done = kConverged;
-        return;
-    }
-    if (st->iteration >= st->max_iterations)
-    {
-        st->done = kIterationLimit;
-        return;
-    }
-    const double rho = st->rho;
-    double beta = 0.0;
-    bool breakdown = false;
-    if (rho == 0.0 && rz == 0.0)
-    {
-        beta = 0.0;
-    }
-    else if (isfinite(rho) && isfinite(rz) && rho > 0.0 && rz > 0.0)
-    {
-        beta = rz / rho;
-        if (!isfinite(beta) || !isfinite(static_cast<S>(beta)))
-        {
-            breakdown = true;
-            beta = 0.0;
-        }
-    }
-    else
-    {
-        breakdown = true;
-    }
-    st->beta = beta;
-    st->rho = rz;
-    if (breakdown) st->done = kBreakdown;
+    alpha = 0.0;
+    return false;
 }
 
-// x += alpha p, r -= alpha q (STEP), z = M^-1 r, partial r.r and r.z.  A unit
-// is one row (None / Jacobi) or one diagonal block of bsize rows (Block).
+// Gathered entries of p_new = z + beta p_old, formed on the fly.
+template <typename S>
+struct DirectionSource
+{
+    const S* __restrict__ v;
+    const S* __restrict__ z;
+    const S* __restrict__ p_old;
+    S beta;
+    using V = S;
+    using G = S;
+    __device__ __forceinline__ S value(int64_t k) const { return v[k]; }
+    template <typename I>
+    __device__ __forceinline__ S gather(I c) const
+    {
+        return fma_s(beta, p_old[c], z[c]);
+    }
+    __device__ __forceinline__ S mul(S a, S b) const { return a * b; }
+};
+
+// First kernel of iteration t (slot par = t % 2), kDirBlock threads.
+template <typename S, typename I>
+__global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, const I* __restrict__ ro,
+                                                           const I* __restrict__ ci, const S* __restrict__ v,
+                                                           const S* __restrict__ z, const S* __restrict__ p_old,
+                                                           S* __restrict__ p_new, S* __restrict__ q,
+                                                           SolverState* st, double* partials, int par,
+                                                           int count_res)
+{
+    if (st->done != kRunning) return;
+    double res[2];
+    reduce_partials<2>(partials + kPartialsRes, count_res, res);
+    const double rr = res[0], rz = res[1];
+    const int completed = st->iter[1 - par] + 1;
+    const double tol = completed == 0 ? fmax(st->abs_tol, st->rel_tol * sqrt(rr)) : st->tol;
+    int code = kRunning;
+    double beta = 0.0;
+    if (!isfinite(rr) || rr < 0.0) code = completed == 0 ? kInitNonFinite : kNonFinite;
+    else if (sqrt(rr) <= tol) code = kConverged;
+    else if (completed == 0 && st->bad_row != ~0ull) code = kBadDiagonal;
+    else if (completed >= st->max_iterations) code = kIterationLimit;
+    else if (completed > 0)
+    {
+        // betaKernel, iterative_solver_kernels.cuh:204-238
+        const double rho_prev = st->rho[1 - par];
+        if (rho_prev == 0.0 && rz == 0.0) beta = 0.0;
+        else if (isfinite(rho_prev) && isfinite(rz) && rho_prev > 0.0 && rz > 0.0)
+        {
+            beta = rz / rho_prev;
+            if (!isfinite(beta) || !isfinite(static_cast<S>(beta))) code = kBreakdown;
+        }
+        else code = kBreakdown;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        st->rr = rr;
+        st->iteration = completed;
+        st->iter[par] = completed;
+        st->rho[par] = rz;
+        if (completed == 0)
+        {
+            st->rr0 = rr;
+            st->tol = tol;
+        }
+        if (code != kRunning) st->done = code;
+    }
+    if (code != kRunning) return;
+
+    __shared__ S lds_all[kDirWaves][kStreamCap];
+    S* lds = lds_all[threadIdx.x / kWave];
+    const S b = static_cast<S>(beta);
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    int64_t begin, end;
+    block_range(chunk_count(rows), kDirWaves, begin, end);
+    double den = 0.0;
+    for (int64_t c = begin + wave; c < end; c += kDirWaves)
+    {
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lds, DirectionSource<S>{v, z, p_old, b}, s, e);
+        const int64_t row = row0 + lane;
+        if (row < rows)
+        {
+            const S pn = fma_s(b, p_old[row], z[row]);
+            p_new[row] = pn;
+            q[row] = acc;
+            den = __builtin_fma(static_cast<double>(pn), static_cast<double>(acc), den);
+        }
+    }
+    double part[1] = {den};
+    publish_partials<1>(part, partials + kPartialsDen);
+}
+
+// Second kernel of iteration t (STEP), or the preconditioning half of the
+// initialisation (INIT) and of a residual refresh (REFRESH).  A unit is one
+// row (None / Jacobi) or one diagonal block of bsize rows (Block).
 template <typename S, int PRE, int MODE>
 __global__ __launch_bounds__(kBlock) void k_update(int64_t units, int64_t bsize, S* __restrict__ x,
                                                    S* __restrict__ r, S* __restrict__ z, const S* __restrict__ p,
                                                    const S* __restrict__ q, const S* __restrict__ inv,
-                                                   SolverState* st, double* partials)
+                                                   SolverState* st, double* partials, int par, int count_den)
 {
     if (st->done != kRunning) return;
-    const S alpha = static_cast<S>(st->alpha);
-    int64_t begin, end;
-    block_rows(units, kBlock, begin, end);
-    double rr = 0.0, rz = 0.0;
-    for (int64_t u = begin + threadIdx.x; u < end; u += kBlock)
+    S alpha = S(0);
+    if (MODE == kModeStep)
     {
-        if (PRE == kPreBlock)
+        double den[1];
+        reduce_partials<1>(partials + kPartialsDen, count_den, den);
+        double a = 0.0;
+        const bool ok = step_length<S>(st->rho[par], den[0], a);
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            st->alpha = a;
+            if (!ok) st->done = kBreakdown;
+        }
+        if (!ok) return;
+        alpha = static_cast<S>(a);
+    }
+    double rr = 0.0, rz = 0.0;
+    if (PRE == kPreBlock)
+    {
+        int64_t begin, end;
+        block_range(units, kBlock, begin, end);
+        for (int64_t u = begin + threadIdx.x; u < end; u += kBlock)
         {
             S ri[kMaxBlock];
             const int64_t base = u * bsize;
@@ -166,53 +240,88 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t units, int64_t bsize,
                 rz = __builtin_fma(static_cast<double>(ri[j]), static_cast<double>(zj), rz);
             }
         }
-        else
+    }
+    else
+    {
+        // four independent coalesced units per thread per step
+        constexpr int U = 4;
+        int64_t begin, end;
+        block_range(units, U * kBlock, begin, end);
+        for (int64_t base = begin + threadIdx.x; base < end; base += U * kBlock)
         {
-            S ri;
-            if (MODE == kModeStep)
+            S rv[U], zv[U];
+#pragma unroll
+            for (int j = 0; j < U; ++j)
             {
-                x[u] = fma_s(alpha, p[u], x[u]);
-                ri = fma_s(-alpha, q[u], r[u]);
-                r[u] = ri;
+                const int64_t u = base + j * kBlock;
+                if (u < end)
+                {
+                    S ri = r[u];
+                    if (MODE == kModeStep)
+                    {
+                        const S pu = p[u], qu = q[u], xu = x[u];
+                        x[u] = fma_s(alpha, pu, xu);
+                        ri = fma_s(-alpha, qu, ri);
+                        r[u] = ri;
+                    }
+                    rv[j] = ri;
+                    zv[j] = PRE == kPreJacobi ? inv[u] * ri : ri;
+                    if (PRE == kPreJacobi) z[u] = zv[j];
+                }
+                else
+                {
+                    rv[j] = S(0);
+                    zv[j] = S(0);
+                }
             }
-            else
+#pragma unroll
+            for (int j = 0; j < U; ++j)
             {
-                ri = r[u];
+                rr = __builtin_fma(static_cast<double>(rv[j]), static_cast<double>(rv[j]), rr);
+                rz = __builtin_fma(static_cast<double>(rv[j]), static_cast<double>(zv[j]), rz);
             }
-            S zi = ri;
-            if (PRE == kPreJacobi)
-            {
-                zi = inv[u] * ri;
-                z[u] = zi;
-            }
-            rr = __builtin_fma(static_cast<double>(ri), static_cast<double>(ri), rr);
-            rz = __builtin_fma(static_cast<double>(ri), static_cast<double>(zi), rz);
         }
     }
     double part[2] = {rr, rz};
-    double total[2];
-    if (last_block_total<2>(part, partials, &st->counters[1], total) && threadIdx.x == 0)
-        finish_update<S>(st, MODE, total[0], total[1]);
+    publish_partials<2>(part, partials + kPartialsRes);
 }
 
-// x += alpha p (first half of a refresh iteration).
+// First half of a refresh iteration: alpha from p.q, x += alpha p.
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_axpy_x(int64_t rows, S* __restrict__ x, const S* __restrict__ p,
-                                                   const SolverState* st)
+                                                   SolverState* st, double* partials, int par, int count_den)
 {
     if (st->done != kRunning) return;
-    const S alpha = static_cast<S>(st->alpha);
+    double den[1];
+    reduce_partials<1>(partials + kPartialsDen, count_den, den);
+    double a = 0.0;
+    const bool ok = step_length<S>(st->rho[par], den[0], a);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        st->alpha = a;
+        if (!ok) st->done = kBreakdown;
+    }
+    if (!ok) return;
+    const S alpha = static_cast<S>(a);
     int64_t begin, end;
-    block_rows(rows, kBlock, begin, end);
+    block_range(rows, kBlock, begin, end);
     for (int64_t i = begin + threadIdx.x; i < end; i += kBlock) x[i] = fma_s(alpha, p[i], x[i]);
 }
 
-// Copies the squared residuals into a fixed-solve state (toDoubleKernel /
+// Squared residuals of a fixed-iteration solve (toDoubleKernel /
 // toDoubleCheckedKernel, iterative_solver_kernels.cuh:240-252).
-__global__ void k_fixed_out(const SolverState* st, double* out)
+__global__ __launch_bounds__(kBlock) void k_fixed_out(const SolverState* st, const double* partials, int count_res,
+                                                      int zero_iterations, double* out)
 {
-    out[0] = st->rr0;
-    out[1] = (st->done == kBreakdown || st->done == kNonFinite) ? NAN : st->rr;
+    double res[2];
+    reduce_partials<2>(partials + kPartialsRes, count_res, res);
+    if (threadIdx.x == 0)
+    {
+        const int d = st->done;
+        const bool bad = d == kBreakdown || d == kNonFinite || d == kInitNonFinite || d == kBadDiagonal;
+        out[0] = zero_iterations ? res[0] : st->rr0;
+        out[1] = bad ? NAN : res[0];
+    }
 }
 
 // ---- host side --------------------------------------------------------------
@@ -225,14 +334,15 @@ struct Problem
     const void* inv_blocks;  // block preconditioner (device), or null
     int64_t bsize;
     int pre;                 // Pre
-    int lanes;
+    int grid_rows;           // workgroups of the row kernels (publish p.q)
+    int grid_units;          // workgroups of the unit kernels (publish r.r, r.z)
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
     double* partials;
 };
 
-template <typename S, typename I, int L>
+template <typename S, typename I>
 struct Calls
 {
     static const I* ro(const pm_csr* a)
@@ -246,118 +356,92 @@ struct Calls
         else return a->col_indices;
     }
 
-    static int grid_units(int64_t units)
-    {
-        int64_t g = ceil_div(units > 0 ? units : 1, kBlock);
-        return static_cast<int>(g < kMaxPartials ? g : kMaxPartials);
-    }
+    static int full_grid(int64_t rows) { return static_cast<int>(ceil_div(chunk_count(rows), kWavesPerBlock)); }
 
     template <int PRE, int MODE>
-    static KernelCall update(const Problem& pb)
+    static KernelCall update_p(const Problem& pb, const void* p, int par)
     {
         const int64_t units = PRE == kPreBlock ? pb.a->rows / pb.bsize : pb.a->rows;
         S* zz = PRE == kPreNone ? static_cast<S*>(pb.r) : static_cast<S*>(pb.z);
         const S* inv = PRE == kPreBlock ? static_cast<const S*>(pb.inv_blocks) : static_cast<const S*>(pb.inv);
-        return make_call(k_update<S, PRE, MODE>, dim3(grid_units(units)), dim3(kBlock), units, pb.bsize,
-                         static_cast<S*>(pb.x), static_cast<S*>(pb.r), zz, static_cast<const S*>(pb.p0),
-                         static_cast<const S*>(pb.q), inv, pb.st, pb.partials);
-    }
-
-    template <int PRE, int MODE>
-    static KernelCall update_p(const Problem& pb, const void* p)
-    {
-        const int64_t units = PRE == kPreBlock ? pb.a->rows / pb.bsize : pb.a->rows;
-        S* zz = PRE == kPreNone ? static_cast<S*>(pb.r) : static_cast<S*>(pb.z);
-        const S* inv = PRE == kPreBlock ? static_cast<const S*>(pb.inv_blocks) : static_cast<const S*>(pb.inv);
-        return make_call(k_update<S, PRE, MODE>, dim3(grid_units(units)), dim3(kBlock), units, pb.bsize,
+        return make_call(k_update<S, PRE, MODE>, dim3(pb.grid_units), dim3(kBlock), units, pb.bsize,
                          static_cast<S*>(pb.x), static_cast<S*>(pb.r), zz, static_cast<const S*>(p),
-                         static_cast<const S*>(pb.q), inv, pb.st, pb.partials);
+                         static_cast<const S*>(pb.q), inv, pb.st, pb.partials, par, pb.grid_rows);
     }
 
     template <int MODE>
-    static KernelCall update_any(const Problem& pb, const void* p)
+    static KernelCall update_any(const Problem& pb, const void* p, int par)
     {
         switch (pb.pre)
         {
-        case kPreNone: return update_p<kPreNone, MODE>(pb, p);
-        case kPreJacobi: return update_p<kPreJacobi, MODE>(pb, p);
-        default: return update_p<kPreBlock, MODE>(pb, p);
+        case kPreNone: return update_p<kPreNone, MODE>(pb, p, par);
+        case kPreJacobi: return update_p<kPreJacobi, MODE>(pb, p, par);
+        default: return update_p<kPreBlock, MODE>(pb, p, par);
         }
     }
 
     static void init(const Problem& pb, std::vector<KernelCall>& out)
     {
-        const int grid = grid_for_rows(pb.a->rows, L);
-        const S* z_for_init = nullptr;
-        (void)z_for_init;
+        const int grid = full_grid(pb.a->rows);
         if (pb.pre == kPreJacobi)
-            out.push_back(make_call(k_residual<S, I, L, true, true>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
+            out.push_back(make_call(k_residual<S, I, true, true>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
                                     ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
                                     static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(pb.inv),
                                     static_cast<S*>(pb.p0), pb.st));
         else
-            out.push_back(make_call(k_residual<S, I, L, false, true>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
+            out.push_back(make_call(k_residual<S, I, false, true>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
                                     ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
                                     static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(nullptr),
                                     static_cast<S*>(pb.p0), pb.st));
-        out.push_back(update_any<kModeInit>(pb, pb.p0));
+        out.push_back(update_any<kModeInit>(pb, pb.p0, 0));
     }
 
-    // One iteration: t is the global iteration index (parity picks p buffers).
+    // Iteration t: t % 2 picks the p buffers and the state slots.
     static void iteration(const Problem& pb, int64_t t, bool refresh, std::vector<KernelCall>& out)
     {
-        const int grid = grid_for_rows(pb.a->rows, L);
-        const void* p_old = (t % 2 == 0) ? pb.p0 : pb.p1;
-        void* p_new = (t % 2 == 0) ? pb.p1 : pb.p0;
+        const int par = static_cast<int>(t % 2);
+        const void* p_old = par == 0 ? pb.p0 : pb.p1;
+        void* p_new = par == 0 ? pb.p1 : pb.p0;
         const S* zsrc = pb.pre == kPreNone ? static_cast<const S*>(pb.r) : static_cast<const S*>(pb.z);
-        out.push_back(make_call(k_direction_spmv<S, I, L>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a), ci(pb.a),
-                                static_cast<const S*>(pb.a->values), zsrc, static_cast<const S*>(p_old),
-                                static_cast<S*>(p_new), static_cast<S*>(pb.q), pb.st, pb.partials));
+        out.push_back(make_call(k_direction_spmv<S, I>, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a),
+                                ci(pb.a), static_cast<const S*>(pb.a->values), zsrc, static_cast<const S*>(p_old),
+                                static_cast<S*>(p_new), static_cast<S*>(pb.q), pb.st, pb.partials, par,
+                                pb.grid_units));
         if (!refresh)
         {
-            out.push_back(update_any<kModeStep>(pb, p_new));
+            out.push_back(update_any<kModeStep>(pb, p_new, par));
             return;
         }
-        out.push_back(make_call(k_axpy_x<S>, dim3(grid_units(pb.a->rows)), dim3(kBlock), pb.a->rows,
-                                static_cast<S*>(pb.x), static_cast<const S*>(p_new), pb.st));
-        out.push_back(make_call(k_residual<S, I, L, false, false>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
-                                ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
-                                static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(nullptr),
-                                static_cast<S*>(nullptr), pb.st));
-        out.push_back(update_any<kModeRefresh>(pb, p_new));
+        out.push_back(make_call(k_axpy_x<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<S*>(pb.x),
+                                static_cast<const S*>(p_new), pb.st, pb.partials, par, pb.grid_rows));
+        out.push_back(make_call(k_residual<S, I, false, false>, dim3(full_grid(pb.a->rows)), dim3(kBlock),
+                                pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
+                                static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x), static_cast<S*>(pb.r),
+                                static_cast<S*>(nullptr), static_cast<S*>(nullptr), pb.st));
+        out.push_back(update_any<kModeRefresh>(pb, p_new, par));
     }
-};
 
-template <typename S, typename I>
-struct BuildCalls
-{
-    template <int L>
-    struct F
+    static void build(const Problem& pb, int what, int64_t first, int64_t count, int refresh_interval,
+                      std::vector<KernelCall>& out)
     {
-        static int run(const Problem& pb, int what, int64_t first, int64_t count, int refresh_interval,
-                       std::vector<KernelCall>& out)
+        if (what == 0)
         {
-            if (what == 0)
-            {
-                Calls<S, I, L>::init(pb, out);
-                return PM_OK;
-            }
-            for (int64_t t = first; t < first + count; ++t)
-            {
-                const bool refresh = refresh_interval > 0 && ((t + 1) % refresh_interval) == 0;
-                Calls<S, I, L>::iteration(pb, t, refresh, out);
-            }
-            return PM_OK;
+            init(pb, out);
+            return;
         }
-    };
+        for (int64_t t = first; t < first + count; ++t)
+        {
+            const bool refresh = refresh_interval > 0 && ((t + 1) % refresh_interval) == 0;
+            iteration(pb, t, refresh, out);
+        }
+    }
 };
 
 template <typename S>
-int build_calls(const Problem& pb, int what, int64_t first, int64_t count, int refresh, std::vector<KernelCall>& out)
+void build_calls(const Problem& pb, int what, int64_t first, int64_t count, int refresh, std::vector<KernelCall>& out)
 {
-    if (use_compact(pb.a))
-        return dispatch_lanes<BuildCalls<S, int32_t>::template F>(pb.lanes, pb, what, first, count, refresh, out);
-    return dispatch_lanes<BuildCalls<S, int64_t>::template F>(pb.lanes, pb, what, first, count, refresh, out);
+    if (use_compact(pb.a)) Calls<S, int32_t>::build(pb, what, first, count, refresh, out);
+    else Calls<S, int64_t>::build(pb, what, first, count, refresh, out);
 }
 
 int validate_options(const pm_solver_options* o)
@@ -400,6 +484,9 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     pb.inv = base + 5 * vec;
     pb.st = ws->state.as<SolverState>();
     pb.partials = ws->partials.as<double>();
+    const int64_t units = pb.pre == kPreBlock ? n / pb.bsize : n;
+    pb.grid_rows = static_cast<int>(std::min<int64_t>(ceil_div(chunk_count(n), kDirWaves), kDirGrid));
+    pb.grid_units = solver_grid_units(units);
     return PM_OK;
 }
 
@@ -408,29 +495,28 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
     // workspace addresses are part of the key: a grown workspace must never
     // replay a plan that still points at its old buffers
     char buf[640];
-    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%d|%lld|%lld|%d|%lld", kind,
+    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld", kind,
                   pb.r, static_cast<void*>(pb.st), static_cast<void*>(pb.partials),
                   (const void*)pb.a->row_offsets, (const void*)pb.a->row_offsets32, (const void*)pb.a->col_indices,
                   (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
                   static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
-                  pb.lanes, static_cast<long long>(first), static_cast<long long>(count), refresh,
+                  static_cast<long long>(first), static_cast<long long>(count), refresh,
                   static_cast<long long>(pb.bsize));
     return buf;
 }
 
 template <typename S>
-int get_plan(pm_workspace* ws, const Problem& pb, int what, int64_t first, int64_t count, int refresh, Plan** out)
+Plan* get_plan(pm_workspace* ws, const Problem& pb, int what, int64_t first, int64_t count, int refresh)
 {
     const std::string key = plan_key(pb, what == 0 ? "init" : "chunk", first, count, refresh);
     Plan* plan = ws->plans->find(key);
     if (!plan)
     {
         auto fresh = std::make_unique<Plan>();
-        PM_TRY(build_calls<S>(pb, what, first, count, refresh, fresh->calls));
+        build_calls<S>(pb, what, first, count, refresh, fresh->calls);
         plan = ws->plans->insert(key, std::move(fresh));
     }
-    *out = plan;
-    return PM_OK;
+    return plan;
 }
 
 struct Policy
@@ -438,7 +524,6 @@ struct Policy
     int chunk = 16;
     bool graph = true;
     int refresh = 0;
-    int lanes = 0;
 };
 
 Policy resolve_policy(const pm_solve_policy* p)
@@ -449,7 +534,6 @@ Policy resolve_policy(const pm_solve_policy* p)
         if (p->chunk_iterations > 0) out.chunk = p->chunk_iterations;
         if (p->use_graph >= 0) out.graph = p->use_graph != 0;
         out.refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
-        out.lanes = p->lanes_per_row;
     }
     if (out.chunk % 2) out.chunk += 1;
     if (out.refresh > 0)
@@ -461,6 +545,30 @@ Policy resolve_policy(const pm_solve_policy* p)
         out.chunk = std::max(k, 2);
     }
     return out;
+}
+
+int finish_report(const SolverState& fin, const pm_solver_options* options, pm_solver_report* report)
+{
+    pm_solver_report rep{};
+    rep.initial_residual = std::sqrt(fin.rr0);
+    rep.final_residual = std::sqrt(fin.rr);
+    rep.iterations = fin.iteration;
+    switch (fin.done)
+    {
+    case kInitNonFinite: return fail(PM_RUNTIME_ERROR, "iterative solver initial residual is not finite");
+    case kBadDiagonal:
+        return fail(PM_RUNTIME_ERROR, "pcg Jacobi diagonal is missing, non-positive, or near zero at row %llu",
+                    fin.bad_row);
+    case kBreakdown:
+    case kNonFinite: return fail(PM_RUNTIME_ERROR, "iterative solver breakdown: matrix is not numerically SPD");
+    default: break;
+    }
+    rep.converged = fin.done == kConverged ? 1 : 0;
+    if (report) *report = rep;
+    if (!rep.converged && options->require_convergence)
+        return fail(PM_RUNTIME_ERROR, "iterative solver did not converge in %d iterations; final residual=%g",
+                    rep.iterations, rep.final_residual);
+    return PM_OK;
 }
 
 template <typename S>
@@ -476,12 +584,10 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     pb.inv_blocks = inv_blocks;
     pb.bsize = pre == kPreBlock ? bsize : 1;
     pb.pre = pre;
-    pb.lanes = policy.lanes > 0 ? policy.lanes : tuned_lanes(a);
     PM_TRY(prepare(ws, pb, stream));
-
-    pm_solver_report rep{};
     if (a->rows == 0)
     {
+        pm_solver_report rep{};
         rep.converged = 1;
         if (report) *report = rep;
         return PM_OK;
@@ -489,72 +595,44 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, stream, pb.st, options->max_iterations,
                        options->relative_tolerance, options->absolute_tolerance);
     PM_HIP(hipGetLastError());
-    Plan* init = nullptr;
-    PM_TRY(get_plan<S>(ws, pb, 0, 0, 0, 0, &init));
-    PM_TRY(init->run(stream, false));
+    PM_TRY(get_plan<S>(ws, pb, 0, 0, 0, 0)->run(stream, false));
+
+    // iteration t = max_iterations must run its first kernel to record the
+    // final decision, so chunks cover iterations [0, max_iterations].
+    const int64_t K = policy.chunk;
+    const int64_t needed = static_cast<int64_t>(options->max_iterations) + 1;
     SolverState* hs = ws->host_state;
-    PM_HIP(hipMemcpyAsync(hs, pb.st, sizeof(SolverState), hipMemcpyDeviceToHost, stream));
-    PM_HIP(hipStreamSynchronize(stream));
-    SolverState fin = *hs;
-    rep.initial_residual = std::sqrt(fin.rr0);
-    rep.final_residual = rep.initial_residual;
-    if (!std::isfinite(fin.rr0))
-        return fail(PM_RUNTIME_ERROR, "iterative solver initial residual is not finite");
-    if (fin.done == kConverged)
-    {
-        rep.converged = 1;
-        if (report) *report = rep;
+    SolverState fin{};
+    std::deque<int64_t> inflight;
+    int64_t launched = 0;
+    int64_t chunk_id = 0;
+    auto issue = [&]() -> int {
+        const int64_t phase = policy.refresh > 0 ? launched % policy.refresh : 0;
+        Plan* plan = get_plan<S>(ws, pb, 1, phase, K, policy.refresh);
+        PM_TRY(plan->run(stream, policy.graph));
+        const int slot = static_cast<int>(chunk_id % 2);
+        PM_HIP(hipMemcpyAsync(hs + slot, pb.st, sizeof(SolverState), hipMemcpyDeviceToHost, stream));
+        PM_HIP(hipEventRecord(ws->events[slot], stream));
+        inflight.push_back(chunk_id);
+        ++chunk_id;
+        launched += K;
         return PM_OK;
-    }
-    if (pre == kPreJacobi && fin.bad_row != ~0ull)
-        return fail(PM_RUNTIME_ERROR, "pcg Jacobi diagonal is missing, non-positive, or near zero at row %llu",
-                    fin.bad_row);
-    if (fin.done == kRunning)
+    };
+    PM_TRY(issue());
+    while (true)
     {
-        const int64_t K = policy.chunk;
-        std::deque<int64_t> inflight;
-        int64_t launched = 0;
-        int64_t chunk_id = 0;
-        auto issue = [&]() -> int {
-            Plan* plan = nullptr;
-            const int64_t first = launched;
-            // chunk plans depend only on (first mod refresh interval) and parity; K is
-            // even and divides the interval, so key on the phase.
-            const int64_t phase = policy.refresh > 0 ? first % policy.refresh : 0;
-            PM_TRY(get_plan<S>(ws, pb, 1, phase, K, policy.refresh, &plan));
-            PM_TRY(plan->run(stream, policy.graph));
-            const int slot = static_cast<int>(chunk_id % 2);
-            PM_HIP(hipMemcpyAsync(hs + slot, pb.st, sizeof(SolverState), hipMemcpyDeviceToHost, stream));
-            PM_HIP(hipEventRecord(ws->events[slot], stream));
-            inflight.push_back(chunk_id);
-            ++chunk_id;
-            launched += K;
-            return PM_OK;
-        };
-        PM_TRY(issue());
-        while (true)
-        {
-            if (launched < options->max_iterations && inflight.size() < 2) PM_TRY(issue());
-            const int64_t oldest = inflight.front();
-            inflight.pop_front();
-            const int slot = static_cast<int>(oldest % 2);
-            PM_HIP(hipEventSynchronize(ws->events[slot]));
-            fin = hs[slot];
-            if (fin.done != kRunning) break;
-            if (inflight.empty() && launched >= options->max_iterations) break;
-        }
+        if (launched < needed && inflight.size() < 2) PM_TRY(issue());
+        const int64_t oldest = inflight.front();
+        inflight.pop_front();
+        const int slot = static_cast<int>(oldest % 2);
+        PM_HIP(hipEventSynchronize(ws->events[slot]));
+        fin = hs[slot];
+        if (fin.done != kRunning) break;
+        if (inflight.empty() && launched >= needed)
+            return fail(PM_RUNTIME_ERROR, "iterative solver stopped without a final state");
     }
     PM_HIP(hipEventRecord(ws->done_event, stream));
-    rep.iterations = fin.iteration;
-    rep.final_residual = std::sqrt(fin.rr);
-    if (fin.done == kBreakdown || fin.done == kNonFinite)
-        return fail(PM_RUNTIME_ERROR, "iterative solver breakdown: matrix is not numerically SPD");
-    rep.converged = fin.done == kConverged ? 1 : 0;
-    if (report) *report = rep;
-    if (!rep.converged && options->require_convergence)
-        return fail(PM_RUNTIME_ERROR, "iterative solver did not converge in %d iterations; final residual=%g",
-                    rep.iterations, rep.final_residual);
-    return PM_OK;
+    return finish_report(fin, options, report);
 }
 
 int solve_dispatch(const pm_csr* a, const void* rhs, void* x, const void* inv_blocks, int64_t bsize, int pre,
@@ -593,7 +671,6 @@ int fixed_solve(const pm_csr* a, const void* rhs, void* x, int32_t iterations, i
     pb.x = x;
     pb.bsize = 1;
     pb.pre = pre;
-    pb.lanes = tuned_lanes(a);
     PM_TRY(prepare(ws, pb, stream));
     if (a->rows == 0)
     {
@@ -601,13 +678,15 @@ int fixed_solve(const pm_csr* a, const void* rhs, void* x, int32_t iterations, i
     }
     else
     {
+        // tolerance 0: every submitted iteration runs unless r becomes exactly 0
         hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, stream, pb.st, iterations, 0.0, 0.0);
         PM_HIP(hipGetLastError());
         std::vector<KernelCall> calls;
-        PM_TRY(build_calls<S>(pb, 0, 0, 0, 0, calls));
-        PM_TRY(build_calls<S>(pb, 1, 0, iterations, 0, calls));
+        build_calls<S>(pb, 0, 0, 0, 0, calls);
+        build_calls<S>(pb, 1, 0, iterations, 0, calls);
         for (const KernelCall& k : calls) PM_TRY(launch(k, stream));
-        hipLaunchKernelGGL(k_fixed_out, dim3(1), dim3(1), 0, stream, pb.st, state->device);
+        hipLaunchKernelGGL(k_fixed_out, dim3(1), dim3(kBlock), 0, stream, pb.st, pb.partials, pb.grid_units,
+                           iterations == 0 ? 1 : 0, state->device);
         PM_HIP(hipGetLastError());
     }
     PM_HIP(hipEventRecord(state->event, stream));
@@ -720,26 +799,27 @@ namespace pm
 {
 
 // r = b - A x in FP64 (FP32 matrix, FP64 vectors) plus partial ||r||^2, ||b||^2.
-template <typename I, int L>
+template <typename I>
 __global__ __launch_bounds__(kBlock) void k_residual64(int64_t rows, const I* __restrict__ ro,
                                                        const I* __restrict__ ci, const float* __restrict__ v,
                                                        const double* __restrict__ b, const double* __restrict__ x,
                                                        double* __restrict__ r, double* partials, unsigned* counter,
                                                        double* out)
 {
-    constexpr int64_t gpb = kBlock / L;
-    const int group = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
+    __shared__ double lds_all[kWavesPerBlock][kStreamCap];
+    double* lds = lds_all[threadIdx.x / kWave];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     int64_t begin, end;
-    block_rows(rows, gpb, begin, end);
+    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
     double rr = 0.0, bb = 0.0;
-    for (int64_t row = begin + group; row < end; row += gpb)
+    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
     {
-        const int64_t s = ro[row], e = ro[row + 1];
-        double acc = 0.0;
-        for (int64_t k = s + lane; k < e; k += L) acc = __builtin_fma(static_cast<double>(v[k]), x[ci[k]], acc);
-        acc = group_sum<L>(acc);
-        if (lane == 0)
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const double acc = chunk_row_sums<double>(row0, rows, ro, ci, lds, Fp64Source<float, double>{v, x}, s, e);
+        const int64_t row = row0 + lane;
+        if (row < rows)
         {
             const double res = b[row] - acc;
             r[row] = res;
@@ -776,24 +856,6 @@ __global__ void k_accumulate(int64_t n, double* __restrict__ x, const float* __r
 
 __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
-template <typename I>
-struct Residual64
-{
-    template <int L>
-    struct F
-    {
-        static int run(const pm_csr* a, const I* ro, const I* ci, const double* b, const double* x, double* r,
-                       double* partials, unsigned* counter, double* out, hipStream_t stream)
-        {
-            hipLaunchKernelGGL((k_residual64<I, L>), dim3(grid_for_rows(a->rows, L)), dim3(kBlock), 0, stream,
-                               a->rows, ro, ci, static_cast<const float*>(a->values), b, x, r, partials, counter,
-                               out);
-            PM_HIP(hipGetLastError());
-            return PM_OK;
-        }
-    };
-};
-
 int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options* inner, double target,
            int32_t max_outer, const pm_solve_policy* policy, pm_refine_report* report, pm_workspace* ws,
            hipStream_t stream)
@@ -809,8 +871,11 @@ int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options*
     float* d = reinterpret_cast<float*>(base + r_bytes + f_bytes);
     unsigned* counter = reinterpret_cast<unsigned*>(base + r_bytes + 2 * f_bytes);
     double* out = reinterpret_cast<double*>(base + r_bytes + 2 * f_bytes + 256);
-    const int lanes = policy && policy->lanes_per_row > 0 ? policy->lanes_per_row : tuned_lanes(a);
+    // the refinement residual uses the upper half of the partial buffer, away
+    // from the solver's reduction slots
+    double* rpartials = ws->partials.as<double>() + 4 * kMaxPartials;
     const int grid = static_cast<int>(std::min<int64_t>(ceil_div(n > 0 ? n : 1, kBlock), 8192));
+    const int rgrid = solver_grid_chunks(n);
 
     pm_solver_options in = *inner;
     in.require_convergence = 0;
@@ -820,11 +885,14 @@ int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options*
     {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, stream, counter);
         if (use_compact(a))
-            PM_TRY(dispatch_lanes<Residual64<int32_t>::F>(lanes, a, a->row_offsets32, a->col_indices32, b, x, r64,
-                                                          ws->partials.as<double>(), counter, out, stream));
+            hipLaunchKernelGGL((k_residual64<int32_t>), dim3(rgrid), dim3(kBlock), 0, stream, n, a->row_offsets32,
+                               a->col_indices32, static_cast<const float*>(a->values), b, x, r64, rpartials, counter,
+                               out);
         else
-            PM_TRY(dispatch_lanes<Residual64<int64_t>::F>(lanes, a, a->row_offsets, a->col_indices, b, x, r64,
-                                                          ws->partials.as<double>(), counter, out, stream));
+            hipLaunchKernelGGL((k_residual64<int64_t>), dim3(rgrid), dim3(kBlock), 0, stream, n, a->row_offsets,
+                               a->col_indices, static_cast<const float*>(a->values), b, x, r64, rpartials, counter,
+                               out);
+        PM_HIP(hipGetLastError());
         double host[2];
         PM_HIP(hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, stream));
         PM_HIP(hipStreamSynchronize(stream));

@@ -30,13 +30,13 @@ struct SpmvRun
     {
         static int run(const pm_csr* a, const void* x, void* y, hipStream_t stream)
         {
-            const int grid = grid_for_rows(a->rows, L);
+            const int grid = static_cast<int>(std::min<int64_t>(ceil_div(a->rows, kBlock / L), 1 << 20));
             if (use_compact(a))
-                hipLaunchKernelGGL((k_spmv<S, int32_t, L>), dim3(grid), dim3(kBlock), 0, stream, a->rows,
+                hipLaunchKernelGGL((k_spmv_lanes<S, int32_t, L>), dim3(grid), dim3(kBlock), 0, stream, a->rows,
                                    a->row_offsets32, a->col_indices32, static_cast<const S*>(a->values),
                                    static_cast<const S*>(x), static_cast<S*>(y));
             else
-                hipLaunchKernelGGL((k_spmv<S, int64_t, L>), dim3(grid), dim3(kBlock), 0, stream, a->rows,
+                hipLaunchKernelGGL((k_spmv_lanes<S, int64_t, L>), dim3(grid), dim3(kBlock), 0, stream, a->rows,
                                    a->row_offsets, a->col_indices, static_cast<const S*>(a->values),
                                    static_cast<const S*>(x), static_cast<S*>(y));
             PM_HIP(hipGetLastError());
@@ -45,9 +45,27 @@ struct SpmvRun
     };
 };
 
+template <typename S>
+int spmv_stream(const pm_csr* a, const void* x, void* y, hipStream_t stream)
+{
+    const int grid = static_cast<int>(ceil_div(chunk_count(a->rows), kWavesPerBlock));
+    if (use_compact(a))
+        hipLaunchKernelGGL((k_spmv<S, int32_t>), dim3(grid), dim3(kBlock), 0, stream, a->rows, a->row_offsets32,
+                           a->col_indices32, static_cast<const S*>(a->values), static_cast<const S*>(x),
+                           static_cast<S*>(y));
+    else
+        hipLaunchKernelGGL((k_spmv<S, int64_t>), dim3(grid), dim3(kBlock), 0, stream, a->rows, a->row_offsets,
+                           a->col_indices, static_cast<const S*>(a->values), static_cast<const S*>(x),
+                           static_cast<S*>(y));
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
+
+// lanes == 0: wave-stream kernel; lanes >= 1: lanes-per-row kernel.
 int spmv_launch(const pm_csr* a, const void* x, void* y, int lanes, hipStream_t stream)
 {
     if (a->rows == 0) return PM_OK;
+    if (lanes == 0) return a->dtype == PM_F32 ? spmv_stream<float>(a, x, y, stream) : spmv_stream<double>(a, x, y, stream);
     if (a->dtype == PM_F32) return dispatch_lanes<SpmvRun<float>::F>(lanes, a, x, y, stream);
     return dispatch_lanes<SpmvRun<double>::F>(lanes, a, x, y, stream);
 }
@@ -99,40 +117,31 @@ __global__ void k_compact(int64_t rows, int64_t nnz, const int64_t* ro, const in
 }
 
 template <typename S, typename V>
-struct RelresRun
+int relres_stream(const pm_csr* a, const void* b, const void* x, double* partials, unsigned* counter, double* out,
+                  hipStream_t stream)
 {
-    template <int L>
-    struct F
-    {
-        static int run(const pm_csr* a, const void* b, const void* x, double* partials, unsigned* counter,
-                       double* out, hipStream_t stream)
-        {
-            const int grid = grid_for_rows(a->rows, L);
-            if (use_compact(a))
-                hipLaunchKernelGGL((k_relres<S, V, int32_t, L>), dim3(grid), dim3(kBlock), 0, stream, a->rows,
-                                   a->row_offsets32, a->col_indices32, static_cast<const S*>(a->values),
-                                   static_cast<const V*>(b), static_cast<const V*>(x), partials, counter, out);
-            else
-                hipLaunchKernelGGL((k_relres<S, V, int64_t, L>), dim3(grid), dim3(kBlock), 0, stream, a->rows,
-                                   a->row_offsets, a->col_indices, static_cast<const S*>(a->values),
-                                   static_cast<const V*>(b), static_cast<const V*>(x), partials, counter, out);
-            PM_HIP(hipGetLastError());
-            return PM_OK;
-        }
-    };
-};
+    const int grid = solver_grid_chunks(a->rows);
+    if (use_compact(a))
+        hipLaunchKernelGGL((k_relres<S, V, int32_t>), dim3(grid), dim3(kBlock), 0, stream, a->rows, a->row_offsets32,
+                           a->col_indices32, static_cast<const S*>(a->values), static_cast<const V*>(b),
+                           static_cast<const V*>(x), partials, counter, out);
+    else
+        hipLaunchKernelGGL((k_relres<S, V, int64_t>), dim3(grid), dim3(kBlock), 0, stream, a->rows, a->row_offsets,
+                           a->col_indices, static_cast<const S*>(a->values), static_cast<const V*>(b),
+                           static_cast<const V*>(x), partials, counter, out);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
 
 __global__ void k_zero_counter(unsigned* counter) { *counter = 0u; }
 
 int relres_launch(const pm_csr* a, const void* b, const void* x, int32_t vec_dtype, double* partials,
-                  unsigned* counter, double* out, int lanes, hipStream_t stream)
+                  unsigned* counter, double* out, hipStream_t stream)
 {
     if (a->dtype == PM_F32 && vec_dtype == PM_F32)
-        return dispatch_lanes<RelresRun<float, float>::F>(lanes, a, b, x, partials, counter, out, stream);
-    if (a->dtype == PM_F32)
-        return dispatch_lanes<RelresRun<float, double>::F>(lanes, a, b, x, partials, counter, out, stream);
-    if (vec_dtype == PM_F64)
-        return dispatch_lanes<RelresRun<double, double>::F>(lanes, a, b, x, partials, counter, out, stream);
+        return relres_stream<float, float>(a, b, x, partials, counter, out, stream);
+    if (a->dtype == PM_F32) return relres_stream<float, double>(a, b, x, partials, counter, out, stream);
+    if (vec_dtype == PM_F64) return relres_stream<double, double>(a, b, x, partials, counter, out, stream);
     return fail(PM_INVALID_ARGUMENT, "true residual: FP32 vectors with an FP64 matrix are not supported");
 }
 
@@ -226,7 +235,7 @@ int pm_true_relres(const pm_csr* a, const void* b, const void* x, int32_t vec_dt
     unsigned* counter = reinterpret_cast<unsigned*>(ws->scratch.as<char>() + 1024);
     double* out = reinterpret_cast<double*>(ws->scratch.as<char>() + 2048);
     hipLaunchKernelGGL(k_zero_counter, dim3(1), dim3(1), 0, stream, counter);
-    PM_TRY(relres_launch(a, b, x, vec_dtype, ws->partials.as<double>(), counter, out, tuned_lanes(a), stream));
+    PM_TRY(relres_launch(a, b, x, vec_dtype, ws->partials.as<double>(), counter, out, stream));
     double host[2] = {0, 0};
     PM_HIP(hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, stream));
     PM_HIP(hipStreamSynchronize(stream));

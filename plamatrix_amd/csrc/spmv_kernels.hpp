@@ -1,13 +1,21 @@
 // CSR SpMV family for gfx950.
 //
-// A row is owned by an aligned group of L lanes (L = 1..64, a power of two,
-// chosen per matrix by the autotuner or from the mean row length).  Lane j of
-// the group streams entries start+j, start+j+L, ... so a wavefront reads
-// 64/L consecutive rows' values and column indices as one contiguous span,
-// then the group reduces with a fixed butterfly.  Each workgroup owns one
-// contiguous range of rows; workgroups are renumbered so that every XCD walks
-// a contiguous eighth of the matrix and its L2 keeps the x entries that the
-// neighbouring rows gather.
+// Wave-stream rows: a wavefront owns 64 consecutive rows (lane i <-> row i).
+// It loads the 65 row offsets with one coalesced load, streams the chunk's
+// contiguous values and column indices with back-to-back coalesced loads,
+// gathers x, stages the products in LDS and then every lane sums its own row
+// left to right.  All loads of a chunk are independent, so a wave keeps many
+// requests in flight instead of walking a dependent offset -> column -> x chain
+// row by row.  A chunk whose rows hold more than CAP entries (long rows) falls
+// back to 8 lanes per row with a fixed butterfly.  Workgroups own contiguous
+// chunk ranges and are renumbered so each XCD walks a contiguous eighth of the
+// matrix, keeping the gathered x entries in its own L2.
+//
+// Reductions across workgroups never use atomics or device-scope fences (on a
+// multi-XCD part such a fence writes back the whole L2): each workgroup
+// publishes its FP64 partial sums, and every workgroup of the *next* kernel
+// re-reduces all of them in the same fixed order.  The result is identical in
+// every workgroup and from run to run.
 #pragma once
 
 #include <cmath>
@@ -17,6 +25,11 @@
 
 namespace pm
 {
+
+constexpr int kChunkRows = 64;               // rows per wavefront chunk
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kStreamCap = 1024;             // staged products per wave (LDS)
+constexpr int kSolverGrid = 1024;            // workgroups of the solver kernels (partials per reduction)
 
 template <typename S>
 __device__ __forceinline__ S fma_s(S a, S b, S c);
@@ -44,35 +57,232 @@ __device__ __forceinline__ double scalar_epsilon<double>()
     return 2.220446049250313080847e-16;
 }
 
-// Contiguous row range of this workgroup (XCD-aware), in multiples of `gpb`
-// rows (row groups per block).
-__device__ __forceinline__ void block_rows(int64_t rows, int64_t gpb, int64_t& begin, int64_t& end)
+// LDS hand-off between lanes of one wavefront.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Contiguous range [begin, end) of `units` owned by this workgroup
+// (XCD-aware), rounded to multiples of `step`.
+__device__ __forceinline__ void block_range(int64_t units, int64_t step, int64_t& begin, int64_t& end)
 {
     const int64_t nb = gridDim.x;
     const int64_t lb = xcd_logical_block(blockIdx.x, nb);
-    const int64_t per = ceil_div(ceil_div(rows, nb), gpb) * gpb;
+    const int64_t per = ceil_div(ceil_div(units, nb), step) * step;
     begin = lb * per;
-    end = begin + per < rows ? begin + per : rows;
+    end = begin + per < units ? begin + per : units;
 }
 
-inline int grid_for_rows(int64_t rows, int lanes)
+__host__ __device__ inline int64_t chunk_count(int64_t rows) { return ceil_div(rows > 0 ? rows : 1, kChunkRows); }
+
+// Grid of a kernel that publishes partial sums: at most kSolverGrid workgroups.
+inline int solver_grid_chunks(int64_t rows)
 {
-    const int64_t gpb = kBlock / lanes;
-    int64_t g = ceil_div(rows > 0 ? rows : 1, gpb);
-    return static_cast<int>(g < kMaxPartials ? g : kMaxPartials);
+    const int64_t g = ceil_div(chunk_count(rows), kWavesPerBlock);
+    return static_cast<int>(g < kSolverGrid ? g : kSolverGrid);
 }
 
-// y = A x
-template <typename S, typename I, int L>
+inline int solver_grid_units(int64_t units)
+{
+    const int64_t g = ceil_div(units > 0 ? units : 1, 4 * kBlock);
+    return static_cast<int>(g < kSolverGrid ? g : kSolverGrid);
+}
+
+constexpr int kMaxLoads = kStreamCap / kWave;  // entries per lane of a staged chunk
+
+// Entry sources of chunk_row_sums: value(k) loads the matrix entry, gather(c)
+// the vector entry of column c, mul forms the product in the accumulation type.
+template <typename S>
+struct SpmvSource
+{
+    const S* __restrict__ v;
+    const S* __restrict__ x;
+    using V = S;
+    using G = S;
+    __device__ __forceinline__ S value(int64_t k) const { return v[k]; }
+    template <typename I>
+    __device__ __forceinline__ S gather(I c) const
+    {
+        return x[c];
+    }
+    __device__ __forceinline__ S mul(S a, S b) const { return a * b; }
+};
+
+// Entries of A times x accumulated in FP64 (x of type X).
+template <typename S, typename X>
+struct Fp64Source
+{
+    const S* __restrict__ v;
+    const X* __restrict__ x;
+    using V = S;
+    using G = X;
+    __device__ __forceinline__ S value(int64_t k) const { return v[k]; }
+    template <typename I>
+    __device__ __forceinline__ X gather(I c) const
+    {
+        return x[c];
+    }
+    __device__ __forceinline__ double mul(S a, X b) const { return static_cast<double>(a) * static_cast<double>(b); }
+};
+
+// Row sums of the 64-row chunk starting at row0 (lane i -> row row0 + i).
+// s_out / e_out receive the lane's entry range; rows at or beyond `rows` get
+// an empty range.  Staged path: every value / column load of the chunk is
+// issued before any gather, and every gather before any product, so a wave has
+// up to 2 x 16 independent loads in flight per phase.
+template <typename Acc, typename I, typename Src>
+__device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const I* __restrict__ ro,
+                                              const I* __restrict__ ci, Acc* lds, const Src& src, int64_t& s_out,
+                                              int64_t& e_out)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t last = row0 + kChunkRows < rows ? row0 + kChunkRows : rows;
+    const int64_t r = row0 + lane;
+    const int64_t my_s = static_cast<int64_t>(ro[r < last ? r : last]);
+    int64_t my_e = __shfl_down(my_s, 1, kWave);
+    if (lane == kWave - 1) my_e = static_cast<int64_t>(ro[last]);
+    const int64_t base = __shfl(my_s, 0, kWave);
+    const int64_t total = __shfl(my_e, kWave - 1, kWave) - base;
+    s_out = my_s;
+    e_out = my_e;
+    Acc acc = Acc(0);
+    if (total <= kStreamCap)
+    {
+        const int n_loads = static_cast<int>((total + kWave - 1) / kWave);
+        typename Src::V vv[kMaxLoads];
+        I cc[kMaxLoads];
+#pragma unroll
+        for (int j = 0; j < kMaxLoads; ++j)
+        {
+            if (j < n_loads)
+            {
+                const int64_t k = lane + static_cast<int64_t>(j) * kWave;
+                if (k < total)
+                {
+                    vv[j] = src.value(base + k);
+                    cc[j] = ci[base + k];
+                }
+            }
+        }
+        typename Src::G gg[kMaxLoads];
+#pragma unroll
+        for (int j = 0; j < kMaxLoads; ++j)
+        {
+            if (j < n_loads)
+            {
+                const int64_t k = lane + static_cast<int64_t>(j) * kWave;
+                if (k < total) gg[j] = src.gather(cc[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kMaxLoads; ++j)
+        {
+            if (j < n_loads)
+            {
+                const int64_t k = lane + static_cast<int64_t>(j) * kWave;
+                if (k < total) lds[k] = src.mul(vv[j], gg[j]);
+            }
+        }
+        wave_lds_sync();
+        for (int64_t k = my_s - base; k < my_e - base; ++k) acc += lds[k];
+        wave_lds_sync();
+    }
+    else
+    {
+        // long rows: 8 lanes per row, 8 rows per pass
+        const int group = lane / 8;
+        const int sub = lane % 8;
+        for (int pass = 0; pass < 8; ++pass)
+        {
+            const int row_lane = pass * 8 + group;
+            const int64_t s = __shfl(my_s, row_lane, kWave);
+            const int64_t e = __shfl(my_e, row_lane, kWave);
+            Acc part = Acc(0);
+            for (int64_t k = s + sub; k < e; k += 8) part += src.mul(src.value(k), src.gather(ci[k]));
+            part = group_sum<8>(part);
+            const Acc got = __shfl(part, (lane % 8) * 8, kWave);
+            if (lane / 8 == pass) acc = got;
+        }
+    }
+    return acc;
+}
+
+// Every workgroup sums `count` published partials (NV values each) in the same
+// order; all threads receive the totals.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ partials, int count, double (&out)[NV])
+{
+    __shared__ double lds[NV * 16];
+    __shared__ double result[NV];
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+    {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < count; b += blockDim.x) s += partials[k * kMaxPartials + b];
+        acc[k] = s;
+    }
+    block_sum<NV>(acc, lds);
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) result[k] = acc[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = result[k];
+    __syncthreads();
+}
+
+// Publish this workgroup's NV partial sums at slot blockIdx.x.
+template <int NV>
+__device__ __forceinline__ void publish_partials(double (&v)[NV], double* __restrict__ partials)
+{
+    __shared__ double lds[NV * 16];
+    block_sum<NV>(v, lds);
+    if (threadIdx.x == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[k * kMaxPartials + blockIdx.x] = v[k];
+    }
+}
+
+// y = A x (standalone; any grid size, no reductions).
+template <typename S, typename I>
 __global__ __launch_bounds__(kBlock) void k_spmv(int64_t rows, const I* __restrict__ ro, const I* __restrict__ ci,
                                                  const S* __restrict__ v, const S* __restrict__ x,
                                                  S* __restrict__ y)
+{
+    __shared__ S lds_all[kWavesPerBlock][kStreamCap];
+    S* lds = lds_all[threadIdx.x / kWave];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    int64_t begin, end;
+    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
+    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
+    {
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lds, SpmvSource<S>{v, x}, s, e);
+        if (row0 + lane < rows) y[row0 + lane] = acc;
+    }
+}
+
+// Lanes-per-row SpMV (y = A x) kept for matrices whose rows are long on
+// average, selectable through the autotuner (lanes > 0).
+template <typename S, typename I, int L>
+__global__ __launch_bounds__(kBlock) void k_spmv_lanes(int64_t rows, const I* __restrict__ ro,
+                                                       const I* __restrict__ ci, const S* __restrict__ v,
+                                                       const S* __restrict__ x, S* __restrict__ y)
 {
     constexpr int64_t gpb = kBlock / L;
     const int group = threadIdx.x / L;
     const int lane = threadIdx.x % L;
     int64_t begin, end;
-    block_rows(rows, gpb, begin, end);
+    block_range(rows, gpb, begin, end);
     for (int64_t row = begin + group; row < end; row += gpb)
     {
         const int64_t s = ro[row], e = ro[row + 1];
@@ -87,9 +297,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t rows, const I* __restri
 //   r = b - A x, accumulated in FP64 and rounded to S;
 //   JACOBI: inv[row] = 1/sum(diagonal entries) (FP64, rejecting a missing,
 //           non-finite or <= eps(S)*16*max|a_row| diagonal, as jacobiInverse
-//           in iterative_solver_cpu.cpp:127-160 does); bad rows recorded;
-//   ZERO_P: p0[row] = 0 (first direction update starts from z).
-template <typename S, typename I, int L, bool JACOBI, bool ZERO_P>
+//           in iterative_solver_cpu.cpp:127-160 does); first bad row recorded;
+//   ZERO_P: p0[row] = 0 (the first direction update then starts from z).
+template <typename S, typename I, bool JACOBI, bool ZERO_P>
 __global__ __launch_bounds__(kBlock) void k_residual(int64_t rows, const I* __restrict__ ro,
                                                      const I* __restrict__ ci, const S* __restrict__ v,
                                                      const S* __restrict__ b, const S* __restrict__ x,
@@ -97,48 +307,36 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t rows, const I* __re
                                                      SolverState* st)
 {
     if (st->done != kRunning) return;
-    constexpr int64_t gpb = kBlock / L;
-    const int group = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
+    __shared__ double lds_all[kWavesPerBlock][kStreamCap];
+    double* lds = lds_all[threadIdx.x / kWave];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     int64_t begin, end;
-    block_rows(rows, gpb, begin, end);
-    for (int64_t row = begin + group; row < end; row += gpb)
+    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
+    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
     {
-        const int64_t s = ro[row], e = ro[row + 1];
-        double acc = 0.0, diag = 0.0, scale = 0.0;
-        int found = 0;
-        for (int64_t k = s + lane; k < e; k += L)
-        {
-            const I c = ci[k];
-            const double a = static_cast<double>(v[k]);
-            acc = __builtin_fma(a, static_cast<double>(x[c]), acc);
-            if (JACOBI)
-            {
-                scale = fmax(scale, fabs(a));
-                if (c == row)
-                {
-                    diag += a;
-                    found = 1;
-                }
-            }
-        }
-        acc = group_sum<L>(acc);
-        if (JACOBI)
-        {
-            diag = group_sum<L>(diag);
-#pragma unroll
-            for (int off = L / 2; off > 0; off >>= 1)
-            {
-                scale = fmax(scale, __shfl_xor(scale, off, kWave));
-                found |= __shfl_xor(found, off, kWave);
-            }
-        }
-        if (lane == 0)
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const double acc = chunk_row_sums<double>(row0, rows, ro, ci, lds, Fp64Source<S, S>{v, x}, s, e);
+        const int64_t row = row0 + lane;
+        if (row < rows)
         {
             r[row] = static_cast<S>(static_cast<double>(b[row]) - acc);
             if (ZERO_P) p0[row] = S(0);
             if (JACOBI)
             {
+                double diag = 0.0, scale = 0.0;
+                bool found = false;
+                for (int64_t k = s; k < e; ++k)
+                {
+                    const double a = static_cast<double>(v[k]);
+                    scale = fmax(scale, fabs(a));
+                    if (static_cast<int64_t>(ci[k]) == row)
+                    {
+                        diag += a;
+                        found = true;
+                    }
+                }
                 const double threshold = static_cast<double>(scalar_epsilon<S>()) * scale * 16.0;
                 const double reciprocal = 1.0 / diag;
                 const bool ok = found && isfinite(diag) && diag > threshold && isfinite(reciprocal);
@@ -149,97 +347,28 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t rows, const I* __re
     }
 }
 
-// First PCG kernel of an iteration:
-//   p_new = z + beta * p_old (computed on the fly for every gathered entry and
-//   written once by the row owner; p_old / p_new ping-pong between
-//   iterations), q = A p_new, den = p_new . q (FP64); the last workgroup turns
-//   den into alpha with the breakdown rules of alphaKernel
-//   (iterative_solver_kernels.cuh:169-202).
-template <typename S, typename I, int L>
-__global__ __launch_bounds__(kBlock) void k_direction_spmv(int64_t rows, const I* __restrict__ ro,
-                                                           const I* __restrict__ ci, const S* __restrict__ v,
-                                                           const S* __restrict__ z, const S* __restrict__ p_old,
-                                                           S* __restrict__ p_new, S* __restrict__ q,
-                                                           SolverState* st, double* partials)
-{
-    if (st->done != kRunning) return;
-    const S beta = static_cast<S>(st->beta);
-    constexpr int64_t gpb = kBlock / L;
-    const int group = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
-    int64_t begin, end;
-    block_rows(rows, gpb, begin, end);
-    double den = 0.0;
-    for (int64_t row = begin + group; row < end; row += gpb)
-    {
-        const int64_t s = ro[row], e = ro[row + 1];
-        S acc = S(0);
-        for (int64_t k = s + lane; k < e; k += L)
-        {
-            const I c = ci[k];
-            acc = fma_s(v[k], fma_s(beta, p_old[c], z[c]), acc);
-        }
-        acc = group_sum<L>(acc);
-        if (lane == 0)
-        {
-            const S pn = fma_s(beta, p_old[row], z[row]);
-            p_new[row] = pn;
-            q[row] = acc;
-            den = __builtin_fma(static_cast<double>(pn), static_cast<double>(acc), den);
-        }
-    }
-    double part[1] = {den};
-    double total[1];
-    if (last_block_total<1>(part, partials, &st->counters[0], total) && threadIdx.x == 0)
-    {
-        const double rho = st->rho;
-        const double d = total[0];
-        double alpha = 0.0;
-        bool breakdown = false;
-        if (rho == 0.0 && d == 0.0)
-        {
-            alpha = 0.0;
-        }
-        else if (isfinite(rho) && isfinite(d) && rho > 0.0 && d > 0.0)
-        {
-            alpha = rho / d;
-            // the step is applied in S: it must be finite in S as well
-            if (!isfinite(alpha) || !isfinite(static_cast<S>(alpha)))
-            {
-                breakdown = true;
-                alpha = 0.0;
-            }
-        }
-        else
-        {
-            breakdown = true;
-        }
-        st->alpha = alpha;
-        if (breakdown) st->done = kBreakdown;
-    }
-}
-
-// FP64 true residual partial sums: sum (b - A x)^2 and sum b^2.
-template <typename S, typename V, typename I, int L>
+// FP64 true residual partial sums: sum (b - A x)^2 and sum b^2, finished by
+// the last workgroup (used once per call, outside the iteration).
+template <typename S, typename V, typename I>
 __global__ __launch_bounds__(kBlock) void k_relres(int64_t rows, const I* __restrict__ ro, const I* __restrict__ ci,
                                                    const S* __restrict__ v, const V* __restrict__ b,
                                                    const V* __restrict__ x, double* partials, unsigned* counter,
                                                    double* out)
 {
-    constexpr int64_t gpb = kBlock / L;
-    const int group = threadIdx.x / L;
-    const int lane = threadIdx.x % L;
+    __shared__ double lds_all[kWavesPerBlock][kStreamCap];
+    double* lds = lds_all[threadIdx.x / kWave];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     int64_t begin, end;
-    block_rows(rows, gpb, begin, end);
+    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
     double rr = 0.0, bb = 0.0;
-    for (int64_t row = begin + group; row < end; row += gpb)
+    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
     {
-        const int64_t s = ro[row], e = ro[row + 1];
-        double acc = 0.0;
-        for (int64_t k = s + lane; k < e; k += L)
-            acc = __builtin_fma(static_cast<double>(v[k]), static_cast<double>(x[ci[k]]), acc);
-        acc = group_sum<L>(acc);
-        if (lane == 0)
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const double acc = chunk_row_sums<double>(row0, rows, ro, ci, lds, Fp64Source<S, V>{v, x}, s, e);
+        const int64_t row = row0 + lane;
+        if (row < rows)
         {
             const double bv = static_cast<double>(b[row]);
             const double res = bv - acc;
@@ -256,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_relres(int64_t rows, const I* __rest
     }
 }
 
-// Dispatch helper: calls F<L>() for the runtime lanes-per-row value.
+// Dispatch helper: calls F<L>::run(...) for the runtime lanes-per-row value.
 template <template <int> class F, typename... Args>
 int dispatch_lanes(int lanes, Args&&... args)
 {
@@ -273,7 +402,7 @@ int dispatch_lanes(int lanes, Args&&... args)
     }
 }
 
-// Lanes per row from the mean row length (used when no tuned value exists).
+// Lanes-per-row for the lanes-per-row kernel from the mean row length.
 inline int heuristic_lanes(int64_t rows, int64_t nnz)
 {
     if (rows <= 0) return 1;

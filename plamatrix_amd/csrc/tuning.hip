@@ -1,4 +1,5 @@
-// SpMV autotuner: times every lanes-per-row variant on the caller's matrix
+// SpMV autotuner: times the wave-stream kernel and every lanes-per-row variant
+// on the caller's matrix
 // with HIP events and keeps the fastest, in memory and in a small text cache
 // on disk (PLAMATRIX_AMD_TUNE_CACHE, default ~/.cache/plamatrix_amd/spmv.txt)
 // so later processes start tuned.
@@ -73,7 +74,8 @@ int tuned_lanes(const pm_csr* a)
     load_locked();
     auto it = g_table.find(key_of(a));
     if (it != g_table.end()) return it->second;
-    return heuristic_lanes(a->rows, a->nnz);
+    const double mean = a->rows > 0 ? static_cast<double>(a->nnz) / static_cast<double>(a->rows) : 0.0;
+    return mean <= 32.0 ? 0 : heuristic_lanes(a->rows, a->nnz);
 }
 
 void record_lanes(const pm_csr* a, int lanes)
@@ -101,9 +103,9 @@ int pm_autotune_spmv(const pm_csr* a, const void* x, void* y, int32_t repeats, i
     hipEvent_t start, stop;
     PM_HIP(hipEventCreate(&start));
     PM_HIP(hipEventCreate(&stop));
-    int best = heuristic_lanes(a->rows, a->nnz);
+    int best = 0;
     float best_time = 1e30f;
-    for (int lanes = 1; lanes <= 64; lanes *= 2)
+    for (int lanes = 0; lanes <= 64; lanes = lanes == 0 ? 1 : lanes * 2)
     {
         PM_TRY(spmv_launch(a, x, y, lanes, stream));  // warm-up
         float times[64];
@@ -136,8 +138,9 @@ int pm_autotune_spmv(const pm_csr* a, const void* x, void* y, int32_t repeats, i
 int pm_tune_record(const pm_csr* a, int32_t lanes)
 {
     PM_TRY(check_csr(a, "tune record"));
-    if (lanes < 1 || lanes > 64 || (lanes & (lanes - 1)) != 0)
-        return fail(PM_INVALID_ARGUMENT, "lanes per row must be a power of two in [1, 64], got %d", lanes);
+    if (lanes < 0 || lanes > 64 || (lanes & (lanes - 1)) != 0)
+        return fail(PM_INVALID_ARGUMENT, "lanes per row must be 0 (wave-stream) or a power of two in [1, 64], got %d",
+                    lanes);
     record_lanes(a, lanes);
     return PM_OK;
 }

@@ -15,25 +15,28 @@
 namespace pm
 {
 
-// Device-resident solver state.  One record per solve; written only by the
-// single last-arriving workgroup of each reduction, read by every kernel.
+// Device-resident solver state.  Written only by workgroup 0; every value a
+// kernel reads is written by an earlier kernel (parity slots t % 2 keep the
+// previous iteration's rho / completed count intact while the current one is
+// recorded), so no workgroup ever reads a value being written in the same
+// launch.
 struct SolverState
 {
-    double rho;        // r . z of the current direction
-    double alpha;
-    double beta;
-    double rr;         // latest ||r||^2
-    double rr0;        // initial ||r||^2
-    double tol;        // absolute tolerance on ||r||
+    double rho[2];      // r . z of the direction of iteration t, slot t % 2
+    double alpha;       // last step length
+    double rr;          // ||r||^2 at the last decision
+    double rr0;         // initial ||r||^2
+    double tol;         // absolute tolerance on ||r||
     double rel_tol;
     double abs_tol;
-    int32_t iteration; // completed iterations
+    int32_t iter[2];    // completed updates seen by iteration t, slot t % 2 (iter[1] = -1 at start)
+    int32_t iteration;  // completed updates (host visible)
     int32_t max_iterations;
-    int32_t done;      // see Done below
+    int32_t done;       // see Done below
     int32_t pad0;
     unsigned long long bad_row;  // first row with an unusable Jacobi diagonal (~0 if none)
     unsigned counters[8];
-    double pad1[4];
+    double pad1[2];
 };
 
 enum Done : int32_t
@@ -43,6 +46,8 @@ enum Done : int32_t
     kIterationLimit = 2,
     kBreakdown = 3,
     kNonFinite = 4,
+    kInitNonFinite = 5,
+    kBadDiagonal = 6,
 };
 
 // COO conversion status (deferred, lowest triplet position first), as the

@@ -172,7 +172,7 @@ def test_spmv_kat(dev, ws, oracle, dtype):
     assert kinds(e) == "invalid_argument"
 
 
-@pytest.mark.parametrize("lanes", [1, 2, 4, 8, 16, 32, 64])
+@pytest.mark.parametrize("lanes", [0, 1, 2, 4, 8, 16, 32, 64])
 @pytest.mark.parametrize("compact", [False, True])
 def test_spmv_random_all_lane_widths(dev, ws, oracle, lanes, compact):
     rng = np.random.default_rng(lanes)
@@ -198,6 +198,32 @@ def test_spmv_random_all_lane_widths(dev, ws, oracle, lanes, compact):
     scale = oracle.spmv(oracle.CSR(rows, cols, ref.row_offsets, ref.col_indices, np.abs(ref32.values.astype(np.float64))),
                         np.abs(x.astype(np.float32).astype(np.float64)))
     assert np.all(np.abs(host(y32) - exact32) <= 2e-6 * scale + 1e-30)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spmv_stream_long_rows_fallback(dev, ws, oracle, dtype):
+    """Rows long enough that a 64-row chunk exceeds the LDS staging capacity
+    take the 8-lanes-per-row path; mixed with short rows and empty rows."""
+    rng = np.random.default_rng(5)
+    rows, cols = 700, 4000
+    counts = rng.integers(0, 12, rows)
+    counts[100:180] = 300
+    counts[::11] = 0
+    r = np.repeat(np.arange(rows), counts)
+    c = rng.integers(0, cols, len(r))
+    v = rng.standard_normal(len(r)).astype(dtype)
+    ref = oracle.coo_to_csr(rows, cols, r, c, v)
+    x = rng.standard_normal(cols).astype(dtype)
+    exact = oracle.spmv(oracle.CSR(rows, cols, ref.row_offsets, ref.col_indices, ref.values.astype(np.float64)),
+                        x.astype(np.float64))
+    scale = oracle.spmv(oracle.CSR(rows, cols, ref.row_offsets, ref.col_indices,
+                                   np.abs(ref.values.astype(np.float64))), np.abs(x.astype(np.float64)))
+    for compact in (False, True):
+        a = capi.to_device_csr(ref, compact=compact)
+        y = torch.empty(rows, dtype=torch.float32 if dtype == np.float32 else torch.float64, device=dev)
+        capi.spmv(a, t(x, dev), y, ws, lanes=0)
+        eps = 4e-6 if dtype == np.float32 else 1e-14
+        assert np.all(np.abs(host(y) - exact) <= eps * scale + 1e-30)
 
 
 def test_spmv_diffusion3d_full_size_row_sums(dev, ws, oracle):
@@ -441,5 +467,5 @@ def test_autotuner_picks_a_valid_width(dev, ws):
     x = torch.ones(h.rows, dtype=torch.float32, device=dev)
     y = torch.empty_like(x)
     lanes, ms = capi.autotune_spmv(a, x, y, repeats=5)
-    assert lanes in (1, 2, 4, 8, 16, 32, 64) and ms > 0
+    assert lanes in (0, 1, 2, 4, 8, 16, 32, 64) and ms > 0
     assert capi.lib().pm_tuned_lanes(capi.ctypes.byref(a.descriptor())) == lanes
