@@ -54,6 +54,8 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--max-inner", type=int, default=100000)
+    ap.add_argument("--inner-rel", type=float, default=1e-3,
+                    help="relative tolerance of each FP32 inner solve of the refinement loop")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: one solve capped at 2000 inner iterations, no CPU leg")
     args = ap.parse_args()
@@ -111,7 +113,7 @@ def main() -> int:
     # ---- time-to-solve ----------------------------------------------------------
     pol = capi.policy(refresh_interval=50, use_graph=args.graph, chunk_iterations=args.chunk)
     x = torch.zeros(n, dtype=torch.float64, device=dev)
-    inner = dict(relative_tolerance=1e-6, max_iterations=args.max_inner)
+    inner = dict(relative_tolerance=args.inner_rel, max_iterations=args.max_inner)
 
     def solve():
         x.zero_()

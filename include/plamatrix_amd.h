@@ -210,6 +210,108 @@ int pm_pcg_refine(const pm_csr* a, const double* rhs, double* x, const pm_solver
                   double target_relres, int32_t max_outer, const pm_solve_policy* policy,
                   pm_refine_report* report, pm_workspace* ws, hipStream_t stream);
 
+/* ---- Damped Schur complement (FP64) --------------------------------------------
+ * Block normal equations in HBM, the device form of BlockNormalEquations<double>
+ * (block_schur.h:197-319): B (pc blocks ps x ps), C (ec blocks es x es),
+ * gradients, cross blocks W (ps x es, row-major, creation order) with their
+ * (primary, eliminated) ids, and primary cross blocks (row < col).  Block
+ * sizes up to 8. */
+typedef struct pm_normal_equations {
+    int64_t primary_count;
+    int64_t eliminated_count;
+    int64_t primary_size;
+    int64_t eliminated_size;
+    const double* primary_diagonal;
+    const double* eliminated_diagonal;
+    const double* primary_gradient;
+    const double* eliminated_gradient;
+    int64_t cross_count;
+    const int64_t* cross_primary;
+    const int64_t* cross_eliminated;
+    const double* cross_values;
+    int64_t primary_cross_count;
+    const int64_t* primary_cross_rows;
+    const int64_t* primary_cross_cols;
+    const double* primary_cross_values;
+} pm_normal_equations;
+
+/* Residuals that each link one primary and one eliminated block
+ * (addResidualBlock, block_schur.h:223-262), row-major Jacobians
+ * residual_size x block size.  weights == NULL takes the Huber IRLS weight
+ * evaluateHuberLoss(||r||^2, huber_delta).weight (robust_loss.h:27-51). */
+typedef struct pm_residual_batch {
+    int64_t count;
+    int64_t residual_size;
+    const int64_t* primary;
+    const int64_t* eliminated;
+    const double* primary_jacobians;
+    const double* eliminated_jacobians;
+    const double* residuals;
+    const double* weights;
+    double huber_delta;
+} pm_residual_batch;
+
+/* SchurComplementSolverOptions<double> (block_schur.h:31-48) for the device
+ * backend. */
+typedef struct pm_schur_options {
+    int32_t max_iterations;
+    int32_t use_initial_guess;
+    double relative_tolerance;
+    double absolute_tolerance;
+    double minimum_diagonal;
+} pm_schur_options;
+
+/* SchurComplementSolverReport (block_schur.h:51-80).  status: 0 converged,
+ * 1 eliminated block not positive definite, 2 preconditioner block not
+ * positive definite, 4 iteration limit. */
+typedef struct pm_schur_report {
+    int32_t converged;
+    int32_t iterations;
+    double initial_residual_norm;
+    double final_residual_norm;
+    int32_t status;
+    int32_t pattern_reused;
+    int32_t assembly_on_device;
+    int32_t pad;
+    int64_t pattern_build_count;
+    char message[96];
+} pm_schur_report;
+
+/* SchurComplementSolverWorkspace<double> (block_schur.h:83-169): the cached
+ * device pattern, normal-equation grouping and per-solve numerics.  Binds to
+ * its first stream like pm_workspace. */
+typedef struct pm_schur_workspace pm_schur_workspace;
+int pm_schur_workspace_create(pm_schur_workspace** ws);
+int pm_schur_workspace_destroy(pm_schur_workspace* ws);
+/* patternBuildCount() (block_schur.h:124-127). */
+int64_t pm_schur_pattern_build_count(const pm_schur_workspace* ws);
+int pm_schur_pattern_size(const pm_schur_workspace* ws, int64_t* slots, int64_t* values, int64_t* terms);
+
+/* Device assembly of the normal equations of a residual batch
+ * (BlockNormalEquations::addResidualBlocks, block_normal_equations.cpp:
+ * 135-312, for one primary and one eliminated block per residual).  Output
+ * arrays live in the workspace until the next call.  Synchronises `stream`
+ * once to report invalid ids or values like the reference's exceptions. */
+int pm_normal_equations_assemble(pm_schur_workspace* ws, int64_t primary_count, int64_t eliminated_count,
+                                 int64_t primary_size, int64_t eliminated_size, const pm_residual_batch* batch,
+                                 pm_normal_equations* out, hipStream_t stream);
+
+/* solveDampedSchurComplement(equations, damping, options, workspace,
+ * primary_step, eliminated_step) with the device backend
+ * (block_schur.cpp:53-459): damping, eliminated inverses, reduced rhs, Schur
+ * CSR on the cached pattern, block-Jacobi PCG with tolerance
+ * max(abs, rel * ||r0||) (block_schur_accelerated.cpp:53-65) and
+ * back-substitution.  primary_step / eliminated_step are device vectors. */
+int pm_schur_solve(pm_schur_workspace* ws, const pm_normal_equations* equations, double damping,
+                   const pm_schur_options* options, double* primary_step, double* eliminated_step,
+                   pm_schur_report* report, pm_workspace* solver_ws, hipStream_t stream);
+
+/* The reduced system of the last solve: scalar CSR (FP64, int64 and int32
+ * indices), right-hand side and inverted diagonal blocks (device pointers
+ * owned by the workspace). */
+int pm_schur_system(const pm_schur_workspace* ws, pm_csr* reduced, const double** rhs,
+                    const double** preconditioner);
+
 #ifdef __cplusplus
 }
 #endif

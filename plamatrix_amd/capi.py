@@ -54,6 +54,33 @@ class pm_refine_report(ctypes.Structure):
                 ("initial_relres", _f64), ("final_relres", _f64)]
 
 
+class pm_normal_equations(ctypes.Structure):
+    _fields_ = [("primary_count", _i64), ("eliminated_count", _i64), ("primary_size", _i64),
+                ("eliminated_size", _i64), ("primary_diagonal", _p), ("eliminated_diagonal", _p),
+                ("primary_gradient", _p), ("eliminated_gradient", _p), ("cross_count", _i64),
+                ("cross_primary", _p), ("cross_eliminated", _p), ("cross_values", _p),
+                ("primary_cross_count", _i64), ("primary_cross_rows", _p), ("primary_cross_cols", _p),
+                ("primary_cross_values", _p)]
+
+
+class pm_residual_batch(ctypes.Structure):
+    _fields_ = [("count", _i64), ("residual_size", _i64), ("primary", _p), ("eliminated", _p),
+                ("primary_jacobians", _p), ("eliminated_jacobians", _p), ("residuals", _p), ("weights", _p),
+                ("huber_delta", _f64)]
+
+
+class pm_schur_options(ctypes.Structure):
+    _fields_ = [("max_iterations", _i32), ("use_initial_guess", _i32), ("relative_tolerance", _f64),
+                ("absolute_tolerance", _f64), ("minimum_diagonal", _f64)]
+
+
+class pm_schur_report(ctypes.Structure):
+    _fields_ = [("converged", _i32), ("iterations", _i32), ("initial_residual_norm", _f64),
+                ("final_residual_norm", _f64), ("status", _i32), ("pattern_reused", _i32),
+                ("assembly_on_device", _i32), ("pad", _i32), ("pattern_build_count", _i64),
+                ("message", ctypes.c_char * 96)]
+
+
 # Every symbol declared in include/plamatrix_amd.h (checked by the CPU tests).
 EXPORTS = [
     "pm_last_error", "pm_version", "pm_workspace_create", "pm_workspace_destroy", "pm_workspace_close",
@@ -61,6 +88,8 @@ EXPORTS = [
     "pm_coo_to_csr_prepare", "pm_coo_to_csr_nnz", "pm_coo_to_csr_finish", "pm_coo_to_csr_async",
     "pm_workspace_check_status", "pm_spmv", "pm_spmv_lanes", "pm_autotune_spmv", "pm_tuned_lanes", "pm_tune_record", "pm_true_relres", "pm_cg", "pm_pcg",
     "pm_block_pcg", "pm_solve_fixed_async", "pm_fixed_finalize", "pm_fixed_state_destroy", "pm_pcg_refine",
+    "pm_schur_workspace_create", "pm_schur_workspace_destroy", "pm_schur_pattern_build_count",
+    "pm_schur_pattern_size", "pm_normal_equations_assemble", "pm_schur_solve", "pm_schur_system",
 ]
 
 _lib = None
@@ -81,6 +110,9 @@ def lib():
         _lib.pm_workspace_close.argtypes = [_p]
         _lib.pm_workspace_check_status.argtypes = [_p]
         _lib.pm_fixed_state_destroy.argtypes = [_p]
+        _lib.pm_schur_workspace_destroy.argtypes = [_p]
+        _lib.pm_schur_pattern_build_count.argtypes = [_p]
+        _lib.pm_schur_pattern_build_count.restype = _i64
     return _lib
 
 
@@ -336,3 +368,155 @@ def autotune_spmv(a: DeviceCSR, x, y, repeats=10):
     check(lib().pm_autotune_spmv(ctypes.byref(d), _ptr(x), _ptr(y), _i32(repeats), ctypes.byref(best),
                                  ctypes.byref(ms), current_stream()))
     return best.value, ms.value
+
+
+# ---- damped Schur complement ----------------------------------------------------
+
+
+class SchurWorkspace:
+    """Owns a pm_schur_workspace (cached device pattern and numerics)."""
+
+    def __init__(self):
+        h = _p()
+        check(lib().pm_schur_workspace_create(ctypes.byref(h)))
+        self.handle = h
+
+    @property
+    def pattern_build_count(self) -> int:
+        return int(lib().pm_schur_pattern_build_count(self.handle))
+
+    def pattern_size(self):
+        s, v, t = _i64(), _i64(), _i64()
+        check(lib().pm_schur_pattern_size(self.handle, ctypes.byref(s), ctypes.byref(v), ctypes.byref(t)))
+        return s.value, v.value, t.value
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.pm_schur_workspace_destroy(h)
+            self.handle = None
+
+
+@dataclass
+class NormalEquations:
+    """Device normal equations: a pm_normal_equations descriptor plus the
+    tensors that own (or alias) its arrays."""
+    desc: pm_normal_equations
+    keep: tuple = ()
+
+    def cross_count(self) -> int:
+        return int(self.desc.cross_count)
+
+
+def assemble_normal_equations(sw: SchurWorkspace, primary_count, eliminated_count, primary_size, eliminated_size,
+                              primary, eliminated, primary_jacobians, eliminated_jacobians, residuals,
+                              weights=None, huber_delta=1.0) -> NormalEquations:
+    """Device BlockNormalEquations::addResidualBlocks over a batch; arrays are
+    device tensors (int64 ids, float64 values)."""
+    rs = residuals.shape[1] if residuals.dim() == 2 else 1
+    b = pm_residual_batch(int(primary.numel()), rs, _ptr(primary), _ptr(eliminated), _ptr(primary_jacobians),
+                          _ptr(eliminated_jacobians), _ptr(residuals),
+                          _ptr(weights) if weights is not None else None, float(huber_delta))
+    out = pm_normal_equations()
+    check(lib().pm_normal_equations_assemble(sw.handle, _i64(primary_count), _i64(eliminated_count),
+                                             _i64(primary_size), _i64(eliminated_size), ctypes.byref(b),
+                                             ctypes.byref(out), current_stream()))
+    return NormalEquations(out, (primary, eliminated, primary_jacobians, eliminated_jacobians, residuals, weights))
+
+
+def upload_normal_equations(pc, ec, ps, es, exported: dict, device="cuda") -> NormalEquations:
+    """Normal equations from host arrays (the layout of BlockNormalEquations'
+    members, e.g. oracle.NormalEquations.export())."""
+    import torch
+
+    def f64(a):
+        return torch.as_tensor(a, dtype=torch.float64).to(device).contiguous()
+
+    def i64(a):
+        return torch.as_tensor(a, dtype=torch.int64).to(device).contiguous()
+
+    t = dict(B=f64(exported["primary_diagonal"]), C=f64(exported["eliminated_diagonal"]),
+             gp=f64(exported["primary_gradient"]), ge=f64(exported["eliminated_gradient"]),
+             cp=i64(exported["cross_primary"]), ce=i64(exported["cross_eliminated"]),
+             W=f64(exported["cross_values"]), pr=i64(exported["primary_cross_rows"]),
+             pcc=i64(exported["primary_cross_cols"]), pv=f64(exported["primary_cross_values"]))
+    d = pm_normal_equations(pc, ec, ps, es, _ptr(t["B"]), _ptr(t["C"]), _ptr(t["gp"]), _ptr(t["ge"]),
+                            len(exported["cross_primary"]), _ptr(t["cp"]), _ptr(t["ce"]), _ptr(t["W"]),
+                            len(exported["primary_cross_rows"]), _ptr(t["pr"]), _ptr(t["pcc"]), _ptr(t["pv"]))
+    return NormalEquations(d, tuple(t.values()))
+
+
+def normal_equations_arrays(ne: NormalEquations) -> dict:
+    """Host copies of the device normal equations (for checks)."""
+    d = ne.desc
+    pc, ec, ps, es = d.primary_count, d.eliminated_count, d.primary_size, d.eliminated_size
+    nc = d.cross_count
+    return dict(primary_diagonal=device_array(d.primary_diagonal, pc * ps * ps),
+                eliminated_diagonal=device_array(d.eliminated_diagonal, ec * es * es),
+                primary_gradient=device_array(d.primary_gradient, pc * ps),
+                eliminated_gradient=device_array(d.eliminated_gradient, ec * es),
+                cross_primary=device_array(d.cross_primary, nc, "int64"),
+                cross_eliminated=device_array(d.cross_eliminated, nc, "int64"),
+                cross_values=device_array(d.cross_values, nc * ps * es))
+
+
+def lib_memcpy(dst: int, src: int, nbytes: int) -> int:
+    """Device-to-device copy on the current stream, synchronised (the HIP
+    runtime symbols resolve through the library's own dependencies)."""
+    import torch
+
+    stream = _p(torch.cuda.current_stream().cuda_stream)
+    err = lib().hipMemcpyAsync(_p(dst), _p(src), ctypes.c_size_t(nbytes), ctypes.c_int(3), stream)
+    if err == 0:
+        err = lib().hipStreamSynchronize(stream)
+    if err != 0:
+        raise PlamatrixError(6, f"hipMemcpyAsync failed with HIP error {err}")
+    return 0
+
+
+@dataclass
+class SchurReport:
+    converged: bool
+    iterations: int
+    initial_residual_norm: float
+    final_residual_norm: float
+    status: int
+    pattern_reused: bool
+    assembly_on_device: bool
+    pattern_build_count: int
+    message: str
+
+
+def schur_solve(sw: SchurWorkspace, ne: NormalEquations, damping, primary_step, eliminated_step, ws: Workspace,
+                max_iterations=100, relative_tolerance=1e-8, absolute_tolerance=1e-12, minimum_diagonal=1e-12,
+                use_initial_guess=False) -> SchurReport:
+    o = pm_schur_options(int(max_iterations), int(bool(use_initial_guess)), float(relative_tolerance),
+                         float(absolute_tolerance), float(minimum_diagonal))
+    r = pm_schur_report()
+    check(lib().pm_schur_solve(sw.handle, ctypes.byref(ne.desc), _f64(damping), ctypes.byref(o),
+                               _ptr(primary_step), _ptr(eliminated_step), ctypes.byref(r), ws.handle,
+                               current_stream()))
+    return SchurReport(bool(r.converged), r.iterations, r.initial_residual_norm, r.final_residual_norm, r.status,
+                       bool(r.pattern_reused), bool(r.assembly_on_device), r.pattern_build_count,
+                       r.message.decode())
+
+
+def schur_system(sw: SchurWorkspace):
+    """(pm_csr descriptor, rhs pointer, preconditioner pointer) of the last solve."""
+    a = pm_csr()
+    rhs = _p()
+    pre = _p()
+    check(lib().pm_schur_system(sw.handle, ctypes.byref(a), ctypes.byref(rhs), ctypes.byref(pre)))
+    return a, rhs.value, pre.value
+
+
+def device_array(ptr: int, n: int, dtype="float64"):
+    """Host numpy copy of n elements at a device pointer."""
+    import numpy as np
+    import torch
+
+    tdt = {"float64": torch.float64, "int64": torch.int64, "int32": torch.int32}[dtype]
+    t = torch.empty(max(n, 1), dtype=tdt, device="cuda")
+    if n > 0:
+        check(lib_memcpy(t.data_ptr(), ptr, n * t.element_size()))
+    return t[:n].cpu().numpy().astype(np.dtype(dtype))

@@ -22,6 +22,7 @@
 #include <cmath>
 
 #include "pm_common.hpp"
+#include "sort.hpp"
 #include "workspace.hpp"
 
 namespace pm
@@ -332,6 +333,35 @@ __global__ __launch_bounds__(kBlock) void k_row_offsets(int64_t rows, const unsi
     }
 }
 
+int radix_sort_pairs(unsigned long long* keys_a, unsigned* vals_a, unsigned long long* keys_b, unsigned* vals_b,
+                     int64_t count, int key_bits, unsigned* hist, unsigned* scratch, hipStream_t stream, bool* in_b)
+{
+    *in_b = false;
+    if (count <= 0) return PM_OK;
+    const int passes = (key_bits + 7) / 8;
+    const int64_t tiles = ceil_div(count, kSortTile);
+    unsigned long long* kin = keys_a;
+    unsigned long long* kout = keys_b;
+    unsigned* vin = vals_a;
+    unsigned* vout = vals_b;
+    for (int pass = 0; pass < passes; ++pass)
+    {
+        const int shift = pass * 8;
+        hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, stream, kin, count, shift, hist, tiles);
+        PM_HIP(hipGetLastError());
+        PM_TRY(scan_u32(hist, hist, 256 * tiles, scratch, nullptr, stream));
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, stream, kin, vin, count, shift, hist,
+                           tiles, kout, vout);
+        PM_HIP(hipGetLastError());
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    *in_b = (passes % 2) == 1;
+    return PM_OK;
+}
+
+int64_t radix_hist_words(int64_t count) { return 256 * ceil_div(count > 0 ? count : 1, kSortTile); }
+
 int bits_for(int64_t n)
 {
     int b = 0;
@@ -428,25 +458,10 @@ int prepare_coo(int64_t rows, int64_t cols, int64_t count, const int64_t* r, con
                                static_cast<const double*>(v), flags & PM_COO_CHECK_FINITE, b.keys_a, b.vals_a,
                                b.status);
         PM_HIP(hipGetLastError());
-        const int key_bits = rb + cb;
-        const int passes = (key_bits + 7) / 8;
-        const int64_t tiles = ceil_div(count, kSortTile);
-        unsigned long long* kin = b.keys_a;
-        unsigned long long* kout = b.keys_b;
-        unsigned* vin = b.vals_a;
-        unsigned* vout = b.vals_b;
-        for (int pass = 0; pass < passes; ++pass)
-        {
-            const int shift = pass * 8;
-            hipLaunchKernelGGL(k_radix_hist, dim3(tiles), dim3(kBlock), 0, stream, kin, count, shift, b.hist, tiles);
-            PM_HIP(hipGetLastError());
-            PM_TRY(scan_u32(b.hist, b.hist, 256 * tiles, b.scratch, nullptr, stream));
-            hipLaunchKernelGGL(k_radix_scatter, dim3(tiles), dim3(kBlock), 0, stream, kin, vin, count, shift, b.hist,
-                               tiles, kout, vout);
-            PM_HIP(hipGetLastError());
-            std::swap(kin, kout);
-            std::swap(vin, vout);
-        }
+        bool in_b = false;
+        PM_TRY(radix_sort_pairs(b.keys_a, b.vals_a, b.keys_b, b.vals_b, count, rb + cb, b.hist, b.scratch, stream,
+                                &in_b));
+        unsigned long long* kin = in_b ? b.keys_b : b.keys_a;
         // kin/vin now hold the sorted keys and permutation
         hipLaunchKernelGGL(k_mark_heads, dim3(grid), dim3(kBlock), 0, stream, kin, count, b.heads);
         PM_HIP(hipGetLastError());

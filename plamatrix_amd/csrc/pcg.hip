@@ -105,7 +105,7 @@ struct DirectionSource
     {
         return fma_s(beta, p_old[c], z[c]);
     }
-    __device__ __forceinline__ S mul(S a, S b) const { return a * b; }
+    __device__ __forceinline__ S madd(S a, S g, S acc) const { return fma_s(a, g, acc); }
 };
 
 // First kernel of iteration t (slot par = t % 2), kDirBlock threads.
@@ -156,8 +156,8 @@ __global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, cons
     }
     if (code != kRunning) return;
 
-    __shared__ S lds_all[kDirWaves][kStreamCap];
-    S* lds = lds_all[threadIdx.x / kWave];
+    __shared__ S lv_all[kDirWaves][kStreamCap];
+    __shared__ I lc_all[kDirWaves][kStreamCap];
     const S b = static_cast<S>(beta);
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -168,7 +168,8 @@ __global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, cons
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lds, DirectionSource<S>{v, z, p_old, b}, s, e);
+        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                         DirectionSource<S>{v, z, p_old, b}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
         {
@@ -806,8 +807,8 @@ __global__ __launch_bounds__(kBlock) void k_residual64(int64_t rows, const I* __
                                                        double* __restrict__ r, double* partials, unsigned* counter,
                                                        double* out)
 {
-    __shared__ double lds_all[kWavesPerBlock][kStreamCap];
-    double* lds = lds_all[threadIdx.x / kWave];
+    __shared__ float lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     int64_t begin, end;
@@ -817,7 +818,8 @@ __global__ __launch_bounds__(kBlock) void k_residual64(int64_t rows, const I* __
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const double acc = chunk_row_sums<double>(row0, rows, ro, ci, lds, Fp64Source<float, double>{v, x}, s, e);
+        const double acc = chunk_row_sums<double>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                                  Fp64Source<float, double>{v, x}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
         {

@@ -1,0 +1,1661 @@
+// Damped Schur-complement solve of block LM normal equations on gfx950.
+//
+// Replaces solveDampedSchurComplement with the accelerated backend
+// (block_schur.cpp:53-459, block_schur_accelerated.cpp:80-299,
+// block_schur_device_assembly.cu:19-256) with a pipeline that keeps every
+// array in HBM:
+//
+//   normal equations  From a residual batch (one primary and one eliminated
+//                     block per residual) with Huber IRLS weights.  Residuals
+//                     are grouped by primary, by eliminated and by (primary,
+//                     eliminated) pair with the stable radix sort; each output
+//                     entry is one thread that folds its group in input order,
+//                     so B, C, W, g come out in the operation order of
+//                     BlockNormalEquations::addResidualBlocks
+//                     (block_normal_equations.cpp:135-312) with no atomics.
+//                     Cross blocks are numbered in first-appearance order, as
+//                     findOrCreateCrossBlock numbers them.
+//   pattern           Built on the device from the cross-block topology and
+//                     cached in the workspace until the topology changes
+//                     (prepareSchurTopology, block_schur_sparse_assembly.h:
+//                     23-272).  Every (left, right) primary pair of every
+//                     eliminated block, the diagonal and the primary cross
+//                     blocks become 2-D block keys; one radix sort orders them
+//                     like the reference's std::map; runs of equal keys are the
+//                     block slots, and each slot's terms stay in (eliminated,
+//                     left, right) order.
+//   prologue          Damping, Cholesky inverses of the eliminated blocks, the
+//                     reduced right-hand side and the transformed cross
+//                     blocks W C^-1, one thread per block or per row.
+//   assembly          One wavefront per block slot, one lane per entry of the
+//                     ps x ps block: the slot's terms are folded in order, an
+//                     atomic-free segmented reduction over the fixed pattern.
+//   solve             Block-Jacobi PCG (pm_block_pcg) on the scalar CSR of the
+//                     reduced system with the inverted diagonal blocks.
+//   back-substitution One thread per eliminated block.
+//
+// Kernels that restate a reference loop keep its operation order and are
+// compiled without floating-point contraction, so their results equal the
+// CPU path's bit for bit given the same inputs.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pm_common.hpp"
+#include "sort.hpp"
+#include "workspace.hpp"
+
+namespace pm
+{
+
+constexpr int kMaxSchurBlock = 8;  // ps and es limit (ps*ps lanes per slot)
+
+struct SchurStatus
+{
+    unsigned long long bad_primary;     // lowest residual with an out-of-range primary block
+    unsigned long long bad_eliminated;  // lowest residual with an out-of-range eliminated block
+    unsigned long long bad_value;       // lowest residual with a non-finite Jacobian / residual entry
+    unsigned long long bad_weight;      // lowest residual with a non-finite or negative weight
+    unsigned long long bad_inverse;     // lowest eliminated block that is not positive definite
+    unsigned long long bad_preconditioner;  // lowest primary block whose Schur diagonal is not PD
+    unsigned long long mismatch;        // topology differences from the cached copy
+    unsigned long long count0;          // small device-side counts read back by the host
+    unsigned long long count1;
+    unsigned long long pad[7];
+};
+
+__global__ void k_schur_status_init(SchurStatus* st)
+{
+    st->bad_primary = ~0ull;
+    st->bad_eliminated = ~0ull;
+    st->bad_value = ~0ull;
+    st->bad_weight = ~0ull;
+    st->bad_inverse = ~0ull;
+    st->bad_preconditioner = ~0ull;
+    st->mismatch = 0;
+    st->count0 = 0;
+    st->count1 = 0;
+}
+
+__device__ __forceinline__ int64_t gtid() { return static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ int64_t gstride() { return static_cast<int64_t>(gridDim.x) * blockDim.x; }
+
+inline int grid_for(int64_t n)
+{
+    return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBlock), 16384)));
+}
+
+// ---- small dense algebra in the reference's operation order -------------------
+
+// evaluateHuberLoss(s, delta).weight (robust_loss.h:27-51).
+__device__ double huber_weight(double s, double delta)
+{
+    if (delta == 0.0 || s <= delta * delta) return 1.0;
+    return delta / sqrt(s);
+}
+
+// invertPositiveDefiniteInto (block_schur_linear_algebra.h:92-147): Cholesky
+// factor, then one forward and one backward substitution per column.
+__device__ bool invert_pd(const double* m, int n, double* inv)
+{
+#pragma clang fp contract(off)
+    double lower[kMaxSchurBlock * kMaxSchurBlock];
+    double mid[kMaxSchurBlock];
+    for (int i = 0; i < n * n; ++i) lower[i] = 0.0;
+    for (int row = 0; row < n; ++row)
+        for (int col = 0; col <= row; ++col)
+        {
+            double value = m[row * n + col];
+            for (int k = 0; k < col; ++k) value -= lower[row * n + k] * lower[col * n + k];
+            if (row == col)
+            {
+                if (!(value > 0.0) || !isfinite(value)) return false;
+                lower[row * n + col] = sqrt(value);
+            }
+            else
+            {
+                lower[row * n + col] = value / lower[col * n + col];
+            }
+        }
+    for (int i = 0; i < n * n; ++i) inv[i] = 0.0;
+    for (int col = 0; col < n; ++col)
+    {
+        for (int row = 0; row < n; ++row)
+        {
+            double value = row == col ? 1.0 : 0.0;
+            for (int k = 0; k < row; ++k) value -= lower[row * n + k] * mid[k];
+            mid[row] = value / lower[row * n + row];
+        }
+        for (int off = 0; off < n; ++off)
+        {
+            const int row = n - 1 - off;
+            double value = mid[row];
+            for (int k = row + 1; k < n; ++k) value -= lower[k * n + row] * inv[k * n + col];
+            inv[row * n + col] = value / lower[row * n + row];
+        }
+    }
+    return true;
+}
+
+// ---- generic helpers ------------------------------------------------------------
+
+// out[x] = number of sorted keys whose (key >> shift) is below x, x in [0, m].
+__global__ __launch_bounds__(kBlock) void k_group_offsets(const unsigned long long* __restrict__ keys, int64_t n,
+                                                          int shift, int64_t m, int64_t* __restrict__ out)
+{
+    for (int64_t x = gtid(); x <= m; x += gstride())
+    {
+        int64_t lo = 0, hi = n;
+        while (lo < hi)
+        {
+            const int64_t mid = (lo + hi) / 2;
+            if (static_cast<int64_t>(keys[mid] >> shift) < x) lo = mid + 1;
+            else hi = mid;
+        }
+        out[x] = lo;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_mismatch(const int64_t* __restrict__ a,
+                                                           const int64_t* __restrict__ b, int64_t n,
+                                                           SchurStatus* st)
+{
+    bool diff = false;
+    for (int64_t i = gtid(); i < n; i += gstride()) diff |= a[i] != b[i];
+    if (diff) atomicAdd(&st->mismatch, 1ull);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_i64(const int64_t* __restrict__ src, int64_t n,
+                                                     int64_t* __restrict__ dst)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_f64(double* __restrict__ dst, int64_t n, double v)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) dst[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_u32_to_i64(const unsigned* __restrict__ src, int64_t n,
+                                                       int64_t* __restrict__ dst)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) dst[i] = static_cast<int64_t>(src[i]);
+}
+
+// ---- normal equations from residuals -------------------------------------------
+
+// Validation, weights and the three grouping keys of every residual.
+__global__ __launch_bounds__(kBlock) void k_residual_keys(
+    int64_t count, int64_t pc, int64_t ec, int rs, int ps, int es, int eb, const int64_t* __restrict__ primary,
+    const int64_t* __restrict__ eliminated, const double* __restrict__ jp, const double* __restrict__ je,
+    const double* __restrict__ r, const double* __restrict__ weights, double delta, double* __restrict__ w_out,
+    unsigned long long* __restrict__ key_p, unsigned long long* __restrict__ key_e,
+    unsigned long long* __restrict__ key_pe, unsigned* __restrict__ val_p, unsigned* __restrict__ val_e,
+    unsigned* __restrict__ val_pe, SchurStatus* st)
+{
+#pragma clang fp contract(off)
+    for (int64_t i = gtid(); i < count; i += gstride())
+    {
+        const int64_t p = primary[i], e = eliminated[i];
+        const bool p_ok = p >= 0 && p < pc;
+        const bool e_ok = e >= 0 && e < ec;
+        if (!p_ok) atomicMin(&st->bad_primary, static_cast<unsigned long long>(i));
+        if (!e_ok) atomicMin(&st->bad_eliminated, static_cast<unsigned long long>(i));
+        bool finite = true;
+        double s = 0.0;
+        for (int k = 0; k < rs; ++k)
+        {
+            const double v = r[i * rs + k];
+            finite &= isfinite(v);
+            s += v * v;
+        }
+        for (int k = 0; k < rs * ps; ++k) finite &= isfinite(jp[i * rs * ps + k]);
+        for (int k = 0; k < rs * es; ++k) finite &= isfinite(je[i * rs * es + k]);
+        if (!finite) atomicMin(&st->bad_value, static_cast<unsigned long long>(i));
+        double w;
+        if (weights)
+        {
+            w = weights[i];
+            if (!isfinite(w) || w < 0.0) atomicMin(&st->bad_weight, static_cast<unsigned long long>(i));
+        }
+        else
+        {
+            w = huber_weight(s, delta);
+        }
+        w_out[i] = w;
+        const unsigned long long kp = p_ok ? static_cast<unsigned long long>(p) : 0ull;
+        const unsigned long long ke = e_ok ? static_cast<unsigned long long>(e) : 0ull;
+        key_p[i] = kp;
+        key_e[i] = ke;
+        key_pe[i] = (kp << eb) | ke;
+        val_p[i] = val_e[i] = val_pe[i] = static_cast<unsigned>(i);
+    }
+}
+
+// Heads of the (primary, eliminated) runs: first[i] = 1 for the first
+// residual (in input order) of each pair.
+__global__ __launch_bounds__(kBlock) void k_pair_heads(const unsigned long long* __restrict__ keys,
+                                                       const unsigned* __restrict__ perm, int64_t n,
+                                                       unsigned* __restrict__ head, unsigned* __restrict__ first)
+{
+    for (int64_t i = gtid(); i < n; i += gstride())
+    {
+        const bool h = i == 0 || keys[i] != keys[i - 1];
+        head[i] = h ? 1u : 0u;
+        if (h) first[perm[i]] = 1u;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_zero_u32(unsigned* __restrict__ p, int64_t n)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) p[i] = 0u;
+}
+
+// Per run (in sorted order): its start, and the cross index (creation order)
+// with the block ids of the cross block.
+__global__ __launch_bounds__(kBlock) void k_pair_runs(const unsigned long long* __restrict__ keys,
+                                                      const unsigned* __restrict__ perm,
+                                                      const unsigned* __restrict__ head,
+                                                      const unsigned* __restrict__ run_id,
+                                                      const unsigned* __restrict__ creation, int64_t n, int eb,
+                                                      int64_t* __restrict__ run_start,
+                                                      unsigned* __restrict__ run_cross,
+                                                      int64_t* __restrict__ cross_primary,
+                                                      int64_t* __restrict__ cross_eliminated)
+{
+    const unsigned long long emask = (1ull << eb) - 1ull;
+    for (int64_t i = gtid(); i < n; i += gstride())
+    {
+        if (!head[i]) continue;
+        const int64_t run = run_id[i];
+        const unsigned c = creation[perm[i]];
+        run_start[run] = i;
+        run_cross[run] = c;
+        cross_primary[c] = static_cast<int64_t>(keys[i] >> eb);
+        cross_eliminated[c] = static_cast<int64_t>(keys[i] & emask);
+    }
+}
+
+// B_p and g_p: thread per (primary, entry); entries [0, ps*ps) are B, then g.
+__global__ __launch_bounds__(kBlock) void k_ne_primary(int64_t pc, int rs, int ps, const int64_t* __restrict__ offs,
+                                                       const unsigned* __restrict__ perm,
+                                                       const double* __restrict__ jp, const double* __restrict__ r,
+                                                       const double* __restrict__ w, double* __restrict__ B,
+                                                       double* __restrict__ g)
+{
+#pragma clang fp contract(off)
+    const int entries = ps * ps + ps;
+    for (int64_t t = gtid(); t < pc * entries; t += gstride())
+    {
+        const int64_t p = t / entries;
+        const int en = static_cast<int>(t % entries);
+        const bool is_b = en < ps * ps;
+        const int column = is_b ? en / ps : en - ps * ps;
+        const int other = is_b ? en % ps : 0;
+        double acc = 0.0;
+        for (int64_t k = offs[p]; k < offs[p + 1]; ++k)
+        {
+            const int64_t i = perm[k];
+            const double wi = w[i];
+            const double* J = jp + i * rs * ps;
+            for (int row = 0; row < rs; ++row)
+            {
+                const double value = J[row * ps + column];
+                acc += wi * value * (is_b ? J[row * ps + other] : r[i * rs + row]);
+            }
+        }
+        if (is_b) B[p * ps * ps + en] = acc;
+        else g[p * ps + column] = acc;
+    }
+}
+
+// C_e and g_e: thread per (eliminated, entry).
+__global__ __launch_bounds__(kBlock) void k_ne_eliminated(int64_t ec, int rs, int es,
+                                                          const int64_t* __restrict__ offs,
+                                                          const unsigned* __restrict__ perm,
+                                                          const double* __restrict__ je,
+                                                          const double* __restrict__ r,
+                                                          const double* __restrict__ w, double* __restrict__ C,
+                                                          double* __restrict__ g)
+{
+#pragma clang fp contract(off)
+    const int entries = es * es + es;
+    for (int64_t t = gtid(); t < ec * entries; t += gstride())
+    {
+        const int64_t e = t / entries;
+        const int en = static_cast<int>(t % entries);
+        const bool is_c = en < es * es;
+        const int column = is_c ? en / es : en - es * es;
+        const int other = is_c ? en % es : 0;
+        double acc = 0.0;
+        for (int64_t k = offs[e]; k < offs[e + 1]; ++k)
+        {
+            const int64_t i = perm[k];
+            const double wi = w[i];
+            const double* J = je + i * rs * es;
+            for (int row = 0; row < rs; ++row)
+            {
+                const double value = J[row * es + column];
+                acc += wi * value * (is_c ? J[row * es + other] : r[i * rs + row]);
+            }
+        }
+        if (is_c) C[e * es * es + en] = acc;
+        else g[e * es + column] = acc;
+    }
+}
+
+// W blocks (ps x es, row-major), thread per (run, entry), stored at the run's
+// cross index.
+__global__ __launch_bounds__(kBlock) void k_ne_cross(int64_t runs, int64_t n, int rs, int ps, int es,
+                                                     const int64_t* __restrict__ run_start,
+                                                     const unsigned* __restrict__ run_cross,
+                                                     const unsigned* __restrict__ perm,
+                                                     const double* __restrict__ jp, const double* __restrict__ je,
+                                                     const double* __restrict__ w, double* __restrict__ W)
+{
+#pragma clang fp contract(off)
+    const int entries = ps * es;
+    for (int64_t t = gtid(); t < runs * entries; t += gstride())
+    {
+        const int64_t run = t / entries;
+        const int en = static_cast<int>(t % entries);
+        const int pcol = en / es, ecol = en % es;
+        const int64_t end = run + 1 < runs ? run_start[run + 1] : n;
+        double acc = 0.0;
+        for (int64_t k = run_start[run]; k < end; ++k)
+        {
+            const int64_t i = perm[k];
+            const double wi = w[i];
+            for (int row = 0; row < rs; ++row)
+                acc += wi * jp[i * rs * ps + row * ps + pcol] * je[i * rs * es + row * es + ecol];
+        }
+        W[static_cast<int64_t>(run_cross[run]) * entries + en] = acc;
+    }
+}
+
+// ---- Schur pattern ---------------------------------------------------------------
+
+// Range checks of caller-supplied topology: cross ids in range, primary
+// cross blocks with row < col < pc.
+__global__ __launch_bounds__(kBlock) void k_validate_topology(int64_t nc, int64_t npc, int64_t pc, int64_t ec,
+                                                              const int64_t* __restrict__ cp,
+                                                              const int64_t* __restrict__ ce,
+                                                              const int64_t* __restrict__ pr,
+                                                              const int64_t* __restrict__ pcl, SchurStatus* st)
+{
+    for (int64_t c = gtid(); c < nc; c += gstride())
+    {
+        if (cp[c] < 0 || cp[c] >= pc) atomicMin(&st->bad_primary, static_cast<unsigned long long>(c));
+        if (ce[c] < 0 || ce[c] >= ec) atomicMin(&st->bad_eliminated, static_cast<unsigned long long>(c));
+    }
+    for (int64_t c = gtid(); c < npc; c += gstride())
+        if (pr[c] < 0 || pcl[c] <= pr[c] || pcl[c] >= pc) atomicMin(&st->bad_value, static_cast<unsigned long long>(c));
+}
+
+// Sort keys of the cross blocks: eliminated id (shift 0) or (primary, eliminated).
+__global__ __launch_bounds__(kBlock) void k_cross_keys(int64_t nc, const int64_t* __restrict__ cp,
+                                                       const int64_t* __restrict__ ce, int shift,
+                                                       unsigned long long* __restrict__ keys,
+                                                       unsigned* __restrict__ vals)
+{
+    for (int64_t c = gtid(); c < nc; c += gstride())
+    {
+        const unsigned long long e = static_cast<unsigned long long>(ce[c]);
+        keys[c] = shift ? ((static_cast<unsigned long long>(cp[c]) << shift) | e) : e;
+        vals[c] = static_cast<unsigned>(c);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_i64(int64_t* __restrict__ dst, int64_t n, int64_t v)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) dst[i] = v;
+}
+
+// count0 = total pair terms sum_e k_e^2 (64-bit; integer atomics are exact).
+__global__ __launch_bounds__(kBlock) void k_sum_pairs(int64_t ec, const int64_t* __restrict__ adj_offs,
+                                                      SchurStatus* st)
+{
+    unsigned long long total = 0;
+    for (int64_t e = gtid(); e < ec; e += gstride())
+    {
+        const unsigned long long k = static_cast<unsigned long long>(adj_offs[e + 1] - adj_offs[e]);
+        total += k * k;
+    }
+    if (total) atomicAdd(&st->count0, total);
+}
+
+// Pair-term count per eliminated block (k_e^2).
+__global__ __launch_bounds__(kBlock) void k_pair_counts(int64_t ec, const int64_t* __restrict__ adj_offs,
+                                                        unsigned* __restrict__ counts)
+{
+    for (int64_t e = gtid(); e < ec; e += gstride())
+    {
+        const int64_t k = adj_offs[e + 1] - adj_offs[e];
+        counts[e] = static_cast<unsigned>(k * k);
+    }
+}
+
+// Keys of every pair term (eliminated-major, then left, then right), then the
+// diagonal and both orientations of each primary cross block.
+__global__ __launch_bounds__(kBlock) void k_pattern_keys(int64_t ec, int64_t pc, int64_t npc, int pb,
+                                                         const int64_t* __restrict__ adj_offs,
+                                                         const unsigned* __restrict__ adj,
+                                                         const unsigned* __restrict__ pair_offs,
+                                                         const int64_t* __restrict__ cross_primary,
+                                                         const int64_t* __restrict__ pc_rows,
+                                                         const int64_t* __restrict__ pc_cols, int64_t terms,
+                                                         unsigned long long* __restrict__ keys,
+                                                         unsigned* __restrict__ vals)
+{
+    for (int64_t e = gtid(); e < ec; e += gstride())
+    {
+        const int64_t a0 = adj_offs[e], k = adj_offs[e + 1] - a0;
+        int64_t t = pair_offs[e];
+        for (int64_t l = 0; l < k; ++l)
+        {
+            const unsigned long long pl = static_cast<unsigned long long>(cross_primary[adj[a0 + l]]);
+            for (int64_t rr = 0; rr < k; ++rr)
+            {
+                const unsigned long long pr = static_cast<unsigned long long>(cross_primary[adj[a0 + rr]]);
+                keys[t] = (pl << pb) | pr;
+                vals[t] = static_cast<unsigned>(t);
+                ++t;
+            }
+        }
+    }
+    for (int64_t b = gtid(); b < pc; b += gstride())
+    {
+        keys[terms + b] = (static_cast<unsigned long long>(b) << pb) | static_cast<unsigned long long>(b);
+        vals[terms + b] = static_cast<unsigned>(terms + b);
+    }
+    for (int64_t c = gtid(); c < npc; c += gstride())
+    {
+        const unsigned long long row = static_cast<unsigned long long>(pc_rows[c]);
+        const unsigned long long col = static_cast<unsigned long long>(pc_cols[c]);
+        const int64_t base = terms + pc + 2 * c;
+        keys[base] = (row << pb) | col;
+        vals[base] = static_cast<unsigned>(base);
+        keys[base + 1] = (col << pb) | row;
+        vals[base + 1] = static_cast<unsigned>(base + 1);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_key_heads(const unsigned long long* __restrict__ keys, int64_t n,
+                                                      unsigned* __restrict__ heads)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) heads[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// Slot records from the run heads: key, run start.
+__global__ __launch_bounds__(kBlock) void k_slots(const unsigned long long* __restrict__ keys,
+                                                  const unsigned* __restrict__ heads,
+                                                  const unsigned* __restrict__ slot_id, int64_t n,
+                                                  unsigned long long* __restrict__ slot_key,
+                                                  int64_t* __restrict__ slot_start)
+{
+    for (int64_t i = gtid(); i < n; i += gstride())
+    {
+        if (!heads[i]) continue;
+        const int64_t s = slot_id[i];
+        slot_key[s] = keys[i];
+        slot_start[s] = i;
+    }
+}
+
+// Decode each sorted entry: term -> (eliminated, left cross, right cross);
+// diagonal / primary-cross markers -> kind in term_e (~0u diagonal, ~1u
+// primary cross, with the primary cross index and orientation in term_left).
+__global__ __launch_bounds__(kBlock) void k_decode_terms(const unsigned* __restrict__ vals, int64_t n,
+                                                         int64_t terms, int64_t pc, int64_t ec,
+                                                         const unsigned* __restrict__ pair_offs,
+                                                         const int64_t* __restrict__ adj_offs,
+                                                         const unsigned* __restrict__ adj,
+                                                         unsigned* __restrict__ term_e,
+                                                         unsigned* __restrict__ term_left,
+                                                         unsigned* __restrict__ term_right)
+{
+    for (int64_t i = gtid(); i < n; i += gstride())
+    {
+        const int64_t t = vals[i];
+        if (t < terms)
+        {
+            int64_t lo = 0, hi = ec;  // last e with pair_offs[e] <= t
+            while (lo < hi)
+            {
+                const int64_t mid = (lo + hi + 1) / 2;
+                if (static_cast<int64_t>(pair_offs[mid]) <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            const int64_t e = lo;
+            const int64_t k = adj_offs[e + 1] - adj_offs[e];
+            const int64_t local = t - static_cast<int64_t>(pair_offs[e]);
+            term_e[i] = static_cast<unsigned>(e);
+            term_left[i] = adj[adj_offs[e] + local / k];
+            term_right[i] = adj[adj_offs[e] + local % k];
+        }
+        else if (t < terms + pc)
+        {
+            term_e[i] = ~0u;
+            term_left[i] = 0;
+            term_right[i] = 0;
+        }
+        else
+        {
+            const int64_t q = t - terms - pc;
+            term_e[i] = ~1u;
+            term_left[i] = static_cast<unsigned>(q / 2);
+            term_right[i] = static_cast<unsigned>(q % 2);
+        }
+    }
+}
+
+// Scalar CSR structure: block row br has slots [bro[br], bro[br+1]); scalar
+// row br*ps + lr starts at bro[br]*ps*ps + lr*(slots in row)*ps; entry
+// (slot, lr, lc) sits at ordinal*ps + lc within that row
+// (block_schur_sparse_assembly.h:129-154).
+__global__ __launch_bounds__(kBlock) void k_scalar_rows(int64_t pc, int ps, const int64_t* __restrict__ bro,
+                                                        int64_t* __restrict__ ro, int32_t* __restrict__ ro32)
+{
+    for (int64_t r = gtid(); r <= pc * ps; r += gstride())
+    {
+        int64_t v;
+        if (r == pc * ps)
+        {
+            v = bro[pc] * ps * ps;
+        }
+        else
+        {
+            const int64_t br = r / ps, lr = r % ps;
+            v = bro[br] * ps * ps + lr * (bro[br + 1] - bro[br]) * ps;
+        }
+        ro[r] = v;
+        ro32[r] = static_cast<int32_t>(v);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scalar_cols(int64_t slots, int ps, int pb,
+                                                        const unsigned long long* __restrict__ slot_key,
+                                                        const int64_t* __restrict__ bro,
+                                                        const int64_t* __restrict__ ro, int64_t* __restrict__ ci,
+                                                        int32_t* __restrict__ ci32)
+{
+    const unsigned long long mask = (1ull << pb) - 1ull;
+    for (int64_t t = gtid(); t < slots * ps * ps; t += gstride())
+    {
+        const int64_t s = t / (ps * ps);
+        const int lr = static_cast<int>((t / ps) % ps), lc = static_cast<int>(t % ps);
+        const int64_t row = static_cast<int64_t>(slot_key[s] >> pb);
+        const int64_t col = static_cast<int64_t>(slot_key[s] & mask);
+        const int64_t pos = ro[row * ps + lr] + (s - bro[row]) * ps + lc;
+        ci[pos] = col * ps + lc;
+        ci32[pos] = static_cast<int32_t>(col * ps + lc);
+    }
+}
+
+// ---- numerics ----------------------------------------------------------------------
+
+// D_ii += lambda * max(|D_ii|, minimum) (block_schur.cpp:100-128) on a copy.
+__global__ __launch_bounds__(kBlock) void k_damp(const double* __restrict__ src, int64_t blocks, int size,
+                                                 double damping, double minimum, double* __restrict__ dst)
+{
+#pragma clang fp contract(off)
+    const int64_t n = blocks * size * size;
+    for (int64_t i = gtid(); i < n; i += gstride())
+    {
+        double v = src[i];
+        const int64_t local = i % (size * size);
+        if (local / size == local % size)
+        {
+            const double magnitude = fabs(v);
+            v += damping * (magnitude < minimum ? minimum : magnitude);
+        }
+        dst[i] = v;
+    }
+}
+
+// C^-1 per eliminated block and y_e = C^-1 g_e.
+__global__ __launch_bounds__(kBlock) void k_eliminated_inverse(int64_t ec, int es, const double* __restrict__ Cd,
+                                                               const double* __restrict__ ge,
+                                                               double* __restrict__ Cinv, double* __restrict__ y,
+                                                               SchurStatus* st)
+{
+#pragma clang fp contract(off)
+    for (int64_t e = gtid(); e < ec; e += gstride())
+    {
+        double m[kMaxSchurBlock * kMaxSchurBlock], inv[kMaxSchurBlock * kMaxSchurBlock];
+        for (int i = 0; i < es * es; ++i) m[i] = Cd[e * es * es + i];
+        if (!invert_pd(m, es, inv))
+        {
+            atomicMin(&st->bad_inverse, static_cast<unsigned long long>(e));
+            for (int i = 0; i < es * es; ++i) inv[i] = 0.0;
+        }
+        for (int i = 0; i < es * es; ++i) Cinv[e * es * es + i] = inv[i];
+        for (int row = 0; row < es; ++row)
+        {
+            double value = 0.0;
+            for (int col = 0; col < es; ++col) value += inv[row * es + col] * ge[e * es + col];
+            y[e * es + row] = value;
+        }
+    }
+}
+
+// rhs_p = -g_p + sum over the primary's cross blocks in (primary, eliminated)
+// order of W C^-1 g_e (block_schur.cpp:158-181).
+__global__ __launch_bounds__(kBlock) void k_reduced_rhs(int64_t pc, int ps, int es, const double* __restrict__ gp,
+                                                        const int64_t* __restrict__ prim_offs,
+                                                        const unsigned* __restrict__ prim_cross,
+                                                        const int64_t* __restrict__ cross_eliminated,
+                                                        const double* __restrict__ W, const double* __restrict__ y,
+                                                        double* __restrict__ rhs)
+{
+#pragma clang fp contract(off)
+    for (int64_t t = gtid(); t < pc * ps; t += gstride())
+    {
+        const int64_t p = t / ps;
+        const int row = static_cast<int>(t % ps);
+        double acc = -gp[t];
+        for (int64_t k = prim_offs[p]; k < prim_offs[p + 1]; ++k)
+        {
+            const int64_t c = prim_cross[k];
+            const double* m = W + c * ps * es + row * es;
+            const double* v = y + cross_eliminated[c] * es;
+            double value = 0.0;
+            for (int col = 0; col < es; ++col) value += m[col] * v[col];
+            acc += 1.0 * value;
+        }
+        rhs[t] = acc;
+    }
+}
+
+// TW_c[row][i] = sum_j Cinv_e[i][j] W_c[row][j] (multiplyMatrixVector per row).
+__global__ __launch_bounds__(kBlock) void k_transform_cross(int64_t nc, int ps, int es,
+                                                            const int64_t* __restrict__ cross_eliminated,
+                                                            const double* __restrict__ Cinv,
+                                                            const double* __restrict__ W, double* __restrict__ TW)
+{
+#pragma clang fp contract(off)
+    for (int64_t t = gtid(); t < nc * ps; t += gstride())
+    {
+        const int64_t c = t / ps;
+        const int row = static_cast<int>(t % ps);
+        const double* inv = Cinv + cross_eliminated[c] * es * es;
+        const double* w = W + c * ps * es + row * es;
+        for (int i = 0; i < es; ++i)
+        {
+            double value = 0.0;
+            for (int j = 0; j < es; ++j) value += inv[i * es + j] * w[j];
+            TW[c * ps * es + row * es + i] = value;
+        }
+    }
+}
+
+// One wavefront per block slot, lane = (lr, lc).  Value = damped B on the
+// diagonal, + each primary cross block (forward, or transposed for the
+// mirrored slot), then - (W_l C^-1)(W_r)^T for every term in order
+// (block_schur_sparse_assembly.h:817-903).
+__global__ __launch_bounds__(kBlock) void k_assemble_slots(
+    int64_t slots, int ps, int es, int pb, const unsigned long long* __restrict__ slot_key,
+    const int64_t* __restrict__ slot_start, int64_t entries, const unsigned* __restrict__ term_e,
+    const unsigned* __restrict__ term_left, const unsigned* __restrict__ term_right,
+    const int64_t* __restrict__ bro, const int64_t* __restrict__ ro, const double* __restrict__ Bd,
+    const double* __restrict__ pcv, const double* __restrict__ TW, const double* __restrict__ W,
+    double* __restrict__ values)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x % kWave;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    const unsigned long long mask = (1ull << pb) - 1ull;
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; s < slots;
+         s += waves)
+    {
+        if (lane >= ps * ps) continue;
+        const int lr = lane / ps, lc = lane % ps;
+        const int64_t row = static_cast<int64_t>(slot_key[s] >> pb);
+        const int64_t col = static_cast<int64_t>(slot_key[s] & mask);
+        const int64_t begin = slot_start[s];
+        const int64_t end = s + 1 < slots ? slot_start[s + 1] : entries;
+        double acc = 0.0;
+        // the run holds terms first, then the diagonal marker, then primary
+        // cross markers in index order
+        if (row == col) acc += Bd[row * ps * ps + lr * ps + lc];
+        for (int64_t k = begin; k < end; ++k)
+        {
+            if (term_e[k] != ~1u) continue;
+            const int64_t q = term_left[k];
+            const double* m = pcv + q * ps * ps;
+            acc += term_right[k] == 0 ? m[lr * ps + lc] : m[lc * ps + lr];
+        }
+        int64_t k = begin;
+        for (; k < end && term_e[k] < ~1u; ++k)
+        {
+            const double* t = TW + static_cast<int64_t>(term_left[k]) * ps * es + lr * es;
+            const double* w = W + static_cast<int64_t>(term_right[k]) * ps * es + lc * es;
+            double value = 0.0;
+            for (int i = 0; i < es; ++i) value += t[i] * w[i];
+            acc += -value;
+        }
+        values[ro[row * ps + lr] + (s - bro[row]) * ps + lc] = acc;
+    }
+}
+
+// Preconditioner: inverse of each assembled diagonal block.
+__global__ __launch_bounds__(kBlock) void k_preconditioner(int64_t pc, int ps, const int64_t* __restrict__ diag_slot,
+                                                           const int64_t* __restrict__ bro,
+                                                           const int64_t* __restrict__ ro,
+                                                           const double* __restrict__ values,
+                                                           double* __restrict__ pinv, SchurStatus* st)
+{
+    for (int64_t b = gtid(); b < pc; b += gstride())
+    {
+        double m[kMaxSchurBlock * kMaxSchurBlock], inv[kMaxSchurBlock * kMaxSchurBlock];
+        const int64_t s = diag_slot[b];
+        for (int lr = 0; lr < ps; ++lr)
+            for (int lc = 0; lc < ps; ++lc) m[lr * ps + lc] = values[ro[b * ps + lr] + (s - bro[b]) * ps + lc];
+        if (!invert_pd(m, ps, inv))
+        {
+            atomicMin(&st->bad_preconditioner, static_cast<unsigned long long>(b));
+            for (int i = 0; i < ps * ps; ++i) inv[i] = 0.0;
+        }
+        for (int i = 0; i < ps * ps; ++i) pinv[b * ps * ps + i] = inv[i];
+    }
+}
+
+// Slot of the diagonal block of every primary.
+__global__ __launch_bounds__(kBlock) void k_diag_slots(int64_t pc, int pb, int64_t slots,
+                                                       const unsigned long long* __restrict__ slot_key,
+                                                       int64_t* __restrict__ diag_slot)
+{
+    for (int64_t b = gtid(); b < pc; b += gstride())
+    {
+        const unsigned long long key = (static_cast<unsigned long long>(b) << pb) | static_cast<unsigned long long>(b);
+        int64_t lo = 0, hi = slots;
+        while (lo < hi)
+        {
+            const int64_t mid = (lo + hi) / 2;
+            if (slot_key[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        diag_slot[b] = lo;
+    }
+}
+
+// Back-substitution (block_schur.cpp:422-454): de_e = C^-1 (-g_e - sum W^T dp).
+__global__ __launch_bounds__(kBlock) void k_back_substitute(int64_t ec, int ps, int es,
+                                                            const int64_t* __restrict__ adj_offs,
+                                                            const unsigned* __restrict__ adj,
+                                                            const int64_t* __restrict__ cross_primary,
+                                                            const double* __restrict__ W,
+                                                            const double* __restrict__ Cinv,
+                                                            const double* __restrict__ ge,
+                                                            const double* __restrict__ dp, double* __restrict__ de)
+{
+#pragma clang fp contract(off)
+    for (int64_t e = gtid(); e < ec; e += gstride())
+    {
+        double point[kMaxSchurBlock];
+        for (int i = 0; i < es; ++i) point[i] = -ge[e * es + i];
+        const int64_t k0 = adj_offs ? adj_offs[e] : 0;
+        const int64_t k1 = adj_offs ? adj_offs[e + 1] : 0;
+        for (int64_t k = k0; k < k1; ++k)
+        {
+            const int64_t c = adj[k];
+            const double* values = W + c * ps * es;
+            const double* primary = dp + cross_primary[c] * ps;
+            for (int column = 0; column < es; ++column)
+            {
+                double value = 0.0;
+                for (int row = 0; row < ps; ++row) value += values[row * es + column] * primary[row];
+                point[column] -= value;
+            }
+        }
+        const double* inv = Cinv + e * es * es;
+        for (int row = 0; row < es; ++row)
+        {
+            double value = 0.0;
+            for (int col = 0; col < es; ++col) value += inv[row * es + col] * point[col];
+            de[e * es + row] = value;
+        }
+    }
+}
+
+// ---- host side ---------------------------------------------------------------------
+
+// Bump allocator over one device buffer: a sizing pass (base null) then a
+// carving pass.
+struct Carve
+{
+    char* base = nullptr;
+    size_t off = 0;
+    template <typename T>
+    T* take(int64_t n)
+    {
+        T* p = base ? reinterpret_cast<T*>(base + off) : nullptr;
+        off += (sizeof(T) * static_cast<size_t>(n > 0 ? n : 1) + 255) & ~size_t(255);
+        return p;
+    }
+};
+
+struct SortSpace
+{
+    unsigned long long* keys_a;
+    unsigned long long* keys_b;
+    unsigned* vals_a;
+    unsigned* vals_b;
+    unsigned* hist;
+    unsigned* scratch;
+    unsigned* total;
+};
+
+SortSpace carve_sort(Carve& c, int64_t n)
+{
+    SortSpace s{};
+    const int64_t hist = radix_hist_words(n);
+    s.keys_a = c.take<unsigned long long>(n);
+    s.keys_b = c.take<unsigned long long>(n);
+    s.vals_a = c.take<unsigned>(n);
+    s.vals_b = c.take<unsigned>(n);
+    s.hist = c.take<unsigned>(hist);
+    s.scratch = c.take<unsigned>(scan_scratch_words(std::max<int64_t>(hist, n)));
+    s.total = c.take<unsigned>(4);
+    return s;
+}
+
+} // namespace pm
+
+using namespace pm;
+
+struct pm_schur_workspace
+{
+    hipStream_t stream = nullptr;
+    bool bound = false;
+    int device = -1;
+    SchurStatus* host_status = nullptr;  // pinned
+    DeviceBuffer status;
+
+    // normal equations assembled from residuals
+    DeviceBuffer ne_topology;  // residual grouping (reused while the linkage is unchanged)
+    DeviceBuffer ne_values;    // B, C, g, W, cross ids, weights
+    DeviceBuffer ne_sort;
+    int64_t ne_sig[6] = {-1, -1, -1, -1, -1, -1};  // count, pc, ec, ps, es, rs
+    int64_t ne_runs = 0;
+    int64_t ne_builds = 0;
+    int64_t* ne_primary_copy = nullptr;
+    int64_t* ne_eliminated_copy = nullptr;
+    int64_t* ne_offs_p = nullptr;
+    int64_t* ne_offs_e = nullptr;
+    unsigned* ne_perm_p = nullptr;
+    unsigned* ne_perm_e = nullptr;
+    unsigned* ne_perm_pe = nullptr;
+    int64_t* ne_run_start = nullptr;
+    unsigned* ne_run_cross = nullptr;
+    int64_t* ne_cross_primary = nullptr;
+    int64_t* ne_cross_eliminated = nullptr;
+
+    // Schur pattern (reused while the cross-block topology is unchanged)
+    DeviceBuffer pattern;
+    DeviceBuffer pattern_sort;
+    int64_t sig[6] = {-1, -1, -1, -1, -1, -1};  // pc, ec, ps, es, nc, npc
+    int64_t pattern_builds = 0;
+    int64_t slots = 0, values_count = 0, terms = 0, entries = 0;
+    int pb = 0, eb = 0;
+    int64_t* cp_copy = nullptr;   // cross_primary
+    int64_t* ce_copy = nullptr;   // cross_eliminated
+    int64_t* pr_copy = nullptr;   // primary cross rows
+    int64_t* pcc_copy = nullptr;  // primary cross cols
+    int64_t* adj_offs = nullptr;
+    unsigned* adj = nullptr;
+    int64_t* prim_offs = nullptr;
+    unsigned* prim_cross = nullptr;
+    unsigned long long* slot_key = nullptr;
+    int64_t* slot_start = nullptr;
+    unsigned* term_e = nullptr;
+    unsigned* term_left = nullptr;
+    unsigned* term_right = nullptr;
+    int64_t* bro = nullptr;
+    int64_t* ro = nullptr;
+    int32_t* ro32 = nullptr;
+    int64_t* ci = nullptr;
+    int32_t* ci32 = nullptr;
+    int64_t* diag_slot = nullptr;
+
+    // per-solve numerics
+    DeviceBuffer numerics;
+    double* Bd = nullptr;
+    double* Cd = nullptr;
+    double* Cinv = nullptr;
+    double* y = nullptr;
+    double* rhs = nullptr;
+    double* TW = nullptr;
+    double* values = nullptr;
+    double* pinv = nullptr;
+
+    int bind(hipStream_t s)
+    {
+        int dev = 0;
+        PM_HIP(hipGetDevice(&dev));
+        if (!bound)
+        {
+            stream = s;
+            device = dev;
+            bound = true;
+            PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_status), sizeof(SchurStatus), hipHostMallocDefault));
+            PM_TRY(status.reserve(sizeof(SchurStatus)));
+            return PM_OK;
+        }
+        if (s != stream)
+            return fail(PM_LOGIC_ERROR, "SchurComplementSolverWorkspace is bound to another stream");
+        if (dev != device)
+            return fail(PM_LOGIC_ERROR, "SchurComplementSolverWorkspace is bound to another device");
+        return PM_OK;
+    }
+
+    SchurStatus* dstatus() const { return status.as<SchurStatus>(); }
+
+    int read_status(hipStream_t s)
+    {
+        PM_HIP(hipMemcpyAsync(host_status, dstatus(), sizeof(SchurStatus), hipMemcpyDeviceToHost, s));
+        PM_HIP(hipStreamSynchronize(s));
+        return PM_OK;
+    }
+
+    ~pm_schur_workspace()
+    {
+        if (bound) (void)hipStreamSynchronize(stream);
+        if (host_status) (void)hipHostFree(host_status);
+        status.release();
+        ne_topology.release();
+        ne_values.release();
+        ne_sort.release();
+        pattern.release();
+        pattern_sort.release();
+        numerics.release();
+    }
+};
+
+namespace pm
+{
+
+int sort_by_key(SortSpace& sp, int64_t n, int bits, hipStream_t stream, unsigned long long** keys, unsigned** vals)
+{
+    bool in_b = false;
+    PM_TRY(radix_sort_pairs(sp.keys_a, sp.vals_a, sp.keys_b, sp.vals_b, n, bits, sp.hist, sp.scratch, stream, &in_b));
+    *keys = in_b ? sp.keys_b : sp.keys_a;
+    *vals = in_b ? sp.vals_b : sp.vals_a;
+    return PM_OK;
+}
+
+int check_schur_sizes(int64_t pc, int64_t ec, int64_t ps, int64_t es)
+{
+    if (pc < 0 || ec < 0 || ps < 0 || es < 0)
+        return fail(PM_INVALID_ARGUMENT, "BlockNormalEquations: block counts and sizes must be non-negative");
+    if ((pc > 0 && ps <= 0) || (ec > 0 && es <= 0))
+        return fail(PM_INVALID_ARGUMENT, "BlockNormalEquations: block sizes must be positive");
+    if (ps > kMaxSchurBlock || es > kMaxSchurBlock)
+        return fail(PM_INVALID_ARGUMENT, "the HIP Schur path supports block sizes up to %d", kMaxSchurBlock);
+    if (pc >= (int64_t(1) << 31) || ec >= (int64_t(1) << 31))
+        return fail(PM_OVERFLOW, "Schur block counts must be below 2^31");
+    return PM_OK;
+}
+
+// Device pattern build from the cross-block topology (cached copies in w).
+int build_pattern(pm_schur_workspace* w, const pm_normal_equations* eq, hipStream_t stream)
+{
+    const int64_t pc = eq->primary_count, ec = eq->eliminated_count, nc = eq->cross_count;
+    const int64_t npc = eq->primary_cross_count;
+    const int ps = static_cast<int>(eq->primary_size);
+    const int pb = std::max(1, bits_for(pc)), eb = std::max(1, bits_for(ec));
+    w->pb = pb;
+    w->eb = eb;
+    SchurStatus* st = w->dstatus();
+
+    // ids must address existing blocks before they index anything
+    hipLaunchKernelGGL(k_validate_topology, dim3(grid_for(std::max<int64_t>(std::max(nc, npc), 1))), dim3(kBlock), 0,
+                       stream, nc, npc, pc, ec, eq->cross_primary, eq->cross_eliminated, eq->primary_cross_rows,
+                       eq->primary_cross_cols, st);
+    PM_HIP(hipGetLastError());
+    PM_TRY(w->read_status(stream));
+    if (w->host_status->bad_primary != ~0ull)
+        return fail(PM_OUT_OF_RANGE, "solveDampedSchurComplement: cross block %llu has an out-of-range primary block",
+                    w->host_status->bad_primary);
+    if (w->host_status->bad_eliminated != ~0ull)
+        return fail(PM_OUT_OF_RANGE,
+                    "solveDampedSchurComplement: cross block %llu has an out-of-range eliminated block",
+                    w->host_status->bad_eliminated);
+    if (w->host_status->bad_value != ~0ull)
+        return fail(PM_OUT_OF_RANGE,
+                    "solveDampedSchurComplement: primary cross block %llu needs row < column < primary count",
+                    w->host_status->bad_value);
+
+    // stage 1: adjacency by eliminated and by primary, pair counts (the sort
+    // space also serves the scan over ec + 1 pair counts)
+    const int64_t n1 = std::max<int64_t>(std::max<int64_t>(nc, ec + 1), 1);
+    Carve s1z;
+    carve_sort(s1z, n1);
+    const size_t stage1 = s1z.off;
+    Carve pz;
+    pz.take<int64_t>(nc);
+    pz.take<int64_t>(nc);
+    pz.take<int64_t>(npc);
+    pz.take<int64_t>(npc);
+    pz.take<int64_t>(ec + 1);
+    pz.take<unsigned>(nc);
+    pz.take<int64_t>(pc + 1);
+    pz.take<unsigned>(nc);
+    pz.take<unsigned>(ec + 1);
+    const size_t head_bytes = pz.off;
+    PM_TRY(w->pattern_sort.reserve(std::max(stage1, head_bytes)));
+    // the first part of the pattern buffer holds the topology copies and
+    // adjacency; the rest is sized once the term count is known
+    Carve sc;
+    sc.base = w->pattern_sort.as<char>();
+    SortSpace sp = carve_sort(sc, n1);
+
+    // temporary holder for the head arrays until the final pattern buffer exists
+    DeviceBuffer head;
+    struct Release
+    {
+        DeviceBuffer& b;
+        ~Release() { b.release(); }
+    } release_head{head};
+    PM_TRY(head.reserve(head_bytes));
+    Carve hc;
+    hc.base = head.as<char>();
+    int64_t* cp = hc.take<int64_t>(nc);
+    int64_t* ce = hc.take<int64_t>(nc);
+    int64_t* pr = hc.take<int64_t>(npc);
+    int64_t* pcl = hc.take<int64_t>(npc);
+    int64_t* adj_offs = hc.take<int64_t>(ec + 1);
+    unsigned* adj = hc.take<unsigned>(nc);
+    int64_t* prim_offs = hc.take<int64_t>(pc + 1);
+    unsigned* prim_cross = hc.take<unsigned>(nc);
+    const size_t prefix_bytes = hc.off;  // same carving as the front of the final pattern buffer
+    unsigned* pair_offs = hc.take<unsigned>(ec + 1);
+
+    const int gc = grid_for(n1);
+    if (nc > 0)
+    {
+        hipLaunchKernelGGL(k_copy_i64, dim3(gc), dim3(kBlock), 0, stream, eq->cross_primary, nc, cp);
+        hipLaunchKernelGGL(k_copy_i64, dim3(gc), dim3(kBlock), 0, stream, eq->cross_eliminated, nc, ce);
+    }
+    if (npc > 0)
+    {
+        hipLaunchKernelGGL(k_copy_i64, dim3(grid_for(npc)), dim3(kBlock), 0, stream, eq->primary_cross_rows, npc, pr);
+        hipLaunchKernelGGL(k_copy_i64, dim3(grid_for(npc)), dim3(kBlock), 0, stream, eq->primary_cross_cols, npc, pcl);
+    }
+    // keys: eliminated id (adjacency, creation order kept by stability)
+    unsigned long long* sk = nullptr;
+    unsigned* sv = nullptr;
+    if (nc > 0)
+    {
+        hipLaunchKernelGGL(k_cross_keys, dim3(gc), dim3(kBlock), 0, stream, nc, cp, ce, 0, sp.keys_a, sp.vals_a);
+        PM_TRY(sort_by_key(sp, nc, eb, stream, &sk, &sv));
+        hipLaunchKernelGGL(k_group_offsets, dim3(grid_for(ec + 1)), dim3(kBlock), 0, stream, sk, nc, 0, ec, adj_offs);
+        PM_HIP(hipMemcpyAsync(adj, sv, sizeof(unsigned) * nc, hipMemcpyDeviceToDevice, stream));
+        // (primary, eliminated) order for the reduced right-hand side
+        hipLaunchKernelGGL(k_cross_keys, dim3(gc), dim3(kBlock), 0, stream, nc, cp, ce, eb, sp.keys_a, sp.vals_a);
+        PM_TRY(sort_by_key(sp, nc, pb + eb, stream, &sk, &sv));
+        hipLaunchKernelGGL(k_group_offsets, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, sk, nc, eb, pc,
+                           prim_offs);
+        PM_HIP(hipMemcpyAsync(prim_cross, sv, sizeof(unsigned) * nc, hipMemcpyDeviceToDevice, stream));
+    }
+    else
+    {
+        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ec + 1)), dim3(kBlock), 0, stream, adj_offs, ec + 1, 0);
+        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, prim_offs, pc + 1, 0);
+    }
+    hipLaunchKernelGGL(k_pair_counts, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, adj_offs, pair_offs);
+    // pair counts may exceed u32 in total only past 2^32 terms; checked below in 64 bits
+    hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(kBlock), 0, stream, ec, adj_offs, st);
+    PM_TRY(scan_u32(pair_offs, pair_offs, ec + 1, sp.scratch, nullptr, stream));
+    PM_HIP(hipGetLastError());
+    PM_TRY(w->read_status(stream));
+    const int64_t terms = static_cast<int64_t>(w->host_status->count0);
+    const int64_t entries = terms + pc + 2 * npc;
+    if (entries >= (int64_t(1) << 31))
+        return fail(PM_OVERFLOW, "Schur pattern has %lld terms; the HIP path supports fewer than 2^31",
+                    static_cast<long long>(terms));
+
+    // stage 2: sort all block keys
+    Carve s2z;
+    carve_sort(s2z, std::max<int64_t>(entries, 1));
+    s2z.take<unsigned>(entries);  // heads
+    s2z.take<unsigned>(entries);  // slot ids
+    PM_TRY(w->pattern_sort.reserve(std::max(s2z.off, stage1)));
+    Carve sc2;
+    sc2.base = w->pattern_sort.as<char>();
+    SortSpace sp2 = carve_sort(sc2, std::max<int64_t>(entries, 1));
+    unsigned* heads = sc2.take<unsigned>(entries);
+    unsigned* slot_id = sc2.take<unsigned>(entries);
+    hipLaunchKernelGGL(k_pattern_keys, dim3(grid_for(std::max(ec, std::max(pc, npc)))), dim3(kBlock), 0, stream, ec,
+                       pc, npc, pb, adj_offs, adj, pair_offs, cp, pr, pcl, terms, sp2.keys_a, sp2.vals_a);
+    PM_HIP(hipGetLastError());
+    PM_TRY(sort_by_key(sp2, entries, 2 * pb, stream, &sk, &sv));
+    const int ge = grid_for(entries);
+    hipLaunchKernelGGL(k_key_heads, dim3(ge), dim3(kBlock), 0, stream, sk, entries, heads);
+    PM_TRY(scan_u32(heads, slot_id, entries, sp2.scratch, sp2.total, stream));
+    unsigned slots32 = 0;
+    PM_HIP(hipMemcpyAsync(&slots32, sp2.total, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    const int64_t slots = slots32;
+    const int64_t values = slots * ps * ps;
+    const int64_t dim = pc * ps;
+    if (values >= (int64_t(1) << 31)) return fail(PM_OVERFLOW, "reduced Schur system has too many values");
+
+    // final pattern buffer
+    Carve fz;
+    fz.take<int64_t>(nc);
+    fz.take<int64_t>(nc);
+    fz.take<int64_t>(npc);
+    fz.take<int64_t>(npc);
+    fz.take<int64_t>(ec + 1);
+    fz.take<unsigned>(nc);
+    fz.take<int64_t>(pc + 1);
+    fz.take<unsigned>(nc);
+    fz.take<unsigned long long>(slots);
+    fz.take<int64_t>(slots + 1);
+    fz.take<unsigned>(entries);
+    fz.take<unsigned>(entries);
+    fz.take<unsigned>(entries);
+    fz.take<int64_t>(pc + 1);
+    fz.take<int64_t>(dim + 1);
+    fz.take<int32_t>(dim + 1);
+    fz.take<int64_t>(values);
+    fz.take<int32_t>(values);
+    fz.take<int64_t>(pc);
+    PM_TRY(w->pattern.reserve(fz.off));
+    Carve f;
+    f.base = w->pattern.as<char>();
+    w->cp_copy = f.take<int64_t>(nc);
+    w->ce_copy = f.take<int64_t>(nc);
+    w->pr_copy = f.take<int64_t>(npc);
+    w->pcc_copy = f.take<int64_t>(npc);
+    w->adj_offs = f.take<int64_t>(ec + 1);
+    w->adj = f.take<unsigned>(nc);
+    w->prim_offs = f.take<int64_t>(pc + 1);
+    w->prim_cross = f.take<unsigned>(nc);
+    w->slot_key = f.take<unsigned long long>(slots);
+    w->slot_start = f.take<int64_t>(slots + 1);
+    w->term_e = f.take<unsigned>(entries);
+    w->term_left = f.take<unsigned>(entries);
+    w->term_right = f.take<unsigned>(entries);
+    w->bro = f.take<int64_t>(pc + 1);
+    w->ro = f.take<int64_t>(dim + 1);
+    w->ro32 = f.take<int32_t>(dim + 1);
+    w->ci = f.take<int64_t>(values);
+    w->ci32 = f.take<int32_t>(values);
+    w->diag_slot = f.take<int64_t>(pc);
+    // the head block was carved in the same order and sizes as the front of
+    // the pattern buffer, so one copy moves cp, ce, pr, pcl, adjacency and
+    // the primary lists
+    PM_HIP(hipMemcpyAsync(w->cp_copy, cp, prefix_bytes, hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_slots, dim3(ge), dim3(kBlock), 0, stream, sk, heads, slot_id, entries, w->slot_key,
+                       w->slot_start);
+    hipLaunchKernelGGL(k_decode_terms, dim3(ge), dim3(kBlock), 0, stream, sv, entries, terms, pc, ec, pair_offs,
+                       adj_offs, adj, w->term_e, w->term_left, w->term_right);
+    hipLaunchKernelGGL(k_group_offsets, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, w->slot_key, slots, pb, pc,
+                       w->bro);
+    hipLaunchKernelGGL(k_scalar_rows, dim3(grid_for(dim + 1)), dim3(kBlock), 0, stream, pc, ps, w->bro, w->ro,
+                       w->ro32);
+    hipLaunchKernelGGL(k_scalar_cols, dim3(grid_for(values)), dim3(kBlock), 0, stream, slots, ps, pb, w->slot_key,
+                       w->bro, w->ro, w->ci, w->ci32);
+    hipLaunchKernelGGL(k_diag_slots, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, pb, slots, w->slot_key,
+                       w->diag_slot);
+    PM_HIP(hipGetLastError());
+    PM_HIP(hipStreamSynchronize(stream));
+    w->slots = slots;
+    w->values_count = values;
+    w->terms = terms;
+    w->entries = entries;
+    ++w->pattern_builds;
+    return PM_OK;
+}
+
+} // namespace pm
+
+namespace pm
+{
+
+int residual_pass(pm_schur_workspace* w, const pm_residual_batch* rb, int64_t pc, int64_t ec, int ps, int es,
+                  double* weights_out, hipStream_t stream, bool sort_needed)
+{
+    const int64_t n = rb->count;
+    const int pbits = std::max(1, bits_for(pc)), ebits = std::max(1, bits_for(ec));
+    Carve sizing;
+    carve_sort(sizing, n);
+    sizing.take<unsigned long long>(n);
+    sizing.take<unsigned long long>(n);
+    sizing.take<unsigned>(n);
+    sizing.take<unsigned>(n);
+    sizing.take<unsigned>(n);
+    sizing.take<unsigned>(n);
+    sizing.take<unsigned>(n);
+    PM_TRY(w->ne_sort.reserve(sizing.off));
+    Carve c;
+    c.base = w->ne_sort.as<char>();
+    SortSpace sp = carve_sort(c, n);
+    unsigned long long* key_e = c.take<unsigned long long>(n);
+    unsigned long long* key_pe = c.take<unsigned long long>(n);
+    unsigned* val_e = c.take<unsigned>(n);
+    unsigned* val_pe = c.take<unsigned>(n);
+    unsigned* heads = c.take<unsigned>(n);
+    unsigned* run_id = c.take<unsigned>(n);
+    unsigned* first = c.take<unsigned>(n);
+
+    SchurStatus* st = w->dstatus();
+    const int g = grid_for(n);
+    if (n > 0)
+    {
+        hipLaunchKernelGGL(k_residual_keys, dim3(g), dim3(kBlock), 0, stream, n, pc, ec,
+                           static_cast<int>(rb->residual_size), ps, es, ebits, rb->primary, rb->eliminated,
+                           rb->primary_jacobians, rb->eliminated_jacobians, rb->residuals, rb->weights,
+                           rb->huber_delta, weights_out, sp.keys_a, key_e, key_pe, sp.vals_a, val_e, val_pe, st);
+        PM_HIP(hipGetLastError());
+    }
+    PM_TRY(w->read_status(stream));
+    const SchurStatus& hs = *w->host_status;
+    if (hs.bad_primary != ~0ull)
+        return fail(PM_OUT_OF_RANGE, "BlockNormalEquations: primary block index out of range (residual %llu)",
+                    hs.bad_primary);
+    if (hs.bad_eliminated != ~0ull)
+        return fail(PM_OUT_OF_RANGE,
+                    "BlockNormalEquations::addResidualBlocks: eliminated block out of range (residual %llu)",
+                    hs.bad_eliminated);
+    if (hs.bad_value != ~0ull)
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations::addResidualBlocks: Jacobian and residual values must be finite "
+                    "(residual %llu)",
+                    hs.bad_value);
+    if (hs.bad_weight != ~0ull)
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations::addResidualBlocks: weight must be finite and non-negative (residual %llu)",
+                    hs.bad_weight);
+    if (!sort_needed) return PM_OK;
+
+    // persistent grouping
+    Carve tz;
+    tz.take<int64_t>(n);
+    tz.take<int64_t>(n);
+    tz.take<int64_t>(pc + 1);
+    tz.take<int64_t>(ec + 1);
+    tz.take<unsigned>(n);
+    tz.take<unsigned>(n);
+    tz.take<unsigned>(n);
+    tz.take<int64_t>(n + 1);
+    tz.take<unsigned>(n);
+    tz.take<int64_t>(n);
+    tz.take<int64_t>(n);
+    PM_TRY(w->ne_topology.reserve(tz.off));
+    Carve t;
+    t.base = w->ne_topology.as<char>();
+    w->ne_primary_copy = t.take<int64_t>(n);
+    w->ne_eliminated_copy = t.take<int64_t>(n);
+    w->ne_offs_p = t.take<int64_t>(pc + 1);
+    w->ne_offs_e = t.take<int64_t>(ec + 1);
+    w->ne_perm_p = t.take<unsigned>(n);
+    w->ne_perm_e = t.take<unsigned>(n);
+    w->ne_perm_pe = t.take<unsigned>(n);
+    w->ne_run_start = t.take<int64_t>(n + 1);
+    w->ne_run_cross = t.take<unsigned>(n);
+    w->ne_cross_primary = t.take<int64_t>(n);
+    w->ne_cross_eliminated = t.take<int64_t>(n);
+    if (n == 0)
+    {
+        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, w->ne_offs_p, pc + 1, 0);
+        hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(ec + 1)), dim3(kBlock), 0, stream, w->ne_offs_e, ec + 1, 0);
+        PM_HIP(hipGetLastError());
+        w->ne_runs = 0;
+        ++w->ne_builds;
+        return PM_OK;
+    }
+    hipLaunchKernelGGL(k_copy_i64, dim3(g), dim3(kBlock), 0, stream, rb->primary, n, w->ne_primary_copy);
+    hipLaunchKernelGGL(k_copy_i64, dim3(g), dim3(kBlock), 0, stream, rb->eliminated, n, w->ne_eliminated_copy);
+    unsigned long long* sk;
+    unsigned* sv;
+    // by primary
+    PM_TRY(sort_by_key(sp, n, pbits, stream, &sk, &sv));
+    hipLaunchKernelGGL(k_group_offsets, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, sk, n, 0, pc, w->ne_offs_p);
+    PM_HIP(hipMemcpyAsync(w->ne_perm_p, sv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, stream));
+    // by eliminated
+    PM_HIP(hipMemcpyAsync(sp.keys_a, key_e, sizeof(unsigned long long) * n, hipMemcpyDeviceToDevice, stream));
+    PM_HIP(hipMemcpyAsync(sp.vals_a, val_e, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, stream));
+    PM_TRY(sort_by_key(sp, n, ebits, stream, &sk, &sv));
+    hipLaunchKernelGGL(k_group_offsets, dim3(grid_for(ec + 1)), dim3(kBlock), 0, stream, sk, n, 0, ec, w->ne_offs_e);
+    PM_HIP(hipMemcpyAsync(w->ne_perm_e, sv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, stream));
+    // by (primary, eliminated): runs are the cross blocks
+    PM_HIP(hipMemcpyAsync(sp.keys_a, key_pe, sizeof(unsigned long long) * n, hipMemcpyDeviceToDevice, stream));
+    PM_HIP(hipMemcpyAsync(sp.vals_a, val_pe, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, stream));
+    PM_TRY(sort_by_key(sp, n, pbits + ebits, stream, &sk, &sv));
+    PM_HIP(hipMemcpyAsync(w->ne_perm_pe, sv, sizeof(unsigned) * n, hipMemcpyDeviceToDevice, stream));
+    hipLaunchKernelGGL(k_zero_u32, dim3(g), dim3(kBlock), 0, stream, first, n);
+    hipLaunchKernelGGL(k_pair_heads, dim3(g), dim3(kBlock), 0, stream, sk, sv, n, heads, first);
+    PM_TRY(scan_u32(heads, run_id, n, sp.scratch, sp.total, stream));
+    PM_TRY(scan_u32(first, first, n, sp.scratch, nullptr, stream));
+    hipLaunchKernelGGL(k_pair_runs, dim3(g), dim3(kBlock), 0, stream, sk, sv, heads, run_id, first, n, ebits,
+                       w->ne_run_start, w->ne_run_cross, w->ne_cross_primary, w->ne_cross_eliminated);
+    PM_HIP(hipGetLastError());
+    unsigned runs = 0;
+    PM_HIP(hipMemcpyAsync(&runs, sp.total, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    w->ne_runs = runs;
+    ++w->ne_builds;
+    return PM_OK;
+}
+
+int check_normal_equations(const pm_normal_equations* eq)
+{
+    if (!eq) return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: null normal equations");
+    PM_TRY(check_schur_sizes(eq->primary_count, eq->eliminated_count, eq->primary_size, eq->eliminated_size));
+    if (eq->cross_count < 0 || eq->primary_cross_count < 0)
+        return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: negative cross-block count");
+    const int64_t pd = eq->primary_count * eq->primary_size, ed = eq->eliminated_count * eq->eliminated_size;
+    if ((pd > 0 && (!eq->primary_diagonal || !eq->primary_gradient)) ||
+        (ed > 0 && (!eq->eliminated_diagonal || !eq->eliminated_gradient)) ||
+        (eq->cross_count > 0 && (!eq->cross_primary || !eq->cross_eliminated || !eq->cross_values)) ||
+        (eq->primary_cross_count > 0 &&
+         (!eq->primary_cross_rows || !eq->primary_cross_cols || !eq->primary_cross_values)))
+        return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: null normal-equation arrays");
+    return PM_OK;
+}
+
+} // namespace pm
+
+extern "C" {
+
+int pm_schur_workspace_create(pm_schur_workspace** out)
+{
+    if (!out) return fail(PM_INVALID_ARGUMENT, "pm_schur_workspace_create: null output");
+    *out = new (std::nothrow) pm_schur_workspace();
+    if (!*out) return fail(PM_RUNTIME_ERROR, "pm_schur_workspace_create: out of host memory");
+    return PM_OK;
+}
+
+int pm_schur_workspace_destroy(pm_schur_workspace* w)
+{
+    delete w;
+    return PM_OK;
+}
+
+int64_t pm_schur_pattern_build_count(const pm_schur_workspace* w) { return w ? w->pattern_builds : 0; }
+
+int pm_schur_pattern_size(const pm_schur_workspace* w, int64_t* slots, int64_t* values, int64_t* terms)
+{
+    if (!w) return fail(PM_INVALID_ARGUMENT, "pm_schur_pattern_size: null workspace");
+    if (slots) *slots = w->slots;
+    if (values) *values = w->values_count;
+    if (terms) *terms = w->terms;
+    return PM_OK;
+}
+
+int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, int64_t ps, int64_t es,
+                                 const pm_residual_batch* rb, pm_normal_equations* out, hipStream_t stream)
+{
+    if (!w || !rb || !out) return fail(PM_INVALID_ARGUMENT, "pm_normal_equations_assemble: null argument");
+    PM_TRY(check_schur_sizes(pc, ec, ps, es));
+    if (rb->count < 0 || rb->residual_size <= 0)
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations::addResidualBlocks: pointers must be non-null and residual size positive");
+    if (rb->count >= (int64_t(1) << 31)) return fail(PM_OVERFLOW, "residual batch must hold fewer than 2^31 residuals");
+    if (rb->count > 0 && (ps <= 0 || es <= 0 || pc <= 0 || ec <= 0))
+        return fail(PM_INVALID_ARGUMENT, "a residual batch needs primary and eliminated blocks");
+    if (rb->count > 0 && (!rb->primary || !rb->eliminated || !rb->primary_jacobians || !rb->eliminated_jacobians ||
+                          !rb->residuals))
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations::addResidualBlocks: pointers must be non-null and residual size positive");
+    if (!rb->weights && (!std::isfinite(rb->huber_delta) || rb->huber_delta < 0.0))
+        return fail(PM_INVALID_ARGUMENT, "evaluateHuberLoss: delta must be finite and non-negative");
+    PM_TRY(w->bind(stream));
+    const int64_t n = rb->count;
+    const int64_t sig[6] = {n, pc, ec, ps, es, rb->residual_size};
+    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, w->dstatus());
+    bool reuse = w->ne_builds > 0 && std::equal(sig, sig + 6, w->ne_sig);
+    if (reuse && n > 0)
+    {
+        hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(n)), dim3(kBlock), 0, stream, rb->primary,
+                           w->ne_primary_copy, n, w->dstatus());
+        hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(n)), dim3(kBlock), 0, stream, rb->eliminated,
+                           w->ne_eliminated_copy, n, w->dstatus());
+        PM_HIP(hipGetLastError());
+    }
+    // values buffer needs the run count: size it for the worst case (n runs)
+    Carve vz;
+    vz.take<double>(pc * ps * ps);
+    vz.take<double>(ec * es * es);
+    vz.take<double>(pc * ps);
+    vz.take<double>(ec * es);
+    vz.take<double>(n * ps * es);
+    vz.take<double>(n);
+    PM_TRY(w->ne_values.reserve(vz.off));
+    Carve v;
+    v.base = w->ne_values.as<char>();
+    double* B = v.take<double>(pc * ps * ps);
+    double* C = v.take<double>(ec * es * es);
+    double* gp = v.take<double>(pc * ps);
+    double* ge = v.take<double>(ec * es);
+    double* W = v.take<double>(n * ps * es);
+    double* weights = v.take<double>(n);
+    if (reuse)
+    {
+        PM_TRY(w->read_status(stream));
+        reuse = w->host_status->mismatch == 0;
+        hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, w->dstatus());
+    }
+    PM_TRY(residual_pass(w, rb, pc, ec, static_cast<int>(ps), static_cast<int>(es), weights, stream, !reuse));
+    std::copy(sig, sig + 6, w->ne_sig);
+    const int rs = static_cast<int>(rb->residual_size);
+    const int64_t runs = w->ne_runs;
+    if (pc * ps > 0)
+        hipLaunchKernelGGL(k_ne_primary, dim3(grid_for(pc * (ps * ps + ps))), dim3(kBlock), 0, stream, pc, rs,
+                           static_cast<int>(ps), w->ne_offs_p, w->ne_perm_p, rb->primary_jacobians, rb->residuals,
+                           weights, B, gp);
+    if (ec * es > 0)
+        hipLaunchKernelGGL(k_ne_eliminated, dim3(grid_for(ec * (es * es + es))), dim3(kBlock), 0, stream, ec, rs,
+                           static_cast<int>(es), w->ne_offs_e, w->ne_perm_e, rb->eliminated_jacobians,
+                           rb->residuals, weights, C, ge);
+    if (runs > 0)
+        hipLaunchKernelGGL(k_ne_cross, dim3(grid_for(runs * ps * es)), dim3(kBlock), 0, stream, runs, n, rs,
+                           static_cast<int>(ps), static_cast<int>(es), w->ne_run_start, w->ne_run_cross,
+                           w->ne_perm_pe, rb->primary_jacobians, rb->eliminated_jacobians, weights, W);
+    PM_HIP(hipGetLastError());
+    out->primary_count = pc;
+    out->eliminated_count = ec;
+    out->primary_size = ps;
+    out->eliminated_size = es;
+    out->primary_diagonal = B;
+    out->eliminated_diagonal = C;
+    out->primary_gradient = gp;
+    out->eliminated_gradient = ge;
+    out->cross_count = runs;
+    out->cross_primary = w->ne_cross_primary;
+    out->cross_eliminated = w->ne_cross_eliminated;
+    out->cross_values = W;
+    out->primary_cross_count = 0;
+    out->primary_cross_rows = nullptr;
+    out->primary_cross_cols = nullptr;
+    out->primary_cross_values = nullptr;
+    return PM_OK;
+}
+
+int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double damping, const pm_schur_options* o,
+                   double* dp, double* de, pm_schur_report* report, pm_workspace* ws, hipStream_t stream)
+{
+    if (!w || !o || !report || !ws) return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: null argument");
+    std::memset(report, 0, sizeof(*report));
+    PM_TRY(check_normal_equations(eq));
+    const int64_t pc = eq->primary_count, ec = eq->eliminated_count;
+    const int ps = static_cast<int>(eq->primary_size), es = static_cast<int>(eq->eliminated_size);
+    const int64_t pd = pc * ps, ed = ec * es, nc = eq->cross_count, npc = eq->primary_cross_count;
+    if ((pd > 0 && !dp) || (ed > 0 && !de) || (dp && dp == de))
+        return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: output vectors must be non-null and distinct");
+    if (!std::isfinite(damping) || damping < 0.0 || o->max_iterations <= 0 || !std::isfinite(o->relative_tolerance) ||
+        o->relative_tolerance < 0.0 || !std::isfinite(o->absolute_tolerance) || o->absolute_tolerance < 0.0 ||
+        !std::isfinite(o->minimum_diagonal) || o->minimum_diagonal <= 0.0)
+        return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: invalid solver options");
+    PM_TRY(w->bind(stream));
+    SchurStatus* st = w->dstatus();
+    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
+
+    // pattern reuse: compare the cross-block topology with the cached copy
+    const int64_t sig[6] = {pc, ec, ps, es, nc, npc};
+    bool reuse = w->pattern_builds > 0 && std::equal(sig, sig + 6, w->sig);
+    if (reuse)
+    {
+        if (nc > 0)
+        {
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(nc)), dim3(kBlock), 0, stream, eq->cross_primary,
+                               w->cp_copy, nc, st);
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(nc)), dim3(kBlock), 0, stream, eq->cross_eliminated,
+                               w->ce_copy, nc, st);
+        }
+        if (npc > 0)
+        {
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(npc)), dim3(kBlock), 0, stream,
+                               eq->primary_cross_rows, w->pr_copy, npc, st);
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(npc)), dim3(kBlock), 0, stream,
+                               eq->primary_cross_cols, w->pcc_copy, npc, st);
+        }
+        PM_HIP(hipGetLastError());
+        PM_TRY(w->read_status(stream));
+        reuse = w->host_status->mismatch == 0;
+    }
+    if (!reuse && pd > 0)
+    {
+        hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
+        PM_TRY(build_pattern(w, eq, stream));
+        std::copy(sig, sig + 6, w->sig);
+    }
+    report->pattern_reused = reuse ? 1 : 0;
+    report->pattern_build_count = w->pattern_builds;
+
+    // numerics
+    Carve nz;
+    nz.take<double>(pc * ps * ps);
+    nz.take<double>(ec * es * es);
+    nz.take<double>(ec * es * es);
+    nz.take<double>(ec * es);
+    nz.take<double>(pd);
+    nz.take<double>(nc * ps * es);
+    nz.take<double>(pd > 0 ? w->values_count : 0);
+    nz.take<double>(pc * ps * ps);
+    PM_TRY(w->numerics.reserve(nz.off));
+    Carve n;
+    n.base = w->numerics.as<char>();
+    w->Bd = n.take<double>(pc * ps * ps);
+    w->Cd = n.take<double>(ec * es * es);
+    w->Cinv = n.take<double>(ec * es * es);
+    w->y = n.take<double>(ec * es);
+    w->rhs = n.take<double>(pd);
+    w->TW = n.take<double>(nc * ps * es);
+    w->values = n.take<double>(pd > 0 ? w->values_count : 0);
+    w->pinv = n.take<double>(pc * ps * ps);
+
+    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
+    if (pc > 0)
+        hipLaunchKernelGGL(k_damp, dim3(grid_for(pc * ps * ps)), dim3(kBlock), 0, stream, eq->primary_diagonal, pc,
+                           ps, damping, o->minimum_diagonal, w->Bd);
+    if (ec > 0)
+    {
+        hipLaunchKernelGGL(k_damp, dim3(grid_for(ec * es * es)), dim3(kBlock), 0, stream, eq->eliminated_diagonal, ec,
+                           es, damping, o->minimum_diagonal, w->Cd);
+        hipLaunchKernelGGL(k_eliminated_inverse, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, es, w->Cd,
+                           eq->eliminated_gradient, w->Cinv, w->y, st);
+    }
+    if (pd > 0)
+    {
+        hipLaunchKernelGGL(k_reduced_rhs, dim3(grid_for(pd)), dim3(kBlock), 0, stream, pc, ps, es,
+                           eq->primary_gradient, w->prim_offs, w->prim_cross, eq->cross_eliminated, eq->cross_values,
+                           w->y, w->rhs);
+        if (nc > 0)
+            hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
+                               eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
+        const int64_t waves_per_block = kBlock / kWave;
+        const int slot_grid =
+            static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
+        hipLaunchKernelGGL(k_assemble_slots, dim3(slot_grid), dim3(kBlock), 0, stream, w->slots, ps, es, w->pb,
+                           w->slot_key, w->slot_start, w->entries, w->term_e, w->term_left, w->term_right, w->bro,
+                           w->ro, w->Bd, eq->primary_cross_values, w->TW, eq->cross_values, w->values);
+        hipLaunchKernelGGL(k_preconditioner, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, ps, w->diag_slot, w->bro,
+                           w->ro, w->values, w->pinv, st);
+    }
+    PM_HIP(hipGetLastError());
+    PM_TRY(w->read_status(stream));
+    report->assembly_on_device = 1;
+    if (w->host_status->bad_inverse != ~0ull)
+    {
+        std::snprintf(report->message, sizeof(report->message), "eliminated block is not positive definite");
+        report->status = 1;
+        return PM_OK;
+    }
+    if (w->host_status->bad_preconditioner != ~0ull)
+    {
+        std::snprintf(report->message, sizeof(report->message), "Schur preconditioner block is not positive definite");
+        report->status = 2;
+        return PM_OK;
+    }
+    if (ed > 0) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(ed)), dim3(kBlock), 0, stream, de, ed, 0.0);
+    if (pd > 0 && !o->use_initial_guess)
+        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(pd)), dim3(kBlock), 0, stream, dp, pd, 0.0);
+    PM_HIP(hipGetLastError());
+    if (pd == 0)
+    {
+        report->converged = 1;
+    }
+    else
+    {
+        pm_csr a{};
+        a.rows = pd;
+        a.cols = pd;
+        a.nnz = w->values_count;
+        a.row_offsets = w->ro;
+        a.col_indices = w->ci;
+        a.values = w->values;
+        a.dtype = PM_F64;
+        a.row_offsets32 = w->ro32;
+        a.col_indices32 = w->ci32;
+        pm_solver_options so{};
+        so.max_iterations = o->max_iterations;
+        so.relative_tolerance = o->relative_tolerance;
+        so.absolute_tolerance = o->absolute_tolerance;
+        so.use_jacobi = 1;
+        so.require_convergence = 0;
+        pm_solver_report sr{};
+        PM_TRY(pm_block_pcg(&a, w->rhs, dp, w->pinv, ps, &so, nullptr, &sr, ws, stream));
+        report->converged = sr.converged;
+        report->iterations = sr.iterations;
+        report->initial_residual_norm = sr.initial_residual;
+        report->final_residual_norm = sr.final_residual;
+    }
+    if (!report->converged)
+    {
+        report->status = 4;
+        std::snprintf(report->message, sizeof(report->message), "PCG iteration limit reached");
+        return PM_OK;
+    }
+    if (ec > 0)
+        hipLaunchKernelGGL(k_back_substitute, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, ps, es,
+                           pd > 0 ? w->adj_offs : nullptr, w->adj, eq->cross_primary, eq->cross_values, w->Cinv,
+                           eq->eliminated_gradient, dp, de);
+    PM_HIP(hipGetLastError());
+    std::snprintf(report->message, sizeof(report->message), "converged");
+    return PM_OK;
+}
+
+int pm_schur_system(const pm_schur_workspace* w, pm_csr* reduced, const double** rhs, const double** preconditioner)
+{
+    if (!w || !reduced) return fail(PM_INVALID_ARGUMENT, "pm_schur_system: null argument");
+    if (w->pattern_builds == 0 || !w->values) return fail(PM_LOGIC_ERROR, "pm_schur_system: no assembled system");
+    std::memset(reduced, 0, sizeof(*reduced));
+    reduced->rows = w->sig[0] * w->sig[2];
+    reduced->cols = reduced->rows;
+    reduced->nnz = w->values_count;
+    reduced->row_offsets = w->ro;
+    reduced->col_indices = w->ci;
+    reduced->values = w->values;
+    reduced->dtype = PM_F64;
+    reduced->row_offsets32 = w->ro32;
+    reduced->col_indices32 = w->ci32;
+    if (rhs) *rhs = w->rhs;
+    if (preconditioner) *preconditioner = w->pinv;
+    return PM_OK;
+}
+
+} // extern "C"

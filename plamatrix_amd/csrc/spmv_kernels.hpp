@@ -1,15 +1,17 @@
 // CSR SpMV family for gfx950.
 //
-// Wave-stream rows: a wavefront owns 64 consecutive rows (lane i <-> row i).
-// It loads the 65 row offsets with one coalesced load, streams the chunk's
-// contiguous values and column indices with back-to-back coalesced loads,
-// gathers x, stages the products in LDS and then every lane sums its own row
-// left to right.  All loads of a chunk are independent, so a wave keeps many
-// requests in flight instead of walking a dependent offset -> column -> x chain
-// row by row.  A chunk whose rows hold more than CAP entries (long rows) falls
-// back to 8 lanes per row with a fixed butterfly.  Workgroups own contiguous
-// chunk ranges and are renumbered so each XCD walks a contiguous eighth of the
-// matrix, keeping the gathered x entries in its own L2.
+// Wave-walk rows: a wavefront owns 64 consecutive rows (lane i <-> row i).
+// It loads the 65 row offsets with one coalesced load and stages the chunk's
+// contiguous values and column indices in LDS with back-to-back coalesced
+// loads.  Then every lane walks its own row left to right out of LDS.  For
+// banded and stencil matrices, entry j of 64 consecutive rows points at
+// nearly consecutive columns, so each x gather of the wave touches a few
+// cache lines instead of one line per lane; that gather cost, not HBM, is
+// what bounds a lane-parallel CSR SpMV on this part.  Chunks of more than
+// kMaxWindows * kStreamCap entries (long rows) fall back to 8 lanes per row
+// with a fixed butterfly.  Workgroups own contiguous chunk ranges and are
+// renumbered so each XCD walks a contiguous eighth of the matrix, keeping
+// the gathered x entries in its own L2.
 //
 // Reductions across workgroups never use atomics or device-scope fences (on a
 // multi-XCD part such a fence writes back the whole L2): each workgroup
@@ -28,7 +30,9 @@ namespace pm
 
 constexpr int kChunkRows = 64;               // rows per wavefront chunk
 constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kStreamCap = 1024;             // staged products per wave (LDS)
+constexpr int kStreamCap = 512;              // entries staged per wave and window (LDS)
+constexpr int kMaxWindows = 4;               // windows per chunk before the long-row fallback
+constexpr int kWalkUnroll = 8;               // independent gathers per lane and step
 constexpr int kSolverGrid = 1024;            // workgroups of the solver kernels (partials per reduction)
 
 template <typename S>
@@ -91,10 +95,11 @@ inline int solver_grid_units(int64_t units)
     return static_cast<int>(g < kSolverGrid ? g : kSolverGrid);
 }
 
-constexpr int kMaxLoads = kStreamCap / kWave;  // entries per lane of a staged chunk
+constexpr int kStageLoads = kStreamCap / kWave;  // staging loads per lane and window
 
 // Entry sources of chunk_row_sums: value(k) loads the matrix entry, gather(c)
-// the vector entry of column c, mul forms the product in the accumulation type.
+// the vector entry of column c, madd(a, g, acc) accumulates a * g in the
+// accumulation type.
 template <typename S>
 struct SpmvSource
 {
@@ -108,7 +113,7 @@ struct SpmvSource
     {
         return x[c];
     }
-    __device__ __forceinline__ S mul(S a, S b) const { return a * b; }
+    __device__ __forceinline__ S madd(S a, S g, S acc) const { return fma_s(a, g, acc); }
 };
 
 // Entries of A times x accumulated in FP64 (x of type X).
@@ -125,18 +130,20 @@ struct Fp64Source
     {
         return x[c];
     }
-    __device__ __forceinline__ double mul(S a, X b) const { return static_cast<double>(a) * static_cast<double>(b); }
+    __device__ __forceinline__ double madd(S a, X g, double acc) const
+    {
+        return __builtin_fma(static_cast<double>(a), static_cast<double>(g), acc);
+    }
 };
 
 // Row sums of the 64-row chunk starting at row0 (lane i -> row row0 + i).
-// s_out / e_out receive the lane's entry range; rows at or beyond `rows` get
-// an empty range.  Staged path: every value / column load of the chunk is
-// issued before any gather, and every gather before any product, so a wave has
-// up to 2 x 16 independent loads in flight per phase.
+// lv / lc are this wavefront's kStreamCap-entry LDS staging arrays.  s_out /
+// e_out receive the lane's entry range; rows at or beyond `rows` get an empty
+// range.  Each row is accumulated left to right.
 template <typename Acc, typename I, typename Src>
 __device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const I* __restrict__ ro,
-                                              const I* __restrict__ ci, Acc* lds, const Src& src, int64_t& s_out,
-                                              int64_t& e_out)
+                                              const I* __restrict__ ci, typename Src::V* lv, I* lc,
+                                              const Src& src, int64_t& s_out, int64_t& e_out)
 {
     const int lane = threadIdx.x % kWave;
     const int64_t last = row0 + kChunkRows < rows ? row0 + kChunkRows : rows;
@@ -149,46 +156,47 @@ __device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const 
     s_out = my_s;
     e_out = my_e;
     Acc acc = Acc(0);
-    if (total <= kStreamCap)
+    if (total <= static_cast<int64_t>(kMaxWindows) * kStreamCap)
     {
-        const int n_loads = static_cast<int>((total + kWave - 1) / kWave);
-        typename Src::V vv[kMaxLoads];
-        I cc[kMaxLoads];
-#pragma unroll
-        for (int j = 0; j < kMaxLoads; ++j)
+        const int s = static_cast<int>(my_s - base);
+        const int e = static_cast<int>(my_e - base);
+        const int n = static_cast<int>(total);
+        for (int w0 = 0; w0 < n; w0 += kStreamCap)
         {
-            if (j < n_loads)
+#pragma unroll
+            for (int j = 0; j < kStageLoads; ++j)
             {
-                const int64_t k = lane + static_cast<int64_t>(j) * kWave;
-                if (k < total)
+                const int k = lane + j * kWave;
+                if (w0 + k < n)
                 {
-                    vv[j] = src.value(base + k);
-                    cc[j] = ci[base + k];
+                    lv[k] = src.value(base + w0 + k);
+                    lc[k] = ci[base + w0 + k];
                 }
             }
-        }
-        typename Src::G gg[kMaxLoads];
-#pragma unroll
-        for (int j = 0; j < kMaxLoads; ++j)
-        {
-            if (j < n_loads)
+            wave_lds_sync();
+            const int lo = (s > w0 ? s : w0) - w0;
+            const int hi = (e < w0 + kStreamCap ? e : w0 + kStreamCap) - w0;
+            for (int k = lo; k < hi; k += kWalkUnroll)
             {
-                const int64_t k = lane + static_cast<int64_t>(j) * kWave;
-                if (k < total) gg[j] = src.gather(cc[j]);
-            }
-        }
+                typename Src::V a[kWalkUnroll];
+                typename Src::G g[kWalkUnroll];
 #pragma unroll
-        for (int j = 0; j < kMaxLoads; ++j)
-        {
-            if (j < n_loads)
-            {
-                const int64_t k = lane + static_cast<int64_t>(j) * kWave;
-                if (k < total) lds[k] = src.mul(vv[j], gg[j]);
+                for (int j = 0; j < kWalkUnroll; ++j)
+                {
+                    if (k + j < hi)
+                    {
+                        a[j] = lv[k + j];
+                        g[j] = src.gather(lc[k + j]);
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kWalkUnroll; ++j)
+                {
+                    if (k + j < hi) acc = src.madd(a[j], g[j], acc);
+                }
             }
+            wave_lds_sync();
         }
-        wave_lds_sync();
-        for (int64_t k = my_s - base; k < my_e - base; ++k) acc += lds[k];
-        wave_lds_sync();
     }
     else
     {
@@ -201,7 +209,7 @@ __device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const 
             const int64_t s = __shfl(my_s, row_lane, kWave);
             const int64_t e = __shfl(my_e, row_lane, kWave);
             Acc part = Acc(0);
-            for (int64_t k = s + sub; k < e; k += 8) part += src.mul(src.value(k), src.gather(ci[k]));
+            for (int64_t k = s + sub; k < e; k += 8) part = src.madd(src.value(k), src.gather(ci[k]), part);
             part = group_sum<8>(part);
             const Acc got = __shfl(part, (lane % 8) * 8, kWave);
             if (lane / 8 == pass) acc = got;
@@ -256,8 +264,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t rows, const I* __restri
                                                  const S* __restrict__ v, const S* __restrict__ x,
                                                  S* __restrict__ y)
 {
-    __shared__ S lds_all[kWavesPerBlock][kStreamCap];
-    S* lds = lds_all[threadIdx.x / kWave];
+    __shared__ S lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     int64_t begin, end;
@@ -266,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t rows, const I* __restri
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lds, SpmvSource<S>{v, x}, s, e);
+        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, x}, s, e);
         if (row0 + lane < rows) y[row0 + lane] = acc;
     }
 }
@@ -307,8 +315,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t rows, const I* __re
                                                      SolverState* st)
 {
     if (st->done != kRunning) return;
-    __shared__ double lds_all[kWavesPerBlock][kStreamCap];
-    double* lds = lds_all[threadIdx.x / kWave];
+    __shared__ S lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     int64_t begin, end;
@@ -317,7 +325,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t rows, const I* __re
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const double acc = chunk_row_sums<double>(row0, rows, ro, ci, lds, Fp64Source<S, S>{v, x}, s, e);
+        const double acc =
+            chunk_row_sums<double>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], Fp64Source<S, S>{v, x}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
         {
@@ -355,8 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_relres(int64_t rows, const I* __rest
                                                    const V* __restrict__ x, double* partials, unsigned* counter,
                                                    double* out)
 {
-    __shared__ double lds_all[kWavesPerBlock][kStreamCap];
-    double* lds = lds_all[threadIdx.x / kWave];
+    __shared__ S lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     int64_t begin, end;
@@ -366,7 +375,8 @@ __global__ __launch_bounds__(kBlock) void k_relres(int64_t rows, const I* __rest
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const double acc = chunk_row_sums<double>(row0, rows, ro, ci, lds, Fp64Source<S, V>{v, x}, s, e);
+        const double acc =
+            chunk_row_sums<double>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], Fp64Source<S, V>{v, x}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
         {
