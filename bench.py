@@ -80,6 +80,16 @@ def main() -> int:
     b = torch.from_numpy(b_host).to(dev)
     ws = capi.Workspace()
 
+    if args.profile:
+        # a plain 512 MiB device copy: the calibration point for the profiler's
+        # FETCH_SIZE / WRITE_SIZE counters (2^29 bytes read, 2^29 written)
+        src = torch.ones(1 << 27, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     # ---- SpMV: tune lanes on rotated copies, then measure HBM bandwidth ----
     copies = [a]
     for _ in range(3):

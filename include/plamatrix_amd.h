@@ -210,6 +210,35 @@ int pm_pcg_refine(const pm_csr* a, const double* rhs, double* x, const pm_solver
                   double target_relres, int32_t max_outer, const pm_solve_policy* policy,
                   pm_refine_report* report, pm_workspace* ws, hipStream_t stream);
 
+/* ---- Batched PCG ---------------------------------------------------------------
+ * Many independent SPD systems solved together (no counterpart in the
+ * reference, which solves one system per call).  The systems are the diagonal
+ * blocks of one square CSR matrix `a` (FP32 or FP64): system s owns rows
+ * [row_begin[s], row_end[s]) (host arrays), every row_begin a multiple of 64,
+ * ranges ascending and disjoint; columns of system s point into its own rows.
+ * Rows outside every range are ignored.  Each system stops on its own
+ * tolerance max(abs, rel * ||r0||), iteration limit or breakdown, with the
+ * rules of the single-system solver; the call returns when all have stopped.
+ * options->use_jacobi selects Jacobi-PCG or CG; policy->refresh_interval
+ * recomputes r = b - A x with FP64 accumulation.  Synchronises `stream` at the
+ * end to fill the host reports. */
+typedef struct pm_batch_report {
+    int32_t converged;
+    int32_t iterations;
+    double initial_residual;
+    double final_residual;
+    int32_t status;   /* 1 converged, 2 iteration limit, 3 breakdown, 4 non-finite,
+                         5 non-finite initial residual, 6 bad Jacobi diagonal */
+    int32_t pad;
+    int64_t bad_row;  /* first rejected diagonal row (status 6), else -1 */
+} pm_batch_report;
+/* Host helper: row_begin[s] for systems of the given sizes packed at 64-row
+ * boundaries, and the total row count of the block-diagonal matrix. */
+int pm_batch_layout(int64_t systems, const int64_t* sizes, int64_t* row_begin, int64_t* total_rows);
+int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                 const void* rhs, void* x, const pm_solver_options* options, const pm_solve_policy* policy,
+                 pm_batch_report* reports, pm_workspace* ws, hipStream_t stream);
+
 /* ---- Damped Schur complement (FP64) --------------------------------------------
  * Block normal equations in HBM, the device form of BlockNormalEquations<double>
  * (block_schur.h:197-319): B (pc blocks ps x ps), C (ec blocks es x es),
@@ -306,7 +335,14 @@ int pm_schur_solve(pm_schur_workspace* ws, const pm_normal_equations* equations,
                    const pm_schur_options* options, double* primary_step, double* eliminated_step,
                    pm_schur_report* report, pm_workspace* solver_ws, hipStream_t stream);
 
-/* The reduced system of the last solve: scalar CSR (FP64, int64 and int32
+/* Assembly only (assembleReducedSchurCsr, block_schur_sparse_assembly.h:
+ * 679-911): pattern (built or reused), damping, eliminated inverses, reduced
+ * rhs, Schur values and inverted diagonal blocks, read with pm_schur_system.
+ * report->status 1 / 2 as for pm_schur_solve.  Synchronises `stream`. */
+int pm_schur_assemble(pm_schur_workspace* ws, const pm_normal_equations* equations, double damping,
+                      double minimum_diagonal, pm_schur_report* report, hipStream_t stream);
+
+/* The reduced system of the last assembly or solve: scalar CSR (FP64, int64 and int32
  * indices), right-hand side and inverted diagonal blocks (device pointers
  * owned by the workspace). */
 int pm_schur_system(const pm_schur_workspace* ws, pm_csr* reduced, const double** rhs,

@@ -54,6 +54,11 @@ class pm_refine_report(ctypes.Structure):
                 ("initial_relres", _f64), ("final_relres", _f64)]
 
 
+class pm_batch_report(ctypes.Structure):
+    _fields_ = [("converged", _i32), ("iterations", _i32), ("initial_residual", _f64), ("final_residual", _f64),
+                ("status", _i32), ("pad", _i32), ("bad_row", _i64)]
+
+
 class pm_normal_equations(ctypes.Structure):
     _fields_ = [("primary_count", _i64), ("eliminated_count", _i64), ("primary_size", _i64),
                 ("eliminated_size", _i64), ("primary_diagonal", _p), ("eliminated_diagonal", _p),
@@ -89,7 +94,8 @@ EXPORTS = [
     "pm_workspace_check_status", "pm_spmv", "pm_spmv_lanes", "pm_autotune_spmv", "pm_tuned_lanes", "pm_tune_record", "pm_true_relres", "pm_cg", "pm_pcg",
     "pm_block_pcg", "pm_solve_fixed_async", "pm_fixed_finalize", "pm_fixed_state_destroy", "pm_pcg_refine",
     "pm_schur_workspace_create", "pm_schur_workspace_destroy", "pm_schur_pattern_build_count",
-    "pm_schur_pattern_size", "pm_normal_equations_assemble", "pm_schur_solve", "pm_schur_system",
+    "pm_schur_pattern_size", "pm_normal_equations_assemble", "pm_schur_solve", "pm_schur_assemble", "pm_schur_system",
+    "pm_batch_layout", "pm_batch_pcg",
 ]
 
 _lib = None
@@ -501,6 +507,15 @@ def schur_solve(sw: SchurWorkspace, ne: NormalEquations, damping, primary_step, 
                        r.message.decode())
 
 
+def schur_assemble(sw: SchurWorkspace, ne: NormalEquations, damping, minimum_diagonal=1e-12) -> SchurReport:
+    r = pm_schur_report()
+    check(lib().pm_schur_assemble(sw.handle, ctypes.byref(ne.desc), _f64(damping), _f64(minimum_diagonal),
+                                  ctypes.byref(r), current_stream()))
+    return SchurReport(bool(r.converged), r.iterations, r.initial_residual_norm, r.final_residual_norm, r.status,
+                       bool(r.pattern_reused), bool(r.assembly_on_device), r.pattern_build_count,
+                       r.message.decode())
+
+
 def schur_system(sw: SchurWorkspace):
     """(pm_csr descriptor, rhs pointer, preconditioner pointer) of the last solve."""
     a = pm_csr()
@@ -520,3 +535,45 @@ def device_array(ptr: int, n: int, dtype="float64"):
     if n > 0:
         check(lib_memcpy(t.data_ptr(), ptr, n * t.element_size()))
     return t[:n].cpu().numpy().astype(np.dtype(dtype))
+
+
+# ---- batched PCG -----------------------------------------------------------------
+
+
+def batch_layout(sizes):
+    """Row starts (64-aligned) and total rows for systems of the given sizes."""
+    import numpy as np
+
+    sizes = np.ascontiguousarray(sizes, dtype=np.int64)
+    begin = np.zeros(len(sizes), dtype=np.int64)
+    total = _i64()
+    check(lib().pm_batch_layout(_i64(len(sizes)), sizes.ctypes.data_as(_p), begin.ctypes.data_as(_p),
+                                ctypes.byref(total)))
+    return begin, total.value
+
+
+@dataclass
+class BatchReport:
+    converged: bool
+    iterations: int
+    initial_residual: float
+    final_residual: float
+    status: int
+    bad_row: int
+
+
+def batch_pcg(a: "DeviceCSR", row_begin, row_end, b, x, ws: Workspace, pol=None, **opts):
+    import numpy as np
+
+    d = a.descriptor()
+    o = options(**opts)
+    p = pol if pol is not None else policy()
+    row_begin = np.ascontiguousarray(row_begin, dtype=np.int64)
+    row_end = np.ascontiguousarray(row_end, dtype=np.int64)
+    systems = len(row_begin)
+    reps = (pm_batch_report * max(systems, 1))()
+    check(lib().pm_batch_pcg(ctypes.byref(d), _i64(systems), row_begin.ctypes.data_as(_p),
+                             row_end.ctypes.data_as(_p), _ptr(b), _ptr(x), ctypes.byref(o), ctypes.byref(p), reps,
+                             ws.handle, current_stream()))
+    return [BatchReport(bool(r.converged), r.iterations, r.initial_residual, r.final_residual, r.status, r.bad_row)
+            for r in reps[:systems]]

@@ -1,0 +1,662 @@
+// Batched Jacobi-PCG / CG over many independent SPD systems in one set of
+// launches (no counterpart in the reference, whose GPU solver handles one
+// system per call with a host convergence check per iteration,
+// iterative_solver.cu:235-402).
+//
+// Layout: the systems are the diagonal blocks of one CSR matrix.  System s
+// owns rows [row_begin[s], row_end[s]); every row_begin is a multiple of 64,
+// so each 64-row wavefront chunk belongs to exactly one system (rows between
+// one system's end and the next start are padding and are never touched).
+// Every kernel walks all chunks of all systems grid-stride, so a batch of
+// 512 uneven systems fills the chip the way one large system does.
+//
+// Per iteration, graph-replayed in chunks of iterations:
+//   direction  p = z + beta_s p, q = A p, per-chunk FP64 partial p.q
+//   alpha      one wavefront per system sums its chunk partials in a fixed
+//              order: alpha_s = rho_s / (p.q)_s (breakdown rules of alphaKernel,
+//              iterative_solver_kernels.cuh:169-202)
+//   update     x += alpha p, r -= alpha q, z = M^-1 r, partial r.r and r.z
+//   beta       per system: convergence test against max(abs, rel ||r0||),
+//              iteration limit, beta rules of betaKernel (:204-238)
+// Every refresh_interval iterations the update is replaced by x += alpha p and
+// r = b - A x accumulated in FP64.  Each system carries its own state and
+// stop flag on the device; a system that has stopped costs one flag read per
+// chunk.  The host only polls the count of stopped systems once per chunk of
+// iterations, with two chunks in flight.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "plan_cache.hpp"
+#include "spmv_kernels.hpp"
+#include "tuning.hpp"
+
+namespace pm
+{
+
+constexpr int kBatchWaves = kWavesPerBlock;  // 4 wavefronts per 256-thread workgroup
+constexpr int kBatchGrid = 2048;
+
+struct BatchState
+{
+    double rho;
+    double alpha;
+    double beta;
+    double rr;
+    double rr0;
+    double tol;
+    int32_t iteration;
+    int32_t done;
+    unsigned long long bad_row;
+};
+
+__global__ __launch_bounds__(kBlock) void k_batch_state_init(int64_t systems, BatchState* st,
+                                                             unsigned long long* done_count)
+{
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < systems;
+         s += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    {
+        BatchState b{};
+        b.bad_row = ~0ull;
+        b.done = kRunning;
+        st[s] = b;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *done_count = 0ull;
+}
+
+// r = b - A x (FP64 accumulation), z = M^-1 r (JACOBI: inverse diagonal,
+// rejected like jacobiInverse, iterative_solver_cpu.cpp:127-160), INIT: p = 0;
+// per-chunk partial r.r and r.z.  Stopped systems are skipped.
+template <typename S, typename I, bool JACOBI, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_batch_residual(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                           const int64_t* __restrict__ sys_end,
+                                                           const I* __restrict__ ro, const I* __restrict__ ci,
+                                                           const S* __restrict__ v, const S* __restrict__ b,
+                                                           const S* __restrict__ x, S* __restrict__ r,
+                                                           S* __restrict__ inv, S* __restrict__ z,
+                                                           S* __restrict__ p, BatchState* st,
+                                                           double* __restrict__ prr, double* __restrict__ prz)
+{
+    __shared__ S lv_all[kBatchWaves][kStreamCap];
+    __shared__ I lc_all[kBatchWaves][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning)
+        {
+            if (lane == 0) prr[c] = prz[c] = 0.0;
+            continue;
+        }
+        const int64_t end = sys_end[s];
+        const int64_t row0 = c * kChunkRows;
+        int64_t ks, ke;
+        const double acc =
+            chunk_row_sums<double>(row0, end, ro, ci, lv_all[wave], lc_all[wave], Fp64Source<S, S>{v, x}, ks, ke);
+        const int64_t row = row0 + lane;
+        double rr = 0.0, rz = 0.0;
+        if (row < end)
+        {
+            const S ri = static_cast<S>(static_cast<double>(b[row]) - acc);
+            r[row] = ri;
+            if (INIT) p[row] = S(0);
+            S zi = ri;
+            if (JACOBI)
+            {
+                S m;
+                if (INIT)
+                {
+                    double diag = 0.0, scale = 0.0;
+                    bool found = false;
+                    for (int64_t k = ks; k < ke; ++k)
+                    {
+                        const double a = static_cast<double>(v[k]);
+                        scale = fmax(scale, fabs(a));
+                        if (static_cast<int64_t>(ci[k]) == row)
+                        {
+                            diag += a;
+                            found = true;
+                        }
+                    }
+                    const double threshold = static_cast<double>(scalar_epsilon<S>()) * scale * 16.0;
+                    const double reciprocal = 1.0 / diag;
+                    const bool ok = found && isfinite(diag) && diag > threshold && isfinite(reciprocal);
+                    m = ok ? static_cast<S>(reciprocal) : static_cast<S>(NAN);
+                    inv[row] = m;
+                    if (!ok) atomicMin(&st[s].bad_row, static_cast<unsigned long long>(row));
+                }
+                else
+                {
+                    m = inv[row];
+                }
+                zi = m * ri;
+            }
+            z[row] = zi;
+            rr = static_cast<double>(ri) * static_cast<double>(ri);
+            rz = static_cast<double>(ri) * static_cast<double>(zi);
+        }
+        rr = wave_sum(rr);
+        rz = wave_sum(rz);
+        if (lane == 0)
+        {
+            prr[c] = rr;
+            prz[c] = rz;
+        }
+    }
+}
+
+// p_new = z + beta p_old, q = A p_new, per-chunk FP64 partial p_new.q.
+template <typename S, typename I>
+__global__ __launch_bounds__(kBlock) void k_batch_direction(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                            const int64_t* __restrict__ sys_end,
+                                                            const I* __restrict__ ro, const I* __restrict__ ci,
+                                                            const S* __restrict__ v, const S* __restrict__ z,
+                                                            const S* __restrict__ p_old, S* __restrict__ p_new,
+                                                            S* __restrict__ q, const BatchState* st,
+                                                            double* __restrict__ pden)
+{
+    __shared__ S lv_all[kBatchWaves][kStreamCap];
+    __shared__ I lc_all[kBatchWaves][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning)
+        {
+            if (lane == 0) pden[c] = 0.0;
+            continue;
+        }
+        const int64_t end = sys_end[s];
+        const int64_t row0 = c * kChunkRows;
+        const S beta = static_cast<S>(st[s].beta);
+        int64_t ks, ke;
+        const S acc = chunk_row_sums<S>(row0, end, ro, ci, lv_all[wave], lc_all[wave],
+                                        DirectionSource<S>{v, z, p_old, beta}, ks, ke);
+        const int64_t row = row0 + lane;
+        double den = 0.0;
+        if (row < end)
+        {
+            const S pn = fma_s(beta, p_old[row], z[row]);
+            p_new[row] = pn;
+            q[row] = acc;
+            den = static_cast<double>(pn) * static_cast<double>(acc);
+        }
+        den = wave_sum(den);
+        if (lane == 0) pden[c] = den;
+    }
+}
+
+// x += alpha p; STEP: r -= alpha q, z = M^-1 r, partial r.r and r.z.
+template <typename S, bool JACOBI, bool STEP>
+__global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                         const int64_t* __restrict__ sys_end, S* __restrict__ x,
+                                                         S* __restrict__ r, S* __restrict__ z,
+                                                         const S* __restrict__ p, const S* __restrict__ q,
+                                                         const S* __restrict__ inv, const BatchState* st,
+                                                         double* __restrict__ prr, double* __restrict__ prz)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning)
+        {
+            if (STEP && lane == 0) prr[c] = prz[c] = 0.0;
+            continue;
+        }
+        const S alpha = static_cast<S>(st[s].alpha);
+        const int64_t row = c * kChunkRows + lane;
+        double rr = 0.0, rz = 0.0;
+        if (row < sys_end[s])
+        {
+            x[row] = fma_s(alpha, p[row], x[row]);
+            if (STEP)
+            {
+                const S ri = fma_s(-alpha, q[row], r[row]);
+                r[row] = ri;
+                const S zi = JACOBI ? inv[row] * ri : ri;
+                z[row] = zi;
+                rr = static_cast<double>(ri) * static_cast<double>(ri);
+                rz = static_cast<double>(ri) * static_cast<double>(zi);
+            }
+        }
+        if (STEP)
+        {
+            rr = wave_sum(rr);
+            rz = wave_sum(rz);
+            if (lane == 0)
+            {
+                prr[c] = rr;
+                prz[c] = rz;
+            }
+        }
+    }
+}
+
+constexpr int kReduceInit = 0;
+constexpr int kReduceAlpha = 1;
+constexpr int kReduceBeta = 2;
+
+// One wavefront per system: fixed-order sum of its chunk partials, then the
+// scalar step of the recurrence for that system.
+template <typename S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_batch_reduce(int64_t systems, const int64_t* __restrict__ sys_chunk0,
+                                                         const int64_t* __restrict__ sys_chunk1, BatchState* st,
+                                                         const double* __restrict__ pa,
+                                                         const double* __restrict__ pb, int32_t max_iterations,
+                                                         double rel_tol, double abs_tol,
+                                                         unsigned long long* done_count)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; s < systems;
+         s += waves)
+    {
+        if (st[s].done != kRunning) continue;
+        double a = 0.0, b = 0.0;
+        for (int64_t c = sys_chunk0[s] + lane; c < sys_chunk1[s]; c += kWave)
+        {
+            a += pa[c];
+            if (MODE != kReduceAlpha) b += pb[c];
+        }
+        a = wave_sum(a);
+        if (MODE != kReduceAlpha) b = wave_sum(b);
+        if (lane != 0) continue;
+        BatchState w = st[s];
+        int code = kRunning;
+        if (MODE == kReduceInit)
+        {
+            const double rr = a, rz = b;
+            w.rr = rr;
+            w.rr0 = rr;
+            w.tol = fmax(abs_tol, rel_tol * sqrt(rr));
+            w.rho = rz;
+            w.beta = 0.0;
+            w.iteration = 0;
+            if (!isfinite(rr) || rr < 0.0) code = kInitNonFinite;
+            else if (sqrt(rr) <= w.tol) code = kConverged;
+            else if (w.bad_row != ~0ull) code = kBadDiagonal;
+            else if (max_iterations <= 0) code = kIterationLimit;
+        }
+        else if (MODE == kReduceAlpha)
+        {
+            double alpha = 0.0;
+            if (!step_length<S>(w.rho, a, alpha)) code = kBreakdown;
+            w.alpha = alpha;
+        }
+        else
+        {
+            const double rr = a, rz = b;
+            w.iteration += 1;
+            w.rr = rr;
+            if (!isfinite(rr) || rr < 0.0) code = kNonFinite;
+            else if (sqrt(rr) <= w.tol) code = kConverged;
+            else if (w.iteration >= max_iterations) code = kIterationLimit;
+            else
+            {
+                const double rho_prev = w.rho;
+                if (rho_prev == 0.0 && rz == 0.0) w.beta = 0.0;
+                else if (isfinite(rho_prev) && isfinite(rz) && rho_prev > 0.0 && rz > 0.0)
+                {
+                    w.beta = rz / rho_prev;
+                    if (!isfinite(w.beta) || !isfinite(static_cast<S>(w.beta))) code = kBreakdown;
+                }
+                else code = kBreakdown;
+                w.rho = rz;
+            }
+        }
+        if (code != kRunning)
+        {
+            w.done = code;
+            atomicAdd(done_count, 1ull);
+        }
+        st[s] = w;
+    }
+}
+
+// ---- host side -----------------------------------------------------------------------
+
+struct BatchProblem
+{
+    const pm_csr* a;
+    const void* rhs;
+    void* x;
+    bool jacobi;
+    int64_t systems;
+    int64_t chunks;
+    int32_t max_iterations;
+    double rel_tol, abs_tol;
+    int grid_chunks, grid_systems;
+    void *r, *z, *q, *p0, *p1, *inv;
+    int32_t* chunk_sys;
+    int64_t* sys_end;
+    int64_t* sys_chunk0;
+    int64_t* sys_chunk1;
+    BatchState* st;
+    double *pden, *prr, *prz;
+    unsigned long long* done_count;
+};
+
+template <typename S, typename I>
+struct BatchCalls
+{
+    static const I* ro(const pm_csr* a)
+    {
+        if constexpr (sizeof(I) == 4) return a->row_offsets32;
+        else return a->row_offsets;
+    }
+    static const I* ci(const pm_csr* a)
+    {
+        if constexpr (sizeof(I) == 4) return a->col_indices32;
+        else return a->col_indices;
+    }
+
+    template <int MODE>
+    static KernelCall reduce(const BatchProblem& b, const double* pa, const double* pb)
+    {
+        return make_call(k_batch_reduce<S, MODE>, dim3(b.grid_systems), dim3(kBlock), b.systems, b.sys_chunk0,
+                         b.sys_chunk1, b.st, pa, pb, b.max_iterations, b.rel_tol, b.abs_tol, b.done_count);
+    }
+
+    template <bool INIT>
+    static KernelCall residual(const BatchProblem& b, void* p)
+    {
+        const auto k = b.jacobi ? k_batch_residual<S, I, true, INIT> : k_batch_residual<S, I, false, INIT>;
+        return make_call(k, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys, b.sys_end, ro(b.a), ci(b.a),
+                         static_cast<const S*>(b.a->values), static_cast<const S*>(b.rhs),
+                         static_cast<const S*>(b.x), static_cast<S*>(b.r), static_cast<S*>(b.inv),
+                         static_cast<S*>(b.z), static_cast<S*>(p), b.st, b.prr, b.prz);
+    }
+
+    static void init(const BatchProblem& b, std::vector<KernelCall>& out)
+    {
+        out.push_back(residual<true>(b, b.p0));
+        out.push_back(reduce<kReduceInit>(b, b.prr, b.prz));
+    }
+
+    static void iteration(const BatchProblem& b, int64_t t, bool refresh, std::vector<KernelCall>& out)
+    {
+        const bool even = t % 2 == 0;
+        const void* p_old = even ? b.p0 : b.p1;
+        void* p_new = even ? b.p1 : b.p0;
+        out.push_back(make_call(k_batch_direction<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys,
+                                b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
+                                static_cast<const S*>(b.z), static_cast<const S*>(p_old), static_cast<S*>(p_new),
+                                static_cast<S*>(b.q), static_cast<const BatchState*>(b.st), b.pden));
+        out.push_back(reduce<kReduceAlpha>(b, b.pden, nullptr));
+        if (!refresh)
+        {
+            const auto k = b.jacobi ? k_batch_update<S, true, true> : k_batch_update<S, false, true>;
+            out.push_back(make_call(k, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys, b.sys_end,
+                                    static_cast<S*>(b.x), static_cast<S*>(b.r), static_cast<S*>(b.z),
+                                    static_cast<const S*>(p_new), static_cast<const S*>(b.q),
+                                    static_cast<const S*>(b.inv), static_cast<const BatchState*>(b.st), b.prr,
+                                    b.prz));
+        }
+        else
+        {
+            out.push_back(make_call(k_batch_update<S, false, false>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
+                                    b.chunk_sys, b.sys_end, static_cast<S*>(b.x), static_cast<S*>(b.r),
+                                    static_cast<S*>(b.z), static_cast<const S*>(p_new), static_cast<const S*>(b.q),
+                                    static_cast<const S*>(b.inv), static_cast<const BatchState*>(b.st), b.prr,
+                                    b.prz));
+            out.push_back(residual<false>(b, nullptr));
+        }
+        out.push_back(reduce<kReduceBeta>(b, b.prr, b.prz));
+    }
+
+    static void build(const BatchProblem& b, int what, int64_t first, int64_t count, int refresh_interval,
+                      std::vector<KernelCall>& out)
+    {
+        if (what == 0)
+        {
+            init(b, out);
+            return;
+        }
+        for (int64_t t = first; t < first + count; ++t)
+            iteration(b, t, refresh_interval > 0 && ((t + 1) % refresh_interval) == 0, out);
+    }
+};
+
+struct BatchPolicy
+{
+    int chunk = 16;
+    bool graph = true;
+    int refresh = 0;
+};
+
+BatchPolicy resolve_batch_policy(const pm_solve_policy* p)
+{
+    BatchPolicy out;
+    if (p)
+    {
+        if (p->chunk_iterations > 0) out.chunk = p->chunk_iterations;
+        if (p->use_graph >= 0) out.graph = p->use_graph != 0;
+        out.refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
+    }
+    if (out.chunk % 2) out.chunk += 1;
+    if (out.refresh > 0)
+    {
+        if (out.refresh % 2) out.refresh += 1;
+        int k = std::min(out.chunk, out.refresh);
+        while (k > 2 && (out.refresh % k != 0 || k % 2 != 0)) --k;
+        out.chunk = std::max(k, 2);
+    }
+    return out;
+}
+
+std::string batch_key(const BatchProblem& b, int what, int64_t first, int64_t count, int refresh)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof(buf),
+                  "batch|%d|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%lld|%lld|%d|%d|%d|%.17g|%.17g", what, b.r,
+                  static_cast<void*>(b.st), static_cast<void*>(b.pden), b.a->values, (const void*)b.a->row_offsets32,
+                  (const void*)b.a->row_offsets, (const void*)b.a->col_indices32, (const void*)b.a->col_indices,
+                  b.rhs, b.x,
+                  static_cast<long long>(b.a->rows), static_cast<long long>(b.systems), b.jacobi ? 1 : 0,
+                  static_cast<long long>(first), static_cast<long long>(count), refresh, b.a->dtype,
+                  b.max_iterations, b.rel_tol, b.abs_tol);
+    return buf;
+}
+
+template <typename S>
+Plan* batch_plan(pm_workspace* ws, const BatchProblem& b, int what, int64_t first, int64_t count, int refresh)
+{
+    const std::string key = batch_key(b, what, first, count, refresh);
+    Plan* plan = ws->plans->find(key);
+    if (!plan)
+    {
+        auto fresh = std::make_unique<Plan>();
+        if (use_compact(b.a)) BatchCalls<S, int32_t>::build(b, what, first, count, refresh, fresh->calls);
+        else BatchCalls<S, int64_t>::build(b, what, first, count, refresh, fresh->calls);
+        plan = ws->plans->insert(key, std::move(fresh));
+    }
+    return plan;
+}
+
+template <typename S>
+int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* reports, pm_workspace* ws,
+                hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_batch_state_init, dim3(static_cast<int>(std::min<int64_t>(ceil_div(b.systems, kBlock), 1024))),
+                       dim3(kBlock), 0, stream, b.systems, b.st, b.done_count);
+    PM_HIP(hipGetLastError());
+    PM_TRY(batch_plan<S>(ws, b, 0, 0, 0, 0)->run(stream, false));
+    const int64_t K = policy.chunk;
+    const int64_t needed = b.max_iterations;
+    unsigned long long* hs = ws->host_batch;
+    std::deque<int64_t> inflight;
+    int64_t launched = 0, chunk_id = 0;
+    auto issue = [&]() -> int {
+        const int64_t phase = policy.refresh > 0 ? launched % policy.refresh : 0;
+        PM_TRY(batch_plan<S>(ws, b, 1, phase, K, policy.refresh)->run(stream, policy.graph));
+        const int slot = static_cast<int>(chunk_id % 2);
+        PM_HIP(hipMemcpyAsync(hs + slot, b.done_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+        PM_HIP(hipEventRecord(ws->events[slot], stream));
+        inflight.push_back(chunk_id);
+        ++chunk_id;
+        launched += K;
+        return PM_OK;
+    };
+    // the initialisation alone may stop every system
+    PM_HIP(hipMemcpyAsync(hs, b.done_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    bool finished = static_cast<int64_t>(hs[0]) >= b.systems;
+    if (!finished && needed > 0)
+    {
+        PM_TRY(issue());
+        while (true)
+        {
+            if (launched < needed && inflight.size() < 2) PM_TRY(issue());
+            const int64_t oldest = inflight.front();
+            inflight.pop_front();
+            const int slot = static_cast<int>(oldest % 2);
+            PM_HIP(hipEventSynchronize(ws->events[slot]));
+            if (static_cast<int64_t>(hs[slot]) >= b.systems) break;
+            if (inflight.empty() && launched >= needed)
+                return fail(PM_RUNTIME_ERROR, "batched solver stopped without a final state");
+        }
+    }
+    std::vector<BatchState> host(static_cast<size_t>(b.systems));
+    PM_HIP(hipMemcpyAsync(host.data(), b.st, sizeof(BatchState) * b.systems, hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    PM_HIP(hipEventRecord(ws->done_event, stream));
+    for (int64_t s = 0; s < b.systems; ++s)
+    {
+        pm_batch_report& r = reports[s];
+        r.status = host[s].done;
+        r.converged = host[s].done == kConverged ? 1 : 0;
+        r.iterations = host[s].iteration;
+        r.initial_residual = std::sqrt(host[s].rr0);
+        r.final_residual = std::sqrt(host[s].rr);
+        r.bad_row = host[s].bad_row == ~0ull ? -1 : static_cast<int64_t>(host[s].bad_row);
+    }
+    return PM_OK;
+}
+
+} // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_batch_layout(int64_t systems, const int64_t* sizes, int64_t* row_begin, int64_t* total_rows)
+{
+    if (systems < 0 || (systems > 0 && (!sizes || !row_begin)) || !total_rows)
+        return fail(PM_INVALID_ARGUMENT, "pm_batch_layout: invalid arguments");
+    int64_t cursor = 0;
+    for (int64_t s = 0; s < systems; ++s)
+    {
+        if (sizes[s] < 0) return fail(PM_INVALID_ARGUMENT, "pm_batch_layout: negative system size");
+        row_begin[s] = cursor;
+        cursor += ceil_div(sizes[s], kChunkRows) * kChunkRows;
+    }
+    *total_rows = cursor;
+    return PM_OK;
+}
+
+int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                 const void* rhs, void* x, const pm_solver_options* options, const pm_solve_policy* policy,
+                 pm_batch_report* reports, pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(check_csr(a, "batched pcg"));
+    if (!options || !ws || !reports || systems < 0 || (systems > 0 && (!row_begin || !row_end)))
+        return fail(PM_INVALID_ARGUMENT, "batched pcg: null argument");
+    if (a->rows != a->cols) return fail(PM_INVALID_ARGUMENT, "batched pcg: matrix must be square");
+    if (options->max_iterations < 0 || !std::isfinite(options->relative_tolerance) ||
+        options->relative_tolerance < 0.0 || options->relative_tolerance > 1.0 ||
+        !std::isfinite(options->absolute_tolerance) || options->absolute_tolerance < 0.0)
+        return fail(PM_INVALID_ARGUMENT, "iterative solver options are invalid");
+    if (a->rows > 0 && (!rhs || !x)) return fail(PM_INVALID_ARGUMENT, "batched pcg: null vectors");
+    if (rhs && rhs == x) return fail(PM_INVALID_ARGUMENT, "batched pcg: rhs and solution must not alias");
+    if (systems >= (int64_t(1) << 31)) return fail(PM_OVERFLOW, "batched pcg: too many systems");
+    const int64_t n = a->rows;
+    const int64_t chunks = chunk_count(n);
+    // host layout -> per-chunk system ids and per-system chunk ranges
+    std::vector<int32_t> chunk_sys(static_cast<size_t>(chunks), -1);
+    std::vector<int64_t> sys(static_cast<size_t>(3 * std::max<int64_t>(systems, 1)));
+    int64_t prev_end = 0;
+    for (int64_t s = 0; s < systems; ++s)
+    {
+        const int64_t b0 = row_begin[s], b1 = row_end[s];
+        if (b0 % kChunkRows != 0)
+            return fail(PM_INVALID_ARGUMENT, "batched pcg: system %lld starts at row %lld, not a multiple of %d",
+                        static_cast<long long>(s), static_cast<long long>(b0), kChunkRows);
+        if (b0 < prev_end || b1 < b0 || b1 > n)
+            return fail(PM_INVALID_ARGUMENT, "batched pcg: system %lld row range [%lld, %lld) is out of order",
+                        static_cast<long long>(s), static_cast<long long>(b0), static_cast<long long>(b1));
+        prev_end = ceil_div(b1, kChunkRows) * kChunkRows;
+        const int64_t c0 = b0 / kChunkRows, c1 = ceil_div(b1, kChunkRows);
+        for (int64_t c = c0; c < c1; ++c) chunk_sys[static_cast<size_t>(c)] = static_cast<int32_t>(s);
+        sys[static_cast<size_t>(s)] = b1;
+        sys[static_cast<size_t>(systems + s)] = c0;
+        sys[static_cast<size_t>(2 * systems + s)] = c1;
+    }
+    PM_TRY(ws->bind(stream));
+    PM_TRY(ws->ensure_events());
+    const size_t es = a->dtype == PM_F64 ? 8 : 4;
+    const size_t vec = ((es * static_cast<size_t>(n > 0 ? n : 1)) + 255) & ~size_t(255);
+    PM_TRY(ws->vectors.reserve(vec * 6));
+    if (n > ws->capacity_rows) ws->capacity_rows = n;
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t s1 = std::max<int64_t>(systems, 1);
+    const size_t c1 = std::max<int64_t>(chunks, 1);
+    const size_t bytes = al(sizeof(int32_t) * c1) + 3 * al(sizeof(int64_t) * s1) + al(sizeof(BatchState) * s1) +
+                         3 * al(sizeof(double) * c1) + 256;
+    PM_TRY(ws->batch.reserve(bytes));
+    if (!ws->host_batch)
+        PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_batch), 2 * sizeof(unsigned long long),
+                             hipHostMallocDefault));
+    BatchProblem b{};
+    b.a = a;
+    b.rhs = rhs;
+    b.x = x;
+    b.jacobi = options->use_jacobi != 0;
+    b.systems = systems;
+    b.chunks = chunks;
+    b.max_iterations = options->max_iterations;
+    b.rel_tol = options->relative_tolerance;
+    b.abs_tol = options->absolute_tolerance;
+    b.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(chunks, kBatchWaves), kBatchGrid)));
+    b.grid_systems = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(systems, kBatchWaves), 1024)));
+    char* vb = ws->vectors.as<char>();
+    b.r = vb;
+    b.z = vb + vec;
+    b.q = vb + 2 * vec;
+    b.p0 = vb + 3 * vec;
+    b.p1 = vb + 4 * vec;
+    b.inv = vb + 5 * vec;
+    char* p = ws->batch.as<char>();
+    b.chunk_sys = reinterpret_cast<int32_t*>(p);
+    p += al(sizeof(int32_t) * c1);
+    b.sys_end = reinterpret_cast<int64_t*>(p);
+    b.sys_chunk0 = b.sys_end + systems;
+    b.sys_chunk1 = b.sys_end + 2 * systems;
+    p += 3 * al(sizeof(int64_t) * s1);
+    b.st = reinterpret_cast<BatchState*>(p);
+    p += al(sizeof(BatchState) * s1);
+    b.pden = reinterpret_cast<double*>(p);
+    p += al(sizeof(double) * c1);
+    b.prr = reinterpret_cast<double*>(p);
+    p += al(sizeof(double) * c1);
+    b.prz = reinterpret_cast<double*>(p);
+    p += al(sizeof(double) * c1);
+    b.done_count = reinterpret_cast<unsigned long long*>(p);
+    // the layout arrays are consumed by the first kernels: copy them
+    // synchronously (they live in host vectors that go out of scope)
+    PM_HIP(hipStreamSynchronize(stream));
+    if (chunks > 0)
+        PM_HIP(hipMemcpy(b.chunk_sys, chunk_sys.data(), sizeof(int32_t) * chunks, hipMemcpyHostToDevice));
+    if (systems > 0) PM_HIP(hipMemcpy(b.sys_end, sys.data(), sizeof(int64_t) * 3 * systems, hipMemcpyHostToDevice));
+    if (systems == 0) return PM_OK;
+    const BatchPolicy pol = resolve_batch_policy(policy);
+    if (a->dtype == PM_F32) return batch_solve<float>(b, pol, reports, ws, stream);
+    return batch_solve<double>(b, pol, reports, ws, stream);
+}
+
+} // extern "C"

@@ -21,6 +21,7 @@
 // iterative_solver_cpu.cpp:127-160.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <string>
@@ -47,9 +48,9 @@ enum Mode
 };
 
 constexpr int kMaxBlock = 8;
-constexpr int kDirBlock = 1024;                 // threads of the direction / SpMV kernel
+constexpr int kDirBlock = kBlock;               // threads of the direction / SpMV kernel
 constexpr int kDirWaves = kDirBlock / kWave;
-constexpr int kDirGrid = 512;                   // at most 512 x 16 waves: one full-occupancy round
+constexpr int kDirGrid = kMaxPartials;          // 2048 x 4 waves: 8 workgroups per CU, chunks dealt round-robin
 constexpr int kPartialsDen = 0;                 // p.q partials (1 value)
 constexpr int kPartialsRes = 2 * kMaxPartials;  // r.r, r.z partials (2 values)
 
@@ -73,43 +74,9 @@ __global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel
     for (int i = 0; i < 8; ++i) st->counters[i] = 0u;
 }
 
-// alpha from rho and p.q (alphaKernel, iterative_solver_kernels.cuh:169-202);
-// the step is applied in S, so it must also be finite in S.
-template <typename S>
-__device__ __forceinline__ bool step_length(double rho, double den, double& alpha)
-{
-    alpha = 0.0;
-    if (rho == 0.0 && den == 0.0) return true;
-    if (isfinite(rho) && isfinite(den) && rho > 0.0 && den > 0.0)
-    {
-        alpha = rho / den;
-        if (isfinite(alpha) && isfinite(static_cast<S>(alpha))) return true;
-    }
-    alpha = 0.0;
-    return false;
-}
-
-// Gathered entries of p_new = z + beta p_old, formed on the fly.
-template <typename S>
-struct DirectionSource
-{
-    const S* __restrict__ v;
-    const S* __restrict__ z;
-    const S* __restrict__ p_old;
-    S beta;
-    using V = S;
-    using G = S;
-    __device__ __forceinline__ S value(int64_t k) const { return v[k]; }
-    template <typename I>
-    __device__ __forceinline__ S gather(I c) const
-    {
-        return fma_s(beta, p_old[c], z[c]);
-    }
-    __device__ __forceinline__ S madd(S a, S g, S acc) const { return fma_s(a, g, acc); }
-};
-
-// First kernel of iteration t (slot par = t % 2), kDirBlock threads.
-template <typename S, typename I>
+// First kernel of iteration t (slot par = t % 2), kDirBlock threads.  NT
+// streams the matrix with non-temporal loads.
+template <typename S, typename I, bool NT>
 __global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, const I* __restrict__ ro,
                                                            const I* __restrict__ ci, const S* __restrict__ v,
                                                            const S* __restrict__ z, const S* __restrict__ p_old,
@@ -161,15 +128,13 @@ __global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, cons
     const S b = static_cast<S>(beta);
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    int64_t begin, end;
-    block_range(chunk_count(rows), kDirWaves, begin, end);
     double den = 0.0;
-    for (int64_t c = begin + wave; c < end; c += kDirWaves)
+    PM_FOR_CHUNKS(c, chunk_count(rows), kDirWaves)
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
-                                         DirectionSource<S>{v, z, p_old, b}, s, e);
+        const S acc = chunk_row_sums<S, I, DirectionSource<S>, NT>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                                                   DirectionSource<S>{v, z, p_old, b}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
         {
@@ -337,6 +302,7 @@ struct Problem
     int pre;                 // Pre
     int grid_rows;           // workgroups of the row kernels (publish p.q)
     int grid_units;          // workgroups of the unit kernels (publish r.r, r.z)
+    bool nt;                 // non-temporal matrix loads in the iteration
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
@@ -404,7 +370,8 @@ struct Calls
         const void* p_old = par == 0 ? pb.p0 : pb.p1;
         void* p_new = par == 0 ? pb.p1 : pb.p0;
         const S* zsrc = pb.pre == kPreNone ? static_cast<const S*>(pb.r) : static_cast<const S*>(pb.z);
-        out.push_back(make_call(k_direction_spmv<S, I>, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a),
+        out.push_back(make_call(pb.nt ? k_direction_spmv<S, I, true> : k_direction_spmv<S, I, false>,
+                                dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a),
                                 ci(pb.a), static_cast<const S*>(pb.a->values), zsrc, static_cast<const S*>(p_old),
                                 static_cast<S*>(p_new), static_cast<S*>(pb.q), pb.st, pb.partials, par,
                                 pb.grid_units));
@@ -488,6 +455,8 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     const int64_t units = pb.pre == kPreBlock ? n / pb.bsize : n;
     pb.grid_rows = static_cast<int>(std::min<int64_t>(ceil_div(chunk_count(n), kDirWaves), kDirGrid));
     pb.grid_units = solver_grid_units(units);
+    const char* nt = std::getenv("PLAMATRIX_AMD_SOLVER_NT");
+    pb.nt = nt != nullptr && nt[0] == '1';
     return PM_OK;
 }
 
@@ -496,13 +465,13 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
     // workspace addresses are part of the key: a grown workspace must never
     // replay a plan that still points at its old buffers
     char buf[640];
-    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld", kind,
+    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld|%d", kind,
                   pb.r, static_cast<void*>(pb.st), static_cast<void*>(pb.partials),
                   (const void*)pb.a->row_offsets, (const void*)pb.a->row_offsets32, (const void*)pb.a->col_indices,
                   (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
                   static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
-                  static_cast<long long>(pb.bsize));
+                  static_cast<long long>(pb.bsize), pb.nt ? 1 : 0);
     return buf;
 }
 
@@ -811,10 +780,8 @@ __global__ __launch_bounds__(kBlock) void k_residual64(int64_t rows, const I* __
     __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    int64_t begin, end;
-    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
     double rr = 0.0, bb = 0.0;
-    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;

@@ -158,11 +158,13 @@ pm_workspace::~pm_workspace()
     scratch.release();
     sort.release();
     refine.release();
+    batch.release();
     for (hipEvent_t& e : events)
         if (e) (void)hipEventDestroy(e);
     if (done_event) (void)hipEventDestroy(done_event);
     if (host_state) (void)hipHostFree(host_state);
     if (host_coo) (void)hipHostFree(host_coo);
+    if (host_batch) (void)hipHostFree(host_batch);
 }
 
 extern "C" {
@@ -205,6 +207,7 @@ int pm_workspace_close(pm_workspace* ws)
     ws->scratch.release();
     ws->sort.release();
     ws->refine.release();
+    ws->batch.release();
     ws->capacity_rows = 0;
     ws->bound = false;
     ws->stream = nullptr;
@@ -216,7 +219,7 @@ size_t pm_workspace_capacity_bytes(const pm_workspace* ws)
 {
     if (!ws) return 0;
     return ws->vectors.bytes + ws->partials.bytes + ws->state.bytes + ws->scratch.bytes + ws->sort.bytes +
-           ws->refine.bytes;
+           ws->refine.bytes + ws->batch.bytes;
 }
 
 int64_t pm_workspace_capacity_rows(const pm_workspace* ws) { return ws ? ws->capacity_rows : 0; }

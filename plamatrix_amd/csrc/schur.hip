@@ -1357,6 +1357,120 @@ int check_normal_equations(const pm_normal_equations* eq)
     return PM_OK;
 }
 
+// Pattern (built or reused), damping, eliminated inverses, reduced rhs, Schur
+// values and preconditioner inverses.  report->status 1 / 2 flags a block
+// that is not positive definite (the reference reports it the same way,
+// block_schur.cpp:140-146, 240-246).
+int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double damping, double minimum_diagonal,
+                   pm_schur_report* report, hipStream_t stream)
+{
+    const int64_t pc = eq->primary_count, ec = eq->eliminated_count;
+    const int ps = static_cast<int>(eq->primary_size), es = static_cast<int>(eq->eliminated_size);
+    const int64_t pd = pc * ps, nc = eq->cross_count, npc = eq->primary_cross_count;
+    PM_TRY(w->bind(stream));
+    SchurStatus* st = w->dstatus();
+    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
+
+    // pattern reuse: compare the cross-block topology with the cached copy
+    const int64_t sig[6] = {pc, ec, ps, es, nc, npc};
+    bool reuse = w->pattern_builds > 0 && std::equal(sig, sig + 6, w->sig);
+    if (reuse)
+    {
+        if (nc > 0)
+        {
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(nc)), dim3(kBlock), 0, stream, eq->cross_primary,
+                               w->cp_copy, nc, st);
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(nc)), dim3(kBlock), 0, stream, eq->cross_eliminated,
+                               w->ce_copy, nc, st);
+        }
+        if (npc > 0)
+        {
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(npc)), dim3(kBlock), 0, stream,
+                               eq->primary_cross_rows, w->pr_copy, npc, st);
+            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(npc)), dim3(kBlock), 0, stream,
+                               eq->primary_cross_cols, w->pcc_copy, npc, st);
+        }
+        PM_HIP(hipGetLastError());
+        PM_TRY(w->read_status(stream));
+        reuse = w->host_status->mismatch == 0;
+    }
+    if (!reuse && pd > 0)
+    {
+        hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
+        PM_TRY(build_pattern(w, eq, stream));
+        std::copy(sig, sig + 6, w->sig);
+    }
+    report->pattern_reused = reuse ? 1 : 0;
+    report->pattern_build_count = w->pattern_builds;
+
+    // numerics
+    Carve nz;
+    nz.take<double>(pc * ps * ps);
+    nz.take<double>(ec * es * es);
+    nz.take<double>(ec * es * es);
+    nz.take<double>(ec * es);
+    nz.take<double>(pd);
+    nz.take<double>(nc * ps * es);
+    nz.take<double>(pd > 0 ? w->values_count : 0);
+    nz.take<double>(pc * ps * ps);
+    PM_TRY(w->numerics.reserve(nz.off));
+    Carve n;
+    n.base = w->numerics.as<char>();
+    w->Bd = n.take<double>(pc * ps * ps);
+    w->Cd = n.take<double>(ec * es * es);
+    w->Cinv = n.take<double>(ec * es * es);
+    w->y = n.take<double>(ec * es);
+    w->rhs = n.take<double>(pd);
+    w->TW = n.take<double>(nc * ps * es);
+    w->values = n.take<double>(pd > 0 ? w->values_count : 0);
+    w->pinv = n.take<double>(pc * ps * ps);
+
+    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
+    if (pc > 0)
+        hipLaunchKernelGGL(k_damp, dim3(grid_for(pc * ps * ps)), dim3(kBlock), 0, stream, eq->primary_diagonal, pc,
+                           ps, damping, minimum_diagonal, w->Bd);
+    if (ec > 0)
+    {
+        hipLaunchKernelGGL(k_damp, dim3(grid_for(ec * es * es)), dim3(kBlock), 0, stream, eq->eliminated_diagonal, ec,
+                           es, damping, minimum_diagonal, w->Cd);
+        hipLaunchKernelGGL(k_eliminated_inverse, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, es, w->Cd,
+                           eq->eliminated_gradient, w->Cinv, w->y, st);
+    }
+    if (pd > 0)
+    {
+        hipLaunchKernelGGL(k_reduced_rhs, dim3(grid_for(pd)), dim3(kBlock), 0, stream, pc, ps, es,
+                           eq->primary_gradient, w->prim_offs, w->prim_cross, eq->cross_eliminated, eq->cross_values,
+                           w->y, w->rhs);
+        if (nc > 0)
+            hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
+                               eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
+        const int64_t waves_per_block = kBlock / kWave;
+        const int slot_grid =
+            static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
+        hipLaunchKernelGGL(k_assemble_slots, dim3(slot_grid), dim3(kBlock), 0, stream, w->slots, ps, es, w->pb,
+                           w->slot_key, w->slot_start, w->entries, w->term_e, w->term_left, w->term_right, w->bro,
+                           w->ro, w->Bd, eq->primary_cross_values, w->TW, eq->cross_values, w->values);
+        hipLaunchKernelGGL(k_preconditioner, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, ps, w->diag_slot, w->bro,
+                           w->ro, w->values, w->pinv, st);
+    }
+    PM_HIP(hipGetLastError());
+    PM_TRY(w->read_status(stream));
+    report->assembly_on_device = 1;
+    if (w->host_status->bad_inverse != ~0ull)
+    {
+        std::snprintf(report->message, sizeof(report->message), "eliminated block is not positive definite");
+        report->status = 1;
+        return PM_OK;
+    }
+    if (w->host_status->bad_preconditioner != ~0ull)
+    {
+        std::snprintf(report->message, sizeof(report->message), "Schur preconditioner block is not positive definite");
+        report->status = 2;
+        return PM_OK;
+    }
+    return PM_OK;
+}
+
 } // namespace pm
 
 extern "C" {
@@ -1483,114 +1597,15 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     PM_TRY(check_normal_equations(eq));
     const int64_t pc = eq->primary_count, ec = eq->eliminated_count;
     const int ps = static_cast<int>(eq->primary_size), es = static_cast<int>(eq->eliminated_size);
-    const int64_t pd = pc * ps, ed = ec * es, nc = eq->cross_count, npc = eq->primary_cross_count;
+    const int64_t pd = pc * ps, ed = ec * es;
     if ((pd > 0 && !dp) || (ed > 0 && !de) || (dp && dp == de))
         return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: output vectors must be non-null and distinct");
     if (!std::isfinite(damping) || damping < 0.0 || o->max_iterations <= 0 || !std::isfinite(o->relative_tolerance) ||
         o->relative_tolerance < 0.0 || !std::isfinite(o->absolute_tolerance) || o->absolute_tolerance < 0.0 ||
         !std::isfinite(o->minimum_diagonal) || o->minimum_diagonal <= 0.0)
         return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: invalid solver options");
-    PM_TRY(w->bind(stream));
-    SchurStatus* st = w->dstatus();
-    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
-
-    // pattern reuse: compare the cross-block topology with the cached copy
-    const int64_t sig[6] = {pc, ec, ps, es, nc, npc};
-    bool reuse = w->pattern_builds > 0 && std::equal(sig, sig + 6, w->sig);
-    if (reuse)
-    {
-        if (nc > 0)
-        {
-            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(nc)), dim3(kBlock), 0, stream, eq->cross_primary,
-                               w->cp_copy, nc, st);
-            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(nc)), dim3(kBlock), 0, stream, eq->cross_eliminated,
-                               w->ce_copy, nc, st);
-        }
-        if (npc > 0)
-        {
-            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(npc)), dim3(kBlock), 0, stream,
-                               eq->primary_cross_rows, w->pr_copy, npc, st);
-            hipLaunchKernelGGL(k_count_mismatch, dim3(grid_for(npc)), dim3(kBlock), 0, stream,
-                               eq->primary_cross_cols, w->pcc_copy, npc, st);
-        }
-        PM_HIP(hipGetLastError());
-        PM_TRY(w->read_status(stream));
-        reuse = w->host_status->mismatch == 0;
-    }
-    if (!reuse && pd > 0)
-    {
-        hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
-        PM_TRY(build_pattern(w, eq, stream));
-        std::copy(sig, sig + 6, w->sig);
-    }
-    report->pattern_reused = reuse ? 1 : 0;
-    report->pattern_build_count = w->pattern_builds;
-
-    // numerics
-    Carve nz;
-    nz.take<double>(pc * ps * ps);
-    nz.take<double>(ec * es * es);
-    nz.take<double>(ec * es * es);
-    nz.take<double>(ec * es);
-    nz.take<double>(pd);
-    nz.take<double>(nc * ps * es);
-    nz.take<double>(pd > 0 ? w->values_count : 0);
-    nz.take<double>(pc * ps * ps);
-    PM_TRY(w->numerics.reserve(nz.off));
-    Carve n;
-    n.base = w->numerics.as<char>();
-    w->Bd = n.take<double>(pc * ps * ps);
-    w->Cd = n.take<double>(ec * es * es);
-    w->Cinv = n.take<double>(ec * es * es);
-    w->y = n.take<double>(ec * es);
-    w->rhs = n.take<double>(pd);
-    w->TW = n.take<double>(nc * ps * es);
-    w->values = n.take<double>(pd > 0 ? w->values_count : 0);
-    w->pinv = n.take<double>(pc * ps * ps);
-
-    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
-    if (pc > 0)
-        hipLaunchKernelGGL(k_damp, dim3(grid_for(pc * ps * ps)), dim3(kBlock), 0, stream, eq->primary_diagonal, pc,
-                           ps, damping, o->minimum_diagonal, w->Bd);
-    if (ec > 0)
-    {
-        hipLaunchKernelGGL(k_damp, dim3(grid_for(ec * es * es)), dim3(kBlock), 0, stream, eq->eliminated_diagonal, ec,
-                           es, damping, o->minimum_diagonal, w->Cd);
-        hipLaunchKernelGGL(k_eliminated_inverse, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, es, w->Cd,
-                           eq->eliminated_gradient, w->Cinv, w->y, st);
-    }
-    if (pd > 0)
-    {
-        hipLaunchKernelGGL(k_reduced_rhs, dim3(grid_for(pd)), dim3(kBlock), 0, stream, pc, ps, es,
-                           eq->primary_gradient, w->prim_offs, w->prim_cross, eq->cross_eliminated, eq->cross_values,
-                           w->y, w->rhs);
-        if (nc > 0)
-            hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
-                               eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
-        const int64_t waves_per_block = kBlock / kWave;
-        const int slot_grid =
-            static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
-        hipLaunchKernelGGL(k_assemble_slots, dim3(slot_grid), dim3(kBlock), 0, stream, w->slots, ps, es, w->pb,
-                           w->slot_key, w->slot_start, w->entries, w->term_e, w->term_left, w->term_right, w->bro,
-                           w->ro, w->Bd, eq->primary_cross_values, w->TW, eq->cross_values, w->values);
-        hipLaunchKernelGGL(k_preconditioner, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, ps, w->diag_slot, w->bro,
-                           w->ro, w->values, w->pinv, st);
-    }
-    PM_HIP(hipGetLastError());
-    PM_TRY(w->read_status(stream));
-    report->assembly_on_device = 1;
-    if (w->host_status->bad_inverse != ~0ull)
-    {
-        std::snprintf(report->message, sizeof(report->message), "eliminated block is not positive definite");
-        report->status = 1;
-        return PM_OK;
-    }
-    if (w->host_status->bad_preconditioner != ~0ull)
-    {
-        std::snprintf(report->message, sizeof(report->message), "Schur preconditioner block is not positive definite");
-        report->status = 2;
-        return PM_OK;
-    }
+    PM_TRY(schur_assemble(w, eq, damping, o->minimum_diagonal, report, stream));
+    if (report->status != 0) return PM_OK;
     if (ed > 0) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(ed)), dim3(kBlock), 0, stream, de, ed, 0.0);
     if (pd > 0 && !o->use_initial_guess)
         hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(pd)), dim3(kBlock), 0, stream, dp, pd, 0.0);
@@ -1636,6 +1651,21 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
                            eq->eliminated_gradient, dp, de);
     PM_HIP(hipGetLastError());
     std::snprintf(report->message, sizeof(report->message), "converged");
+    return PM_OK;
+}
+
+int pm_schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double damping, double minimum_diagonal,
+                      pm_schur_report* report, hipStream_t stream)
+{
+    if (!w || !report) return fail(PM_INVALID_ARGUMENT, "pm_schur_assemble: null argument");
+    std::memset(report, 0, sizeof(*report));
+    PM_TRY(check_normal_equations(eq));
+    if (!std::isfinite(damping) || damping < 0.0 || !std::isfinite(minimum_diagonal) || minimum_diagonal <= 0.0)
+        return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: invalid solver options");
+    if (eq->primary_count * eq->primary_size == 0)
+        return fail(PM_INVALID_ARGUMENT, "pm_schur_assemble: the reduced system is empty");
+    PM_TRY(schur_assemble(w, eq, damping, minimum_diagonal, report, stream));
+    if (report->status == 0) std::snprintf(report->message, sizeof(report->message), "assembled");
     return PM_OK;
 }
 

@@ -48,7 +48,7 @@ struct SpmvRun
 template <typename S>
 int spmv_stream(const pm_csr* a, const void* x, void* y, hipStream_t stream)
 {
-    const int grid = static_cast<int>(ceil_div(chunk_count(a->rows), kWavesPerBlock));
+    const int grid = static_cast<int>(std::min<int64_t>(ceil_div(chunk_count(a->rows), kWavesPerBlock), kSpmvGrid));
     if (use_compact(a))
         hipLaunchKernelGGL((k_spmv<S, int32_t>), dim3(grid), dim3(kBlock), 0, stream, a->rows, a->row_offsets32,
                            a->col_indices32, static_cast<const S*>(a->values), static_cast<const S*>(x),

@@ -61,6 +61,14 @@ __device__ __forceinline__ double scalar_epsilon<double>()
     return 2.220446049250313080847e-16;
 }
 
+// Load of a streamed-once array element, optionally non-temporal (the line is
+// not kept in the caches).
+template <typename T>
+__device__ __forceinline__ T stream_load(const T* p, bool nt)
+{
+    return nt ? __builtin_nontemporal_load(p) : *p;
+}
+
 // LDS hand-off between lanes of one wavefront.
 __device__ __forceinline__ void wave_lds_sync()
 {
@@ -79,6 +87,16 @@ __device__ __forceinline__ void block_range(int64_t units, int64_t step, int64_t
     begin = lb * per;
     end = begin + per < units ? begin + per : units;
 }
+
+// Chunks are dealt to wavefronts round-robin over the whole grid: wave w of
+// workgroup b takes chunks b*W + w, b*W + w + grid*W, ...  Neighbouring
+// chunks run concurrently on different CUs, which keeps the HBM pages and the
+// gathered x lines of one region hot at the same time.
+#define PM_FOR_CHUNKS(c, chunks, W)                                                                  \
+    for (int64_t c = static_cast<int64_t>(blockIdx.x) * (W) + threadIdx.x / kWave; c < (chunks);       \
+         c += static_cast<int64_t>(gridDim.x) * (W))
+
+constexpr int kSpmvGrid = 2048;  // workgroups of the standalone SpMV (8 per CU)
 
 __host__ __device__ inline int64_t chunk_count(int64_t rows) { return ceil_div(rows > 0 ? rows : 1, kChunkRows); }
 
@@ -107,7 +125,7 @@ struct SpmvSource
     const S* __restrict__ x;
     using V = S;
     using G = S;
-    __device__ __forceinline__ S value(int64_t k) const { return v[k]; }
+    __device__ __forceinline__ S value(int64_t k, bool nt) const { return stream_load(v + k, nt); }
     template <typename I>
     __device__ __forceinline__ S gather(I c) const
     {
@@ -124,7 +142,7 @@ struct Fp64Source
     const X* __restrict__ x;
     using V = S;
     using G = X;
-    __device__ __forceinline__ S value(int64_t k) const { return v[k]; }
+    __device__ __forceinline__ S value(int64_t k, bool nt) const { return stream_load(v + k, nt); }
     template <typename I>
     __device__ __forceinline__ X gather(I c) const
     {
@@ -136,31 +154,61 @@ struct Fp64Source
     }
 };
 
-// Row sums of the 64-row chunk starting at row0 (lane i -> row row0 + i).
-// lv / lc are this wavefront's kStreamCap-entry LDS staging arrays.  s_out /
-// e_out receive the lane's entry range; rows at or beyond `rows` get an empty
-// range.  Each row is accumulated left to right.
-template <typename Acc, typename I, typename Src>
-__device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const I* __restrict__ ro,
-                                              const I* __restrict__ ci, typename Src::V* lv, I* lc,
-                                              const Src& src, int64_t& s_out, int64_t& e_out)
+// Gathered entries of p_new = z + beta p_old, formed on the fly.
+template <typename S>
+struct DirectionSource
+{
+    const S* __restrict__ v;
+    const S* __restrict__ z;
+    const S* __restrict__ p_old;
+    S beta;
+    using V = S;
+    using G = S;
+    __device__ __forceinline__ S value(int64_t k, bool nt) const { return stream_load(v + k, nt); }
+    template <typename I>
+    __device__ __forceinline__ S gather(I c) const
+    {
+        return fma_s(beta, p_old[c], z[c]);
+    }
+    __device__ __forceinline__ S madd(S a, S g, S acc) const { return fma_s(a, g, acc); }
+};
+
+// Value of lane `lane_id` in every lane (v_readlane: no LDS traffic).
+template <typename T>
+__device__ __forceinline__ T lane_value(T x, int lane_id)
+{
+    if constexpr (sizeof(T) == 8)
+    {
+        const unsigned long long u = static_cast<unsigned long long>(x);
+        const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u & 0xffffffffull), lane_id));
+        const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), lane_id));
+        return static_cast<T>((static_cast<unsigned long long>(hi) << 32) | lo);
+    }
+    else
+    {
+        return static_cast<T>(__builtin_amdgcn_readlane(static_cast<int>(x), lane_id));
+    }
+}
+
+// Chunks whose entries do not fit one LDS window: up to kMaxWindows windows
+// walked like the single-window path, else 8 lanes per row with a fixed
+// butterfly.  Inlined on purpose: an out-of-line call gives the kernel a
+// stack (scratch) and ~70 VGPRs, which halves the throughput of a wavefront
+// that loops over many chunks.
+template <typename Acc, typename I, typename Src, bool NT>
+__device__ __forceinline__ Acc chunk_row_sums_long(int64_t base, int64_t total, int64_t my_s, int64_t my_e,
+                                                const I* __restrict__ ci, typename Src::V* lv, I* lc,
+                                                const Src src)
 {
     const int lane = threadIdx.x % kWave;
-    const int64_t last = row0 + kChunkRows < rows ? row0 + kChunkRows : rows;
-    const int64_t r = row0 + lane;
-    const int64_t my_s = static_cast<int64_t>(ro[r < last ? r : last]);
-    int64_t my_e = __shfl_down(my_s, 1, kWave);
-    if (lane == kWave - 1) my_e = static_cast<int64_t>(ro[last]);
-    const int64_t base = __shfl(my_s, 0, kWave);
-    const int64_t total = __shfl(my_e, kWave - 1, kWave) - base;
-    s_out = my_s;
-    e_out = my_e;
     Acc acc = Acc(0);
     if (total <= static_cast<int64_t>(kMaxWindows) * kStreamCap)
     {
         const int s = static_cast<int>(my_s - base);
         const int e = static_cast<int>(my_e - base);
         const int n = static_cast<int>(total);
+        const auto* vb = src.v + base;
+        const I* cb = ci + base;
         for (int w0 = 0; w0 < n; w0 += kStreamCap)
         {
 #pragma unroll
@@ -169,52 +217,90 @@ __device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const 
                 const int k = lane + j * kWave;
                 if (w0 + k < n)
                 {
-                    lv[k] = src.value(base + w0 + k);
-                    lc[k] = ci[base + w0 + k];
+                    lv[k] = stream_load(vb + w0 + k, NT);
+                    lc[k] = stream_load(cb + w0 + k, NT);
                 }
             }
             wave_lds_sync();
             const int lo = (s > w0 ? s : w0) - w0;
             const int hi = (e < w0 + kStreamCap ? e : w0 + kStreamCap) - w0;
-            for (int k = lo; k < hi; k += kWalkUnroll)
-            {
-                typename Src::V a[kWalkUnroll];
-                typename Src::G g[kWalkUnroll];
-#pragma unroll
-                for (int j = 0; j < kWalkUnroll; ++j)
-                {
-                    if (k + j < hi)
-                    {
-                        a[j] = lv[k + j];
-                        g[j] = src.gather(lc[k + j]);
-                    }
-                }
-#pragma unroll
-                for (int j = 0; j < kWalkUnroll; ++j)
-                {
-                    if (k + j < hi) acc = src.madd(a[j], g[j], acc);
-                }
-            }
+            for (int k = lo; k < hi; ++k) acc = src.madd(lv[k], src.gather(lc[k]), acc);
             wave_lds_sync();
         }
+        return acc;
     }
-    else
+    const int group = lane / 8;
+    const int sub = lane % 8;
+    for (int pass = 0; pass < 8; ++pass)
     {
-        // long rows: 8 lanes per row, 8 rows per pass
-        const int group = lane / 8;
-        const int sub = lane % 8;
-        for (int pass = 0; pass < 8; ++pass)
+        const int row_lane = pass * 8 + group;
+        const int64_t s = __shfl(my_s, row_lane, kWave);
+        const int64_t e = __shfl(my_e, row_lane, kWave);
+        Acc part = Acc(0);
+        for (int64_t k = s + sub; k < e; k += 8) part = src.madd(stream_load(src.v + k, NT), src.gather(ci[k]), part);
+        part = group_sum<8>(part);
+        const Acc got = __shfl(part, (lane % 8) * 8, kWave);
+        if (lane / 8 == pass) acc = got;
+    }
+    return acc;
+}
+
+// Row sums of the 64-row chunk starting at row0 (lane i -> row row0 + i).
+// lv / lc are this wavefront's kStreamCap-entry LDS staging arrays.  s_out /
+// e_out receive the lane's entry range; rows at or beyond `rows` get an empty
+// range.  Each row is accumulated left to right.
+template <typename Acc, typename I, typename Src, bool NT = false>
+__device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const I* __restrict__ ro,
+                                              const I* __restrict__ ci, typename Src::V* lv, I* lc,
+                                              const Src& src, int64_t& s_out, int64_t& e_out)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t last = row0 + kChunkRows < rows ? row0 + kChunkRows : rows;
+    const int64_t r = row0 + lane;
+    const I s_i = ro[r < last ? r : last];
+    const I e_i = ro[r + 1 < last ? r + 1 : last];
+    const int64_t base = static_cast<int64_t>(lane_value(s_i, 0));
+    const int64_t total = static_cast<int64_t>(lane_value(e_i, kWave - 1)) - base;
+    s_out = static_cast<int64_t>(s_i);
+    e_out = static_cast<int64_t>(e_i);
+    if (total > kStreamCap) return chunk_row_sums_long<Acc, I, Src, NT>(base, total, s_out, e_out, ci, lv, lc, src);
+    const int n = static_cast<int>(total);
+    const int s = static_cast<int>(s_out - base);
+    const int e = static_cast<int>(e_out - base);
+    const auto* vb = src.v + base;
+    const I* cb = ci + base;
+#pragma unroll
+    for (int j = 0; j < kStageLoads; ++j)
+    {
+        const int k = lane + j * kWave;
+        if (k < n)
         {
-            const int row_lane = pass * 8 + group;
-            const int64_t s = __shfl(my_s, row_lane, kWave);
-            const int64_t e = __shfl(my_e, row_lane, kWave);
-            Acc part = Acc(0);
-            for (int64_t k = s + sub; k < e; k += 8) part = src.madd(src.value(k), src.gather(ci[k]), part);
-            part = group_sum<8>(part);
-            const Acc got = __shfl(part, (lane % 8) * 8, kWave);
-            if (lane / 8 == pass) acc = got;
+            lv[k] = stream_load(vb + k, NT);
+            lc[k] = stream_load(cb + k, NT);
         }
     }
+    wave_lds_sync();
+    Acc acc = Acc(0);
+    for (int k = s; k < e; k += kWalkUnroll)
+    {
+        typename Src::V a[kWalkUnroll];
+        typename Src::G g[kWalkUnroll];
+#pragma unroll
+        for (int j = 0; j < kWalkUnroll; ++j)
+        {
+            if (k + j < e)
+            {
+                a[j] = lv[k + j];
+                g[j] = src.gather(lc[k + j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kWalkUnroll; ++j)
+        {
+            if (k + j < e) acc = src.madd(a[j], g[j], acc);
+        }
+    }
+    wave_lds_sync();
     return acc;
 }
 
@@ -268,13 +354,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int64_t rows, const I* __restri
     __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    int64_t begin, end;
-    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
-    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, x}, s, e);
+        const S acc = chunk_row_sums<S, I, SpmvSource<S>, true>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                                              SpmvSource<S>{v, x}, s, e);
         if (row0 + lane < rows) y[row0 + lane] = acc;
     }
 }
@@ -319,9 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(int64_t rows, const I* __re
     __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    int64_t begin, end;
-    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
-    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
@@ -368,10 +451,8 @@ __global__ __launch_bounds__(kBlock) void k_relres(int64_t rows, const I* __rest
     __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    int64_t begin, end;
-    block_range(chunk_count(rows), kWavesPerBlock, begin, end);
     double rr = 0.0, bb = 0.0;
-    for (int64_t c = begin + wave; c < end; c += kWavesPerBlock)
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
@@ -393,6 +474,22 @@ __global__ __launch_bounds__(kBlock) void k_relres(int64_t rows, const I* __rest
         out[0] = total[0];
         out[1] = total[1];
     }
+}
+
+// alpha from rho and p.q (alphaKernel, iterative_solver_kernels.cuh:169-202);
+// the step is applied in S, so it must also be finite in S.
+template <typename S>
+__device__ __forceinline__ bool step_length(double rho, double den, double& alpha)
+{
+    alpha = 0.0;
+    if (rho == 0.0 && den == 0.0) return true;
+    if (isfinite(rho) && isfinite(den) && rho > 0.0 && den > 0.0)
+    {
+        alpha = rho / den;
+        if (isfinite(alpha) && isfinite(static_cast<S>(alpha))) return true;
+    }
+    alpha = 0.0;
+    return false;
 }
 
 // Dispatch helper: calls F<L>::run(...) for the runtime lanes-per-row value.

@@ -119,6 +119,7 @@ struct pm_workspace
     pm::DeviceBuffer scratch;   // miscellaneous (relres, validation)
     pm::DeviceBuffer sort;      // COO conversion buffers
     pm::DeviceBuffer refine;    // FP64 residual / FP32 correction of the refinement loop
+    pm::DeviceBuffer batch;     // batched solver: layout, per-system state, per-chunk partials
     int64_t capacity_rows = 0;
 
     pm::SolverState* host_state = nullptr;  // pinned, 2 slots
@@ -133,6 +134,7 @@ struct pm_workspace
     bool coo_prepared = false;
     bool coo_pending_status = false;
     pm::CooStatus* host_coo = nullptr;      // pinned
+    unsigned long long* host_batch = nullptr;  // pinned, 2 slots: stopped-system counts
 
     // cached solver chunk plans
     struct PlanCache;

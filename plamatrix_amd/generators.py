@@ -92,16 +92,22 @@ def diffusion3d_csr(n: int = 128, seed: int = 2, sigma: float = 1.5, z_scale: fl
     Face coefficients are log-normal ``exp(sigma * N(0,1))``; z faces are
     multiplied by ``z_scale`` (10:1 anisotropy for 0.1).  The SPD matrix is
     scaled as ``D^-1/2 A D^-1/2`` so its diagonal is exactly one."""
+    return diffusion3d_box_csr(n, n, n, seed, sigma, z_scale, dtype)
+
+
+def diffusion3d_box_csr(nx: int, ny: int, nz: int, seed: int = 2, sigma: float = 1.5, z_scale: float = 0.1,
+                        dtype=np.float32) -> CSRHost:
+    """diffusion3d_csr on an nx x ny x nz grid (x fastest)."""
     rng = np.random.default_rng(seed)
-    kx = np.exp(sigma * rng.standard_normal((n, n, n + 1)))          # [z, y, x-face]
-    ky = np.exp(sigma * rng.standard_normal((n, n + 1, n)))          # [z, y-face, x]
-    kz = np.exp(sigma * rng.standard_normal((n + 1, n, n))) * z_scale  # [z-face, y, x]
+    kx = np.exp(sigma * rng.standard_normal((nz, ny, nx + 1)))          # [z, y, x-face]
+    ky = np.exp(sigma * rng.standard_normal((nz, ny + 1, nx)))          # [z, y-face, x]
+    kz = np.exp(sigma * rng.standard_normal((nz + 1, ny, nx))) * z_scale  # [z-face, y, x]
     diag = (kx[:, :, :-1] + kx[:, :, 1:] + ky[:, :-1, :] + ky[:, 1:, :] + kz[:-1] + kz[1:]).reshape(-1)
-    N = n ** 3
+    N = nx * ny * nz
     idx = np.arange(N)
-    x, y, z = idx % n, (idx // n) % n, idx // (n * n)
-    offsets = [-n * n, -n, -1, 0, 1, n, n * n]
-    valid = np.stack([z > 0, y > 0, x > 0, np.ones(N, bool), x < n - 1, y < n - 1, z < n - 1], axis=1)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    offsets = [-nx * ny, -nx, -1, 0, 1, nx, nx * ny]
+    valid = np.stack([z > 0, y > 0, x > 0, np.ones(N, bool), x < nx - 1, y < ny - 1, z < nz - 1], axis=1)
     vals = np.empty((N, 7))
     vals[:, 0] = -kz[:-1].reshape(-1)          # face between z-1 and z
     vals[:, 1] = -ky[:, :-1, :].reshape(-1)
@@ -116,6 +122,45 @@ def diffusion3d_csr(n: int = 128, seed: int = 2, sigma: float = 1.5, z_scale: fl
     vals = vals * inv_sqrt[:, None] * inv_sqrt[cols_clipped]
     vals[:, 3] = 1.0
     return _stencil_csr(N, offsets, valid, vals, dtype)
+
+
+def batch_sizes(count: int = 512, seed: int = 4, lo: int = 1000, hi: int = 200000) -> np.ndarray:
+    """Log-uniform system sizes in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+
+
+def grid2d_dims(n: int) -> tuple:
+    nx = max(1, int(round(np.sqrt(n))))
+    return nx, max(1, int(round(n / nx)))
+
+
+def grid3d_dims(n: int) -> tuple:
+    nx = max(1, int(round(n ** (1.0 / 3.0))))
+    ny = nx
+    return nx, ny, max(1, int(round(n / (nx * ny))))
+
+
+def block_diagonal(systems: list, dtype=np.float32):
+    """Pack CSR systems as the diagonal blocks of one matrix, each starting at
+    a multiple of 64 rows (padding rows are empty).  Returns the CSR and the
+    row_begin / row_end arrays."""
+    sizes = np.array([s.rows for s in systems], dtype=np.int64)
+    begin = np.zeros(len(systems), dtype=np.int64)
+    cursor = 0
+    for i, n in enumerate(sizes):
+        begin[i] = cursor
+        cursor += -(-int(n) // 64) * 64
+    total = cursor
+    end = begin + sizes
+    counts = np.zeros(total, dtype=np.int64)
+    for s, b0 in zip(systems, begin):
+        counts[b0:b0 + s.rows] = np.diff(s.row_offsets)
+    ro = np.zeros(total + 1, dtype=np.int64)
+    np.cumsum(counts, out=ro[1:])
+    ci = np.concatenate([s.col_indices + b0 for s, b0 in zip(systems, begin)]) if systems else np.zeros(0, np.int64)
+    vals = np.concatenate([s.values.astype(dtype) for s in systems]) if systems else np.zeros(0, dtype)
+    return CSRHost(total, total, ro, ci.astype(np.int64), vals), begin, end
 
 
 def rhs_uniform(n: int, seed: int = 1, dtype=np.float64) -> np.ndarray:

@@ -1,0 +1,149 @@
+"""GPU parity of the batched PCG through the C ABI.
+
+Each system of a batch must behave as if solved alone: its solution agrees
+with the oracle's CPU PCG on that system within relative 1e-6 (FP64) / 1e-4
+(FP32), its iteration count with the single-system GPU solve, and its FP64
+true residual is recomputed; stop conditions (convergence, iteration limit,
+bad diagonal, zero right-hand side) are per system.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from plamatrix_amd import capi, generators  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    capi.lib()
+    return torch.device("cuda:0")
+
+
+@pytest.fixture
+def ws(dev):
+    w = capi.Workspace()
+    yield w
+    torch.cuda.synchronize()
+
+
+def mixed_systems(count, seed, lo=200, hi=6000, dtype=np.float64):
+    sizes = generators.batch_sizes(count, seed=seed, lo=lo, hi=hi)
+    systems = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            systems.append(generators.laplacian2d_csr(*generators.grid2d_dims(int(n)), dtype=dtype))
+        else:
+            systems.append(generators.diffusion3d_box_csr(*generators.grid3d_dims(int(n)), seed=100 + i,
+                                                          dtype=dtype))
+    return systems
+
+
+def pack(systems, dev, dtype, seed=1):
+    big, begin, end = generators.block_diagonal(systems, dtype=dtype)
+    rng = np.random.default_rng(seed)
+    b = np.zeros(big.rows, dtype=dtype)
+    for s, b0 in zip(systems, begin):
+        b[b0:b0 + s.rows] = rng.uniform(-1.0, 1.0, s.rows)
+    a = capi.to_device_csr(big, compact=True)
+    return a, begin, end, big, b
+
+
+def relres(sys, b, x):
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((sys.values.astype(np.float64), sys.col_indices, sys.row_offsets), shape=(sys.rows, sys.rows))
+    b = b.astype(np.float64)
+    return np.linalg.norm(b - A @ x.astype(np.float64)) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("dtype,rtol", [(np.float64, 1e-6), (np.float32, 1e-4)])
+def test_batch_matches_oracle_per_system(dev, ws, oracle, dtype, rtol):
+    systems = mixed_systems(12, seed=7, dtype=dtype)
+    a, begin, end, big, b = pack(systems, dev, dtype)
+    bt = torch.from_numpy(b).to(dev)
+    x = torch.zeros(big.rows, dtype=bt.dtype, device=dev)
+    tol = 1e-8 if dtype == np.float64 else 1e-5
+    reps = capi.batch_pcg(a, begin, end, bt, x, ws, relative_tolerance=tol, max_iterations=3000)
+    xs = x.cpu().numpy()
+    for s, b0, e0, rep in zip(systems, begin, end, reps):
+        assert rep.converged and rep.status == 1, rep
+        ref_sys = oracle.CSR(s.rows, s.rows, s.row_offsets, s.col_indices, s.values)
+        xr, rr = oracle.pcg(ref_sys, b[b0:e0], relative_tolerance=tol, max_iterations=3000)
+        assert rr.converged
+        got = xs[b0:e0].astype(np.float64)
+        assert np.linalg.norm(got - xr) <= rtol * np.linalg.norm(xr)
+        if dtype == np.float64:
+            assert abs(rep.iterations - rr.iterations) <= 2
+        assert relres(s, b[b0:e0], got) <= tol * 10
+
+
+def test_batch_equals_single_solves(dev, ws):
+    systems = mixed_systems(6, seed=11, dtype=np.float64)
+    a, begin, end, big, b = pack(systems, dev, np.float64)
+    bt = torch.from_numpy(b).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    reps = capi.batch_pcg(a, begin, end, bt, x, ws, relative_tolerance=1e-10, max_iterations=5000)
+    for s, b0, e0, rep in zip(systems, begin, end, reps):
+        single = capi.to_device_csr(s)
+        xs = torch.zeros(s.rows, dtype=torch.float64, device=dev)
+        r1 = capi.pcg(single, bt[b0:e0].contiguous(), xs, ws, relative_tolerance=1e-10, max_iterations=5000)
+        # same recurrence; only the dot-product partition differs
+        assert abs(rep.iterations - r1.iterations) <= 1
+        xb, x1 = x[b0:e0].cpu().numpy(), xs.cpu().numpy()
+        assert np.linalg.norm(xb - x1) <= 1e-6 * np.linalg.norm(x1)
+
+
+def test_batch_per_system_stops(dev, ws):
+    lap = generators.laplacian2d_csr(20, 20)
+    bad = generators.laplacian2d_csr(10, 10)
+    bad.values = bad.values.copy()
+    assert bad.col_indices[0] == 0  # row 0 starts with its diagonal
+    bad.values[0] = 0.0
+    systems = [lap, generators.laplacian2d_csr(30, 30), bad, generators.laplacian2d_csr(8, 8)]
+    a, begin, end, big, b = pack(systems, dev, np.float64)
+    b[begin[3]:end[3]] = 0.0  # zero right-hand side: converged at iteration 0
+    bt = torch.from_numpy(b).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    reps = capi.batch_pcg(a, begin, end, bt, x, ws, relative_tolerance=1e-10, max_iterations=25)
+    assert reps[0].status == 2 and reps[0].iterations == 25  # 20x20 needs more than 25
+    assert reps[1].status == 2
+    assert reps[2].status == 6 and reps[2].bad_row == begin[2]
+    assert reps[3].status == 1 and reps[3].iterations == 0
+    assert torch.count_nonzero(x[begin[2]:end[2]]).item() == 0  # never updated
+    reps = capi.batch_pcg(a, begin[:2], end[:2], bt, x.zero_(), ws, relative_tolerance=1e-10, max_iterations=2000)
+    assert all(r.converged for r in reps)
+
+
+def test_batch_cg_refresh_and_gaps(dev, ws):
+    systems = mixed_systems(5, seed=3, dtype=np.float32)
+    a, begin, end, big, b = pack(systems, dev, np.float32)
+    bt = torch.from_numpy(b).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float32, device=dev)
+    pol = capi.policy(refresh_interval=10, chunk_iterations=10)
+    reps = capi.batch_pcg(a, begin, end, bt, x, ws, pol=pol, relative_tolerance=1e-5, max_iterations=4000,
+                          use_jacobi=False)
+    for s, b0, e0, rep in zip(systems, begin, end, reps):
+        assert rep.converged
+        assert relres(s, b[b0:e0], x[b0:e0].cpu().numpy()) <= 1e-4
+    # a subset of the systems (others are ignored rows)
+    x.zero_()
+    reps = capi.batch_pcg(a, begin[1::2], end[1::2], bt, x, ws, relative_tolerance=1e-5, max_iterations=4000)
+    assert all(r.converged for r in reps)
+    assert torch.count_nonzero(x[begin[0]:end[0]]).item() == 0
+
+
+def test_batch_rejects_bad_layout(dev, ws):
+    systems = mixed_systems(2, seed=1)
+    a, begin, end, big, b = pack(systems, dev, np.float64)
+    bt = torch.from_numpy(b).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    with pytest.raises(capi.PlamatrixError) as e:
+        capi.batch_pcg(a, begin + 1, end, bt, x, ws)
+    assert e.value.kind == "invalid_argument"
+    with pytest.raises(capi.PlamatrixError) as e:
+        capi.batch_pcg(a, begin[::-1].copy(), end[::-1].copy(), bt, x, ws)
+    assert e.value.kind == "invalid_argument"
