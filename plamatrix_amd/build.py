@@ -2,7 +2,9 @@
 
 Every ``.hip`` file under ``plamatrix_amd/csrc`` is compiled with hipcc for
 gfx950 (packed-FP32 instructions disabled) and linked into one shared library
-exporting the C ABI declared in ``include/plamatrix_amd.h``.
+exporting the C ABI declared in ``include/plamatrix_amd.h``.  The C++ API
+test program ``tests/cpp/test_api`` (host code over the C++ headers in
+``include/plamatrix``) is linked against it.
 """
 from __future__ import annotations
 
@@ -65,7 +67,22 @@ def build(jobs: int | None = None) -> str:
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr}")
     os.replace(tmp, LIBRARY)
+    build_cpp_tests()
     return LIBRARY
+
+
+CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_api.cpp")
+CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_api")
+
+
+def build_cpp_tests() -> str:
+    """Host-only C++ program over the drop-in headers, run by a GPU test."""
+    cmd = [hipcc(), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), CPP_TEST_SRC,
+           "-L" + HERE, "-lplamatrix_amd", "-Wl,-rpath,$ORIGIN/../../plamatrix_amd", "-o", CPP_TEST_BIN]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"C++ API test build failed:\n{proc.stderr}")
+    return CPP_TEST_BIN
 
 
 if __name__ == "__main__":
