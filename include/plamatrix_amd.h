@@ -239,6 +239,57 @@ int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, con
                  const void* rhs, void* x, const pm_solver_options* options, const pm_solve_policy* policy,
                  pm_batch_report* reports, pm_workspace* ws, hipStream_t stream);
 
+/* ---- Factored preconditioner M^-1 = G^T G ----------------------------------------
+ * PCG with z = G^T (G r) (no reference counterpart).  g is lower-triangular
+ * CSR, gt its transpose; both need 32-bit index mirrors and a's dtype.  Same
+ * stop rules, reports and errors as pm_pcg. */
+int pm_pcg_factored(const pm_csr* a, const void* rhs, void* x, const pm_csr* g, const pm_csr* gt,
+                    const pm_solver_options* options, const pm_solve_policy* policy, pm_solver_report* report,
+                    pm_workspace* ws, hipStream_t stream);
+/* pm_pcg_refine with the factored preconditioner in the FP32 inner solves. */
+int pm_pcg_refine_factored(const pm_csr* a, const double* rhs, double* x, const pm_csr* g, const pm_csr* gt,
+                           const pm_solver_options* inner, double target_relres, int32_t max_outer,
+                           const pm_solve_policy* policy, pm_refine_report* report, pm_workspace* ws,
+                           hipStream_t stream);
+
+/* ---- Learned factorized sparse approximate inverse -------------------------------
+ * A 3-layer message-passing network (hidden 64, BF16 weights, FP32
+ * accumulation, MFMA dense layers) over the graph of an SPD matrix emits G on
+ * the lower pattern of A with a positive diagonal (M^-1 = G^T G is SPD by
+ * construction); trained on the GPU to minimise
+ * (1/P) sum_p ||(I - G A G^T) z_p||^2 / ||z_p||^2 over Gaussian probes.
+ * No reference counterpart.  A freshly created network starts from
+ * pm_fsai_initial_parameters(seed): random hidden layers and a decoder whose
+ * G is exactly D^-1/2, i.e. the Jacobi preconditioner. */
+typedef struct pm_fsai_net pm_fsai_net;
+typedef struct pm_fsai_graph pm_fsai_graph;
+int32_t pm_fsai_parameter_count(void);
+/* Host initial parameters (count = pm_fsai_parameter_count()). */
+int pm_fsai_initial_parameters(uint64_t seed, float* parameters, int64_t count);
+int pm_fsai_net_create(uint64_t seed, pm_fsai_net** net);
+int pm_fsai_net_destroy(pm_fsai_net* net);
+/* Host FP32 parameters in (Adam state reset) / out; count must equal
+ * pm_fsai_parameter_count(). */
+int pm_fsai_net_set_parameters(pm_fsai_net* net, const float* parameters, int64_t count);
+int pm_fsai_net_get_parameters(const pm_fsai_net* net, float* parameters, int64_t count);
+/* Graph of an SPD matrix (square, 32-bit index mirrors, positive diagonal):
+ * node features, normalised couplings, lower and upper patterns, storage for
+ * G and the activations.  The matrix must outlive the graph. */
+int pm_fsai_graph_create(const pm_csr* a, pm_fsai_graph** graph, hipStream_t stream);
+int pm_fsai_graph_destroy(pm_fsai_graph* graph);
+/* Evaluate the network: G and G^T values in the graph. */
+int pm_fsai_forward(pm_fsai_net* net, pm_fsai_graph* graph, hipStream_t stream);
+/* CSR descriptors of G (lower) and G^T (upper), owned by the graph. */
+int pm_fsai_factors(const pm_fsai_graph* graph, pm_csr* g, pm_csr* gt);
+/* Probe loss and full parameter gradient (host) for `probes` (1..8) probes
+ * drawn from `seed`; synchronises `stream`. */
+int pm_fsai_loss_grad(pm_fsai_net* net, pm_fsai_graph* graph, int32_t probes, uint64_t seed, double* loss,
+                      float* gradient, hipStream_t stream);
+/* One Adam step (lr) on that loss; *loss (if not null) is the loss before the
+ * step and synchronises `stream`. */
+int pm_fsai_train_step(pm_fsai_net* net, pm_fsai_graph* graph, int32_t probes, uint64_t seed, float lr,
+                       double* loss, hipStream_t stream);
+
 /* ---- Damped Schur complement (FP64) --------------------------------------------
  * Block normal equations in HBM, the device form of BlockNormalEquations<double>
  * (block_schur.h:197-319): B (pc blocks ps x ps), C (ec blocks es x es),

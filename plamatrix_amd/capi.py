@@ -95,7 +95,10 @@ EXPORTS = [
     "pm_block_pcg", "pm_solve_fixed_async", "pm_fixed_finalize", "pm_fixed_state_destroy", "pm_pcg_refine",
     "pm_schur_workspace_create", "pm_schur_workspace_destroy", "pm_schur_pattern_build_count",
     "pm_schur_pattern_size", "pm_normal_equations_assemble", "pm_schur_solve", "pm_schur_assemble", "pm_schur_system",
-    "pm_batch_layout", "pm_batch_pcg",
+    "pm_batch_layout", "pm_batch_pcg", "pm_pcg_factored", "pm_pcg_refine_factored",
+    "pm_fsai_parameter_count", "pm_fsai_initial_parameters", "pm_fsai_net_create", "pm_fsai_net_destroy",
+    "pm_fsai_net_set_parameters", "pm_fsai_net_get_parameters", "pm_fsai_graph_create", "pm_fsai_graph_destroy",
+    "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step",
 ]
 
 _lib = None
@@ -119,6 +122,9 @@ def lib():
         _lib.pm_schur_workspace_destroy.argtypes = [_p]
         _lib.pm_schur_pattern_build_count.argtypes = [_p]
         _lib.pm_schur_pattern_build_count.restype = _i64
+        _lib.pm_fsai_parameter_count.restype = _i32
+        _lib.pm_fsai_net_destroy.argtypes = [_p]
+        _lib.pm_fsai_graph_destroy.argtypes = [_p]
     return _lib
 
 
@@ -530,7 +536,7 @@ def device_array(ptr: int, n: int, dtype="float64"):
     import numpy as np
     import torch
 
-    tdt = {"float64": torch.float64, "int64": torch.int64, "int32": torch.int32}[dtype]
+    tdt = {"float64": torch.float64, "float32": torch.float32, "int64": torch.int64, "int32": torch.int32}[dtype]
     t = torch.empty(max(n, 1), dtype=tdt, device="cuda")
     if n > 0:
         check(lib_memcpy(t.data_ptr(), ptr, n * t.element_size()))
@@ -577,3 +583,124 @@ def batch_pcg(a: "DeviceCSR", row_begin, row_end, b, x, ws: Workspace, pol=None,
                              ws.handle, current_stream()))
     return [BatchReport(bool(r.converged), r.iterations, r.initial_residual, r.final_residual, r.status, r.bad_row)
             for r in reps[:systems]]
+
+
+# ---- learned factorized preconditioner ---------------------------------------------
+
+
+def fsai_parameter_count() -> int:
+    return int(lib().pm_fsai_parameter_count())
+
+
+def fsai_initial_parameters(seed: int = 0):
+    import numpy as np
+
+    out = np.zeros(fsai_parameter_count(), dtype=np.float32)
+    check(lib().pm_fsai_initial_parameters(ctypes.c_uint64(seed), out.ctypes.data_as(_p), _i64(len(out))))
+    return out
+
+
+class FsaiNet:
+    """Owns a pm_fsai_net (FP32 master parameters, Adam state, BF16 weights)."""
+
+    def __init__(self, seed: int = 0):
+        h = _p()
+        check(lib().pm_fsai_net_create(ctypes.c_uint64(seed), ctypes.byref(h)))
+        self.handle = h
+
+    @property
+    def parameters(self):
+        import numpy as np
+
+        out = np.zeros(fsai_parameter_count(), dtype=np.float32)
+        check(lib().pm_fsai_net_get_parameters(self.handle, out.ctypes.data_as(_p), _i64(len(out))))
+        return out
+
+    @parameters.setter
+    def parameters(self, values):
+        import numpy as np
+
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        check(lib().pm_fsai_net_set_parameters(self.handle, values.ctypes.data_as(_p), _i64(len(values))))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.pm_fsai_net_destroy(h)
+            self.handle = None
+
+
+class FsaiGraph:
+    """Owns a pm_fsai_graph for one SPD matrix (a DeviceCSR with 32-bit indices)."""
+
+    def __init__(self, a: "DeviceCSR"):
+        self.a = a
+        self.desc = a.descriptor()
+        h = _p()
+        check(lib().pm_fsai_graph_create(ctypes.byref(self.desc), ctypes.byref(h), current_stream()))
+        self.handle = h
+
+    def factors(self):
+        g, gt = pm_csr(), pm_csr()
+        check(lib().pm_fsai_factors(self.handle, ctypes.byref(g), ctypes.byref(gt)))
+        return g, gt
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and _lib is not None:
+            _lib.pm_fsai_graph_destroy(h)
+            self.handle = None
+
+
+def fsai_forward(net: FsaiNet, graph: FsaiGraph):
+    check(lib().pm_fsai_forward(net.handle, graph.handle, current_stream()))
+
+
+def fsai_loss_grad(net: FsaiNet, graph: FsaiGraph, probes: int = 8, seed: int = 0):
+    import numpy as np
+
+    loss = ctypes.c_double()
+    grad = np.zeros(fsai_parameter_count(), dtype=np.float32)
+    check(lib().pm_fsai_loss_grad(net.handle, graph.handle, _i32(probes), ctypes.c_uint64(seed), ctypes.byref(loss),
+                                  grad.ctypes.data_as(_p), current_stream()))
+    return loss.value, grad
+
+
+def fsai_train_step(net: FsaiNet, graph: FsaiGraph, probes: int = 8, seed: int = 0, lr: float = 1e-3) -> float:
+    loss = ctypes.c_double()
+    check(lib().pm_fsai_train_step(net.handle, graph.handle, _i32(probes), ctypes.c_uint64(seed),
+                                   ctypes.c_float(lr), ctypes.byref(loss), current_stream()))
+    return loss.value
+
+
+def pcg_factored(a: "DeviceCSR", b, x, graph: FsaiGraph, ws: Workspace, pol=None, **opts) -> "Report":
+    d = a.descriptor()
+    g, gt = graph.factors()
+    o = options(**opts)
+    p = pol if pol is not None else policy()
+    r = pm_solver_report()
+    check(lib().pm_pcg_factored(ctypes.byref(d), _ptr(b), _ptr(x), ctypes.byref(g), ctypes.byref(gt),
+                                ctypes.byref(o), ctypes.byref(p), ctypes.byref(r), ws.handle, current_stream()))
+    return _report(r)
+
+
+def pcg_refine_factored(a: "DeviceCSR", b64, x64, graph: FsaiGraph, ws: Workspace, target=1e-8, max_outer=10,
+                        pol=None, **inner) -> "RefineReport":
+    d = a.descriptor()
+    g, gt = graph.factors()
+    o = options(**inner)
+    p = pol if pol is not None else policy()
+    rep = pm_refine_report()
+    check(lib().pm_pcg_refine_factored(ctypes.byref(d), _ptr(b64), _ptr(x64), ctypes.byref(g), ctypes.byref(gt),
+                                       ctypes.byref(o), _f64(target), _i32(max_outer), ctypes.byref(p),
+                                       ctypes.byref(rep), ws.handle, current_stream()))
+    return RefineReport(bool(rep.converged), rep.outer_iterations, rep.inner_iterations, rep.initial_relres,
+                        rep.final_relres)
+
+
+def csr_arrays(d: pm_csr):
+    """Host copies (row_offsets, col_indices, values) of a CSR descriptor's arrays."""
+    ro = device_array(d.row_offsets32, d.rows + 1, "int32").astype("int64")
+    ci = device_array(d.col_indices32, d.nnz, "int32").astype("int64")
+    vals = device_array(d.values, d.nnz, "float64" if d.dtype == PM_F64 else "float32")
+    return ro, ci, vals

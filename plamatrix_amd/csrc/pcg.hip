@@ -38,6 +38,7 @@ enum Pre
     kPreNone = 0,
     kPreJacobi = 1,
     kPreBlock = 2,
+    kPreFactored = 3,  // M^-1 = G^T G with G, G^T given as CSR
 };
 
 enum Mode
@@ -274,6 +275,86 @@ __global__ __launch_bounds__(kBlock) void k_axpy_x(int64_t rows, S* __restrict__
     for (int64_t i = begin + threadIdx.x; i < end; i += kBlock) x[i] = fma_s(alpha, p[i], x[i]);
 }
 
+// Factored preconditioner, step half: alpha from p.q, x += alpha p,
+// r -= alpha q (z follows from the two sparse products).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_step_xr(int64_t rows, S* __restrict__ x, S* __restrict__ r,
+                                                    const S* __restrict__ p, const S* __restrict__ q, SolverState* st,
+                                                    double* partials, int par, int count_den)
+{
+    if (st->done != kRunning) return;
+    double den[1];
+    reduce_partials<1>(partials + kPartialsDen, count_den, den);
+    double a = 0.0;
+    const bool ok = step_length<S>(st->rho[par], den[0], a);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        st->alpha = a;
+        if (!ok) st->done = kBreakdown;
+    }
+    if (!ok) return;
+    const S alpha = static_cast<S>(a);
+    int64_t begin, end;
+    block_range(rows, kBlock, begin, end);
+    for (int64_t i = begin + threadIdx.x; i < end; i += kBlock)
+    {
+        x[i] = fma_s(alpha, p[i], x[i]);
+        r[i] = fma_s(-alpha, q[i], r[i]);
+    }
+}
+
+// t = G r (G lower, CSR), skipped once the solve has stopped.
+template <typename S, typename I>
+__global__ __launch_bounds__(kBlock) void k_factor_lower(int64_t rows, const I* __restrict__ ro,
+                                                         const I* __restrict__ ci, const S* __restrict__ v,
+                                                         const S* __restrict__ r, S* __restrict__ t,
+                                                         const SolverState* st)
+{
+    if (st->done != kRunning) return;
+    __shared__ S lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
+    {
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, r}, s, e);
+        if (row0 + lane < rows) t[row0 + lane] = acc;
+    }
+}
+
+// z = G^T t (G^T as CSR) and the partial sums r.r, r.z of the next direction.
+template <typename S, typename I>
+__global__ __launch_bounds__(kBlock) void k_factor_upper(int64_t rows, const I* __restrict__ ro,
+                                                         const I* __restrict__ ci, const S* __restrict__ v,
+                                                         const S* __restrict__ t, const S* __restrict__ r,
+                                                         S* __restrict__ z, const SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
+    __shared__ S lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    double rr = 0.0, rz = 0.0;
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
+    {
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, t}, s, e);
+        const int64_t row = row0 + lane;
+        if (row < rows)
+        {
+            z[row] = acc;
+            const double rv = static_cast<double>(r[row]);
+            rr = __builtin_fma(rv, rv, rr);
+            rz = __builtin_fma(rv, static_cast<double>(acc), rz);
+        }
+    }
+    double part[2] = {rr, rz};
+    publish_partials<2>(part, partials + kPartialsRes);
+}
+
 // Squared residuals of a fixed-iteration solve (toDoubleKernel /
 // toDoubleCheckedKernel, iterative_solver_kernels.cuh:240-252).
 __global__ __launch_bounds__(kBlock) void k_fixed_out(const SolverState* st, const double* partials, int count_res,
@@ -298,6 +379,8 @@ struct Problem
     const void* rhs;
     void* x;
     const void* inv_blocks;  // block preconditioner (device), or null
+    const pm_csr* gf;        // factored preconditioner: G (lower) and G^T, or null
+    const pm_csr* gtf;
     int64_t bsize;
     int pre;                 // Pre
     int grid_rows;           // workgroups of the row kernels (publish p.q)
@@ -360,7 +443,23 @@ struct Calls
                                     ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
                                     static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(nullptr),
                                     static_cast<S*>(pb.p0), pb.st));
-        out.push_back(update_any<kModeInit>(pb, pb.p0, 0));
+        if (pb.pre == kPreFactored) factored_apply(pb, out);
+        else out.push_back(update_any<kModeInit>(pb, pb.p0, 0));
+    }
+
+    // z = G^T (G r) and the r.r, r.z partials (grid_rows workgroups).
+    static void factored_apply(const Problem& pb, std::vector<KernelCall>& out)
+    {
+        const int64_t rows = pb.a->rows;
+        const int grid = static_cast<int>(std::min<int64_t>(full_grid(rows), kSpmvGrid));
+        out.push_back(make_call(k_factor_lower<S, int32_t>, dim3(grid), dim3(kBlock), rows, pb.gf->row_offsets32,
+                                pb.gf->col_indices32, static_cast<const S*>(pb.gf->values),
+                                static_cast<const S*>(pb.r), static_cast<S*>(pb.inv),
+                                static_cast<const SolverState*>(pb.st)));
+        out.push_back(make_call(k_factor_upper<S, int32_t>, dim3(pb.grid_rows), dim3(kBlock), rows,
+                                pb.gtf->row_offsets32, pb.gtf->col_indices32, static_cast<const S*>(pb.gtf->values),
+                                static_cast<const S*>(pb.inv), static_cast<const S*>(pb.r), static_cast<S*>(pb.z),
+                                static_cast<const SolverState*>(pb.st), pb.partials));
     }
 
     // Iteration t: t % 2 picks the p buffers and the state slots.
@@ -374,9 +473,17 @@ struct Calls
                                 dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a),
                                 ci(pb.a), static_cast<const S*>(pb.a->values), zsrc, static_cast<const S*>(p_old),
                                 static_cast<S*>(p_new), static_cast<S*>(pb.q), pb.st, pb.partials, par,
-                                pb.grid_units));
+                                pb.pre == kPreFactored ? pb.grid_rows : pb.grid_units));
         if (!refresh)
         {
+            if (pb.pre == kPreFactored)
+            {
+                out.push_back(make_call(k_step_xr<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows,
+                                        static_cast<S*>(pb.x), static_cast<S*>(pb.r), static_cast<const S*>(p_new),
+                                        static_cast<const S*>(pb.q), pb.st, pb.partials, par, pb.grid_rows));
+                factored_apply(pb, out);
+                return;
+            }
             out.push_back(update_any<kModeStep>(pb, p_new, par));
             return;
         }
@@ -386,7 +493,8 @@ struct Calls
                                 pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
                                 static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x), static_cast<S*>(pb.r),
                                 static_cast<S*>(nullptr), static_cast<S*>(nullptr), pb.st));
-        out.push_back(update_any<kModeRefresh>(pb, p_new, par));
+        if (pb.pre == kPreFactored) factored_apply(pb, out);
+        else out.push_back(update_any<kModeRefresh>(pb, p_new, par));
     }
 
     static void build(const Problem& pb, int what, int64_t first, int64_t count, int refresh_interval,
@@ -465,13 +573,16 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
     // workspace addresses are part of the key: a grown workspace must never
     // replay a plan that still points at its old buffers
     char buf[640];
-    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld|%d", kind,
+    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld|%d|%p|%p|%p|%p",
+                  kind,
                   pb.r, static_cast<void*>(pb.st), static_cast<void*>(pb.partials),
                   (const void*)pb.a->row_offsets, (const void*)pb.a->row_offsets32, (const void*)pb.a->col_indices,
                   (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
                   static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
-                  static_cast<long long>(pb.bsize), pb.nt ? 1 : 0);
+                  static_cast<long long>(pb.bsize), pb.nt ? 1 : 0, pb.gf ? pb.gf->values : nullptr,
+                  pb.gf ? (const void*)pb.gf->row_offsets32 : nullptr, pb.gtf ? pb.gtf->values : nullptr,
+                  pb.gtf ? (const void*)pb.gtf->row_offsets32 : nullptr);
     return buf;
 }
 
@@ -544,7 +655,7 @@ int finish_report(const SolverState& fin, const pm_solver_options* options, pm_s
 template <typename S>
 int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_blocks, int64_t bsize, int pre,
                    const pm_solver_options* options, const pm_solve_policy* policy_in, pm_solver_report* report,
-                   pm_workspace* ws, hipStream_t stream)
+                   pm_workspace* ws, hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr)
 {
     const Policy policy = resolve_policy(policy_in);
     Problem pb{};
@@ -552,6 +663,8 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     pb.rhs = rhs;
     pb.x = x;
     pb.inv_blocks = inv_blocks;
+    pb.gf = gf;
+    pb.gtf = gtf;
     pb.bsize = pre == kPreBlock ? bsize : 1;
     pb.pre = pre;
     PM_TRY(prepare(ws, pb, stream));
@@ -607,14 +720,29 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
 
 int solve_dispatch(const pm_csr* a, const void* rhs, void* x, const void* inv_blocks, int64_t bsize, int pre,
                    const pm_solver_options* options, const pm_solve_policy* policy, pm_solver_report* report,
-                   pm_workspace* ws, hipStream_t stream)
+                   pm_workspace* ws, hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr)
 {
     PM_TRY(validate_options(options));
     PM_TRY(check_system(a, rhs, x, "iterative solver"));
     if (!ws) return fail(PM_INVALID_ARGUMENT, "iterative solver: null workspace");
     if (a->dtype == PM_F32)
-        return adaptive_solve<float>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream);
-    return adaptive_solve<double>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream);
+        return adaptive_solve<float>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream, gf, gtf);
+    return adaptive_solve<double>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream, gf, gtf);
+}
+
+// The factors of M^-1 = G^T G: square, same size and dtype as A, with 32-bit
+// index mirrors.
+int check_factors(const pm_csr* a, const pm_csr* g, const pm_csr* gt)
+{
+    for (const pm_csr* f : {g, gt})
+    {
+        PM_TRY(check_csr(f, "factored preconditioner"));
+        if (f->rows != a->rows || f->cols != a->cols || f->dtype != a->dtype)
+            return fail(PM_INVALID_ARGUMENT, "factored preconditioner: factor shape or dtype differs from the matrix");
+        if (!f->row_offsets32 || !f->col_indices32)
+            return fail(PM_INVALID_ARGUMENT, "factored preconditioner: factors need 32-bit index mirrors");
+    }
+    return PM_OK;
 }
 
 } // namespace pm
@@ -827,7 +955,7 @@ __global__ void k_zero_u32(unsigned* p) { *p = 0u; }
 
 int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options* inner, double target,
            int32_t max_outer, const pm_solve_policy* policy, pm_refine_report* report, pm_workspace* ws,
-           hipStream_t stream)
+           hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr)
 {
     const int64_t n = a->rows;
     const size_t r_bytes = (sizeof(double) * static_cast<size_t>(n) + 255) & ~size_t(255);
@@ -880,8 +1008,8 @@ int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options*
         hipLaunchKernelGGL(k_scale_to_f32, dim3(grid), dim3(kBlock), 0, stream, n, r64, 1.0 / norm, rf, d);
         PM_HIP(hipGetLastError());
         pm_solver_report inner_rep{};
-        PM_TRY(adaptive_solve<float>(a, rf, d, nullptr, 1, in.use_jacobi ? kPreJacobi : kPreNone, &in, policy,
-                                     &inner_rep, ws, stream));
+        const int pre = gf ? kPreFactored : (in.use_jacobi ? kPreJacobi : kPreNone);
+        PM_TRY(adaptive_solve<float>(a, rf, d, nullptr, 1, pre, &in, policy, &inner_rep, ws, stream, gf, gtf));
         rep.inner_iterations += inner_rep.iterations;
         hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(kBlock), 0, stream, n, x, d, norm);
         PM_HIP(hipGetLastError());
@@ -910,4 +1038,34 @@ extern "C" int pm_pcg_refine(const pm_csr* a, const double* rhs, double* x, cons
         return PM_OK;
     }
     return pm::refine(a, rhs, x, inner, target_relres, max_outer, policy, report, ws, stream);
+}
+
+extern "C" int pm_pcg_factored(const pm_csr* a, const void* rhs, void* x, const pm_csr* g, const pm_csr* gt,
+                               const pm_solver_options* options, const pm_solve_policy* policy,
+                               pm_solver_report* report, pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(pm::check_csr(a, "factored pcg"));
+    PM_TRY(pm::check_factors(a, g, gt));
+    return pm::solve_dispatch(a, rhs, x, nullptr, 1, pm::kPreFactored, options, policy, report, ws, stream, g, gt);
+}
+
+extern "C" int pm_pcg_refine_factored(const pm_csr* a, const double* rhs, double* x, const pm_csr* g,
+                                      const pm_csr* gt, const pm_solver_options* inner, double target_relres,
+                                      int32_t max_outer, const pm_solve_policy* policy, pm_refine_report* report,
+                                      pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(pm::validate_options(inner));
+    PM_TRY(pm::check_system(a, rhs, x, "refinement"));
+    if (a->dtype != PM_F32) return pm::fail(PM_INVALID_ARGUMENT, "refinement expects an FP32 matrix");
+    PM_TRY(pm::check_factors(a, g, gt));
+    if (!ws) return pm::fail(PM_INVALID_ARGUMENT, "refinement: null workspace");
+    if (!(target_relres >= 0.0) || max_outer < 0) return pm::fail(PM_INVALID_ARGUMENT, "refinement: invalid target");
+    PM_TRY(ws->bind(stream));
+    PM_TRY(ws->ensure_events());
+    if (a->rows == 0)
+    {
+        if (report) *report = pm_refine_report{1, 0, 0, 0.0, 0.0};
+        return PM_OK;
+    }
+    return pm::refine(a, rhs, x, inner, target_relres, max_outer, policy, report, ws, stream, g, gt);
 }

@@ -1,0 +1,163 @@
+"""GPU checks of the learned factorized preconditioner (learned.hip) and the
+factored PCG, against the CPU oracle oracle/fsai.py.
+
+- initial network: G = D^-1/2, so factored PCG reproduces Jacobi-PCG;
+- forward: G values within BF16 rounding of the oracle (same rounding points);
+- loss and gradient of a training step agree with the oracle's autograd
+  (which is itself pinned to finite differences on the CPU);
+- factored PCG agrees with the oracle's FP64 PCG with z = G^T G r
+  (iterations within 2, solution within relative 1e-6), FP64 true residual
+  recomputed;
+- training on generated systems lowers the probe loss and the PCG iteration
+  count on a held-out system compared with Jacobi.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import fsai  # noqa: E402
+from plamatrix_amd import capi, generators  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    capi.lib()
+    return torch.device("cuda:0")
+
+
+@pytest.fixture
+def ws(dev):
+    w = capi.Workspace()
+    yield w
+    torch.cuda.synchronize()
+
+
+def system(nx, ny, nz, seed, dtype=np.float32):
+    return generators.diffusion3d_box_csr(nx, ny, nz, seed=seed, dtype=dtype)
+
+
+def perturbed_parameters(seed):
+    rng = np.random.default_rng(seed)
+    p = capi.fsai_initial_parameters(seed)
+    p[fsai.OFF_WOFF:fsai.OFF_BOFF] = 0.05 * rng.standard_normal(fsai.HID)
+    p[fsai.OFF_WDIAG:fsai.OFF_BDIAG] = 0.05 * rng.standard_normal(fsai.HID)
+    p[fsai.OFF_ALPHA] = -0.4
+    p[fsai.OFF_BOFF] = 0.01
+    return p
+
+
+def test_parameter_layout_matches_oracle(dev):
+    assert capi.fsai_parameter_count() == fsai.PARAMS
+    p = capi.fsai_initial_parameters(3)
+    assert p[fsai.OFF_WOFF:fsai.OFF_BDIAG].tolist() == [0.0] * (fsai.OFF_BDIAG - fsai.OFF_WOFF)
+    assert abs(p[fsai.OFF_BDIAG] - np.log(np.e - 1.0)) < 1e-7
+
+
+def test_initial_network_is_jacobi(dev, ws):
+    h = system(12, 10, 8, seed=4)
+    a = capi.to_device_csr(h, compact=True)
+    net = capi.FsaiNet(seed=1)
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    g, gt = graph.factors()
+    ro, ci, vals = capi.csr_arrays(g)
+    rows = np.repeat(np.arange(h.rows), np.diff(ro))
+    on = rows == ci
+    np.testing.assert_allclose(vals[on], 1.0, rtol=2e-7)  # unit diagonal: D^-1/2 = I
+    assert np.all(vals[~on] == 0.0)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=5)).to(dev)
+    x1 = torch.zeros_like(b)
+    x2 = torch.zeros_like(b)
+    r1 = capi.pcg(a, b, x1, ws, relative_tolerance=1e-5, max_iterations=2000)
+    r2 = capi.pcg_factored(a, b, x2, graph, ws, relative_tolerance=1e-5, max_iterations=2000)
+    assert r1.converged and r2.converged
+    assert abs(r1.iterations - r2.iterations) <= 1
+
+
+def test_forward_matches_oracle(dev):
+    h = system(9, 8, 7, seed=6)
+    a = capi.to_device_csr(h, compact=True)
+    p = perturbed_parameters(2)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = p
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    g, gt = graph.factors()
+    ro, ci, vals = capi.csr_arrays(g)
+    rows, cols, ref = fsai.factors(h.row_offsets, h.col_indices, h.values, p)
+    assert np.array_equal(np.repeat(np.arange(h.rows), np.diff(ro)), rows) and np.array_equal(ci, cols)
+    err = np.abs(vals - ref) / (np.abs(ref) + 1e-3)
+    assert err.max() < 2e-2, err.max()
+    # G^T is the transpose of G
+    tro, tci, tvals = capi.csr_arrays(gt)
+    import scipy.sparse as sp
+
+    G = sp.csr_matrix((vals.astype(np.float64), ci, ro), shape=(h.rows, h.rows))
+    GT = sp.csr_matrix((tvals.astype(np.float64), tci, tro), shape=(h.rows, h.rows))
+    assert abs(G.T - GT).max() == 0.0
+
+
+def test_loss_and_gradient_match_oracle(dev):
+    h = system(8, 7, 6, seed=8)
+    a = capi.to_device_csr(h, compact=True)
+    p = perturbed_parameters(4)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = p
+    graph = capi.FsaiGraph(a)
+    loss, grad = capi.fsai_loss_grad(net, graph, probes=8, seed=21)
+    ref_loss, ref_grad = fsai.loss_and_grad(h.row_offsets, h.col_indices, h.values, p, 8, 21)
+    assert abs(loss - ref_loss) <= 2e-3 * ref_loss, (loss, ref_loss)
+    cos = float(grad @ ref_grad / (np.linalg.norm(grad) * np.linalg.norm(ref_grad)))
+    assert cos > 0.995, cos
+    assert np.linalg.norm(grad - ref_grad) <= 0.1 * np.linalg.norm(ref_grad)
+    # the decoder scalars individually
+    for k in (fsai.OFF_ALPHA, fsai.OFF_BOFF, fsai.OFF_BDIAG):
+        assert abs(grad[k] - ref_grad[k]) <= 0.05 * abs(ref_grad[k]) + 1e-6, (k, grad[k], ref_grad[k])
+
+
+def test_factored_pcg_matches_oracle(dev, ws):
+    h = system(10, 9, 8, seed=10, dtype=np.float64)
+    a = capi.to_device_csr(h, compact=True)
+    p = perturbed_parameters(5)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = p
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    g, _ = graph.factors()
+    gro, gci, gvals = capi.csr_arrays(g)
+    rows = np.repeat(np.arange(h.rows), np.diff(gro))
+    rhs = generators.rhs_uniform(h.rows, seed=3)
+    x_ref, it_ref = fsai.factored_pcg(h.row_offsets, h.col_indices, h.values, rows, gci, gvals, rhs,
+                                      relative_tolerance=1e-10, max_iterations=3000)
+    b = torch.from_numpy(rhs).to(dev)
+    x = torch.zeros_like(b)
+    rep = capi.pcg_factored(a, b, x, graph, ws, relative_tolerance=1e-10, max_iterations=3000)
+    assert rep.converged and abs(rep.iterations - it_ref) <= 2, (rep.iterations, it_ref)
+    got = x.cpu().numpy()
+    assert np.linalg.norm(got - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    assert capi.true_relres(a, b, x, ws) <= 1e-9
+
+
+def test_training_improves_on_jacobi(dev, ws):
+    net = capi.FsaiNet(seed=7)
+    train = [capi.FsaiGraph(capi.to_device_csr(system(16, 16, 12, seed=100 + i), compact=True)) for i in range(4)]
+    first = [capi.fsai_loss_grad(net, g, probes=8, seed=999)[0] for g in train]
+    for step in range(240):
+        capi.fsai_train_step(net, train[step % 4], probes=8, seed=step, lr=2e-3)
+    last = [capi.fsai_loss_grad(net, g, probes=8, seed=999)[0] for g in train]
+    assert np.mean(last) < 0.8 * np.mean(first), (first, last)
+    h = system(18, 16, 14, seed=555)
+    a = capi.to_device_csr(h, compact=True)
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=9)).to(dev)
+    xj, xl = torch.zeros_like(b), torch.zeros_like(b)
+    rj = capi.pcg(a, b, xj, ws, relative_tolerance=1e-5, max_iterations=5000)
+    rl = capi.pcg_factored(a, b, xl, graph, ws, relative_tolerance=1e-5, max_iterations=5000)
+    assert rj.converged and rl.converged
+    assert rl.iterations < rj.iterations, (rl.iterations, rj.iterations)
+    assert capi.true_relres(a, b, xl, ws) <= 5e-5
