@@ -3,10 +3,14 @@
 Workload: 3-D 7-point variable-coefficient diffusion on a 128^3 grid
 (n = 2,097,152, nnz = 14,581,760; log-normal face coefficients, sigma 1.5,
 10:1 z anisotropy, unit diagonal), RHS N(0,1).  A step is one complete solve
-from x = 0 to an FP64 true relative residual <= 1e-6: Jacobi-PCG with FP32
-SpMV and vectors, FP64 dot products, FP64-accumulated residual refresh every
-50 iterations, and FP64 outer refinement.  Inputs are resident in HBM before
-the timed region.
+from x = 0 to an FP64 true relative residual <= 1e-6 with FP32 SpMV and
+vectors, FP64 dot products, FP64-accumulated residual refresh every 50
+iterations, and FP64 outer refinement.  Two preconditioners are timed:
+Jacobi, and the learned factor G (M^-1 = G^T G) whose network inference on
+this matrix runs inside the timed solve.  The headline value is the learned
+solve, or the Jacobi solve when the learned one needs at least as many
+iterations (the fallback rule).  Inputs are resident in HBM before the timed
+region.
 
 Also measured: the CSR SpMV kernel's HBM bandwidth, rotating over four copies
 of the matrix (4 x 125 MB, more than the 256 MB Infinity Cache) so the
@@ -36,10 +40,28 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4) -> 
 
 
 def pcg_iteration_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4) -> int:
-    """Algorithmic bytes of one fused Jacobi-PCG iteration (two kernels): the
-    SpMV streams (values, columns, offsets), direction update reads z and p and
-    writes p and q, the vector update reads x, p, r, q, M^-1 and writes x, r, z."""
-    return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + 12 * n * value_bytes
+    """Algorithmic bytes of one Jacobi-PCG iteration (three kernels): the
+    direction kernel reads z and p and writes p; the SpMV streams values,
+    columns and offsets, reads p once and writes q; the vector update reads
+    x, p, r, q, M^-1 and writes x, r, z."""
+    return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + 13 * n * value_bytes
+
+
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_spmv_diffusion128_traffic.txt")
+
+
+def committed_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (already scaled by the copy-kernel calibration), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) >= 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def main() -> int:
@@ -56,14 +78,17 @@ def main() -> int:
     ap.add_argument("--max-inner", type=int, default=100000)
     ap.add_argument("--inner-rel", type=float, default=1e-3,
                     help="relative tolerance of each FP32 inner solve of the refinement loop")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "plamatrix_amd", "fsai_weights.npy"),
+                    help="trained network parameters (float32 .npy written by plamatrix_amd.train_fsai)")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: one solve capped at 2000 inner iterations, no CPU leg")
     args = ap.parse_args()
     if args.profile:
         args.steps, args.warmup, args.no_cpu, args.max_inner, args.spmv_rounds = 1, 0, True, 2000, 20
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this path has no multi-GPU mode (systems shard without "
-              "communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the solve path runs one process per GPU and independent "
+              "systems shard across GPUs without communication. Run with --gpus 1.", file=sys.stderr)
         return 2
 
     import torch
@@ -125,25 +150,58 @@ def main() -> int:
     x = torch.zeros(n, dtype=torch.float64, device=dev)
     inner = dict(relative_tolerance=args.inner_rel, max_iterations=args.max_inner)
 
-    def solve():
+    def timed(run):
+        for _ in range(args.warmup):
+            run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reps = [run() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps * 1e3, reps[-1]
+
+    def jacobi_solve():
         x.zero_()
         return capi.pcg_refine(a, b, x, ws, target=1e-6, max_outer=10, pol=pol, **inner)
 
-    for _ in range(args.warmup):
-        solve()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reports = []
-    for _ in range(args.steps):
-        reports.append(solve())
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_solve = elapsed / args.steps * 1e3
-    rep = reports[-1]
+    ms_per_solve, rep = timed(jacobi_solve)
     relres = capi.true_relres(a, b, x, ws)
     iterations = rep.inner_iterations
     it_bytes = pcg_iteration_bytes(n, nnz)
     it_ms = ms_per_solve / max(iterations, 1)
+
+    # ---- learned preconditioner: G inferred on this matrix inside the timed solve ----
+    learned = None
+    if not args.no_learned:
+        net = capi.FsaiNet(seed=0)
+        net.parameters = np.load(args.weights)
+        torch.cuda.synchronize()
+        f0 = time.perf_counter()
+        fg = capi.FsaiGraph(a)  # features, couplings, lower/upper patterns: once per sparsity pattern
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - f0) * 1e3
+
+        def learned_solve():
+            x.zero_()
+            capi.fsai_forward(net, fg)
+            return capi.pcg_refine_factored(a, b, x, fg, ws, target=1e-6, max_outer=10, pol=pol, **inner)
+
+        l_ms, l_rep = timed(learned_solve)
+        l_relres = capi.true_relres(a, b, x, ws)
+        torch.cuda.synchronize()
+        f0 = time.perf_counter()
+        for _ in range(3):
+            capi.fsai_forward(net, fg)
+        torch.cuda.synchronize()
+        fwd_ms = (time.perf_counter() - f0) / 3 * 1e3
+        fallback = not (l_rep.converged and l_relres <= 1e-6 and l_rep.inner_iterations < iterations)
+        learned = {"ms": round(l_ms, 3), "converged": l_rep.converged, "outer": l_rep.outer_iterations,
+                   "inner_iterations": l_rep.inner_iterations, "true_relres": l_relres,
+                   "forward_ms": round(fwd_ms, 3), "graph_build_ms": round(graph_ms, 3),
+                   "ms_per_iteration": round((l_ms - fwd_ms) / max(l_rep.inner_iterations, 1), 4),
+                   "time_ratio_vs_jacobi": round(l_ms / ms_per_solve, 4),
+                   "iteration_ratio_vs_jacobi": round(l_rep.inner_iterations / max(iterations, 1), 4),
+                   "fallback_to_jacobi": fallback, "weights": os.path.relpath(args.weights, ROOT)}
+        del net, fg
 
     # to 1e-8 as well (refinement) - reported, not the headline
     x.zero_()
@@ -169,14 +227,21 @@ def main() -> int:
                          f"restating iterative_solver_cpu.cpp) on the full 128^3 system took {cpu_s:.1f} s; "
                          f"scaled to the {iterations} iterations the GPU solve needed"}
 
+    kernel = "k_spmv<float,int>" if lanes == 0 else f"k_spmv_lanes<float,int,{lanes}>"
+    # headline: learned preconditioner with its Jacobi fallback (taken when the
+    # learned solve needs at least as many iterations or misses the residual)
+    if learned is not None and not learned["fallback_to_jacobi"]:
+        value = learned["ms"]
+    else:
+        value = ms_per_solve
     line = {
         "metric": METRIC,
-        "value": round(ms_per_solve, 3),
+        "value": round(value, 3),
         "unit": "ms",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_solve, 3),
+        "ms_per_step": round(value, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
@@ -184,16 +249,19 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"3-D 7-point variable-coefficient diffusion {args.grid}^3 (n={n}, nnz={nnz}), "
-                        "Jacobi-PCG FP32 SpMV/vectors + FP64 dots, residual refresh every 50, FP64 refinement, "
-                        "solve from x=0 to FP64 true relres 1e-6",
+                        "solve from x=0 to FP64 true relres 1e-6: learned G^T G preconditioner (G inferred inside "
+                        "the timed solve) with Jacobi-PCG fallback; FP32 SpMV/vectors + FP64 dots, residual "
+                        "refresh every 50, FP64 refinement",
+            "headline": "jacobi (fallback)" if learned is None or learned["fallback_to_jacobi"] else "learned",
             "rows": n, "nnz": nnz, "index_bits": 32, "lanes_per_row": lanes, "graph": bool(args.graph),
         },
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_spmv<float,int>" if lanes == 0 else f"k_spmv_lanes<float,int,{lanes}>", "bytes_per_launch": spmv_b,
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": committed_traffic(kernel),
+                     "kernel": kernel, "bytes_per_launch": spmv_b,
                      "ms_per_launch": round(spmv_ms, 5)},
         "cpu_baseline": cpu,
-        "solve": {"converged": rep.converged, "outer": rep.outer_iterations, "inner_iterations": iterations,
+        "learned": learned,
+        "solve": {"preconditioner": "jacobi", "ms": round(ms_per_solve, 3), "converged": rep.converged, "outer": rep.outer_iterations, "inner_iterations": iterations,
                   "true_relres": relres, "ms_per_iteration": round(it_ms, 4),
                   "iteration_bytes": it_bytes, "iteration_gbs": round(it_bytes / (it_ms * 1e-3) / 1e9, 1),
                   "to_1e8": {"ms": round(ms_1e8, 2), "outer": rep8.outer_iterations,

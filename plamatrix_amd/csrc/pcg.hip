@@ -1,11 +1,14 @@
 // Conjugate gradients on gfx950: CG, Jacobi-PCG, block-Jacobi PCG, the
 // fixed-iteration asynchronous variant and FP32/FP64 iterative refinement.
 //
-// One iteration is two kernels:
-//   k_direction_spmv  reduce the previous update's r.r, r.z  -> stop test, beta
-//                     p = z + beta p (ping-pong), q = A p, publish p.q
-//   k_update          reduce p.q -> alpha; x += alpha p, r -= alpha q,
-//                     z = M^-1 r, publish r.r and r.z
+// One iteration is three kernels:
+//   k_direction  reduce the previous update's r.r, r.z -> stop test, beta;
+//                p = z + beta p (ping-pong buffers)
+//   k_spmv_den   q = A p (one gather per entry), publish p.q
+//   k_update     reduce p.q -> alpha; x += alpha p, r -= alpha q,
+//                z = M^-1 r, publish r.r and r.z
+// (with the factored preconditioner M^-1 = G^T G the update is x, r only and
+// z = G^T (G r) follows as two more sparse products that publish r.r, r.z).
 // Every dot product is accumulated in FP64 in a fixed order; every workgroup
 // re-reduces the previous kernel's partial sums itself, so alpha, beta and the
 // stopping test are derived identically everywhere on the device without
@@ -75,15 +78,14 @@ __global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel
     for (int i = 0; i < 8; ++i) st->counters[i] = 0u;
 }
 
-// First kernel of iteration t (slot par = t % 2), kDirBlock threads.  NT
-// streams the matrix with non-temporal loads.
-template <typename S, typename I, bool NT>
-__global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, const I* __restrict__ ro,
-                                                           const I* __restrict__ ci, const S* __restrict__ v,
-                                                           const S* __restrict__ z, const S* __restrict__ p_old,
-                                                           S* __restrict__ p_new, S* __restrict__ q,
-                                                           SolverState* st, double* partials, int par,
-                                                           int count_res)
+// First kernel of iteration t (slot par = t % 2): every workgroup reduces the
+// previous update's r.r and r.z partials in the same order, decides stop /
+// beta exactly alike, and forms p_new = z + beta p_old for its rows.  Block 0
+// records the decision in the state.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_direction(int64_t rows, const S* __restrict__ z,
+                                                      const S* __restrict__ p_old, S* __restrict__ p_new,
+                                                      SolverState* st, double* partials, int par, int count_res)
 {
     if (st->done != kRunning) return;
     double res[2];
@@ -123,10 +125,32 @@ __global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, cons
         if (code != kRunning) st->done = code;
     }
     if (code != kRunning) return;
+    const S b = static_cast<S>(beta);
+    constexpr int U = 4;
+    int64_t begin, end;
+    block_range(rows, U * kBlock, begin, end);
+    for (int64_t base = begin + threadIdx.x; base < end; base += U * kBlock)
+    {
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+        {
+            const int64_t i = base + j * kBlock;
+            if (i < end) p_new[i] = fma_s(b, p_old[i], z[i]);
+        }
+    }
+}
 
+// Second kernel of iteration t: q = A p_new and the per-workgroup partial
+// p_new.q.  NT streams the matrix with non-temporal loads.
+template <typename S, typename I, bool NT>
+__global__ __launch_bounds__(kDirBlock) void k_spmv_den(int64_t rows, const I* __restrict__ ro,
+                                                        const I* __restrict__ ci, const S* __restrict__ v,
+                                                        const S* __restrict__ p, S* __restrict__ q,
+                                                        const SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
     __shared__ S lv_all[kDirWaves][kStreamCap];
     __shared__ I lc_all[kDirWaves][kStreamCap];
-    const S b = static_cast<S>(beta);
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     double den = 0.0;
@@ -134,15 +158,13 @@ __global__ __launch_bounds__(kDirBlock) void k_direction_spmv(int64_t rows, cons
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S, I, DirectionSource<S>, NT>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
-                                                                   DirectionSource<S>{v, z, p_old, b}, s, e);
+        const S acc = chunk_row_sums<S, I, SpmvSource<S>, NT>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                                              SpmvSource<S>{v, p}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
         {
-            const S pn = fma_s(b, p_old[row], z[row]);
-            p_new[row] = pn;
             q[row] = acc;
-            den = __builtin_fma(static_cast<double>(pn), static_cast<double>(acc), den);
+            den = __builtin_fma(static_cast<double>(p[row]), static_cast<double>(acc), den);
         }
     }
     double part[1] = {den};
@@ -469,11 +491,13 @@ struct Calls
         const void* p_old = par == 0 ? pb.p0 : pb.p1;
         void* p_new = par == 0 ? pb.p1 : pb.p0;
         const S* zsrc = pb.pre == kPreNone ? static_cast<const S*>(pb.r) : static_cast<const S*>(pb.z);
-        out.push_back(make_call(pb.nt ? k_direction_spmv<S, I, true> : k_direction_spmv<S, I, false>,
-                                dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a),
-                                ci(pb.a), static_cast<const S*>(pb.a->values), zsrc, static_cast<const S*>(p_old),
-                                static_cast<S*>(p_new), static_cast<S*>(pb.q), pb.st, pb.partials, par,
+        out.push_back(make_call(k_direction<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, zsrc,
+                                static_cast<const S*>(p_old), static_cast<S*>(p_new), pb.st, pb.partials, par,
                                 pb.pre == kPreFactored ? pb.grid_rows : pb.grid_units));
+        out.push_back(make_call(pb.nt ? k_spmv_den<S, I, true> : k_spmv_den<S, I, false>, dim3(pb.grid_rows),
+                                dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
+                                static_cast<const S*>(p_new), static_cast<S*>(pb.q),
+                                static_cast<const SolverState*>(pb.st), pb.partials));
         if (!refresh)
         {
             if (pb.pre == kPreFactored)
