@@ -8,8 +8,8 @@ vectors, FP64 dot products, FP64-accumulated residual refresh every 50
 iterations, and FP64 outer refinement.  Two preconditioners are timed:
 Jacobi, and the learned factor G (M^-1 = G^T G) whose network inference on
 this matrix runs inside the timed solve.  The headline value is the learned
-solve, or the Jacobi solve when the learned one needs at least as many
-iterations (the fallback rule).  Inputs are resident in HBM before the timed
+solve (graph build, inference and factored PCG), or the Jacobi solve when
+the learned one is not faster or misses the residual (the fallback rule).  Inputs are resident in HBM before the timed
 region.
 
 Also measured: the CSR SpMV kernel's HBM bandwidth, rotating over four copies
@@ -193,13 +193,18 @@ def main() -> int:
             capi.fsai_forward(net, fg)
         torch.cuda.synchronize()
         fwd_ms = (time.perf_counter() - f0) / 3 * 1e3
-        fallback = not (l_rep.converged and l_relres <= 1e-6 and l_rep.inner_iterations < iterations)
-        learned = {"ms": round(l_ms, 3), "converged": l_rep.converged, "outer": l_rep.outer_iterations,
+        # one-off solve: the per-pattern graph build, network inference and the
+        # factored solve; with a reused pattern only the last two recur
+        total_ms = graph_ms + l_ms
+        fallback = not (l_rep.converged and l_relres <= 1e-6 and total_ms < ms_per_solve)
+        learned = {"ms": round(total_ms, 3), "solve_ms_pattern_reused": round(l_ms, 3),
+                   "converged": l_rep.converged, "outer": l_rep.outer_iterations,
                    "inner_iterations": l_rep.inner_iterations, "true_relres": l_relres,
                    "forward_ms": round(fwd_ms, 3), "graph_build_ms": round(graph_ms, 3),
                    "ms_per_iteration": round((l_ms - fwd_ms) / max(l_rep.inner_iterations, 1), 4),
-                   "time_ratio_vs_jacobi": round(l_ms / ms_per_solve, 4),
+                   "time_ratio_vs_jacobi": round(total_ms / ms_per_solve, 4),
                    "iteration_ratio_vs_jacobi": round(l_rep.inner_iterations / max(iterations, 1), 4),
+                   "fewer_iterations_than_jacobi": l_rep.inner_iterations < iterations,
                    "fallback_to_jacobi": fallback, "weights": os.path.relpath(args.weights, ROOT)}
         del net, fg
 
@@ -229,7 +234,7 @@ def main() -> int:
 
     kernel = "k_spmv<float,int>" if lanes == 0 else f"k_spmv_lanes<float,int,{lanes}>"
     # headline: learned preconditioner with its Jacobi fallback (taken when the
-    # learned solve needs at least as many iterations or misses the residual)
+    # learned solve, graph build included, is not faster or misses the residual)
     if learned is not None and not learned["fallback_to_jacobi"]:
         value = learned["ms"]
     else:

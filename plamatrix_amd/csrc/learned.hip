@@ -202,23 +202,17 @@ __global__ __launch_bounds__(kBlock) void k_row_starts(int64_t n, int64_t nl, co
 
 // ---- BF16 MFMA GEMM ---------------------------------------------------------------
 //
-// Y[n x N] = A[n x K] * B[K x N] with A row-major (lda = K) and B supplied
-// transposed (Bt: N x K row-major), FP32 accumulation.  One wavefront owns 16
-// rows and all N columns; per K step of 32 a lane loads 8 consecutive BF16 of
-// A (row l%16) and of Bt (column l%16 of the tile), the operand layout of
+// Y[n x N] = A[n x K] * B[K x N] stored FP32 (the backward input gradient
+// dP W^T), A row-major (lda = K) and B supplied transposed (Bt: N x K
+// row-major), FP32 accumulation.  One wavefront owns 16 rows and all N
+// columns; per K step of 32 a lane loads 8 consecutive BF16 of A (row l%16)
+// and of Bt (column l%16 of the tile), the operand layout of
 // v_mfma_f32_16x16x32_bf16, whose result lane l holds rows (l/16)*4 + i of
 // column l%16.  A may be the concatenation of two row-major blocks [A0 | A1]
-// (each K/2 wide) so [H, M] never has to be materialised.
-//
-// Epilogues: kEpiRelu  H = relu(Y + b) stored BF16
-//            kEpiGrad  Y stored FP32 (input gradient), optionally split in two
-constexpr int kEpiRelu = 0;
-constexpr int kEpiStoreF = 1;
-
-template <int K, int N, int EPI>
+// (each K/2 wide).
+template <int K, int N>
 __global__ __launch_bounds__(kBlock) void k_gemm(int64_t n, const __bf16* __restrict__ a0,
                                                  const __bf16* __restrict__ a1, const __bf16* __restrict__ bt,
-                                                 const float* __restrict__ bias, __bf16* __restrict__ hout,
                                                  float* __restrict__ yout)
 {
     constexpr int KH = (K > 64) ? K / 2 : K;  // width of each A block
@@ -265,10 +259,149 @@ __global__ __launch_bounds__(kBlock) void k_gemm(int64_t n, const __bf16* __rest
             {
                 const int64_t row = row0 + (lane / 16) * 4 + i;
                 if (row >= n) continue;
-                if (EPI == kEpiRelu) hout[row * N + col] = to_bf(fmaxf(acc[j][i] + bias[col], 0.f));
-                else yout[row * N + col] = acc[j][i];
+                yout[row * N + col] = acc[j][i];
             }
         }
+    }
+}
+
+// One network layer with the graph aggregation fused in:
+//     H_out = relu([A, S A] W + b)   (AGG, K = 128, A = H_in of width 64)
+//     H_out = relu(A W + b)          (!AGG, K = 32, A = X)
+// One wavefront per 16-row tile, all 64 output columns; each workgroup owns a
+// contiguous, XCD-local range of tiles, so the rows a tile gathers from
+// (neighbours in the matrix ordering) are mostly in the same XCD's L2.  W^T
+// is staged once per workgroup in LDS.  Each lane gathers the M = S A
+// channels its own MFMA operand needs (row l%16, channels (l/16)*8 and
+// 32 + (l/16)*8), accumulating in FP32 in CSR order and rounding to BF16
+// exactly as a separate aggregation would, so the result equals aggregation
+// followed by a plain GEMM.  STORE_M keeps M for the backward pass.  The
+// 16 x 64 BF16 output tile is staged in LDS and written as 2 KB of contiguous
+// 16-byte stores.
+template <int K, bool AGG, bool STORE_M>
+__global__ __launch_bounds__(kBlock) void k_layer(int64_t n, const int32_t* __restrict__ ro,
+                                                  const int32_t* __restrict__ ci, const float* __restrict__ s,
+                                                  const __bf16* __restrict__ a, const __bf16* __restrict__ bt,
+                                                  const float* __restrict__ bias, __bf16* __restrict__ hout,
+                                                  __bf16* __restrict__ mout)
+{
+    static_assert((AGG && K == 2 * kHid) || (!AGG && K == kFeat), "layer shape");
+    constexpr int KS = K / 32;                 // MFMA K steps
+    constexpr int KA = AGG ? kHid : kFeat;     // width of a row of A
+    __shared__ __attribute__((aligned(16))) __bf16 wlds[kHid * K];
+    __shared__ __attribute__((aligned(16))) __bf16 stage[kBlock / kWave][16 * kHid];
+    for (int e = threadIdx.x * 8; e < kHid * K; e += kBlock * 8)
+        *reinterpret_cast<bf16x8*>(wlds + e) = *reinterpret_cast<const bf16x8*>(bt + e);
+    __syncthreads();
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    const int quad = lane / 16;
+    float bcol[kHid / 16];
+#pragma unroll
+    for (int j = 0; j < kHid / 16; ++j) bcol[j] = bias[j * 16 + lane % 16];
+    __bf16* st = stage[wave];
+    int64_t tb, te;
+    block_range(ceil_div(n, int64_t(16)), 1, tb, te);
+    for (int64_t t = tb + wave; t < te; t += kBlock / kWave)
+    {
+        const int64_t row0 = t * 16;
+        const int64_t row = row0 + lane % 16;
+        const bool valid = row < n;
+        bf16x8 av[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) av[ks][q] = to_bf(0.f);
+        if (valid)
+        {
+            av[0] = *reinterpret_cast<const bf16x8*>(a + row * KA + quad * 8);
+            if (AGG) av[1] = *reinterpret_cast<const bf16x8*>(a + row * KA + 32 + quad * 8);
+        }
+        if (AGG)
+        {
+            float m1[8], m2[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+            {
+                m1[q] = 0.f;
+                m2[q] = 0.f;
+            }
+            if (valid)
+            {
+                const int32_t k1 = ro[row + 1];
+                constexpr int U = 4;
+                // batches of U entries, all gathers of a batch in flight at
+                // once; entries past the row end and the diagonal are skipped
+                for (int32_t k = ro[row]; k < k1; k += U)
+                {
+                    int32_t jj[U];
+                    float sv[U];
+                    bf16x8 g1[U], g2[U];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                    {
+                        const bool in = k + u < k1;
+                        jj[u] = in ? ci[k + u] : static_cast<int32_t>(row);
+                        sv[u] = in ? s[k + u] : 0.f;
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                    {
+                        g1[u] = *reinterpret_cast<const bf16x8*>(a + static_cast<int64_t>(jj[u]) * kHid + quad * 8);
+                        g2[u] = *reinterpret_cast<const bf16x8*>(a + static_cast<int64_t>(jj[u]) * kHid + 32 + quad * 8);
+                    }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+                    {
+                        if (jj[u] == row) continue;
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                        {
+                            m1[q] = fmaf(sv[u], bf(g1[u][q]), m1[q]);
+                            m2[q] = fmaf(sv[u], bf(g2[u][q]), m2[q]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+            {
+                av[KS - 2][q] = to_bf(m1[q]);
+                av[KS - 1][q] = to_bf(m2[q]);
+            }
+            if (STORE_M && valid)
+            {
+                *reinterpret_cast<bf16x8*>(mout + row * kHid + quad * 8) = av[KS - 2];
+                *reinterpret_cast<bf16x8*>(mout + row * kHid + 32 + quad * 8) = av[KS - 1];
+            }
+        }
+        floatx4 acc[kHid / 16];
+#pragma unroll
+        for (int j = 0; j < kHid / 16; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+            for (int j = 0; j < kHid / 16; ++j)
+            {
+                const bf16x8 bv = *reinterpret_cast<const bf16x8*>(wlds + (j * 16 + lane % 16) * K + ks * 32 + quad * 8);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[ks], bv, acc[j], 0, 0, 0);
+            }
+#pragma unroll
+        for (int j = 0; j < kHid / 16; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                st[(quad * 4 + i) * kHid + j * 16 + lane % 16] = to_bf(fmaxf(acc[j][i] + bcol[j], 0.f));
+        wave_lds_sync();
+        // lane -> row lane/4 of the tile, 16 channels from (lane%4)*16
+        const int srow = lane / 4, scol = (lane % 4) * 16;
+        if (row0 + srow < n)
+        {
+            const bf16x8 o0 = *reinterpret_cast<const bf16x8*>(st + srow * kHid + scol);
+            const bf16x8 o1 = *reinterpret_cast<const bf16x8*>(st + srow * kHid + scol + 8);
+            *reinterpret_cast<bf16x8*>(hout + (row0 + srow) * kHid + scol) = o0;
+            *reinterpret_cast<bf16x8*>(hout + (row0 + srow) * kHid + scol + 8) = o1;
+        }
+        wave_lds_sync();
     }
 }
 
@@ -336,25 +469,6 @@ __global__ __launch_bounds__(kBlock) void k_reduce_parts(const float* __restrict
 
 // ---- graph kernels ----------------------------------------------------------------
 
-// M[i] = sum_{j != i} S_ij H[j] (64 channels, one wavefront per row, lane = channel).
-__global__ __launch_bounds__(kBlock) void k_aggregate(int64_t n, const int32_t* __restrict__ ro,
-                                                      const int32_t* __restrict__ ci, const float* __restrict__ s,
-                                                      const __bf16* __restrict__ h, __bf16* __restrict__ m)
-{
-    const int lane = threadIdx.x % kWave;
-    const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / kWave);
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; i < n; i += waves)
-    {
-        float acc = 0.f;
-        for (int32_t k = ro[i]; k < ro[i + 1]; ++k)
-        {
-            const int32_t j = ci[k];
-            if (j != i) acc = fmaf(s[k], bf(h[static_cast<int64_t>(j) * kHid + lane]), acc);
-        }
-        m[i * kHid + lane] = to_bf(acc);
-    }
-}
-
 // dH[i] += sum_j S_ij dM[j] (S symmetric), FP32 in and out.
 __global__ __launch_bounds__(kBlock) void k_aggregate_back(int64_t n, const int32_t* __restrict__ ro,
                                                            const int32_t* __restrict__ ci, const float* __restrict__ s,
@@ -377,8 +491,10 @@ __global__ __launch_bounds__(kBlock) void k_aggregate_back(int64_t n, const int3
 __device__ __forceinline__ float softplus(float x) { return x > 20.f ? x : log1pf(expf(x)); }
 __device__ __forceinline__ float sigmoid(float x) { return 1.f / (1.f + expf(-x)); }
 
-// Decoder: G on the lower pattern (one wavefront per row, lane = channel),
-// also the raw outputs for the backward pass.
+// Decoder: G on the lower pattern, also the raw outputs for the backward
+// pass.  Eight lanes per row, each holding 8 of the 64 channels of H_i (one
+// 16-byte load); an entry's 64-term dot product is 8 FP32 terms per lane in
+// channel order, then a 3-step butterfly over the row's 8 lanes.
 template <typename G>
 __global__ __launch_bounds__(kBlock) void k_decode(int64_t n, const int32_t* __restrict__ lro,
                                                    const int32_t* __restrict__ lci, const int32_t* __restrict__ lsrc,
@@ -386,28 +502,61 @@ __global__ __launch_bounds__(kBlock) void k_decode(int64_t n, const int32_t* __r
                                                    const __bf16* __restrict__ h, const float* __restrict__ params,
                                                    float* __restrict__ out, G* __restrict__ g)
 {
+    constexpr int kLanesPerRow = 8;
+    constexpr int kRowsPerWave = kWave / kLanesPerRow;
     const int lane = threadIdx.x % kWave;
-    const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / kWave);
-    const float w_off = params[kOffWoff + lane], w_diag = params[kOffWdiag + lane];
-    const float b_off = params[kOffBoff], alpha = params[kOffAlpha], b_diag = params[kOffBdiag];
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; i < n; i += waves)
+    const int sub = lane % kLanesPerRow;
+    float w_off[8], w_diag[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
     {
-        const float hi = bf(h[i * kHid + lane]);
-        for (int32_t k = lro[i]; k < lro[i + 1]; ++k)
+        w_off[q] = params[kOffWoff + sub * 8 + q];
+        w_diag[q] = params[kOffWdiag + sub * 8 + q];
+    }
+    const float b_off = params[kOffBoff], alpha = params[kOffAlpha], b_diag = params[kOffBdiag];
+    // contiguous XCD-local row range per workgroup; 32 rows per pass
+    constexpr int kRowsPerPass = (kBlock / kWave) * kRowsPerWave;
+    int64_t rb, re;
+    block_range(n, kRowsPerPass, rb, re);
+    for (int64_t i = rb + (threadIdx.x / kWave) * kRowsPerWave + lane / kLanesPerRow; i < re; i += kRowsPerPass)
+    {
+        const bf16x8 hv = *reinterpret_cast<const bf16x8*>(h + i * kHid + sub * 8);
+        float hi[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) hi[q] = bf(hv[q]);
+        const int32_t k1 = lro[i + 1];
+        for (int32_t k = lro[i]; k < k1; ++k)
         {
             const int32_t j = lci[k];
-            float o;
+            float part = 0.f;
             if (j == i)
             {
-                o = wave_sum(w_diag * hi) + b_diag;
-                if (lane == 0) g[k] = static_cast<G>(softplus(o) * dinv[i]);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) part = fmaf(w_diag[q], hi[q], part);
             }
             else
             {
-                o = wave_sum(w_off * hi * bf(h[static_cast<int64_t>(j) * kHid + lane])) + b_off + alpha * s[lsrc[k]];
-                if (lane == 0) g[k] = static_cast<G>(o * dinv[j]);
+                const bf16x8 hj = *reinterpret_cast<const bf16x8*>(h + static_cast<int64_t>(j) * kHid + sub * 8);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) part = fmaf(w_off[q] * hi[q], bf(hj[q]), part);
             }
-            if (lane == 0) out[k] = o;
+#pragma unroll
+            for (int off = kLanesPerRow / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, kLanesPerRow);
+            if (sub == 0)
+            {
+                float o;
+                if (j == i)
+                {
+                    o = part + b_diag;
+                    g[k] = static_cast<G>(softplus(o) * dinv[i]);
+                }
+                else
+                {
+                    o = part + b_off + alpha * s[lsrc[k]];
+                    g[k] = static_cast<G>(o * dinv[j]);
+                }
+                out[k] = o;
+            }
         }
     }
 }
@@ -902,22 +1051,25 @@ int graph_build(pm_fsai_graph* gr, const pm_csr* a, hipStream_t stream)
     return PM_OK;
 }
 
-int net_forward(pm_fsai_net* net, pm_fsai_graph* gr, const Act& a, hipStream_t stream)
+// keep_m stores every layer's aggregation M for the backward pass.
+int net_forward(pm_fsai_net* net, pm_fsai_graph* gr, const Act& a, hipStream_t stream, bool keep_m)
 {
     const int64_t n = gr->n;
-    const int gw = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(ceil_div(n, 16), 4), 8192)));
-    const int gr_rows = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 4), 16384)));
+    const int gw = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(ceil_div(n, 16), 4), 2048)));
+    const int gr_rows = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, 32), 16384)));
     const float* P = net->params.as<float>();
     const __bf16* wt = net->wt.as<__bf16>();
-    hipLaunchKernelGGL((k_gemm<kFeat, kHid, kEpiRelu>), dim3(gw), dim3(kBlock), 0, stream, n, gr->x, nullptr, wt,
-                       P + kOffBin, a.h[0], nullptr);
+    hipLaunchKernelGGL((k_layer<kFeat, false, false>), dim3(gw), dim3(kBlock), 0, stream, n, nullptr, nullptr, nullptr,
+                       gr->x, wt, P + kOffBin, a.h[0], nullptr);
     for (int l = 1; l <= kLayers; ++l)
     {
-        hipLaunchKernelGGL(k_aggregate, dim3(gr_rows), dim3(kBlock), 0, stream, n, gr->ro, gr->ci, gr->s, a.h[l - 1],
-                           a.m[l]);
         const __bf16* wtl = wt + kFeat * kHid + (l - 1) * 2 * kHid * kHid;
-        hipLaunchKernelGGL((k_gemm<2 * kHid, kHid, kEpiRelu>), dim3(gw), dim3(kBlock), 0, stream, n, a.h[l - 1],
-                           a.m[l], wtl, P + kOffBl + (l - 1) * kHid, a.h[l], nullptr);
+        if (keep_m)
+            hipLaunchKernelGGL((k_layer<2 * kHid, true, true>), dim3(gw), dim3(kBlock), 0, stream, n, gr->ro, gr->ci,
+                               gr->s, a.h[l - 1], wtl, P + kOffBl + (l - 1) * kHid, a.h[l], a.m[l]);
+        else
+            hipLaunchKernelGGL((k_layer<2 * kHid, true, false>), dim3(gw), dim3(kBlock), 0, stream, n, gr->ro, gr->ci,
+                               gr->s, a.h[l - 1], wtl, P + kOffBl + (l - 1) * kHid, a.h[l], nullptr);
     }
     if (gr->dtype == PM_F32)
         hipLaunchKernelGGL(k_decode<float>, dim3(gr_rows), dim3(kBlock), 0, stream, n, gr->lro, gr->lci, gr->lsrc,
@@ -986,8 +1138,7 @@ int net_loss_grad(pm_fsai_net* net, pm_fsai_graph* gr, const Act& a, int probes,
                            kPartBlocks, 2 * kHid * kHid, grad + kOffWl + (l - 1) * 2 * kHid * kHid);
         // [dH, dM] = dP W_l^T: the GEMM's B^T operand is W_l itself (2kHid x kHid row-major)
         const __bf16* wl = w + kFeat * kHid + (l - 1) * 2 * kHid * kHid;
-        hipLaunchKernelGGL((k_gemm<kHid, 2 * kHid, kEpiStoreF>), dim3(gw), dim3(kBlock), 0, stream, n, a.dp, nullptr,
-                           wl, nullptr, nullptr, a.y);
+        hipLaunchKernelGGL((k_gemm<kHid, 2 * kHid>), dim3(gw), dim3(kBlock), 0, stream, n, a.dp, nullptr, wl, a.y);
         hipLaunchKernelGGL(k_split_grad, dim3(grid_of(n * kHid)), dim3(kBlock), 0, stream, n, a.y, a.dh, a.dm);
         hipLaunchKernelGGL(k_aggregate_back, dim3(gr_rows), dim3(kBlock), 0, stream, n, gr->ro, gr->ci, gr->s, a.dm,
                            a.dh);
@@ -1150,7 +1301,7 @@ int pm_fsai_forward(pm_fsai_net* net, pm_fsai_graph* gr, hipStream_t stream)
     if (!net || !gr) return fail(PM_INVALID_ARGUMENT, "pm_fsai_forward: null argument");
     size_t bytes = 0;
     const Act a = carve_act(gr, &bytes);
-    return net_forward(net, gr, a, stream);
+    return net_forward(net, gr, a, stream, false);
 }
 
 int pm_fsai_loss_grad(pm_fsai_net* net, pm_fsai_graph* gr, int32_t probes, uint64_t seed, double* loss,
@@ -1160,7 +1311,7 @@ int pm_fsai_loss_grad(pm_fsai_net* net, pm_fsai_graph* gr, int32_t probes, uint6
     if (probes < 1 || probes > kMaxProbes) return fail(PM_INVALID_ARGUMENT, "probes must be in [1, %d]", kMaxProbes);
     size_t bytes = 0;
     const Act a = carve_act(gr, &bytes);
-    PM_TRY(net_forward(net, gr, a, stream));
+    PM_TRY(net_forward(net, gr, a, stream, true));
     PM_TRY(net_loss_grad(net, gr, a, probes, seed, stream));
     if (loss) PM_HIP(hipMemcpyAsync(loss, a.stat, sizeof(double), hipMemcpyDeviceToHost, stream));
     if (host_grad)
@@ -1178,7 +1329,7 @@ int pm_fsai_train_step(pm_fsai_net* net, pm_fsai_graph* gr, int32_t probes, uint
     if (!(lr > 0.f) || !std::isfinite(lr)) return fail(PM_INVALID_ARGUMENT, "learning rate must be positive");
     size_t bytes = 0;
     const Act a = carve_act(gr, &bytes);
-    PM_TRY(net_forward(net, gr, a, stream));
+    PM_TRY(net_forward(net, gr, a, stream, true));
     PM_TRY(net_loss_grad(net, gr, a, probes, seed, stream));
     net->steps += 1;
     const float b1 = 0.9f, b2 = 0.999f;

@@ -311,12 +311,27 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
 {
     __shared__ double lds[NV * 16];
     __shared__ double result[NV];
+    // every load is issued before the first add (kBlock threads, at most
+    // kMaxPartials / kBlock values each); the adds run in slot order
+    constexpr int kPer = kMaxPartials / kBlock;
+    double vals[NV][kPer];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+    {
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+        {
+            const int b = threadIdx.x + j * kBlock;
+            vals[k][j] = b < count ? partials[k * kMaxPartials + b] : 0.0;
+        }
+    }
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
     {
         double s = 0.0;
-        for (int b = threadIdx.x; b < count; b += blockDim.x) s += partials[k * kMaxPartials + b];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) s += vals[k][j];
         acc[k] = s;
     }
     block_sum<NV>(acc, lds);
