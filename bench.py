@@ -64,11 +64,16 @@ def committed_traffic(kernel: str):
     return None
 
 
+WORKLOADS = ("diffusion128", "poisson256", "lm-schur", "batch512", "learned-eval")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="diffusion128", choices=WORKLOADS,
+                    help="diffusion128 is the headline; the others measure the remaining solve-path workloads")
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--spmv-rounds", type=int, default=50)
     ap.add_argument("--cpu-iterations", type=int, default=150)
@@ -81,6 +86,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner solve")
     ap.add_argument("--weights", default=os.path.join(ROOT, "plamatrix_amd", "fsai_weights.npy"),
                     help="trained network parameters (float32 .npy written by plamatrix_amd.train_fsai)")
+    ap.add_argument("--eval-systems", type=int, default=256, help="learned-eval: held-out systems per kind")
     ap.add_argument("--profile", action="store_true",
                     help="short run for profilers: one solve capped at 2000 inner iterations, no CPU leg")
     args = ap.parse_args()
@@ -95,6 +101,21 @@ def main() -> int:
 
     from plamatrix_amd import capi, generators
 
+    run = {"diffusion128": run_diffusion, "poisson256": run_poisson, "lm-schur": run_schur,
+           "batch512": run_batch, "learned-eval": run_learned_eval}[args.workload]
+    line = run(args, torch, capi, generators)
+    print(json.dumps(line))
+    return 0
+
+
+def base_line(args, metric, value, unit, higher_is_better, dtype, config, roofline, cpu):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": value if unit == "ms" else None,
+            "higher_is_better": higher_is_better, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+
+
+def run_diffusion(args, torch, capi, generators):
     dev = torch.device("cuda:0")
     t_gen = time.perf_counter()
     h = generators.diffusion3d_csr(args.grid)
@@ -221,16 +242,20 @@ def main() -> int:
     if not args.no_cpu:
         import oracle
 
+        use_ref = oracle.ref_available()
         ref = oracle.CSR(n, n, h.row_offsets, h.col_indices, h.values)
         k = args.cpu_iterations
         c0 = time.perf_counter()
-        _, crep = oracle.pcg(ref, b_host.astype(np.float32), relative_tolerance=0.0, max_iterations=k)
+        _, crep = oracle.pcg(ref, b_host.astype(np.float32), relative_tolerance=0.0, max_iterations=k,
+                             use_ref=use_ref)
         cpu_s = time.perf_counter() - c0
         per_it = cpu_s / max(crep.iterations, 1)
-        cpu = {"value": round(per_it * iterations * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
-               "sample": f"{crep.iterations} FP64 Jacobi-PCG iterations of the oracle (oracle/sparse.c, serial, "
-                         f"restating iterative_solver_cpu.cpp) on the full 128^3 system took {cpu_s:.1f} s; "
-                         f"scaled to the {iterations} iterations the GPU solve needed"}
+        who = ("the reference's own CPU pcg (iterative_solver_cpu.cpp built from source, serial)" if use_ref
+               else "the oracle's restatement of iterative_solver_cpu.cpp (oracle/sparse.c, serial)")
+        cpu = {"value": round(per_it * iterations * 1e3, 1), "unit": "ms", "cores": 1,
+               "kind": "reference" if use_ref else "port",
+               "sample": f"{crep.iterations} FP64 Jacobi-PCG iterations of {who} on the full 128^3 system "
+                         f"took {cpu_s:.1f} s; scaled to the {iterations} iterations the GPU solve needed"}
 
     kernel = "k_spmv<float,int>" if lanes == 0 else f"k_spmv_lanes<float,int,{lanes}>"
     # headline: learned preconditioner with its Jacobi fallback (taken when the
@@ -274,9 +299,390 @@ def main() -> int:
         "spmv": {"gbs": round(spmv_gbs, 1), "ms": round(spmv_ms, 5), "lanes_candidates_ms": candidates},
         "setup_s": round(gen_s, 2),
     }
-    print(json.dumps(line))
-    return 0
+    return line
 
+
+
+def _median_ms(fn, repeats):
+    """Median wall time of fn() in ms, device-synchronised around each call."""
+    import torch
+
+    times = []
+    out = None
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+    times.sort()
+    return times[len(times) // 2], out
+
+
+def _event_ms(torch, fn, repeats):
+    """Mean device time of fn() in ms from HIP events on the current stream."""
+    fn()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(repeats):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / repeats
+
+
+def _cpu_median_s(fn, repeats):
+    times = []
+    out = None
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        out = fn()
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    return times[len(times) // 2], out
+
+
+def run_poisson(args, torch, capi, generators):
+    """2-D 5-point Poisson 256^2 from shuffled COO triplets with 10% split
+    duplicates: device COO->CSR (steady state, output preallocated) and FP64
+    Jacobi-PCG to relative 1e-8 (max 2000) from x = 0, RHS U[-1,1] seed 1."""
+    dev = torch.device("cuda:0")
+    coo = generators.poisson2d_coo(256)
+    n, t_count = coo.rows, len(coo.values)
+    r = torch.from_numpy(coo.row_indices).to(dev)
+    c = torch.from_numpy(coo.col_indices).to(dev)
+    v = torch.from_numpy(coo.values).to(dev)
+    ws = capi.Workspace()
+    a = capi.coo_to_csr(n, n, r, c, v, ws)
+    nnz = a.nnz
+    def convert():
+        # a workspace's deferred status must be read before its next conversion
+        capi.coo_to_csr_async(n, n, r, c, v, a, ws)
+        torch.cuda.synchronize()
+        ws.check_status()
+
+    conv_ms, _ = _median_ms(convert, 20)
+    conv_bytes = t_count * (8 + 8 + 8) + nnz * (8 + 8) + (n + 1) * 8
+    sync_ms, _ = _median_ms(lambda: capi.coo_to_csr(n, n, r, c, v, ws), 5)
+    rhs = generators.rhs_uniform(n, seed=1)
+    b = torch.from_numpy(rhs).to(dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+
+    def solve():
+        x.zero_()
+        return capi.pcg(a, b, x, ws, relative_tolerance=1e-8, max_iterations=2000)
+
+    for _ in range(args.warmup):
+        solve()
+    solve_ms, rep = _median_ms(solve, max(args.steps, 3))
+    relres = capi.true_relres(a, b, x, ws)
+    it_bytes = pcg_iteration_bytes(n, nnz, value_bytes=8, index_bytes=4 if a.row_offsets32 is not None else 8)
+    it_ms = solve_ms / max(rep.iterations, 1)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        use_ref = oracle.ref_available()
+        c_conv, ref_csr = _cpu_median_s(lambda: oracle.coo_to_csr(n, n, coo.row_indices, coo.col_indices,
+                                                                   coo.values, use_ref=use_ref), 5)
+        c_solve, (_, crep) = _cpu_median_s(lambda: oracle.pcg(ref_csr, rhs, relative_tolerance=1e-8,
+                                                               max_iterations=2000, use_ref=use_ref), 3)
+        who = "the reference's own CPU cooToCsr and pcg (built from source)" if use_ref else "the oracle"
+        cpu = {"value": round(c_solve * 1e3, 2), "unit": "ms", "cores": 1, "kind": "reference" if use_ref else "port",
+               "sample": f"full workload, serial, median of 3 ({who}): PCG {crep.iterations} iterations "
+                         f"{c_solve * 1e3:.1f} ms; COO->CSR median of 5: {c_conv * 1e3:.2f} ms"}
+    line = base_line(args, "FP64 Jacobi-PCG time-to-solve at relres 1e-8, 2-D Poisson 256^2", round(solve_ms, 3),
+                     "ms", False, "f64",
+                     {"workload": f"2-D 5-point Poisson 256^2 (n={n}, nnz={nnz}) from {t_count} shuffled COO "
+                                  "triplets with 10% split duplicates; device COO->CSR, FP64 Jacobi-PCG from x=0 "
+                                  "to relres 1e-8, max 2000", "rows": n, "nnz": nnz, "triplets": t_count},
+                     {"bound": "hbm", "achieved": round(it_bytes / (it_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": None, "kernel": "PCG iteration (k_direction + k_spmv_den + k_update)",
+                      "bytes_per_launch": it_bytes, "ms_per_launch": round(it_ms, 5),
+                      "note": "latency-bound: 5.2 MB per iteration"},
+                     cpu)
+    line["solve"] = {"converged": rep.converged, "iterations": rep.iterations, "true_relres": relres}
+    line["coo_to_csr"] = {"ms_steady": round(conv_ms, 4), "ms_synchronous": round(sync_ms, 4),
+                          "bytes": conv_bytes, "gbs": round(conv_bytes / (conv_ms * 1e-3) / 1e9, 1)}
+    return line
+
+
+def _schur_assembly_bytes(inst, slots, values, terms):
+    """Algorithmic bytes of the Schur numeric assembly: W, C, B read, the reduced
+    values written, three int64 indices per term."""
+    w_count = len(inst.primary)
+    return (w_count * 18 * 8 + inst.eliminated_count * 9 * 8 + inst.primary_count * 36 * 8 + values * 8
+            + terms * 3 * 8)
+
+
+def run_schur(args, torch, capi, generators):
+    """Synthetic LM / Schur system (2,000 primaries x 6, 200,000 secondaries x 3,
+    1.2M residuals of dimension 2, Huber delta 1, lambda 1e-3): device normal
+    equations from the Jacobians, reduced Schur assembly over the reused
+    pattern, FP64 block-Jacobi PCG to relative 1e-6, back-substitution."""
+    dev = torch.device("cuda:0")
+    inst = generators.lm_schur_instance()
+    t = lambda arr, dt: torch.from_numpy(np.ascontiguousarray(arr, dtype=dt)).to(dev)
+    batch = dict(primary=t(inst.primary, np.int64), eliminated=t(inst.eliminated, np.int64),
+                 primary_jacobians=t(inst.primary_jacobians, np.float64),
+                 eliminated_jacobians=t(inst.eliminated_jacobians, np.float64),
+                 residuals=t(inst.residuals, np.float64))
+    pc, ec = inst.primary_count, inst.eliminated_count
+    sw = capi.SchurWorkspace()
+    ws = capi.Workspace()
+    dp = torch.zeros(pc * 6, dtype=torch.float64, device=dev)
+    de = torch.zeros(ec * 3, dtype=torch.float64, device=dev)
+
+    def step():
+        ne = capi.assemble_normal_equations(sw, pc, ec, 6, 3, **batch, weights=None, huber_delta=inst.huber_delta)
+        rep = capi.schur_solve(sw, ne, inst.damping, dp, de, ws, max_iterations=2000, relative_tolerance=1e-6,
+                               absolute_tolerance=0.0)
+        return ne, rep
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ne, first = step()  # builds the pattern
+    torch.cuda.synchronize()
+    first_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(args.warmup):
+        step()
+    ms, (ne, rep) = _median_ms(step, max(args.steps, 3))
+    if not rep.converged:
+        raise RuntimeError(rep.message)
+    slots, values, terms = sw.pattern_size()
+    ne_ms = _event_ms(torch, lambda: capi.assemble_normal_equations(
+        sw, pc, ec, 6, 3, **batch, weights=None, huber_delta=inst.huber_delta), 5)
+    asm_ms = _event_ms(torch, lambda: capi.schur_assemble(sw, ne, inst.damping), 5)
+    asm_bytes = _schur_assembly_bytes(inst, slots, values, terms)
+    r_count = len(inst.primary)
+    ne_bytes = r_count * (12 + 6 + 2) * 8 + r_count * 2 * 8
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        if oracle.ref_available():
+            w = generators.huber_weights(inst.residuals, inst.huber_delta)
+            c_s, (_, _, crep) = _cpu_median_s(lambda: oracle.ref_schur_solve_batch(
+                pc, ec, 6, 3, inst.primary, inst.eliminated, inst.primary_jacobians, inst.eliminated_jacobians,
+                inst.residuals, w, inst.damping, backend=0, max_iterations=2000, relative_tolerance=1e-6,
+                absolute_tolerance=0.0), 3)
+            cpu = {"value": round(c_s * 1e3, 1), "unit": "ms", "cores": 1, "kind": "reference",
+                   "sample": f"full workload, median of 3: the reference's own solveDampedSchurComplement with "
+                             f"the CPU linear backend (built from source, serial) from the same residual batch "
+                             f"and Huber weights, {crep.iterations} block-Jacobi PCG iterations"}
+    line = base_line(args, "LM Schur step time (device normal equations + reduced Schur assembly + block-Jacobi "
+                           "PCG to relres 1e-6 + back-substitution)", round(ms, 3), "ms", False, "f64",
+                     {"workload": f"LM/Schur: {pc} primaries x 6, {ec} secondaries x 3, {r_count} residuals x 2, "
+                                  "Huber delta 1, lambda 1e-3; reduced 12,000 x 12,000 system in 6x6 blocks, "
+                                  "pattern reused", "slots": slots, "values": values, "terms": terms},
+                     {"bound": "hbm", "achieved": round(asm_bytes / (asm_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                      "unit": "GB/s", "frac": round(asm_bytes / (asm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "traffic": None, "kernel": "reduced Schur numeric assembly (pm_schur_assemble)",
+                      "bytes_per_launch": asm_bytes, "ms_per_launch": round(asm_ms, 4)},
+                     cpu)
+    line["solve"] = {"converged": rep.converged, "iterations": rep.iterations,
+                     "final_residual_norm": rep.final_residual_norm, "initial_residual_norm": rep.initial_residual_norm,
+                     "first_call_ms_with_pattern_build": round(first_ms, 2)}
+    line["normal_equations"] = {"ms": round(ne_ms, 4), "bytes_read": ne_bytes,
+                                "gbs": round(ne_bytes / (ne_ms * 1e-3) / 1e9, 1)}
+    return line
+
+
+def _batch_systems(generators, capi, torch, count=512, seed=4):
+    """The batch: sizes log-uniform in [1e3, 2e5] (seed 4), members cycling 2-D
+    Laplacian / 3-D diffusion / reduced LM Schur system (primaries = n/6,
+    3 secondaries per primary, assembled on the device).  Returns the
+    block-diagonal device CSR (FP32, int32), row ranges and member kinds."""
+    dev = torch.device("cuda:0")
+    sizes = generators.batch_sizes(count, seed=seed)
+    kinds = ["laplace2d", "diffusion3d", "schur"]
+    members = []
+    for i, n in enumerate(sizes):
+        kind = kinds[i % 3]
+        if kind == "laplace2d":
+            h = generators.laplacian2d_csr(*generators.grid2d_dims(int(n)), dtype=np.float32)
+            members.append(("host", h))
+        elif kind == "diffusion3d":
+            h = generators.diffusion3d_box_csr(*generators.grid3d_dims(int(n)), seed=100 + i, dtype=np.float32)
+            members.append(("host", h))
+        else:
+            pc = max(int(n) // 6, 20)
+            inst = generators.lm_schur_instance(primary_count=pc, eliminated_count=3 * pc, seed=5000 + i)
+            t = lambda arr, dt: torch.from_numpy(np.ascontiguousarray(arr, dtype=dt)).to(dev)
+            sw = capi.SchurWorkspace()
+            ne = capi.assemble_normal_equations(sw, pc, 3 * pc, 6, 3, t(inst.primary, np.int64),
+                                                t(inst.eliminated, np.int64), t(inst.primary_jacobians, np.float64),
+                                                t(inst.eliminated_jacobians, np.float64),
+                                                t(inst.residuals, np.float64), weights=None,
+                                                huber_delta=inst.huber_delta)
+            rep = capi.schur_assemble(sw, ne, inst.damping)
+            if rep.status != 0:
+                raise RuntimeError(rep.message)
+            a, _, _ = capi.schur_system(sw)
+            rows = a.rows
+            ro = torch.empty(rows + 1, dtype=torch.int64, device=dev)
+            ci = torch.empty(a.nnz, dtype=torch.int64, device=dev)
+            va = torch.empty(a.nnz, dtype=torch.float64, device=dev)
+            capi.lib_memcpy(ro.data_ptr(), a.row_offsets, ro.numel() * 8)
+            capi.lib_memcpy(ci.data_ptr(), a.col_indices, ci.numel() * 8)
+            capi.lib_memcpy(va.data_ptr(), a.values, va.numel() * 8)
+            members.append(("device", (rows, ro, ci, va.to(torch.float32))))
+            del sw, ne
+    lens = [m[1].rows if m[0] == "host" else m[1][0] for m in members]
+    begin, total = capi.batch_layout(lens)
+    nnzs = [int(m[1].nnz) if m[0] == "host" else int(m[1][2].numel()) for m in members]
+    nnz_total = int(sum(nnzs))
+    ro_all = torch.zeros(total + 1, dtype=torch.int32, device=dev)
+    ci_all = torch.empty(nnz_total, dtype=torch.int32, device=dev)
+    va_all = torch.empty(nnz_total, dtype=torch.float32, device=dev)
+    counts = torch.zeros(total, dtype=torch.int64, device=dev)
+    k = 0
+    for (where, m), b0, ln, z in zip(members, begin, lens, nnzs):
+        if where == "host":
+            ro = torch.from_numpy(m.row_offsets).to(dev)
+            ci = torch.from_numpy(m.col_indices).to(dev)
+            va = torch.from_numpy(m.values.astype(np.float32)).to(dev)
+        else:
+            _, ro, ci, va = m
+        counts[b0:b0 + ln] = ro[1:] - ro[:-1]
+        ci_all[k:k + z] = (ci + int(b0)).to(torch.int32)
+        va_all[k:k + z] = va
+        k += z
+    # rows between systems (64-row alignment padding) are empty
+    ro_all[1:] = torch.cumsum(counts, 0).to(torch.int32)
+    a = capi.DeviceCSR(total, total, None, None, va_all, ro_all, ci_all)
+    end = begin + np.asarray(lens, dtype=np.int64)
+    return a, begin, end, [kinds[i % 3] for i in range(count)], lens, nnzs, members
+
+
+def run_batch(args, torch, capi, generators):
+    """512 uneven SPD systems in one batched solve: per-system FP32 SpMV with
+    FP64 dots, per-system relative tolerance 1e-6, device-side convergence
+    flags and graph-replayed iterations."""
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    a, begin, end, kinds, lens, nnzs, members = _batch_systems(generators, capi, torch)
+    gen_s = time.perf_counter() - t0
+    total = a.rows
+    rng = np.random.default_rng(7)
+    b_host = np.zeros(total, dtype=np.float32)
+    for b0, ln in zip(begin, lens):
+        b_host[b0:b0 + ln] = rng.standard_normal(ln)
+    b = torch.from_numpy(b_host).to(dev)
+    x = torch.zeros(total, dtype=torch.float32, device=dev)
+    ws = capi.Workspace()
+
+    def solve():
+        x.zero_()
+        return capi.batch_pcg(a, begin, end, b, x, ws, relative_tolerance=1e-6, max_iterations=20000)
+
+    for _ in range(args.warmup):
+        solve()
+    ms, reps = _median_ms(solve, max(args.steps, 3))
+    its = np.array([r.iterations for r in reps])
+    conv = sum(r.converged for r in reps)
+    # per-system FP64 true residuals (host, FP64) of the last solve
+    xh = x.cpu().numpy().astype(np.float64)
+    import scipy.sparse as sp
+
+    worst = 0.0
+    for (where, m), b0, ln in zip(members, begin, lens):
+        if where == "host":
+            A = sp.csr_matrix((m.values.astype(np.float64), m.col_indices, m.row_offsets), shape=(ln, ln))
+        else:
+            _, ro, ci, va = m
+            A = sp.csr_matrix((va.double().cpu().numpy(), ci.cpu().numpy(), ro.cpu().numpy()), shape=(ln, ln))
+        bb = b_host[b0:b0 + ln].astype(np.float64)
+        worst = max(worst, float(np.linalg.norm(bb - A @ xh[b0:b0 + ln]) / np.linalg.norm(bb)))
+    # algorithmic bytes actually streamed: each system's iteration traffic times its iterations
+    streamed = sum(pcg_iteration_bytes(ln, z) * int(it) for ln, z, it in zip(lens, nnzs, its))
+    gbs = streamed / (ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        use_ref = oracle.ref_available()
+        sample = list(range(0, len(lens), 32))
+        c_s = 0.0
+        work_sample = 0.0
+        for i in sample:
+            where, m = members[i]
+            if where == "host":
+                csr = oracle.CSR(lens[i], lens[i], m.row_offsets, m.col_indices, m.values.astype(np.float64))
+            else:
+                _, ro, ci, va = m
+                csr = oracle.CSR(lens[i], lens[i], ro.cpu().numpy(), ci.cpu().numpy(), va.double().cpu().numpy())
+            rhs = b_host[begin[i]:begin[i] + lens[i]].astype(np.float64)
+            c0 = time.perf_counter()
+            _, crep = oracle.pcg(csr, rhs, relative_tolerance=1e-6, max_iterations=20000, use_ref=use_ref)
+            c_s += time.perf_counter() - c0
+            work_sample += nnzs[i] * max(crep.iterations, 1)
+        work_all = float(sum(z * max(int(it), 1) for z, it in zip(nnzs, its)))
+        who = "the reference's own CPU pcg (built from source)" if use_ref else "the oracle's CPU pcg"
+        cpu = {"value": round(c_s * work_all / work_sample * 1e3, 1), "unit": "ms", "cores": 1,
+               "kind": "reference" if use_ref else "port",
+               "sample": f"{len(sample)} of the 512 systems (every 32nd) solved one by one, serial FP64 Jacobi-PCG "
+                         f"to 1e-6 by {who}, {c_s:.1f} s; scaled by sum(nnz x iterations) over all systems"}
+    line = base_line(args, "Batched PCG time-to-solve, 512 uneven systems at per-system relres 1e-6", round(ms, 3),
+                     "ms", False, "f32",
+                     {"workload": f"512 systems, n log-uniform in [1e3, 2e5] (seed 4), cycling 2-D Laplacian / 3-D "
+                                  f"diffusion / reduced LM Schur; {total} rows (64-aligned), {int(sum(nnzs))} nnz; "
+                                  "Jacobi-PCG FP32 SpMV + FP64 dots, one batched solve",
+                      "rows": int(sum(lens)), "nnz": int(sum(nnzs)), "systems": len(lens)},
+                     {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                      "kernel": "batched PCG iteration (sum over systems of iteration bytes x iterations / solve time)",
+                      "bytes_per_launch": int(streamed), "ms_per_launch": round(ms, 3)},
+                     cpu)
+    line["solve"] = {"converged": int(conv), "systems": len(reps), "iterations_max": int(its.max()),
+                     "iterations_mean": round(float(its.mean()), 1), "worst_true_relres": worst,
+                     "setup_s": round(gen_s, 1)}
+    return line
+
+
+def run_learned_eval(args, torch, capi, generators):
+    """Held-out evaluation of the trained network: scaled-down 3-D diffusion
+    boxes and reduced LM Schur systems (eval seeds 2*10^6 + i), Jacobi-PCG vs
+    learned G^T G PCG to relres 1e-6 (FP32), graph build + inference included
+    in the learned time, Jacobi fallback when the learned solve is not faster."""
+    from plamatrix_amd import train_fsai
+
+    net = capi.FsaiNet(seed=0)
+    net.parameters = np.load(args.weights)
+    out = {}
+    for kind in ("diffusion", "schur"):
+        systems = [train_fsai.make_system(kind, 2_000_000 + i + (50_000 if kind == "schur" else 0))
+                   for i in range(args.eval_systems)]
+        res = train_fsai.evaluate(net, systems)
+        jt = np.array([r["jacobi"]["ms"] for r in res])
+        lt = np.array([r["learned"]["ms"] for r in res])
+        ji = np.array([r["jacobi"]["iterations"] for r in res])
+        li = np.array([r["learned"]["iterations"] for r in res])
+        out[kind] = {"systems": len(res), "jacobi_ms_mean": round(float(jt.mean()), 4),
+                     "learned_ms_mean": round(float(lt.mean()), 4),
+                     "time_ratio": round(float(lt.sum() / jt.sum()), 4),
+                     "jacobi_iterations_mean": round(float(ji.mean()), 2),
+                     "learned_iterations_mean": round(float(li.mean()), 2),
+                     "iteration_ratio": round(float(li.sum() / ji.sum()), 4),
+                     "slower_than_jacobi_fraction": round(float((lt >= jt).mean()), 4),
+                     "more_iterations_than_jacobi_fraction": round(float((li >= ji).mean()), 4),
+                     "worst_learned_relres": max(r["learned"]["relres"] for r in res),
+                     "worst_jacobi_relres": max(r["jacobi"]["relres"] for r in res),
+                     "all_converged": bool(all(r["learned"]["converged"] and r["jacobi"]["converged"] for r in res))}
+    total_j = sum(out[k]["jacobi_ms_mean"] * out[k]["systems"] for k in out)
+    total_l = sum(out[k]["learned_ms_mean"] * out[k]["systems"] for k in out)
+    line = base_line(args, "Learned vs Jacobi PCG wall-time ratio on held-out systems at relres 1e-6",
+                     round(total_l / total_j, 4), "ratio", False, "f32",
+                     {"workload": f"{args.eval_systems} held-out scaled-down 3-D diffusion boxes (16..33 per side) "
+                                  f"and {args.eval_systems} reduced LM Schur systems (300 primaries, 6,000 "
+                                  "secondaries), eval seeds 2*10^6+i, FP32 PCG to relres 1e-6",
+                      "weights": os.path.relpath(args.weights, ROOT)},
+                     None, None)
+    line["per_kind"] = out
+    return line
 
 if __name__ == "__main__":
     sys.exit(main())
