@@ -239,6 +239,27 @@ int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, con
                  const void* rhs, void* x, const pm_solver_options* options, const pm_solve_policy* policy,
                  pm_batch_report* reports, pm_workspace* ws, hipStream_t stream);
 
+/* Batched FP64 iterative refinement (no reference counterpart): for every
+ * system, r = b - A x with FP64 accumulation over FP64 x; systems above
+ * target_relres (||r|| / ||b||) solve A d = r / ||r|| with the FP32 batched
+ * PCG under `inner` (its relative tolerance and iteration limit; absolute
+ * tolerance ignored) and take x += ||r|| d, for at most max_outer steps.
+ * a is FP32; rhs and x are FP64 and x is the initial guess.  Synchronises
+ * `stream` once per outer step. */
+typedef struct pm_batch_refine_report {
+    int32_t converged;          /* final relres <= target */
+    int32_t outer_iterations;   /* outer steps taken */
+    int64_t inner_iterations;   /* FP32 PCG iterations over all outer steps */
+    double initial_relres;
+    double final_relres;        /* FP64 ||b - A x|| / ||b|| at exit */
+    int32_t last_inner_status;  /* status of the last inner solve (pm_batch_report codes), 0 if none */
+    int32_t pad;
+} pm_batch_refine_report;
+int pm_batch_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                        const double* rhs, double* x, const pm_solver_options* inner, double target_relres,
+                        int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
+                        pm_workspace* ws, hipStream_t stream);
+
 /* ---- Factored preconditioner M^-1 = G^T G ----------------------------------------
  * PCG with z = G^T (G r) (no reference counterpart).  g is lower-triangular
  * CSR, gt its transpose; both need 32-bit index mirrors and a's dtype.  Same

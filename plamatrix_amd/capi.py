@@ -59,6 +59,11 @@ class pm_batch_report(ctypes.Structure):
                 ("status", _i32), ("pad", _i32), ("bad_row", _i64)]
 
 
+class pm_batch_refine_report(ctypes.Structure):
+    _fields_ = [("converged", _i32), ("outer_iterations", _i32), ("inner_iterations", _i64),
+                ("initial_relres", _f64), ("final_relres", _f64), ("last_inner_status", _i32), ("pad", _i32)]
+
+
 class pm_normal_equations(ctypes.Structure):
     _fields_ = [("primary_count", _i64), ("eliminated_count", _i64), ("primary_size", _i64),
                 ("eliminated_size", _i64), ("primary_diagonal", _p), ("eliminated_diagonal", _p),
@@ -95,7 +100,7 @@ EXPORTS = [
     "pm_block_pcg", "pm_solve_fixed_async", "pm_fixed_finalize", "pm_fixed_state_destroy", "pm_pcg_refine",
     "pm_schur_workspace_create", "pm_schur_workspace_destroy", "pm_schur_pattern_build_count",
     "pm_schur_pattern_size", "pm_normal_equations_assemble", "pm_schur_solve", "pm_schur_assemble", "pm_schur_system",
-    "pm_batch_layout", "pm_batch_pcg", "pm_pcg_factored", "pm_pcg_refine_factored",
+    "pm_batch_layout", "pm_batch_pcg", "pm_batch_pcg_refine", "pm_pcg_factored", "pm_pcg_refine_factored",
     "pm_fsai_parameter_count", "pm_fsai_initial_parameters", "pm_fsai_net_create", "pm_fsai_net_destroy",
     "pm_fsai_net_set_parameters", "pm_fsai_net_get_parameters", "pm_fsai_graph_create", "pm_fsai_graph_destroy",
     "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step",
@@ -583,6 +588,36 @@ def batch_pcg(a: "DeviceCSR", row_begin, row_end, b, x, ws: Workspace, pol=None,
                              ws.handle, current_stream()))
     return [BatchReport(bool(r.converged), r.iterations, r.initial_residual, r.final_residual, r.status, r.bad_row)
             for r in reps[:systems]]
+
+
+@dataclass
+class BatchRefineReport:
+    converged: bool
+    outer_iterations: int
+    inner_iterations: int
+    initial_relres: float
+    final_relres: float
+    last_inner_status: int
+
+
+def batch_pcg_refine(a: "DeviceCSR", row_begin, row_end, b64, x64, ws: Workspace, target=1e-6, max_outer=10,
+                     pol=None, **inner):
+    """Batched FP64 refinement: FP32 batched PCG inner solves, FP64 x and
+    residuals; one report per system."""
+    import numpy as np
+
+    d = a.descriptor()
+    o = options(**inner)
+    p = pol if pol is not None else policy()
+    row_begin = np.ascontiguousarray(row_begin, dtype=np.int64)
+    row_end = np.ascontiguousarray(row_end, dtype=np.int64)
+    systems = len(row_begin)
+    reps = (pm_batch_refine_report * max(systems, 1))()
+    check(lib().pm_batch_pcg_refine(ctypes.byref(d), _i64(systems), row_begin.ctypes.data_as(_p),
+                                    row_end.ctypes.data_as(_p), _ptr(b64), _ptr(x64), ctypes.byref(o), _f64(target),
+                                    _i32(max_outer), ctypes.byref(p), reps, ws.handle, current_stream()))
+    return [BatchRefineReport(bool(r.converged), r.outer_iterations, r.inner_iterations, r.initial_relres,
+                              r.final_relres, r.last_inner_status) for r in reps[:systems]]
 
 
 # ---- learned factorized preconditioner ---------------------------------------------

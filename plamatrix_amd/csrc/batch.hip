@@ -539,33 +539,15 @@ int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* rep
     return PM_OK;
 }
 
-} // namespace pm
-
-using namespace pm;
-
-extern "C" {
-
-int pm_batch_layout(int64_t systems, const int64_t* sizes, int64_t* row_begin, int64_t* total_rows)
-{
-    if (systems < 0 || (systems > 0 && (!sizes || !row_begin)) || !total_rows)
-        return fail(PM_INVALID_ARGUMENT, "pm_batch_layout: invalid arguments");
-    int64_t cursor = 0;
-    for (int64_t s = 0; s < systems; ++s)
-    {
-        if (sizes[s] < 0) return fail(PM_INVALID_ARGUMENT, "pm_batch_layout: negative system size");
-        row_begin[s] = cursor;
-        cursor += ceil_div(sizes[s], kChunkRows) * kChunkRows;
-    }
-    *total_rows = cursor;
-    return PM_OK;
-}
-
-int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
-                 const void* rhs, void* x, const pm_solver_options* options, const pm_solve_policy* policy,
-                 pm_batch_report* reports, pm_workspace* ws, hipStream_t stream)
+// Validation, host layout (per-chunk system ids, per-system row end and
+// chunk range, copied to the device) and workspace carving shared by the
+// batched solve and its refinement loop.
+int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end, const void* rhs,
+                void* x, const pm_solver_options* options, bool have_reports, pm_workspace* ws, hipStream_t stream,
+                BatchProblem& b)
 {
     PM_TRY(check_csr(a, "batched pcg"));
-    if (!options || !ws || !reports || systems < 0 || (systems > 0 && (!row_begin || !row_end)))
+    if (!options || !ws || !have_reports || systems < 0 || (systems > 0 && (!row_begin || !row_end)))
         return fail(PM_INVALID_ARGUMENT, "batched pcg: null argument");
     if (a->rows != a->cols) return fail(PM_INVALID_ARGUMENT, "batched pcg: matrix must be square");
     if (options->max_iterations < 0 || !std::isfinite(options->relative_tolerance) ||
@@ -612,7 +594,7 @@ int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, con
     if (!ws->host_batch)
         PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_batch), 2 * sizeof(unsigned long long),
                              hipHostMallocDefault));
-    BatchProblem b{};
+    b = BatchProblem{};
     b.a = a;
     b.rhs = rhs;
     b.x = x;
@@ -653,10 +635,294 @@ int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, con
     if (chunks > 0)
         PM_HIP(hipMemcpy(b.chunk_sys, chunk_sys.data(), sizeof(int32_t) * chunks, hipMemcpyHostToDevice));
     if (systems > 0) PM_HIP(hipMemcpy(b.sys_end, sys.data(), sizeof(int64_t) * 3 * systems, hipMemcpyHostToDevice));
+    return PM_OK;
+}
+
+// ---- batched FP64 iterative refinement ----------------------------------------------
+//
+// Per outer step: r = b - A x with FP64 accumulation over FP64 x (per-chunk
+// partial r.r and b.b), one wavefront per system reduces them in chunk order
+// and decides converged (relres <= target), out of outer steps, or active with
+// scale 1/||r||; the active systems' scaled residuals are solved by the FP32
+// batched PCG (inactive systems get a zero right-hand side and stop at its
+// initialisation) and x += ||r|| d.  The host reads the per-system records
+// once per outer step.
+
+struct BatchRefineState
+{
+    double norm;
+    double initial;
+    double relres;
+    int32_t active;
+    int32_t converged;
+    int32_t outer;
+    int32_t pad;
+};
+
+template <typename I>
+__global__ __launch_bounds__(kBlock) void k_batch_residual64(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                             const int64_t* __restrict__ sys_end,
+                                                             const I* __restrict__ ro, const I* __restrict__ ci,
+                                                             const float* __restrict__ v,
+                                                             const double* __restrict__ b,
+                                                             const double* __restrict__ x, double* __restrict__ r,
+                                                             const BatchRefineState* __restrict__ rs,
+                                                             double* __restrict__ prr, double* __restrict__ pbb)
+{
+    __shared__ float lv_all[kBatchWaves][kStreamCap];
+    __shared__ I lc_all[kBatchWaves][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || !rs[s].active)
+        {
+            if (lane == 0) prr[c] = pbb[c] = 0.0;
+            continue;
+        }
+        const int64_t end = sys_end[s];
+        const int64_t row0 = c * kChunkRows;
+        int64_t ks, ke;
+        const double acc = chunk_row_sums<double>(row0, end, ro, ci, lv_all[wave], lc_all[wave],
+                                                  Fp64Source<float, double>{v, x}, ks, ke);
+        const int64_t row = row0 + lane;
+        double rr = 0.0, bb = 0.0;
+        if (row < end)
+        {
+            const double bi = b[row];
+            const double ri = bi - acc;
+            r[row] = ri;
+            rr = ri * ri;
+            bb = bi * bi;
+        }
+        rr = wave_sum(rr);
+        bb = wave_sum(bb);
+        if (lane == 0)
+        {
+            prr[c] = rr;
+            pbb[c] = bb;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_refine_init(int64_t systems, BatchRefineState* rs)
+{
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < systems;
+         s += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    {
+        BatchRefineState w{};
+        w.active = 1;
+        rs[s] = w;
+    }
+}
+
+// One wavefront per active system: relative residual, stop decision, scale.
+__global__ __launch_bounds__(kBlock) void k_batch_refine_decide(int64_t systems, const int64_t* __restrict__ sys_chunk0,
+                                                                const int64_t* __restrict__ sys_chunk1,
+                                                                const double* __restrict__ prr,
+                                                                const double* __restrict__ pbb, BatchRefineState* rs,
+                                                                int32_t outer, double target, int32_t max_outer)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; s < systems;
+         s += waves)
+    {
+        if (!rs[s].active) continue;
+        double rr = 0.0, bb = 0.0;
+        for (int64_t c = sys_chunk0[s] + lane; c < sys_chunk1[s]; c += kWave)
+        {
+            rr += prr[c];
+            bb += pbb[c];
+        }
+        rr = wave_sum(rr);
+        bb = wave_sum(bb);
+        if (lane != 0) continue;
+        BatchRefineState w = rs[s];
+        const double relres = bb > 0.0 ? sqrt(rr / bb) : sqrt(rr);
+        if (outer == 0) w.initial = relres;
+        w.relres = relres;
+        w.outer = outer;
+        if (!isfinite(rr)) w.active = 0;
+        else if (relres <= target || rr == 0.0)
+        {
+            w.active = 0;
+            w.converged = 1;
+        }
+        else if (outer >= max_outer) w.active = 0;
+        else w.norm = sqrt(rr);
+        rs[s] = w;
+    }
+}
+
+// Active systems: rf = r / ||r||, d = 0; others: rf = 0, d = 0.
+__global__ __launch_bounds__(kBlock) void k_batch_refine_scale(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                               const int64_t* __restrict__ sys_end,
+                                                               const double* __restrict__ r,
+                                                               const BatchRefineState* __restrict__ rs,
+                                                               float* __restrict__ rf, float* __restrict__ d)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        const int64_t row = c * kChunkRows + lane;
+        const bool act = s >= 0 && rs[s].active && row < sys_end[s];
+        rf[row] = act ? static_cast<float>(r[row] / rs[s].norm) : 0.f;
+        d[row] = 0.f;
+    }
+}
+
+// Active systems: x += ||r|| d.
+__global__ __launch_bounds__(kBlock) void k_batch_refine_accumulate(int64_t chunks,
+                                                                    const int32_t* __restrict__ chunk_sys,
+                                                                    const int64_t* __restrict__ sys_end,
+                                                                    const BatchRefineState* __restrict__ rs,
+                                                                    const float* __restrict__ d,
+                                                                    double* __restrict__ x)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        const int64_t row = c * kChunkRows + lane;
+        if (s >= 0 && rs[s].active && row < sys_end[s])
+            x[row] = __builtin_fma(rs[s].norm, static_cast<double>(d[row]), x[row]);
+    }
+}
+
+int batch_refine(BatchProblem& b, const double* rhs, double* x, double target, int32_t max_outer,
+                 const BatchPolicy& pol, pm_batch_refine_report* reports, pm_workspace* ws, hipStream_t stream)
+{
+    const int64_t n = b.a->rows, systems = b.systems, chunks = b.chunks;
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t rows_alloc = static_cast<size_t>(std::max<int64_t>(chunks, 1)) * kChunkRows;
+    const size_t c1 = static_cast<size_t>(std::max<int64_t>(chunks, 1));
+    const size_t s1 = static_cast<size_t>(std::max<int64_t>(systems, 1));
+    const size_t bytes = al(sizeof(double) * rows_alloc) + 2 * al(sizeof(float) * rows_alloc) +
+                         2 * al(sizeof(double) * c1) + al(sizeof(BatchRefineState) * s1);
+    PM_TRY(ws->refine.reserve(bytes));
+    char* p = ws->refine.as<char>();
+    double* r64 = reinterpret_cast<double*>(p);
+    p += al(sizeof(double) * rows_alloc);
+    float* rf = reinterpret_cast<float*>(p);
+    p += al(sizeof(float) * rows_alloc);
+    float* d = reinterpret_cast<float*>(p);
+    p += al(sizeof(float) * rows_alloc);
+    double* prr = reinterpret_cast<double*>(p);
+    p += al(sizeof(double) * c1);
+    double* pbb = reinterpret_cast<double*>(p);
+    p += al(sizeof(double) * c1);
+    BatchRefineState* rs = reinterpret_cast<BatchRefineState*>(p);
+    b.rhs = rf;
+    b.x = d;
+    const int gs = static_cast<int>(std::min<int64_t>(ceil_div(systems, kBlock), 1024));
+    hipLaunchKernelGGL(k_batch_refine_init, dim3(gs), dim3(kBlock), 0, stream, systems, rs);
+    std::vector<BatchRefineState> host(s1);
+    std::vector<pm_batch_report> inner(s1);
+    std::vector<int64_t> inner_total(s1, 0);
+    std::vector<int32_t> last_status(s1, 0);
+    for (int32_t outer = 0;; ++outer)
+    {
+        if (use_compact(b.a))
+            hipLaunchKernelGGL(k_batch_residual64<int32_t>, dim3(b.grid_chunks), dim3(kBlock), 0, stream, chunks,
+                               b.chunk_sys, b.sys_end, b.a->row_offsets32, b.a->col_indices32,
+                               static_cast<const float*>(b.a->values), rhs, x, r64, rs, prr, pbb);
+        else
+            hipLaunchKernelGGL(k_batch_residual64<int64_t>, dim3(b.grid_chunks), dim3(kBlock), 0, stream, chunks,
+                               b.chunk_sys, b.sys_end, b.a->row_offsets, b.a->col_indices,
+                               static_cast<const float*>(b.a->values), rhs, x, r64, rs, prr, pbb);
+        hipLaunchKernelGGL(k_batch_refine_decide, dim3(b.grid_systems), dim3(kBlock), 0, stream, systems,
+                           b.sys_chunk0, b.sys_chunk1, prr, pbb, rs, outer, target, max_outer);
+        PM_HIP(hipGetLastError());
+        PM_HIP(hipMemcpyAsync(host.data(), rs, sizeof(BatchRefineState) * systems, hipMemcpyDeviceToHost, stream));
+        PM_HIP(hipStreamSynchronize(stream));
+        int64_t active = 0;
+        for (int64_t s = 0; s < systems; ++s) active += host[s].active ? 1 : 0;
+        if (active == 0) break;
+        hipLaunchKernelGGL(k_batch_refine_scale, dim3(b.grid_chunks), dim3(kBlock), 0, stream, chunks, b.chunk_sys,
+                           b.sys_end, r64, rs, rf, d);
+        PM_HIP(hipGetLastError());
+        PM_TRY(batch_solve<float>(b, pol, inner.data(), ws, stream));
+        for (int64_t s = 0; s < systems; ++s)
+            if (host[s].active)
+            {
+                inner_total[s] += inner[s].iterations;
+                last_status[s] = inner[s].status;
+            }
+        hipLaunchKernelGGL(k_batch_refine_accumulate, dim3(b.grid_chunks), dim3(kBlock), 0, stream, chunks,
+                           b.chunk_sys, b.sys_end, rs, d, x);
+        PM_HIP(hipGetLastError());
+    }
+    PM_HIP(hipEventRecord(ws->done_event, stream));
+    for (int64_t s = 0; s < systems; ++s)
+    {
+        pm_batch_refine_report& o = reports[s];
+        o = pm_batch_refine_report{};
+        o.converged = host[s].converged;
+        o.outer_iterations = host[s].outer;
+        o.inner_iterations = inner_total[s];
+        o.initial_relres = host[s].initial;
+        o.final_relres = host[s].relres;
+        o.last_inner_status = last_status[s];
+    }
+    (void)n;
+    return PM_OK;
+}
+
+} // namespace pm
+
+using namespace pm;
+
+extern "C" {
+
+int pm_batch_layout(int64_t systems, const int64_t* sizes, int64_t* row_begin, int64_t* total_rows)
+{
+    if (systems < 0 || (systems > 0 && (!sizes || !row_begin)) || !total_rows)
+        return fail(PM_INVALID_ARGUMENT, "pm_batch_layout: invalid arguments");
+    int64_t cursor = 0;
+    for (int64_t s = 0; s < systems; ++s)
+    {
+        if (sizes[s] < 0) return fail(PM_INVALID_ARGUMENT, "pm_batch_layout: negative system size");
+        row_begin[s] = cursor;
+        cursor += ceil_div(sizes[s], kChunkRows) * kChunkRows;
+    }
+    *total_rows = cursor;
+    return PM_OK;
+}
+
+int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                 const void* rhs, void* x, const pm_solver_options* options, const pm_solve_policy* policy,
+                 pm_batch_report* reports, pm_workspace* ws, hipStream_t stream)
+{
+    BatchProblem b;
+    PM_TRY(batch_setup(a, systems, row_begin, row_end, rhs, x, options, reports != nullptr, ws, stream, b));
     if (systems == 0) return PM_OK;
     const BatchPolicy pol = resolve_batch_policy(policy);
     if (a->dtype == PM_F32) return batch_solve<float>(b, pol, reports, ws, stream);
     return batch_solve<double>(b, pol, reports, ws, stream);
+}
+
+int pm_batch_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                        const double* rhs, double* x, const pm_solver_options* inner, double target_relres,
+                        int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
+                        pm_workspace* ws, hipStream_t stream)
+{
+    if (!(target_relres >= 0.0) || max_outer < 0)
+        return fail(PM_INVALID_ARGUMENT, "batched refinement: invalid target or outer limit");
+    PM_TRY(check_csr(a, "batched refinement"));
+    if (a->dtype != PM_F32) return fail(PM_INVALID_ARGUMENT, "batched refinement expects an FP32 matrix");
+    if (!inner) return fail(PM_INVALID_ARGUMENT, "batched refinement: null inner options");
+    pm_solver_options in = *inner;
+    in.absolute_tolerance = 0.0;
+    in.require_convergence = 0;
+    // the inner solves run on workspace vectors; rhs / x here are FP64 and
+    // only checked for presence
+    BatchProblem b;
+    PM_TRY(batch_setup(a, systems, row_begin, row_end, rhs, x, &in, reports != nullptr, ws, stream, b));
+    if (systems == 0) return PM_OK;
+    return batch_refine(b, rhs, x, target_relres, max_outer, resolve_batch_policy(policy), reports, ws, stream);
 }
 
 } // extern "C"

@@ -147,3 +147,52 @@ def test_batch_rejects_bad_layout(dev, ws):
     with pytest.raises(capi.PlamatrixError) as e:
         capi.batch_pcg(a, begin[::-1].copy(), end[::-1].copy(), bt, x, ws)
     assert e.value.kind == "invalid_argument"
+
+
+def test_batch_refinement_reaches_fp64_target(dev, ws, oracle):
+    # FP32 matrices and inner solves, FP64 x and residuals: every system must
+    # reach an FP64 true relative residual below the target the FP32
+    # recurrence alone cannot reach, and agree with the oracle's FP64 solve
+    systems = mixed_systems(9, seed=21, dtype=np.float32)
+    systems.append(generators.laplacian2d_csr(16, 16, dtype=np.float32))
+    a, begin, end, big, b = pack(systems, dev, np.float32)
+    b64 = b.astype(np.float64)
+    b64[begin[-1]:end[-1]] = 0.0  # zero right-hand side: converged before any inner solve
+    bt = torch.from_numpy(b64).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    target = 1e-10
+    reps = capi.batch_pcg_refine(a, begin, end, bt, x, ws, target=target, max_outer=20, relative_tolerance=1e-3,
+                                 max_iterations=5000)
+    xs = x.cpu().numpy()
+    for s, b0, e0, rep in zip(systems[:-1], begin, end, reps):
+        assert rep.converged and rep.final_relres <= target, rep
+        assert rep.outer_iterations >= 2 and rep.inner_iterations > 0
+        got = xs[b0:e0]
+        true = relres(s, b64[b0:e0], got)
+        assert true <= target * 1.01
+        assert abs(true - rep.final_relres) <= 5e-2 * rep.final_relres + 1e-14
+        ref_sys = oracle.CSR(s.rows, s.rows, s.row_offsets, s.col_indices, s.values.astype(np.float64))
+        xr, rr = oracle.pcg(ref_sys, b64[b0:e0], relative_tolerance=1e-12, max_iterations=20000)
+        assert rr.converged
+        assert np.linalg.norm(got - xr) <= 1e-6 * np.linalg.norm(xr)
+    last = reps[-1]
+    assert last.converged and last.outer_iterations == 0 and last.inner_iterations == 0
+    assert torch.count_nonzero(x[begin[-1]:end[-1]]).item() == 0
+    # a warm start at the solution takes no outer step
+    reps2 = capi.batch_pcg_refine(a, begin, end, bt, x, ws, target=1e-8, max_outer=20, relative_tolerance=1e-3)
+    assert all(r.converged and r.outer_iterations == 0 for r in reps2)
+
+
+def test_batch_refinement_outer_limit(dev, ws):
+    systems = mixed_systems(3, seed=5, dtype=np.float32)
+    a, begin, end, big, b = pack(systems, dev, np.float32)
+    bt = torch.from_numpy(b.astype(np.float64)).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    reps = capi.batch_pcg_refine(a, begin, end, bt, x, ws, target=1e-12, max_outer=1, relative_tolerance=1e-2,
+                                 max_iterations=2000)
+    for rep in reps:
+        assert not rep.converged and rep.outer_iterations == 1
+        assert rep.final_relres < rep.initial_relres
+    with pytest.raises(capi.PlamatrixError) as e:
+        capi.batch_pcg_refine(a, begin, end, bt, x, ws, target=-1.0)
+    assert e.value.kind == "invalid_argument"
