@@ -53,6 +53,14 @@ struct BatchState
     unsigned long long bad_row;
 };
 
+// Chunks of the systems still running, as a device list with its length in
+// device memory (so replayed graphs follow every rebuild): a superset of the
+// running systems' chunks, since systems only stop between rebuilds.
+#define PM_FOR_LISTED(i, list_count, W)                                                               \
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * (W) + threadIdx.x / kWave,                    \
+                 i##_n = static_cast<int64_t>(*(list_count));                                         \
+         i < i##_n; i += static_cast<int64_t>(gridDim.x) * (W))
+
 __global__ __launch_bounds__(kBlock) void k_batch_state_init(int64_t systems, BatchState* st,
                                                              unsigned long long* done_count)
 {
@@ -78,14 +86,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_residual(int64_t chunks, const
                                                            const S* __restrict__ x, S* __restrict__ r,
                                                            S* __restrict__ inv, S* __restrict__ z,
                                                            S* __restrict__ p, BatchState* st,
-                                                           double* __restrict__ prr, double* __restrict__ prz)
+                                                           double* __restrict__ prr, double* __restrict__ prz,
+                                                           const int64_t* __restrict__ list,
+                                                           const unsigned long long* __restrict__ list_count)
 {
     __shared__ S lv_all[kBatchWaves][kStreamCap];
     __shared__ I lc_all[kBatchWaves][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
     {
+        const int64_t c = list[li];
         const int32_t s = chunk_sys[c];
         if (s < 0 || st[s].done != kRunning)
         {
@@ -107,33 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_residual(int64_t chunks, const
             S zi = ri;
             if (JACOBI)
             {
-                S m;
-                if (INIT)
-                {
-                    double diag = 0.0, scale = 0.0;
-                    bool found = false;
-                    for (int64_t k = ks; k < ke; ++k)
-                    {
-                        const double a = static_cast<double>(v[k]);
-                        scale = fmax(scale, fabs(a));
-                        if (static_cast<int64_t>(ci[k]) == row)
-                        {
-                            diag += a;
-                            found = true;
-                        }
-                    }
-                    const double threshold = static_cast<double>(scalar_epsilon<S>()) * scale * 16.0;
-                    const double reciprocal = 1.0 / diag;
-                    const bool ok = found && isfinite(diag) && diag > threshold && isfinite(reciprocal);
-                    m = ok ? static_cast<S>(reciprocal) : static_cast<S>(NAN);
-                    inv[row] = m;
-                    if (!ok) atomicMin(&st[s].bad_row, static_cast<unsigned long long>(row));
-                }
-                else
-                {
-                    m = inv[row];
-                }
-                zi = m * ri;
+                zi = inv[row] * ri;
             }
             z[row] = zi;
             rr = static_cast<double>(ri) * static_cast<double>(ri);
@@ -157,14 +142,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_direction(int64_t chunks, cons
                                                             const S* __restrict__ v, const S* __restrict__ z,
                                                             const S* __restrict__ p_old, S* __restrict__ p_new,
                                                             S* __restrict__ q, const BatchState* st,
-                                                            double* __restrict__ pden)
+                                                            double* __restrict__ pden,
+                                                            const int64_t* __restrict__ list,
+                                                            const unsigned long long* __restrict__ list_count)
 {
     __shared__ S lv_all[kBatchWaves][kStreamCap];
     __shared__ I lc_all[kBatchWaves][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
     {
+        const int64_t c = list[li];
         const int32_t s = chunk_sys[c];
         if (s < 0 || st[s].done != kRunning)
         {
@@ -198,11 +186,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const i
                                                          S* __restrict__ r, S* __restrict__ z,
                                                          const S* __restrict__ p, const S* __restrict__ q,
                                                          const S* __restrict__ inv, const BatchState* st,
-                                                         double* __restrict__ prr, double* __restrict__ prz)
+                                                         double* __restrict__ prr, double* __restrict__ prz,
+                                                         const int64_t* __restrict__ list,
+                                                         const unsigned long long* __restrict__ list_count)
 {
     const int lane = threadIdx.x % kWave;
-    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
     {
+        const int64_t c = list[li];
         const int32_t s = chunk_sys[c];
         if (s < 0 || st[s].done != kRunning)
         {
@@ -238,6 +229,112 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const i
     }
 }
 
+// Rebuild the list of chunks whose system is still running: per workgroup
+// an ordered compaction in LDS, one atomic per workgroup for its offset.
+__global__ __launch_bounds__(kBlock) void k_batch_list_reset(unsigned long long* list_count) { *list_count = 0ull; }
+
+__global__ __launch_bounds__(kBlock) void k_batch_list(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                       const BatchState* __restrict__ st, int64_t* __restrict__ list,
+                                                       unsigned long long* list_count)
+{
+    __shared__ unsigned wave_counts[kBlock / kWave];
+    __shared__ unsigned long long base;
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    for (int64_t c0 = static_cast<int64_t>(blockIdx.x) * kBlock; c0 < chunks;
+         c0 += static_cast<int64_t>(gridDim.x) * kBlock)
+    {
+        const int64_t c = c0 + threadIdx.x;
+        bool on = false;
+        if (c < chunks)
+        {
+            const int32_t s = chunk_sys[c];
+            on = s >= 0 && st[s].done == kRunning;
+        }
+        const unsigned long long ballot = __ballot(on);
+        const unsigned before = static_cast<unsigned>(__popcll(ballot & ((1ull << lane) - 1ull)));
+        if (lane == 0) wave_counts[wave] = static_cast<unsigned>(__popcll(ballot));
+        __syncthreads();
+        unsigned offset = 0, total = 0;
+        for (int w = 0; w < kBlock / kWave; ++w)
+        {
+            if (w < wave) offset += wave_counts[w];
+            total += wave_counts[w];
+        }
+        if (threadIdx.x == 0) base = total ? atomicAdd(list_count, static_cast<unsigned long long>(total)) : 0ull;
+        __syncthreads();
+        if (on) list[base + offset + before] = c;
+        __syncthreads();
+    }
+}
+
+// Jacobi inverse diagonal of every system row (jacobiInverse,
+// iterative_solver_cpu.cpp:127-160): eight lanes per row scan its entries
+// (coalesced), the largest magnitude sets the threshold; a row with one
+// diagonal entry takes it as is, several are summed in CSR order.  Rows of a
+// rejected diagonal get NaN and the system records its first such row.
+template <typename S, typename I>
+__global__ __launch_bounds__(kBlock) void k_batch_jacobi_inverse(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                                 const int64_t* __restrict__ sys_end,
+                                                                 const I* __restrict__ ro, const I* __restrict__ ci,
+                                                                 const S* __restrict__ v, S* __restrict__ inv,
+                                                                 BatchState* st)
+{
+    constexpr int L = 8;
+    const int lane = threadIdx.x % kWave;
+    const int sub = lane % L;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        if (s < 0) continue;
+        const int64_t end = sys_end[s];
+        for (int pass = 0; pass < L; ++pass)
+        {
+            const int64_t row = c * kChunkRows + pass * L + lane / L;
+            const bool valid = row < end;
+            double scale = 0.0, diag = 0.0;
+            int count = 0;
+            int64_t k0 = 0, k1 = 0;
+            if (valid)
+            {
+                k0 = static_cast<int64_t>(ro[row]);
+                k1 = static_cast<int64_t>(ro[row + 1]);
+                for (int64_t k = k0 + sub; k < k1; k += L)
+                {
+                    const double a = static_cast<double>(v[k]);
+                    scale = fmax(scale, fabs(a));
+                    if (static_cast<int64_t>(ci[k]) == row)
+                    {
+                        diag = a;
+                        ++count;
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1)
+            {
+                scale = fmax(scale, __shfl_xor(scale, off, L));
+                diag += __shfl_xor(diag, off, L);
+                count += __shfl_xor(count, off, L);
+            }
+            if (valid && sub == 0)
+            {
+                if (count > 1)
+                {
+                    diag = 0.0;
+                    for (int64_t k = k0; k < k1; ++k)
+                        if (static_cast<int64_t>(ci[k]) == row) diag += static_cast<double>(v[k]);
+                }
+                const double threshold = static_cast<double>(scalar_epsilon<S>()) * scale * 16.0;
+                const double reciprocal = 1.0 / diag;
+                const bool ok = count > 0 && isfinite(diag) && diag > threshold && isfinite(reciprocal);
+                inv[row] = ok ? static_cast<S>(reciprocal) : static_cast<S>(NAN);
+                if (!ok) atomicMin(&st[s].bad_row, static_cast<unsigned long long>(row));
+            }
+        }
+    }
+}
+
 constexpr int kReduceInit = 0;
 constexpr int kReduceAlpha = 1;
 constexpr int kReduceBeta = 2;
@@ -259,10 +356,26 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(int64_t systems, const 
     {
         if (st[s].done != kRunning) continue;
         double a = 0.0, b = 0.0;
-        for (int64_t c = sys_chunk0[s] + lane; c < sys_chunk1[s]; c += kWave)
+        const int64_t c1 = sys_chunk1[s];
+        constexpr int U = 8;
+        for (int64_t c = sys_chunk0[s] + lane; c < c1; c += U * kWave)
         {
-            a += pa[c];
-            if (MODE != kReduceAlpha) b += pb[c];
+            // all loads of U strided partials first, then the adds in order
+            double va[U], vb[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                const int64_t cc = c + u * kWave;
+                va[u] = cc < c1 ? pa[cc] : 0.0;
+                vb[u] = (MODE != kReduceAlpha && cc < c1) ? pb[cc] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+            {
+                if (c + u * kWave >= c1) break;
+                a += va[u];
+                if (MODE != kReduceAlpha) b += vb[u];
+            }
         }
         a = wave_sum(a);
         if (MODE != kReduceAlpha) b = wave_sum(b);
@@ -340,6 +453,8 @@ struct BatchProblem
     BatchState* st;
     double *pden, *prr, *prz;
     unsigned long long* done_count;
+    int64_t* list;                    // chunks of running systems
+    unsigned long long* list_count;
 };
 
 template <typename S, typename I>
@@ -370,13 +485,27 @@ struct BatchCalls
         return make_call(k, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys, b.sys_end, ro(b.a), ci(b.a),
                          static_cast<const S*>(b.a->values), static_cast<const S*>(b.rhs),
                          static_cast<const S*>(b.x), static_cast<S*>(b.r), static_cast<S*>(b.inv),
-                         static_cast<S*>(b.z), static_cast<S*>(p), b.st, b.prr, b.prz);
+                         static_cast<S*>(b.z), static_cast<S*>(p), b.st, b.prr, b.prz,
+                         static_cast<const int64_t*>(b.list), static_cast<const unsigned long long*>(b.list_count));
     }
 
     static void init(const BatchProblem& b, std::vector<KernelCall>& out)
     {
+        if (b.jacobi)
+            out.push_back(make_call(k_batch_jacobi_inverse<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
+                                    b.chunk_sys, b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
+                                    static_cast<S*>(b.inv), b.st));
+        list(b, out);
         out.push_back(residual<true>(b, b.p0));
         out.push_back(reduce<kReduceInit>(b, b.prr, b.prz));
+    }
+
+    static void list(const BatchProblem& b, std::vector<KernelCall>& out)
+    {
+        out.push_back(make_call(k_batch_list_reset, dim3(1), dim3(1), b.list_count));
+        const int grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(b.chunks, kBlock), 4096)));
+        out.push_back(make_call(k_batch_list, dim3(grid), dim3(kBlock), b.chunks, b.chunk_sys,
+                                static_cast<const BatchState*>(b.st), b.list, b.list_count));
     }
 
     static void iteration(const BatchProblem& b, int64_t t, bool refresh, std::vector<KernelCall>& out)
@@ -387,7 +516,9 @@ struct BatchCalls
         out.push_back(make_call(k_batch_direction<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys,
                                 b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
                                 static_cast<const S*>(b.z), static_cast<const S*>(p_old), static_cast<S*>(p_new),
-                                static_cast<S*>(b.q), static_cast<const BatchState*>(b.st), b.pden));
+                                static_cast<S*>(b.q), static_cast<const BatchState*>(b.st), b.pden,
+                                static_cast<const int64_t*>(b.list),
+                                static_cast<const unsigned long long*>(b.list_count)));
         out.push_back(reduce<kReduceAlpha>(b, b.pden, nullptr));
         if (!refresh)
         {
@@ -396,7 +527,8 @@ struct BatchCalls
                                     static_cast<S*>(b.x), static_cast<S*>(b.r), static_cast<S*>(b.z),
                                     static_cast<const S*>(p_new), static_cast<const S*>(b.q),
                                     static_cast<const S*>(b.inv), static_cast<const BatchState*>(b.st), b.prr,
-                                    b.prz));
+                                    b.prz, static_cast<const int64_t*>(b.list),
+                                    static_cast<const unsigned long long*>(b.list_count)));
         }
         else
         {
@@ -404,7 +536,8 @@ struct BatchCalls
                                     b.chunk_sys, b.sys_end, static_cast<S*>(b.x), static_cast<S*>(b.r),
                                     static_cast<S*>(b.z), static_cast<const S*>(p_new), static_cast<const S*>(b.q),
                                     static_cast<const S*>(b.inv), static_cast<const BatchState*>(b.st), b.prr,
-                                    b.prz));
+                                    b.prz, static_cast<const int64_t*>(b.list),
+                                    static_cast<const unsigned long long*>(b.list_count)));
             out.push_back(residual<false>(b, nullptr));
         }
         out.push_back(reduce<kReduceBeta>(b, b.prr, b.prz));
@@ -416,6 +549,11 @@ struct BatchCalls
         if (what == 0)
         {
             init(b, out);
+            return;
+        }
+        if (what == 2)
+        {
+            list(b, out);
             return;
         }
         for (int64_t t = first; t < first + count; ++t)
@@ -507,6 +645,7 @@ int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* rep
     PM_HIP(hipMemcpyAsync(hs, b.done_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
     PM_HIP(hipStreamSynchronize(stream));
     bool finished = static_cast<int64_t>(hs[0]) >= b.systems;
+    unsigned long long listed_at = hs[0];  // stopped systems when the list was last built
     if (!finished && needed > 0)
     {
         PM_TRY(issue());
@@ -518,6 +657,12 @@ int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* rep
             const int slot = static_cast<int>(oldest % 2);
             PM_HIP(hipEventSynchronize(ws->events[slot]));
             if (static_cast<int64_t>(hs[slot]) >= b.systems) break;
+            if (hs[slot] > listed_at)
+            {
+                // drop the stopped systems' chunks from the chunks still to be issued
+                listed_at = hs[slot];
+                PM_TRY(batch_plan<S>(ws, b, 2, 0, 0, 0)->run(stream, false));
+            }
             if (inflight.empty() && launched >= needed)
                 return fail(PM_RUNTIME_ERROR, "batched solver stopped without a final state");
         }
@@ -589,7 +734,7 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     const size_t s1 = std::max<int64_t>(systems, 1);
     const size_t c1 = std::max<int64_t>(chunks, 1);
     const size_t bytes = al(sizeof(int32_t) * c1) + 3 * al(sizeof(int64_t) * s1) + al(sizeof(BatchState) * s1) +
-                         3 * al(sizeof(double) * c1) + 256;
+                         3 * al(sizeof(double) * c1) + 256 + al(sizeof(int64_t) * c1) + 256;
     PM_TRY(ws->batch.reserve(bytes));
     if (!ws->host_batch)
         PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_batch), 2 * sizeof(unsigned long long),
@@ -629,6 +774,10 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     b.prz = reinterpret_cast<double*>(p);
     p += al(sizeof(double) * c1);
     b.done_count = reinterpret_cast<unsigned long long*>(p);
+    p += 256;
+    b.list = reinterpret_cast<int64_t*>(p);
+    p += al(sizeof(int64_t) * c1);
+    b.list_count = reinterpret_cast<unsigned long long*>(p);
     // the layout arrays are consumed by the first kernels: copy them
     // synchronously (they live in host vectors that go out of scope)
     PM_HIP(hipStreamSynchronize(stream));

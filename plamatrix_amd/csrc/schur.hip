@@ -277,13 +277,20 @@ __global__ __launch_bounds__(kBlock) void k_pair_runs(const unsigned long long* 
 }
 
 // B_p and g_p: thread per (primary, entry); entries [0, ps*ps) are B, then g.
-__global__ __launch_bounds__(kBlock) void k_ne_primary(int64_t pc, int rs, int ps, const int64_t* __restrict__ offs,
+// RS > 0 (residual size known at compile time) loads the operands of
+// kNeBatch residuals before accumulating them in order; RS == 0 takes rs at
+// run time, one residual at a time.
+constexpr int kNeBatch = 8;
+template <int RS>
+__global__ __launch_bounds__(kBlock) void k_ne_primary(int64_t pc, int rs_runtime, int ps,
+                                                       const int64_t* __restrict__ offs,
                                                        const unsigned* __restrict__ perm,
                                                        const double* __restrict__ jp, const double* __restrict__ r,
                                                        const double* __restrict__ w, double* __restrict__ B,
                                                        double* __restrict__ g)
 {
 #pragma clang fp contract(off)
+    const int rs = RS > 0 ? RS : rs_runtime;
     const int entries = ps * ps + ps;
     for (int64_t t = gtid(); t < pc * entries; t += gstride())
     {
@@ -293,15 +300,47 @@ __global__ __launch_bounds__(kBlock) void k_ne_primary(int64_t pc, int rs, int p
         const int column = is_b ? en / ps : en - ps * ps;
         const int other = is_b ? en % ps : 0;
         double acc = 0.0;
-        for (int64_t k = offs[p]; k < offs[p + 1]; ++k)
+        const int64_t k1 = offs[p + 1];
+        if (RS > 0)
         {
-            const int64_t i = perm[k];
-            const double wi = w[i];
-            const double* J = jp + i * rs * ps;
-            for (int row = 0; row < rs; ++row)
+            for (int64_t k = offs[p]; k < k1; k += kNeBatch)
             {
-                const double value = J[row * ps + column];
-                acc += wi * value * (is_b ? J[row * ps + other] : r[i * rs + row]);
+                double wi[kNeBatch], a[kNeBatch][RS > 0 ? RS : 1], b[kNeBatch][RS > 0 ? RS : 1];
+#pragma unroll
+                for (int u = 0; u < kNeBatch; ++u)
+                {
+                    const bool in = k + u < k1;
+                    const int64_t i = in ? perm[k + u] : 0;
+                    const double* J = jp + i * RS * ps;
+                    wi[u] = in ? w[i] : 0.0;
+#pragma unroll
+                    for (int row = 0; row < RS; ++row)
+                    {
+                        a[u][row] = in ? J[row * ps + column] : 0.0;
+                        b[u][row] = in ? (is_b ? J[row * ps + other] : r[i * RS + row]) : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kNeBatch; ++u)
+                {
+                    if (k + u >= k1) break;
+#pragma unroll
+                    for (int row = 0; row < RS; ++row) acc += wi[u] * a[u][row] * b[u][row];
+                }
+            }
+        }
+        else
+        {
+            for (int64_t k = offs[p]; k < k1; ++k)
+            {
+                const int64_t i = perm[k];
+                const double wi = w[i];
+                const double* J = jp + i * rs * ps;
+                for (int row = 0; row < rs; ++row)
+                {
+                    const double value = J[row * ps + column];
+                    acc += wi * value * (is_b ? J[row * ps + other] : r[i * rs + row]);
+                }
             }
         }
         if (is_b) B[p * ps * ps + en] = acc;
@@ -639,14 +678,33 @@ __global__ __launch_bounds__(kBlock) void k_eliminated_inverse(int64_t ec, int e
     }
 }
 
+// Per cross block c and row: v_c[row] = W_c[row] . y_e (one thread each).
+__global__ __launch_bounds__(kBlock) void k_cross_rhs(int64_t nc, int ps, int es,
+                                                      const int64_t* __restrict__ cross_eliminated,
+                                                      const double* __restrict__ W, const double* __restrict__ y,
+                                                      double* __restrict__ crv)
+{
+#pragma clang fp contract(off)
+    for (int64_t t = gtid(); t < nc * ps; t += gstride())
+    {
+        const int64_t c = t / ps;
+        const int row = static_cast<int>(t % ps);
+        const double* m = W + c * ps * es + row * es;
+        const double* v = y + cross_eliminated[c] * es;
+        double value = 0.0;
+        for (int col = 0; col < es; ++col) value += m[col] * v[col];
+        crv[t] = value;
+    }
+}
+
 // rhs_p = -g_p + sum over the primary's cross blocks in (primary, eliminated)
-// order of W C^-1 g_e (block_schur.cpp:158-181).
-__global__ __launch_bounds__(kBlock) void k_reduced_rhs(int64_t pc, int ps, int es, const double* __restrict__ gp,
+// order of W C^-1 g_e (block_schur.cpp:158-181), the products precomputed by
+// k_cross_rhs; the ordered sum loads kRhsBatch products ahead of the adds.
+constexpr int kRhsBatch = 16;
+__global__ __launch_bounds__(kBlock) void k_reduced_rhs(int64_t pc, int ps, const double* __restrict__ gp,
                                                         const int64_t* __restrict__ prim_offs,
                                                         const unsigned* __restrict__ prim_cross,
-                                                        const int64_t* __restrict__ cross_eliminated,
-                                                        const double* __restrict__ W, const double* __restrict__ y,
-                                                        double* __restrict__ rhs)
+                                                        const double* __restrict__ crv, double* __restrict__ rhs)
 {
 #pragma clang fp contract(off)
     for (int64_t t = gtid(); t < pc * ps; t += gstride())
@@ -654,14 +712,16 @@ __global__ __launch_bounds__(kBlock) void k_reduced_rhs(int64_t pc, int ps, int 
         const int64_t p = t / ps;
         const int row = static_cast<int>(t % ps);
         double acc = -gp[t];
-        for (int64_t k = prim_offs[p]; k < prim_offs[p + 1]; ++k)
+        const int64_t k1 = prim_offs[p + 1];
+        for (int64_t k = prim_offs[p]; k < k1; k += kRhsBatch)
         {
-            const int64_t c = prim_cross[k];
-            const double* m = W + c * ps * es + row * es;
-            const double* v = y + cross_eliminated[c] * es;
-            double value = 0.0;
-            for (int col = 0; col < es; ++col) value += m[col] * v[col];
-            acc += 1.0 * value;
+            double v[kRhsBatch];
+#pragma unroll
+            for (int u = 0; u < kRhsBatch; ++u)
+                v[u] = k + u < k1 ? crv[static_cast<int64_t>(prim_cross[k + u]) * ps + row] : 0.0;
+#pragma unroll
+            for (int u = 0; u < kRhsBatch; ++u)
+                if (k + u < k1) acc += 1.0 * v[u];
         }
         rhs[t] = acc;
     }
@@ -692,9 +752,14 @@ __global__ __launch_bounds__(kBlock) void k_transform_cross(int64_t nc, int ps, 
 // One wavefront per block slot, lane = (lr, lc).  Value = damped B on the
 // diagonal, + each primary cross block (forward, or transposed for the
 // mirrored slot), then - (W_l C^-1)(W_r)^T for every term in order
-// (block_schur_sparse_assembly.h:817-903).
+// (block_schur_sparse_assembly.h:817-903).  ES > 0 (the eliminated block
+// size known at compile time) loads the operands of kTermBatch terms before
+// accumulating them in order; ES == 0 takes es at run time, one term at a
+// time.
+constexpr int kTermBatch = 8;
+template <int ES>
 __global__ __launch_bounds__(kBlock) void k_assemble_slots(
-    int64_t slots, int ps, int es, int pb, const unsigned long long* __restrict__ slot_key,
+    int64_t slots, int ps, int es_runtime, int pb, const unsigned long long* __restrict__ slot_key,
     const int64_t* __restrict__ slot_start, int64_t entries, const unsigned* __restrict__ term_e,
     const unsigned* __restrict__ term_left, const unsigned* __restrict__ term_right,
     const int64_t* __restrict__ bro, const int64_t* __restrict__ ro, const double* __restrict__ Bd,
@@ -702,6 +767,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
     double* __restrict__ values)
 {
 #pragma clang fp contract(off)
+    const int es = ES > 0 ? ES : es_runtime;
     const int lane = threadIdx.x % kWave;
     const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     const unsigned long long mask = (1ull << pb) - 1ull;
@@ -718,21 +784,55 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
         // the run holds terms first, then the diagonal marker, then primary
         // cross markers in index order
         if (row == col) acc += Bd[row * ps * ps + lr * ps + lc];
-        for (int64_t k = begin; k < end; ++k)
+        // the markers are the few entries at the run's end
+        int64_t tend = end;
+        while (tend > begin && term_e[tend - 1] >= ~1u) --tend;
+        for (int64_t k = tend; k < end; ++k)
         {
             if (term_e[k] != ~1u) continue;
             const int64_t q = term_left[k];
             const double* m = pcv + q * ps * ps;
             acc += term_right[k] == 0 ? m[lr * ps + lc] : m[lc * ps + lr];
         }
-        int64_t k = begin;
-        for (; k < end && term_e[k] < ~1u; ++k)
+        if (ES > 0)
         {
-            const double* t = TW + static_cast<int64_t>(term_left[k]) * ps * es + lr * es;
-            const double* w = W + static_cast<int64_t>(term_right[k]) * ps * es + lc * es;
-            double value = 0.0;
-            for (int i = 0; i < es; ++i) value += t[i] * w[i];
-            acc += -value;
+            for (int64_t k = begin; k < tend; k += kTermBatch)
+            {
+                double t[kTermBatch][ES > 0 ? ES : 1], w[kTermBatch][ES > 0 ? ES : 1];
+#pragma unroll
+                for (int u = 0; u < kTermBatch; ++u)
+                {
+                    const bool in = k + u < tend;
+                    const int64_t l = in ? term_left[k + u] : 0;
+                    const int64_t r = in ? term_right[k + u] : 0;
+#pragma unroll
+                    for (int i = 0; i < ES; ++i)
+                    {
+                        t[u][i] = in ? TW[l * ps * ES + lr * ES + i] : 0.0;
+                        w[u][i] = in ? W[r * ps * ES + lc * ES + i] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kTermBatch; ++u)
+                {
+                    if (k + u >= tend) break;
+                    double value = 0.0;
+#pragma unroll
+                    for (int i = 0; i < ES; ++i) value += t[u][i] * w[u][i];
+                    acc += -value;
+                }
+            }
+        }
+        else
+        {
+            for (int64_t k = begin; k < tend; ++k)
+            {
+                const double* t = TW + static_cast<int64_t>(term_left[k]) * ps * es + lr * es;
+                const double* w = W + static_cast<int64_t>(term_right[k]) * ps * es + lc * es;
+                double value = 0.0;
+                for (int i = 0; i < es; ++i) value += t[i] * w[i];
+                acc += -value;
+            }
         }
         values[ro[row * ps + lr] + (s - bro[row]) * ps + lc] = acc;
     }
@@ -928,6 +1028,7 @@ struct pm_schur_workspace
     double* TW = nullptr;
     double* values = nullptr;
     double* pinv = nullptr;
+    double* crv = nullptr;  // per (cross block, row) products of the reduced rhs
 
     int bind(hipStream_t s)
     {
@@ -1413,6 +1514,7 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     nz.take<double>(nc * ps * es);
     nz.take<double>(pd > 0 ? w->values_count : 0);
     nz.take<double>(pc * ps * ps);
+    nz.take<double>(nc * ps);
     PM_TRY(w->numerics.reserve(nz.off));
     Carve n;
     n.base = w->numerics.as<char>();
@@ -1424,6 +1526,7 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     w->TW = n.take<double>(nc * ps * es);
     w->values = n.take<double>(pd > 0 ? w->values_count : 0);
     w->pinv = n.take<double>(pc * ps * ps);
+    w->crv = n.take<double>(nc * ps);
 
     hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
     if (pc > 0)
@@ -1438,18 +1541,21 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     }
     if (pd > 0)
     {
-        hipLaunchKernelGGL(k_reduced_rhs, dim3(grid_for(pd)), dim3(kBlock), 0, stream, pc, ps, es,
-                           eq->primary_gradient, w->prim_offs, w->prim_cross, eq->cross_eliminated, eq->cross_values,
-                           w->y, w->rhs);
+        if (nc > 0)
+            hipLaunchKernelGGL(k_cross_rhs, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
+                               eq->cross_eliminated, eq->cross_values, w->y, w->crv);
+        hipLaunchKernelGGL(k_reduced_rhs, dim3(grid_for(pd)), dim3(kBlock), 0, stream, pc, ps, eq->primary_gradient,
+                           w->prim_offs, w->prim_cross, static_cast<const double*>(w->crv), w->rhs);
         if (nc > 0)
             hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
                                eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
         const int64_t waves_per_block = kBlock / kWave;
         const int slot_grid =
             static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
-        hipLaunchKernelGGL(k_assemble_slots, dim3(slot_grid), dim3(kBlock), 0, stream, w->slots, ps, es, w->pb,
-                           w->slot_key, w->slot_start, w->entries, w->term_e, w->term_left, w->term_right, w->bro,
-                           w->ro, w->Bd, eq->primary_cross_values, w->TW, eq->cross_values, w->values);
+        hipLaunchKernelGGL(es == 3 ? k_assemble_slots<3> : k_assemble_slots<0>, dim3(slot_grid), dim3(kBlock), 0,
+                           stream, w->slots, ps, es, w->pb, w->slot_key, w->slot_start, w->entries, w->term_e,
+                           w->term_left, w->term_right, w->bro, w->ro, w->Bd, eq->primary_cross_values, w->TW,
+                           eq->cross_values, w->values);
         hipLaunchKernelGGL(k_preconditioner, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, ps, w->diag_slot, w->bro,
                            w->ro, w->values, w->pinv, st);
     }
@@ -1558,7 +1664,8 @@ int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, 
     const int rs = static_cast<int>(rb->residual_size);
     const int64_t runs = w->ne_runs;
     if (pc * ps > 0)
-        hipLaunchKernelGGL(k_ne_primary, dim3(grid_for(pc * (ps * ps + ps))), dim3(kBlock), 0, stream, pc, rs,
+        hipLaunchKernelGGL(rs == 2 ? k_ne_primary<2> : k_ne_primary<0>, dim3(grid_for(pc * (ps * ps + ps))),
+                           dim3(kBlock), 0, stream, pc, rs,
                            static_cast<int>(ps), w->ne_offs_p, w->ne_perm_p, rb->primary_jacobians, rb->residuals,
                            weights, B, gp);
     if (ec * es > 0)
