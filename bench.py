@@ -651,8 +651,8 @@ def run_batch(args, torch, capi, generators):
 def run_learned_eval(args, torch, capi, generators):
     """Held-out evaluation of the trained network: scaled-down 3-D diffusion
     boxes and reduced LM Schur systems (eval seeds 2*10^6 + i), Jacobi-PCG vs
-    learned G^T G PCG to relres 1e-6 (FP32), graph build + inference included
-    in the learned time, Jacobi fallback when the learned solve is not faster."""
+    learned G^T G PCG, FP32 inner solves refined in FP64 to a true relres of
+    1e-6, graph build + inference included in the learned time."""
     from plamatrix_amd import train_fsai
 
     net = capi.FsaiNet(seed=0)
@@ -683,7 +683,8 @@ def run_learned_eval(args, torch, capi, generators):
                      round(total_l / total_j, 4), "ratio", False, "f32",
                      {"workload": f"{args.eval_systems} held-out scaled-down 3-D diffusion boxes (16..33 per side) "
                                   f"and {args.eval_systems} reduced LM Schur systems (300 primaries, 6,000 "
-                                  "secondaries), eval seeds 2*10^6+i, FP32 PCG to relres 1e-6",
+                                  "secondaries), eval seeds 2*10^6+i; FP32 inner PCG with FP64 refinement to an "
+                                  "FP64 true relres of 1e-6",
                       "weights": os.path.relpath(args.weights, ROOT)},
                      None, None)
     line["per_kind"] = out

@@ -294,4 +294,32 @@ std::vector<pm_batch_report> batchedPcg(const CSRMatrix<Scalar, Device::GPU>& ma
     return reports;
 }
 
+/// batchedPcg with FP64 refinement per system: FP32 matrix and inner solves,
+/// FP64 right-hand sides and solutions (the initial guess), every system
+/// driven to an FP64 true relative residual of target_relres or max_outer
+/// outer steps.
+inline std::vector<pm_batch_refine_report> batchedPcgRefine(
+    const CSRMatrix<float, Device::GPU>& matrix, const std::vector<Index>& row_begin,
+    const std::vector<Index>& row_end, const DenseMatrix<double, Device::GPU>& rhs,
+    DenseMatrix<double, Device::GPU>& solution, IterativeSolverWorkspace<float>& workspace,
+    double target_relres = 1e-6, int max_outer = 10,
+    const IterativeSolverOptions& inner = {1000000, 1e-3, 0.0, true, false}, int refresh_interval = 50,
+    hipStream_t stream = nullptr)
+{
+    if (row_begin.size() != row_end.size())
+        throw std::invalid_argument("batchedPcgRefine: row range arrays differ in size");
+    if (rhs.rows() != matrix.rows() || solution.rows() != matrix.rows() || rhs.cols() != 1 || solution.cols() != 1)
+        throw std::invalid_argument("batchedPcgRefine requires Nx1 rhs/solution matching the matrix");
+    const pm_csr d = matrix.descriptor(true, stream);
+    const pm_solver_options o = detail::toAbi(inner);
+    pm_solve_policy policy{};
+    policy.use_graph = -1;
+    policy.refresh_interval = refresh_interval;
+    std::vector<pm_batch_refine_report> reports(row_begin.size());
+    detail::check(pm_batch_pcg_refine(&d, static_cast<int64_t>(row_begin.size()), row_begin.data(), row_end.data(),
+                                      rhs.data(), solution.data(), &o, target_relres, max_outer, &policy,
+                                      reports.data(), workspace.handle(), stream));
+    return reports;
+}
+
 } // namespace plamatrix

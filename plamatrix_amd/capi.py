@@ -114,6 +114,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBRARY):
             raise RuntimeError(f"{LIBRARY} is not built; run __graft_entry__.build() (HIP library required)")
+        # torch ships its own HIP runtime under the same soname: load it first so
+        # the library binds to that one runtime (loaded the other way round,
+        # torch later finds no devices)
+        import torch  # noqa: F401
+
         _lib = ctypes.CDLL(LIBRARY)
         _lib.pm_last_error.restype = ctypes.c_char_p
         _lib.pm_workspace_capacity_bytes.restype = ctypes.c_size_t

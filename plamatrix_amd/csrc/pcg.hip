@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_direction(int64_t rows, const S* __r
 
 // Second kernel of iteration t: q = A p_new and the per-workgroup partial
 // p_new.q.  NT streams the matrix with non-temporal loads.
-template <typename S, typename I, bool NT>
+template <typename S, typename I, bool NT, bool DMA>
 __global__ __launch_bounds__(kDirBlock) void k_spmv_den(int64_t rows, const I* __restrict__ ro,
                                                         const I* __restrict__ ci, const S* __restrict__ v,
                                                         const S* __restrict__ p, S* __restrict__ q,
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den(int64_t rows, const I* _
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S, I, SpmvSource<S>, NT>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+        const S acc = chunk_row_sums<S, I, SpmvSource<S>, NT, DMA>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
                                                               SpmvSource<S>{v, p}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_step_xr(int64_t rows, S* __restrict_
 }
 
 // t = G r (G lower, CSR), skipped once the solve has stopped.
-template <typename S, typename I>
+template <typename S, typename I, bool DMA>
 __global__ __launch_bounds__(kBlock) void k_factor_lower(int64_t rows, const I* __restrict__ ro,
                                                          const I* __restrict__ ci, const S* __restrict__ v,
                                                          const S* __restrict__ r, S* __restrict__ t,
@@ -341,13 +341,14 @@ __global__ __launch_bounds__(kBlock) void k_factor_lower(int64_t rows, const I* 
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, r}, s, e);
+        const S acc = chunk_row_sums<S, I, SpmvSource<S>, false, DMA>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                                                      SpmvSource<S>{v, r}, s, e);
         if (row0 + lane < rows) t[row0 + lane] = acc;
     }
 }
 
 // z = G^T t (G^T as CSR) and the partial sums r.r, r.z of the next direction.
-template <typename S, typename I>
+template <typename S, typename I, bool DMA>
 __global__ __launch_bounds__(kBlock) void k_factor_upper(int64_t rows, const I* __restrict__ ro,
                                                          const I* __restrict__ ci, const S* __restrict__ v,
                                                          const S* __restrict__ t, const S* __restrict__ r,
@@ -363,7 +364,8 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper(int64_t rows, const I* 
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
-        const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, t}, s, e);
+        const S acc = chunk_row_sums<S, I, SpmvSource<S>, false, DMA>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                                                      SpmvSource<S>{v, t}, s, e);
         const int64_t row = row0 + lane;
         if (row < rows)
         {
@@ -408,6 +410,7 @@ struct Problem
     int grid_rows;           // workgroups of the row kernels (publish p.q)
     int grid_units;          // workgroups of the unit kernels (publish r.r, r.z)
     bool nt;                 // non-temporal matrix loads in the iteration
+    bool dma;                // 4-byte matrix operands staged into LDS by global_load_lds
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
@@ -474,11 +477,13 @@ struct Calls
     {
         const int64_t rows = pb.a->rows;
         const int grid = static_cast<int>(std::min<int64_t>(full_grid(rows), kSpmvGrid));
-        out.push_back(make_call(k_factor_lower<S, int32_t>, dim3(grid), dim3(kBlock), rows, pb.gf->row_offsets32,
+        out.push_back(make_call(pb.dma ? k_factor_lower<S, int32_t, true> : k_factor_lower<S, int32_t, false>,
+                                dim3(grid), dim3(kBlock), rows, pb.gf->row_offsets32,
                                 pb.gf->col_indices32, static_cast<const S*>(pb.gf->values),
                                 static_cast<const S*>(pb.r), static_cast<S*>(pb.inv),
                                 static_cast<const SolverState*>(pb.st)));
-        out.push_back(make_call(k_factor_upper<S, int32_t>, dim3(pb.grid_rows), dim3(kBlock), rows,
+        out.push_back(make_call(pb.dma ? k_factor_upper<S, int32_t, true> : k_factor_upper<S, int32_t, false>,
+                                dim3(pb.grid_rows), dim3(kBlock), rows,
                                 pb.gtf->row_offsets32, pb.gtf->col_indices32, static_cast<const S*>(pb.gtf->values),
                                 static_cast<const S*>(pb.inv), static_cast<const S*>(pb.r), static_cast<S*>(pb.z),
                                 static_cast<const SolverState*>(pb.st), pb.partials));
@@ -494,7 +499,9 @@ struct Calls
         out.push_back(make_call(k_direction<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, zsrc,
                                 static_cast<const S*>(p_old), static_cast<S*>(p_new), pb.st, pb.partials, par,
                                 pb.pre == kPreFactored ? pb.grid_rows : pb.grid_units));
-        out.push_back(make_call(pb.nt ? k_spmv_den<S, I, true> : k_spmv_den<S, I, false>, dim3(pb.grid_rows),
+        const auto spmv_den = pb.nt ? (pb.dma ? k_spmv_den<S, I, true, true> : k_spmv_den<S, I, true, false>)
+                                    : (pb.dma ? k_spmv_den<S, I, false, true> : k_spmv_den<S, I, false, false>);
+        out.push_back(make_call(spmv_den, dim3(pb.grid_rows),
                                 dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
                                 static_cast<const S*>(p_new), static_cast<S*>(pb.q),
                                 static_cast<const SolverState*>(pb.st), pb.partials));
@@ -589,6 +596,10 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     pb.grid_units = solver_grid_units(units);
     const char* nt = std::getenv("PLAMATRIX_AMD_SOLVER_NT");
     pb.nt = nt != nullptr && nt[0] == '1';
+    // LDS-DMA staging speeds up the HBM-streamed SpMV but slows the solver's
+    // cache-resident iteration (measured: 53.3 vs 51.1 us per c1 iteration)
+    const char* dma = std::getenv("PLAMATRIX_AMD_SOLVER_DMA");
+    pb.dma = dma != nullptr && dma[0] == '1';
     return PM_OK;
 }
 
@@ -604,7 +615,7 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
                   (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
                   static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
-                  static_cast<long long>(pb.bsize), pb.nt ? 1 : 0, pb.gf ? pb.gf->values : nullptr,
+                  static_cast<long long>(pb.bsize), (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0), pb.gf ? pb.gf->values : nullptr,
                   pb.gf ? (const void*)pb.gf->row_offsets32 : nullptr, pb.gtf ? pb.gtf->values : nullptr,
                   pb.gtf ? (const void*)pb.gtf->row_offsets32 : nullptr);
     return buf;

@@ -4,12 +4,13 @@ against Jacobi-PCG.
 Training systems come from the method's own generators with seeds disjoint
 from evaluation (train 10^6 + i, eval 2 * 10^6 + i): 3-D variable-coefficient
 diffusion boxes (the 128^3 generator, scaled down) and reduced LM Schur systems
-(scaled down, assembled by the device Schur pipeline).  Each step draws 8
-Gaussian probes and takes one Adam step on
+(scaled down, assembled by the device Schur pipeline), alternating, each used
+for one step by default (20,000 systems).  Each step draws 8 Gaussian probes
+and takes one Adam step on
 (1/P) sum_p ||(I - G A G^T) z_p||^2 / ||z_p||^2 with the library's own BF16
 MFMA forward and backward kernels.
 
-    python -m plamatrix_amd.train_fsai --steps 3000 --out plamatrix_amd/fsai_weights.npy
+    python -m plamatrix_amd.train_fsai --systems 20000 --steps 20000 --out plamatrix_amd/fsai_weights.npy
 """
 from __future__ import annotations
 
@@ -36,14 +37,15 @@ def diffusion_system(seed: int, lo: int = 16, hi: int = 34):
     return generators.diffusion3d_box_csr(nx, ny, nz, seed=seed, dtype=np.float32)
 
 
-def schur_system(seed: int, primaries: int = 300, ratio: int = 20):
+def schur_system(seed: int, primaries: int = 300, ratio: int = 20, instance=None):
     """Reduced Schur system of a scaled-down LM instance, FP32 CSR on the host,
     assembled on the device."""
     import torch
 
     from plamatrix_amd import capi, generators
 
-    inst = generators.lm_schur_instance(primary_count=primaries, eliminated_count=primaries * ratio, seed=seed)
+    inst = instance if instance is not None else generators.lm_schur_instance(
+        primary_count=primaries, eliminated_count=primaries * ratio, seed=seed)
     dev = torch.device("cuda:0")
     t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
     sw = capi.SchurWorkspace()
@@ -67,79 +69,118 @@ def make_system(kind: str, seed: int):
     return diffusion_system(seed) if kind == "diffusion" else schur_system(seed)
 
 
-def evaluate(net, systems, rel=1e-6, max_iterations=20000):
-    """Jacobi-PCG vs learned-G PCG: iterations and wall time per system (FP32);
-    the learned time includes the graph build and the network inference."""
+def evaluate(net, systems, rel=1e-6, max_iterations=20000, inner_rel=1e-3):
+    """Jacobi-PCG vs learned-G PCG on each system: FP32 inner solves with FP64
+    refinement to an FP64 true relative residual of `rel`; iterations (inner,
+    summed over outer steps), wall time and the true residual.  The learned
+    time includes the graph build and the network inference."""
     import torch
 
     from plamatrix_amd import capi, generators
 
     dev = torch.device("cuda:0")
     ws = capi.Workspace()
+    pol = capi.policy(refresh_interval=50)
+    inner = dict(relative_tolerance=inner_rel, max_iterations=max_iterations)
     rows = []
     for h in systems:
         a = capi.to_device_csr(h, compact=True)
-        b = torch.from_numpy(generators.rhs_normal(h.rows, seed=h.rows)).to(dev)
+        b = torch.from_numpy(generators.rhs_normal(h.rows, seed=h.rows).astype(np.float64)).to(dev)
         out = {}
         for name in ("jacobi", "learned"):
             x = torch.zeros_like(b)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             if name == "jacobi":
-                rep = capi.pcg(a, b, x, ws, relative_tolerance=rel, max_iterations=max_iterations)
+                rep = capi.pcg_refine(a, b, x, ws, target=rel, max_outer=10, pol=pol, **inner)
             else:
                 graph = capi.FsaiGraph(a)
                 capi.fsai_forward(net, graph)
-                rep = capi.pcg_factored(a, b, x, graph, ws, relative_tolerance=rel, max_iterations=max_iterations)
+                rep = capi.pcg_refine_factored(a, b, x, graph, ws, target=rel, max_outer=10, pol=pol, **inner)
             torch.cuda.synchronize()
-            out[name] = dict(iterations=rep.iterations, converged=rep.converged,
-                             ms=(time.perf_counter() - t0) * 1e3,
-                             relres=capi.true_relres(a, b, x, ws))
+            out[name] = dict(iterations=rep.inner_iterations, outer=rep.outer_iterations, converged=rep.converged,
+                             ms=(time.perf_counter() - t0) * 1e3, relres=capi.true_relres(a, b, x, ws))
         rows.append(out)
     return rows
 
 
-def train(steps: int, seed: int, lr: float, kinds: list[str], systems_per_kind: int, log_every: int = 100):
-    import torch  # noqa: F401
+def host_system(kind: str, seed: int):
+    """Host-side part of a training system: a diffusion CSR, or an LM instance
+    whose Schur system is assembled on the device by the consumer."""
+    from plamatrix_amd import generators
+
+    if kind == "diffusion":
+        return diffusion_system(seed)
+    return generators.lm_schur_instance(primary_count=300, eliminated_count=6000, seed=seed)
+
+
+def device_system(kind: str, seed: int, host):
+    if kind == "diffusion":
+        return host
+    return schur_system(seed, instance=host)
+
+
+def train_seed(kind: str, i: int) -> int:
+    return 1_000_000 + i + (50_000 if kind == "schur" else 0)
+
+
+def train(steps: int, seed: int, lr: float, kinds: list[str], systems: int, log_every: int = 100):
+    """One Adam step per training system: system i (kinds alternating, seeds
+    10^6 + i, Schur seeds offset by 50,000) is generated on a host thread
+    ahead of use, moved to the device, and used for `steps / systems` steps."""
+    import queue
+    import threading
 
     from plamatrix_amd import capi
 
     net = capi.FsaiNet(seed=seed)
-    graphs = []
-    keep = []
-    for kind in kinds:
-        for i in range(systems_per_kind):
-            h = make_system(kind, 1_000_000 + i + (50_000 if kind == "schur" else 0))
-            a = capi.to_device_csr(h, compact=True)
-            keep.append(a)
-            graphs.append(capi.FsaiGraph(a))
+    order = [(kinds[i % len(kinds)], i // len(kinds)) for i in range(systems)]
+    ahead: queue.Queue = queue.Queue(maxsize=16)
+
+    def produce():
+        for kind, i in order:
+            ahead.put((kind, i, host_system(kind, train_seed(kind, i))))
+        ahead.put(None)
+
+    threading.Thread(target=produce, daemon=True).start()
+    per_system = max(1, steps // max(systems, 1))
     history = []
     t0 = time.perf_counter()
     running = []
-    for step in range(steps):
-        # cosine decay to lr / 10
-        rate = lr * (0.1 + 0.9 * 0.5 * (1.0 + math.cos(math.pi * step / max(steps, 1))))
-        loss = capi.fsai_train_step(net, graphs[step % len(graphs)], probes=8, seed=step + 1, lr=rate)
-        running.append(loss)
-        if (step + 1) % log_every == 0 or step + 1 == steps:
-            history.append(dict(step=step + 1, loss=float(np.mean(running)), seconds=time.perf_counter() - t0))
-            print(json.dumps(history[-1]), flush=True)
-            running = []
+    step = 0
+    total = per_system * systems
+    while True:
+        item = ahead.get()
+        if item is None:
+            break
+        kind, i, host = item
+        a = capi.to_device_csr(device_system(kind, train_seed(kind, i), host), compact=True)
+        graph = capi.FsaiGraph(a)
+        for _ in range(per_system):
+            # cosine decay to lr / 10
+            rate = lr * (0.1 + 0.9 * 0.5 * (1.0 + math.cos(math.pi * step / max(total, 1))))
+            running.append(capi.fsai_train_step(net, graph, probes=8, seed=step + 1, lr=rate))
+            step += 1
+            if step % log_every == 0 or step == total:
+                history.append(dict(step=step, loss=float(np.mean(running)), seconds=time.perf_counter() - t0))
+                print(json.dumps(history[-1]), flush=True)
+                running = []
+        del graph, a
     return net, history
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--lr", type=float, default=3e-3)
-    ap.add_argument("--systems", type=int, default=12, help="training systems per kind")
+    ap.add_argument("--systems", type=int, default=20000, help="training systems (kinds alternate)")
     ap.add_argument("--kinds", default="diffusion,schur")
-    ap.add_argument("--eval", type=int, default=8, help="held-out systems per kind")
+    ap.add_argument("--eval", type=int, default=16, help="held-out systems per kind for the summary")
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     args = ap.parse_args()
     kinds = [k for k in args.kinds.split(",") if k]
-    net, history = train(args.steps, args.seed, args.lr, kinds, args.systems)
+    net, history = train(args.steps, args.seed, args.lr, kinds, args.systems, log_every=500)
     np.save(args.out, net.parameters)
     summary = {}
     for kind in kinds:

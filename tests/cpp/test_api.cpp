@@ -196,6 +196,38 @@ static void refinement()
     EXPECT(rep.finalRelres <= 1e-8);
 }
 
+// FP32 batch with FP64 refinement per system: the same two Poisson systems,
+// closed-form solutions to 1e-10 although the matrix is FP32.
+static void batchedRefinement()
+{
+    COOMatrix<float, Device::GPU> coo(69, 69);
+    for (Index r = 0; r < 8; ++r)
+    {
+        if (r > 0) coo.add(r, r - 1, -1);
+        coo.add(r, r, 2);
+        if (r + 1 < 8) coo.add(r, r + 1, -1);
+    }
+    for (Index r = 0; r < 5; ++r)
+    {
+        if (r > 0) coo.add(64 + r, 64 + r - 1, -2);
+        coo.add(64 + r, 64 + r, 4);
+        if (r + 1 < 5) coo.add(64 + r, 64 + r + 1, -2);
+    }
+    auto a = coo.toCsr();
+    std::vector<double> bh(69, 0.0);
+    for (int i = 0; i < 8; ++i) bh[static_cast<std::size_t>(i)] = 1.0;
+    for (int i = 64; i < 69; ++i) bh[static_cast<std::size_t>(i)] = 2.0;
+    auto b = upload(bh);
+    DenseMatrix<double, Device::GPU> x(69, 1);
+    IterativeSolverWorkspace<float> ws;
+    const auto reps = batchedPcgRefine(a, {0, 64}, {8, 69}, b, x, ws, 1e-12, 10);
+    EXPECT(reps.size() == 2 && reps[0].converged && reps[1].converged);
+    EXPECT(reps[0].final_relres <= 1e-12 && reps[1].final_relres <= 1e-12);
+    const auto got = download(x);
+    for (int i = 0; i < 8; ++i) EXPECT(std::fabs(got[static_cast<std::size_t>(i)] - (i + 1) * (8 - i) / 2.0) <= 1e-10);
+    for (int i = 0; i < 5; ++i) EXPECT(std::fabs(got[static_cast<std::size_t>(64 + i)] - (i + 1) * (5 - i) / 2.0) <= 1e-10);
+}
+
 // Two systems in one batched launch.
 static void batched()
 {
@@ -297,6 +329,7 @@ int main()
         {"block pcg one iteration", blockPcgOneIteration},
         {"fp64 refinement", refinement},
         {"batched pcg", batched},
+        {"batched fp64 refinement", batchedRefinement},
         {"schur", schur},
     };
     for (const auto& c : cases)
