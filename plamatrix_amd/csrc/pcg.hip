@@ -88,6 +88,20 @@ __global__ __launch_bounds__(kBlock) void k_direction(int64_t rows, const S* __r
                                                       SolverState* st, double* partials, int par, int count_res)
 {
     if (st->done != kRunning) return;
+    // the first step's operands are loaded before the cross-workgroup
+    // reduction so their latency overlaps it
+    constexpr int U = 4;
+    int64_t begin, end;
+    block_range(rows, U * kBlock, begin, end);
+    const int64_t first = begin + threadIdx.x;
+    S fz[U], fp[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+    {
+        const int64_t i = first + j * kBlock;
+        fz[j] = i < end ? z[i] : S(0);
+        fp[j] = i < end ? p_old[i] : S(0);
+    }
     double res[2];
     reduce_partials<2>(partials + kPartialsRes, count_res, res);
     const double rr = res[0], rz = res[1];
@@ -126,16 +140,14 @@ __global__ __launch_bounds__(kBlock) void k_direction(int64_t rows, const S* __r
     }
     if (code != kRunning) return;
     const S b = static_cast<S>(beta);
-    constexpr int U = 4;
-    int64_t begin, end;
-    block_range(rows, U * kBlock, begin, end);
-    for (int64_t base = begin + threadIdx.x; base < end; base += U * kBlock)
+    for (int64_t base = first; base < end; base += U * kBlock)
     {
+        const bool pre = base == first;
 #pragma unroll
         for (int j = 0; j < U; ++j)
         {
             const int64_t i = base + j * kBlock;
-            if (i < end) p_new[i] = fma_s(b, p_old[i], z[i]);
+            if (i < end) p_new[i] = fma_s(b, pre ? fp[j] : p_old[i], pre ? fz[j] : z[i]);
         }
     }
 }
@@ -181,6 +193,32 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t units, int64_t bsize,
                                                    SolverState* st, double* partials, int par, int count_den)
 {
     if (st->done != kRunning) return;
+    // rows (None / Jacobi): the first step's operands are loaded before the
+    // cross-workgroup reduction so their latency overlaps it
+    constexpr int U = 4;
+    int64_t ubegin = 0, uend = 0;
+    S fr[U], fp[U], fq[U], fx[U], fi[U];
+    if (PRE != kPreBlock)
+    {
+        block_range(units, U * kBlock, ubegin, uend);
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+        {
+            const int64_t u = ubegin + threadIdx.x + j * kBlock;
+            fr[j] = fp[j] = fq[j] = fx[j] = fi[j] = S(0);
+            if (u < uend)
+            {
+                fr[j] = r[u];
+                if (MODE == kModeStep)
+                {
+                    fp[j] = p[u];
+                    fq[j] = q[u];
+                    fx[j] = x[u];
+                }
+                if (PRE == kPreJacobi) fi[j] = inv[u];
+            }
+        }
+    }
     S alpha = S(0);
     if (MODE == kModeStep)
     {
@@ -232,29 +270,29 @@ __global__ __launch_bounds__(kBlock) void k_update(int64_t units, int64_t bsize,
     }
     else
     {
-        // four independent coalesced units per thread per step
-        constexpr int U = 4;
-        int64_t begin, end;
-        block_range(units, U * kBlock, begin, end);
-        for (int64_t base = begin + threadIdx.x; base < end; base += U * kBlock)
+        // four independent coalesced units per thread per step; the first
+        // step uses the operands loaded before the reduction
+        const int64_t first = ubegin + threadIdx.x;
+        for (int64_t base = first; base < uend; base += U * kBlock)
         {
             S rv[U], zv[U];
 #pragma unroll
             for (int j = 0; j < U; ++j)
             {
                 const int64_t u = base + j * kBlock;
-                if (u < end)
+                if (u < uend)
                 {
-                    S ri = r[u];
+                    const bool pre = base == first;
+                    S ri = pre ? fr[j] : r[u];
                     if (MODE == kModeStep)
                     {
-                        const S pu = p[u], qu = q[u], xu = x[u];
+                        const S pu = pre ? fp[j] : p[u], qu = pre ? fq[j] : q[u], xu = pre ? fx[j] : x[u];
                         x[u] = fma_s(alpha, pu, xu);
                         ri = fma_s(-alpha, qu, ri);
                         r[u] = ri;
                     }
                     rv[j] = ri;
-                    zv[j] = PRE == kPreJacobi ? inv[u] * ri : ri;
+                    zv[j] = PRE == kPreJacobi ? (pre ? fi[j] : inv[u]) * ri : ri;
                     if (PRE == kPreJacobi) z[u] = zv[j];
                 }
                 else
