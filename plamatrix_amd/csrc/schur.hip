@@ -769,10 +769,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
 #pragma clang fp contract(off)
     const int es = ES > 0 ? ES : es_runtime;
     const int lane = threadIdx.x % kWave;
-    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     const unsigned long long mask = (1ull << pb) - 1ull;
-    for (int64_t s = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; s < slots;
-         s += waves)
+    // each workgroup owns a contiguous, XCD-local range of slots: neighbouring
+    // slots share their W C^-1 and W blocks, which then stay in one L2
+    const int64_t wpb = blockDim.x / kWave;
+    const int64_t per = ceil_div(ceil_div(slots, static_cast<int64_t>(gridDim.x)), wpb) * wpb;
+    const int64_t sbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t send = sbegin + per < slots ? sbegin + per : slots;
+    for (int64_t s = sbegin + threadIdx.x / kWave; s < send; s += blockDim.x / kWave)
     {
         if (lane >= ps * ps) continue;
         const int lr = lane / ps, lc = lane % ps;
