@@ -51,6 +51,14 @@ __device__ __forceinline__ int64_t xcd_logical_block(int64_t b, int64_t nb)
     return xcd * q + (xcd < r ? xcd : r) + idx;
 }
 
+// LDS hand-off between lanes of one wavefront.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- deterministic reductions ---------------------------------------------
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)

@@ -753,11 +753,12 @@ __global__ __launch_bounds__(kBlock) void k_transform_cross(int64_t nc, int ps, 
 // diagonal, + each primary cross block (forward, or transposed for the
 // mirrored slot), then - (W_l C^-1)(W_r)^T for every term in order
 // (block_schur_sparse_assembly.h:817-903).  ES > 0 (the eliminated block
-// size known at compile time) loads the operands of kTermBatch terms before
-// accumulating them in order; ES == 0 takes es at run time, one term at a
-// time.
-constexpr int kTermBatch = 8;
-template <int ES>
+// size known at compile time): the whole wavefront stages the W C^-1 and W
+// blocks of kTermStage terms into LDS with coalesced loads (indices first,
+// then data), then each lane accumulates its entry over them in term order.
+// ES == 0 takes es at run time, one term at a time.
+constexpr int kTermStage = 16;
+template <int ES, int PS>
 __global__ __launch_bounds__(kBlock) void k_assemble_slots(
     int64_t slots, int ps, int es_runtime, int pb, const unsigned long long* __restrict__ slot_key,
     const int64_t* __restrict__ slot_start, int64_t entries, const unsigned* __restrict__ term_e,
@@ -767,19 +768,24 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
     double* __restrict__ values)
 {
 #pragma clang fp contract(off)
+    static_assert(ES == 0 || PS > 0, "the staged path needs both block sizes");
+    constexpr int kStageVals = kTermStage * 2 * (PS > 0 ? PS : 1) * (ES > 0 ? ES : 1);
+    constexpr int kStagePerLane = (kStageVals + kWave - 1) / kWave;
+    __shared__ double stage_all[ES > 0 ? kBlock / kWave : 1][ES > 0 ? kStageVals : 1];
     const int es = ES > 0 ? ES : es_runtime;
     const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
     const unsigned long long mask = (1ull << pb) - 1ull;
+    const bool active = lane < ps * ps;
+    const int lr = active ? lane / ps : 0, lc = active ? lane % ps : 0;
     // each workgroup owns a contiguous, XCD-local range of slots: neighbouring
     // slots share their W C^-1 and W blocks, which then stay in one L2
     const int64_t wpb = blockDim.x / kWave;
     const int64_t per = ceil_div(ceil_div(slots, static_cast<int64_t>(gridDim.x)), wpb) * wpb;
     const int64_t sbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
     const int64_t send = sbegin + per < slots ? sbegin + per : slots;
-    for (int64_t s = sbegin + threadIdx.x / kWave; s < send; s += blockDim.x / kWave)
+    for (int64_t s = sbegin + wave; s < send; s += wpb)
     {
-        if (lane >= ps * ps) continue;
-        const int lr = lane / ps, lc = lane % ps;
         const int64_t row = static_cast<int64_t>(slot_key[s] >> pb);
         const int64_t col = static_cast<int64_t>(slot_key[s] & mask);
         const int64_t begin = slot_start[s];
@@ -787,47 +793,71 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
         double acc = 0.0;
         // the run holds terms first, then the diagonal marker, then primary
         // cross markers in index order
-        if (row == col) acc += Bd[row * ps * ps + lr * ps + lc];
+        if (active && row == col) acc += Bd[row * ps * ps + lr * ps + lc];
         // the markers are the few entries at the run's end
         int64_t tend = end;
         while (tend > begin && term_e[tend - 1] >= ~1u) --tend;
-        for (int64_t k = tend; k < end; ++k)
+        if (active)
         {
-            if (term_e[k] != ~1u) continue;
-            const int64_t q = term_left[k];
-            const double* m = pcv + q * ps * ps;
-            acc += term_right[k] == 0 ? m[lr * ps + lc] : m[lc * ps + lr];
+            for (int64_t k = tend; k < end; ++k)
+            {
+                if (term_e[k] != ~1u) continue;
+                const int64_t q = term_left[k];
+                const double* m = pcv + q * ps * ps;
+                acc += term_right[k] == 0 ? m[lr * ps + lc] : m[lc * ps + lr];
+            }
         }
         if (ES > 0)
         {
-            for (int64_t k = begin; k < tend; k += kTermBatch)
+            double* st = stage_all[ES > 0 ? wave : 0];
+            constexpr int bvals = (PS > 0 ? PS : 1) * (ES > 0 ? ES : 1);  // doubles of one W C^-1 or W block
+            constexpr int per_term = 2 * bvals;                                // W C^-1 block, then W block
+            for (int64_t k = begin; k < tend; k += kTermStage)
             {
-                double t[kTermBatch][ES > 0 ? ES : 1], w[kTermBatch][ES > 0 ? ES : 1];
+                const int nt = static_cast<int>(tend - k < kTermStage ? tend - k : kTermStage);
+                const int total = nt * per_term;
+                unsigned id[kStagePerLane];
 #pragma unroll
-                for (int u = 0; u < kTermBatch; ++u)
+                for (int m = 0; m < kStagePerLane; ++m)
                 {
-                    const bool in = k + u < tend;
-                    const int64_t l = in ? term_left[k + u] : 0;
-                    const int64_t r = in ? term_right[k + u] : 0;
+                    const int f = lane + m * kWave;
+                    id[m] = 0u;
+                    if (f < total) id[m] = (f % per_term) < bvals ? term_left[k + f / per_term] : term_right[k + f / per_term];
+                }
+                double val[kStagePerLane];
 #pragma unroll
-                    for (int i = 0; i < ES; ++i)
+                for (int m = 0; m < kStagePerLane; ++m)
+                {
+                    const int f = lane + m * kWave;
+                    const int within = f % per_term;
+                    val[m] = 0.0;
+                    if (f < total)
+                        val[m] = within < bvals ? TW[static_cast<int64_t>(id[m]) * bvals + within]
+                                                : W[static_cast<int64_t>(id[m]) * bvals + (within - bvals)];
+                }
+#pragma unroll
+                for (int m = 0; m < kStagePerLane; ++m)
+                {
+                    const int f = lane + m * kWave;
+                    if (f < total) st[f] = val[m];
+                }
+                wave_lds_sync();
+                if (active)
+                {
+                    for (int u = 0; u < nt; ++u)
                     {
-                        t[u][i] = in ? TW[l * ps * ES + lr * ES + i] : 0.0;
-                        w[u][i] = in ? W[r * ps * ES + lc * ES + i] : 0.0;
+                        const double* t = st + u * per_term + lr * ES;
+                        const double* w = st + u * per_term + bvals + lc * ES;
+                        double value = 0.0;
+#pragma unroll
+                        for (int i = 0; i < ES; ++i) value += t[i] * w[i];
+                        acc += -value;
                     }
                 }
-#pragma unroll
-                for (int u = 0; u < kTermBatch; ++u)
-                {
-                    if (k + u >= tend) break;
-                    double value = 0.0;
-#pragma unroll
-                    for (int i = 0; i < ES; ++i) value += t[u][i] * w[u][i];
-                    acc += -value;
-                }
+                wave_lds_sync();
             }
         }
-        else
+        else if (active)
         {
             for (int64_t k = begin; k < tend; ++k)
             {
@@ -838,7 +868,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
                 acc += -value;
             }
         }
-        values[ro[row * ps + lr] + (s - bro[row]) * ps + lc] = acc;
+        if (active) values[ro[row * ps + lr] + (s - bro[row]) * ps + lc] = acc;
     }
 }
 
@@ -1556,7 +1586,8 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         const int64_t waves_per_block = kBlock / kWave;
         const int slot_grid =
             static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
-        hipLaunchKernelGGL(es == 3 ? k_assemble_slots<3> : k_assemble_slots<0>, dim3(slot_grid), dim3(kBlock), 0,
+        const auto assemble = es == 3 && ps == 6 ? k_assemble_slots<3, 6> : k_assemble_slots<0, 0>;
+        hipLaunchKernelGGL(assemble, dim3(slot_grid), dim3(kBlock), 0,
                            stream, w->slots, ps, es, w->pb, w->slot_key, w->slot_start, w->entries, w->term_e,
                            w->term_left, w->term_right, w->bro, w->ro, w->Bd, eq->primary_cross_values, w->TW,
                            eq->cross_values, w->values);

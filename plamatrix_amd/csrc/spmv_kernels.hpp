@@ -69,14 +69,6 @@ __device__ __forceinline__ T stream_load(const T* p, bool nt)
     return nt ? __builtin_nontemporal_load(p) : *p;
 }
 
-// LDS hand-off between lanes of one wavefront.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Contiguous range [begin, end) of `units` owned by this workgroup
 // (XCD-aware), rounded to multiples of `step`.
 __device__ __forceinline__ void block_range(int64_t units, int64_t step, int64_t& begin, int64_t& end)
