@@ -757,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_transform_cross(int64_t nc, int ps, 
 // blocks of kTermStage terms into LDS with coalesced loads (indices first,
 // then data), then each lane accumulates its entry over them in term order.
 // ES == 0 takes es at run time, one term at a time.
-constexpr int kTermStage = 16;
+constexpr int kTermStage = 8;
 template <int ES, int PS>
 __global__ __launch_bounds__(kBlock) void k_assemble_slots(
     int64_t slots, int ps, int es_runtime, int pb, const unsigned long long* __restrict__ slot_key,
