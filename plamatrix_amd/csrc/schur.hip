@@ -348,6 +348,93 @@ __global__ __launch_bounds__(kBlock) void k_ne_primary(int64_t pc, int rs_runtim
     }
 }
 
+// B_p and g_p with one wavefront per primary (lane = entry, ps*ps + ps of
+// them, RS x PS Jacobians): the wavefront stages the Jacobian rows, residuals
+// and weights of kNeStage residuals into LDS with coalesced loads, then each
+// lane accumulates its entry over them in the primary's residual order, the
+// arithmetic of k_ne_primary.
+constexpr int kNeStage = 16;
+template <int RS, int PS>
+__global__ __launch_bounds__(kBlock) void k_ne_primary_staged(int64_t pc, const int64_t* __restrict__ offs,
+                                                              const unsigned* __restrict__ perm,
+                                                              const double* __restrict__ jp,
+                                                              const double* __restrict__ r,
+                                                              const double* __restrict__ w, double* __restrict__ B,
+                                                              double* __restrict__ g)
+{
+#pragma clang fp contract(off)
+    static_assert(PS * PS + PS <= kWave, "one lane per entry");
+    constexpr int kPer = RS * PS + RS + 1;  // J rows, residual, weight
+    constexpr int kVals = kNeStage * kPer;
+    constexpr int kPerLane = (kVals + kWave - 1) / kWave;
+    __shared__ double stage_all[kBlock / kWave][kVals];
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    double* st = stage_all[wave];
+    const bool active = lane < PS * PS + PS;
+    const bool is_b = lane < PS * PS;
+    const int column = is_b ? lane / PS : lane - PS * PS;
+    const int other = is_b ? lane % PS : 0;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / kWave);
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + wave; p < pc; p += waves)
+    {
+        const int64_t k0 = offs[p], k1 = offs[p + 1];
+        double acc = 0.0;
+        for (int64_t k = k0; k < k1; k += kNeStage)
+        {
+            const int nr = static_cast<int>(k1 - k < kNeStage ? k1 - k : kNeStage);
+            const int total = nr * kPer;
+            unsigned id[kPerLane];
+#pragma unroll
+            for (int m = 0; m < kPerLane; ++m)
+            {
+                const int f = lane + m * kWave;
+                id[m] = f < total ? perm[k + f / kPer] : 0u;
+            }
+            double val[kPerLane];
+#pragma unroll
+            for (int m = 0; m < kPerLane; ++m)
+            {
+                const int f = lane + m * kWave;
+                const int within = f % kPer;
+                const int64_t i = id[m];
+                val[m] = 0.0;
+                if (f < total)
+                    val[m] = within < RS * PS ? jp[i * RS * PS + within]
+                                              : (within < RS * PS + RS ? r[i * RS + (within - RS * PS)] : w[i]);
+            }
+#pragma unroll
+            for (int m = 0; m < kPerLane; ++m)
+            {
+                const int f = lane + m * kWave;
+                if (f < total) st[f] = val[m];
+            }
+            wave_lds_sync();
+            if (active)
+            {
+                for (int u = 0; u < nr; ++u)
+                {
+                    const double* J = st + u * kPer;
+                    const double* res = J + RS * PS;
+                    const double wi = J[RS * PS + RS];
+#pragma unroll
+                    for (int row = 0; row < RS; ++row)
+                    {
+                        const double value = J[row * PS + column];
+                        acc += wi * value * (is_b ? J[row * PS + other] : res[row]);
+                    }
+                }
+            }
+            wave_lds_sync();
+        }
+        if (active)
+        {
+            if (is_b) B[p * PS * PS + lane] = acc;
+            else g[p * PS + column] = acc;
+        }
+    }
+}
+
 // C_e and g_e: thread per (eliminated, entry).
 __global__ __launch_bounds__(kBlock) void k_ne_eliminated(int64_t ec, int rs, int es,
                                                           const int64_t* __restrict__ offs,
@@ -1699,10 +1786,17 @@ int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, 
     const int rs = static_cast<int>(rb->residual_size);
     const int64_t runs = w->ne_runs;
     if (pc * ps > 0)
-        hipLaunchKernelGGL(rs == 2 ? k_ne_primary<2> : k_ne_primary<0>, dim3(grid_for(pc * (ps * ps + ps))),
-                           dim3(kBlock), 0, stream, pc, rs,
-                           static_cast<int>(ps), w->ne_offs_p, w->ne_perm_p, rb->primary_jacobians, rb->residuals,
-                           weights, B, gp);
+    {
+        if (rs == 2 && ps == 6)
+            hipLaunchKernelGGL((k_ne_primary_staged<2, 6>),
+                               dim3(static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(pc, 4), 65536)))),
+                               dim3(kBlock), 0, stream, pc, w->ne_offs_p, w->ne_perm_p, rb->primary_jacobians,
+                               rb->residuals, weights, B, gp);
+        else
+            hipLaunchKernelGGL(rs == 2 ? k_ne_primary<2> : k_ne_primary<0>, dim3(grid_for(pc * (ps * ps + ps))),
+                               dim3(kBlock), 0, stream, pc, rs, static_cast<int>(ps), w->ne_offs_p, w->ne_perm_p,
+                               rb->primary_jacobians, rb->residuals, weights, B, gp);
+    }
     if (ec * es > 0)
         hipLaunchKernelGGL(k_ne_eliminated, dim3(grid_for(ec * (es * es + es))), dim3(kBlock), 0, stream, ec, rs,
                            static_cast<int>(es), w->ne_offs_e, w->ne_perm_e, rb->eliminated_jacobians,
