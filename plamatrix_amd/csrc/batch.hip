@@ -25,6 +25,7 @@
 // iterations, with two chunks in flight.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <string>
@@ -134,8 +135,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_residual(int64_t chunks, const
     }
 }
 
+// p_new = z + beta p_old for the chunks of running systems (SPLIT
+// direction: the SpMV then gathers p_new once per entry).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_pupdate(const int32_t* __restrict__ chunk_sys,
+                                                          const int64_t* __restrict__ sys_end,
+                                                          const S* __restrict__ z, const S* __restrict__ p_old,
+                                                          S* __restrict__ p_new, const BatchState* st,
+                                                          const int64_t* __restrict__ list,
+                                                          const unsigned long long* __restrict__ list_count)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning) continue;
+        const int64_t row = c * kChunkRows + lane;
+        if (row < sys_end[s]) p_new[row] = fma_s(static_cast<S>(st[s].beta), p_old[row], z[row]);
+    }
+}
+
 // p_new = z + beta p_old, q = A p_new, per-chunk FP64 partial p_new.q.
-template <typename S, typename I>
+// SPLIT: p_new was formed by k_batch_pupdate and is gathered directly.
+template <typename S, typename I, bool SPLIT>
 __global__ __launch_bounds__(kBlock) void k_batch_direction(int64_t chunks, const int32_t* __restrict__ chunk_sys,
                                                             const int64_t* __restrict__ sys_end,
                                                             const I* __restrict__ ro, const I* __restrict__ ci,
@@ -163,14 +186,26 @@ __global__ __launch_bounds__(kBlock) void k_batch_direction(int64_t chunks, cons
         const int64_t row0 = c * kChunkRows;
         const S beta = static_cast<S>(st[s].beta);
         int64_t ks, ke;
-        const S acc = chunk_row_sums<S>(row0, end, ro, ci, lv_all[wave], lc_all[wave],
-                                        DirectionSource<S>{v, z, p_old, beta}, ks, ke);
+        S acc;
+        if (SPLIT)
+            acc = chunk_row_sums<S>(row0, end, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, p_new}, ks, ke);
+        else
+            acc = chunk_row_sums<S>(row0, end, ro, ci, lv_all[wave], lc_all[wave],
+                                    DirectionSource<S>{v, z, p_old, beta}, ks, ke);
         const int64_t row = row0 + lane;
         double den = 0.0;
         if (row < end)
         {
-            const S pn = fma_s(beta, p_old[row], z[row]);
-            p_new[row] = pn;
+            S pn;
+            if (SPLIT)
+            {
+                pn = p_new[row];
+            }
+            else
+            {
+                pn = fma_s(beta, p_old[row], z[row]);
+                p_new[row] = pn;
+            }
             q[row] = acc;
             den = static_cast<double>(pn) * static_cast<double>(acc);
         }
@@ -455,6 +490,7 @@ struct BatchProblem
     unsigned long long* done_count;
     int64_t* list;                    // chunks of running systems
     unsigned long long* list_count;
+    bool split;                       // direction update as its own kernel (SpMV gathers p once)
 };
 
 template <typename S, typename I>
@@ -513,7 +549,13 @@ struct BatchCalls
         const bool even = t % 2 == 0;
         const void* p_old = even ? b.p0 : b.p1;
         void* p_new = even ? b.p1 : b.p0;
-        out.push_back(make_call(k_batch_direction<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys,
+        if (b.split)
+            out.push_back(make_call(k_batch_pupdate<S>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys, b.sys_end,
+                                    static_cast<const S*>(b.z), static_cast<const S*>(p_old), static_cast<S*>(p_new),
+                                    static_cast<const BatchState*>(b.st), static_cast<const int64_t*>(b.list),
+                                    static_cast<const unsigned long long*>(b.list_count)));
+        out.push_back(make_call(b.split ? k_batch_direction<S, I, true> : k_batch_direction<S, I, false>,
+                                dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys,
                                 b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
                                 static_cast<const S*>(b.z), static_cast<const S*>(p_old), static_cast<S*>(p_new),
                                 static_cast<S*>(b.q), static_cast<const BatchState*>(b.st), b.pden,
@@ -592,13 +634,13 @@ std::string batch_key(const BatchProblem& b, int what, int64_t first, int64_t co
 {
     char buf[512];
     std::snprintf(buf, sizeof(buf),
-                  "batch|%d|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%lld|%lld|%d|%d|%d|%.17g|%.17g", what, b.r,
+                  "batch|%d|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%lld|%lld|%d|%d|%d|%.17g|%.17g|%d", what, b.r,
                   static_cast<void*>(b.st), static_cast<void*>(b.pden), b.a->values, (const void*)b.a->row_offsets32,
                   (const void*)b.a->row_offsets, (const void*)b.a->col_indices32, (const void*)b.a->col_indices,
                   b.rhs, b.x,
                   static_cast<long long>(b.a->rows), static_cast<long long>(b.systems), b.jacobi ? 1 : 0,
                   static_cast<long long>(first), static_cast<long long>(count), refresh, b.a->dtype,
-                  b.max_iterations, b.rel_tol, b.abs_tol);
+                  b.max_iterations, b.rel_tol, b.abs_tol, b.split ? 1 : 0);
     return buf;
 }
 
@@ -749,6 +791,10 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     b.max_iterations = options->max_iterations;
     b.rel_tol = options->relative_tolerance;
     b.abs_tol = options->absolute_tolerance;
+    // the fused direction measured faster on the 512-system batch (537 vs
+    // 640 ms); PLAMATRIX_AMD_BATCH_SPLIT=1 selects the split form
+    const char* split = std::getenv("PLAMATRIX_AMD_BATCH_SPLIT");
+    b.split = split != nullptr && split[0] == '1';
     b.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(chunks, kBatchWaves), kBatchGrid)));
     b.grid_systems = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(systems, kBatchWaves), 1024)));
     char* vb = ws->vectors.as<char>();
