@@ -64,7 +64,7 @@ def committed_traffic(kernel: str):
     return None
 
 
-WORKLOADS = ("diffusion128", "poisson256", "lm-schur", "batch512", "learned-eval")
+WORKLOADS = ("diffusion128", "poisson256", "lm-schur", "batch512", "learned-eval", "spmm8")
 
 
 def main() -> int:
@@ -102,7 +102,7 @@ def main() -> int:
     from plamatrix_amd import capi, generators
 
     run = {"diffusion128": run_diffusion, "poisson256": run_poisson, "lm-schur": run_schur,
-           "batch512": run_batch, "learned-eval": run_learned_eval}[args.workload]
+           "batch512": run_batch, "learned-eval": run_learned_eval, "spmm8": run_spmm}[args.workload]
     line = run(args, torch, capi, generators)
     print(json.dumps(line))
     return 0
@@ -688,6 +688,61 @@ def run_learned_eval(args, torch, capi, generators):
                       "weights": os.path.relpath(args.weights, ROOT)},
                      None, None)
     line["per_kind"] = out
+    return line
+
+
+def run_spmm(args, torch, capi, generators):
+    """Y = A B with 8 column-major right-hand sides on the 128^3 diffusion
+    matrix (FP32, int32 indices), rotating over four copies of the matrix as
+    the SpMV measurement does, against eight separate SpMVs."""
+    dev = torch.device("cuda:0")
+    k = 8
+    h = generators.diffusion3d_csr(args.grid)
+    n, nnz = h.rows, h.nnz
+    a = capi.to_device_csr(h, compact=True)
+    copies = [a] + [capi.DeviceCSR(n, n, None, None, a.values.clone(), a.row_offsets32.clone(),
+                                   a.col_indices32.clone()) for _ in range(3)]
+    bs = [torch.randn((k, n), dtype=torch.float32, device=dev) for _ in copies]
+    ys = [torch.empty((k, n), dtype=torch.float32, device=dev) for _ in copies]
+    ws = capi.Workspace()
+    rounds = 20
+
+    def spmm_all():
+        for i in range(len(copies)):
+            capi.spmm(copies[i], bs[i], ys[i], ws)
+
+    def spmv_all():
+        for i in range(len(copies)):
+            for j in range(k):
+                capi.spmv(copies[i], bs[i][j], ys[i][j], ws)
+
+    spmm_ms = _event_ms(torch, spmm_all, rounds) / len(copies)
+    spmv_ms = _event_ms(torch, spmv_all, 5) / len(copies)
+    byts = nnz * 8 + (n + 1) * 4 + 2 * k * n * 4
+    gbs = byts / (spmm_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        use_ref = oracle.ref_available()
+        csr = oracle.CSR(n, n, h.row_offsets, h.col_indices, h.values)
+        bh = np.asfortranarray(bs[0].cpu().numpy().T)
+        c_s, _ = _cpu_median_s(lambda: oracle.spmm(csr, bh, use_ref=use_ref), 3)
+        cpu = {"value": round(c_s * 1e3, 1), "unit": "ms", "cores": 1, "kind": "reference" if use_ref else "port",
+               "sample": "full workload, median of 3: the reference's own CPU spmm (built from source, serial)"
+                         if use_ref else "full workload, median of 3: the oracle's CPU spmm"}
+    line = base_line(args, "CSR SpMM time, 8 right-hand sides, 3-D 7-point diffusion 128^3", round(spmm_ms, 4), "ms",
+                     False, "f32",
+                     {"workload": f"Y = A B, A 3-D 7-point diffusion {args.grid}^3 (n={n}, nnz={nnz}) FP32/int32, "
+                                  "B and Y 8 column-major columns; four rotated matrix copies", "rows": n, "nnz": nnz,
+                      "columns": k},
+                     {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                      "kernel": "k_spmm_pack<float> + k_spmm_tile<float,int> (one 8-column tile)",
+                      "bytes_per_launch": byts, "ms_per_launch": round(spmm_ms, 5)},
+                     cpu)
+    line["eight_spmv_ms"] = round(spmv_ms, 4)
+    line["speedup_vs_eight_spmv"] = round(spmv_ms / spmm_ms, 2)
     return line
 
 if __name__ == "__main__":

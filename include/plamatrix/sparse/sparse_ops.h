@@ -94,10 +94,12 @@ template <typename Scalar>
 void spmvAsync(const CSRMatrix<Scalar, Device::GPU>& csr, const DenseMatrix<Scalar, Device::GPU>& x,
                DenseMatrix<Scalar, Device::GPU>& output, SparseOpsWorkspace& workspace, hipStream_t stream = nullptr)
 {
+    // dimension mismatches are runtime_error, aliasing invalid_argument, as in
+    // the reference (sparse_ops_cpu.cpp:138-156, sparse_ops_test.cpp:791-800)
     if (x.rows() != csr.cols() || x.cols() != 1)
-        throw std::invalid_argument("spmv GPU dimension mismatch");
+        throw std::runtime_error("spmv GPU dimension mismatch");
     if (output.rows() != csr.rows() || output.cols() != 1)
-        throw std::invalid_argument("spmv GPU output dimension mismatch");
+        throw std::runtime_error("spmv GPU output dimension mismatch");
     const pm_csr d = csr.descriptor(true, stream);
     detail::check(pm_spmv(&d, x.data(), output.data(), workspace.handle(), stream));
 }
@@ -119,6 +121,39 @@ DenseMatrix<Scalar, Device::GPU> spmv(const CSRMatrix<Scalar, Device::GPU>& csr,
 {
     DenseMatrix<Scalar, Device::GPU> y(csr.rows(), 1);
     spmv(csr, x, y);
+    return y;
+}
+
+/// Enqueue Y = A B (column-major B and Y) on `stream` without host
+/// synchronisation; the matrix streams once for all columns.
+template <typename Scalar>
+void spmmAsync(const CSRMatrix<Scalar, Device::GPU>& csr, const DenseMatrix<Scalar, Device::GPU>& B,
+               DenseMatrix<Scalar, Device::GPU>& output, SparseOpsWorkspace& workspace, hipStream_t stream = nullptr)
+{
+    if (B.rows() != csr.cols()) throw std::runtime_error("spmm dimension mismatch");
+    if (output.rows() != csr.rows() || output.cols() != B.cols())
+        throw std::runtime_error("spmm output dimension mismatch");
+    const pm_csr d = csr.descriptor(true, stream);
+    detail::check(pm_spmm(&d, B.data(), B.cols(), output.data(), workspace.handle(), stream));
+}
+
+/// Synchronous Y = A B into existing storage.
+template <typename Scalar>
+void spmm(const CSRMatrix<Scalar, Device::GPU>& csr, const DenseMatrix<Scalar, Device::GPU>& B,
+          DenseMatrix<Scalar, Device::GPU>& output)
+{
+    SparseOpsWorkspace ws;
+    spmmAsync(csr, B, output, ws, nullptr);
+    detail::checkHip(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+}
+
+/// Synchronous Y = A B.
+template <typename Scalar>
+DenseMatrix<Scalar, Device::GPU> spmm(const CSRMatrix<Scalar, Device::GPU>& csr,
+                                      const DenseMatrix<Scalar, Device::GPU>& B)
+{
+    DenseMatrix<Scalar, Device::GPU> y(csr.rows(), B.cols());
+    spmm(csr, B, y);
     return y;
 }
 

@@ -120,6 +120,14 @@ int pm_workspace_check_status(pm_workspace* ws);
  * y = A x (spmvAsync, sparse_ops.h:158-166; sparse_ops.cu:687-740).  x and y
  * have a->dtype; accumulation in that precision.  x and y must not alias. */
 int pm_spmv(const pm_csr* a, const void* x, void* y, pm_workspace* ws, hipStream_t stream);
+/* spmmAsync (sparse_ops.h:180-188; CPU rule sparse_ops_cpu.cpp:244-268):
+ * Y = A B with B (a->cols x k) and Y (a->rows x k) column-major, leading
+ * dimensions a->cols and a->rows, in a's dtype.  Each (row, column) sum runs
+ * left to right over the row in the scalar type.  B == Y is rejected
+ * (invalid_argument); k == 0 or a->rows == 0 does nothing; nnz == 0 writes
+ * zeros.  ws is required (it holds a packed copy of 8 columns of B at a
+ * time).  Stream-ordered, no host synchronisation. */
+int pm_spmm(const pm_csr* a, const void* b, int64_t k, void* y, pm_workspace* ws, hipStream_t stream);
 /* Same with an explicit lanes-per-row choice (1..64, power of two); used by
  * the autotuner. */
 int pm_spmv_lanes(const pm_csr* a, const void* x, void* y, int32_t lanes_per_row, pm_workspace* ws,

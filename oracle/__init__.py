@@ -167,6 +167,23 @@ def spmv(csr: CSR, x: np.ndarray, use_ref=False) -> np.ndarray:
     return y
 
 
+def spmm(csr: CSR, b: np.ndarray, use_ref=False) -> np.ndarray:
+    """spmm (sparse_ops_cpu.cpp:244-268): B (cols x k) and the result (rows x k)
+    column-major, Scalar accumulation per column."""
+    b = np.asfortranarray(np.asarray(b, dtype=csr.values.dtype).reshape(csr.cols, -1))
+    k = b.shape[1]
+    y = np.zeros((csr.rows, k), dtype=csr.values.dtype, order="F")
+    sfx = _suffix(csr.values.dtype)
+    if use_ref:
+        _check(getattr(ref(), "ref_spmm_" + sfx)(_i64(csr.rows), _i64(csr.cols), _i64(csr.nnz),
+                                                  _ptr(csr.row_offsets), _ptr(csr.col_indices),
+                                                  _ptr(csr.values), _i64(k), _ptr(b), _ptr(y)), ref())
+    else:
+        _check(getattr(lib(), "or_spmm_" + sfx)(_i64(csr.rows), _i64(csr.cols), _i64(k), _ptr(csr.row_offsets),
+                                                 _ptr(csr.col_indices), _ptr(csr.values), _ptr(b), _ptr(y)))
+    return y
+
+
 def _options(max_iterations=1000, relative_tolerance=1e-6, absolute_tolerance=0.0, use_jacobi=True,
              require_convergence=False) -> _Options:
     return _Options(int(max_iterations), float(relative_tolerance), float(absolute_tolerance),

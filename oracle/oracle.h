@@ -19,6 +19,7 @@
  *                     :73-118, :276-293, :320-369, :371-430), brute-force
  *                     permutation check on tiny inputs, oracle/_ref
  *   or_spmv_*         KATs sparse_ops_test.cpp:120-193, row-sum closed form
+ *   or_spmm_*         KATs sparse_ops_test.cpp:196-268, columns = SpMV, reference build
  *   or_solve_*        closed forms iterative_solver_test.cpp:103-335, 782-820,
  *                     scipy.sparse.linalg.spsolve, oracle/_ref
  *   or_block_pcg_*    2x2 block system in one iteration
@@ -66,6 +67,12 @@ int or_spmv_f64(int64_t rows, const int64_t* row_offsets,
 int or_spmv_f32(int64_t rows, const int64_t* row_offsets,
                 const int64_t* col_indices, const float* values,
                 const float* x, float* y);
+
+/* ---- SpMM (sparse_ops_cpu.cpp:244-268): per column as SpMV, column-major */
+int or_spmm_f64(int64_t rows, int64_t cols, int64_t k, const int64_t* row_offsets,
+                const int64_t* col_indices, const double* values, const double* b, double* y);
+int or_spmm_f32(int64_t rows, int64_t cols, int64_t k, const int64_t* row_offsets,
+                const int64_t* col_indices, const float* values, const float* b, float* y);
 
 /* ---- CG / Jacobi-PCG (iterative_solver_cpu.cpp:15-304) ----------------- */
 typedef struct or_options {

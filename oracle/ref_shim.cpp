@@ -97,6 +97,19 @@ int spmvImpl(Index rows, Index cols, Index nnz, const Index* ro, const Index* ci
     });
 }
 
+template <typename S>
+int spmmImpl(Index rows, Index cols, Index nnz, const Index* ro, const Index* ci, const S* v, Index k, const S* b,
+             S* y)
+{
+    return guarded([&] {
+        auto m = makeCsr(rows, cols, nnz, ro, ci, v);
+        DenseMatrix<S, Device::CPU> bin(cols, k);
+        if (cols * k > 0) std::memcpy(bin.data(), b, sizeof(S) * static_cast<std::size_t>(cols * k));
+        auto out = spmm(m, bin);
+        if (rows * k > 0) std::memcpy(y, out.data(), sizeof(S) * static_cast<std::size_t>(rows * k));
+    });
+}
+
 struct RefOptions
 {
     int maxIterations;
@@ -182,6 +195,18 @@ int ref_spmv_f32(Index rows, Index cols, Index nnz, const Index* ro, const Index
                  const float* x, float* y)
 {
     return spmvImpl(rows, cols, nnz, ro, ci, v, x, y);
+}
+
+int ref_spmm_f64(Index rows, Index cols, Index nnz, const Index* ro, const Index* ci, const double* v, Index k,
+                 const double* b, double* y)
+{
+    return spmmImpl(rows, cols, nnz, ro, ci, v, k, b, y);
+}
+
+int ref_spmm_f32(Index rows, Index cols, Index nnz, const Index* ro, const Index* ci, const float* v, Index k,
+                 const float* b, float* y)
+{
+    return spmmImpl(rows, cols, nnz, ro, ci, v, k, b, y);
 }
 
 int ref_solve_f64(Index rows, Index cols, Index nnz, const Index* ro, const Index* ci, const double* v,

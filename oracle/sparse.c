@@ -5,6 +5,7 @@
  * Follows:
  *   src/sparse/sparse_ops_cpu.cpp:24-134   combineCoo / writeCombinedCoo
  *   src/sparse/sparse_ops_cpu.cpp:214-233  spmv
+ *   src/sparse/sparse_ops_cpu.cpp:244-268  spmm
  *   src/sparse/iterative_solver_cpu.cpp:15-284  validateOptions, validateSystem,
  *                                          dot, multiply, jacobiInverse, solve
  */
@@ -149,6 +150,34 @@ DEFINE_COO_TO_CSR(f32, float)
 
 DEFINE_SPMV(f64, double)
 DEFINE_SPMV(f32, float)
+
+/* ------------------------------------------------------------------------ */
+/* spmm: sparse_ops_cpu.cpp:244-268.  Column by column, `sum += v*b` in
+ * Scalar precision, B (cols x k) and Y (rows x k) column-major. */
+#define DEFINE_SPMM(SUFFIX, Scalar)                                                        \
+    int or_spmm_##SUFFIX(int64_t rows, int64_t cols, int64_t k, const int64_t* row_offsets, \
+                         const int64_t* col_indices, const Scalar* values,                 \
+                         const Scalar* b, Scalar* y)                                       \
+    {                                                                                      \
+        if (b == y && b != NULL)                                                           \
+            return or_fail(OR_INVALID_ARGUMENT, "spmm input and output data must not alias"); \
+        for (int64_t col = 0; col < k; ++col)                                              \
+        {                                                                                  \
+            const Scalar* bc = b + col * cols;                                             \
+            Scalar* yc = y + col * rows;                                                   \
+            for (int64_t row = 0; row < rows; ++row)                                       \
+            {                                                                              \
+                Scalar sum = (Scalar)0;                                                    \
+                for (int64_t p = row_offsets[row]; p < row_offsets[row + 1]; ++p)          \
+                    sum += values[p] * bc[col_indices[p]];                                 \
+                yc[row] = sum;                                                             \
+            }                                                                              \
+        }                                                                                  \
+        return OR_OK;                                                                      \
+    }
+
+DEFINE_SPMM(f64, double)
+DEFINE_SPMM(f32, float)
 
 /* ------------------------------------------------------------------------ */
 /* iterative_solver_cpu.cpp:15-29 */

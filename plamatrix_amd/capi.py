@@ -94,7 +94,7 @@ class pm_schur_report(ctypes.Structure):
 # Every symbol declared in include/plamatrix_amd.h (checked by the CPU tests).
 EXPORTS = [
     "pm_last_error", "pm_version", "pm_workspace_create", "pm_workspace_destroy", "pm_workspace_close",
-    "pm_workspace_capacity_bytes", "pm_workspace_capacity_rows", "pm_csr_validate", "pm_csr_compact_indices",
+    "pm_workspace_capacity_bytes", "pm_workspace_capacity_rows", "pm_csr_validate", "pm_spmm", "pm_csr_compact_indices",
     "pm_coo_to_csr_prepare", "pm_coo_to_csr_nnz", "pm_coo_to_csr_finish", "pm_coo_to_csr_async",
     "pm_workspace_check_status", "pm_spmv", "pm_spmv_lanes", "pm_autotune_spmv", "pm_tuned_lanes", "pm_tune_record", "pm_true_relres", "pm_cg", "pm_pcg",
     "pm_block_pcg", "pm_solve_fixed_async", "pm_fixed_finalize", "pm_fixed_state_destroy", "pm_pcg_refine",
@@ -262,6 +262,15 @@ def spmv(a: DeviceCSR, x, y, ws: Workspace | None = None, lanes: int | None = No
     else:
         check(lib().pm_spmv(ctypes.byref(d), _ptr(x), _ptr(y), h, current_stream()))
     return y
+
+
+def spmm(a: DeviceCSR, b, y, ws: Workspace | None = None):
+    """Y = A B (pm_spmm) with B (cols x k) and Y (rows x k) column-major:
+    torch tensors of shape (k, cols) and (k, rows), one column per row."""
+    k = int(b.shape[0]) if b.dim() == 2 else 1
+    d = a.descriptor()
+    ws = ws if ws is not None else Workspace()
+    check(lib().pm_spmm(ctypes.byref(d), _ptr(b), _i64(k), _ptr(y), ws.handle, current_stream()))
 
 
 def validate(a: DeviceCSR, ws: Workspace):

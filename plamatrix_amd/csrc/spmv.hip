@@ -61,6 +61,33 @@ int spmv_stream(const pm_csr* a, const void* x, void* y, hipStream_t stream)
     return PM_OK;
 }
 
+// Y = A B in tiles of kSpmmTile columns through a packed row-major copy of
+// the tile's columns of B in the workspace scratch.
+template <typename S>
+int spmm_tiles(const pm_csr* a, const S* b, int64_t k, S* y, pm_workspace* ws, hipStream_t stream)
+{
+    const int64_t cols = a->cols > 0 ? a->cols : 1;
+    PM_TRY(ws->scratch.reserve(sizeof(S) * static_cast<size_t>(cols) * kSpmmTile + 256));
+    S* bt = ws->scratch.as<S>();
+    const int grid = static_cast<int>(std::min<int64_t>(ceil_div(chunk_count(a->rows), kWavesPerBlock), kSpmvGrid));
+    const int pgrid = static_cast<int>(std::min<int64_t>(ceil_div(cols, kBlock), 8192));
+    for (int64_t j0 = 0; j0 < k; j0 += kSpmmTile)
+    {
+        const int64_t kt = k - j0 < kSpmmTile ? k - j0 : kSpmmTile;
+        if (a->cols > 0) hipLaunchKernelGGL(k_spmm_pack<S>, dim3(pgrid), dim3(kBlock), 0, stream, a->cols, k, j0, b, bt);
+        if (use_compact(a))
+            hipLaunchKernelGGL((k_spmm_tile<S, int32_t>), dim3(grid), dim3(kBlock), 0, stream, a->rows, kt,
+                               a->row_offsets32, a->col_indices32, static_cast<const S*>(a->values), bt,
+                               y + j0 * a->rows);
+        else
+            hipLaunchKernelGGL((k_spmm_tile<S, int64_t>), dim3(grid), dim3(kBlock), 0, stream, a->rows, kt,
+                               a->row_offsets, a->col_indices, static_cast<const S*>(a->values), bt,
+                               y + j0 * a->rows);
+        PM_HIP(hipGetLastError());
+    }
+    return PM_OK;
+}
+
 // lanes == 0: wave-stream kernel; lanes >= 1: lanes-per-row kernel.
 int spmv_launch(const pm_csr* a, const void* x, void* y, int lanes, hipStream_t stream)
 {
@@ -165,6 +192,21 @@ int pm_spmv(const pm_csr* a, const void* x, void* y, pm_workspace* ws, hipStream
 {
     PM_TRY(check_csr(a, "spmv"));
     return pm_spmv_lanes(a, x, y, tuned_lanes(a), ws, stream);
+}
+
+int pm_spmm(const pm_csr* a, const void* b, int64_t k, void* y, pm_workspace* ws, hipStream_t stream)
+{
+    PM_TRY(check_csr(a, "spmm"));
+    if (k < 0) return fail(PM_INVALID_ARGUMENT, "spmm: negative column count");
+    if (!ws) return fail(PM_INVALID_ARGUMENT, "spmm: null workspace");
+    if (a->rows > 0 && k > 0 && !y) return fail(PM_INVALID_ARGUMENT, "spmm: null output");
+    if (a->cols > 0 && k > 0 && !b) return fail(PM_INVALID_ARGUMENT, "spmm: null input");
+    if (b != nullptr && b == y) return fail(PM_INVALID_ARGUMENT, "spmm input and output data must not alias");
+    PM_TRY(ws->bind(stream));
+    if (a->rows == 0 || k == 0) return PM_OK;
+    return a->dtype == PM_F32 ? spmm_tiles<float>(a, static_cast<const float*>(b), k, static_cast<float*>(y), ws, stream)
+                              : spmm_tiles<double>(a, static_cast<const double*>(b), k, static_cast<double*>(y), ws,
+                                                   stream);
 }
 
 int pm_csr_validate(const pm_csr* a, pm_workspace* ws, hipStream_t stream)

@@ -370,6 +370,117 @@ __device__ __forceinline__ void publish_partials(double (&v)[NV], double* __rest
     }
 }
 
+// SpMM (spmm, sparse_ops_cpu.cpp:244-268) with B (cols x k) and Y (rows x k)
+// column-major, in tiles of kSpmmTile columns: k_spmm_pack copies the tile's
+// columns of B into a row-major cols x kSpmmTile block (zero past column k),
+// so an entry's operands are one contiguous 32- or 64-byte load; k_spmm_tile
+// stages each 64-row chunk's values and column indices in LDS once and walks
+// them for all columns of the tile.  Every (row, column) sum runs left to
+// right with the SpMV's FMA order, so a column of Y equals the SpMV of that
+// column of B bit for bit.  Chunks too long for one LDS window take the SpMV
+// chunk routine once per column.
+constexpr int kSpmmTile = 8;
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_spmm_pack(int64_t cols, int64_t k, int64_t j0, const S* __restrict__ b,
+                                                      S* __restrict__ bt)
+{
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < cols;
+         i += static_cast<int64_t>(gridDim.x) * kBlock)
+    {
+        S row[kSpmmTile];
+#pragma unroll
+        for (int t = 0; t < kSpmmTile; ++t) row[t] = j0 + t < k ? b[(j0 + t) * cols + i] : S(0);
+#pragma unroll
+        for (int t = 0; t < kSpmmTile; ++t) bt[i * kSpmmTile + t] = row[t];
+    }
+}
+
+// Column t of the packed block as an SpMV operand.
+template <typename S>
+struct PackedSource
+{
+    const S* __restrict__ v;
+    const S* __restrict__ bt;
+    int t;
+    using V = S;
+    using G = S;
+    __device__ __forceinline__ S value(int64_t k, bool nt) const { return stream_load(v + k, nt); }
+    template <typename I>
+    __device__ __forceinline__ S gather(I c) const
+    {
+        return bt[static_cast<int64_t>(c) * kSpmmTile + t];
+    }
+    __device__ __forceinline__ S madd(S a, S g, S acc) const { return fma_s(a, g, acc); }
+};
+
+template <typename S, typename I>
+__global__ __launch_bounds__(kBlock) void k_spmm_tile(int64_t rows, int64_t kt, const I* __restrict__ ro,
+                                                      const I* __restrict__ ci, const S* __restrict__ v,
+                                                      const S* __restrict__ bt, S* __restrict__ y)
+{
+    __shared__ S lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    S* lv = lv_all[wave];
+    I* lc = lc_all[wave];
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
+    {
+        const int64_t row0 = c * kChunkRows;
+        const int64_t last = row0 + kChunkRows < rows ? row0 + kChunkRows : rows;
+        const int64_t r = row0 + lane;
+        const I s_i = ro[r < last ? r : last];
+        const I e_i = ro[r + 1 < last ? r + 1 : last];
+        const int64_t base = static_cast<int64_t>(lane_value(s_i, 0));
+        const int64_t total = static_cast<int64_t>(lane_value(e_i, kWave - 1)) - base;
+        if (total > kStreamCap)
+        {
+            for (int t = 0; t < kt; ++t)
+            {
+                int64_t s, e;
+                const S acc = chunk_row_sums<S>(row0, rows, ro, ci, lv, lc, PackedSource<S>{v, bt, t}, s, e);
+                if (r < rows) y[t * rows + r] = acc;
+            }
+            continue;
+        }
+        const int n = static_cast<int>(total);
+#pragma unroll
+        for (int j = 0; j < kStageLoads; ++j)
+        {
+            const int kk = lane + j * kWave;
+            if (kk < n)
+            {
+                lv[kk] = v[base + kk];
+                lc[kk] = ci[base + kk];
+            }
+        }
+        wave_lds_sync();
+        const int s = static_cast<int>(static_cast<int64_t>(s_i) - base);
+        const int e = static_cast<int>(static_cast<int64_t>(e_i) - base);
+        S acc[kSpmmTile];
+#pragma unroll
+        for (int t = 0; t < kSpmmTile; ++t) acc[t] = S(0);
+        for (int q = s; q < e; ++q)
+        {
+            const S a = lv[q];
+            const S* g = bt + static_cast<int64_t>(lc[q]) * kSpmmTile;
+            S gv[kSpmmTile];
+#pragma unroll
+            for (int t = 0; t < kSpmmTile; ++t) gv[t] = g[t];
+#pragma unroll
+            for (int t = 0; t < kSpmmTile; ++t) acc[t] = fma_s(a, gv[t], acc[t]);
+        }
+        if (r < rows)
+        {
+#pragma unroll
+            for (int t = 0; t < kSpmmTile; ++t)
+                if (t < kt) y[t * rows + r] = acc[t];
+        }
+        wave_lds_sync();
+    }
+}
+
 // y = A x (standalone; any grid size, no reductions).  Streams the matrix
 // once from HBM: 4-byte values and indices are staged by global_load_lds,
 // which measured 27.8 us against 31.1 us for VGPR staging on the 128^3

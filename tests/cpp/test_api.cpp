@@ -94,7 +94,34 @@ static void spmvKat()
     const auto y = download(spmv(a, x));
     EXPECT((y == std::vector<Scalar>{9, -2, 20}));
     DenseMatrix<Scalar, Device::GPU> wrong(2, 1);
-    EXPECT(throws<std::invalid_argument>([&] { spmv(a, x, wrong); }));
+    EXPECT(throws<std::runtime_error>([&] { spmv(a, x, wrong); }));
+}
+
+// sparse_ops_test.cpp:196-233 and 704-737: column-major SpMM, output reuse.
+template <typename Scalar>
+static void spmmKat()
+{
+    COOMatrix<Scalar, Device::GPU> coo(3, 3);
+    coo.add(1, 1, 4);
+    coo.add(0, 2, 2);
+    coo.add(2, 0, -1);
+    coo.add(0, 0, 3);
+    auto a = coo.toCsr();
+    auto b = upload<Scalar>({1, 2, 3, 10, 20, 30});  // 3 x 2, column-major
+    DenseMatrix<Scalar, Device::GPU> bm(3, 2);
+    detail::checkHip(hipMemcpy(bm.data(), b.data(), 6 * sizeof(Scalar), hipMemcpyDeviceToDevice), "hipMemcpy");
+    const auto y = download(spmm(a, bm));
+    EXPECT((y == std::vector<Scalar>{9, 8, -1, 90, 80, -10}));
+    DenseMatrix<Scalar, Device::GPU> reuse(3, 2);
+    Scalar* before = reuse.data();
+    spmm(a, bm, reuse);
+    EXPECT(reuse.data() == before && download(reuse) == y);
+    DenseMatrix<Scalar, Device::GPU> wrong(2, 2), square(3, 3);
+    EXPECT(throws<std::runtime_error>([&] { spmm(a, bm, wrong); }));
+    SparseOpsWorkspace ws;
+    EXPECT(throws<std::invalid_argument>([&] { spmmAsync(a, square, square, ws); }));
+    auto x3 = upload<Scalar>({1, 2, 3});
+    EXPECT(throws<std::invalid_argument>([&] { spmvAsync(a, x3, x3, ws); }));
 }
 
 template <typename Scalar>
@@ -324,6 +351,8 @@ int main()
         {"coo cancellation double", [] { cooCancellation<double>(1e16); }},
         {"spmv kat float", [] { spmvKat<float>(); }},
         {"spmv kat double", [] { spmvKat<double>(); }},
+        {"spmm kat float", [] { spmmKat<float>(); }},
+        {"spmm kat double", [] { spmmKat<double>(); }},
         {"pcg closed form double", [] { pcgClosedForm<double>(); }},
         {"pcg closed form float", [] { pcgClosedForm<float>(); }},
         {"block pcg one iteration", blockPcgOneIteration},

@@ -469,3 +469,86 @@ def test_autotuner_picks_a_valid_width(dev, ws):
     lanes, ms = capi.autotune_spmv(a, x, y, repeats=5)
     assert lanes in (0, 1, 2, 4, 8, 16, 32, 64) and ms > 0
     assert capi.lib().pm_tuned_lanes(capi.ctypes.byref(a.descriptor())) == lanes
+
+
+# ---- SpMM -----------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spmm_kat(dev, ws, oracle, dtype):
+    # sparse_ops_test.cpp:704-737 (column-major B and output)
+    ref = oracle.coo_to_csr(3, 3, [1, 0, 2, 0], [1, 2, 0, 0], np.array([4, 2, -1, 3], dtype))
+    a = capi.to_device_csr(ref)
+    b = t(np.array([[1, 2, 3], [10, 20, 30]], dtype), dev)  # column j of B is row j here
+    y = torch.full((2, 3), -100, dtype=b.dtype, device=dev)
+    capi.spmm(a, b, y, ws)
+    assert host(y).tolist() == [[9, 8, -1], [90, 80, -10]]
+    with pytest.raises(capi.PlamatrixError) as e:
+        capi.spmm(a, b, b, ws)
+    assert kinds(e) == "invalid_argument"
+    empty = capi.to_device_csr(oracle.coo_to_csr(0, 3, [], [], np.zeros(0, dtype)))
+    capi.spmm(empty, b, torch.empty((2, 0), dtype=b.dtype, device=dev), ws)
+
+
+@pytest.mark.parametrize("k", [1, 3, 8, 13])
+@pytest.mark.parametrize("compact", [False, True])
+def test_spmm_random_against_oracle(dev, ws, oracle, k, compact):
+    rng = np.random.default_rng(k)
+    rows, cols = 3000, 2500
+    counts = rng.integers(0, 90, rows)
+    counts[::7] = 0
+    r = np.repeat(np.arange(rows), counts)
+    c = rng.integers(0, cols, len(r))
+    v = rng.standard_normal(len(r))
+    ref = oracle.coo_to_csr(rows, cols, r, c, v)
+    b = rng.standard_normal((cols, k))
+    exact = oracle.spmm(ref, b)
+    a = capi.to_device_csr(ref, compact=compact)
+    y = torch.empty((k, rows), dtype=torch.float64, device=dev)
+    capi.spmm(a, t(np.ascontiguousarray(b.T), dev), y, ws)
+    np.testing.assert_allclose(host(y).T, exact, rtol=1e-12, atol=1e-12)
+    ref32 = oracle.CSR(rows, cols, ref.row_offsets, ref.col_indices, ref.values.astype(np.float32))
+    b32 = b.astype(np.float32)
+    a32 = capi.to_device_csr(ref32, compact=compact)
+    y32 = torch.empty((k, rows), dtype=torch.float32, device=dev)
+    capi.spmm(a32, t(np.ascontiguousarray(b32.T), dev), y32, ws)
+    abs_ref = oracle.CSR(rows, cols, ref.row_offsets, ref.col_indices, np.abs(ref32.values.astype(np.float64)))
+    exact32 = oracle.spmm(oracle.CSR(rows, cols, ref.row_offsets, ref.col_indices, ref32.values.astype(np.float64)),
+                          b32.astype(np.float64))
+    scale = oracle.spmm(abs_ref, np.abs(b32.astype(np.float64)))
+    assert np.all(np.abs(host(y32).T - exact32) <= 4e-6 * scale + 1e-30)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_spmm_columns_equal_spmv_including_long_rows(dev, ws, oracle, dtype):
+    """Each column of A B is bit-for-bit the SpMV of that column (same
+    left-to-right FMA order), on short, long and empty rows."""
+    rng = np.random.default_rng(7)
+    rows, cols = 700, 4000
+    counts = rng.integers(0, 12, rows)
+    counts[100:180] = 300
+    counts[::11] = 0
+    r = np.repeat(np.arange(rows), counts)
+    c = rng.integers(0, cols, len(r))
+    ref = oracle.coo_to_csr(rows, cols, r, c, rng.standard_normal(len(r)).astype(dtype))
+    a = capi.to_device_csr(ref, compact=True)
+    b = t(rng.standard_normal((5, cols)).astype(dtype), dev)
+    y = torch.empty((5, rows), dtype=b.dtype, device=dev)
+    capi.spmm(a, b, y, ws)
+    for j in range(5):
+        yj = torch.empty(rows, dtype=b.dtype, device=dev)
+        capi.spmv(a, b[j].contiguous(), yj, ws, lanes=0)
+        assert torch.equal(y[j], yj), j
+
+
+def test_spmm_diffusion3d_full_size_eight_columns(dev, ws):
+    """128^3 diffusion, 8 right-hand sides (the probe count of the learned
+    preconditioner): every column equals its SpMV bit for bit."""
+    h = generators.diffusion3d_csr(128)
+    a = capi.to_device_csr(h, compact=True)
+    b = torch.randn((8, h.rows), dtype=torch.float32, device=dev)
+    y = torch.empty_like(b)
+    capi.spmm(a, b, y, ws)
+    yj = torch.empty(h.rows, dtype=torch.float32, device=dev)
+    for j in range(8):
+        capi.spmv(a, b[j].contiguous(), yj, ws)
+        assert torch.equal(y[j], yj), j

@@ -180,6 +180,52 @@ def test_spmv_matches_reference_bitwise(ref_lib):
         assert np.array_equal(ref_lib.spmv(csr, x).view(np.uint8), ref_lib.spmv(csr, x, use_ref=True).view(np.uint8))
 
 
+def test_spmm_kat_column_major(oracle):
+    # sparse_ops_test.cpp:196-233
+    for dtype in (np.float32, np.float64):
+        csr = oracle.coo_to_csr(3, 3, [1, 0, 2, 0], [1, 2, 0, 0], np.array([4.0, 2.0, -1.0, 3.0], dtype))
+        b = np.array([[1.0, 10.0], [2.0, 20.0], [3.0, 30.0]], dtype)
+        y = oracle.spmm(csr, b)
+        assert y.tolist() == [[9.0, 90.0], [8.0, 80.0], [-1.0, -10.0]]
+
+
+def test_spmm_empty_and_aliasing(oracle):
+    # sparse_ops_test.cpp:237-265
+    csr = oracle.coo_to_csr(0, 3, [], [], np.zeros(0, np.float32))
+    assert oracle.spmm(csr, np.zeros((3, 2), np.float32)).shape == (0, 2)
+    two = oracle.coo_to_csr(2, 2, [0, 1], [0, 1], np.array([2.0, 3.0]))
+    b = np.zeros((2, 2), order="F")
+    with pytest.raises(oracle.OracleError) as e:
+        oracle._check(oracle.lib().or_spmm_f64(oracle._i64(2), oracle._i64(2), oracle._i64(2),
+                                               oracle._ptr(two.row_offsets), oracle._ptr(two.col_indices),
+                                               oracle._ptr(two.values), oracle._ptr(b), oracle._ptr(b)))
+    assert "alias" in str(e.value)
+
+
+def test_spmm_columns_are_spmv(oracle):
+    # the definition: column j of A B is A times column j of B, bit for bit
+    rng = np.random.default_rng(12)
+    a = sp.random(300, 200, density=0.04, random_state=13, format="coo")
+    for dtype in (np.float32, np.float64):
+        csr = oracle.coo_to_csr(300, 200, a.row, a.col, a.data.astype(dtype))
+        b = rng.standard_normal((200, 5)).astype(dtype)
+        y = oracle.spmm(csr, b)
+        for j in range(5):
+            assert np.array_equal(y[:, j], oracle.spmv(csr, np.ascontiguousarray(b[:, j])))
+        np.testing.assert_allclose(y.astype(np.float64), a @ b.astype(np.float64),
+                                   rtol=1e-4 if dtype == np.float32 else 1e-12, atol=1e-4 if dtype == np.float32 else 1e-12)
+
+
+def test_spmm_matches_reference_bitwise(ref_lib):
+    rng = np.random.default_rng(21)
+    for dtype in (np.float32, np.float64):
+        a = sp.random(250, 180, density=0.05, random_state=22, format="coo")
+        csr = ref_lib.coo_to_csr(250, 180, a.row, a.col, a.data.astype(dtype))
+        b = rng.standard_normal((180, 3)).astype(dtype)
+        mine, theirs = ref_lib.spmm(csr, b), ref_lib.spmm(csr, b, use_ref=True)
+        assert np.array_equal(mine.ravel(order="F").view(np.uint8), theirs.ravel(order="F").view(np.uint8))
+
+
 # ---- CG / PCG ------------------------------------------------------------
 
 @pytest.mark.parametrize("dtype,tol", [(np.float32, 2e-4), (np.float64, 1e-10)])
