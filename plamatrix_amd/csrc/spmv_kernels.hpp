@@ -445,31 +445,54 @@ __global__ __launch_bounds__(kBlock) void k_spmm_tile(int64_t rows, int64_t kt, 
             continue;
         }
         const int n = static_cast<int>(total);
-#pragma unroll
-        for (int j = 0; j < kStageLoads; ++j)
+        if constexpr (sizeof(S) == 4 && sizeof(I) == 4)
         {
-            const int kk = lane + j * kWave;
-            if (kk < n)
+#pragma unroll
+            for (int j = 0; j < kStageLoads; ++j)
             {
-                lv[kk] = v[base + kk];
-                lc[kk] = ci[base + kk];
+                const int kk = lane + j * kWave;
+                if (kk < n)
+                {
+                    __builtin_amdgcn_global_load_lds(v + base + kk, lv + j * kWave, 4, 0, 0);
+                    __builtin_amdgcn_global_load_lds(ci + base + kk, lc + j * kWave, 4, 0, 0);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        else
+        {
+#pragma unroll
+            for (int j = 0; j < kStageLoads; ++j)
+            {
+                const int kk = lane + j * kWave;
+                if (kk < n)
+                {
+                    lv[kk] = v[base + kk];
+                    lc[kk] = ci[base + kk];
+                }
             }
         }
         wave_lds_sync();
         const int s = static_cast<int>(static_cast<int64_t>(s_i) - base);
         const int e = static_cast<int>(static_cast<int64_t>(e_i) - base);
+        // an entry's tile operands: kSpmmTile contiguous values, 16-byte loads
+        using Vec = S __attribute__((ext_vector_type(16 / sizeof(S))));
+        constexpr int kVecs = kSpmmTile * sizeof(S) / 16;
         S acc[kSpmmTile];
 #pragma unroll
         for (int t = 0; t < kSpmmTile; ++t) acc[t] = S(0);
         for (int q = s; q < e; ++q)
         {
             const S a = lv[q];
-            const S* g = bt + static_cast<int64_t>(lc[q]) * kSpmmTile;
-            S gv[kSpmmTile];
+            const Vec* g = reinterpret_cast<const Vec*>(bt + static_cast<int64_t>(lc[q]) * kSpmmTile);
+            Vec gv[kVecs];
 #pragma unroll
-            for (int t = 0; t < kSpmmTile; ++t) gv[t] = g[t];
+            for (int u = 0; u < kVecs; ++u) gv[u] = g[u];
 #pragma unroll
-            for (int t = 0; t < kSpmmTile; ++t) acc[t] = fma_s(a, gv[t], acc[t]);
+            for (int u = 0; u < kVecs; ++u)
+#pragma unroll
+                for (int w = 0; w < 16 / static_cast<int>(sizeof(S)); ++w)
+                    acc[u * (16 / sizeof(S)) + w] = fma_s(a, gv[u][w], acc[u * (16 / sizeof(S)) + w]);
         }
         if (r < rows)
         {
