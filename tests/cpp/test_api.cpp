@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "plamatrix/optimization/block_schur.h"
+#include "plamatrix/optimization/levenberg_marquardt.h"
 #include "plamatrix/sparse/coo_matrix.h"
 #include "plamatrix/sparse/iterative_solver.h"
 #include "plamatrix/sparse/sparse_ops.h"
@@ -255,6 +256,97 @@ static void batchedRefinement()
     for (int i = 0; i < 5; ++i) EXPECT(std::fabs(got[static_cast<std::size_t>(64 + i)] - (i + 1) * (5 - i) / 2.0) <= 1e-10);
 }
 
+// block_schur_test.cpp:550-569, then a short LM loop over a linear
+// least-squares problem: damped device Schur solves with the damping the
+// strategy picks, the pattern built once for the whole loop.
+static void levenbergMarquardt()
+{
+    LevenbergMarquardtOptions<double> lo;
+    lo.initialDamping = 1e-3;
+    lo.minimumDamping = 1e-8;
+    lo.maximumDamping = 1e4;
+    lo.decreaseFactor = 0.25;
+    lo.increaseFactor = 8.0;
+    LevenbergMarquardtStrategy<double> kat(lo);
+    kat.acceptStep();
+    EXPECT(kat.damping() == 2.5e-4 && kat.acceptedSteps() == 1 && kat.rejectedSteps() == 0);
+    kat.rejectStep();
+    EXPECT(kat.damping() == 2e-3 && kat.acceptedSteps() == 1 && kat.rejectedSteps() == 1);
+    lo.decreaseFactor = 1.5;
+    EXPECT(throws<std::invalid_argument>([&] { LevenbergMarquardtStrategy<double> bad(lo); }));
+
+    // r_i(x) = pj_i . x_p + ej_i x_e + c_i for 2 primaries (size 2), 2 eliminated (size 1)
+    struct Res
+    {
+        Index p, e;
+        double pj[2], ej, c;
+    };
+    const Res res[] = {{0, 0, {1.0, 2.0}, 0.5, -1.0},  {1, 0, {-0.5, 1.0}, 1.5, 0.25},
+                       {0, 1, {2.0, -1.0}, -0.75, 2.0}, {1, 1, {1.25, 0.5}, 2.0, -0.5},
+                       {0, 0, {0.3, -0.4}, 1.1, 0.7},   {1, 1, {-0.9, 0.2}, -0.6, 1.3}};
+    std::vector<double> xp(4, 0.0), xe(2, 0.0);
+    auto residual = [&](const Res& q, const std::vector<double>& p, const std::vector<double>& e) {
+        return q.pj[0] * p[static_cast<std::size_t>(2 * q.p)] + q.pj[1] * p[static_cast<std::size_t>(2 * q.p + 1)] +
+               q.ej * e[static_cast<std::size_t>(q.e)] + q.c;
+    };
+    auto cost = [&](const std::vector<double>& p, const std::vector<double>& e) {
+        double c = 0.0;
+        for (const auto& q : res)
+        {
+            const double r = residual(q, p, e);
+            c += r * r;
+        }
+        return c;
+    };
+    SchurComplementSolverOptions<double> o;
+    o.maxIterations = 50;
+    o.relativeTolerance = 1e-13;
+    o.absoluteTolerance = 1e-14;
+    SchurComplementSolverWorkspace<double> ws;
+    LevenbergMarquardtStrategy<double> lm;
+    double current = cost(xp, xe);
+    const double start = current;
+    for (int it = 0; it < 12; ++it)
+    {
+        BlockNormalEquations<double> eq(2, 2, 2, 1);
+        for (const auto& q : res)
+        {
+            const double r = residual(q, xp, xe);
+            eq.addResidualBlock(q.p, q.e, q.pj, &q.ej, &r, 1, 1.0);
+        }
+        std::vector<double> dp, de;
+        const auto rep = solveDampedSchurComplement(eq, lm.damping(), o, ws, &dp, &de);
+        EXPECT(rep.converged);
+        std::vector<double> tp(xp), te(xe);
+        for (std::size_t i = 0; i < tp.size(); ++i) tp[i] += dp[i];
+        for (std::size_t i = 0; i < te.size(); ++i) te[i] += de[i];
+        const double trial = cost(tp, te);
+        if (std::isfinite(trial) && trial < current)
+        {
+            xp = tp;
+            xe = te;
+            current = trial;
+            lm.acceptStep();
+        }
+        else
+        {
+            lm.rejectStep();
+        }
+    }
+    EXPECT(ws.patternBuildCount() == 1);
+    EXPECT(lm.acceptedSteps() >= 1 && current < 0.5 * start);
+    // at the least-squares minimum the gradient J^T r vanishes
+    double g[6] = {0, 0, 0, 0, 0, 0};
+    for (const auto& q : res)
+    {
+        const double r = residual(q, xp, xe);
+        g[2 * q.p] += q.pj[0] * r;
+        g[2 * q.p + 1] += q.pj[1] * r;
+        g[4 + q.e] += q.ej * r;
+    }
+    for (double v : g) EXPECT(std::fabs(v) <= 1e-6);
+}
+
 // Two systems in one batched launch.
 static void batched()
 {
@@ -359,6 +451,7 @@ int main()
         {"fp64 refinement", refinement},
         {"batched pcg", batched},
         {"batched fp64 refinement", batchedRefinement},
+        {"levenberg-marquardt loop", levenbergMarquardt},
         {"schur", schur},
     };
     for (const auto& c : cases)
