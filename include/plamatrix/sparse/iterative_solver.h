@@ -59,7 +59,12 @@ public:
         }
         return *this;
     }
-    void closeAsyncAllocation() { detail::check(pm_workspace_close(_ws)); }
+    /// Release storage (a no-op after a move); the bound stream must have
+    /// completed its work, else std::logic_error.
+    void closeAsyncAllocation()
+    {
+        if (_ws) detail::check(pm_workspace_close(_ws));
+    }
     Index capacitySize() const noexcept { return pm_workspace_capacity_rows(_ws); }
     pm_workspace* handle() const noexcept { return _ws; }
 
@@ -94,6 +99,12 @@ struct AsyncIterativeSolverState
 
     int submittedIterations = 0;
     pm_fixed_state* handle() const noexcept { return _state; }
+    /// Release the device state once the solve's stream has completed;
+    /// repeated calls are harmless.
+    void closeAsyncAllocation() noexcept
+    {
+        if (_state) pm_fixed_state_destroy(std::exchange(_state, nullptr));
+    }
     pm_fixed_state** slot() noexcept { return &_state; }
 
 private:
@@ -123,13 +134,30 @@ inline IterativeSolverReport fromAbi(const pm_solver_report& r)
     return out;
 }
 
-template <typename Scalar>
-void checkSystem(const CSRMatrix<Scalar, Device::GPU>& a, const DenseMatrix<Scalar, Device::GPU>& rhs,
-                 const DenseMatrix<Scalar, Device::GPU>& x)
+// Shape, aliasing, stream and trust rules of the solvers
+// (iterative_solver_cuda_detail.h:40-82 of the reference): stream-ordered
+// operands are used on their owning stream; untrusted structure is
+// revalidated by the adaptive solvers (throwing std::invalid_argument before
+// any launch when it is malformed) and rejected by the fixed-iteration ones.
+template <typename Scalar, typename Rhs, typename Sol>
+void checkSystem(const CSRMatrix<Scalar, Device::GPU>& a, const Rhs& rhs, const Sol& x, hipStream_t stream,
+                 bool require_prevalidated = false)
 {
     if (a.rows() != a.cols()) throw std::invalid_argument("iterative solver requires a square matrix");
     if (rhs.rows() != a.rows() || rhs.cols() != 1 || x.rows() != a.rows() || x.cols() != 1)
         throw std::invalid_argument("iterative solver vector dimensions do not match the matrix");
+    if (rhs.data() != nullptr && static_cast<const void*>(rhs.data()) == static_cast<const void*>(x.data()))
+        throw std::invalid_argument("iterative solver rhs and solution must not alias");
+    checkStorageStream("iterative solver matrix", a, stream);
+    checkStorageStream("iterative solver rhs", rhs, stream);
+    checkStorageStream("iterative solver solution", x, stream);
+    if (!a.isStructureUsableOnStream(stream))
+    {
+        if (require_prevalidated)
+            throw std::logic_error("fixed iterative solver requires prevalidated CSR structure; "
+                                   "call matrix.validateStructure(stream) after mutable array access");
+        a.validateStructure(stream);
+    }
 }
 } // namespace detail
 
@@ -139,7 +167,7 @@ IterativeSolverReport cg(const CSRMatrix<Scalar, Device::GPU>& matrix, const Den
                          DenseMatrix<Scalar, Device::GPU>& solution, IterativeSolverWorkspace<Scalar>& workspace,
                          const IterativeSolverOptions& options = {}, hipStream_t stream = nullptr)
 {
-    detail::checkSystem(matrix, rhs, solution);
+    detail::checkSystem(matrix, rhs, solution, stream);
     const pm_csr d = matrix.descriptor(true, stream);
     const pm_solver_options o = detail::toAbi(options);
     pm_solver_report r{};
@@ -153,7 +181,7 @@ IterativeSolverReport pcg(const CSRMatrix<Scalar, Device::GPU>& matrix, const De
                           DenseMatrix<Scalar, Device::GPU>& solution, IterativeSolverWorkspace<Scalar>& workspace,
                           const IterativeSolverOptions& options = {}, hipStream_t stream = nullptr)
 {
-    detail::checkSystem(matrix, rhs, solution);
+    detail::checkSystem(matrix, rhs, solution, stream);
     const pm_csr d = matrix.descriptor(true, stream);
     const pm_solver_options o = detail::toAbi(options);
     pm_solver_report r{};
@@ -170,9 +198,10 @@ IterativeSolverReport blockPcg(const CSRMatrix<Scalar, Device::GPU>& matrix,
                                IterativeSolverWorkspace<Scalar>& workspace, const IterativeSolverOptions& options = {},
                                hipStream_t stream = nullptr)
 {
-    detail::checkSystem(matrix, rhs, solution);
+    detail::checkSystem(matrix, rhs, solution, stream);
     if (block_size <= 0 || inverse_blocks.size() != matrix.rows() * block_size)
         throw std::invalid_argument("blockPcg inverse blocks must hold rows * block_size values");
+    detail::checkStorageStream("blockPcg inverse blocks", inverse_blocks, stream);
     const pm_csr d = matrix.descriptor(true, stream);
     const pm_solver_options o = detail::toAbi(options);
     pm_solver_report r{};
@@ -189,7 +218,7 @@ AsyncIterativeSolverState fixedAsync(const CSRMatrix<Scalar, Device::GPU>& matri
                                      DenseMatrix<Scalar, Device::GPU>& solution, int iterations, int preconditioned,
                                      IterativeSolverWorkspace<Scalar>& workspace, hipStream_t stream)
 {
-    checkSystem(matrix, rhs, solution);
+    checkSystem(matrix, rhs, solution, stream, true);
     const pm_csr d = matrix.descriptor(true, stream);
     AsyncIterativeSolverState state;
     check(pm_solve_fixed_async(&d, rhs.data(), solution.data(), iterations, preconditioned, state.slot(),
@@ -252,6 +281,7 @@ inline RefinementReport pcgRefine(const CSRMatrix<float, Device::GPU>& matrix,
                                   const IterativeSolverOptions& inner = {1000000, 1e-3, 0.0, true, false},
                                   int refresh_interval = 50, hipStream_t stream = nullptr)
 {
+    detail::checkSystem(matrix, rhs, solution, stream);
     const pm_csr d = matrix.descriptor(true, stream);
     const pm_solver_options o = detail::toAbi(inner);
     pm_solve_policy policy{};
@@ -282,7 +312,7 @@ std::vector<pm_batch_report> batchedPcg(const CSRMatrix<Scalar, Device::GPU>& ma
                                         hipStream_t stream = nullptr)
 {
     if (row_begin.size() != row_end.size()) throw std::invalid_argument("batchedPcg: row range arrays differ in size");
-    detail::checkSystem(matrix, rhs, solution);
+    detail::checkSystem(matrix, rhs, solution, stream);
     const pm_csr d = matrix.descriptor(true, stream);
     const pm_solver_options o = detail::toAbi(options);
     pm_solve_policy policy{};
@@ -308,8 +338,7 @@ inline std::vector<pm_batch_refine_report> batchedPcgRefine(
 {
     if (row_begin.size() != row_end.size())
         throw std::invalid_argument("batchedPcgRefine: row range arrays differ in size");
-    if (rhs.rows() != matrix.rows() || solution.rows() != matrix.rows() || rhs.cols() != 1 || solution.cols() != 1)
-        throw std::invalid_argument("batchedPcgRefine requires Nx1 rhs/solution matching the matrix");
+    detail::checkSystem(matrix, rhs, solution, stream);
     const pm_csr d = matrix.descriptor(true, stream);
     const pm_solver_options o = detail::toAbi(inner);
     pm_solve_policy policy{};

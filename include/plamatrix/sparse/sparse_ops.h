@@ -28,26 +28,61 @@ public:
     }
     SparseOpsWorkspace(const SparseOpsWorkspace&) = delete;
     SparseOpsWorkspace& operator=(const SparseOpsWorkspace&) = delete;
-    SparseOpsWorkspace(SparseOpsWorkspace&& o) noexcept : _ws(std::exchange(o._ws, nullptr)) {}
+    SparseOpsWorkspace(SparseOpsWorkspace&& o) noexcept
+        : _ws(std::exchange(o._ws, nullptr)), _statusOutput(std::exchange(o._statusOutput, nullptr)),
+          _statusFinalize(std::exchange(o._statusFinalize, nullptr))
+    {
+    }
     SparseOpsWorkspace& operator=(SparseOpsWorkspace&& o) noexcept
     {
         if (this != &o)
         {
             if (_ws) pm_workspace_destroy(_ws);
             _ws = std::exchange(o._ws, nullptr);
+            _statusOutput = std::exchange(o._statusOutput, nullptr);
+            _statusFinalize = std::exchange(o._statusFinalize, nullptr);
         }
         return *this;
     }
 
-    /// Release storage; the owning stream must have completed its work.
-    void closeAsyncAllocation() { detail::check(pm_workspace_close(_ws)); }
-    /// Deferred status of the last asynchronous COO conversion.
-    void checkStatus(const char* = nullptr) { detail::check(pm_workspace_check_status(_ws)); }
+    /// Release storage (a no-op after a move); the owning stream must have
+    /// completed its work, else std::logic_error.
+    void closeAsyncAllocation()
+    {
+        if (_ws) detail::check(pm_workspace_close(_ws));
+    }
+
+    /// Deferred status of the last asynchronous COO conversion: throws its
+    /// error, if any, and marks the conversion's output trusted on success.
+    /// Before the stream has finished it is a std::logic_error and the status
+    /// stays pending.
+    void checkStatus(const char* = nullptr)
+    {
+        if (!_ws) return;
+        const int status = pm_workspace_check_status(_ws);
+        if (status == PM_LOGIC_ERROR) detail::throwStatus(status);
+        if (_statusOutput && _statusFinalize) _statusFinalize(_statusOutput, status == PM_OK);
+        _statusOutput = nullptr;
+        _statusFinalize = nullptr;
+        detail::check(status);
+    }
     std::size_t capacityBytes() const noexcept { return pm_workspace_capacity_bytes(_ws); }
     pm_workspace* handle() const noexcept { return _ws; }
 
+    // The next checkStatus() settles the trust of `output`.
+    template <typename Scalar>
+    void beginStatus(CSRMatrix<Scalar, Device::GPU>* output) noexcept
+    {
+        _statusOutput = output;
+        _statusFinalize = [](void* m, bool trusted) {
+            detail::CSRMatrixAccess::completeAsyncWrite(*static_cast<CSRMatrix<Scalar, Device::GPU>*>(m), trusted);
+        };
+    }
+
 private:
     pm_workspace* _ws = nullptr;
+    void* _statusOutput = nullptr;
+    void (*_statusFinalize)(void*, bool) = nullptr;
 };
 
 /// Synchronously convert device COO triplets to sorted, duplicate-combined
@@ -66,10 +101,13 @@ CSRMatrix<Scalar, Device::GPU> cooToCsr(Index rows, Index cols, const DenseMatri
     int64_t nnz = 0;
     detail::check(pm_coo_to_csr_nnz(workspace.handle(), &nnz));
     CSRMatrix<Scalar, Device::GPU> out(rows, cols, nnz);
-    detail::check(pm_coo_to_csr_finish(out.rowOffsets(), out.colIndices(), out.values(), nnz, workspace.handle(),
-                                       stream));
+    using Access = detail::CSRMatrixAccess;
+    Access::beginAsyncWrite(out, stream, "cooToCsr");
+    detail::check(pm_coo_to_csr_finish(Access::rowOffsets(out), Access::colIndices(out), Access::values(out), nnz,
+                                       workspace.handle(), stream));
     detail::checkHip(hipStreamSynchronize(stream), "hipStreamSynchronize");
     workspace.checkStatus("cooToCsr");
+    Access::completeAsyncWrite(out, true);
     return out;
 }
 
@@ -82,11 +120,21 @@ void cooToCsrAsync(Index rows, Index cols, const DenseMatrix<Index, Device::GPU>
 {
     if (row_indices.size() != values.size() || col_indices.size() != values.size())
         throw std::invalid_argument("cooToCsr row, column, and value arrays must have equal lengths");
+    detail::checkStorageStream("cooToCsrAsync rows", row_indices, stream);
+    detail::checkStorageStream("cooToCsrAsync columns", col_indices, stream);
+    detail::checkStorageStream("cooToCsrAsync values", values, stream);
+    detail::checkStorageStream("cooToCsrAsync output", output, stream);
     if (output.rows() != rows || output.cols() != cols)
-        throw std::invalid_argument("cooToCsrAsync output dimensions do not match");
+        throw std::runtime_error("cooToCsrAsync output dimensions do not match");
+    using Access = detail::CSRMatrixAccess;
+    if (values.data() != nullptr && values.data() == Access::values(output))
+        throw std::invalid_argument("cooToCsrAsync values and output must not alias");
+    Access::beginAsyncWrite(output, stream, "cooToCsrAsync");
     detail::check(pm_coo_to_csr_async(rows, cols, values.size(), row_indices.data(), col_indices.data(),
-                                      values.data(), detail::dtypeOf<Scalar>(), PM_COO_CHECK_FINITE, output.rowOffsets(),
-                                      output.colIndices(), output.values(), output.nnz(), workspace.handle(), stream));
+                                      values.data(), detail::dtypeOf<Scalar>(), PM_COO_CHECK_FINITE,
+                                      Access::rowOffsets(output), Access::colIndices(output), Access::values(output),
+                                      output.nnz(), workspace.handle(), stream));
+    workspace.beginStatus(&output);
 }
 
 /// Enqueue y = A x on `stream` without host synchronisation.
@@ -100,6 +148,12 @@ void spmvAsync(const CSRMatrix<Scalar, Device::GPU>& csr, const DenseMatrix<Scal
         throw std::runtime_error("spmv GPU dimension mismatch");
     if (output.rows() != csr.rows() || output.cols() != 1)
         throw std::runtime_error("spmv GPU output dimension mismatch");
+    if (x.data() != nullptr && x.data() == output.data())
+        throw std::invalid_argument("spmv GPU input and output data must not alias");
+    detail::checkStorageStream("spmvAsync CSR", csr, stream);
+    detail::checkCsrStructureStream("spmvAsync CSR", csr, stream);
+    detail::checkStorageStream("spmvAsync input", x, stream);
+    detail::checkStorageStream("spmvAsync output", output, stream);
     const pm_csr d = csr.descriptor(true, stream);
     detail::check(pm_spmv(&d, x.data(), output.data(), workspace.handle(), stream));
 }
@@ -133,6 +187,12 @@ void spmmAsync(const CSRMatrix<Scalar, Device::GPU>& csr, const DenseMatrix<Scal
     if (B.rows() != csr.cols()) throw std::runtime_error("spmm dimension mismatch");
     if (output.rows() != csr.rows() || output.cols() != B.cols())
         throw std::runtime_error("spmm output dimension mismatch");
+    if (B.data() != nullptr && B.data() == output.data())
+        throw std::invalid_argument("spmm GPU input and output data must not alias");
+    detail::checkStorageStream("spmmAsync CSR", csr, stream);
+    detail::checkCsrStructureStream("spmmAsync CSR", csr, stream);
+    detail::checkStorageStream("spmmAsync input", B, stream);
+    detail::checkStorageStream("spmmAsync output", output, stream);
     const pm_csr d = csr.descriptor(true, stream);
     detail::check(pm_spmm(&d, B.data(), B.cols(), output.data(), workspace.handle(), stream));
 }

@@ -70,8 +70,9 @@ typedef struct pm_workspace pm_workspace;
 int pm_workspace_create(pm_workspace** ws);
 /* Synchronises the bound stream, then frees everything. */
 int pm_workspace_destroy(pm_workspace* ws);
-/* closeAsyncAllocation(): releases storage; PM_LOGIC_ERROR while work that
- * uses it is still pending on the bound stream. */
+/* closeAsyncAllocation(): releases storage and unbinds the stream;
+ * PM_LOGIC_ERROR while a COO conversion's status is unchecked or while the
+ * bound stream still has work queued (sparse_ops.cu:644-660). */
 int pm_workspace_close(pm_workspace* ws);
 size_t pm_workspace_capacity_bytes(const pm_workspace* ws);
 /* Vector capacity (rows) of the iterative-solver storage (capacitySize()). */

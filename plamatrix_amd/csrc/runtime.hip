@@ -192,13 +192,17 @@ int pm_workspace_destroy(pm_workspace* ws)
 int pm_workspace_close(pm_workspace* ws)
 {
     if (!ws) return pm::fail(PM_INVALID_ARGUMENT, "pm_workspace_close: null workspace");
-    if (ws->bound && ws->done_event)
+    if (ws->coo_pending_status)
+        return pm::fail(PM_LOGIC_ERROR, "SparseOpsWorkspace::closeAsyncAllocation requires checkStatus first");
+    if (ws->bound)
     {
-        const hipError_t q = hipEventQuery(ws->done_event);
+        // any work still queued on the bound stream may use this storage
+        // (sparse_ops.cu:640-660 of the reference queries the stream too)
+        const hipError_t q = hipStreamQuery(ws->stream);
         if (q == hipErrorNotReady)
             return pm::fail(PM_LOGIC_ERROR,
                             "closeAsyncAllocation requires the workspace stream to be synchronized first");
-        if (q != hipSuccess) return pm::check_hip(q, "hipEventQuery", __FILE__, __LINE__);
+        if (q != hipSuccess) return pm::check_hip(q, "hipStreamQuery", __FILE__, __LINE__);
     }
     ws->plans = std::make_unique<pm_workspace::PlanCache>();
     ws->vectors.release();

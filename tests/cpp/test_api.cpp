@@ -2,8 +2,10 @@
 // (sparse_ops_test.cpp, iterative_solver_test.cpp, block_schur_test.cpp):
 // known answers, closed forms and the exception classes.  Prints one line per
 // case and "ALL PASSED" at the end; exits non-zero on the first failure.
+#include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <thread>
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
@@ -123,6 +125,19 @@ static void spmmKat()
     EXPECT(throws<std::invalid_argument>([&] { spmmAsync(a, square, square, ws); }));
     auto x3 = upload<Scalar>({1, 2, 3});
     EXPECT(throws<std::invalid_argument>([&] { spmvAsync(a, x3, x3, ws); }));
+}
+
+// diag(2, 4) with the int64 arrays read back through const access.
+template <typename Scalar>
+static bool isDiag24(const CSRMatrix<Scalar, Device::GPU>& a)
+{
+    Index ro[3] = {}, ci[2] = {};
+    Scalar v[2] = {};
+    detail::checkHip(hipMemcpy(ro, a.rowOffsets(), sizeof(ro), hipMemcpyDeviceToHost), "hipMemcpy");
+    detail::checkHip(hipMemcpy(ci, a.colIndices(), sizeof(ci), hipMemcpyDeviceToHost), "hipMemcpy");
+    detail::checkHip(hipMemcpy(v, a.values(), sizeof(v), hipMemcpyDeviceToHost), "hipMemcpy");
+    return ro[0] == 0 && ro[1] == 1 && ro[2] == 2 && ci[0] == 0 && ci[1] == 1 && v[0] == Scalar(2) &&
+           v[1] == Scalar(4);
 }
 
 template <typename Scalar>
@@ -347,6 +362,410 @@ static void levenbergMarquardt()
     for (double v : g) EXPECT(std::fabs(v) <= 1e-6);
 }
 
+// ---- async transfer and stream contract (csr_matrix_test.cpp:184-450,
+// sparse_ops_test.cpp:838-950, iterative_solver_test.cpp:562-680) ----------
+
+struct Stream
+{
+    hipStream_t s = nullptr;
+    Stream() { detail::checkHip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags"); }
+    ~Stream()
+    {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    void sync() const { detail::checkHip(hipStreamSynchronize(s), "hipStreamSynchronize"); }
+};
+
+// 3 x 4 with nnz 4: rows {0: (0, 1.5) (3, -2)}, {1: (1, 3.25)}, {2: (2, 4)}.
+template <typename Scalar>
+static void fillTransferInput(CSRMatrix<Scalar, Device::CPU>& m)
+{
+    const Index ro[] = {0, 2, 3, 4};
+    const Index ci[] = {0, 3, 1, 2};
+    const Scalar v[] = {Scalar(1.5), Scalar(-2), Scalar(3.25), Scalar(4)};
+    for (int i = 0; i < 4; ++i) m.rowOffsets()[i] = ro[i];
+    for (int i = 0; i < 4; ++i)
+    {
+        m.colIndices()[i] = ci[i];
+        m.values()[i] = v[i];
+    }
+    m.validateStructure();
+}
+
+template <typename Scalar>
+static bool isTransferInput(const CSRMatrix<Scalar, Device::CPU>& m)
+{
+    const Index ro[] = {0, 2, 3, 4};
+    const Index ci[] = {0, 3, 1, 2};
+    const Scalar v[] = {Scalar(1.5), Scalar(-2), Scalar(3.25), Scalar(4)};
+    if (m.rows() != 3 || m.cols() != 4 || m.nnz() != 4) return false;
+    for (int i = 0; i < 4; ++i)
+        if (m.rowOffsets()[i] != ro[i] || m.colIndices()[i] != ci[i] || m.values()[i] != v[i]) return false;
+    return true;
+}
+
+template <typename Scalar>
+static void csrTransfers()
+{
+    using Cpu = CSRMatrix<Scalar, Device::CPU>;
+    using Gpu = CSRMatrix<Scalar, Device::GPU>;
+    Cpu pageable(3, 4, 4);
+    fillTransferInput(pageable);
+    auto pinned = Cpu::pinned(3, 4, 4);
+    fillTransferInput(pinned);
+    EXPECT(pinned.isPinnedHost() && !pageable.isPinnedHost());
+
+    // synchronous round trip; the upload is trusted
+    {
+        const auto gpu = pageable.toGpu();
+        EXPECT(gpu.hasValidatedStructure());
+        EXPECT(isTransferInput(gpu.toCpu()));
+    }
+    // asynchronous round trip on a non-default stream
+    {
+        Stream st;
+        auto gpu = pinned.toGpuAsync(st.s);
+        auto back = gpu.toCpuAsync(st.s);
+        EXPECT(gpu.isAsyncAllocation() && gpu.asyncAllocationStream() == st.s);
+        EXPECT(back.isPinnedHost());
+        st.sync();
+        gpu.closeAsyncAllocation();
+        st.sync();
+        EXPECT(isTransferInput(back));
+    }
+    // async copies keep the output's existing storage
+    {
+        Gpu gpu(3, 4, 4);
+        auto back = Cpu::pinned(3, 4, 4);
+        const Scalar* gv = static_cast<const Gpu&>(gpu).values();
+        const Scalar* hv = static_cast<const Cpu&>(back).values();
+        Stream st;
+        pinned.copyToGpuAsync(gpu, st.s);
+        gpu.copyToCpuAsync(back, st.s);
+        st.sync();
+        EXPECT(static_cast<const Gpu&>(gpu).values() == gv && static_cast<const Cpu&>(back).values() == hv);
+        EXPECT(isTransferInput(back));
+    }
+    // a reused pinned download is revalidated after an async overwrite
+    {
+        auto valid = pageable.toGpu();
+        auto invalid = pageable.toGpu();
+        const Index bad[] = {0, 3, 2, 4};
+        detail::checkHip(hipMemcpy(invalid.rowOffsets(), bad, sizeof(bad), hipMemcpyHostToDevice), "hipMemcpy");
+        Stream st;
+        auto reused = valid.toCpuAsync(st.s);
+        st.sync();
+        reused.validateStructure(st.s);
+        EXPECT(reused.hasValidatedStructure());
+        invalid.copyToCpuAsync(reused, st.s);
+        st.sync();
+        EXPECT(throws<std::invalid_argument>([&] { reused.validateStructure(st.s); }));
+    }
+    // pageable host storage cannot take part in async copies
+    {
+        Gpu gpu(3, 4, 4);
+        EXPECT(throws<std::invalid_argument>([&] { pageable.copyToGpuAsync(gpu); }));
+        EXPECT(throws<std::invalid_argument>([&] { (void)pageable.toGpuAsync(); }));
+        const auto src = pageable.toGpu();
+        Cpu out(3, 4, 4);
+        EXPECT(throws<std::invalid_argument>([&] { src.copyToCpuAsync(out); }));
+        Cpu wrong(3, 4, 3);
+        EXPECT(throws<std::runtime_error>([&] { src.copyToCpuAsync(wrong); }));
+    }
+    // closeAsyncAllocation is for stream-ordered storage only
+    {
+        Gpu ordinary(3, 4, 4);
+        EXPECT(!ordinary.isAsyncAllocation());
+        EXPECT(throws<std::logic_error>([&] { ordinary.closeAsyncAllocation(); }));
+    }
+    // stream-ordered storage rejects copies on another stream
+    {
+        Stream owner, other;
+        auto gpu = pinned.toGpuAsync(owner.s);
+        EXPECT(throws<std::logic_error>([&] { (void)gpu.toCpuAsync(other.s); }));
+        EXPECT(throws<std::logic_error>([&] { pinned.copyToGpuAsync(gpu, other.s); }));
+        EXPECT(throws<std::logic_error>([&] { (void)gpu.toCpu(); }));
+        owner.sync();
+        gpu.closeAsyncAllocation();
+        owner.sync();
+    }
+    // ordinary storage rejects overlapping writes from two streams
+    {
+        Gpu gpu(3, 4, 4);
+        Stream first, second;
+        pinned.copyToGpuAsync(gpu, first.s);
+        EXPECT(!gpu.hasValidatedStructure() && gpu.isStructureUsableOnStream(first.s));
+        EXPECT(throws<std::logic_error>([&] { pinned.copyToGpuAsync(gpu, second.s); }));
+        EXPECT(throws<std::logic_error>([&] { gpu.validateStructure(second.s); }));
+        first.sync();
+        gpu.validateStructure(first.s);
+        EXPECT(gpu.hasValidatedStructure());
+        pinned.copyToGpuAsync(gpu, second.s);
+        second.sync();
+    }
+    // a download may not overlap a pending upload from another stream
+    {
+        Gpu gpu(3, 4, 4);
+        auto out = Cpu::pinned(3, 4, 4);
+        Stream first, second;
+        pinned.copyToGpuAsync(gpu, first.s);
+        EXPECT(throws<std::logic_error>([&] { gpu.copyToCpuAsync(out, second.s); }));
+        first.sync();
+        gpu.validateStructure(first.s);
+        gpu.copyToCpuAsync(out, second.s);
+        second.sync();
+        out.validateStructure(second.s);
+        EXPECT(out.hasValidatedStructure() && isTransferInput(out));
+    }
+    // two downloads into one pinned matrix from two streams
+    {
+        const auto gpu = pageable.toGpu();
+        auto out = Cpu::pinned(3, 4, 4);
+        Stream first, second;
+        gpu.copyToCpuAsync(out, first.s);
+        EXPECT(throws<std::logic_error>([&] { gpu.copyToCpuAsync(out, second.s); }));
+        first.sync();
+        out.validateStructure(first.s);
+        gpu.copyToCpuAsync(out, second.s);
+        second.sync();
+    }
+    // ownership of stream-ordered storage moves; repeated close is harmless
+    {
+        Stream st;
+        auto source = pinned.toGpuAsync(st.s);
+        const Scalar* v = static_cast<const Gpu&>(source).values();
+        Gpu moved(std::move(source));
+        EXPECT(!source.isAsyncAllocation() && source.rows() == 0);
+        EXPECT(moved.isAsyncAllocation() && static_cast<const Gpu&>(moved).values() == v);
+        st.sync();
+        moved.closeAsyncAllocation();
+        moved.closeAsyncAllocation();
+        EXPECT(moved.rows() == 0 && moved.nnz() == 0);
+        st.sync();
+    }
+    // move assignment releases the target's storage and takes the source's
+    {
+        Stream st;
+        auto source = pinned.toGpuAsync(st.s);
+        auto target = pinned.toGpuAsync(st.s);
+        const Scalar* v = static_cast<const Gpu&>(source).values();
+        target = std::move(source);
+        EXPECT(!source.isAsyncAllocation() && target.isAsyncAllocation());
+        EXPECT(static_cast<const Gpu&>(target).values() == v);
+        st.sync();
+        target.closeAsyncAllocation();
+        st.sync();
+    }
+    // dense stream-ordered storage
+    {
+        Stream st;
+        auto d = DenseMatrix<Scalar, Device::GPU>::uninitializedAsync(4, 1, st.s);
+        EXPECT(d.isAsyncAllocation() && d.asyncAllocationStream() == st.s);
+        auto h = DenseMatrix<Scalar, Device::CPU>::pinned(4, 1);
+        EXPECT(h.isPinnedHost());
+        for (Index i = 0; i < 4; ++i) h(i, 0) = Scalar(i + 1);
+        h.copyToGpuAsync(d, st.s);
+        auto back = d.toCpuAsync(st.s);
+        st.sync();
+        EXPECT(back(3, 0) == Scalar(4) && back(0, 0) == Scalar(1));
+        d.closeAsyncAllocation();
+        d.closeAsyncAllocation();
+        EXPECT(d.rows() == 0 && !d.isAsyncAllocation());
+        DenseMatrix<Scalar, Device::GPU> ordinary(2, 1);
+        EXPECT(throws<std::logic_error>([&] { ordinary.closeAsyncAllocation(); }));
+        DenseMatrix<Scalar, Device::CPU> wrong(3, 1);
+        EXPECT(throws<std::runtime_error>([&] { ordinary.copyToCpuAsync(wrong); }));
+        st.sync();
+    }
+}
+
+template <typename Scalar>
+static void sparseOpsStreamRules()
+{
+    // products reject stream-ordered operands on another stream
+    {
+        auto csr = CSRMatrix<Scalar, Device::CPU>::pinned(2, 2, 2);
+        csr.rowOffsets()[1] = 1;
+        csr.rowOffsets()[2] = 2;
+        csr.colIndices()[1] = 1;
+        csr.values()[0] = Scalar(2);
+        csr.values()[1] = Scalar(3);
+        auto in = DenseMatrix<Scalar, Device::CPU>::pinned(2, 1);
+        in(0, 0) = Scalar(4);
+        in(1, 0) = Scalar(5);
+        Stream owner, other;
+        auto a = csr.toGpuAsync(owner.s);
+        auto x = DenseMatrix<Scalar, Device::GPU>::uninitializedAsync(2, 1, owner.s);
+        auto y = DenseMatrix<Scalar, Device::GPU>::uninitializedAsync(2, 1, owner.s);
+        in.copyToGpuAsync(x, owner.s);
+        SparseOpsWorkspace ws;
+        EXPECT(throws<std::logic_error>([&] { spmvAsync(a, x, y, ws, other.s); }));
+        spmvAsync(a, x, y, ws, owner.s);
+        auto out = y.toCpuAsync(owner.s);
+        owner.sync();
+        EXPECT(out(0, 0) == Scalar(8) && out(1, 0) == Scalar(15));
+        ws.closeAsyncAllocation();
+        a.closeAsyncAllocation();
+        x.closeAsyncAllocation();
+        y.closeAsyncAllocation();
+        owner.sync();
+    }
+    // a workspace is bound to its first stream until closed; moves carry it
+    {
+        COOMatrix<Scalar, Device::GPU> coo(2, 2);
+        coo.add(0, 0, 2);
+        coo.add(1, 1, 3);
+        const auto a = coo.toCsr();
+        const auto x = upload<Scalar>({4, 5});
+        DenseMatrix<Scalar, Device::GPU> y(2, 1);
+        Stream first, second;
+        SparseOpsWorkspace ws;
+        spmvAsync(a, x, y, ws, first.s);
+        first.sync();
+        SparseOpsWorkspace moved(std::move(ws));
+        EXPECT(ws.capacityBytes() == 0);
+        EXPECT(throws<std::logic_error>([&] { spmvAsync(a, x, y, moved, second.s); }));
+        moved.closeAsyncAllocation();
+        moved.closeAsyncAllocation();
+        spmvAsync(a, x, y, moved, second.s);
+        second.sync();
+        EXPECT((download(y) == std::vector<Scalar>{8, 15}));
+        moved.closeAsyncAllocation();
+    }
+    // closing while the bound stream still has work is a logic_error
+    {
+        COOMatrix<Scalar, Device::GPU> coo(1, 1);
+        const auto a = coo.toCsr();
+        const auto x = upload<Scalar>({3});
+        DenseMatrix<Scalar, Device::GPU> y(1, 1);
+        // destruction order: the guard releases the host function, the
+        // workspace and the stream drain it, then the flag goes
+        std::atomic<bool> release{false};
+        Stream st;
+        SparseOpsWorkspace ws;
+        struct Releaser
+        {
+            std::atomic<bool>& flag;
+            ~Releaser() { flag.store(true, std::memory_order_release); }
+        } guard{release};
+        spmvAsync(a, x, y, ws, st.s);
+        detail::checkHip(hipLaunchHostFunc(
+                             st.s,
+                             [](void* p) {
+                                 auto* flag = static_cast<std::atomic<bool>*>(p);
+                                 while (!flag->load(std::memory_order_acquire)) std::this_thread::yield();
+                             },
+                             &release),
+                         "hipLaunchHostFunc");
+        EXPECT(throws<std::logic_error>([&] { ws.closeAsyncAllocation(); }));
+        release.store(true, std::memory_order_release);
+        st.sync();
+        ws.closeAsyncAllocation();
+    }
+    // escaped mutable storage is no longer trusted by SpMV
+    {
+        auto a = poisson1d<Scalar>(3);
+        (void)a.values();
+        a.validateStructure();
+        const auto x = upload<Scalar>({1, 1, 1});
+        DenseMatrix<Scalar, Device::GPU> y(3, 1);
+        EXPECT(!a.hasValidatedStructure());
+        EXPECT(throws<std::logic_error>([&] { spmv(a, x, y); }));
+    }
+    // an async conversion's output is trusted once its status is checked
+    {
+        const auto r = upload<Index>({1, 0, 1});
+        const auto c = upload<Index>({1, 0, 1});
+        const auto v = upload<Scalar>({1, 2, 3});
+        Stream st;
+        CSRMatrix<Scalar, Device::GPU> out(2, 2, 2);
+        SparseOpsWorkspace ws;
+        cooToCsrAsync(2, 2, r, c, v, out, ws, st.s);
+        EXPECT(!out.hasValidatedStructure() && out.isStructureUsableOnStream(nullptr) == false);
+        st.sync();
+        ws.checkStatus();
+        EXPECT(out.hasValidatedStructure());
+        EXPECT(isDiag24(out));
+    }
+}
+
+template <typename Scalar>
+static void solverTrustRules()
+{
+    // a device COO result feeds the fixed-iteration solver directly
+    {
+        const auto r = upload<Index>({0, 1});
+        const auto c = upload<Index>({0, 1});
+        const auto v = upload<Scalar>({2, 4});
+        SparseOpsWorkspace cws;
+        const auto a = cooToCsr<Scalar>(2, 2, r, c, v, cws);
+        EXPECT(a.hasValidatedStructure());
+        const auto b = upload<Scalar>({2, 4});
+        DenseMatrix<Scalar, Device::GPU> x(2, 1);
+        IterativeSolverWorkspace<Scalar> ws;
+        auto state = pcgFixedIterationsAsync(a, b, x, 1, ws);
+        detail::checkHip(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+        EXPECT(finalizeIterativeSolverReport(state).converged);
+        state.closeAsyncAllocation();
+        state.closeAsyncAllocation();
+        ws.closeAsyncAllocation();
+        cws.closeAsyncAllocation();
+    }
+    // malformed structure written through a raw pointer is caught before any launch
+    {
+        auto a = poisson1d<Scalar>(2);
+        const Index bad[] = {0, 2, 1};
+        detail::checkHip(hipMemcpy(a.rowOffsets(), bad, sizeof(bad), hipMemcpyHostToDevice), "hipMemcpy");
+        const auto b = upload<Scalar>({1, 1});
+        DenseMatrix<Scalar, Device::GPU> x(2, 1);
+        IterativeSolverWorkspace<Scalar> ws;
+        EXPECT(throws<std::invalid_argument>([&] { cg(a, b, x, ws); }));
+        EXPECT((download(x) == std::vector<Scalar>{0, 0}));
+    }
+    // escaped storage is rejected by the fixed-iteration solvers even after validation
+    {
+        auto a = poisson1d<Scalar>(2);
+        (void)a.rowOffsets();
+        a.validateStructure();
+        const auto b = upload<Scalar>({1, 1});
+        DenseMatrix<Scalar, Device::GPU> x(2, 1);
+        IterativeSolverWorkspace<Scalar> ws;
+        EXPECT(throws<std::logic_error>([&] { (void)cgFixedIterationsAsync(a, b, x, 1, ws); }));
+        // the adaptive solver revalidates and solves: [2 -1; -1 2] x = 1 -> x = 1
+        const auto rep = cg(a, b, x, ws);
+        EXPECT(rep.converged);
+        const auto xv = download(x);
+        EXPECT(std::fabs(xv[0] - 1) < 1e-5 && std::fabs(xv[1] - 1) < 1e-5);
+        EXPECT(throws<std::invalid_argument>([&] { cg(a, b, const_cast<DenseMatrix<Scalar, Device::GPU>&>(b), ws); }));
+    }
+    // an async upload is consumed on its copy stream after validation there
+    {
+        auto src = CSRMatrix<Scalar, Device::CPU>::pinned(2, 2, 2);
+        src.rowOffsets()[1] = 1;
+        src.rowOffsets()[2] = 2;
+        src.colIndices()[1] = 1;
+        src.values()[0] = Scalar(2);
+        src.values()[1] = Scalar(4);
+        CSRMatrix<Scalar, Device::GPU> a(2, 2, 2);
+        Stream copy, other;
+        src.copyToGpuAsync(a, copy.s);
+        EXPECT(throws<std::logic_error>([&] { a.validateStructure(other.s); }));
+        copy.sync();
+        EXPECT(isDiag24(a));
+        a.validateStructure(copy.s);
+        const auto b = upload<Scalar>({2, 4});
+        DenseMatrix<Scalar, Device::GPU> x(2, 1);
+        IterativeSolverWorkspace<Scalar> ws;
+        auto state = pcgFixedIterationsAsync(a, b, x, 1, ws, copy.s);
+        copy.sync();
+        EXPECT(finalizeIterativeSolverReport(state).converged);
+        EXPECT((download(x) == std::vector<Scalar>{1, 1}));
+        state.closeAsyncAllocation();
+        ws.closeAsyncAllocation();
+    }
+}
+
 // Two systems in one batched launch.
 static void batched()
 {
@@ -453,6 +872,12 @@ int main()
         {"batched fp64 refinement", batchedRefinement},
         {"levenberg-marquardt loop", levenbergMarquardt},
         {"schur", schur},
+        {"csr transfers float", [] { csrTransfers<float>(); }},
+        {"csr transfers double", [] { csrTransfers<double>(); }},
+        {"sparse ops stream rules float", [] { sparseOpsStreamRules<float>(); }},
+        {"sparse ops stream rules double", [] { sparseOpsStreamRules<double>(); }},
+        {"solver trust rules float", [] { solverTrustRules<float>(); }},
+        {"solver trust rules double", [] { solverTrustRules<double>(); }},
     };
     for (const auto& c : cases)
     {
