@@ -7,6 +7,7 @@
 // block-Jacobi PCG and back-substitution (pm_schur_solve).
 #pragma once
 
+#include <chrono>
 #include <cmath>
 #include <memory>
 #include <stdexcept>
@@ -53,12 +54,16 @@ enum class SchurComplementLinearBackend
     Cuda,
     OpenCl,
     Hip,
+    /// Dense FP64 Cholesky of the reduced system on the device; DenseCpu
+    /// requests are served by it as well.
+    DenseHip,
 };
 
 template <typename Scalar>
 struct SchurComplementSolverOptions
 {
-    /// Only Hip is served by this library; the other values are rejected.
+    /// Hip (block-Jacobi PCG) and DenseHip / DenseCpu (dense Cholesky, both
+    /// on the device) are served; Cpu, Cuda and OpenCl are rejected.
     SchurComplementLinearBackend linearBackend = SchurComplementLinearBackend::Hip;
     int deviceIndex = -1;
     int maxIterations = 100;
@@ -76,8 +81,21 @@ struct SchurComplementSolverReport
     int iterations = 0;
     Scalar initialResidualNorm = Scalar(0);
     Scalar finalResidualNorm = Scalar(0);
+    /// Phase timings of the reference's report.  The phases run back to back
+    /// on the device without host synchronisation, so only the whole device
+    /// solve (linearSolveSeconds) is timed; the other phases read 0.
+    double smallBlockInverseSeconds = 0.0;
+    double schurAccumulationSeconds = 0.0;
+    double csrConversionSeconds = 0.0;
+    double schurAssemblySeconds = 0.0;
+    double choleskyFactorizationSeconds = 0.0;
+    double triangularSolveSeconds = 0.0;
+    double residualCheckSeconds = 0.0;
+    double linearSolveSeconds = 0.0;
+    double backSubstitutionSeconds = 0.0;
     bool schurPatternReused = false;
     bool schurAssemblyOnDevice = false;
+    bool mixedPrecisionUsed = false;
     std::string deviceName;
     std::string message;
 };
@@ -387,8 +405,11 @@ SchurComplementSolverReport<Scalar> solveDampedSchurComplement(const BlockNormal
 {
     if (!primary_step || !eliminated_step || primary_step == eliminated_step)
         throw std::invalid_argument("solveDampedSchurComplement: output vectors must be non-null and distinct");
-    if (options.linearBackend != SchurComplementLinearBackend::Hip)
-        throw std::invalid_argument("solveDampedSchurComplement: this library serves SchurComplementLinearBackend::Hip");
+    const bool dense = options.linearBackend == SchurComplementLinearBackend::DenseHip ||
+                       options.linearBackend == SchurComplementLinearBackend::DenseCpu;
+    if (options.linearBackend != SchurComplementLinearBackend::Hip && !dense)
+        throw std::invalid_argument(
+            "solveDampedSchurComplement: this library serves the Hip, DenseHip and DenseCpu backends");
     if (options.deviceIndex >= 0) detail::checkHip(hipSetDevice(options.deviceIndex), "hipSetDevice");
     const std::size_t pd = static_cast<std::size_t>(eq._pc * eq._ps), ed = static_cast<std::size_t>(eq._ec * eq._es);
     const bool guess = options.useInitialGuess && primary_step->size() == pd;
@@ -424,14 +445,19 @@ SchurComplementSolverReport<Scalar> solveDampedSchurComplement(const BlockNormal
     o.relative_tolerance = options.relativeTolerance;
     o.absolute_tolerance = options.absoluteTolerance;
     o.minimum_diagonal = options.minimumDiagonal;
+    o.linear_backend = dense ? PM_SCHUR_DENSE_CHOLESKY : PM_SCHUR_BLOCK_PCG;
     pm_schur_report r{};
+    const auto start = std::chrono::steady_clock::now();
     detail::check(pm_schur_solve(workspace.schur(), &d, damping, &o, dp.ptr, de.ptr, &r, workspace.solver(), nullptr));
     detail::checkHip(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
+    const double solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
     if (pd) detail::checkHip(hipMemcpy(primary_step->data(), dp.ptr, sizeof(Scalar) * pd, hipMemcpyDeviceToHost), "hipMemcpy");
     if (ed)
         detail::checkHip(hipMemcpy(eliminated_step->data(), de.ptr, sizeof(Scalar) * ed, hipMemcpyDeviceToHost),
                          "hipMemcpy");
     SchurComplementSolverReport<Scalar> report;
+    report.linearBackend = dense ? SchurComplementLinearBackend::DenseHip : SchurComplementLinearBackend::Hip;
+    report.linearSolveSeconds = solve_seconds;
     report.converged = r.converged != 0;
     report.iterations = r.iterations;
     report.initialResidualNorm = r.initial_residual_norm;

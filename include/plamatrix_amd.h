@@ -362,18 +362,28 @@ typedef struct pm_residual_batch {
 } pm_residual_batch;
 
 /* SchurComplementSolverOptions<double> (block_schur.h:31-48) for the device
- * backend. */
+ * backends.  linear_backend selects the reduced solve: PM_SCHUR_BLOCK_PCG
+ * (the accelerated backend: block-Jacobi PCG on the Schur CSR) or
+ * PM_SCHUR_DENSE_CHOLESKY (the DenseCpu backend, block_schur.cpp:329-359 and
+ * block_schur_dense_solver.h:300-398: dense FP64 Cholesky of the lower
+ * triangle, one "iteration", converged when ||r|| <= max(abs + rel ||b||,
+ * 100 eps max(1, ||b||)); use_initial_guess and max_iterations are ignored). */
+enum { PM_SCHUR_BLOCK_PCG = 0, PM_SCHUR_DENSE_CHOLESKY = 1 };
 typedef struct pm_schur_options {
     int32_t max_iterations;
     int32_t use_initial_guess;
     double relative_tolerance;
     double absolute_tolerance;
     double minimum_diagonal;
+    int32_t linear_backend;
+    int32_t pad;
 } pm_schur_options;
 
 /* SchurComplementSolverReport (block_schur.h:51-80).  status: 0 converged,
  * 1 eliminated block not positive definite, 2 preconditioner block not
- * positive definite, 4 iteration limit. */
+ * positive definite, 4 iteration limit, 8 reduced Schur complement not
+ * positive definite (dense backend), 16 dense Cholesky residual above the
+ * tolerance. */
 typedef struct pm_schur_report {
     int32_t converged;
     int32_t iterations;
@@ -428,6 +438,23 @@ int pm_schur_assemble(pm_schur_workspace* ws, const pm_normal_equations* equatio
  * owned by the workspace). */
 int pm_schur_system(const pm_schur_workspace* ws, pm_csr* reduced, const double** rhs,
                     const double** preconditioner);
+
+/* ---- dense direct solve (no reference counterpart) -------------------------
+ * The dense Cholesky solve behind PM_SCHUR_DENSE_CHOLESKY, for callers that
+ * hold a symmetric positive definite FP64 CSR matrix: the lower triangle
+ * (duplicates summed) is factored in a rows x rows dense array in the
+ * workspace (rows <= 65536), then b is solved into x (distinct device
+ * vectors).  report->not_positive_definite is set when a pivot is not positive
+ * and finite (x is then meaningless); residual_norm is ||b - A x|| with the
+ * CSR values.  Synchronises `stream`. */
+typedef struct pm_dense_report {
+    int32_t not_positive_definite;
+    int32_t pad;
+    double rhs_norm;
+    double residual_norm;
+} pm_dense_report;
+int pm_dense_cholesky_solve(const pm_csr* a, const double* b, double* x, pm_dense_report* report, pm_workspace* ws,
+                            hipStream_t stream);
 
 #ifdef __cplusplus
 }

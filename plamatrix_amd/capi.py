@@ -79,9 +79,17 @@ class pm_residual_batch(ctypes.Structure):
                 ("huber_delta", _f64)]
 
 
+PM_SCHUR_BLOCK_PCG = 0
+PM_SCHUR_DENSE_CHOLESKY = 1
+
+
 class pm_schur_options(ctypes.Structure):
     _fields_ = [("max_iterations", _i32), ("use_initial_guess", _i32), ("relative_tolerance", _f64),
-                ("absolute_tolerance", _f64), ("minimum_diagonal", _f64)]
+                ("absolute_tolerance", _f64), ("minimum_diagonal", _f64), ("linear_backend", _i32), ("pad", _i32)]
+
+
+class pm_dense_report(ctypes.Structure):
+    _fields_ = [("not_positive_definite", _i32), ("pad", _i32), ("rhs_norm", _f64), ("residual_norm", _f64)]
 
 
 class pm_schur_report(ctypes.Structure):
@@ -103,7 +111,7 @@ EXPORTS = [
     "pm_batch_layout", "pm_batch_pcg", "pm_batch_pcg_refine", "pm_pcg_factored", "pm_pcg_refine_factored",
     "pm_fsai_parameter_count", "pm_fsai_initial_parameters", "pm_fsai_net_create", "pm_fsai_net_destroy",
     "pm_fsai_net_set_parameters", "pm_fsai_net_get_parameters", "pm_fsai_graph_create", "pm_fsai_graph_destroy",
-    "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step",
+    "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step", "pm_dense_cholesky_solve",
 ]
 
 _lib = None
@@ -284,6 +292,22 @@ def true_relres(a: DeviceCSR, b, x, ws: Workspace) -> float:
     check(lib().pm_true_relres(ctypes.byref(d), _ptr(b), _ptr(x), _i32(dtype_code(x)), ctypes.byref(out), ws.handle,
                                current_stream()))
     return out.value
+
+
+@dataclass
+class DenseReport:
+    not_positive_definite: bool
+    rhs_norm: float
+    residual_norm: float
+
+
+def dense_cholesky_solve(a: DeviceCSR, b, x, ws: Workspace) -> DenseReport:
+    """Dense FP64 Cholesky solve of an SPD CSR system (lower triangle used)."""
+    d = a.descriptor()
+    r = pm_dense_report()
+    check(lib().pm_dense_cholesky_solve(ctypes.byref(d), _ptr(b), _ptr(x), ctypes.byref(r), ws.handle,
+                                        current_stream()))
+    return DenseReport(bool(r.not_positive_definite), r.rhs_norm, r.residual_norm)
 
 
 @dataclass
@@ -520,9 +544,11 @@ class SchurReport:
 
 def schur_solve(sw: SchurWorkspace, ne: NormalEquations, damping, primary_step, eliminated_step, ws: Workspace,
                 max_iterations=100, relative_tolerance=1e-8, absolute_tolerance=1e-12, minimum_diagonal=1e-12,
-                use_initial_guess=False) -> SchurReport:
+                use_initial_guess=False, dense=False) -> SchurReport:
+    """dense=True selects the dense Cholesky backend (the reference's DenseCpu)."""
     o = pm_schur_options(int(max_iterations), int(bool(use_initial_guess)), float(relative_tolerance),
-                         float(absolute_tolerance), float(minimum_diagonal))
+                         float(absolute_tolerance), float(minimum_diagonal),
+                         PM_SCHUR_DENSE_CHOLESKY if dense else PM_SCHUR_BLOCK_PCG, 0)
     r = pm_schur_report()
     check(lib().pm_schur_solve(sw.handle, ctypes.byref(ne.desc), _f64(damping), ctypes.byref(o),
                                _ptr(primary_step), _ptr(eliminated_step), ctypes.byref(r), ws.handle,

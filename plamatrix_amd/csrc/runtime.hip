@@ -159,6 +159,7 @@ pm_workspace::~pm_workspace()
     sort.release();
     refine.release();
     batch.release();
+    dense.release();
     for (hipEvent_t& e : events)
         if (e) (void)hipEventDestroy(e);
     if (done_event) (void)hipEventDestroy(done_event);
@@ -212,6 +213,7 @@ int pm_workspace_close(pm_workspace* ws)
     ws->sort.release();
     ws->refine.release();
     ws->batch.release();
+    ws->dense.release();
     ws->capacity_rows = 0;
     ws->bound = false;
     ws->stream = nullptr;
@@ -223,7 +225,7 @@ size_t pm_workspace_capacity_bytes(const pm_workspace* ws)
 {
     if (!ws) return 0;
     return ws->vectors.bytes + ws->partials.bytes + ws->state.bytes + ws->scratch.bytes + ws->sort.bytes +
-           ws->refine.bytes + ws->batch.bytes;
+           ws->refine.bytes + ws->batch.bytes + ws->dense.bytes;
 }
 
 int64_t pm_workspace_capacity_rows(const pm_workspace* ws) { return ws ? ws->capacity_rows : 0; }

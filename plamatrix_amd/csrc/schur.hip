@@ -39,8 +39,10 @@
 // CPU path's bit for bit given the same inputs.
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 
+#include "dense_cholesky.hpp"
 #include "pm_common.hpp"
 #include "sort.hpp"
 #include "workspace.hpp"
@@ -1838,12 +1840,21 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: output vectors must be non-null and distinct");
     if (!std::isfinite(damping) || damping < 0.0 || o->max_iterations <= 0 || !std::isfinite(o->relative_tolerance) ||
         o->relative_tolerance < 0.0 || !std::isfinite(o->absolute_tolerance) || o->absolute_tolerance < 0.0 ||
-        !std::isfinite(o->minimum_diagonal) || o->minimum_diagonal <= 0.0)
+        !std::isfinite(o->minimum_diagonal) || o->minimum_diagonal <= 0.0 ||
+        (o->linear_backend != PM_SCHUR_BLOCK_PCG && o->linear_backend != PM_SCHUR_DENSE_CHOLESKY))
         return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: invalid solver options");
+    const bool dense = o->linear_backend == PM_SCHUR_DENSE_CHOLESKY;
     PM_TRY(schur_assemble(w, eq, damping, o->minimum_diagonal, report, stream));
+    if (dense && report->status == 2)
+    {
+        // a diagonal block of the reduced system is not positive definite, so
+        // neither is the system: the dense backend's failure
+        report->status = 8;
+        std::snprintf(report->message, sizeof(report->message), "reduced Schur complement is not positive definite");
+    }
     if (report->status != 0) return PM_OK;
     if (ed > 0) hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(ed)), dim3(kBlock), 0, stream, de, ed, 0.0);
-    if (pd > 0 && !o->use_initial_guess)
+    if (pd > 0 && !o->use_initial_guess && !dense)
         hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(pd)), dim3(kBlock), 0, stream, dp, pd, 0.0);
     PM_HIP(hipGetLastError());
     if (pd == 0)
@@ -1862,6 +1873,34 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         a.dtype = PM_F64;
         a.row_offsets32 = w->ro32;
         a.col_indices32 = w->ci32;
+        if (dense)
+        {
+            PM_TRY(ws->bind(stream));
+            DenseCholeskyOutcome dc;
+            PM_TRY(dense_cholesky_solve(&a, w->rhs, dp, ws, &dc, stream));
+            report->iterations = 1;
+            report->initial_residual_norm = dc.rhs_norm;
+            report->final_residual_norm = dc.residual_norm;
+            if (dc.not_positive_definite)
+            {
+                report->status = 8;
+                std::snprintf(report->message, sizeof(report->message),
+                              "reduced Schur complement is not positive definite");
+                return PM_OK;
+            }
+            // block_schur_dense_solver.h:384-388
+            const double tol = o->absolute_tolerance + o->relative_tolerance * dc.rhs_norm;
+            const double floor = 100.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, dc.rhs_norm);
+            report->converged = std::isfinite(dc.residual_norm) && dc.residual_norm <= std::max(tol, floor);
+            if (!report->converged)
+            {
+                report->status = 16;
+                std::snprintf(report->message, sizeof(report->message), "dense Cholesky residual exceeds tolerance");
+                return PM_OK;
+            }
+        }
+        else
+        {
         pm_solver_options so{};
         so.max_iterations = o->max_iterations;
         so.relative_tolerance = o->relative_tolerance;
@@ -1874,6 +1913,7 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         report->iterations = sr.iterations;
         report->initial_residual_norm = sr.initial_residual;
         report->final_residual_norm = sr.final_residual;
+        }
     }
     if (!report->converged)
     {

@@ -120,6 +120,7 @@ struct pm_workspace
     pm::DeviceBuffer sort;      // COO conversion buffers
     pm::DeviceBuffer refine;    // FP64 residual / FP32 correction of the refinement loop
     pm::DeviceBuffer batch;     // batched solver: layout, per-system state, per-chunk partials
+    pm::DeviceBuffer dense;     // dense Cholesky: n x n lower triangle, diagonal-block inverses
     int64_t capacity_rows = 0;
 
     pm::SolverState* host_state = nullptr;  // pinned, 2 slots

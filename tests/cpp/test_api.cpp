@@ -851,6 +851,132 @@ static void schur()
     EXPECT(evaluateHuberLoss(25.0, 3.0).cost == 10.5);
 }
 
+// block_schur_test.cpp:235-405: the DenseCpu backend (served on the device)
+// on the synthetic and multi-primary systems, a coupled chain of 24 primary
+// blocks solved twice bitwise equal, an eliminated-heavy system against the
+// PCG backend, and a singular reduced system.
+static void denseSchur()
+{
+    SchurComplementSolverOptions<double> o;
+    o.linearBackend = SchurComplementLinearBackend::DenseCpu;
+    o.relativeTolerance = 1e-12;
+    o.absoluteTolerance = 1e-14;
+    {
+        BlockNormalEquations<double> eq(2, 2, 2, 1);
+        const struct
+        {
+            Index p, e;
+            double pj[2], ej, r;
+        } res[] = {{0, 0, {1.0, 2.0}, 0.5, -1.0},
+                   {1, 0, {-0.5, 1.0}, 1.5, 0.25},
+                   {0, 1, {2.0, -1.0}, -0.75, 2.0},
+                   {1, 1, {1.25, 0.5}, 2.0, -0.5}};
+        for (const auto& q : res) eq.addResidualBlock(q.p, q.e, q.pj, &q.ej, &q.r, 1, 1.0);
+        const double ppj[2] = {0.4, -0.2}, pr = 0.75;
+        eq.addPrimaryResidualBlock(0, ppj, &pr, 1, 2.0);
+        const double eej = 0.8, er = -0.3;
+        eq.addEliminatedResidualBlock(1, &eej, &er, 1, 1.5);
+        SchurComplementSolverWorkspace<double> ws;
+        std::vector<double> dp, de;
+        const auto rep = solveDampedSchurComplement(eq, 0.1, o, ws, &dp, &de);
+        EXPECT(rep.converged && rep.iterations == 1);
+        EXPECT(rep.linearBackend == SchurComplementLinearBackend::DenseHip);
+        EXPECT(rep.linearSolveSeconds >= rep.choleskyFactorizationSeconds + rep.triangularSolveSeconds +
+                                             rep.residualCheckSeconds);
+        const double expected[] = {-0.532196868252671, 0.717080672008074, 0.078633780070238, -0.210823022930113,
+                                   0.017900956933130, 0.253200998620430};
+        for (int i = 0; i < 4; ++i) EXPECT(std::fabs(dp[static_cast<std::size_t>(i)] - expected[i]) <= 1e-11);
+        for (int i = 0; i < 2; ++i) EXPECT(std::fabs(de[static_cast<std::size_t>(i)] - expected[4 + i]) <= 1e-11);
+        const auto again = solveDampedSchurComplement(eq, 0.1, o, ws, &dp, &de);
+        EXPECT(again.converged && again.schurPatternReused && ws.patternBuildCount() == 1);
+    }
+    {
+        BlockNormalEquations<double> multi(2, 1, 1, 1);
+        const double j0 = 1.0, j1 = 2.0, je = 3.0, r = 4.0;
+        multi.addResidualBlocks({0, 1}, {&j0, &j1}, 0, &je, &r, 1);
+        const double k0 = 0.5, k1 = -1.0, r2 = 0.2;
+        multi.addPrimaryResidualBlocks({0, 1}, {&k0, &k1}, &r2, 1);
+        std::vector<double> dp, de;
+        auto dense = o;
+        dense.linearBackend = SchurComplementLinearBackend::DenseHip;
+        const auto rep = solveDampedSchurComplement(multi, 0.1, dense, &dp, &de);
+        EXPECT(rep.converged);
+        EXPECT(std::fabs(dp[0] + 1.345185185185185) <= 1e-11 && std::fabs(dp[1] + 0.512592592592593) <= 1e-11);
+        EXPECT(std::fabs(de[0] + 0.493827160493827) <= 1e-11);
+    }
+    {
+        constexpr Index blocks = 24, size = 6;
+        BlockNormalEquations<double> eq(blocks, 0, size, 1);
+        double identity[size * size] = {};
+        for (Index d = 0; d < size; ++d) identity[d * size + d] = 1.0;
+        for (Index b = 0; b < blocks; ++b)
+        {
+            double r[size];
+            for (Index c = 0; c < size; ++c) r[c] = 0.01 * static_cast<double>(1 + b + c);
+            eq.addPrimaryResidualBlock(b, identity, r, size);
+        }
+        const double left[size] = {1.0, 0.2, -0.1, 0.3, 0.05, -0.2};
+        const double right[size] = {-0.8, 0.1, 0.25, -0.2, 0.15, 0.3};
+        for (Index b = 0; b + 1 < blocks; ++b)
+        {
+            const double r = 0.02 * static_cast<double>(b + 1);
+            eq.addPrimaryResidualBlocks({b, b + 1}, {left, right}, &r, 1);
+        }
+        std::vector<double> p1, e1, p2, e2;
+        const auto first = solveDampedSchurComplement(eq, 0.01, o, &p1, &e1);
+        const auto second = solveDampedSchurComplement(eq, 0.01, o, &p2, &e2);
+        EXPECT(first.converged && second.converged);
+        EXPECT(p1.size() == static_cast<std::size_t>(blocks * size) && e1.empty());
+        EXPECT(p1 == p2 && e1 == e2);
+        for (double v : p1) EXPECT(std::isfinite(v));
+    }
+    {
+        constexpr Index pc = 36, ec = 240, ps = 8, es = 3;
+        BlockNormalEquations<double> eq(pc, ec, ps, es);
+        double identity[ps * ps] = {};
+        for (Index d = 0; d < ps; ++d) identity[d * ps + d] = 1.0;
+        const double zero[ps] = {};
+        for (Index p = 0; p < pc; ++p) eq.addPrimaryResidualBlock(p, identity, zero, ps, 0.25);
+        for (Index e = 0; e < ec; ++e)
+            for (Index obs = 0; obs < 4; ++obs)
+            {
+                const Index p = (e * 7 + obs * 5) % pc;
+                double pj[2 * ps], ej[2 * es];
+                for (Index i = 0; i < 2 * ps; ++i) pj[i] = 0.01 * static_cast<double>(1 + (i + e + obs) % 13);
+                for (Index i = 0; i < 2 * es; ++i) ej[i] = 0.02 * static_cast<double>(1 + (i + e + 2 * obs) % 7);
+                const double r[2] = {0.001 * static_cast<double>(e + 1), -0.002 * static_cast<double>(obs + 1)};
+                eq.addResidualBlock(p, e, pj, ej, r, 2);
+            }
+        auto dense = o;
+        dense.relativeTolerance = 1e-11;
+        dense.absoluteTolerance = 1e-13;
+        std::vector<double> dp, de, pp, pe;
+        const auto rep = solveDampedSchurComplement(eq, 0.1, dense, &dp, &de);
+        auto pcg = dense;
+        pcg.linearBackend = SchurComplementLinearBackend::Hip;
+        pcg.maxIterations = 500;
+        const auto prep = solveDampedSchurComplement(eq, 0.1, pcg, &pp, &pe);
+        EXPECT(rep.converged && prep.converged);
+        double diff = 0, norm = 0;
+        for (std::size_t i = 0; i < dp.size(); ++i)
+        {
+            diff = std::max(diff, std::fabs(dp[i] - pp[i]));
+            norm = std::max(norm, std::fabs(pp[i]));
+        }
+        for (std::size_t i = 0; i < de.size(); ++i) diff = std::max(diff, std::fabs(de[i] - pe[i]));
+        EXPECT(diff <= 1e-9 * std::max(1.0, norm));
+    }
+    {
+        // primary 1 has no residual and there is no damping: S = diag(s0, 0)
+        BlockNormalEquations<double> eq(2, 0, 1, 1);
+        const double j = 2.0, r = 1.0;
+        eq.addPrimaryResidualBlock(0, &j, &r, 1);
+        std::vector<double> dp, de;
+        const auto rep = solveDampedSchurComplement(eq, 0.0, o, &dp, &de);
+        EXPECT(!rep.converged && rep.message == "reduced Schur complement is not positive definite");
+    }
+}
+
 int main()
 {
     const struct
@@ -872,6 +998,7 @@ int main()
         {"batched fp64 refinement", batchedRefinement},
         {"levenberg-marquardt loop", levenbergMarquardt},
         {"schur", schur},
+        {"dense schur", denseSchur},
         {"csr transfers float", [] { csrTransfers<float>(); }},
         {"csr transfers double", [] { csrTransfers<double>(); }},
         {"sparse ops stream rules float", [] { sparseOpsStreamRules<float>(); }},
