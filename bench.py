@@ -64,7 +64,7 @@ def committed_traffic(kernel: str):
     return None
 
 
-WORKLOADS = ("diffusion128", "poisson256", "lm-schur", "batch512", "learned-eval", "spmm8")
+WORKLOADS = ("diffusion128", "poisson256", "lm-schur", "lm-schur-dense", "batch512", "learned-eval", "spmm8")
 
 
 def main() -> int:
@@ -102,6 +102,7 @@ def main() -> int:
     from plamatrix_amd import capi, generators
 
     run = {"diffusion128": run_diffusion, "poisson256": run_poisson, "lm-schur": run_schur,
+           "lm-schur-dense": run_schur_dense,
            "batch512": run_batch, "learned-eval": run_learned_eval, "spmm8": run_spmm}[args.workload]
     line = run(args, torch, capi, generators)
     print(json.dumps(line))
@@ -488,6 +489,79 @@ def run_schur(args, torch, capi, generators):
                      "first_call_ms_with_pattern_build": round(first_ms, 2)}
     line["normal_equations"] = {"ms": round(ne_ms, 4), "bytes_read": ne_bytes,
                                 "gbs": round(ne_bytes / (ne_ms * 1e-3) / 1e9, 1)}
+    return line
+
+
+def run_schur_dense(args, torch, capi, generators):
+    """The LM / Schur step of run_schur with the dense backend (the reference's
+    DenseCpu): device normal equations, reduced assembly, dense FP64 Cholesky
+    of the 12,000 x 12,000 reduced system, triangular solves, residual check
+    and back-substitution."""
+    dev = torch.device("cuda:0")
+    inst = generators.lm_schur_instance()
+    t = lambda arr, dt: torch.from_numpy(np.ascontiguousarray(arr, dtype=dt)).to(dev)
+    batch = dict(primary=t(inst.primary, np.int64), eliminated=t(inst.eliminated, np.int64),
+                 primary_jacobians=t(inst.primary_jacobians, np.float64),
+                 eliminated_jacobians=t(inst.eliminated_jacobians, np.float64),
+                 residuals=t(inst.residuals, np.float64))
+    pc, ec = inst.primary_count, inst.eliminated_count
+    sw = capi.SchurWorkspace()
+    ws = capi.Workspace()
+    dp = torch.zeros(pc * 6, dtype=torch.float64, device=dev)
+    de = torch.zeros(ec * 3, dtype=torch.float64, device=dev)
+
+    def step():
+        ne = capi.assemble_normal_equations(sw, pc, ec, 6, 3, **batch, weights=None, huber_delta=inst.huber_delta)
+        rep = capi.schur_solve(sw, ne, inst.damping, dp, de, ws, relative_tolerance=1e-10, absolute_tolerance=0.0,
+                               dense=True)
+        return ne, rep
+
+    step()
+    for _ in range(args.warmup):
+        step()
+    ms, (ne, rep) = _median_ms(step, max(args.steps, 3))
+    if not rep.converged:
+        raise RuntimeError(rep.message)
+    # the dense solve alone on the assembled reduced system, timed with HIP
+    # events on the stream it runs on
+    a, rhs_ptr, _ = capi.schur_system(sw)
+    n = a.rows
+    x = torch.empty(n, dtype=torch.float64, device=dev)
+    solve_ms = _event_ms(torch, lambda: capi.dense_cholesky_solve(a, rhs_ptr, x.data_ptr(), ws), 3)
+    flops = n ** 3 / 3 + 2 * n ** 2  # factorization + two triangular solves
+    tflops = flops / (solve_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        if oracle.ref_available():
+            # bounded sample: the same generator at half the primaries (a 6,000 x
+            # 6,000 reduced system), scaled by 8 = 2^3 for the cubic factorization
+            half = generators.lm_schur_instance(primary_count=pc // 2, eliminated_count=ec // 2)
+            w = generators.huber_weights(half.residuals, half.huber_delta)
+            c_s, (_, _, crep) = _cpu_median_s(lambda: oracle.ref_schur_solve_batch(
+                pc // 2, ec // 2, 6, 3, half.primary, half.eliminated, half.primary_jacobians,
+                half.eliminated_jacobians, half.residuals, w, half.damping, backend=1, relative_tolerance=1e-10,
+                absolute_tolerance=0.0), 1)
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            cpu = {"value": round(c_s * 8e3, 1), "unit": "ms", "cores": threads, "kind": "reference",
+                   "sample": f"the reference's own solveDampedSchurComplement with the DenseCpu backend (built "
+                             f"from source, OpenMP, {threads} threads) on the same generator at {pc // 2} "
+                             f"primaries x {ec // 2} secondaries (6,000 x 6,000 reduced system): "
+                             f"{c_s * 1e3:.0f} ms, scaled by 8 for the cubic factorization"}
+    line = base_line(args, "LM Schur step time with the dense Cholesky backend (device normal equations + reduced "
+                           "Schur assembly + dense FP64 Cholesky + triangular solves + residual check + "
+                           "back-substitution)", round(ms, 3), "ms", False, "f64",
+                     {"workload": f"LM/Schur: {pc} primaries x 6, {ec} secondaries x 3, {len(inst.primary)} "
+                                  "residuals x 2, Huber delta 1, lambda 1e-3; reduced 12,000 x 12,000 system "
+                                  "factored densely", "rows": n},
+                     {"bound": "mfma", "achieved": round(tflops, 2), "peak": 78.6, "unit": "TFLOP/s",
+                      "frac": round(tflops / 78.6, 4), "traffic": None,
+                      "kernel": "pm_dense_cholesky_solve (k_dc_potrf + k_dc_gemm FP64 MFMA + triangular solves)",
+                      "flops_per_launch": int(flops), "ms_per_launch": round(solve_ms, 3)},
+                     cpu)
+    line["solve"] = {"converged": rep.converged, "final_residual_norm": rep.final_residual_norm,
+                     "initial_residual_norm": rep.initial_residual_norm}
     return line
 
 

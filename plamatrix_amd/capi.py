@@ -301,11 +301,16 @@ class DenseReport:
     residual_norm: float
 
 
-def dense_cholesky_solve(a: DeviceCSR, b, x, ws: Workspace) -> DenseReport:
-    """Dense FP64 Cholesky solve of an SPD CSR system (lower triangle used)."""
-    d = a.descriptor()
+def dense_cholesky_solve(a, b, x, ws: Workspace) -> DenseReport:
+    """Dense FP64 Cholesky solve of an SPD CSR system (lower triangle used).
+    a: DeviceCSR or pm_csr descriptor; b, x: tensors or device addresses."""
+    d = a if isinstance(a, pm_csr) else a.descriptor()
+
+    def addr(v):
+        return _p(v) if isinstance(v, int) else _ptr(v)
+
     r = pm_dense_report()
-    check(lib().pm_dense_cholesky_solve(ctypes.byref(d), _ptr(b), _ptr(x), ctypes.byref(r), ws.handle,
+    check(lib().pm_dense_cholesky_solve(ctypes.byref(d), addr(b), addr(x), ctypes.byref(r), ws.handle,
                                         current_stream()))
     return DenseReport(bool(r.not_positive_definite), r.rhs_norm, r.residual_norm)
 

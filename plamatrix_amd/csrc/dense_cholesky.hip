@@ -52,6 +52,16 @@ constexpr int kDcPanel = 2 * kDcTile;
 
 __device__ __forceinline__ int pk(int i, int j) { return i * (i + 1) / 2 + j; }
 
+// Broadcast lane `src` (a constant) of a double through two v_readlane_b32.
+__device__ __forceinline__ double lane_value(double v, int src)
+{
+    const unsigned long long bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffull), src);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), src);
+    return __longlong_as_double((static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                static_cast<unsigned>(lo));
+}
+
 __global__ void k_dc_flag_reset(int* fail, unsigned* counter)
 {
     *fail = 0;
@@ -96,65 +106,174 @@ __global__ __launch_bounds__(kBlock) void k_dc_scatter(int64_t n, int64_t nnz, c
 // the inverse of its factor, zero-padded to 128 x 128 column-major, to inv.
 // A pivot that is not positive and finite raises *fail and is replaced by 1
 // so that later arithmetic stays finite.
+//
+// Both steps are blocked by 16 inside LDS (packed lower triangles of L and
+// X = L^-1), so the block costs about 40 workgroup barriers:
+//   factor, per 16-column step: one wavefront factors the 16 x 16 diagonal
+//   block in registers (a row per lane, readlane broadcasts), then one thread per row
+//   solves the rows below against it, then the trailing lower triangle takes
+//   the rank-16 update;
+//   inverse: each wavefront inverts one 16 x 16 diagonal block (a column per
+//   lane), then for block distance d = 1..7 every X[j+d][j] is
+//   -X[j+d][j+d] sum_{j<=k<j+d} L[j+d][k] X[k][j], through an LDS scratch.
+constexpr int kDcSub = 16;
+constexpr int kDcSubBlocks = kDcTile / kDcSub;
+
 __global__ __launch_bounds__(1024) void k_dc_potrf(int64_t n, int64_t c0, int w, double* a, double* inv, int* fail)
 {
     __shared__ double L[kDcPacked];
     __shared__ double X[kDcPacked];
+    __shared__ double U[(kDcSubBlocks - 1) * kDcSub * kDcSub];
     __shared__ int bad;
     if (*fail) return;
-    const int tid = threadIdx.x;
-    for (int idx = tid; idx < w * w; idx += 1024)
-    {
-        const int i = idx % w, j = idx / w;
-        if (j <= i) L[pk(i, j)] = a[(c0 + i) + (c0 + j) * n];
-    }
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // column j, rows i >= j: thread (i, j) = (tid % 128, tid / 128 + 8 m)
+    const int ti = tid & (kDcTile - 1), tj = tid >> 7;
+    for (int j = tj; j < w; j += 1024 / kDcTile)
+        if (ti >= j && ti < w) L[pk(ti, j)] = a[(c0 + ti) + (c0 + j) * n];
     if (tid == 0) bad = 0;
     __syncthreads();
-    for (int j = 0; j < w; ++j)
+
+    for (int j0 = 0; j0 < w; j0 += kDcSub)
     {
-        if (tid == 0)
+        const int jb = min(kDcSub, w - j0);
+        if (wave == 0)
         {
-            double d = L[pk(j, j)];
-            if (!(d > 0.0) || !isfinite(d))
+            double row[kDcSub];
+#pragma unroll
+            for (int c = 0; c < kDcSub; ++c) row[c] = (lane < jb && c <= lane) ? L[pk(j0 + lane, j0 + c)] : 0.0;
+#pragma unroll
+            for (int c = 0; c < kDcSub; ++c)
             {
-                bad = 1;
-                d = 1.0;
+                if (c < jb)
+                {
+                    double d = lane_value(row[c], c);
+                    if (!(d > 0.0) || !isfinite(d))
+                    {
+                        if (lane == 0) bad = 1;
+                        d = 1.0;
+                    }
+                    d = sqrt(d);
+                    if (lane == c) row[c] = d;
+                    if (lane > c && lane < jb) row[c] /= d;
+#pragma unroll
+                    for (int m = c + 1; m < kDcSub; ++m)
+                    {
+                        const double dm = lane_value(row[c], m);
+                        if (lane >= m && lane < jb) row[m] -= row[c] * dm;
+                    }
+                }
             }
-            L[pk(j, j)] = sqrt(d);
+#pragma unroll
+            for (int c = 0; c < kDcSub; ++c)
+                if (lane < jb && c <= lane) L[pk(j0 + lane, j0 + c)] = row[c];
         }
         __syncthreads();
-        if (tid > j && tid < w) L[pk(tid, j)] /= L[pk(j, j)];
+        const int rest = j0 + jb;
+        if (rest + tid < w)
+        {
+            const int i = rest + tid;
+            double x[kDcSub];
+#pragma unroll
+            for (int c = 0; c < kDcSub; ++c) x[c] = c < jb ? L[pk(i, j0 + c)] : 0.0;
+#pragma unroll
+            for (int c = 0; c < kDcSub; ++c)
+            {
+                if (c < jb)
+                {
+                    double v = x[c];
+#pragma unroll
+                    for (int t = 0; t < c; ++t) v -= x[t] * L[pk(j0 + c, j0 + t)];
+                    x[c] = v / L[pk(j0 + c, j0 + c)];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < kDcSub; ++c)
+                if (c < jb) L[pk(i, j0 + c)] = x[c];
+        }
         __syncthreads();
-        const int i = j + 1 + (tid >> 3);
+        const int i = rest + (tid >> 3);
         if (i < w)
         {
-            const double lij = L[pk(i, j)];
-            for (int t = j + 1 + (tid & 7); t <= i; t += 8) L[pk(i, t)] -= lij * L[pk(t, j)];
+            double li[kDcSub];
+#pragma unroll
+            for (int c = 0; c < kDcSub; ++c) li[c] = c < jb ? L[pk(i, j0 + c)] : 0.0;
+            for (int t = rest + (tid & 7); t <= i; t += 8)
+            {
+                double s = 0.0;
+#pragma unroll
+                for (int c = 0; c < kDcSub; ++c)
+                    if (c < jb) s += li[c] * L[pk(t, j0 + c)];
+                L[pk(i, t)] -= s;
+            }
         }
         __syncthreads();
     }
-    // X = L^-1, row by row: X[i][j] = (delta_ij - sum_{j<=t<i} L[i][t] X[t][j]) / L[i][i]
-    const int col = tid >> 3, q = tid & 7;
-    for (int i = 0; i < w; ++i)
+
+    // inverse: diagonal 16 x 16 blocks, one wavefront each, a column per lane
+    const int nblk = (w + kDcSub - 1) / kDcSub;
+    if (wave < nblk)
     {
-        double s = 0.0;
-        if (col <= i)
-            for (int t = col + q; t < i; t += 8) s += L[pk(i, t)] * X[pk(t, col)];
-        s += __shfl_xor(s, 1);
-        s += __shfl_xor(s, 2);
-        s += __shfl_xor(s, 4);
-        if (col <= i && q == 0) X[pk(i, col)] = ((i == col ? 1.0 : 0.0) - s) / L[pk(i, i)];
+        const int b0 = wave * kDcSub;
+        const int bs = min(kDcSub, w - b0);
+        const int c = lane;
+        double x[kDcSub];
+#pragma unroll
+        for (int r = 0; r < kDcSub; ++r)
+        {
+            double v = 0.0;
+            if (c < bs && r < bs && r >= c)
+            {
+                v = r == c ? 1.0 : 0.0;
+#pragma unroll
+                for (int t = 0; t < r; ++t)
+                    if (t >= c) v -= L[pk(b0 + r, b0 + t)] * x[t];
+                v /= L[pk(b0 + r, b0 + r)];
+            }
+            x[r] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < kDcSub; ++r)
+            if (c < bs && r < bs && r >= c) X[pk(b0 + r, b0 + c)] = x[r];
+    }
+    __syncthreads();
+    for (int d = 1; d < nblk; ++d)
+    {
+        const int count = (nblk - d) * kDcSub * kDcSub;
+        for (int e = tid; e < count; e += 1024)
+        {
+            const int j = e / (kDcSub * kDcSub), r = (e / kDcSub) % kDcSub, c = e % kDcSub;
+            const int gi = (j + d) * kDcSub + r, gc = j * kDcSub + c;
+            double s = 0.0;
+            if (gi < w)
+                for (int k = j; k < j + d; ++k)
+                    for (int t = 0; t < kDcSub; ++t)
+                    {
+                        const int gt = k * kDcSub + t;
+                        if (gt >= gc) s += L[pk(gi, gt)] * X[pk(gt, gc)];
+                    }
+            U[e] = s;
+        }
+        __syncthreads();
+        for (int e = tid; e < count; e += 1024)
+        {
+            const int j = e / (kDcSub * kDcSub), r = (e / kDcSub) % kDcSub, c = e % kDcSub;
+            const int b0 = (j + d) * kDcSub, gi = b0 + r, gc = j * kDcSub + c;
+            if (gi < w)
+            {
+                double s = 0.0;
+                for (int t = 0; t <= r; ++t)
+                    if (b0 + t < w) s += X[pk(gi, b0 + t)] * U[j * kDcSub * kDcSub + t * kDcSub + c];
+                X[pk(gi, gc)] = -s;
+            }
+        }
         __syncthreads();
     }
-    for (int idx = tid; idx < w * w; idx += 1024)
+
+    for (int j = tj; j < kDcTile; j += 1024 / kDcTile)
     {
-        const int i = idx % w, j = idx / w;
-        if (j <= i) a[(c0 + i) + (c0 + j) * n] = L[pk(i, j)];
-    }
-    for (int idx = tid; idx < kDcTile * kDcTile; idx += 1024)
-    {
-        const int i = idx % kDcTile, j = idx / kDcTile;
-        inv[idx] = (i < w && j <= i) ? X[pk(i, j)] : 0.0;
+        if (j < w && ti >= j && ti < w) a[(c0 + ti) + (c0 + j) * n] = L[pk(ti, j)];
+        inv[ti + j * kDcTile] = (ti < w && j <= ti) ? X[pk(ti, j)] : 0.0;
     }
     if (tid == 0 && bad) *fail = 1;
 }
@@ -173,14 +292,16 @@ __global__ __launch_bounds__(1024) void k_dc_potrf(int64_t n, int64_t c0, int w,
 // P may alias C (the in-place panel solve): each workgroup reads all K columns
 // of its own rows before it writes them, and workgroups own disjoint rows.
 template <bool SUB>
-__global__ __launch_bounds__(256) void k_dc_gemm(int64_t rows, int64_t cols, int K, const double* P, int64_t ldp,
-                                                 const double* __restrict__ Q, int64_t ldq, double* C, int64_t ldc,
-                                                 int tri, int mask_upper, const int* fail)
+__global__ __launch_bounds__(256, 2) void k_dc_gemm(int64_t rows, int64_t cols, int K, const double* P, int64_t ldp,
+                                                    const double* __restrict__ Q, int64_t ldq, double* C, int64_t ldc,
+                                                    int tri, int mask_upper, const int* fail)
 {
     if (*fail) return;
     int64_t ti, tj;
     const int64_t tr = (rows + kDcTile - 1) / kDcTile;
-    const int64_t b = blockIdx.x;
+    // each XCD takes a contiguous run of tiles (mostly whole tile rows), so a
+    // row's P slice stays in that XCD's L2
+    const int64_t b = xcd_logical_block(blockIdx.x, gridDim.x);
     if (tri)
     {
         int64_t t = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(b) + 1.0) - 1.0) * 0.5);
@@ -196,8 +317,10 @@ __global__ __launch_bounds__(256) void k_dc_gemm(int64_t rows, int64_t cols, int
     }
     const int64_t r0 = ti * kDcTile, c0 = tj * kDcTile;
 
-    __shared__ double Ps[kDcKc * kDcLd];
-    __shared__ double Qs[kDcKc * kDcLd];
+    // two K chunks in flight: the MFMAs read one LDS buffer while the next
+    // chunk is stored into the other, one barrier per chunk
+    __shared__ double Ps[2][kDcKc * kDcLd];
+    __shared__ double Qs[2][kDcKc * kDcLd];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
     const int li = lane & 15, lk = lane >> 4;
@@ -221,20 +344,25 @@ __global__ __launch_bounds__(256) void k_dc_gemm(int64_t rows, int64_t cols, int
             qr[e] = (c0 + i < cols && kg < K) ? Q[(c0 + i) + static_cast<int64_t>(kg) * ldq] : 0.0;
         }
     };
-    fetch(0);
-    for (int kc = 0; kc < K; kc += kDcKc)
-    {
-        __syncthreads();
+    auto stage = [&](int buf) {
 #pragma unroll
         for (int e = 0; e < kPer; ++e)
         {
             const int idx = tid + e * 256;
             const int i = idx & (kDcTile - 1), k = idx / kDcTile;
-            Ps[k * kDcLd + i] = pr[e];
-            Qs[k * kDcLd + i] = qr[e];
+            Ps[buf][k * kDcLd + i] = pr[e];
+            Qs[buf][k * kDcLd + i] = qr[e];
         }
-        __syncthreads();
-        if (kc + kDcKc < K) fetch(kc + kDcKc);
+    };
+    fetch(0);
+    stage(0);
+    if (kDcKc < K) fetch(kDcKc);
+    __syncthreads();
+    int buf = 0;
+    for (int kc = 0; kc < K; kc += kDcKc)
+    {
+        const double* ps = Ps[buf];
+        const double* qs = Qs[buf];
 #pragma unroll
         for (int k4 = 0; k4 < kDcKc; k4 += 4)
         {
@@ -243,14 +371,21 @@ __global__ __launch_bounds__(256) void k_dc_gemm(int64_t rows, int64_t cols, int
 #pragma unroll
             for (int t = 0; t < 4; ++t)
             {
-                av[t] = Qs[k * kDcLd + wc + 16 * t + li];
-                bv[t] = Ps[k * kDcLd + wr + 16 * t + li];
+                av[t] = qs[k * kDcLd + wc + 16 * t + li];
+                bv[t] = ps[k * kDcLd + wr + 16 * t + li];
             }
 #pragma unroll
             for (int x = 0; x < 4; ++x)
 #pragma unroll
                 for (int y = 0; y < 4; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
         }
+        if (kc + kDcKc < K)
+        {
+            stage(buf ^ 1);
+            if (kc + 2 * kDcKc < K) fetch(kc + 2 * kDcKc);
+        }
+        __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int x = 0; x < 4; ++x)
@@ -407,6 +542,15 @@ int dense_cholesky_solve(const pm_csr* a, const double* b, double* x, pm_workspa
                            a->col_indices, static_cast<const double*>(a->values), d.a);
     PM_HIP(hipGetLastError());
 
+    // Lookahead: the trailing update of panel p is split into the next
+    // panel's columns (on `stream`, right away) and the rest (on the
+    // workspace's second stream), so that factoring panel p+1 overlaps the
+    // bulk update.  Panel q is written by the bulk update of panel q-2 at the
+    // latest, which `stream` waits for before the update that precedes it.
+    PM_TRY(ws->ensure_aux());
+    hipStream_t aux = ws->aux_stream;
+    hipEvent_t panel_done = ws->aux_events[0], bulk_done = ws->aux_events[1];
+    bool bulk_pending = false;
     for (int64_t p0 = 0; p0 < n; p0 += kDcPanel)
     {
         const int w0 = static_cast<int>(std::min<int64_t>(kDcTile, n - p0));
@@ -423,13 +567,31 @@ int dense_cholesky_solve(const pm_csr* a, const double* b, double* x, pm_workspa
         }
         const int K = static_cast<int>(std::min<int64_t>(kDcPanel, n - p0));
         const int64_t t0 = p0 + K;
-        if (t0 < n)
+        if (t0 >= n) break;
+        const double* panel = d.a + t0 + p0 * n;
+        if (bulk_pending)
         {
-            const double* panel = d.a + t0 + p0 * n;
-            PM_TRY(launch_gemm<true>(n - t0, n - t0, K, panel, n, panel, n, d.a + t0 + t0 * n, n, true, true, d.fail,
-                                     stream));
+            PM_HIP(hipStreamWaitEvent(stream, bulk_done, 0));
+            bulk_pending = false;
+        }
+        // the next panel's columns
+        const int kn = static_cast<int>(std::min<int64_t>(kDcPanel, n - t0));
+        PM_TRY(launch_gemm<true>(n - t0, kn, K, panel, n, panel, n, d.a + t0 + t0 * n, n, false, true, d.fail,
+                                 stream));
+        // everything right of them
+        const int64_t r0 = t0 + kn;
+        if (r0 < n)
+        {
+            const double* rest = d.a + r0 + p0 * n;
+            PM_HIP(hipEventRecord(panel_done, stream));
+            PM_HIP(hipStreamWaitEvent(aux, panel_done, 0));
+            PM_TRY(launch_gemm<true>(n - r0, n - r0, K, rest, n, rest, n, d.a + r0 + r0 * n, n, true, true, d.fail,
+                                     aux));
+            PM_HIP(hipEventRecord(bulk_done, aux));
+            bulk_pending = true;
         }
     }
+    if (bulk_pending) PM_HIP(hipStreamWaitEvent(stream, bulk_done, 0));
 
     // x <- b, forward into y, backward into x
     PM_HIP(hipMemcpyAsync(x, b, sizeof(double) * static_cast<size_t>(n), hipMemcpyDeviceToDevice, stream));

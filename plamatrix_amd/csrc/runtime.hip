@@ -149,6 +149,14 @@ int pm_workspace::ensure_events()
     return PM_OK;
 }
 
+int pm_workspace::ensure_aux()
+{
+    if (!aux_stream) PM_HIP(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+    for (hipEvent_t& e : aux_events)
+        if (!e) PM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return PM_OK;
+}
+
 pm_workspace::~pm_workspace()
 {
     plans.reset();
@@ -163,6 +171,13 @@ pm_workspace::~pm_workspace()
     for (hipEvent_t& e : events)
         if (e) (void)hipEventDestroy(e);
     if (done_event) (void)hipEventDestroy(done_event);
+    if (aux_stream)
+    {
+        (void)hipStreamSynchronize(aux_stream);
+        (void)hipStreamDestroy(aux_stream);
+    }
+    for (hipEvent_t& e : aux_events)
+        if (e) (void)hipEventDestroy(e);
     if (host_state) (void)hipHostFree(host_state);
     if (host_coo) (void)hipHostFree(host_coo);
     if (host_batch) (void)hipHostFree(host_batch);

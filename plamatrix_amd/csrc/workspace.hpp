@@ -126,6 +126,10 @@ struct pm_workspace
     pm::SolverState* host_state = nullptr;  // pinned, 2 slots
     hipEvent_t events[2] = {nullptr, nullptr};
     hipEvent_t done_event = nullptr;        // completion of the last enqueued work
+    // second stream (default priority) for work that overlaps the bound
+    // stream inside one call; every call joins it back before returning
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t aux_events[2] = {nullptr, nullptr};
 
     // COO conversion in flight
     int64_t coo_rows = 0, coo_cols = 0, coo_count = 0;
@@ -143,5 +147,6 @@ struct pm_workspace
 
     int bind(hipStream_t s);
     int ensure_events();
+    int ensure_aux();
     ~pm_workspace();
 };

@@ -6,7 +6,9 @@ FP64 MFMA in tile order), so each case is checked against what the mathematics
 fixes: the solution of the same symmetric positive definite system by LAPACK
 (numpy.linalg.solve) within a forward-error bound stated per case, the residual
 norm the kernel reports against the one recomputed from its x, exact
-detection of indefinite matrices, and bitwise repeatability.
+detection of indefinite matrices, and bitwise repeatability.  The Schur
+backend is also compared with the reference's own DenseCpu solve
+(oracle/_ref, built from the reference's sources) on the same residuals.
 """
 import numpy as np
 import pytest
@@ -220,6 +222,27 @@ def test_schur_dense_backend_matches_lapack_and_pcg(dev, ws):
     got = np.concatenate([dp.cpu().numpy(), de.cpu().numpy()])
     other = np.concatenate([dp2.cpu().numpy(), de2.cpu().numpy()])
     assert np.linalg.norm(got - other) <= 1e-8 * np.linalg.norm(other)
+
+
+def test_schur_dense_matches_reference_dense_cpu(dev, ws, ref_lib):
+    # the reference's own DenseCpu backend (oracle/_ref, built from its sources)
+    # on the same residual batch and weights
+    pc, ec = 40, 2000
+    sw = capi.SchurWorkspace()
+    inst, ne = schur_instance(dev, sw, pc, ec, seed=5)
+    dp = torch.zeros(pc * 6, dtype=torch.float64, device=dev)
+    de = torch.zeros(ec * 3, dtype=torch.float64, device=dev)
+    rep = capi.schur_solve(sw, ne, inst.damping, dp, de, ws, relative_tolerance=1e-12, absolute_tolerance=0.0,
+                           dense=True)
+    rdp, rde, rrep = ref_lib.ref_schur_solve_batch(pc, ec, 6, 3, inst.primary, inst.eliminated,
+                                                   inst.primary_jacobians, inst.eliminated_jacobians, inst.residuals,
+                                                   inst.weights, inst.damping, backend=1, relative_tolerance=1e-12,
+                                                   absolute_tolerance=0.0)
+    assert rep.converged and rrep.converged and rrep.iterations == rep.iterations == 1
+    got = np.concatenate([dp.cpu().numpy(), de.cpu().numpy()])
+    ref = np.concatenate([rdp, rde])
+    assert np.linalg.norm(got - ref) <= 1e-10 * np.linalg.norm(ref)
+    assert rep.initial_residual_norm == pytest.approx(rrep.initial_residual, rel=1e-12)
 
 
 def test_schur_dense_full_size(dev, ws):
