@@ -1,0 +1,150 @@
+// Detection and construction of the diagonal layout (see dia.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdint>
+#include <cstring>
+
+#include "dia.hpp"
+#include "workspace.hpp"
+
+namespace pm
+{
+
+namespace
+{
+
+constexpr int kEmpty = INT_MIN;
+
+// table[0..kMaxDia): distinct offsets (kEmpty where unused); table[kMaxDia]:
+// flags (1: a row's columns are not strictly increasing, 2: more than kMaxDia
+// diagonals).  Each workgroup collects its rows' offsets in LDS first, then
+// merges them into the global table with at most kMaxDia compare-and-swaps.
+__global__ void k_dia_reset(int* table)
+{
+    const int t = threadIdx.x;
+    if (t < kMaxDia) table[t] = kEmpty;
+    if (t == kMaxDia) table[t] = 0;
+}
+
+__device__ __forceinline__ bool dia_insert(int* slots, int off, bool shared_mem)
+{
+    for (int s = 0; s < kMaxDia; ++s)
+    {
+        int cur = shared_mem ? __atomic_load_n(&slots[s], __ATOMIC_RELAXED)
+                             : __hip_atomic_load(&slots[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == off) return true;
+        if (cur == kEmpty)
+        {
+            cur = atomicCAS(&slots[s], kEmpty, off);
+            if (cur == kEmpty || cur == off) return true;
+        }
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t rows, const int32_t* __restrict__ ro,
+                                                      const int32_t* __restrict__ ci, int* table)
+{
+    __shared__ int local[kMaxDia];
+    __shared__ int flags;
+    if (threadIdx.x < kMaxDia) local[threadIdx.x] = kEmpty;
+    if (threadIdx.x == 0) flags = 0;
+    __syncthreads();
+    int my_flags = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += stride)
+    {
+        int prev = -1;
+        const int end = ro[r + 1];
+        for (int k = ro[r]; k < end; ++k)
+        {
+            const int c = ci[k];
+            if (c <= prev) my_flags |= 1;
+            prev = c;
+            if (!dia_insert(local, static_cast<int>(c - r), true)) my_flags |= 2;
+        }
+    }
+    if (my_flags) atomicOr(&flags, my_flags);
+    __syncthreads();
+    if (threadIdx.x < kMaxDia)
+    {
+        const int off = local[threadIdx.x];
+        if (off != kEmpty && !dia_insert(table, off, false)) atomicOr(&table[kMaxDia], 2);
+    }
+    if (threadIdx.x == 0 && flags) atomicOr(&table[kMaxDia], flags);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_dia_fill(int64_t rows, const int32_t* __restrict__ ro,
+                                                    const int32_t* __restrict__ ci, const S* __restrict__ v,
+                                                    DiaArgs dia, S* __restrict__ dv, uint16_t* __restrict__ mask)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += stride)
+    {
+        unsigned m = 0;
+        int d = 0;
+        const int end = ro[r + 1];
+        for (int k = ro[r]; k < end; ++k)
+        {
+            const int off = static_cast<int>(ci[k] - r);
+            while (d < dia.ndiag && dia.off[d] < off) ++d;  // columns increase, so d only moves forward
+            if (d >= dia.ndiag || dia.off[d] != off) continue;  // cannot happen after detection
+            dv[static_cast<int64_t>(d) * rows + r] = v[k];
+            m |= 1u << d;
+        }
+        mask[r] = static_cast<uint16_t>(m);
+    }
+}
+
+} // namespace
+
+int dia_build(pm_workspace* ws, const pm_csr* a, DiaLayout* out, hipStream_t stream)
+{
+    *out = DiaLayout{};
+    const int64_t n = a->rows;
+    if (n <= 0 || a->nnz <= 0 || !a->row_offsets32 || !a->col_indices32 || a->rows != a->cols) return PM_OK;
+    const size_t es = a->dtype == PM_F64 ? 8 : 4;
+    const size_t table_bytes = 256;
+    PM_TRY(ws->dia.reserve(table_bytes));
+    int* table = ws->dia.as<int>();
+    hipLaunchKernelGGL(k_dia_reset, dim3(1), dim3(64), 0, stream, table);
+    const int grid = static_cast<int>(std::min<int64_t>(ceil_div(n, kBlock), 2048));
+    hipLaunchKernelGGL(k_dia_detect, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32, a->col_indices32,
+                       table);
+    PM_HIP(hipGetLastError());
+    int host[kMaxDia + 1];
+    PM_HIP(hipMemcpyAsync(host, table, sizeof(host), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    if (host[kMaxDia] != 0) return PM_OK;
+    DiaArgs args{};
+    for (int s = 0; s < kMaxDia; ++s)
+        if (host[s] != kEmpty) args.off[args.ndiag++] = host[s];
+    std::sort(args.off, args.off + args.ndiag);
+    if (args.ndiag == 0 || static_cast<double>(args.ndiag) * static_cast<double>(n) >
+                               kDiaMaxFill * static_cast<double>(a->nnz))
+        return PM_OK;
+    const size_t values_bytes = (es * static_cast<size_t>(args.ndiag) * static_cast<size_t>(n) + 255) & ~size_t(255);
+    const size_t mask_bytes = (sizeof(uint16_t) * static_cast<size_t>(n) + 255) & ~size_t(255);
+    PM_TRY(ws->dia.reserve(table_bytes + values_bytes + mask_bytes));
+    char* base = ws->dia.as<char>();
+    void* dv = base + table_bytes;
+    uint16_t* mask = reinterpret_cast<uint16_t*>(base + table_bytes + values_bytes);
+    if (a->dtype == PM_F64)
+        hipLaunchKernelGGL(k_dia_fill<double>, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32,
+                           a->col_indices32, static_cast<const double*>(a->values), args, static_cast<double*>(dv),
+                           mask);
+    else
+        hipLaunchKernelGGL(k_dia_fill<float>, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32,
+                           a->col_indices32, static_cast<const float*>(a->values), args, static_cast<float*>(dv), mask);
+    PM_HIP(hipGetLastError());
+    out->active = true;
+    out->args = args;
+    out->values = dv;
+    out->mask = mask;
+    return PM_OK;
+}
+
+} // namespace pm

@@ -29,6 +29,7 @@
 #include <deque>
 #include <string>
 
+#include "dia.hpp"
 #include "plan_cache.hpp"
 #include "spmv_kernels.hpp"
 #include "tuning.hpp"
@@ -166,7 +167,13 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den(int64_t rows, const I* _
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     double den = 0.0;
-    PM_FOR_CHUNKS(c, chunk_count(rows), kDirWaves)
+    // XCD-contiguous row ranges, as in k_spmv_den_dia (the p.q partials of
+    // both kernels cover the same rows per workgroup)
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    for (int64_t c = cbegin + wave; c < cend; c += kDirWaves)
     {
         const int64_t row0 = c * kChunkRows;
         int64_t s, e;
@@ -175,6 +182,59 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den(int64_t rows, const I* _
         const int64_t row = row0 + lane;
         if (row < rows)
         {
+            q[row] = acc;
+            den = __builtin_fma(static_cast<double>(p[row]), static_cast<double>(acc), den);
+        }
+    }
+    double part[1] = {den};
+    publish_partials<1>(part, partials + kPartialsDen);
+}
+
+// k_spmv_den on the diagonal layout: a lane per row adds the row's present
+// diagonals in increasing column order with the CSR row walk's fused
+// multiply-adds, so q and p.q are bitwise those of k_spmv_den.
+template <typename S, int ND>
+__global__ __launch_bounds__(kDirBlock) void k_spmv_den_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                            const uint16_t* __restrict__ mask,
+                                                            const S* __restrict__ p, S* __restrict__ q,
+                                                            const SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
+    const int lane = threadIdx.x % kWave;
+    const int nd = ND > 0 ? ND : dia.ndiag;
+    double den = 0.0;
+    constexpr int kSlots = ND > 0 ? ND : kMaxDia;
+    // each XCD walks one contiguous range of rows, so the gathered p entries
+    // (the same rows plus the diagonals' reach) are fetched into its L2 once
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    for (int64_t c = cbegin + threadIdx.x / kWave; c < cend; c += kDirWaves)
+    {
+        const int64_t row = c * kChunkRows + lane;
+        if (row < rows)
+        {
+            // every load is issued before the mask is known (absent slots read
+            // an in-range value and a clamped gather that the mask discards),
+            // so the row's loads are all in flight together
+            const unsigned m = mask[row];
+            S vals[kSlots], xs[kSlots];
+#pragma unroll
+            for (int d = 0; d < kSlots; ++d)
+            {
+                if (d < nd)
+                {
+                    int64_t col = row + dia.off[d];
+                    col = col < 0 ? 0 : (col >= rows ? rows - 1 : col);
+                    vals[d] = dv[d * rows + row];
+                    xs[d] = p[col];
+                }
+            }
+            S acc = S(0);
+#pragma unroll
+            for (int d = 0; d < kSlots; ++d)
+                if (d < nd && ((m >> d) & 1u)) acc = fma_s(vals[d], xs[d], acc);
             q[row] = acc;
             den = __builtin_fma(static_cast<double>(p[row]), static_cast<double>(acc), den);
         }
@@ -449,6 +509,7 @@ struct Problem
     int grid_units;          // workgroups of the unit kernels (publish r.r, r.z)
     bool nt;                 // non-temporal matrix loads in the iteration
     bool dma;                // 4-byte matrix operands staged into LDS by global_load_lds
+    DiaLayout dia;           // diagonal layout of A for q = A p, when it pays
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
@@ -537,12 +598,22 @@ struct Calls
         out.push_back(make_call(k_direction<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, zsrc,
                                 static_cast<const S*>(p_old), static_cast<S*>(p_new), pb.st, pb.partials, par,
                                 pb.pre == kPreFactored ? pb.grid_rows : pb.grid_units));
-        const auto spmv_den = pb.nt ? (pb.dma ? k_spmv_den<S, I, true, true> : k_spmv_den<S, I, true, false>)
-                                    : (pb.dma ? k_spmv_den<S, I, false, true> : k_spmv_den<S, I, false, false>);
-        out.push_back(make_call(spmv_den, dim3(pb.grid_rows),
-                                dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
-                                static_cast<const S*>(p_new), static_cast<S*>(pb.q),
-                                static_cast<const SolverState*>(pb.st), pb.partials));
+        if (pb.dia.active)
+        {
+            const int nd = pb.dia.args.ndiag;
+            const auto dia_den = nd == 7 ? k_spmv_den_dia<S, 7> : nd == 5 ? k_spmv_den_dia<S, 5> : k_spmv_den_dia<S, 0>;
+            out.push_back(make_call(dia_den, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, pb.dia.args,
+                                    static_cast<const S*>(pb.dia.values), pb.dia.mask, static_cast<const S*>(p_new),
+                                    static_cast<S*>(pb.q), static_cast<const SolverState*>(pb.st), pb.partials));
+        }
+        else
+        {
+            const auto spmv_den = pb.nt ? (pb.dma ? k_spmv_den<S, I, true, true> : k_spmv_den<S, I, true, false>)
+                                        : (pb.dma ? k_spmv_den<S, I, false, true> : k_spmv_den<S, I, false, false>);
+            out.push_back(make_call(spmv_den, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a),
+                                    static_cast<const S*>(pb.a->values), static_cast<const S*>(p_new),
+                                    static_cast<S*>(pb.q), static_cast<const SolverState*>(pb.st), pb.partials));
+        }
         if (!refresh)
         {
             if (pb.pre == kPreFactored)
@@ -641,12 +712,27 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     return PM_OK;
 }
 
+// The diagonal layout is tried for large matrices with few entries per row
+// (its detection reads the matrix once and synchronises the stream once);
+// PLAMATRIX_AMD_DIA=0 keeps every solve on the CSR kernels.
+bool dia_wanted(const pm_csr* a)
+{
+    const char* env = std::getenv("PLAMATRIX_AMD_DIA");
+    if (env != nullptr && env[0] == '0') return false;
+    return use_compact(a) && a->rows >= 65536 && a->nnz <= static_cast<int64_t>(kMaxDia) * a->rows;
+}
+
 std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t count, int refresh)
 {
     // workspace addresses are part of the key: a grown workspace must never
     // replay a plan that still points at its old buffers
-    char buf[640];
-    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld|%d|%p|%p|%p|%p",
+    char buf[768];
+    unsigned long long dia_sig = 1469598103934665603ull;  // FNV-1a over the diagonal offsets
+    if (pb.dia.active)
+        for (int d = 0; d < pb.dia.args.ndiag; ++d)
+            dia_sig = (dia_sig ^ static_cast<unsigned>(pb.dia.args.off[d])) * 1099511628211ull;
+    std::snprintf(buf, sizeof(buf),
+                  "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld|%d|%p|%p|%p|%p|%d|%d|%llx|%p|%p",
                   kind,
                   pb.r, static_cast<void*>(pb.st), static_cast<void*>(pb.partials),
                   (const void*)pb.a->row_offsets, (const void*)pb.a->row_offsets32, (const void*)pb.a->col_indices,
@@ -655,7 +741,8 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
                   static_cast<long long>(pb.bsize), (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0), pb.gf ? pb.gf->values : nullptr,
                   pb.gf ? (const void*)pb.gf->row_offsets32 : nullptr, pb.gtf ? pb.gtf->values : nullptr,
-                  pb.gtf ? (const void*)pb.gtf->row_offsets32 : nullptr);
+                  pb.gtf ? (const void*)pb.gtf->row_offsets32 : nullptr, pb.dia.active ? 1 : 0, pb.dia.args.ndiag,
+                  dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask));
     return buf;
 }
 
@@ -741,6 +828,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     pb.bsize = pre == kPreBlock ? bsize : 1;
     pb.pre = pre;
     PM_TRY(prepare(ws, pb, stream));
+    if (dia_wanted(a)) PM_TRY(dia_build(ws, a, &pb.dia, stream));
     if (a->rows == 0)
     {
         pm_solver_report rep{};

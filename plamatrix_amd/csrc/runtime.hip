@@ -168,6 +168,7 @@ pm_workspace::~pm_workspace()
     refine.release();
     batch.release();
     dense.release();
+    dia.release();
     for (hipEvent_t& e : events)
         if (e) (void)hipEventDestroy(e);
     if (done_event) (void)hipEventDestroy(done_event);
@@ -229,6 +230,7 @@ int pm_workspace_close(pm_workspace* ws)
     ws->refine.release();
     ws->batch.release();
     ws->dense.release();
+    ws->dia.release();
     ws->capacity_rows = 0;
     ws->bound = false;
     ws->stream = nullptr;
@@ -240,7 +242,8 @@ size_t pm_workspace_capacity_bytes(const pm_workspace* ws)
 {
     if (!ws) return 0;
     return ws->vectors.bytes + ws->partials.bytes + ws->state.bytes + ws->scratch.bytes + ws->sort.bytes +
-           ws->refine.bytes + ws->batch.bytes + ws->dense.bytes;
+           ws->refine.bytes + ws->batch.bytes + ws->dense.bytes +
+           ws->dia.bytes;
 }
 
 int64_t pm_workspace_capacity_rows(const pm_workspace* ws) { return ws ? ws->capacity_rows : 0; }

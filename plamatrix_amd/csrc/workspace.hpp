@@ -121,6 +121,7 @@ struct pm_workspace
     pm::DeviceBuffer refine;    // FP64 residual / FP32 correction of the refinement loop
     pm::DeviceBuffer batch;     // batched solver: layout, per-system state, per-chunk partials
     pm::DeviceBuffer dense;     // dense Cholesky: n x n lower triangle, diagonal-block inverses
+    pm::DeviceBuffer dia;       // diagonal layout of the solver's matrix
     int64_t capacity_rows = 0;
 
     pm::SolverState* host_state = nullptr;  // pinned, 2 slots

@@ -1,0 +1,122 @@
+"""GPU parity of the diagonal (DIA) layout the iterative solvers use for large
+stencil matrices (plamatrix_amd/csrc/dia.hip).
+
+The layout adds each row's present entries in increasing column order with the
+same fused multiply-adds as the CSR row walk, so a solve with it must be
+bitwise the solve without it (PLAMATRIX_AMD_DIA=0): same iteration count, same
+residual, same solution bits.  That the layout was really used is read from
+the workspace's storage, which grows by the diagonal values.  Matrices the
+layout does not fit (unsorted rows, too many diagonals) fall back to CSR.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from plamatrix_amd import capi, generators  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def dev():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    capi.lib()
+    return torch.device("cuda:0")
+
+
+def host(x):
+    torch.cuda.synchronize()
+    return x.detach().cpu().numpy()
+
+
+def from_scipy(m, dtype):
+    m = m.tocsr()
+    m.sort_indices()
+    return generators.CSRHost(m.shape[0], m.shape[1], m.indptr.astype(np.int64), m.indices.astype(np.int64),
+                              m.data.astype(dtype))
+
+
+def nine_point(nx):
+    import scipy.sparse as sp
+
+    t = sp.diags([1.0, 1.0, 1.0], [-1, 0, 1], shape=(nx, nx))
+    return 9.0 * sp.identity(nx * nx) - sp.kron(t, t)
+
+
+def solve_both(dev, h, dtype, monkeypatch, rel, refresh=50):
+    a = capi.to_device_csr(h, compact=True)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=31).astype(dtype)).to(dev)
+    out = {}
+    grow = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_DIA", flag)
+        ws = capi.Workspace()
+        x = torch.zeros(h.rows, dtype=b.dtype, device=dev)
+        before = ws.capacity_bytes
+        rep = capi.pcg(a, b, x, ws, capi.policy(refresh_interval=refresh), relative_tolerance=rel,
+                       max_iterations=20000)
+        torch.cuda.synchronize()
+        grow[flag] = ws.capacity_bytes - before
+        out[flag] = (rep, host(x).copy())
+    return out, grow
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_diffusion3d_dia_is_bitwise_csr(dev, monkeypatch, dtype):
+    h = generators.diffusion3d_csr(48, seed=7, dtype=dtype)  # 110,592 rows, 7 diagonals
+    out, grow = solve_both(dev, h, dtype, monkeypatch, 1e-6 if dtype == np.float32 else 1e-10)
+    es = np.dtype(dtype).itemsize
+    assert grow["1"] - grow["0"] >= 7 * h.rows * es  # the diagonal values were built
+    (r0, x0), (r1, x1) = out["0"], out["1"]
+    assert r0.converged and r1.converged
+    assert r0.iterations == r1.iterations
+    assert r0.final_residual == r1.final_residual
+    assert np.array_equal(x0, x1)
+
+
+def test_laplacian2d_five_diagonals(dev, monkeypatch):
+    h = generators.laplacian2d_csr(400, dtype=np.float32)  # 160,000 rows
+    out, grow = solve_both(dev, h, np.float32, monkeypatch, 1e-6)
+    assert grow["1"] - grow["0"] >= 5 * h.rows * 4
+    assert out["0"][0].iterations == out["1"][0].iterations
+    assert np.array_equal(out["0"][1], out["1"][1])
+
+
+def test_nine_point_generic_path(dev, monkeypatch):
+    h = from_scipy(nine_point(320), np.float64)  # 102,400 rows, 9 diagonals (runtime diagonal count)
+    out, grow = solve_both(dev, h, np.float64, monkeypatch, 1e-10)
+    assert grow["1"] - grow["0"] >= 9 * h.rows * 8
+    assert out["0"][0].iterations == out["1"][0].iterations
+    assert np.array_equal(out["0"][1], out["1"][1])
+
+
+def test_unsorted_rows_fall_back(dev, monkeypatch):
+    h = generators.diffusion3d_csr(48, seed=9, dtype=np.float64)
+    # reverse the entries of every row: same matrix, columns decreasing
+    ro, ci, v = h.row_offsets, h.col_indices.copy(), h.values.copy()
+    for r in range(h.rows):
+        s, e = ro[r], ro[r + 1]
+        ci[s:e] = ci[s:e][::-1]
+        v[s:e] = v[s:e][::-1]
+    rev = generators.CSRHost(h.rows, h.cols, ro, ci, v)
+    out, grow = solve_both(dev, rev, np.float64, monkeypatch, 1e-10)
+    assert grow["1"] - grow["0"] < h.rows * 8  # no diagonal values
+    assert out["1"][0].converged and np.array_equal(out["0"][1], out["1"][1])
+
+
+def test_refinement_with_dia_reaches_target(dev, monkeypatch):
+    monkeypatch.setenv("PLAMATRIX_AMD_DIA", "1")
+    h = generators.diffusion3d_csr(64, seed=11)  # FP32 matrix, 262,144 rows
+    a = capi.to_device_csr(h, compact=True)
+    b = generators.rhs_normal(h.rows, seed=12).astype(np.float64)
+    x = torch.zeros(h.rows, dtype=torch.float64, device=dev)
+    ws = capi.Workspace()
+    rep = capi.pcg_refine(a, torch.from_numpy(b).to(dev), x, ws, target=1e-8, max_outer=20,
+                          pol=capi.policy(refresh_interval=50), relative_tolerance=1e-3, max_iterations=20000)
+    assert rep.converged and rep.final_relres <= 1e-8
+    import scipy.sparse as sp
+
+    m = sp.csr_matrix((h.values.astype(np.float64), h.col_indices, h.row_offsets), shape=(h.rows, h.cols))
+    xs = host(x)
+    assert np.linalg.norm(b - m @ xs) <= 1.2e-8 * np.linalg.norm(b)
