@@ -39,11 +39,14 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4) -> 
     return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + 2 * n * value_bytes
 
 
-def pcg_iteration_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4) -> int:
+def pcg_iteration_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4, diagonals: int = 0) -> int:
     """Algorithmic bytes of one Jacobi-PCG iteration (three kernels): the
     direction kernel reads z and p and writes p; the SpMV streams values,
-    columns and offsets, reads p once and writes q; the vector update reads
-    x, p, r, q, M^-1 and writes x, r, z."""
+    columns and offsets (or, on the diagonal layout, `diagonals` value slots
+    and a 2-byte mask per row), reads p once and writes q; the vector update
+    reads x, p, r, q, M^-1 and writes x, r, z."""
+    if diagonals:
+        return diagonals * n * value_bytes + 2 * n + 13 * n * value_bytes
     return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + 13 * n * value_bytes
 
 
@@ -188,7 +191,8 @@ def run_diffusion(args, torch, capi, generators):
     ms_per_solve, rep = timed(jacobi_solve)
     relres = capi.true_relres(a, b, x, ws)
     iterations = rep.inner_iterations
-    it_bytes = pcg_iteration_bytes(n, nnz)
+    diagonals = ws.solver_diagonals
+    it_bytes = pcg_iteration_bytes(n, nnz, diagonals=diagonals)
     it_ms = ms_per_solve / max(iterations, 1)
 
     # ---- learned preconditioner: G inferred on this matrix inside the timed solve ----
@@ -294,6 +298,7 @@ def run_diffusion(args, torch, capi, generators):
         "learned": learned,
         "solve": {"preconditioner": "jacobi", "ms": round(ms_per_solve, 3), "converged": rep.converged, "outer": rep.outer_iterations, "inner_iterations": iterations,
                   "true_relres": relres, "ms_per_iteration": round(it_ms, 4),
+                  "matrix_layout": f"diagonal ({diagonals} diagonals)" if diagonals else "csr",
                   "iteration_bytes": it_bytes, "iteration_gbs": round(it_bytes / (it_ms * 1e-3) / 1e9, 1),
                   "to_1e8": {"ms": round(ms_1e8, 2), "outer": rep8.outer_iterations,
                              "inner_iterations": rep8.inner_iterations, "relres": rep8.final_relres}},

@@ -77,6 +77,13 @@ int pm_workspace_close(pm_workspace* ws);
 size_t pm_workspace_capacity_bytes(const pm_workspace* ws);
 /* Vector capacity (rows) of the iterative-solver storage (capacitySize()). */
 int64_t pm_workspace_capacity_rows(const pm_workspace* ws);
+/* No reference counterpart: the number of diagonals of the diagonal layout
+ * the last adaptive solve streamed A in (0: the CSR kernels).  Square matrices
+ * of at least 65,536 rows whose entries lie on at most 16 diagonals, with
+ * strictly increasing columns in every row, are re-laid diagonal-major at the
+ * start of each solve (PLAMATRIX_AMD_DIA=0 disables it); results are bitwise
+ * those of the CSR kernels. */
+int32_t pm_workspace_solver_diagonals(const pm_workspace* ws);
 
 /* Structural validation on the device (CSRMatrix::validateStructure,
  * csr_matrix.h:197-237, 633-667): offsets start at 0, end at nnz and are

@@ -79,37 +79,20 @@ __global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel
     for (int i = 0; i < 8; ++i) st->counters[i] = 0u;
 }
 
-// First kernel of iteration t (slot par = t % 2): every workgroup reduces the
-// previous update's r.r and r.z partials in the same order, decides stop /
-// beta exactly alike, and forms p_new = z + beta p_old for its rows.  Block 0
-// records the decision in the state.
+// The decision at the head of iteration t (slot par = t % 2): every
+// workgroup reduces the previous update's r.r and r.z partials in the same
+// order and derives the same stop code and beta; workgroup 0 records them.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_direction(int64_t rows, const S* __restrict__ z,
-                                                      const S* __restrict__ p_old, S* __restrict__ p_new,
-                                                      SolverState* st, double* partials, int par, int count_res)
+__device__ __forceinline__ int direction_decision(SolverState* st, const double* partials, int par, int count_res,
+                                                  double& beta)
 {
-    if (st->done != kRunning) return;
-    // the first step's operands are loaded before the cross-workgroup
-    // reduction so their latency overlaps it
-    constexpr int U = 4;
-    int64_t begin, end;
-    block_range(rows, U * kBlock, begin, end);
-    const int64_t first = begin + threadIdx.x;
-    S fz[U], fp[U];
-#pragma unroll
-    for (int j = 0; j < U; ++j)
-    {
-        const int64_t i = first + j * kBlock;
-        fz[j] = i < end ? z[i] : S(0);
-        fp[j] = i < end ? p_old[i] : S(0);
-    }
     double res[2];
     reduce_partials<2>(partials + kPartialsRes, count_res, res);
     const double rr = res[0], rz = res[1];
     const int completed = st->iter[1 - par] + 1;
     const double tol = completed == 0 ? fmax(st->abs_tol, st->rel_tol * sqrt(rr)) : st->tol;
     int code = kRunning;
-    double beta = 0.0;
+    beta = 0.0;
     if (!isfinite(rr) || rr < 0.0) code = completed == 0 ? kInitNonFinite : kNonFinite;
     else if (sqrt(rr) <= tol) code = kConverged;
     else if (completed == 0 && st->bad_row != ~0ull) code = kBadDiagonal;
@@ -139,7 +122,35 @@ __global__ __launch_bounds__(kBlock) void k_direction(int64_t rows, const S* __r
         }
         if (code != kRunning) st->done = code;
     }
-    if (code != kRunning) return;
+    return code;
+}
+
+// First kernel of iteration t (slot par = t % 2): every workgroup reduces the
+// previous update's r.r and r.z partials in the same order, decides stop /
+// beta exactly alike, and forms p_new = z + beta p_old for its rows.  Block 0
+// records the decision in the state.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_direction(int64_t rows, const S* __restrict__ z,
+                                                      const S* __restrict__ p_old, S* __restrict__ p_new,
+                                                      SolverState* st, double* partials, int par, int count_res)
+{
+    if (st->done != kRunning) return;
+    // the first step's operands are loaded before the cross-workgroup
+    // reduction so their latency overlaps it
+    constexpr int U = 4;
+    int64_t begin, end;
+    block_range(rows, U * kBlock, begin, end);
+    const int64_t first = begin + threadIdx.x;
+    S fz[U], fp[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+    {
+        const int64_t i = first + j * kBlock;
+        fz[j] = i < end ? z[i] : S(0);
+        fp[j] = i < end ? p_old[i] : S(0);
+    }
+    double beta = 0.0;
+    if (direction_decision<S>(st, partials, par, count_res, beta) != kRunning) return;
     const S b = static_cast<S>(beta);
     for (int64_t base = first; base < end; base += U * kBlock)
     {
@@ -237,6 +248,62 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den_dia(int64_t rows, DiaArg
                 if (d < nd && ((m >> d) & 1u)) acc = fma_s(vals[d], xs[d], acc);
             q[row] = acc;
             den = __builtin_fma(static_cast<double>(p[row]), static_cast<double>(acc), den);
+        }
+    }
+    double part[1] = {den};
+    publish_partials<1>(part, partials + kPartialsDen);
+}
+
+// k_direction and k_spmv_den_dia in one kernel: after the decision every
+// workgroup forms p_new = z + beta p_old on the fly for its gathers (the same
+// fused multiply-add as k_direction), writes p_new for its own rows and
+// publishes p_new.q, so the iteration is bitwise the unfused one with one
+// kernel and one pass over z and p_old fewer.
+template <typename S, int ND>
+__global__ __launch_bounds__(kDirBlock) void k_dir_spmv_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                            const uint16_t* __restrict__ mask,
+                                                            const S* __restrict__ z, const S* __restrict__ p_old,
+                                                            S* __restrict__ p_new, S* __restrict__ q,
+                                                            SolverState* st, double* partials, int par, int count_res)
+{
+    if (st->done != kRunning) return;
+    double beta = 0.0;
+    if (direction_decision<S>(st, partials, par, count_res, beta) != kRunning) return;
+    const S b = static_cast<S>(beta);
+    const int lane = threadIdx.x % kWave;
+    const int nd = ND > 0 ? ND : dia.ndiag;
+    double den = 0.0;
+    constexpr int kSlots = ND > 0 ? ND : kMaxDia;
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    for (int64_t c = cbegin + threadIdx.x / kWave; c < cend; c += kDirWaves)
+    {
+        const int64_t row = c * kChunkRows + lane;
+        if (row < rows)
+        {
+            const unsigned m = mask[row];
+            S vals[kSlots], xs[kSlots];
+#pragma unroll
+            for (int d = 0; d < kSlots; ++d)
+            {
+                if (d < nd)
+                {
+                    int64_t col = row + dia.off[d];
+                    col = col < 0 ? 0 : (col >= rows ? rows - 1 : col);
+                    vals[d] = dv[d * rows + row];
+                    xs[d] = fma_s(b, p_old[col], z[col]);
+                }
+            }
+            S acc = S(0);
+#pragma unroll
+            for (int d = 0; d < kSlots; ++d)
+                if (d < nd && ((m >> d) & 1u)) acc = fma_s(vals[d], xs[d], acc);
+            const S pn = fma_s(b, p_old[row], z[row]);
+            p_new[row] = pn;
+            q[row] = acc;
+            den = __builtin_fma(static_cast<double>(pn), static_cast<double>(acc), den);
         }
     }
     double part[1] = {den};
@@ -510,6 +577,7 @@ struct Problem
     bool nt;                 // non-temporal matrix loads in the iteration
     bool dma;                // 4-byte matrix operands staged into LDS by global_load_lds
     DiaLayout dia;           // diagonal layout of A for q = A p, when it pays
+    bool fuse_direction;     // on the diagonal layout: p_new formed inside the SpMV
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
@@ -595,12 +663,28 @@ struct Calls
         const void* p_old = par == 0 ? pb.p0 : pb.p1;
         void* p_new = par == 0 ? pb.p1 : pb.p0;
         const S* zsrc = pb.pre == kPreNone ? static_cast<const S*>(pb.r) : static_cast<const S*>(pb.z);
-        out.push_back(make_call(k_direction<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, zsrc,
-                                static_cast<const S*>(p_old), static_cast<S*>(p_new), pb.st, pb.partials, par,
-                                pb.pre == kPreFactored ? pb.grid_rows : pb.grid_units));
-        if (pb.dia.active)
+        const int count_res = pb.pre == kPreFactored ? pb.grid_rows : pb.grid_units;
+        const int nd = pb.dia.args.ndiag;
+        if (pb.dia.active && pb.fuse_direction)
         {
-            const int nd = pb.dia.args.ndiag;
+            const auto fused = nd == 7 ? k_dir_spmv_dia<S, 7> : nd == 5 ? k_dir_spmv_dia<S, 5> : k_dir_spmv_dia<S, 0>;
+            out.push_back(make_call(fused, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, pb.dia.args,
+                                    static_cast<const S*>(pb.dia.values), pb.dia.mask, zsrc,
+                                    static_cast<const S*>(p_old), static_cast<S*>(p_new), static_cast<S*>(pb.q),
+                                    pb.st, pb.partials, par, count_res));
+        }
+        else
+        {
+            out.push_back(make_call(k_direction<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, zsrc,
+                                    static_cast<const S*>(p_old), static_cast<S*>(p_new), pb.st, pb.partials, par,
+                                    count_res));
+        }
+        if (pb.dia.active && pb.fuse_direction)
+        {
+            // q and p.q came with the direction
+        }
+        else if (pb.dia.active)
+        {
             const auto dia_den = nd == 7 ? k_spmv_den_dia<S, 7> : nd == 5 ? k_spmv_den_dia<S, 5> : k_spmv_den_dia<S, 0>;
             out.push_back(make_call(dia_den, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, pb.dia.args,
                                     static_cast<const S*>(pb.dia.values), pb.dia.mask, static_cast<const S*>(p_new),
@@ -709,6 +793,11 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     // cache-resident iteration (measured: 53.3 vs 51.1 us per c1 iteration)
     const char* dma = std::getenv("PLAMATRIX_AMD_SOLVER_DMA");
     pb.dma = dma != nullptr && dma[0] == '1';
+    // fusing the direction into the diagonal-layout SpMV saves a pass over z
+    // and p_old but measured slower (41.7 vs 40.6 us per c1 iteration): off
+    // unless PLAMATRIX_AMD_FUSE_DIRECTION=1
+    const char* fuse = std::getenv("PLAMATRIX_AMD_FUSE_DIRECTION");
+    pb.fuse_direction = fuse != nullptr && fuse[0] == '1';
     return PM_OK;
 }
 
@@ -739,7 +828,8 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
                   (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
                   static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
-                  static_cast<long long>(pb.bsize), (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0), pb.gf ? pb.gf->values : nullptr,
+                  static_cast<long long>(pb.bsize), (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0),
+                  pb.gf ? pb.gf->values : nullptr,
                   pb.gf ? (const void*)pb.gf->row_offsets32 : nullptr, pb.gtf ? pb.gtf->values : nullptr,
                   pb.gtf ? (const void*)pb.gtf->row_offsets32 : nullptr, pb.dia.active ? 1 : 0, pb.dia.args.ndiag,
                   dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask));
@@ -829,6 +919,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     pb.pre = pre;
     PM_TRY(prepare(ws, pb, stream));
     if (dia_wanted(a)) PM_TRY(dia_build(ws, a, &pb.dia, stream));
+    ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
     if (a->rows == 0)
     {
         pm_solver_report rep{};

@@ -62,8 +62,11 @@ def solve_both(dev, h, dtype, monkeypatch, rel, refresh=50):
     return out, grow
 
 
+@pytest.mark.parametrize("fuse", ["0", "1"])
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_diffusion3d_dia_is_bitwise_csr(dev, monkeypatch, dtype):
+def test_diffusion3d_dia_is_bitwise_csr(dev, monkeypatch, dtype, fuse):
+    # fuse=1: the direction p = z + beta p formed inside the diagonal SpMV
+    monkeypatch.setenv("PLAMATRIX_AMD_FUSE_DIRECTION", fuse)
     h = generators.diffusion3d_csr(48, seed=7, dtype=dtype)  # 110,592 rows, 7 diagonals
     out, grow = solve_both(dev, h, dtype, monkeypatch, 1e-6 if dtype == np.float32 else 1e-10)
     es = np.dtype(dtype).itemsize
