@@ -194,7 +194,7 @@ int pm_cg(const pm_csr* a, const void* rhs, void* x, const pm_solver_options* op
 int pm_pcg(const pm_csr* a, const void* rhs, void* x, const pm_solver_options* options,
            const pm_solve_policy* policy, pm_solver_report* report, pm_workspace* ws, hipStream_t stream);
 /* inverse_blocks: rows/block_size row-major block_size^2 blocks (rows*block_size
- * entries), block_size <= 8. */
+ * entries), block_size <= 16. */
 int pm_block_pcg(const pm_csr* a, const void* rhs, void* x, const void* inverse_blocks, int64_t block_size,
                  const pm_solver_options* options, const pm_solve_policy* policy, pm_solver_report* report,
                  pm_workspace* ws, hipStream_t stream);
@@ -332,7 +332,7 @@ int pm_fsai_train_step(pm_fsai_net* net, pm_fsai_graph* graph, int32_t probes, u
  * (block_schur.h:197-319): B (pc blocks ps x ps), C (ec blocks es x es),
  * gradients, cross blocks W (ps x es, row-major, creation order) with their
  * (primary, eliminated) ids, and primary cross blocks (row < col).  Block
- * sizes up to 8. */
+ * sizes up to 16. */
 typedef struct pm_normal_equations {
     int64_t primary_count;
     int64_t eliminated_count;

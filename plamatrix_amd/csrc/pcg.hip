@@ -52,7 +52,7 @@ enum Mode
     kModeRefresh = 2,
 };
 
-constexpr int kMaxBlock = 8;
+constexpr int kMaxBlock = 16;
 constexpr int kDirBlock = kBlock;               // threads of the direction / SpMV kernel
 constexpr int kDirWaves = kDirBlock / kWave;
 constexpr int kDirGrid = kMaxPartials;          // 2048 x 4 waves: 8 workgroups per CU, chunks dealt round-robin

@@ -50,7 +50,8 @@
 namespace pm
 {
 
-constexpr int kMaxSchurBlock = 8;  // ps and es limit (ps*ps lanes per slot)
+constexpr int kMaxSchurBlock = 16;  // ps and es limit
+constexpr int kSmallSchurBlock = 8;  // kernels with per-thread block arrays use this size when it suffices
 
 struct SchurStatus
 {
@@ -98,11 +99,12 @@ __device__ double huber_weight(double s, double delta)
 
 // invertPositiveDefiniteInto (block_schur_linear_algebra.h:92-147): Cholesky
 // factor, then one forward and one backward substitution per column.
+template <int MB>
 __device__ bool invert_pd(const double* m, int n, double* inv)
 {
 #pragma clang fp contract(off)
-    double lower[kMaxSchurBlock * kMaxSchurBlock];
-    double mid[kMaxSchurBlock];
+    double lower[MB * MB];
+    double mid[MB];
     for (int i = 0; i < n * n; ++i) lower[i] = 0.0;
     for (int row = 0; row < n; ++row)
         for (int col = 0; col <= row; ++col)
@@ -742,6 +744,7 @@ __global__ __launch_bounds__(kBlock) void k_damp(const double* __restrict__ src,
 }
 
 // C^-1 per eliminated block and y_e = C^-1 g_e.
+template <int MB>
 __global__ __launch_bounds__(kBlock) void k_eliminated_inverse(int64_t ec, int es, const double* __restrict__ Cd,
                                                                const double* __restrict__ ge,
                                                                double* __restrict__ Cinv, double* __restrict__ y,
@@ -750,9 +753,9 @@ __global__ __launch_bounds__(kBlock) void k_eliminated_inverse(int64_t ec, int e
 #pragma clang fp contract(off)
     for (int64_t e = gtid(); e < ec; e += gstride())
     {
-        double m[kMaxSchurBlock * kMaxSchurBlock], inv[kMaxSchurBlock * kMaxSchurBlock];
+        double m[MB * MB], inv[MB * MB];
         for (int i = 0; i < es * es; ++i) m[i] = Cd[e * es * es + i];
-        if (!invert_pd(m, es, inv))
+        if (!invert_pd<MB>(m, es, inv))
         {
             atomicMin(&st->bad_inverse, static_cast<unsigned long long>(e));
             for (int i = 0; i < es * es; ++i) inv[i] = 0.0;
@@ -865,8 +868,6 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     const unsigned long long mask = (1ull << pb) - 1ull;
-    const bool active = lane < ps * ps;
-    const int lr = active ? lane / ps : 0, lc = active ? lane % ps : 0;
     // each workgroup owns a contiguous, XCD-local range of slots: neighbouring
     // slots share their W C^-1 and W blocks, which then stay in one L2
     const int64_t wpb = blockDim.x / kWave;
@@ -879,6 +880,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
         const int64_t col = static_cast<int64_t>(slot_key[s] & mask);
         const int64_t begin = slot_start[s];
         const int64_t end = s + 1 < slots ? slot_start[s + 1] : entries;
+        // blocks above 8 x 8 have more entries than lanes: 64 at a time
+        for (int e0 = 0; e0 < ps * ps; e0 += kWave)
+        {
+        const int ent = e0 + lane;
+        const bool active = ent < ps * ps;
+        const int lr = active ? ent / ps : 0, lc = active ? ent % ps : 0;
         double acc = 0.0;
         // the run holds terms first, then the diagonal marker, then primary
         // cross markers in index order
@@ -958,10 +965,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
             }
         }
         if (active) values[ro[row * ps + lr] + (s - bro[row]) * ps + lc] = acc;
+        }
     }
 }
 
 // Preconditioner: inverse of each assembled diagonal block.
+template <int MB>
 __global__ __launch_bounds__(kBlock) void k_preconditioner(int64_t pc, int ps, const int64_t* __restrict__ diag_slot,
                                                            const int64_t* __restrict__ bro,
                                                            const int64_t* __restrict__ ro,
@@ -970,11 +979,11 @@ __global__ __launch_bounds__(kBlock) void k_preconditioner(int64_t pc, int ps, c
 {
     for (int64_t b = gtid(); b < pc; b += gstride())
     {
-        double m[kMaxSchurBlock * kMaxSchurBlock], inv[kMaxSchurBlock * kMaxSchurBlock];
+        double m[MB * MB], inv[MB * MB];
         const int64_t s = diag_slot[b];
         for (int lr = 0; lr < ps; ++lr)
             for (int lc = 0; lc < ps; ++lc) m[lr * ps + lc] = values[ro[b * ps + lr] + (s - bro[b]) * ps + lc];
-        if (!invert_pd(m, ps, inv))
+        if (!invert_pd<MB>(m, ps, inv))
         {
             atomicMin(&st->bad_preconditioner, static_cast<unsigned long long>(b));
             for (int i = 0; i < ps * ps; ++i) inv[i] = 0.0;
@@ -1003,6 +1012,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_slots(int64_t pc, int pb, int64
 }
 
 // Back-substitution (block_schur.cpp:422-454): de_e = C^-1 (-g_e - sum W^T dp).
+template <int MB>
 __global__ __launch_bounds__(kBlock) void k_back_substitute(int64_t ec, int ps, int es,
                                                             const int64_t* __restrict__ adj_offs,
                                                             const unsigned* __restrict__ adj,
@@ -1015,7 +1025,7 @@ __global__ __launch_bounds__(kBlock) void k_back_substitute(int64_t ec, int ps, 
 #pragma clang fp contract(off)
     for (int64_t e = gtid(); e < ec; e += gstride())
     {
-        double point[kMaxSchurBlock];
+        double point[MB];
         for (int i = 0; i < es; ++i) point[i] = -ge[e * es + i];
         const int64_t k0 = adj_offs ? adj_offs[e] : 0;
         const int64_t k1 = adj_offs ? adj_offs[e + 1] : 0;
@@ -1659,7 +1669,8 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     {
         hipLaunchKernelGGL(k_damp, dim3(grid_for(ec * es * es)), dim3(kBlock), 0, stream, eq->eliminated_diagonal, ec,
                            es, damping, minimum_diagonal, w->Cd);
-        hipLaunchKernelGGL(k_eliminated_inverse, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, es, w->Cd,
+        const auto inverse_kernel = es <= kSmallSchurBlock ? k_eliminated_inverse<kSmallSchurBlock> : k_eliminated_inverse<kMaxSchurBlock>;
+        hipLaunchKernelGGL(inverse_kernel, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, es, w->Cd,
                            eq->eliminated_gradient, w->Cinv, w->y, st);
     }
     if (pd > 0)
@@ -1680,7 +1691,8 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
                            stream, w->slots, ps, es, w->pb, w->slot_key, w->slot_start, w->entries, w->term_e,
                            w->term_left, w->term_right, w->bro, w->ro, w->Bd, eq->primary_cross_values, w->TW,
                            eq->cross_values, w->values);
-        hipLaunchKernelGGL(k_preconditioner, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, ps, w->diag_slot, w->bro,
+        const auto preconditioner_kernel = ps <= kSmallSchurBlock ? k_preconditioner<kSmallSchurBlock> : k_preconditioner<kMaxSchurBlock>;
+        hipLaunchKernelGGL(preconditioner_kernel, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, ps, w->diag_slot, w->bro,
                            w->ro, w->values, w->pinv, st);
     }
     PM_HIP(hipGetLastError());
@@ -1922,9 +1934,13 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         return PM_OK;
     }
     if (ec > 0)
-        hipLaunchKernelGGL(k_back_substitute, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, ps, es,
+    {
+        const auto back_kernel =
+            es <= kSmallSchurBlock ? k_back_substitute<kSmallSchurBlock> : k_back_substitute<kMaxSchurBlock>;
+        hipLaunchKernelGGL(back_kernel, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, ps, es,
                            pd > 0 ? w->adj_offs : nullptr, w->adj, eq->cross_primary, eq->cross_values, w->Cinv,
                            eq->eliminated_gradient, dp, de);
+    }
     PM_HIP(hipGetLastError());
     std::snprintf(report->message, sizeof(report->message), "converged");
     return PM_OK;

@@ -931,7 +931,7 @@ static void denseSchur()
         for (double v : p1) EXPECT(std::isfinite(v));
     }
     {
-        constexpr Index pc = 36, ec = 240, ps = 8, es = 3;
+        constexpr Index pc = 36, ec = 240, ps = 9, es = 3;
         BlockNormalEquations<double> eq(pc, ec, ps, es);
         double identity[ps * ps] = {};
         for (Index d = 0; d < ps; ++d) identity[d * ps + d] = 1.0;

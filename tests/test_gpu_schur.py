@@ -333,3 +333,37 @@ def test_full_size_lm_system(dev, sw, ws):
     assert values == slots * 36 and S.nnz == values
     # symmetry of the assembled reduced system
     assert abs(S - S.T).max() <= 1e-9 * abs(S).max()
+
+
+@pytest.mark.parametrize("ps,es", [(9, 3), (12, 10), (16, 16)])
+def test_large_blocks_match_oracle_and_lapack(dev, sw, ws, oracle, ps, es):
+    # blocks above 8 x 8 (9-parameter cameras; the reference test at
+    # block_schur_test.cpp:341-405 uses 9 x 3): slot entries are assembled 64
+    # at a time; both device backends against the oracle's CPU solve and the
+    # dense solution of the assembled reduced system
+    rng = np.random.default_rng(ps * 100 + es)
+    pc, ec = 10, 40
+    eq = oracle.NormalEquations(pc, ec, ps, es)
+    for p in range(pc):
+        eq.add_primary_residual_block(p, np.eye(ps).ravel(), np.zeros(ps), 0.25)
+    for e in range(ec):
+        for obs in range(4):
+            p = (e * 7 + obs * 5) % pc
+            eq.add_residual_block(p, e, rng.standard_normal(2 * ps) * 0.1, rng.standard_normal(2 * es) * 0.2,
+                                  rng.standard_normal(2) * 0.01, 1.0)
+    ne = capi.upload_normal_equations(pc, ec, ps, es, eq.export(), device=dev)
+    rdp, rde, rrep = eq.solve(0.1, max_iterations=500, relative_tolerance=1e-12, absolute_tolerance=0.0)
+    assert rrep.converged
+    ref = np.concatenate([rdp, rde])
+    for dense in (False, True):
+        dp = torch.zeros(pc * ps, dtype=torch.float64, device=dev)
+        de = torch.zeros(ec * es, dtype=torch.float64, device=dev)
+        rep = capi.schur_solve(sw, ne, 0.1, dp, de, ws, max_iterations=500, relative_tolerance=1e-12,
+                               absolute_tolerance=0.0, dense=dense)
+        assert rep.converged, rep.message
+        got = np.concatenate([dp.cpu().numpy(), de.cpu().numpy()])
+        assert np.linalg.norm(got - ref) <= 1e-8 * np.linalg.norm(ref)
+        relres, S, rhs = reduced_relres(sw, dp)
+        assert relres <= 1e-11
+        x = np.linalg.solve(S.toarray(), rhs)
+        assert np.linalg.norm(dp.cpu().numpy() - x) <= 1e-9 * np.linalg.norm(x)
