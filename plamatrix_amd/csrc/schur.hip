@@ -99,17 +99,24 @@ __device__ double huber_weight(double s, double delta)
 
 // invertPositiveDefiniteInto (block_schur_linear_algebra.h:92-147): Cholesky
 // factor, then one forward and one backward substitution per column.
-template <int MB>
-__device__ bool invert_pd(const double* m, int n, double* inv)
+// N > 0 fixes the size at compile time: the loops unroll and the arrays stay
+// in registers instead of per-lane scratch memory.
+template <int MB, int N = 0>
+__device__ bool invert_pd(const double* m, int n_rt, double* inv)
 {
 #pragma clang fp contract(off)
+    const int n = N > 0 ? N : n_rt;
     double lower[MB * MB];
     double mid[MB];
+#pragma unroll
     for (int i = 0; i < n * n; ++i) lower[i] = 0.0;
+#pragma unroll
     for (int row = 0; row < n; ++row)
+#pragma unroll
         for (int col = 0; col <= row; ++col)
         {
             double value = m[row * n + col];
+#pragma unroll
             for (int k = 0; k < col; ++k) value -= lower[row * n + k] * lower[col * n + k];
             if (row == col)
             {
@@ -121,19 +128,25 @@ __device__ bool invert_pd(const double* m, int n, double* inv)
                 lower[row * n + col] = value / lower[col * n + col];
             }
         }
+#pragma unroll
     for (int i = 0; i < n * n; ++i) inv[i] = 0.0;
+#pragma unroll
     for (int col = 0; col < n; ++col)
     {
+#pragma unroll
         for (int row = 0; row < n; ++row)
         {
             double value = row == col ? 1.0 : 0.0;
+#pragma unroll
             for (int k = 0; k < row; ++k) value -= lower[row * n + k] * mid[k];
             mid[row] = value / lower[row * n + row];
         }
+#pragma unroll
         for (int off = 0; off < n; ++off)
         {
             const int row = n - 1 - off;
             double value = mid[row];
+#pragma unroll
             for (int k = row + 1; k < n; ++k) value -= lower[k * n + row] * inv[k * n + col];
             inv[row * n + col] = value / lower[row * n + row];
         }
@@ -744,34 +757,43 @@ __global__ __launch_bounds__(kBlock) void k_damp(const double* __restrict__ src,
 }
 
 // C^-1 per eliminated block and y_e = C^-1 g_e.
-template <int MB>
-__global__ __launch_bounds__(kBlock) void k_eliminated_inverse(int64_t ec, int es, const double* __restrict__ Cd,
+template <int MB, int N = 0>
+__global__ __launch_bounds__(kBlock) void k_eliminated_inverse(int64_t ec, int es_rt, const double* __restrict__ Cd,
                                                                const double* __restrict__ ge,
                                                                double* __restrict__ Cinv, double* __restrict__ y,
                                                                SchurStatus* st)
 {
 #pragma clang fp contract(off)
+    const int es = N > 0 ? N : es_rt;
     for (int64_t e = gtid(); e < ec; e += gstride())
     {
         double m[MB * MB], inv[MB * MB];
+#pragma unroll
         for (int i = 0; i < es * es; ++i) m[i] = Cd[e * es * es + i];
-        if (!invert_pd<MB>(m, es, inv))
+        if (!invert_pd<MB, N>(m, es, inv))
         {
             atomicMin(&st->bad_inverse, static_cast<unsigned long long>(e));
+#pragma unroll
             for (int i = 0; i < es * es; ++i) inv[i] = 0.0;
         }
+#pragma unroll
         for (int i = 0; i < es * es; ++i) Cinv[e * es * es + i] = inv[i];
+#pragma unroll
         for (int row = 0; row < es; ++row)
         {
             double value = 0.0;
+#pragma unroll
             for (int col = 0; col < es; ++col) value += inv[row * es + col] * ge[e * es + col];
             y[e * es + row] = value;
         }
     }
 }
 
-// Per cross block c and row: v_c[row] = W_c[row] . y_e (one thread each).
+// Per cross block c and row: v_c[row] = W_c[row] . y_e (one thread each),
+// stored in the primaries' summation order (position k of prim_cross), so
+// that k_reduced_rhs streams each primary's products contiguously.
 __global__ __launch_bounds__(kBlock) void k_cross_rhs(int64_t nc, int ps, int es,
+                                                      const unsigned* __restrict__ prim_cross,
                                                       const int64_t* __restrict__ cross_eliminated,
                                                       const double* __restrict__ W, const double* __restrict__ y,
                                                       double* __restrict__ crv)
@@ -779,7 +801,7 @@ __global__ __launch_bounds__(kBlock) void k_cross_rhs(int64_t nc, int ps, int es
 #pragma clang fp contract(off)
     for (int64_t t = gtid(); t < nc * ps; t += gstride())
     {
-        const int64_t c = t / ps;
+        const int64_t c = prim_cross[t / ps];
         const int row = static_cast<int>(t % ps);
         const double* m = W + c * ps * es + row * es;
         const double* v = y + cross_eliminated[c] * es;
@@ -792,10 +814,9 @@ __global__ __launch_bounds__(kBlock) void k_cross_rhs(int64_t nc, int ps, int es
 // rhs_p = -g_p + sum over the primary's cross blocks in (primary, eliminated)
 // order of W C^-1 g_e (block_schur.cpp:158-181), the products precomputed by
 // k_cross_rhs; the ordered sum loads kRhsBatch products ahead of the adds.
-constexpr int kRhsBatch = 16;
+constexpr int kRhsBatch = 64;  // one thread per rhs entry (pc * ps of them): deep prefetch for the few chains
 __global__ __launch_bounds__(kBlock) void k_reduced_rhs(int64_t pc, int ps, const double* __restrict__ gp,
                                                         const int64_t* __restrict__ prim_offs,
-                                                        const unsigned* __restrict__ prim_cross,
                                                         const double* __restrict__ crv, double* __restrict__ rhs)
 {
 #pragma clang fp contract(off)
@@ -810,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void k_reduced_rhs(int64_t pc, int ps, cons
             double v[kRhsBatch];
 #pragma unroll
             for (int u = 0; u < kRhsBatch; ++u)
-                v[u] = k + u < k1 ? crv[static_cast<int64_t>(prim_cross[k + u]) * ps + row] : 0.0;
+                v[u] = k + u < k1 ? crv[(k + u) * ps + row] : 0.0;
 #pragma unroll
             for (int u = 0; u < kRhsBatch; ++u)
                 if (k + u < k1) acc += 1.0 * v[u];
@@ -970,24 +991,29 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
 }
 
 // Preconditioner: inverse of each assembled diagonal block.
-template <int MB>
-__global__ __launch_bounds__(kBlock) void k_preconditioner(int64_t pc, int ps, const int64_t* __restrict__ diag_slot,
+template <int MB, int N = 0>
+__global__ __launch_bounds__(kBlock) void k_preconditioner(int64_t pc, int ps_rt, const int64_t* __restrict__ diag_slot,
                                                            const int64_t* __restrict__ bro,
                                                            const int64_t* __restrict__ ro,
                                                            const double* __restrict__ values,
                                                            double* __restrict__ pinv, SchurStatus* st)
 {
+    const int ps = N > 0 ? N : ps_rt;
     for (int64_t b = gtid(); b < pc; b += gstride())
     {
         double m[MB * MB], inv[MB * MB];
         const int64_t s = diag_slot[b];
+#pragma unroll
         for (int lr = 0; lr < ps; ++lr)
+#pragma unroll
             for (int lc = 0; lc < ps; ++lc) m[lr * ps + lc] = values[ro[b * ps + lr] + (s - bro[b]) * ps + lc];
-        if (!invert_pd<MB>(m, ps, inv))
+        if (!invert_pd<MB, N>(m, ps, inv))
         {
             atomicMin(&st->bad_preconditioner, static_cast<unsigned long long>(b));
+#pragma unroll
             for (int i = 0; i < ps * ps; ++i) inv[i] = 0.0;
         }
+#pragma unroll
         for (int i = 0; i < ps * ps; ++i) pinv[b * ps * ps + i] = inv[i];
     }
 }
@@ -1008,6 +1034,64 @@ __global__ __launch_bounds__(kBlock) void k_diag_slots(int64_t pc, int pb, int64
             else hi = mid;
         }
         diag_slot[b] = lo;
+    }
+}
+
+// Back-substitution, first half: t_c[col] = sum_row W_c[row][col] dp_p[row]
+// for every cross block c (a thread per (c, col): W is read in order).
+__global__ __launch_bounds__(kBlock) void k_cross_point(int64_t nc, int ps, int es,
+                                                        const int64_t* __restrict__ cross_primary,
+                                                        const double* __restrict__ W, const double* __restrict__ dp,
+                                                        double* __restrict__ t)
+{
+#pragma clang fp contract(off)
+    for (int64_t i = gtid(); i < nc * es; i += gstride())
+    {
+        const int64_t c = i / es;
+        const int column = static_cast<int>(i % es);
+        const double* values = W + c * ps * es + column;
+        const double* primary = dp + cross_primary[c] * ps;
+        double value = 0.0;
+        for (int row = 0; row < ps; ++row) value += values[row * es] * primary[row];
+        t[i] = value;
+    }
+}
+
+// Second half: point = -g_e - t_c over the eliminated block's crosses in
+// adjacency order, de_e = C^-1 point.  Together the arithmetic of
+// k_back_substitute, operation for operation.
+template <int MB, int N = 0>
+__global__ __launch_bounds__(kBlock) void k_back_substitute_point(int64_t ec, int es_rt,
+                                                                  const int64_t* __restrict__ adj_offs,
+                                                                  const unsigned* __restrict__ adj,
+                                                                  const double* __restrict__ t,
+                                                                  const double* __restrict__ Cinv,
+                                                                  const double* __restrict__ ge,
+                                                                  double* __restrict__ de)
+{
+#pragma clang fp contract(off)
+    const int es = N > 0 ? N : es_rt;
+    for (int64_t e = gtid(); e < ec; e += gstride())
+    {
+        double point[MB];
+#pragma unroll
+        for (int i = 0; i < es; ++i) point[i] = -ge[e * es + i];
+        const int64_t k1 = adj_offs[e + 1];
+        for (int64_t k = adj_offs[e]; k < k1; ++k)
+        {
+            const double* tc = t + static_cast<int64_t>(adj[k]) * es;
+#pragma unroll
+            for (int column = 0; column < es; ++column) point[column] -= tc[column];
+        }
+        const double* inv = Cinv + e * es * es;
+#pragma unroll
+        for (int row = 0; row < es; ++row)
+        {
+            double value = 0.0;
+#pragma unroll
+            for (int col = 0; col < es; ++col) value += inv[row * es + col] * point[col];
+            de[e * es + row] = value;
+        }
     }
 }
 
@@ -1161,7 +1245,7 @@ struct pm_schur_workspace
     double* TW = nullptr;
     double* values = nullptr;
     double* pinv = nullptr;
-    double* crv = nullptr;  // per (cross block, row) products of the reduced rhs
+    double* crv = nullptr;  // per (cross block, row) products of the reduced rhs, then of the back-substitution
 
     int bind(hipStream_t s)
     {
@@ -1647,7 +1731,7 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     nz.take<double>(nc * ps * es);
     nz.take<double>(pd > 0 ? w->values_count : 0);
     nz.take<double>(pc * ps * ps);
-    nz.take<double>(nc * ps);
+    nz.take<double>(nc * (ps > es ? ps : es));
     PM_TRY(w->numerics.reserve(nz.off));
     Carve n;
     n.base = w->numerics.as<char>();
@@ -1659,7 +1743,7 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     w->TW = n.take<double>(nc * ps * es);
     w->values = n.take<double>(pd > 0 ? w->values_count : 0);
     w->pinv = n.take<double>(pc * ps * ps);
-    w->crv = n.take<double>(nc * ps);
+    w->crv = n.take<double>(nc * (ps > es ? ps : es));
 
     hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
     if (pc > 0)
@@ -1669,7 +1753,9 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     {
         hipLaunchKernelGGL(k_damp, dim3(grid_for(ec * es * es)), dim3(kBlock), 0, stream, eq->eliminated_diagonal, ec,
                            es, damping, minimum_diagonal, w->Cd);
-        const auto inverse_kernel = es <= kSmallSchurBlock ? k_eliminated_inverse<kSmallSchurBlock> : k_eliminated_inverse<kMaxSchurBlock>;
+        const auto inverse_kernel = es == 3                  ? k_eliminated_inverse<3, 3>
+                                    : es <= kSmallSchurBlock ? k_eliminated_inverse<kSmallSchurBlock>
+                                                             : k_eliminated_inverse<kMaxSchurBlock>;
         hipLaunchKernelGGL(inverse_kernel, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, es, w->Cd,
                            eq->eliminated_gradient, w->Cinv, w->y, st);
     }
@@ -1677,9 +1763,9 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     {
         if (nc > 0)
             hipLaunchKernelGGL(k_cross_rhs, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
-                               eq->cross_eliminated, eq->cross_values, w->y, w->crv);
-        hipLaunchKernelGGL(k_reduced_rhs, dim3(grid_for(pd)), dim3(kBlock), 0, stream, pc, ps, eq->primary_gradient,
-                           w->prim_offs, w->prim_cross, static_cast<const double*>(w->crv), w->rhs);
+                               w->prim_cross, eq->cross_eliminated, eq->cross_values, w->y, w->crv);
+        hipLaunchKernelGGL(k_reduced_rhs, dim3(static_cast<unsigned>(ceil_div(pd, kWave))), dim3(kWave), 0, stream, pc,
+                           ps, eq->primary_gradient, w->prim_offs, static_cast<const double*>(w->crv), w->rhs);
         if (nc > 0)
             hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
                                eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
@@ -1691,7 +1777,9 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
                            stream, w->slots, ps, es, w->pb, w->slot_key, w->slot_start, w->entries, w->term_e,
                            w->term_left, w->term_right, w->bro, w->ro, w->Bd, eq->primary_cross_values, w->TW,
                            eq->cross_values, w->values);
-        const auto preconditioner_kernel = ps <= kSmallSchurBlock ? k_preconditioner<kSmallSchurBlock> : k_preconditioner<kMaxSchurBlock>;
+        const auto preconditioner_kernel = ps == 6                  ? k_preconditioner<6, 6>
+                                           : ps <= kSmallSchurBlock ? k_preconditioner<kSmallSchurBlock>
+                                                                    : k_preconditioner<kMaxSchurBlock>;
         hipLaunchKernelGGL(preconditioner_kernel, dim3(grid_for(pc)), dim3(kBlock), 0, stream, pc, ps, w->diag_slot, w->bro,
                            w->ro, w->values, w->pinv, st);
     }
@@ -1933,7 +2021,18 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         std::snprintf(report->message, sizeof(report->message), "PCG iteration limit reached");
         return PM_OK;
     }
-    if (ec > 0)
+    if (ec > 0 && pd > 0 && eq->cross_count > 0)
+    {
+        const int64_t nc = eq->cross_count;
+        hipLaunchKernelGGL(k_cross_point, dim3(grid_for(nc * es)), dim3(kBlock), 0, stream, nc, ps, es,
+                           eq->cross_primary, eq->cross_values, dp, w->crv);
+        const auto point_kernel = es == 3                  ? k_back_substitute_point<3, 3>
+                                  : es <= kSmallSchurBlock ? k_back_substitute_point<kSmallSchurBlock>
+                                                           : k_back_substitute_point<kMaxSchurBlock>;
+        hipLaunchKernelGGL(point_kernel, dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec, es, w->adj_offs, w->adj,
+                           static_cast<const double*>(w->crv), w->Cinv, eq->eliminated_gradient, de);
+    }
+    else if (ec > 0)
     {
         const auto back_kernel =
             es <= kSmallSchurBlock ? k_back_substitute<kSmallSchurBlock> : k_back_substitute<kMaxSchurBlock>;
