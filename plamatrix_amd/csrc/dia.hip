@@ -7,7 +7,6 @@
 #include <cstring>
 
 #include "dia.hpp"
-#include "workspace.hpp"
 
 namespace pm
 {
@@ -101,15 +100,27 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(int64_t rows, const int32_t
 
 } // namespace
 
-int dia_build(pm_workspace* ws, const pm_csr* a, DiaLayout* out, hipStream_t stream)
+unsigned long long dia_signature(const DiaLayout& d)
+{
+    unsigned long long sig = 1469598103934665603ull;  // FNV-1a
+    auto mix = [&](unsigned long long v) { sig = (sig ^ v) * 1099511628211ull; };
+    mix(d.active ? 1 : 0);
+    if (!d.active) return sig;
+    for (int i = 0; i < d.args.ndiag; ++i) mix(static_cast<unsigned>(d.args.off[i]));
+    mix(reinterpret_cast<unsigned long long>(d.values));
+    mix(reinterpret_cast<unsigned long long>(d.mask));
+    return sig;
+}
+
+int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream)
 {
     *out = DiaLayout{};
     const int64_t n = a->rows;
     if (n <= 0 || a->nnz <= 0 || !a->row_offsets32 || !a->col_indices32 || a->rows != a->cols) return PM_OK;
     const size_t es = a->dtype == PM_F64 ? 8 : 4;
     const size_t table_bytes = 256;
-    PM_TRY(ws->dia.reserve(table_bytes));
-    int* table = ws->dia.as<int>();
+    PM_TRY(storage.reserve(table_bytes));
+    int* table = storage.as<int>();
     hipLaunchKernelGGL(k_dia_reset, dim3(1), dim3(64), 0, stream, table);
     const int grid = static_cast<int>(std::min<int64_t>(ceil_div(n, kBlock), 2048));
     hipLaunchKernelGGL(k_dia_detect, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32, a->col_indices32,
@@ -128,8 +139,8 @@ int dia_build(pm_workspace* ws, const pm_csr* a, DiaLayout* out, hipStream_t str
         return PM_OK;
     const size_t values_bytes = (es * static_cast<size_t>(args.ndiag) * static_cast<size_t>(n) + 255) & ~size_t(255);
     const size_t mask_bytes = (sizeof(uint16_t) * static_cast<size_t>(n) + 255) & ~size_t(255);
-    PM_TRY(ws->dia.reserve(table_bytes + values_bytes + mask_bytes));
-    char* base = ws->dia.as<char>();
+    PM_TRY(storage.reserve(table_bytes + values_bytes + mask_bytes));
+    char* base = storage.as<char>();
     void* dv = base + table_bytes;
     uint16_t* mask = reinterpret_cast<uint16_t*>(base + table_bytes + values_bytes);
     if (a->dtype == PM_F64)

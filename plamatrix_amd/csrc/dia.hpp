@@ -15,6 +15,7 @@
 #include <cstdint>
 
 #include "pm_common.hpp"
+#include "spmv_kernels.hpp"
 
 struct pm_workspace;
 
@@ -37,12 +38,44 @@ struct DiaLayout
     const uint16_t* mask = nullptr; // bit d of mask[row]: entry on diagonal d present
 };
 
-// Builds the layout into the workspace when the square CSR `a` (int32
+// Builds the layout into `storage` when the square CSR `a` (int32
 // mirrors present) has at most kMaxDia distinct diagonals, strictly
 // increasing columns in every row, and the diagonals' slots are at least
 // 1/kDiaMaxFill full; otherwise out->active stays false.  Synchronises
 // `stream` once to read the detected diagonals.
 constexpr double kDiaMaxFill = 1.3;
-int dia_build(pm_workspace* ws, const pm_csr* a, DiaLayout* out, hipStream_t stream);
+int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream);
+
+// A 64-bit signature of a layout (diagonals and storage), for plan keys.
+unsigned long long dia_signature(const DiaLayout& d);
+
+// Row sum of the diagonal layout: the row's present entries in increasing
+// column order, acc = fma(value, x[col], acc) from 0 -- the CSR row walk's
+// arithmetic.  Every load is issued before the mask is known (absent slots
+// read an in-range value and a clamped gather that the mask discards).
+template <typename S, int ND>
+__device__ __forceinline__ S dia_row(int64_t row, int64_t rows, const DiaArgs& dia, const S* __restrict__ dv,
+                                     unsigned m, const S* __restrict__ x)
+{
+    constexpr int kSlots = ND > 0 ? ND : kMaxDia;
+    const int nd = ND > 0 ? ND : dia.ndiag;
+    S vals[kSlots], xs[kSlots];
+#pragma unroll
+    for (int d = 0; d < kSlots; ++d)
+    {
+        if (d < nd)
+        {
+            int64_t col = row + dia.off[d];
+            col = col < 0 ? 0 : (col >= rows ? rows - 1 : col);
+            vals[d] = dv[d * rows + row];
+            xs[d] = x[col];
+        }
+    }
+    S acc = S(0);
+#pragma unroll
+    for (int d = 0; d < kSlots; ++d)
+        if (d < nd && ((m >> d) & 1u)) acc = fma_s(vals[d], xs[d], acc);
+    return acc;
+}
 
 } // namespace pm

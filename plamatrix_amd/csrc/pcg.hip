@@ -544,6 +544,49 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper(int64_t rows, const I* 
     publish_partials<2>(part, partials + kPartialsRes);
 }
 
+// k_factor_lower / k_factor_upper on the diagonal layouts of G and G^T (same
+// chunks per workgroup, same row arithmetic, so the same bits).
+template <typename S, int ND>
+__global__ __launch_bounds__(kBlock) void k_factor_lower_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                             const uint16_t* __restrict__ mask,
+                                                             const S* __restrict__ r, S* __restrict__ t,
+                                                             const SolverState* st)
+{
+    if (st->done != kRunning) return;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
+    {
+        const int64_t row = c * kChunkRows + lane;
+        if (row < rows) t[row] = dia_row<S, ND>(row, rows, dia, dv, mask[row], r);
+    }
+}
+
+template <typename S, int ND>
+__global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                             const uint16_t* __restrict__ mask,
+                                                             const S* __restrict__ t, const S* __restrict__ r,
+                                                             S* __restrict__ z, const SolverState* st,
+                                                             double* partials)
+{
+    if (st->done != kRunning) return;
+    const int lane = threadIdx.x % kWave;
+    double rr = 0.0, rz = 0.0;
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
+    {
+        const int64_t row = c * kChunkRows + lane;
+        if (row < rows)
+        {
+            const S acc = dia_row<S, ND>(row, rows, dia, dv, mask[row], t);
+            z[row] = acc;
+            const double rv = static_cast<double>(r[row]);
+            rr = __builtin_fma(rv, rv, rr);
+            rz = __builtin_fma(rv, static_cast<double>(acc), rz);
+        }
+    }
+    double part[2] = {rr, rz};
+    publish_partials<2>(part, partials + kPartialsRes);
+}
+
 // Squared residuals of a fixed-iteration solve (toDoubleKernel /
 // toDoubleCheckedKernel, iterative_solver_kernels.cuh:240-252).
 __global__ __launch_bounds__(kBlock) void k_fixed_out(const SolverState* st, const double* partials, int count_res,
@@ -577,6 +620,7 @@ struct Problem
     bool nt;                 // non-temporal matrix loads in the iteration
     bool dma;                // 4-byte matrix operands staged into LDS by global_load_lds
     DiaLayout dia;           // diagonal layout of A for q = A p, when it pays
+    DiaLayout dia_g, dia_gt; // ... of the factors G and G^T (both or neither)
     bool fuse_direction;     // on the diagonal layout: p_new formed inside the SpMV
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
@@ -644,6 +688,20 @@ struct Calls
     {
         const int64_t rows = pb.a->rows;
         const int grid = static_cast<int>(std::min<int64_t>(full_grid(rows), kSpmvGrid));
+        if (pb.dia_g.active)
+        {
+            const auto lower = pb.dia_g.args.ndiag == 4 ? k_factor_lower_dia<S, 4> : k_factor_lower_dia<S, 0>;
+            const auto upper = pb.dia_gt.args.ndiag == 4 ? k_factor_upper_dia<S, 4> : k_factor_upper_dia<S, 0>;
+            out.push_back(make_call(lower, dim3(grid), dim3(kBlock), rows, pb.dia_g.args,
+                                    static_cast<const S*>(pb.dia_g.values), pb.dia_g.mask,
+                                    static_cast<const S*>(pb.r), static_cast<S*>(pb.inv),
+                                    static_cast<const SolverState*>(pb.st)));
+            out.push_back(make_call(upper, dim3(pb.grid_rows), dim3(kBlock), rows, pb.dia_gt.args,
+                                    static_cast<const S*>(pb.dia_gt.values), pb.dia_gt.mask,
+                                    static_cast<const S*>(pb.inv), static_cast<const S*>(pb.r), static_cast<S*>(pb.z),
+                                    static_cast<const SolverState*>(pb.st), pb.partials));
+            return;
+        }
         out.push_back(make_call(pb.dma ? k_factor_lower<S, int32_t, true> : k_factor_lower<S, int32_t, false>,
                                 dim3(grid), dim3(kBlock), rows, pb.gf->row_offsets32,
                                 pb.gf->col_indices32, static_cast<const S*>(pb.gf->values),
@@ -816,10 +874,8 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
     // workspace addresses are part of the key: a grown workspace must never
     // replay a plan that still points at its old buffers
     char buf[768];
-    unsigned long long dia_sig = 1469598103934665603ull;  // FNV-1a over the diagonal offsets
-    if (pb.dia.active)
-        for (int d = 0; d < pb.dia.args.ndiag; ++d)
-            dia_sig = (dia_sig ^ static_cast<unsigned>(pb.dia.args.off[d])) * 1099511628211ull;
+    const unsigned long long dia_sig =
+        dia_signature(pb.dia) ^ (dia_signature(pb.dia_g) * 31ull) ^ (dia_signature(pb.dia_gt) * 131ull);
     std::snprintf(buf, sizeof(buf),
                   "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld|%d|%p|%p|%p|%p|%d|%d|%llx|%p|%p",
                   kind,
@@ -918,7 +974,13 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     pb.bsize = pre == kPreBlock ? bsize : 1;
     pb.pre = pre;
     PM_TRY(prepare(ws, pb, stream));
-    if (dia_wanted(a)) PM_TRY(dia_build(ws, a, &pb.dia, stream));
+    if (dia_wanted(a)) PM_TRY(dia_build(ws->dia, a, &pb.dia, stream));
+    if (pb.pre == kPreFactored && dia_wanted(gf) && dia_wanted(gtf))
+    {
+        PM_TRY(dia_build(ws->dia_g, gf, &pb.dia_g, stream));
+        PM_TRY(dia_build(ws->dia_gt, gtf, &pb.dia_gt, stream));
+        if (!pb.dia_g.active || !pb.dia_gt.active) pb.dia_g.active = pb.dia_gt.active = false;
+    }
     ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
     if (a->rows == 0)
     {

@@ -169,6 +169,8 @@ pm_workspace::~pm_workspace()
     batch.release();
     dense.release();
     dia.release();
+    dia_g.release();
+    dia_gt.release();
     for (hipEvent_t& e : events)
         if (e) (void)hipEventDestroy(e);
     if (done_event) (void)hipEventDestroy(done_event);
@@ -231,6 +233,8 @@ int pm_workspace_close(pm_workspace* ws)
     ws->batch.release();
     ws->dense.release();
     ws->dia.release();
+    ws->dia_g.release();
+    ws->dia_gt.release();
     ws->capacity_rows = 0;
     ws->bound = false;
     ws->stream = nullptr;
@@ -243,7 +247,7 @@ size_t pm_workspace_capacity_bytes(const pm_workspace* ws)
     if (!ws) return 0;
     return ws->vectors.bytes + ws->partials.bytes + ws->state.bytes + ws->scratch.bytes + ws->sort.bytes +
            ws->refine.bytes + ws->batch.bytes + ws->dense.bytes +
-           ws->dia.bytes;
+           ws->dia.bytes + ws->dia_g.bytes + ws->dia_gt.bytes;
 }
 
 int64_t pm_workspace_capacity_rows(const pm_workspace* ws) { return ws ? ws->capacity_rows : 0; }

@@ -122,6 +122,8 @@ struct pm_workspace
     pm::DeviceBuffer batch;     // batched solver: layout, per-system state, per-chunk partials
     pm::DeviceBuffer dense;     // dense Cholesky: n x n lower triangle, diagonal-block inverses
     pm::DeviceBuffer dia;       // diagonal layout of the solver's matrix
+    pm::DeviceBuffer dia_g;     // ... of the factored preconditioner's G
+    pm::DeviceBuffer dia_gt;    // ... and of G^T
     int32_t last_diagonals = 0; // diagonals of the last solve's layout (0: CSR kernels)
     int64_t capacity_rows = 0;
 

@@ -123,3 +123,27 @@ def test_refinement_with_dia_reaches_target(dev, monkeypatch):
     m = sp.csr_matrix((h.values.astype(np.float64), h.col_indices, h.row_offsets), shape=(h.rows, h.cols))
     xs = host(x)
     assert np.linalg.norm(b - m @ xs) <= 1.2e-8 * np.linalg.norm(b)
+
+
+def test_learned_factors_on_the_diagonal_layout(dev, monkeypatch):
+    # the factored preconditioner's G and G^T (4 diagonals each on a 3-D
+    # 7-point pattern) take the diagonal layout too; the solve stays bitwise
+    h = generators.diffusion3d_box_csr(48, 48, 48, seed=13, dtype=np.float32)
+    a = capi.to_device_csr(h, compact=True)
+    net = capi.FsaiNet(seed=0)
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=14)).to(dev)
+    out, grow = {}, {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_DIA", flag)
+        ws = capi.Workspace()
+        x = torch.zeros_like(b)
+        rep = capi.pcg_factored(a, b, x, graph, ws, relative_tolerance=1e-6, max_iterations=5000)
+        torch.cuda.synchronize()
+        grow[flag] = ws.capacity_bytes
+        out[flag] = (rep, host(x).copy())
+    # A (7 diagonals) plus G and G^T (4 each)
+    assert grow["1"] - grow["0"] >= (7 + 4 + 4) * h.rows * 4
+    assert out["0"][0].converged and out["0"][0].iterations == out["1"][0].iterations
+    assert np.array_equal(out["0"][1], out["1"][1])
