@@ -72,21 +72,63 @@ template <typename S>
 __global__ __launch_bounds__(kBlock) void k_graph_rows(int64_t n, const int32_t* __restrict__ ro,
                                                        const int32_t* __restrict__ ci, const S* __restrict__ v,
                                                        float* __restrict__ dinv, unsigned* __restrict__ lower_count,
-                                                       unsigned* __restrict__ bad)
+                                                       unsigned* __restrict__ upper_count, unsigned* __restrict__ bad,
+                                                       unsigned* __restrict__ unsorted)
 {
     for (int64_t i = gid(); i < n; i += gstr())
     {
         double d = 0.0;
-        unsigned lc = 0;
+        unsigned lc = 0, uc = 0;
+        int32_t prev = -1;
+        bool increasing = true;
         for (int32_t k = ro[i]; k < ro[i + 1]; ++k)
         {
             if (ci[k] == i) d += static_cast<double>(v[k]);
             if (ci[k] <= i) ++lc;
+            if (ci[k] >= i) ++uc;
+            if (ci[k] <= prev) increasing = false;
+            prev = ci[k];
         }
         const bool ok = d > 0.0 && isfinite(d);
         if (!ok) atomicMin(bad, static_cast<unsigned>(i < 0xffffffffll ? i : 0xfffffffe));
+        if (!increasing) atomicOr(unsorted, 1u);
         dinv[i] = ok ? static_cast<float>(1.0 / sqrt(d)) : 0.0f;
         lower_count[i] = lc;
+        upper_count[i] = uc;
+    }
+}
+
+// G^T's pattern straight from A's when A's pattern is symmetric: row i of G^T
+// is A's row i from the diagonal on, and entry (i, j) of G^T is entry (j, i)
+// of G, found in G's short row j.  A missing match raises *asym (the caller
+// then builds G^T by sorting instead).
+__global__ __launch_bounds__(kBlock) void k_graph_upper_sym(int64_t n, const int32_t* __restrict__ ro,
+                                                            const int32_t* __restrict__ ci,
+                                                            const int32_t* __restrict__ uro,
+                                                            const int32_t* __restrict__ lro,
+                                                            const int32_t* __restrict__ lci,
+                                                            int32_t* __restrict__ uci, int32_t* __restrict__ uperm,
+                                                            unsigned* __restrict__ asym)
+{
+    for (int64_t i = gid(); i < n; i += gstr())
+    {
+        int32_t out = uro[i];
+        for (int32_t k = ro[i]; k < ro[i + 1]; ++k)
+        {
+            const int32_t j = ci[k];
+            if (j < i) continue;
+            int32_t match = -1;
+            for (int32_t q = lro[j]; q < lro[j + 1]; ++q)
+                if (lci[q] == i)
+                {
+                    match = q;
+                    break;
+                }
+            if (match < 0) atomicOr(asym, 1u);
+            uci[out] = j;
+            uperm[out] = match < 0 ? 0 : match;
+            ++out;
+        }
     }
 }
 
@@ -143,6 +185,13 @@ __global__ __launch_bounds__(kBlock) void k_graph_features(int64_t n, const int3
 }
 
 // Lower pattern (col <= row) in row order, with the source entry of A.
+__global__ void k_graph_flags(unsigned* flags)
+{
+    flags[0] = 0xffffffffu;  // first row with a bad diagonal
+    flags[1] = 0u;           // a row with unsorted or repeated columns
+    flags[2] = 0u;           // an upper entry without its lower mirror
+}
+
 __global__ __launch_bounds__(kBlock) void k_graph_lower(int64_t n, const int32_t* __restrict__ ro,
                                                         const int32_t* __restrict__ ci,
                                                         const unsigned* __restrict__ lro, int32_t* __restrict__ lro32,
@@ -162,8 +211,11 @@ __global__ __launch_bounds__(kBlock) void k_graph_lower(int64_t n, const int32_t
             lci[out] = j;
             lsrc[out] = k;
             if (j == i) diag = static_cast<int32_t>(out);
-            keys[out] = (static_cast<unsigned long long>(j) << rb) | static_cast<unsigned long long>(i);
-            vals[out] = out;
+            if (keys)
+            {
+                keys[out] = (static_cast<unsigned long long>(j) << rb) | static_cast<unsigned long long>(i);
+                vals[out] = out;
+            }
             ++out;
         }
         ldiag[i] = diag;
@@ -949,29 +1001,28 @@ int graph_build(pm_fsai_graph* gr, const pm_csr* a, hipStream_t stream)
     gr->ci = a->col_indices32;
     gr->values = a->values;
     gr->stream = stream;
-    // temporaries
-    DeviceBuffer tmp;
+    // temporaries: per-row counts, their scans and the scan scratch; the
+    // key/value arrays of the sorted transpose only when A's pattern is not
+    // symmetric
+    DeviceBuffer tmp, sort_tmp;
     struct Release
     {
+        DeviceBuffer& a;
         DeviceBuffer& b;
-        ~Release() { b.release(); }
-    } rel{tmp};
+        ~Release()
+        {
+            a.release();
+            b.release();
+        }
+    } rel{tmp, sort_tmp};
     const int64_t cap = std::max<int64_t>(a->nnz, 1);
-    const int64_t hist = radix_hist_words(cap);
-    const size_t tbytes = sizeof(unsigned) * (n + 2) * 2 + sizeof(unsigned long long) * cap * 2 +
-                          sizeof(unsigned) * cap * 2 + sizeof(unsigned) * (hist + scan_scratch_words(std::max(hist, cap + n + 2))) + 4096;
+    const size_t tbytes = sizeof(unsigned) * (n + 2) * 3 + sizeof(unsigned) * scan_scratch_words(n + 2) + 4096;
     PM_TRY(tmp.reserve(tbytes));
-    char* t = tmp.as<char>();
-    unsigned* counts = reinterpret_cast<unsigned*>(t);
+    unsigned* counts = tmp.as<unsigned>();
     unsigned* lro_u = counts + (n + 2);
-    unsigned long long* keys_a = reinterpret_cast<unsigned long long*>(
-        (reinterpret_cast<uintptr_t>(lro_u + (n + 2)) + 255) & ~uintptr_t(255));
-    unsigned long long* keys_b = keys_a + cap;
-    unsigned* vals_a = reinterpret_cast<unsigned*>(keys_b + cap);
-    unsigned* vals_b = vals_a + cap;
-    unsigned* hist_buf = vals_b + cap;
-    unsigned* scratch = hist_buf + hist;
-    unsigned* bad = scratch + scan_scratch_words(std::max(hist, cap + n + 2)) + 8;
+    unsigned* ucounts = lro_u + (n + 2);
+    unsigned* scratch = ucounts + (n + 2);
+    unsigned* flags = scratch + scan_scratch_words(n + 2) + 8;  // [bad row, unsorted, asym]
 
     // graph buffer: dinv, s, x, lro, ldiag (n), lci, lsrc (nl<=nnz), uro, uci, uperm, 64-bit mirrors, g, gt, out
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -1003,24 +1054,17 @@ int graph_build(pm_fsai_graph* gr, const pm_csr* a, hipStream_t stream)
     gr->gt = take(cap * es);
     gr->out = reinterpret_cast<float*>(take(cap * 4));
 
-    PM_HIP(hipMemsetAsync(bad, 0xff, sizeof(unsigned), stream));
+    hipLaunchKernelGGL(k_graph_flags, dim3(1), dim3(1), 0, stream, flags);
     const int g = grid_of(n);
     if (a->dtype == PM_F32)
         hipLaunchKernelGGL(k_graph_rows<float>, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci,
-                           static_cast<const float*>(a->values), gr->dinv, counts, bad);
+                           static_cast<const float*>(a->values), gr->dinv, counts, ucounts, flags, flags + 1);
     else
         hipLaunchKernelGGL(k_graph_rows<double>, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci,
-                           static_cast<const double*>(a->values), gr->dinv, counts, bad);
+                           static_cast<const double*>(a->values), gr->dinv, counts, ucounts, flags, flags + 1);
     PM_HIP(hipGetLastError());
-    unsigned total = 0;
     PM_TRY(scan_u32(counts, lro_u, n + 1, scratch, nullptr, stream));
-    PM_HIP(hipMemcpyAsync(&total, lro_u + n, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-    unsigned hbad = 0;
-    PM_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
-    PM_HIP(hipStreamSynchronize(stream));
-    if (hbad != 0xffffffffu)
-        return fail(PM_RUNTIME_ERROR, "learned preconditioner: row %u has a missing or non-positive diagonal", hbad);
-    gr->nl = total;
+    PM_TRY(scan_u32(ucounts, reinterpret_cast<unsigned*>(gr->uro), n + 1, scratch, nullptr, stream));
     if (a->dtype == PM_F32)
         hipLaunchKernelGGL(k_graph_features<float>, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci,
                            static_cast<const float*>(a->values), gr->dinv, gr->s, gr->x);
@@ -1029,15 +1073,50 @@ int graph_build(pm_fsai_graph* gr, const pm_csr* a, hipStream_t stream)
                            static_cast<const double*>(a->values), gr->dinv, gr->s, gr->x);
     const int rb = std::max(1, bits_for(n));
     hipLaunchKernelGGL(k_graph_lower, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci, lro_u, gr->lro, gr->lci,
-                       gr->lsrc, gr->ldiag, keys_a, vals_a, rb);
+                       gr->lsrc, gr->ldiag, static_cast<unsigned long long*>(nullptr), static_cast<unsigned*>(nullptr),
+                       rb);
+    hipLaunchKernelGGL(k_graph_upper_sym, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci, gr->uro, gr->lro,
+                       gr->lci, gr->uci, gr->uperm, flags + 2);
     PM_HIP(hipGetLastError());
-    bool in_b = false;
-    PM_TRY(radix_sort_pairs(keys_a, vals_a, keys_b, vals_b, gr->nl, 2 * rb, hist_buf, scratch, stream, &in_b));
-    const unsigned long long* sk = in_b ? keys_b : keys_a;
-    const unsigned* sv = in_b ? vals_b : vals_a;
-    hipLaunchKernelGGL(k_graph_upper, dim3(grid_of(gr->nl)), dim3(kBlock), 0, stream, gr->nl, sk, sv, rb, gr->uci,
-                       gr->uperm);
-    hipLaunchKernelGGL(k_row_starts, dim3(grid_of(n + 1)), dim3(kBlock), 0, stream, n, gr->nl, sk, rb, gr->uro);
+    struct
+    {
+        unsigned flags[3];
+        unsigned lower_total, upper_total;
+    } h{};
+    PM_HIP(hipMemcpyAsync(h.flags, flags, sizeof(h.flags), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipMemcpyAsync(&h.lower_total, lro_u + n, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipMemcpyAsync(&h.upper_total, gr->uro + n, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    if (h.flags[0] != 0xffffffffu)
+        return fail(PM_RUNTIME_ERROR, "learned preconditioner: row %u has a missing or non-positive diagonal",
+                    h.flags[0]);
+    gr->nl = h.lower_total;
+    if (h.flags[1] != 0 || h.flags[2] != 0 || h.upper_total != h.lower_total)
+    {
+        // A's pattern is not symmetric (or its rows are not sorted): G^T from
+        // the (col, row)-sorted entries of G
+        const int64_t hist = radix_hist_words(cap);
+        const size_t sbytes = sizeof(unsigned long long) * cap * 2 + sizeof(unsigned) * cap * 2 +
+                              sizeof(unsigned) * (hist + scan_scratch_words(std::max(hist, cap + n + 2))) + 4096;
+        PM_TRY(sort_tmp.reserve(sbytes));
+        unsigned long long* keys_a = sort_tmp.as<unsigned long long>();
+        unsigned long long* keys_b = keys_a + cap;
+        unsigned* vals_a = reinterpret_cast<unsigned*>(keys_b + cap);
+        unsigned* vals_b = vals_a + cap;
+        unsigned* hist_buf = vals_b + cap;
+        unsigned* sort_scratch = hist_buf + hist;
+        hipLaunchKernelGGL(k_graph_lower, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci, lro_u, gr->lro,
+                           gr->lci, gr->lsrc, gr->ldiag, keys_a, vals_a, rb);
+        PM_HIP(hipGetLastError());
+        bool in_b = false;
+        PM_TRY(radix_sort_pairs(keys_a, vals_a, keys_b, vals_b, gr->nl, 2 * rb, hist_buf, sort_scratch, stream,
+                                &in_b));
+        const unsigned long long* sk = in_b ? keys_b : keys_a;
+        const unsigned* sv = in_b ? vals_b : vals_a;
+        hipLaunchKernelGGL(k_graph_upper, dim3(grid_of(gr->nl)), dim3(kBlock), 0, stream, gr->nl, sk, sv, rb,
+                           gr->uci, gr->uperm);
+        hipLaunchKernelGGL(k_row_starts, dim3(grid_of(n + 1)), dim3(kBlock), 0, stream, n, gr->nl, sk, rb, gr->uro);
+    }
     hipLaunchKernelGGL(k_widen, dim3(grid_of(n + 1)), dim3(kBlock), 0, stream, n + 1, gr->lro, gr->lro64);
     hipLaunchKernelGGL(k_widen, dim3(grid_of(n + 1)), dim3(kBlock), 0, stream, n + 1, gr->uro, gr->uro64);
     hipLaunchKernelGGL(k_widen, dim3(grid_of(gr->nl)), dim3(kBlock), 0, stream, gr->nl, gr->lci, gr->lci64);

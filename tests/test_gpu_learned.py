@@ -101,6 +101,53 @@ def test_forward_matches_oracle(dev):
     assert abs(G.T - GT).max() == 0.0
 
 
+def pattern_variant(h, kind, seed):
+    """The system's matrix with its pattern made asymmetric (some entries above
+    the diagonal dropped) or its rows' entries reversed; values are kept."""
+    ro, ci, v = h.row_offsets, h.col_indices, h.values
+    if kind == "reversed":
+        ci, v = ci.copy(), v.copy()
+        for r in range(h.rows):
+            s, e = ro[r], ro[r + 1]
+            ci[s:e] = ci[s:e][::-1]
+            v[s:e] = v[s:e][::-1]
+        return generators.CSRHost(h.rows, h.cols, ro, ci, v)
+    rows = np.repeat(np.arange(h.rows), np.diff(ro))
+    rng = np.random.default_rng(seed)
+    keep = ~((ci > rows) & (rng.random(ci.size) < 0.2))
+    counts = np.bincount(rows[keep], minlength=h.rows)
+    nro = np.concatenate([[0], np.cumsum(counts)]).astype(ro.dtype)
+    return generators.CSRHost(h.rows, h.cols, nro, ci[keep], v[keep])
+
+
+@pytest.mark.parametrize("kind", ["symmetric", "asymmetric", "reversed"])
+def test_transpose_factor_pattern(dev, kind):
+    # a symmetric sorted pattern takes G^T's pattern from A's rows directly;
+    # the other two build it by sorting G's entries: G^T is G's exact
+    # transpose either way
+    h = system(11, 9, 7, seed=17)
+    if kind != "symmetric":
+        h = pattern_variant(h, kind, seed=18)
+    a = capi.to_device_csr(h, compact=True)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = perturbed_parameters(6)
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    import scipy.sparse as sp
+
+    g, gt = graph.factors()
+    ro, ci, vals = capi.csr_arrays(g)
+    tro, tci, tvals = capi.csr_arrays(gt)
+    assert tro[-1] == ro[-1]
+    for r in range(h.rows):  # G^T's rows in increasing column order
+        assert np.all(np.diff(tci[tro[r]:tro[r + 1]]) > 0)
+    G = sp.csr_matrix((vals.astype(np.float64), ci, ro), shape=(h.rows, h.rows))
+    GT = sp.csr_matrix((tvals.astype(np.float64), tci, tro), shape=(h.rows, h.rows))
+    assert (G.T != GT).nnz == 0
+    rows = np.repeat(np.arange(h.rows), np.diff(ro))
+    assert np.all(ci <= rows) and np.count_nonzero(ci == rows) == h.rows
+
+
 def test_loss_and_gradient_match_oracle(dev):
     h = system(8, 7, 6, seed=8)
     a = capi.to_device_csr(h, compact=True)
