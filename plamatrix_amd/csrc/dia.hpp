@@ -53,9 +53,10 @@ unsigned long long dia_signature(const DiaLayout& d);
 // column order, acc = fma(value, x[col], acc) from 0 -- the CSR row walk's
 // arithmetic.  Every load is issued before the mask is known (absent slots
 // read an in-range value and a clamped gather that the mask discards).
-template <typename S, int ND>
-__device__ __forceinline__ S dia_row(int64_t row, int64_t rows, const DiaArgs& dia, const S* __restrict__ dv,
-                                     unsigned m, const S* __restrict__ x)
+// gather(col) yields the vector entry of column col.
+template <typename S, int ND, typename Gather>
+__device__ __forceinline__ S dia_row_gather(int64_t row, int64_t rows, const DiaArgs& dia, const S* __restrict__ dv,
+                                            unsigned m, const Gather& gather)
 {
     constexpr int kSlots = ND > 0 ? ND : kMaxDia;
     const int nd = ND > 0 ? ND : dia.ndiag;
@@ -68,7 +69,7 @@ __device__ __forceinline__ S dia_row(int64_t row, int64_t rows, const DiaArgs& d
             int64_t col = row + dia.off[d];
             col = col < 0 ? 0 : (col >= rows ? rows - 1 : col);
             vals[d] = dv[d * rows + row];
-            xs[d] = x[col];
+            xs[d] = gather(col);
         }
     }
     S acc = S(0);
@@ -76,6 +77,13 @@ __device__ __forceinline__ S dia_row(int64_t row, int64_t rows, const DiaArgs& d
     for (int d = 0; d < kSlots; ++d)
         if (d < nd && ((m >> d) & 1u)) acc = fma_s(vals[d], xs[d], acc);
     return acc;
+}
+
+template <typename S, int ND>
+__device__ __forceinline__ S dia_row(int64_t row, int64_t rows, const DiaArgs& dia, const S* __restrict__ dv,
+                                     unsigned m, const S* __restrict__ x)
+{
+    return dia_row_gather<S, ND>(row, rows, dia, dv, m, [x](int64_t c) { return x[c]; });
 }
 
 } // namespace pm

@@ -462,14 +462,12 @@ __global__ __launch_bounds__(kBlock) void k_axpy_x(int64_t rows, S* __restrict__
     for (int64_t i = begin + threadIdx.x; i < end; i += kBlock) x[i] = fma_s(alpha, p[i], x[i]);
 }
 
-// Factored preconditioner, step half: alpha from p.q, x += alpha p,
-// r -= alpha q (z follows from the two sparse products).
+// alpha = rho / p.q from the published p.q partials, recorded in the state
+// by workgroup 0 (with the breakdown flag when it is not usable).  Every
+// workgroup reduces the same partials in the same order, so all agree.
 template <typename S>
-__global__ __launch_bounds__(kBlock) void k_step_xr(int64_t rows, S* __restrict__ x, S* __restrict__ r,
-                                                    const S* __restrict__ p, const S* __restrict__ q, SolverState* st,
-                                                    double* partials, int par, int count_den)
+__device__ __forceinline__ bool step_alpha(SolverState* st, const double* partials, int par, int count_den, S& alpha)
 {
-    if (st->done != kRunning) return;
     double den[1];
     reduce_partials<1>(partials + kPartialsDen, count_den, den);
     double a = 0.0;
@@ -479,8 +477,20 @@ __global__ __launch_bounds__(kBlock) void k_step_xr(int64_t rows, S* __restrict_
         st->alpha = a;
         if (!ok) st->done = kBreakdown;
     }
-    if (!ok) return;
-    const S alpha = static_cast<S>(a);
+    alpha = static_cast<S>(a);
+    return ok;
+}
+
+// Factored preconditioner, step half: alpha from p.q, x += alpha p,
+// r -= alpha q (z follows from the two sparse products).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_step_xr(int64_t rows, S* __restrict__ x, S* __restrict__ r,
+                                                    const S* __restrict__ p, const S* __restrict__ q, SolverState* st,
+                                                    double* partials, int par, int count_den)
+{
+    if (st->done != kRunning) return;
+    S alpha;
+    if (!step_alpha<S>(st, partials, par, count_den, alpha)) return;
     int64_t begin, end;
     block_range(rows, kBlock, begin, end);
     for (int64_t i = begin + threadIdx.x; i < end; i += kBlock)
@@ -512,14 +522,51 @@ __global__ __launch_bounds__(kBlock) void k_factor_lower(int64_t rows, const I* 
     }
 }
 
-// z = G^T t (G^T as CSR) and the partial sums r.r, r.z of the next direction.
+// k_step_xr and k_factor_lower in one pass: alpha, x += alpha p on the
+// workgroup's rows, and t = G r_new with r_new = r - alpha q formed at every
+// gathered column (the same rounding as k_step_xr's r).  r itself is left for
+// k_factor_upper<STEP=true> to rewrite, since other workgroups gather it here.
 template <typename S, typename I, bool DMA>
-__global__ __launch_bounds__(kBlock) void k_factor_upper(int64_t rows, const I* __restrict__ ro,
-                                                         const I* __restrict__ ci, const S* __restrict__ v,
-                                                         const S* __restrict__ t, const S* __restrict__ r,
-                                                         S* __restrict__ z, const SolverState* st, double* partials)
+__global__ __launch_bounds__(kBlock) void k_step_lower(int64_t rows, const I* __restrict__ ro,
+                                                       const I* __restrict__ ci, const S* __restrict__ v,
+                                                       S* __restrict__ x, const S* __restrict__ r,
+                                                       const S* __restrict__ p, const S* __restrict__ q,
+                                                       S* __restrict__ t, SolverState* st, double* partials, int par,
+                                                       int count_den)
 {
     if (st->done != kRunning) return;
+    S alpha;
+    if (!step_alpha<S>(st, partials, par, count_den, alpha)) return;
+    __shared__ S lv_all[kWavesPerBlock][kStreamCap];
+    __shared__ I lc_all[kWavesPerBlock][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
+    {
+        const int64_t row0 = c * kChunkRows;
+        int64_t s, e;
+        const S acc = chunk_row_sums<S, I, StepSource<S>, false, DMA>(row0, rows, ro, ci, lv_all[wave], lc_all[wave],
+                                                                      StepSource<S>{v, r, q, -alpha}, s, e);
+        const int64_t row = row0 + lane;
+        if (row < rows)
+        {
+            t[row] = acc;
+            x[row] = fma_s(alpha, p[row], x[row]);
+        }
+    }
+}
+
+// z = G^T t (G^T as CSR) and the partial sums r.r, r.z of the next direction.
+// STEP: r -= alpha q first, on this row only (alpha from the state).
+template <typename S, typename I, bool DMA, bool STEP = false>
+__global__ __launch_bounds__(kBlock) void k_factor_upper(int64_t rows, const I* __restrict__ ro,
+                                                         const I* __restrict__ ci, const S* __restrict__ v,
+                                                         const S* __restrict__ t, S* __restrict__ r,
+                                                         const S* __restrict__ q, S* __restrict__ z,
+                                                         const SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
+    const S neg_alpha = STEP ? -static_cast<S>(st->alpha) : S(0);
     __shared__ S lv_all[kWavesPerBlock][kStreamCap];
     __shared__ I lc_all[kWavesPerBlock][kStreamCap];
     const int wave = threadIdx.x / kWave;
@@ -535,7 +582,13 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper(int64_t rows, const I* 
         if (row < rows)
         {
             z[row] = acc;
-            const double rv = static_cast<double>(r[row]);
+            S rs = r[row];
+            if constexpr (STEP)
+            {
+                rs = fma_s(neg_alpha, q[row], rs);
+                r[row] = rs;
+            }
+            const double rv = static_cast<double>(rs);
             rr = __builtin_fma(rv, rv, rr);
             rz = __builtin_fma(rv, static_cast<double>(acc), rz);
         }
@@ -561,14 +614,40 @@ __global__ __launch_bounds__(kBlock) void k_factor_lower_dia(int64_t rows, DiaAr
     }
 }
 
+// k_step_lower on the diagonal layout of G.
 template <typename S, int ND>
-__global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
-                                                             const uint16_t* __restrict__ mask,
-                                                             const S* __restrict__ t, const S* __restrict__ r,
-                                                             S* __restrict__ z, const SolverState* st,
-                                                             double* partials)
+__global__ __launch_bounds__(kBlock) void k_step_lower_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                           const uint16_t* __restrict__ mask, S* __restrict__ x,
+                                                           const S* __restrict__ r, const S* __restrict__ p,
+                                                           const S* __restrict__ q, S* __restrict__ t,
+                                                           SolverState* st, double* partials, int par, int count_den)
 {
     if (st->done != kRunning) return;
+    S alpha;
+    if (!step_alpha<S>(st, partials, par, count_den, alpha)) return;
+    const S neg_alpha = -alpha;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
+    {
+        const int64_t row = c * kChunkRows + lane;
+        if (row < rows)
+        {
+            t[row] = dia_row_gather<S, ND>(row, rows, dia, dv, mask[row],
+                                           [=](int64_t col) { return fma_s(neg_alpha, q[col], r[col]); });
+            x[row] = fma_s(alpha, p[row], x[row]);
+        }
+    }
+}
+
+template <typename S, int ND, bool STEP = false>
+__global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                             const uint16_t* __restrict__ mask,
+                                                             const S* __restrict__ t, S* __restrict__ r,
+                                                             const S* __restrict__ q, S* __restrict__ z,
+                                                             const SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
+    const S neg_alpha = STEP ? -static_cast<S>(st->alpha) : S(0);
     const int lane = threadIdx.x % kWave;
     double rr = 0.0, rz = 0.0;
     PM_FOR_CHUNKS(c, chunk_count(rows), kWavesPerBlock)
@@ -578,7 +657,13 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaAr
         {
             const S acc = dia_row<S, ND>(row, rows, dia, dv, mask[row], t);
             z[row] = acc;
-            const double rv = static_cast<double>(r[row]);
+            S rs = r[row];
+            if constexpr (STEP)
+            {
+                rs = fma_s(neg_alpha, q[row], rs);
+                r[row] = rs;
+            }
+            const double rv = static_cast<double>(rs);
             rr = __builtin_fma(rv, rv, rr);
             rz = __builtin_fma(rv, static_cast<double>(acc), rz);
         }
@@ -622,6 +707,7 @@ struct Problem
     DiaLayout dia;           // diagonal layout of A for q = A p, when it pays
     DiaLayout dia_g, dia_gt; // ... of the factors G and G^T (both or neither)
     bool fuse_direction;     // on the diagonal layout: p_new formed inside the SpMV
+    bool fuse_step;          // factored: the step folded into G r and G^T t
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
@@ -683,35 +769,62 @@ struct Calls
         else out.push_back(update_any<kModeInit>(pb, pb.p0, 0));
     }
 
-    // z = G^T (G r) and the r.r, r.z partials (grid_rows workgroups).
-    static void factored_apply(const Problem& pb, std::vector<KernelCall>& out)
+    // z = G^T (G r) and the r.r, r.z partials (grid_rows workgroups).  With
+    // p_step (the new direction of iteration parity par) the step comes first:
+    // alpha, x += alpha p, r -= alpha q -- folded into the two products unless
+    // fuse_step is off.
+    static void factored_apply(const Problem& pb, std::vector<KernelCall>& out, const void* p_step = nullptr,
+                               int par = 0)
     {
         const int64_t rows = pb.a->rows;
         const int grid = static_cast<int>(std::min<int64_t>(full_grid(rows), kSpmvGrid));
+        const bool fused = p_step != nullptr && pb.fuse_step;
+        if (p_step != nullptr && !fused)
+            out.push_back(make_call(k_step_xr<S>, dim3(pb.grid_units), dim3(kBlock), rows, static_cast<S*>(pb.x),
+                                    static_cast<S*>(pb.r), static_cast<const S*>(p_step),
+                                    static_cast<const S*>(pb.q), pb.st, pb.partials, par, pb.grid_rows));
+        const auto* r_in = static_cast<const S*>(pb.r);
+        auto* t = static_cast<S*>(pb.inv);
         if (pb.dia_g.active)
         {
-            const auto lower = pb.dia_g.args.ndiag == 4 ? k_factor_lower_dia<S, 4> : k_factor_lower_dia<S, 0>;
-            const auto upper = pb.dia_gt.args.ndiag == 4 ? k_factor_upper_dia<S, 4> : k_factor_upper_dia<S, 0>;
-            out.push_back(make_call(lower, dim3(grid), dim3(kBlock), rows, pb.dia_g.args,
-                                    static_cast<const S*>(pb.dia_g.values), pb.dia_g.mask,
-                                    static_cast<const S*>(pb.r), static_cast<S*>(pb.inv),
-                                    static_cast<const SolverState*>(pb.st)));
+            const bool four = pb.dia_g.args.ndiag == 4;
+            if (fused)
+                out.push_back(make_call(four ? k_step_lower_dia<S, 4> : k_step_lower_dia<S, 0>, dim3(grid),
+                                        dim3(kBlock), rows, pb.dia_g.args, static_cast<const S*>(pb.dia_g.values),
+                                        pb.dia_g.mask, static_cast<S*>(pb.x), r_in, static_cast<const S*>(p_step),
+                                        static_cast<const S*>(pb.q), t, pb.st, pb.partials, par, pb.grid_rows));
+            else
+                out.push_back(make_call(four ? k_factor_lower_dia<S, 4> : k_factor_lower_dia<S, 0>, dim3(grid),
+                                        dim3(kBlock), rows, pb.dia_g.args, static_cast<const S*>(pb.dia_g.values),
+                                        pb.dia_g.mask, r_in, t, static_cast<const SolverState*>(pb.st)));
+            const bool four_t = pb.dia_gt.args.ndiag == 4;
+            const auto upper = fused ? (four_t ? k_factor_upper_dia<S, 4, true> : k_factor_upper_dia<S, 0, true>)
+                                     : (four_t ? k_factor_upper_dia<S, 4, false> : k_factor_upper_dia<S, 0, false>);
             out.push_back(make_call(upper, dim3(pb.grid_rows), dim3(kBlock), rows, pb.dia_gt.args,
                                     static_cast<const S*>(pb.dia_gt.values), pb.dia_gt.mask,
-                                    static_cast<const S*>(pb.inv), static_cast<const S*>(pb.r), static_cast<S*>(pb.z),
-                                    static_cast<const SolverState*>(pb.st), pb.partials));
+                                    static_cast<const S*>(t), static_cast<S*>(pb.r), static_cast<const S*>(pb.q),
+                                    static_cast<S*>(pb.z), static_cast<const SolverState*>(pb.st), pb.partials));
             return;
         }
-        out.push_back(make_call(pb.dma ? k_factor_lower<S, int32_t, true> : k_factor_lower<S, int32_t, false>,
-                                dim3(grid), dim3(kBlock), rows, pb.gf->row_offsets32,
-                                pb.gf->col_indices32, static_cast<const S*>(pb.gf->values),
-                                static_cast<const S*>(pb.r), static_cast<S*>(pb.inv),
-                                static_cast<const SolverState*>(pb.st)));
-        out.push_back(make_call(pb.dma ? k_factor_upper<S, int32_t, true> : k_factor_upper<S, int32_t, false>,
-                                dim3(pb.grid_rows), dim3(kBlock), rows,
-                                pb.gtf->row_offsets32, pb.gtf->col_indices32, static_cast<const S*>(pb.gtf->values),
-                                static_cast<const S*>(pb.inv), static_cast<const S*>(pb.r), static_cast<S*>(pb.z),
-                                static_cast<const SolverState*>(pb.st), pb.partials));
+        if (fused)
+            out.push_back(make_call(pb.dma ? k_step_lower<S, int32_t, true> : k_step_lower<S, int32_t, false>,
+                                    dim3(grid), dim3(kBlock), rows, pb.gf->row_offsets32, pb.gf->col_indices32,
+                                    static_cast<const S*>(pb.gf->values), static_cast<S*>(pb.x), r_in,
+                                    static_cast<const S*>(p_step), static_cast<const S*>(pb.q), t, pb.st,
+                                    pb.partials, par, pb.grid_rows));
+        else
+            out.push_back(make_call(pb.dma ? k_factor_lower<S, int32_t, true> : k_factor_lower<S, int32_t, false>,
+                                    dim3(grid), dim3(kBlock), rows, pb.gf->row_offsets32, pb.gf->col_indices32,
+                                    static_cast<const S*>(pb.gf->values), r_in, t,
+                                    static_cast<const SolverState*>(pb.st)));
+        const auto upper = fused ? (pb.dma ? k_factor_upper<S, int32_t, true, true>
+                                           : k_factor_upper<S, int32_t, false, true>)
+                                 : (pb.dma ? k_factor_upper<S, int32_t, true, false>
+                                           : k_factor_upper<S, int32_t, false, false>);
+        out.push_back(make_call(upper, dim3(pb.grid_rows), dim3(kBlock), rows, pb.gtf->row_offsets32,
+                                pb.gtf->col_indices32, static_cast<const S*>(pb.gtf->values),
+                                static_cast<const S*>(t), static_cast<S*>(pb.r), static_cast<const S*>(pb.q),
+                                static_cast<S*>(pb.z), static_cast<const SolverState*>(pb.st), pb.partials));
     }
 
     // Iteration t: t % 2 picks the p buffers and the state slots.
@@ -760,10 +873,7 @@ struct Calls
         {
             if (pb.pre == kPreFactored)
             {
-                out.push_back(make_call(k_step_xr<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows,
-                                        static_cast<S*>(pb.x), static_cast<S*>(pb.r), static_cast<const S*>(p_new),
-                                        static_cast<const S*>(pb.q), pb.st, pb.partials, par, pb.grid_rows));
-                factored_apply(pb, out);
+                factored_apply(pb, out, p_new, par);
                 return;
             }
             out.push_back(update_any<kModeStep>(pb, p_new, par));
@@ -856,6 +966,10 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     // unless PLAMATRIX_AMD_FUSE_DIRECTION=1
     const char* fuse = std::getenv("PLAMATRIX_AMD_FUSE_DIRECTION");
     pb.fuse_direction = fuse != nullptr && fuse[0] == '1';
+    // the factored step (x, r update) rides in the two factor products; its
+    // separate kernel comes back with PLAMATRIX_AMD_FUSE_STEP=0
+    const char* fuse_step = std::getenv("PLAMATRIX_AMD_FUSE_STEP");
+    pb.fuse_step = fuse_step == nullptr || fuse_step[0] != '0';
     return PM_OK;
 }
 
@@ -884,7 +998,7 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
                   (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
                   static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
-                  static_cast<long long>(pb.bsize), (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0),
+                  static_cast<long long>(pb.bsize), (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0),
                   pb.gf ? pb.gf->values : nullptr,
                   pb.gf ? (const void*)pb.gf->row_offsets32 : nullptr, pb.gtf ? pb.gtf->values : nullptr,
                   pb.gtf ? (const void*)pb.gtf->row_offsets32 : nullptr, pb.dia.active ? 1 : 0, pb.dia.args.ndiag,

@@ -165,6 +165,25 @@ struct DirectionSource
     __device__ __forceinline__ S madd(S a, S g, S acc) const { return fma_s(a, g, acc); }
 };
 
+// Gathered entries of r_new = r - alpha q, formed on the fly (neg_alpha = -alpha).
+template <typename S>
+struct StepSource
+{
+    const S* __restrict__ v;
+    const S* __restrict__ r;
+    const S* __restrict__ q;
+    S neg_alpha;
+    using V = S;
+    using G = S;
+    __device__ __forceinline__ S value(int64_t k, bool nt) const { return stream_load(v + k, nt); }
+    template <typename I>
+    __device__ __forceinline__ S gather(I c) const
+    {
+        return fma_s(neg_alpha, q[c], r[c]);
+    }
+    __device__ __forceinline__ S madd(S a, S g, S acc) const { return fma_s(a, g, acc); }
+};
+
 // Value of lane `lane_id` in every lane (v_readlane: no LDS traffic).
 template <typename T>
 __device__ __forceinline__ T lane_value(T x, int lane_id)

@@ -11,6 +11,8 @@ factored PCG, against the CPU oracle oracle/fsai.py.
 - training on generated systems lowers the probe loss and the PCG iteration
   count on a held-out system compared with Jacobi.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -20,6 +22,9 @@ torch = pytest.importorskip("torch")
 
 from oracle import fsai  # noqa: E402
 from plamatrix_amd import capi, generators  # noqa: E402
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "plamatrix_amd",
+                       "fsai_weights.npy")
 
 
 @pytest.fixture(scope="module")
@@ -187,6 +192,33 @@ def test_factored_pcg_matches_oracle(dev, ws):
     got = x.cpu().numpy()
     assert np.linalg.norm(got - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
     assert capi.true_relres(a, b, x, ws) <= 1e-9
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("dims", [(14, 12, 10), (48, 40, 40)])  # CSR factors; diagonal-layout factors
+def test_fused_step_is_bitwise(dev, monkeypatch, dtype, dims):
+    # x += alpha p and r -= alpha q folded into the two factor products give
+    # the bits of the separate step kernel (PLAMATRIX_AMD_FUSE_STEP=0)
+    h = system(*dims, seed=19, dtype=dtype)
+    a = capi.to_device_csr(h, compact=True)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = np.load(WEIGHTS)  # the shipped trained network
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=20).astype(dtype)).to(dev)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_FUSE_STEP", flag)
+        w = capi.Workspace()
+        x = torch.zeros_like(b)
+        rep = capi.pcg_factored(a, b, x, graph, w, capi.policy(refresh_interval=20),
+                                relative_tolerance=1e-5 if dtype == np.float32 else 1e-10, max_iterations=4000)
+        torch.cuda.synchronize()
+        out[flag] = (rep, x.cpu().numpy())
+    (r0, x0), (r1, x1) = out["0"], out["1"]
+    assert r0.iterations == r1.iterations and r0.final_residual == r1.final_residual, (r0, r1)
+    assert np.array_equal(x0, x1)
+    assert r0.converged and r1.converged, (r0, r1)
 
 
 def test_training_improves_on_jacobi(dev, ws):
