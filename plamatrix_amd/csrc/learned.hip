@@ -99,9 +99,14 @@ __global__ __launch_bounds__(kBlock) void k_graph_rows(int64_t n, const int32_t*
 }
 
 // G^T's pattern straight from A's when A's pattern is symmetric: row i of G^T
-// is A's row i from the diagonal on, and entry (i, j) of G^T is entry (j, i)
-// of G, found in G's short row j.  A missing match raises *asym (the caller
-// then builds G^T by sorting instead).
+// is A's row i from the diagonal on (its last uc entries when the row is
+// sorted), and entry (i, j) of G^T is entry (j, i) of G, found by bisection in
+// G's row j.  Eight lanes share a row.  A missing match raises *asym and the
+// caller builds G^T by sorting instead; writes stay inside row i's uc slots
+// whatever the input, so an unsorted row is caught by the flags, not by a
+// stray store.
+constexpr int kUpperLanes = 8;
+
 __global__ __launch_bounds__(kBlock) void k_graph_upper_sym(int64_t n, const int32_t* __restrict__ ro,
                                                             const int32_t* __restrict__ ci,
                                                             const int32_t* __restrict__ uro,
@@ -110,25 +115,29 @@ __global__ __launch_bounds__(kBlock) void k_graph_upper_sym(int64_t n, const int
                                                             int32_t* __restrict__ uci, int32_t* __restrict__ uperm,
                                                             unsigned* __restrict__ asym)
 {
-    for (int64_t i = gid(); i < n; i += gstr())
+    const int sub = static_cast<int>(gid() % kUpperLanes);
+    for (int64_t i = gid() / kUpperLanes; i < n; i += gstr() / kUpperLanes)
     {
-        int32_t out = uro[i];
-        for (int32_t k = ro[i]; k < ro[i + 1]; ++k)
+        const int32_t uc = uro[i + 1] - uro[i];
+        const int32_t first = ro[i + 1] - uc;
+        bool missing = false;
+        for (int32_t k = first + sub; k < ro[i + 1]; k += kUpperLanes)
         {
             const int32_t j = ci[k];
-            if (j < i) continue;
-            int32_t match = -1;
-            for (int32_t q = lro[j]; q < lro[j + 1]; ++q)
-                if (lci[q] == i)
-                {
-                    match = q;
-                    break;
-                }
-            if (match < 0) atomicOr(asym, 1u);
+            int32_t lo = lro[j], hi = lro[j + 1];  // G's row j: columns <= j, increasing
+            while (lo < hi)
+            {
+                const int32_t mid = lo + (hi - lo) / 2;
+                if (lci[mid] < i) lo = mid + 1;
+                else hi = mid;
+            }
+            const bool found = lo < lro[j + 1] && lci[lo] == i;
+            missing |= !found;
+            const int32_t out = uro[i] + (k - first);
             uci[out] = j;
-            uperm[out] = match < 0 ? 0 : match;
-            ++out;
+            uperm[out] = found ? lo : 0;
         }
+        if (missing) atomicOr(asym, 1u);
     }
 }
 
@@ -1075,8 +1084,8 @@ int graph_build(pm_fsai_graph* gr, const pm_csr* a, hipStream_t stream)
     hipLaunchKernelGGL(k_graph_lower, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci, lro_u, gr->lro, gr->lci,
                        gr->lsrc, gr->ldiag, static_cast<unsigned long long*>(nullptr), static_cast<unsigned*>(nullptr),
                        rb);
-    hipLaunchKernelGGL(k_graph_upper_sym, dim3(g), dim3(kBlock), 0, stream, n, gr->ro, gr->ci, gr->uro, gr->lro,
-                       gr->lci, gr->uci, gr->uperm, flags + 2);
+    hipLaunchKernelGGL(k_graph_upper_sym, dim3(grid_of(n * kUpperLanes)), dim3(kBlock), 0, stream, n, gr->ro, gr->ci,
+                       gr->uro, gr->lro, gr->lci, gr->uci, gr->uperm, flags + 2);
     PM_HIP(hipGetLastError());
     struct
     {

@@ -708,6 +708,7 @@ struct Problem
     DiaLayout dia_g, dia_gt; // ... of the factors G and G^T (both or neither)
     bool fuse_direction;     // on the diagonal layout: p_new formed inside the SpMV
     bool fuse_step;          // factored: the step folded into G r and G^T t
+    int fuse_step_env;       // PLAMATRIX_AMD_FUSE_STEP: 0 off, 1 on, -1 unset
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
@@ -930,6 +931,20 @@ int check_system(const pm_csr* a, const void* rhs, const void* x, const char* op
     return PM_OK;
 }
 
+// The factored step (x += alpha p, r -= alpha q) rides in the two factor
+// products when G is on the diagonal layout (128^3 diffusion: 63.1 vs 64.7 us per
+// iteration) or the system is small enough to be launch-bound; on large CSR
+// factors the products' second gathers of r and q cost more than the kernel
+// saved (2.1M rows: 29.8 + 35.0 us vs 12.4 + 19.9 + 22.9 us).
+// PLAMATRIX_AMD_FUSE_STEP=0 / 1 forces either form.
+constexpr int64_t kFuseStepCsrRows = 65536;
+
+bool fuse_step_wanted(const Problem& pb)
+{
+    if (pb.fuse_step_env >= 0) return pb.fuse_step_env == 1;
+    return pb.dia_g.active || pb.a->rows < kFuseStepCsrRows;
+}
+
 size_t elem_size(int32_t dtype) { return dtype == PM_F64 ? 8 : 4; }
 
 // Reserve workspace storage and fill the workspace part of a Problem.
@@ -958,18 +973,17 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     const char* nt = std::getenv("PLAMATRIX_AMD_SOLVER_NT");
     pb.nt = nt != nullptr && nt[0] == '1';
     // LDS-DMA staging speeds up the HBM-streamed SpMV but slows the solver's
-    // cache-resident iteration (measured: 53.3 vs 51.1 us per c1 iteration)
+    // cache-resident iteration (measured: 53.3 vs 51.1 us per iteration on the 128^3 diffusion matrix)
     const char* dma = std::getenv("PLAMATRIX_AMD_SOLVER_DMA");
     pb.dma = dma != nullptr && dma[0] == '1';
     // fusing the direction into the diagonal-layout SpMV saves a pass over z
-    // and p_old but measured slower (41.7 vs 40.6 us per c1 iteration): off
+    // and p_old but measured slower (41.7 vs 40.6 us per 128^3 diffusion iteration): off
     // unless PLAMATRIX_AMD_FUSE_DIRECTION=1
     const char* fuse = std::getenv("PLAMATRIX_AMD_FUSE_DIRECTION");
     pb.fuse_direction = fuse != nullptr && fuse[0] == '1';
-    // the factored step (x, r update) rides in the two factor products; its
-    // separate kernel comes back with PLAMATRIX_AMD_FUSE_STEP=0
     const char* fuse_step = std::getenv("PLAMATRIX_AMD_FUSE_STEP");
-    pb.fuse_step = fuse_step == nullptr || fuse_step[0] != '0';
+    pb.fuse_step_env = fuse_step == nullptr ? -1 : (fuse_step[0] == '0' ? 0 : 1);
+    pb.fuse_step = fuse_step_wanted(pb);
     return PM_OK;
 }
 
@@ -1095,6 +1109,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         PM_TRY(dia_build(ws->dia_gt, gtf, &pb.dia_gt, stream));
         if (!pb.dia_g.active || !pb.dia_gt.active) pb.dia_g.active = pb.dia_gt.active = false;
     }
+    pb.fuse_step = fuse_step_wanted(pb);
     ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
     if (a->rows == 0)
     {
