@@ -39,15 +39,18 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4) -> 
     return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + 2 * n * value_bytes
 
 
-def pcg_iteration_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4, diagonals: int = 0) -> int:
+def pcg_iteration_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4, diagonals: int = 0,
+                        unit_diagonal: bool = False) -> int:
     """Algorithmic bytes of one Jacobi-PCG iteration (three kernels): the
     direction kernel reads z and p and writes p; the SpMV streams values,
     columns and offsets (or, on the diagonal layout, `diagonals` value slots
     and a 2-byte mask per row), reads p once and writes q; the vector update
-    reads x, p, r, q, M^-1 and writes x, r, z."""
+    reads x, p, r, q, M^-1 and writes x, r, z.  With a unit diagonal (every
+    Jacobi inverse exactly 1) z is r: no M^-1 read and no z write."""
+    vectors = 11 if unit_diagonal else 13
     if diagonals:
-        return diagonals * n * value_bytes + 2 * n + 13 * n * value_bytes
-    return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + 13 * n * value_bytes
+        return diagonals * n * value_bytes + 2 * n + vectors * n * value_bytes
+    return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + vectors * n * value_bytes
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_spmv_diffusion128_traffic.txt")
@@ -192,7 +195,10 @@ def run_diffusion(args, torch, capi, generators):
     relres = capi.true_relres(a, b, x, ws)
     iterations = rep.inner_iterations
     diagonals = ws.solver_diagonals
-    it_bytes = pcg_iteration_bytes(n, nnz, diagonals=diagonals)
+    # the matrix is scaled to a unit diagonal: its Jacobi solve runs as the
+    # identical unpreconditioned iteration (no M^-1 read, no z write)
+    unit_diagonal = ws.solver_preconditioner == "none"
+    it_bytes = pcg_iteration_bytes(n, nnz, diagonals=diagonals, unit_diagonal=unit_diagonal)
     it_ms = ms_per_solve / max(iterations, 1)
 
     # ---- learned preconditioner: G inferred on this matrix inside the timed solve ----
@@ -296,7 +302,7 @@ def run_diffusion(args, torch, capi, generators):
                      "ms_per_launch": round(spmv_ms, 5)},
         "cpu_baseline": cpu,
         "learned": learned,
-        "solve": {"preconditioner": "jacobi", "ms": round(ms_per_solve, 3), "converged": rep.converged, "outer": rep.outer_iterations, "inner_iterations": iterations,
+        "solve": {"preconditioner": "jacobi", "jacobi_kernels": "unit diagonal: z = r" if unit_diagonal else "z = M^-1 r", "ms": round(ms_per_solve, 3), "converged": rep.converged, "outer": rep.outer_iterations, "inner_iterations": iterations,
                   "true_relres": relres, "ms_per_iteration": round(it_ms, 4),
                   "matrix_layout": f"diagonal ({diagonals} diagonals)" if diagonals else "csr",
                   "iteration_bytes": it_bytes, "iteration_gbs": round(it_bytes / (it_ms * 1e-3) / 1e9, 1),

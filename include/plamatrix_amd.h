@@ -84,6 +84,13 @@ int64_t pm_workspace_capacity_rows(const pm_workspace* ws);
  * start of each solve (PLAMATRIX_AMD_DIA=0 disables it); results are bitwise
  * those of the CSR kernels. */
 int32_t pm_workspace_solver_diagonals(const pm_workspace* ws);
+/* No reference counterpart: the preconditioner the last adaptive solve ran
+ * (0: none, 1: Jacobi, 2: block-Jacobi, 3: factored G^T G).  A Jacobi solve of
+ * a matrix of at least 65,536 rows whose every Jacobi inverse is exactly 1 (a
+ * diagonally scaled matrix) runs as the unpreconditioned iteration, which is
+ * the same computation bit for bit (z = 1 * r), and reports 0
+ * (PLAMATRIX_AMD_UNIT_DIAGONAL=0 keeps the Jacobi kernels). */
+int32_t pm_workspace_solver_preconditioner(const pm_workspace* ws);
 
 /* Structural validation on the device (CSRMatrix::validateStructure,
  * csr_matrix.h:197-237, 633-667): offsets start at 0, end at nnz and are

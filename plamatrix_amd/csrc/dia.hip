@@ -18,8 +18,9 @@ constexpr int kEmpty = INT_MIN;
 
 // table[0..kMaxDia): distinct offsets (kEmpty where unused); table[kMaxDia]:
 // flags (1: a row's columns are not strictly increasing, 2: more than kMaxDia
-// diagonals).  Each workgroup collects its rows' offsets in LDS first, then
-// merges them into the global table with at most kMaxDia compare-and-swaps.
+// diagonals, 4: with UNIT, a row whose Jacobi inverse is not exactly 1).  Each
+// workgroup collects its rows' offsets in LDS first, then merges them into the
+// global table with at most kMaxDia compare-and-swaps.
 __global__ void k_dia_reset(int* table)
 {
     const int t = threadIdx.x;
@@ -43,8 +44,14 @@ __device__ __forceinline__ bool dia_insert(int* slots, int off, bool shared_mem)
     return false;
 }
 
+// The Jacobi inverse of a row as the solver's initialisation forms it (sum of
+// the diagonal entries, rejected when missing, non-finite or not above
+// 16 eps(S) max|a_row|, reciprocal in FP64 rounded to S): UNIT asks whether it
+// is exactly 1 in every row, i.e. whether z = D^-1 r is r itself.
+template <typename S, bool UNIT>
 __global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t rows, const int32_t* __restrict__ ro,
-                                                      const int32_t* __restrict__ ci, int* table)
+                                                      const int32_t* __restrict__ ci, const S* __restrict__ v,
+                                                      int* table)
 {
     __shared__ int local[kMaxDia];
     __shared__ int flags;
@@ -57,12 +64,31 @@ __global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t rows, const int32
     {
         int prev = -1;
         const int end = ro[r + 1];
+        double diag = 0.0, scale = 0.0;
+        bool found = false;
         for (int k = ro[r]; k < end; ++k)
         {
             const int c = ci[k];
             if (c <= prev) my_flags |= 1;
             prev = c;
             if (!dia_insert(local, static_cast<int>(c - r), true)) my_flags |= 2;
+            if constexpr (UNIT)
+            {
+                const double a = static_cast<double>(v[k]);
+                scale = fmax(scale, fabs(a));
+                if (c == r)
+                {
+                    diag += a;
+                    found = true;
+                }
+            }
+        }
+        if constexpr (UNIT)
+        {
+            const double threshold = static_cast<double>(scalar_epsilon<S>()) * scale * 16.0;
+            const double reciprocal = 1.0 / diag;
+            const bool ok = found && isfinite(diag) && diag > threshold && isfinite(reciprocal);
+            if (!ok || static_cast<S>(reciprocal) != S(1)) my_flags |= 4;
         }
     }
     if (my_flags) atomicOr(&flags, my_flags);
@@ -112,9 +138,10 @@ unsigned long long dia_signature(const DiaLayout& d)
     return sig;
 }
 
-int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream)
+int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream, bool* unit_diagonal)
 {
     *out = DiaLayout{};
+    if (unit_diagonal) *unit_diagonal = false;
     const int64_t n = a->rows;
     if (n <= 0 || a->nnz <= 0 || !a->row_offsets32 || !a->col_indices32 || a->rows != a->cols) return PM_OK;
     const size_t es = a->dtype == PM_F64 ? 8 : 4;
@@ -123,13 +150,24 @@ int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_
     int* table = storage.as<int>();
     hipLaunchKernelGGL(k_dia_reset, dim3(1), dim3(64), 0, stream, table);
     const int grid = static_cast<int>(std::min<int64_t>(ceil_div(n, kBlock), 2048));
-    hipLaunchKernelGGL(k_dia_detect, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32, a->col_indices32,
-                       table);
+    if (a->dtype == PM_F64)
+    {
+        const auto detect = unit_diagonal ? k_dia_detect<double, true> : k_dia_detect<double, false>;
+        hipLaunchKernelGGL(detect, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32, a->col_indices32,
+                           static_cast<const double*>(a->values), table);
+    }
+    else
+    {
+        const auto detect = unit_diagonal ? k_dia_detect<float, true> : k_dia_detect<float, false>;
+        hipLaunchKernelGGL(detect, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32, a->col_indices32,
+                           static_cast<const float*>(a->values), table);
+    }
     PM_HIP(hipGetLastError());
     int host[kMaxDia + 1];
     PM_HIP(hipMemcpyAsync(host, table, sizeof(host), hipMemcpyDeviceToHost, stream));
     PM_HIP(hipStreamSynchronize(stream));
-    if (host[kMaxDia] != 0) return PM_OK;
+    if (unit_diagonal) *unit_diagonal = (host[kMaxDia] & 4) == 0;
+    if ((host[kMaxDia] & 3) != 0) return PM_OK;
     DiaArgs args{};
     for (int s = 0; s < kMaxDia; ++s)
         if (host[s] != kEmpty) args.off[args.ndiag++] = host[s];

@@ -42,9 +42,12 @@ struct DiaLayout
 // mirrors present) has at most kMaxDia distinct diagonals, strictly
 // increasing columns in every row, and the diagonals' slots are at least
 // 1/kDiaMaxFill full; otherwise out->active stays false.  Synchronises
-// `stream` once to read the detected diagonals.
+// `stream` once to read the detected diagonals.  With `unit_diagonal`, the
+// same pass also reports whether every row's Jacobi inverse is exactly 1 (a
+// diagonally scaled matrix), whether or not the layout is built.
 constexpr double kDiaMaxFill = 1.3;
-int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream);
+int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream,
+              bool* unit_diagonal = nullptr);
 
 // A 64-bit signature of a layout (diagonals and storage), for plan keys.
 unsigned long long dia_signature(const DiaLayout& d);

@@ -1102,7 +1102,18 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     pb.bsize = pre == kPreBlock ? bsize : 1;
     pb.pre = pre;
     PM_TRY(prepare(ws, pb, stream));
-    if (dia_wanted(a)) PM_TRY(dia_build(ws->dia, a, &pb.dia, stream));
+    if (dia_wanted(a))
+    {
+        // Jacobi on a matrix whose every Jacobi inverse is exactly 1 is the
+        // unpreconditioned iteration bit for bit (z = 1 * r = r): it then runs
+        // as such, without the inverse-diagonal reads and z writes
+        // (PLAMATRIX_AMD_UNIT_DIAGONAL=0 keeps the Jacobi kernels)
+        const char* unit_env = std::getenv("PLAMATRIX_AMD_UNIT_DIAGONAL");
+        const bool check_unit = pb.pre == kPreJacobi && (unit_env == nullptr || unit_env[0] != '0');
+        bool unit = false;
+        PM_TRY(dia_build(ws->dia, a, &pb.dia, stream, check_unit ? &unit : nullptr));
+        if (check_unit && unit) pb.pre = kPreNone;
+    }
     if (pb.pre == kPreFactored && dia_wanted(gf) && dia_wanted(gtf))
     {
         PM_TRY(dia_build(ws->dia_g, gf, &pb.dia_g, stream));
@@ -1111,6 +1122,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     }
     pb.fuse_step = fuse_step_wanted(pb);
     ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
+    ws->last_preconditioner = pb.pre;
     if (a->rows == 0)
     {
         pm_solver_report rep{};

@@ -147,3 +147,55 @@ def test_learned_factors_on_the_diagonal_layout(dev, monkeypatch):
     assert grow["1"] - grow["0"] >= (7 + 4 + 4) * h.rows * 4
     assert out["0"][0].converged and out["0"][0].iterations == out["1"][0].iterations
     assert np.array_equal(out["0"][1], out["1"][1])
+
+
+def jacobi_outcome(dev, a, b, monkeypatch, unit_flag, rel, kernels=None):
+    monkeypatch.setenv("PLAMATRIX_AMD_UNIT_DIAGONAL", unit_flag)
+    ws = capi.Workspace()
+    x = torch.zeros_like(b)
+    try:
+        rep = capi.pcg(a, b, x, ws, capi.policy(refresh_interval=50), relative_tolerance=rel, max_iterations=20000)
+    except Exception as e:  # the Jacobi diagonal rules reject a row
+        torch.cuda.synchronize()
+        return ("error", type(e).__name__, str(e))
+    torch.cuda.synchronize()
+    if kernels is not None:
+        assert ws.solver_preconditioner == kernels
+    return (rep.converged, rep.iterations, rep.final_residual, host(x).tobytes())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_unit_diagonal_jacobi_is_bitwise(dev, monkeypatch, dtype):
+    # the diffusion matrix is scaled to a unit diagonal: Jacobi's z = 1 * r is
+    # r, so the solve runs without the inverse diagonal and must keep its bits
+    h = generators.diffusion3d_csr(48, seed=23, dtype=dtype)
+    a = capi.to_device_csr(h, compact=True)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=24).astype(dtype)).to(dev)
+    rel = 1e-6 if dtype == np.float32 else 1e-10
+    ref = jacobi_outcome(dev, a, b, monkeypatch, "0", rel, kernels="jacobi")
+    got = jacobi_outcome(dev, a, b, monkeypatch, "1", rel, kernels="none")
+    assert ref[0] is True and ref == got
+
+
+@pytest.mark.parametrize("change", ["diagonal", "huge_off_diagonal"])
+def test_unit_diagonal_check_is_exact(dev, monkeypatch, change):
+    # one row breaks the unit-inverse property: a diagonal of 2, or (FP32) an
+    # off-diagonal so large that the Jacobi rules reject the row's diagonal;
+    # the outcome must be the Jacobi kernels' either way
+    h = generators.diffusion3d_csr(48, seed=25, dtype=np.float32)
+    v = h.values.copy()
+    row = h.rows // 2
+    s, e = h.row_offsets[row], h.row_offsets[row + 1]
+    cols = h.col_indices[s:e]
+    if change == "diagonal":
+        v[s + int(np.nonzero(cols == row)[0][0])] = 2.0
+    else:
+        v[s + int(np.nonzero(cols != row)[0][0])] = -1e8
+    m = generators.CSRHost(h.rows, h.cols, h.row_offsets, h.col_indices, v)
+    a = capi.to_device_csr(m, compact=True)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=26)).to(dev)
+    ref = jacobi_outcome(dev, a, b, monkeypatch, "0", 1e-6)
+    got = jacobi_outcome(dev, a, b, monkeypatch, "1", 1e-6, kernels="jacobi" if change == "diagonal" else None)
+    assert ref == got
+    if change == "huge_off_diagonal":
+        assert ref[0] == "error" and "diagonal" in ref[2]
