@@ -405,37 +405,66 @@ __global__ __launch_bounds__(256, 2) void k_dc_gemm(int64_t rows, int64_t cols, 
 }
 
 // Forward block step: y_k = inv_k z_k (z_k final), y_k -> yout, and
-// z[r] -= L[r][k-block] y_k for the rows below the block.
+// z[r] -= L[r][k-block] y_k for the rows below the block.  inv_k is a full
+// 128 x 128 column-major array with zeros above the diagonal and outside the
+// block, so every dot runs over all 128 slots with independent loads.
+//   y_k: row i by two threads (64 columns each), halves added in fixed order;
+//   update: 64 rows per workgroup, lane = row, wave v = columns 32v..32v+31,
+//   the four partial sums added in fixed order.
+constexpr int kDcSolveRows = 64;
+static_assert(kBlock == 4 * kWave && kDcSolveRows == kWave && kDcTile == 2 * kWave,
+              "the block solves assume 4 wavefronts of 64 lanes and 128-wide blocks");
+
 __global__ __launch_bounds__(kBlock) void k_dc_forward(int64_t n, int64_t c0, int w, const double* __restrict__ a,
                                                       const double* __restrict__ inv, double* z, double* yout,
                                                       const int* fail)
 {
     __shared__ double zb[kDcTile];
     __shared__ double yk[kDcTile];
+    __shared__ double halves[2][kDcTile];
+    __shared__ double part[kBlock / kWave][kDcSolveRows];
     if (*fail) return;
-    const int tid = threadIdx.x;
-    if (tid < w) zb[tid] = z[c0 + tid];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    if (tid < kDcTile) zb[tid] = tid < w ? z[c0 + tid] : 0.0;
     __syncthreads();
-    if (tid < w)
     {
+        const int i = tid & (kDcTile - 1), h = tid / kDcTile;
         double s = 0.0;
-        for (int j = 0; j <= tid; ++j) s += inv[tid + j * kDcTile] * zb[j];
-        yk[tid] = s;
-        if (blockIdx.x == 0) yout[c0 + tid] = s;
+#pragma unroll 16
+        for (int jj = 0; jj < kDcTile / 2; ++jj)
+        {
+            const int j = h * (kDcTile / 2) + jj;
+            s += inv[i + j * kDcTile] * zb[j];
+        }
+        halves[h][i] = s;
     }
     __syncthreads();
-    const int64_t r = c0 + w + static_cast<int64_t>(blockIdx.x) * kBlock + tid;
+    if (tid < kDcTile)
+    {
+        const double y = halves[0][tid] + halves[1][tid];
+        yk[tid] = tid < w ? y : 0.0;
+        if (blockIdx.x == 0 && tid < w) yout[c0 + tid] = y;
+    }
+    __syncthreads();
+    const int64_t r = c0 + w + static_cast<int64_t>(blockIdx.x) * kDcSolveRows + lane;
+    double s = 0.0;
     if (r < n)
     {
-        double s = z[r];
-        for (int j = 0; j < w; ++j) s -= a[r + (c0 + j) * n] * yk[j];
-        z[r] = s;
+        const double* col = a + r + (c0 + wave * (kDcTile / 4)) * n;
+#pragma unroll
+        for (int jj = 0; jj < kDcTile / 4; ++jj)
+            if (wave * (kDcTile / 4) + jj < w) s += col[jj * n] * yk[wave * (kDcTile / 4) + jj];
     }
+    part[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && r < n) z[r] -= ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
 }
 
 // Backward block step: x_k = inv_k^T y_k (y_k final), x_k -> x, and
-// y[c] -= sum_i L[c0+i][c] x_k[i] for the columns before the block (16 lanes
-// per column, contiguous in i).
+// y[c] -= sum_i L[c0+i][c] x_k[i] for the columns before the block.
+//   x_k: output c by one wavefront (lanes over the 128 slots of inv_k's
+//   column c, coalesced), butterfly sum; 32 outputs per wavefront;
+//   update: 16 lanes per column over the block's rows, butterfly sum.
 __global__ __launch_bounds__(kBlock) void k_dc_backward(int64_t n, int64_t c0, int w, const double* __restrict__ a,
                                                        const double* __restrict__ inv, double* y, double* x,
                                                        const int* fail)
@@ -443,15 +472,22 @@ __global__ __launch_bounds__(kBlock) void k_dc_backward(int64_t n, int64_t c0, i
     __shared__ double yb[kDcTile];
     __shared__ double xk[kDcTile];
     if (*fail) return;
-    const int tid = threadIdx.x;
-    if (tid < w) yb[tid] = y[c0 + tid];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    if (tid < kDcTile) yb[tid] = tid < w ? y[c0 + tid] : 0.0;
     __syncthreads();
-    if (tid < w)
+    const double y0 = yb[lane], y1 = yb[lane + kWave];
+#pragma unroll 8
+    for (int cc = 0; cc < kDcTile / 4; ++cc)
     {
-        double s = 0.0;
-        for (int i = tid; i < w; ++i) s += inv[i + tid * kDcTile] * yb[i];
-        xk[tid] = s;
-        if (blockIdx.x == 0) x[c0 + tid] = s;
+        const int c = wave * (kDcTile / 4) + cc;
+        double s = inv[lane + c * kDcTile] * y0 + inv[lane + kWave + c * kDcTile] * y1;
+#pragma unroll
+        for (int m = 1; m < kWave; m <<= 1) s += __shfl_xor(s, m);
+        if (lane == 0)
+        {
+            xk[c] = c < w ? s : 0.0;
+            if (blockIdx.x == 0 && c < w) x[c0 + c] = s;
+        }
     }
     __syncthreads();
     const int64_t col = static_cast<int64_t>(blockIdx.x) * (kBlock / 16) + (tid >> 4);
@@ -599,7 +635,7 @@ int dense_cholesky_solve(const pm_csr* a, const double* b, double* x, pm_workspa
     {
         const int64_t c0 = k * kDcTile;
         const int w = static_cast<int>(std::min<int64_t>(kDcTile, n - c0));
-        const int64_t grid = std::max<int64_t>(1, ceil_div(n - c0 - w, kBlock));
+        const int64_t grid = std::max<int64_t>(1, ceil_div(n - c0 - w, kDcSolveRows));
         hipLaunchKernelGGL(k_dc_forward, dim3(static_cast<unsigned>(grid)), dim3(kBlock), 0, stream, n, c0, w, d.a,
                            d.inv + k * kDcTile * kDcTile, x, d.y, d.fail);
     }
