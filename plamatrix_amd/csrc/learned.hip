@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pm_common.hpp"
@@ -932,6 +933,8 @@ struct pm_fsai_graph
     DeviceBuffer buf;      // graph arrays
     DeviceBuffer act;      // activations and gradients
     DeviceBuffer parts;    // per-workgroup partials
+    DeviceBuffer tmp;      // graph-build counts and scan scratch
+    int device = 0;
     float* dinv = nullptr;
     float* s = nullptr;
     __bf16* x = nullptr;
@@ -1013,21 +1016,16 @@ int graph_build(pm_fsai_graph* gr, const pm_csr* a, hipStream_t stream)
     // temporaries: per-row counts, their scans and the scan scratch; the
     // key/value arrays of the sorted transpose only when A's pattern is not
     // symmetric
-    DeviceBuffer tmp, sort_tmp;
+    DeviceBuffer sort_tmp;
     struct Release
     {
-        DeviceBuffer& a;
         DeviceBuffer& b;
-        ~Release()
-        {
-            a.release();
-            b.release();
-        }
-    } rel{tmp, sort_tmp};
+        ~Release() { b.release(); }
+    } rel{sort_tmp};
     const int64_t cap = std::max<int64_t>(a->nnz, 1);
     const size_t tbytes = sizeof(unsigned) * (n + 2) * 3 + sizeof(unsigned) * scan_scratch_words(n + 2) + 4096;
-    PM_TRY(tmp.reserve(tbytes));
-    unsigned* counts = tmp.as<unsigned>();
+    PM_TRY(gr->tmp.reserve(tbytes));
+    unsigned* counts = gr->tmp.as<unsigned>();
     unsigned* lro_u = counts + (n + 2);
     unsigned* ucounts = lro_u + (n + 2);
     unsigned* scratch = ucounts + (n + 2);
@@ -1341,6 +1339,62 @@ int pm_fsai_net_get_parameters(const pm_fsai_net* net, float* host, int64_t coun
     return PM_OK;
 }
 
+namespace
+{
+
+// Device storage of destroyed graphs, kept for the next graph on the same
+// device: a caller that builds a graph per system (a new sparsity pattern for
+// every solve) would otherwise pay hipMalloc and hipFree -- whose device-wide
+// synchronisation costs more than a small system's whole graph build -- for
+// each of a graph's four buffers.  At most kPooledGraphs sets of at most
+// kPooledGraphBytes each are kept, until the process ends.
+constexpr size_t kPooledGraphs = 4;
+constexpr size_t kPooledGraphBytes = size_t(1) << 30;
+
+struct GraphStorage
+{
+    int device = 0;
+    pm::DeviceBuffer buf, act, parts, tmp;
+};
+
+std::mutex g_graph_pool_mutex;
+std::vector<GraphStorage> g_graph_pool;
+
+void take_pooled_storage(pm_fsai_graph* gr)
+{
+    std::lock_guard<std::mutex> lock(g_graph_pool_mutex);
+    for (size_t i = 0; i < g_graph_pool.size(); ++i)
+        if (g_graph_pool[i].device == gr->device)
+        {
+            gr->buf = g_graph_pool[i].buf;
+            gr->act = g_graph_pool[i].act;
+            gr->parts = g_graph_pool[i].parts;
+            gr->tmp = g_graph_pool[i].tmp;
+            g_graph_pool.erase(g_graph_pool.begin() + static_cast<std::ptrdiff_t>(i));
+            return;
+        }
+}
+
+void release_storage(pm_fsai_graph* gr)
+{
+    const size_t total = gr->buf.bytes + gr->act.bytes + gr->parts.bytes + gr->tmp.bytes;
+    {
+        std::lock_guard<std::mutex> lock(g_graph_pool_mutex);
+        if (g_graph_pool.size() < kPooledGraphs && total <= kPooledGraphBytes)
+        {
+            g_graph_pool.push_back(GraphStorage{gr->device, gr->buf, gr->act, gr->parts, gr->tmp});
+            gr->buf = gr->act = gr->parts = gr->tmp = pm::DeviceBuffer{};
+            return;
+        }
+    }
+    gr->buf.release();
+    gr->act.release();
+    gr->parts.release();
+    gr->tmp.release();
+}
+
+} // namespace
+
 int pm_fsai_graph_create(const pm_csr* a, pm_fsai_graph** out, hipStream_t stream)
 {
     PM_TRY(check_csr(a, "learned preconditioner"));
@@ -1351,12 +1405,18 @@ int pm_fsai_graph_create(const pm_csr* a, pm_fsai_graph** out, hipStream_t strea
     if (a->rows < 1) return fail(PM_INVALID_ARGUMENT, "learned preconditioner: empty matrix");
     auto* gr = new (std::nothrow) pm_fsai_graph();
     if (!gr) return fail(PM_RUNTIME_ERROR, "pm_fsai_graph_create: out of host memory");
+    if (hipGetDevice(&gr->device) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        delete gr;
+        return fail(PM_RUNTIME_ERROR, "pm_fsai_graph_create: no current HIP device");
+    }
+    take_pooled_storage(gr);
     const int st = graph_build(gr, a, stream);
     if (st != PM_OK)
     {
-        gr->buf.release();
-        gr->act.release();
-        gr->parts.release();
+        (void)hipStreamSynchronize(stream);
+        release_storage(gr);
         delete gr;
         return st;
     }
@@ -1368,9 +1428,7 @@ int pm_fsai_graph_destroy(pm_fsai_graph* gr)
 {
     if (!gr) return PM_OK;
     (void)hipStreamSynchronize(gr->stream);
-    gr->buf.release();
-    gr->act.release();
-    gr->parts.release();
+    release_storage(gr);
     delete gr;
     return PM_OK;
 }

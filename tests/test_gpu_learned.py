@@ -153,6 +153,31 @@ def test_transpose_factor_pattern(dev, kind):
     assert np.all(ci <= rows) and np.count_nonzero(ci == rows) == h.rows
 
 
+def test_graphs_reuse_storage(dev):
+    # destroyed graphs hand their device storage to the next graph: a graph
+    # built on reused storage (after graphs of other sizes) gives the bits of
+    # the first graph of the same matrix
+    import gc
+
+    p = perturbed_parameters(8)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = p
+    first = {}
+    for dims in [(9, 8, 7), (13, 11, 9), (6, 5, 4), (13, 11, 9), (9, 8, 7)]:
+        h = system(*dims, seed=30 + dims[0])
+        a = capi.to_device_csr(h, compact=True)
+        graph = capi.FsaiGraph(a)
+        capi.fsai_forward(net, graph)
+        g, gt = graph.factors()
+        got = tuple(x.copy() for x in capi.csr_arrays(g) + capi.csr_arrays(gt))
+        if dims in first:
+            assert all(np.array_equal(u, v) for u, v in zip(first[dims], got)), dims
+        else:
+            first[dims] = got
+        del graph
+        gc.collect()
+
+
 def test_loss_and_gradient_match_oracle(dev):
     h = system(8, 7, 6, seed=8)
     a = capi.to_device_csr(h, compact=True)
