@@ -320,6 +320,9 @@ int pm_fsai_net_get_parameters(const pm_fsai_net* net, float* parameters, int64_
  * node features, normalised couplings, lower and upper patterns, storage for
  * G and the activations.  The matrix must outlive the graph. */
 int pm_fsai_graph_create(const pm_csr* a, pm_fsai_graph** graph, hipStream_t stream);
+/* Waits for the graph's stream, then keeps its device buffers for the next
+ * graph created on the same device (at most 4 sets of at most 1 GiB each,
+ * held until the process ends); larger or surplus buffers are freed. */
 int pm_fsai_graph_destroy(pm_fsai_graph* graph);
 /* Evaluate the network: G and G^T values in the graph. */
 int pm_fsai_forward(pm_fsai_net* net, pm_fsai_graph* graph, hipStream_t stream);
