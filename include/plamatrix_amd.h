@@ -320,10 +320,19 @@ int pm_fsai_net_get_parameters(const pm_fsai_net* net, float* parameters, int64_
  * node features, normalised couplings, lower and upper patterns, storage for
  * G and the activations.  The matrix must outlive the graph. */
 int pm_fsai_graph_create(const pm_csr* a, pm_fsai_graph** graph, hipStream_t stream);
-/* Waits for the graph's stream, then keeps its device buffers for the next
- * graph created on the same device (at most 4 sets of at most 1 GiB each,
- * held until the process ends); larger or surplus buffers are freed. */
+/* Waits for every stream of the graph's device (forward, training and
+ * factored solves may use the graph on any stream), then keeps its device
+ * buffers for the next graph created on the same device: at most 4 sets of at
+ * most 1 GiB each, held until pm_fsai_graph_pool_release or the end of the
+ * process; larger or surplus buffers are freed.  G / G^T descriptors from
+ * pm_fsai_factors are invalid once their graph is destroyed.  The environment
+ * variable PLAMATRIX_AMD_GRAPH_POOL=0 frees every destroyed graph's storage
+ * instead. */
 int pm_fsai_graph_destroy(pm_fsai_graph* graph);
+/* Frees the pooled storage of destroyed graphs on the current device. */
+int pm_fsai_graph_pool_release(void);
+/* Bytes the pool holds on the current device. */
+int pm_fsai_graph_pool_bytes(int64_t* bytes);
 /* Evaluate the network: G and G^T values in the graph. */
 int pm_fsai_forward(pm_fsai_net* net, pm_fsai_graph* graph, hipStream_t stream);
 /* CSR descriptors of G (lower) and G^T (upper), owned by the graph. */

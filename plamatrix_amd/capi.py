@@ -111,6 +111,7 @@ EXPORTS = [
     "pm_batch_layout", "pm_batch_pcg", "pm_batch_pcg_refine", "pm_pcg_factored", "pm_pcg_refine_factored",
     "pm_fsai_parameter_count", "pm_fsai_initial_parameters", "pm_fsai_net_create", "pm_fsai_net_destroy",
     "pm_fsai_net_set_parameters", "pm_fsai_net_get_parameters", "pm_fsai_graph_create", "pm_fsai_graph_destroy",
+    "pm_fsai_graph_pool_release", "pm_fsai_graph_pool_bytes",
     "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step", "pm_dense_cholesky_solve",
 ]
 

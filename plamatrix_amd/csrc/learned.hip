@@ -1347,9 +1347,19 @@ namespace
 // every solve) would otherwise pay hipMalloc and hipFree -- whose device-wide
 // synchronisation costs more than a small system's whole graph build -- for
 // each of a graph's four buffers.  At most kPooledGraphs sets of at most
-// kPooledGraphBytes each are kept, until the process ends.
+// kPooledGraphBytes each are kept, until pm_fsai_graph_pool_release or the
+// end of the process; PLAMATRIX_AMD_GRAPH_POOL=0 turns the pool off.
 constexpr size_t kPooledGraphs = 4;
 constexpr size_t kPooledGraphBytes = size_t(1) << 30;
+
+bool pool_enabled()
+{
+    static const bool on = [] {
+        const char* env = std::getenv("PLAMATRIX_AMD_GRAPH_POOL");
+        return env == nullptr || env[0] != '0';
+    }();
+    return on;
+}
 
 struct GraphStorage
 {
@@ -1378,6 +1388,7 @@ void take_pooled_storage(pm_fsai_graph* gr)
 void release_storage(pm_fsai_graph* gr)
 {
     const size_t total = gr->buf.bytes + gr->act.bytes + gr->parts.bytes + gr->tmp.bytes;
+    if (pool_enabled())
     {
         std::lock_guard<std::mutex> lock(g_graph_pool_mutex);
         if (g_graph_pool.size() < kPooledGraphs && total <= kPooledGraphBytes)
@@ -1427,9 +1438,65 @@ int pm_fsai_graph_create(const pm_csr* a, pm_fsai_graph** out, hipStream_t strea
 int pm_fsai_graph_destroy(pm_fsai_graph* gr)
 {
     if (!gr) return PM_OK;
-    (void)hipStreamSynchronize(gr->stream);
+    // forward, training and factored solves may have used this graph's
+    // storage on any stream; the next graph may reuse it at once, so every
+    // stream of the device must have drained first
+    int current = 0;
+    const bool switched = hipGetDevice(&current) == hipSuccess && current != gr->device &&
+                          hipSetDevice(gr->device) == hipSuccess;
+    (void)hipDeviceSynchronize();
     release_storage(gr);
+    if (switched) (void)hipSetDevice(current);
     delete gr;
+    return PM_OK;
+}
+
+// Frees the storage that destroyed graphs left for reuse on the current device.
+int pm_fsai_graph_pool_release(void)
+{
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return fail(PM_RUNTIME_ERROR, "pm_fsai_graph_pool_release: no current HIP device");
+    }
+    std::vector<GraphStorage> mine;
+    {
+        std::lock_guard<std::mutex> lock(g_graph_pool_mutex);
+        for (size_t i = 0; i < g_graph_pool.size();)
+            if (g_graph_pool[i].device == device)
+            {
+                mine.push_back(g_graph_pool[i]);
+                g_graph_pool.erase(g_graph_pool.begin() + static_cast<std::ptrdiff_t>(i));
+            }
+            else ++i;
+    }
+    for (GraphStorage& st : mine)
+    {
+        st.buf.release();
+        st.act.release();
+        st.parts.release();
+        st.tmp.release();
+    }
+    return PM_OK;
+}
+
+// Bytes of device storage the pool holds on the current device.
+int pm_fsai_graph_pool_bytes(int64_t* bytes)
+{
+    if (!bytes) return fail(PM_INVALID_ARGUMENT, "pm_fsai_graph_pool_bytes: null output");
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return fail(PM_RUNTIME_ERROR, "pm_fsai_graph_pool_bytes: no current HIP device");
+    }
+    std::lock_guard<std::mutex> lock(g_graph_pool_mutex);
+    int64_t total = 0;
+    for (const GraphStorage& st : g_graph_pool)
+        if (st.device == device)
+            total += static_cast<int64_t>(st.buf.bytes + st.act.bytes + st.parts.bytes + st.tmp.bytes);
+    *bytes = total;
     return PM_OK;
 }
 

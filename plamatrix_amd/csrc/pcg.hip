@@ -997,27 +997,41 @@ bool dia_wanted(const pm_csr* a)
     return use_compact(a) && a->rows >= 65536 && a->nnz <= static_cast<int64_t>(kMaxDia) * a->rows;
 }
 
+// Every operand address and size a cached plan's kernel arguments capture.
+void append_matrix_key(std::string& key, const pm_csr* m)
+{
+    char buf[160];
+    if (m == nullptr)
+    {
+        key += "|-";
+        return;
+    }
+    std::snprintf(buf, sizeof(buf), "|%p|%p|%p|%p|%p|%lld|%lld|%lld|%d", (const void*)m->row_offsets,
+                  (const void*)m->row_offsets32, (const void*)m->col_indices, (const void*)m->col_indices32, m->values,
+                  static_cast<long long>(m->rows), static_cast<long long>(m->cols), static_cast<long long>(m->nnz),
+                  m->dtype);
+    key += buf;
+}
+
 std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t count, int refresh)
 {
     // workspace addresses are part of the key: a grown workspace must never
-    // replay a plan that still points at its old buffers
-    char buf[768];
+    // replay a plan that still points at its old buffers; so are all three
+    // matrices' arrays, since a caller may move any one of them between solves
+    char buf[512];
     const unsigned long long dia_sig =
         dia_signature(pb.dia) ^ (dia_signature(pb.dia_g) * 31ull) ^ (dia_signature(pb.dia_gt) * 131ull);
-    std::snprintf(buf, sizeof(buf),
-                  "%s|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%d|%lld|%lld|%d|%lld|%d|%p|%p|%p|%p|%d|%d|%llx|%p|%p",
-                  kind,
-                  pb.r, static_cast<void*>(pb.st), static_cast<void*>(pb.partials),
-                  (const void*)pb.a->row_offsets, (const void*)pb.a->row_offsets32, (const void*)pb.a->col_indices,
-                  (const void*)pb.a->col_indices32, pb.a->values, pb.rhs, pb.x, pb.inv_blocks,
-                  static_cast<long long>(pb.a->rows), static_cast<long long>(pb.a->nnz), pb.a->dtype, pb.pre,
+    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%d|%lld|%lld|%d|%lld|%d|%d|%d|%llx|%p|%p", kind, pb.r,
+                  static_cast<void*>(pb.st), static_cast<void*>(pb.partials), pb.rhs, pb.x, pb.inv_blocks, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
-                  static_cast<long long>(pb.bsize), (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0),
-                  pb.gf ? pb.gf->values : nullptr,
-                  pb.gf ? (const void*)pb.gf->row_offsets32 : nullptr, pb.gtf ? pb.gtf->values : nullptr,
-                  pb.gtf ? (const void*)pb.gtf->row_offsets32 : nullptr, pb.dia.active ? 1 : 0, pb.dia.args.ndiag,
-                  dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask));
-    return buf;
+                  static_cast<long long>(pb.bsize),
+                  (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0),
+                  pb.dia.active ? 1 : 0, pb.dia.args.ndiag, dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask));
+    std::string key(buf);
+    append_matrix_key(key, pb.a);
+    append_matrix_key(key, pb.gf);
+    append_matrix_key(key, pb.gtf);
+    return key;
 }
 
 template <typename S>

@@ -1,8 +1,9 @@
 // SpMV autotuner: times the wave-stream kernel and every lanes-per-row variant
 // on the caller's matrix
-// with HIP events and keeps the fastest, in memory and in a small text cache
-// on disk (PLAMATRIX_AMD_TUNE_CACHE, default ~/.cache/plamatrix_amd/spmv.txt)
-// so later processes start tuned.
+// with HIP events and keeps the fastest in memory.  Only when the environment
+// variable PLAMATRIX_AMD_TUNE_CACHE names a file is the table also read from
+// and written to that file, so later processes start tuned; without it the
+// library touches no file.
 #include <cstdio>
 #include <cstdlib>
 #include <map>
@@ -32,27 +33,34 @@ Key key_of(const pm_csr* a)
 
 std::string cache_path()
 {
-    if (const char* p = std::getenv("PLAMATRIX_AMD_TUNE_CACHE")) return p;
-    const char* home = std::getenv("HOME");
-    return std::string(home ? home : "/tmp") + "/.cache/plamatrix_amd/spmv.txt";
+    const char* p = std::getenv("PLAMATRIX_AMD_TUNE_CACHE");
+    return p != nullptr ? std::string(p) : std::string();
 }
+
+bool valid_lanes(int lanes) { return lanes >= 0 && lanes <= 64 && (lanes & (lanes - 1)) == 0; }
 
 void load_locked()
 {
     if (g_loaded) return;
     g_loaded = true;
-    FILE* f = std::fopen(cache_path().c_str(), "r");
+    const std::string path = cache_path();
+    if (path.empty()) return;
+    FILE* f = std::fopen(path.c_str(), "r");
     if (!f) return;
     long long rows, nnz;
     int dtype, width, lanes;
+    // entries that no launch could use (a stale or damaged file) are dropped
     while (std::fscanf(f, "%lld %lld %d %d %d", &rows, &nnz, &dtype, &width, &lanes) == 5)
-        g_table[Key(rows, nnz, dtype, width)] = lanes;
+        if (valid_lanes(lanes) && rows >= 0 && nnz >= 0 && (dtype == PM_F32 || dtype == PM_F64) &&
+            (width == 32 || width == 64))
+            g_table[Key(rows, nnz, dtype, width)] = lanes;
     std::fclose(f);
 }
 
 void save_locked()
 {
     const std::string path = cache_path();
+    if (path.empty()) return;
     const size_t slash = path.rfind('/');
     if (slash != std::string::npos)
     {
@@ -138,7 +146,7 @@ int pm_autotune_spmv(const pm_csr* a, const void* x, void* y, int32_t repeats, i
 int pm_tune_record(const pm_csr* a, int32_t lanes)
 {
     PM_TRY(check_csr(a, "tune record"));
-    if (lanes < 0 || lanes > 64 || (lanes & (lanes - 1)) != 0)
+    if (!valid_lanes(lanes))
         return fail(PM_INVALID_ARGUMENT, "lanes per row must be 0 (wave-stream) or a power of two in [1, 64], got %d",
                     lanes);
     record_lanes(a, lanes);

@@ -265,3 +265,82 @@ def test_training_improves_on_jacobi(dev, ws):
     assert rj.converged and rl.converged
     assert rl.iterations < rj.iterations, (rl.iterations, rj.iterations)
     assert capi.true_relres(a, b, xl, ws) <= 5e-5
+
+
+def _factor_copy(d, dev):
+    """Caller-owned device copies of a factor descriptor's arrays."""
+    ro, ci, vals = capi.csr_arrays(d)
+    t = dict(ro=torch.from_numpy(ro.astype(np.int32)).to(dev), ci=torch.from_numpy(ci.astype(np.int32)).to(dev),
+             ro64=torch.from_numpy(ro).to(dev), ci64=torch.from_numpy(ci).to(dev),
+             v=torch.from_numpy(vals).to(dev))
+    return t
+
+
+def _descriptor(t, rows):
+    return capi.pm_csr(rows, rows, int(t["ci"].numel()), t["ro64"].data_ptr(), t["ci64"].data_ptr(),
+                       t["v"].data_ptr(), capi.dtype_code(t["v"]), t["ro"].data_ptr(), t["ci"].data_ptr())
+
+
+def _factored(a, b, x, g, gt, ws):
+    import ctypes
+
+    d = a.descriptor()
+    o = capi.options(relative_tolerance=1e-6, max_iterations=2000)
+    p = capi.policy()
+    r = capi.pm_solver_report()
+    capi.check(capi.lib().pm_pcg_factored(ctypes.byref(d), capi._ptr(b), capi._ptr(x), ctypes.byref(g),
+                                          ctypes.byref(gt), ctypes.byref(o), ctypes.byref(p), ctypes.byref(r),
+                                          ws.handle, capi.current_stream()))
+    torch.cuda.synchronize()
+    return capi._report(r)
+
+
+def test_cached_plan_follows_moved_factor_indices(dev):
+    # a second solve whose factors differ from the first only in the address of
+    # their column-index arrays must not replay the first solve's launch plan;
+    # the first arrays are overwritten with in-range zeros, so a stale plan would
+    # give a different (wrong) preconditioner rather than a fault
+    h = system(12, 10, 9, seed=41)
+    a = capi.to_device_csr(h, compact=True)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = np.load(WEIGHTS)
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    g0, gt0 = graph.factors()
+    fg, fgt = _factor_copy(g0, dev), _factor_copy(gt0, dev)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=42)).to(dev)
+    ws = capi.Workspace()
+    x1 = torch.zeros_like(b)
+    r1 = _factored(a, b, x1, _descriptor(fg, h.rows), _descriptor(fgt, h.rows), ws)
+    moved_g, moved_gt = dict(fg), dict(fgt)
+    moved_g["ci"], moved_gt["ci"] = fg["ci"].clone(), fgt["ci"].clone()
+    fg["ci"].zero_()
+    fgt["ci"].zero_()
+    x2 = torch.zeros_like(b)
+    r2 = _factored(a, b, x2, _descriptor(moved_g, h.rows), _descriptor(moved_gt, h.rows), ws)
+    fresh = capi.Workspace()
+    x3 = torch.zeros_like(b)
+    r3 = _factored(a, b, x3, _descriptor(moved_g, h.rows), _descriptor(moved_gt, h.rows), fresh)
+    assert r1.converged and r2.converged and r3.converged
+    assert (r1.iterations, r1.final_residual) == (r2.iterations, r2.final_residual) == (r3.iterations,
+                                                                                         r3.final_residual)
+    assert torch.equal(x1, x2) and torch.equal(x2, x3)
+
+
+def test_graph_storage_pool_release(dev):
+    import ctypes
+    import gc
+
+    lib = capi.lib()
+    capi.check(lib.pm_fsai_graph_pool_release())
+    held = ctypes.c_int64(-1)
+    capi.check(lib.pm_fsai_graph_pool_bytes(ctypes.byref(held)))
+    assert held.value == 0
+    graph = capi.FsaiGraph(capi.to_device_csr(system(10, 9, 8, seed=43), compact=True))
+    del graph
+    gc.collect()
+    capi.check(lib.pm_fsai_graph_pool_bytes(ctypes.byref(held)))
+    assert held.value > 0
+    capi.check(lib.pm_fsai_graph_pool_release())
+    capi.check(lib.pm_fsai_graph_pool_bytes(ctypes.byref(held)))
+    assert held.value == 0

@@ -92,3 +92,80 @@ def test_factor_gives_spd_preconditioner():
     G[rows, cols] = g
     assert np.all(np.triu(G, 1) == 0) and np.all(np.diag(G) > 0)
     assert np.linalg.eigvalsh(G.T @ G).min() > 0
+
+
+def _diag_of(ro, ci, vals):
+    n = len(ro) - 1
+    return np.array([vals[ro[i]:ro[i + 1]][ci[ro[i]:ro[i + 1]] == i].sum() for i in range(n)])
+
+
+def test_factored_pcg_with_jacobi_factor_reproduces_oracle_pcg():
+    """G = D^-1/2 makes G^T G = D^-1, so factored_pcg must follow the oracle's
+    Jacobi-PCG (itself bitwise equal to the reference build) iterate by
+    iterate: the same iteration count, and every truncated run within 1e-12."""
+    import oracle
+
+    a = generators.diffusion3d_box_csr(9, 8, 7, seed=5, dtype=np.float64)
+    ro, ci, vals = a.row_offsets, a.col_indices, a.values
+    n = len(ro) - 1
+    d = _diag_of(ro, ci, vals)
+    idx = np.arange(n)
+    g = 1.0 / np.sqrt(d)
+    b = generators.rhs_uniform(n, seed=9)
+    csr = oracle.CSR(n, n, ro, ci, vals)
+    x_ref, rep = oracle.pcg(csr, b, relative_tolerance=1e-10, max_iterations=500)
+    x, its = fsai.factored_pcg(ro, ci, vals, idx, idx, g, b, relative_tolerance=1e-10, max_iterations=500)
+    assert rep.converged and its == rep.iterations
+    assert np.linalg.norm(x - x_ref) <= 1e-12 * np.linalg.norm(x_ref)
+    for k in (1, 2, 3, 5, 8, 13):
+        xk_ref, _ = oracle.pcg(csr, b, relative_tolerance=1e-30, max_iterations=k)
+        xk, itk = fsai.factored_pcg(ro, ci, vals, idx, idx, g, b, relative_tolerance=1e-30, max_iterations=k)
+        assert itk == k
+        assert np.linalg.norm(xk - xk_ref) <= 1e-12 * np.linalg.norm(xk_ref), k
+
+
+def test_factored_pcg_with_inverse_cholesky_factor_converges_in_one_step():
+    """G = L^-1 with A = L L^T gives G^T G = A^-1 exactly: PCG stops after
+    one iteration with x = A^-1 b.  The transposed product G G^T is not
+    A^-1, and the same run then needs many iterations."""
+    import scipy.linalg as sla
+
+    a = generators.diffusion3d_box_csr(5, 4, 3, seed=6, dtype=np.float64)
+    ro, ci, vals = a.row_offsets, a.col_indices, a.values
+    n = len(ro) - 1
+    A = np.zeros((n, n))
+    A[np.repeat(np.arange(n), np.diff(ro)), ci] = vals
+    L = np.linalg.cholesky(A)
+    G = sla.solve_triangular(L, np.eye(n), lower=True)
+    rows, cols = np.nonzero(np.tril(np.ones((n, n))))
+    b = generators.rhs_uniform(n, seed=2)
+    x, its = fsai.factored_pcg(ro, ci, vals, rows, cols, G[rows, cols], b, relative_tolerance=1e-9,
+                               max_iterations=200)
+    assert its == 1
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=1e-10, atol=1e-12)
+    # sensitivity: the transposed factor (upper-triangular, applied as G G^T) is not the inverse
+    _, its_t = fsai.factored_pcg(ro, ci, vals, cols, rows, G[rows, cols], b, relative_tolerance=1e-9,
+                                 max_iterations=200)
+    assert its_t > 3
+
+
+def test_factored_pcg_solution_matches_direct_solve():
+    """Any lower G with a positive diagonal gives an SPD M^-1, so PCG
+    converges to A^-1 b: compared with scipy's sparse direct solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    a = generators.diffusion3d_box_csr(8, 7, 6, seed=8, dtype=np.float64)
+    ro, ci, vals = a.row_offsets, a.col_indices, a.values
+    n = len(ro) - 1
+    rows = np.repeat(np.arange(n), np.diff(ro))
+    lower = ci <= rows
+    rng = np.random.default_rng(4)
+    g = np.where(ci[lower] == rows[lower], 0.5 + rng.random(lower.sum()), 0.2 * rng.standard_normal(lower.sum()))
+    b = generators.rhs_uniform(n, seed=3)
+    x, its = fsai.factored_pcg(ro, ci, vals, rows[lower], ci[lower], g, b, relative_tolerance=1e-12,
+                               max_iterations=2000)
+    assert its < 2000
+    A = sp.csr_matrix((vals, ci, ro), shape=(n, n)).tocsc()
+    x_direct = spla.spsolve(A, b)
+    assert np.linalg.norm(x - x_direct) <= 1e-9 * np.linalg.norm(x_direct)
