@@ -1,10 +1,12 @@
 // Block normal equations and the damped Schur-complement solve
 // (optimization/block_schur.h:22-351 and robust_loss.h:27-51 of the
 // reference).  BlockNormalEquations accumulates on the host with the
-// reference's arithmetic and cross-block numbering; solveDampedSchurComplement
-// uploads the blocks once and runs the whole solve in HBM: damping, eliminated
-// inverses, reduced right-hand side, Schur CSR on the cached device pattern,
-// block-Jacobi PCG and back-substitution (pm_schur_solve).
+// reference's arithmetic and cross-block numbering, in float or double;
+// solveDampedSchurComplement uploads the blocks once and runs the whole solve
+// in HBM in FP64: damping, eliminated inverses, reduced right-hand side, Schur
+// CSR on the cached device pattern, block-Jacobi PCG (or the dense Cholesky)
+// and back-substitution (pm_schur_solve).  Float equations are widened to
+// FP64 for the device solve and the steps rounded back to float.
 #pragma once
 
 #include <chrono>
@@ -47,6 +49,14 @@ RobustLossEvaluation<Scalar> evaluateHuberLoss(Scalar squared_residual_norm, Sca
     return {delta * norm - Scalar(0.5) * delta * delta, delta / norm};
 }
 
+/// Linear solver of the reduced system.  Every served backend runs on the
+/// device:
+///   Cpu       block-Jacobi PCG with the reference Cpu backend's stopping rule
+///             ||r|| <= abs + rel ||r0|| (block_schur_cpu_solver.h:34-35);
+///   Hip       block-Jacobi PCG with the accelerated rule max(abs, rel ||r0||)
+///             (the reference's Cuda / OpenCl semantics);
+///   DenseCpu, DenseHip  dense FP64 Cholesky of the reduced system.
+/// Cuda and OpenCl are rejected (std::invalid_argument).
 enum class SchurComplementLinearBackend
 {
     Cpu,
@@ -54,29 +64,31 @@ enum class SchurComplementLinearBackend
     Cuda,
     OpenCl,
     Hip,
-    /// Dense FP64 Cholesky of the reduced system on the device; DenseCpu
-    /// requests are served by it as well.
     DenseHip,
 };
 
 template <typename Scalar>
 struct SchurComplementSolverOptions
 {
-    /// Hip (block-Jacobi PCG) and DenseHip / DenseCpu (dense Cholesky, both
-    /// on the device) are served; Cpu, Cuda and OpenCl are rejected.
-    SchurComplementLinearBackend linearBackend = SchurComplementLinearBackend::Hip;
+    SchurComplementLinearBackend linearBackend = SchurComplementLinearBackend::Cpu;
     int deviceIndex = -1;
     int maxIterations = 100;
     Scalar relativeTolerance = Scalar(1e-8);
     Scalar absoluteTolerance = Scalar(1e-12);
     Scalar minimumDiagonal = Scalar(1e-12);
     bool useInitialGuess = false;
+    /// Seed the FP64 block-PCG (Hip backend, double equations) with an FP32
+    /// block-PCG pass, used when it converges (block_schur.h:43-44).
+    bool useMixedPrecision = false;
+    /// Kept for source compatibility (block_schur.h:45-46): the reduced
+    /// values are always assembled and consumed in HBM here.
+    bool schurValuesOnDevice = true;
 };
 
 template <typename Scalar>
 struct SchurComplementSolverReport
 {
-    SchurComplementLinearBackend linearBackend = SchurComplementLinearBackend::Hip;
+    SchurComplementLinearBackend linearBackend = SchurComplementLinearBackend::Cpu;
     bool converged = false;
     int iterations = 0;
     Scalar initialResidualNorm = Scalar(0);
@@ -158,7 +170,8 @@ SchurComplementSolverReport<Scalar> solveDampedSchurComplement(const BlockNormal
 template <typename Scalar>
 class BlockNormalEquations
 {
-    static_assert(std::is_same_v<Scalar, double>, "the device Schur path is FP64");
+    static_assert(std::is_same_v<Scalar, double> || std::is_same_v<Scalar, float>,
+                  "BlockNormalEquations holds float or double blocks");
 
 public:
     BlockNormalEquations(Index primary_block_count, Index eliminated_block_count, Index primary_block_size,
@@ -396,6 +409,16 @@ struct Upload
 };
 } // namespace detail
 
+namespace detail
+{
+// FP64 device copy of a host vector of float or double.
+template <typename T>
+std::vector<double> widened(const std::vector<T>& v)
+{
+    return std::vector<double>(v.begin(), v.end());
+}
+} // namespace detail
+
 template <typename Scalar>
 SchurComplementSolverReport<Scalar> solveDampedSchurComplement(const BlockNormalEquations<Scalar>& eq, Scalar damping,
                                                                const SchurComplementSolverOptions<Scalar>& options,
@@ -403,25 +426,30 @@ SchurComplementSolverReport<Scalar> solveDampedSchurComplement(const BlockNormal
                                                                std::vector<Scalar>* primary_step,
                                                                std::vector<Scalar>* eliminated_step)
 {
+    using Backend = SchurComplementLinearBackend;
     if (!primary_step || !eliminated_step || primary_step == eliminated_step)
         throw std::invalid_argument("solveDampedSchurComplement: output vectors must be non-null and distinct");
-    const bool dense = options.linearBackend == SchurComplementLinearBackend::DenseHip ||
-                       options.linearBackend == SchurComplementLinearBackend::DenseCpu;
-    if (options.linearBackend != SchurComplementLinearBackend::Hip && !dense)
+    const Backend backend = options.linearBackend;
+    const bool dense = backend == Backend::DenseHip || backend == Backend::DenseCpu;
+    if (backend != Backend::Hip && backend != Backend::Cpu && !dense)
         throw std::invalid_argument(
-            "solveDampedSchurComplement: this library serves the Hip, DenseHip and DenseCpu backends");
+            "solveDampedSchurComplement: this library serves the Cpu, DenseCpu, Hip and DenseHip backends");
     if (options.deviceIndex >= 0) detail::checkHip(hipSetDevice(options.deviceIndex), "hipSetDevice");
     const std::size_t pd = static_cast<std::size_t>(eq._pc * eq._ps), ed = static_cast<std::size_t>(eq._ec * eq._es);
     const bool guess = options.useInitialGuess && primary_step->size() == pd;
     if (!guess) primary_step->assign(pd, Scalar(0));
     eliminated_step->assign(ed, Scalar(0));
 
-    detail::Upload<Scalar> B(eq._B), C(eq._C), gp(eq._gp), ge(eq._ge), W(eq._crossValues), P(eq._pcValues);
+    // the device solve is FP64 whatever Scalar is
+    detail::Upload<double> B(detail::widened(eq._B)), C(detail::widened(eq._C)), gp(detail::widened(eq._gp)),
+        ge(detail::widened(eq._ge)), W(detail::widened(eq._crossValues)), P(detail::widened(eq._pcValues));
     detail::Upload<Index> cp(eq._crossPrimary), ce(eq._crossEliminated), pr(eq._pcRows), pc(eq._pcCols);
-    detail::Upload<Scalar> dp(pd), de(ed);
+    detail::Upload<double> dp(pd), de(ed);
     if (guess && pd)
-        detail::checkHip(hipMemcpy(dp.ptr, primary_step->data(), sizeof(Scalar) * pd, hipMemcpyHostToDevice),
-                         "hipMemcpy");
+    {
+        const std::vector<double> start = detail::widened(*primary_step);
+        detail::checkHip(hipMemcpy(dp.ptr, start.data(), sizeof(double) * pd, hipMemcpyHostToDevice), "hipMemcpy");
+    }
     pm_normal_equations d{};
     d.primary_count = eq._pc;
     d.eliminated_count = eq._ec;
@@ -442,28 +470,35 @@ SchurComplementSolverReport<Scalar> solveDampedSchurComplement(const BlockNormal
     pm_schur_options o{};
     o.max_iterations = options.maxIterations;
     o.use_initial_guess = guess ? 1 : 0;
-    o.relative_tolerance = options.relativeTolerance;
-    o.absolute_tolerance = options.absoluteTolerance;
-    o.minimum_diagonal = options.minimumDiagonal;
+    o.relative_tolerance = static_cast<double>(options.relativeTolerance);
+    o.absolute_tolerance = static_cast<double>(options.absoluteTolerance);
+    o.minimum_diagonal = static_cast<double>(options.minimumDiagonal);
     o.linear_backend = dense ? PM_SCHUR_DENSE_CHOLESKY : PM_SCHUR_BLOCK_PCG;
+    o.flags = (backend == Backend::Cpu ? PM_SCHUR_TOLERANCE_SUM : 0) |
+              (backend == Backend::Hip && options.useMixedPrecision && std::is_same_v<Scalar, double>
+                   ? PM_SCHUR_MIXED_PRECISION
+                   : 0);
     pm_schur_report r{};
     const auto start = std::chrono::steady_clock::now();
-    detail::check(pm_schur_solve(workspace.schur(), &d, damping, &o, dp.ptr, de.ptr, &r, workspace.solver(), nullptr));
+    detail::check(pm_schur_solve(workspace.schur(), &d, static_cast<double>(damping), &o, dp.ptr, de.ptr, &r,
+                                 workspace.solver(), nullptr));
     detail::checkHip(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     const double solve_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
-    if (pd) detail::checkHip(hipMemcpy(primary_step->data(), dp.ptr, sizeof(Scalar) * pd, hipMemcpyDeviceToHost), "hipMemcpy");
-    if (ed)
-        detail::checkHip(hipMemcpy(eliminated_step->data(), de.ptr, sizeof(Scalar) * ed, hipMemcpyDeviceToHost),
-                         "hipMemcpy");
+    std::vector<double> host_p(pd), host_e(ed);
+    if (pd) detail::checkHip(hipMemcpy(host_p.data(), dp.ptr, sizeof(double) * pd, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (ed) detail::checkHip(hipMemcpy(host_e.data(), de.ptr, sizeof(double) * ed, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (std::size_t i = 0; i < pd; ++i) (*primary_step)[i] = static_cast<Scalar>(host_p[i]);
+    for (std::size_t i = 0; i < ed; ++i) (*eliminated_step)[i] = static_cast<Scalar>(host_e[i]);
     SchurComplementSolverReport<Scalar> report;
-    report.linearBackend = dense ? SchurComplementLinearBackend::DenseHip : SchurComplementLinearBackend::Hip;
+    report.linearBackend = backend;
     report.linearSolveSeconds = solve_seconds;
     report.converged = r.converged != 0;
     report.iterations = r.iterations;
-    report.initialResidualNorm = r.initial_residual_norm;
-    report.finalResidualNorm = r.final_residual_norm;
+    report.initialResidualNorm = static_cast<Scalar>(r.initial_residual_norm);
+    report.finalResidualNorm = static_cast<Scalar>(r.final_residual_norm);
     report.schurPatternReused = r.pattern_reused != 0;
     report.schurAssemblyOnDevice = r.assembly_on_device != 0;
+    report.mixedPrecisionUsed = r.mixed_precision_used != 0;
     report.message = r.message;
     hipDeviceProp_t prop{};
     int dev = 0;

@@ -1,8 +1,7 @@
 // COO triplet list (sparse/coo_matrix.h:13-74 of the reference).  Triplets
-// are kept on the host in insertion order; toCsr() converts them on the
-// device with the stable (row, col) sort and left fold of duplicates, which
-// is bit-identical to the CPU conversion, and returns the CSR on the requested
-// device.
+// are kept on the host in insertion order.  toCsr() of a GPU matrix converts
+// them on the device (stable (row, col) sort and left fold of duplicates,
+// bit-identical to the host conversion); a CPU matrix converts on the host.
 #pragma once
 
 #include <stdexcept>
@@ -39,19 +38,24 @@ public:
 
     CSRMatrix<Scalar, Dev> toCsr() const
     {
-        const Index n = nnz();
-        DenseMatrix<Index, Device::CPU> r(n, 1), c(n, 1);
-        DenseMatrix<Scalar, Device::CPU> v(n, 1);
-        for (Index i = 0; i < n; ++i)
+        if constexpr (Dev == Device::CPU)
         {
-            r.data()[i] = _rowIndices[static_cast<std::size_t>(i)];
-            c.data()[i] = _colIndices[static_cast<std::size_t>(i)];
-            v.data()[i] = _values[static_cast<std::size_t>(i)];
+            return cooToCsr<Scalar>(_rows, _cols, _rowIndices, _colIndices, _values);
         }
-        SparseOpsWorkspace ws;
-        auto gpu = cooToCsr<Scalar>(_rows, _cols, r.toGpu(), c.toGpu(), v.toGpu(), ws);
-        if constexpr (Dev == Device::GPU) return gpu;
-        else return gpu.toCpu();
+        else
+        {
+            const Index n = nnz();
+            DenseMatrix<Index, Device::CPU> r(n, 1), c(n, 1);
+            DenseMatrix<Scalar, Device::CPU> v(n, 1);
+            for (Index i = 0; i < n; ++i)
+            {
+                r.data()[i] = _rowIndices[static_cast<std::size_t>(i)];
+                c.data()[i] = _colIndices[static_cast<std::size_t>(i)];
+                v.data()[i] = _values[static_cast<std::size_t>(i)];
+            }
+            SparseOpsWorkspace ws;
+            return cooToCsr<Scalar>(_rows, _cols, r.toGpu(), c.toGpu(), v.toGpu(), ws);
+        }
     }
 
 private:

@@ -570,6 +570,13 @@ struct CSRMatrixAccess
     {
         m.completeAsyncWrite(trusted);
     }
+    /// A host matrix just written by a conversion that guarantees a valid
+    /// structure (the host cooToCsr).
+    template <typename Scalar>
+    static void markTrusted(CSRMatrix<Scalar, Device::CPU>& m) noexcept
+    {
+        m.resetTrust();
+    }
 };
 
 // Consumers' stream rules (sparse_ops.cu:74-97 of the reference).

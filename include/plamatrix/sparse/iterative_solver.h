@@ -162,6 +162,44 @@ void checkSystem(const CSRMatrix<Scalar, Device::GPU>& a, const Rhs& rhs, const 
 } // namespace detail
 
 /// Adaptively solve an SPD GPU system with conjugate gradients.
+namespace detail
+{
+template <typename Scalar>
+IterativeSolverReport hostSolve(const CSRMatrix<Scalar, Device::CPU>& matrix,
+                                const DenseMatrix<Scalar, Device::CPU>& rhs,
+                                DenseMatrix<Scalar, Device::CPU>& solution, const IterativeSolverOptions& options,
+                                bool preconditioned)
+{
+    host::checkSolverOptions(options.maxIterations, options.relativeTolerance, options.absoluteTolerance);
+    host::checkSystem(matrix.rows(), matrix.cols(), matrix.nnz(), matrix.rowOffsets(), matrix.colIndices(),
+                      matrix.values(), rhs.rows(), rhs.cols(), solution.rows(), solution.cols(), rhs.data(),
+                      static_cast<const Scalar*>(solution.data()));
+    const host::SolveOutcome o = host::conjugateGradients(
+        matrix.rows(), matrix.rowOffsets(), matrix.colIndices(), matrix.values(), rhs.data(), solution.data(),
+        options.maxIterations, options.relativeTolerance, options.absoluteTolerance,
+        preconditioned && options.useJacobiPreconditioner, options.requireConvergence);
+    return IterativeSolverReport{o.converged, o.iterations, o.initialResidual, o.finalResidual};
+}
+} // namespace detail
+
+/// Conjugate gradients on a host system (iterative_solver.h:94-99 of the
+/// reference): double-precision iteration, solution in/out.
+template <typename Scalar>
+IterativeSolverReport cg(const CSRMatrix<Scalar, Device::CPU>& matrix, const DenseMatrix<Scalar, Device::CPU>& rhs,
+                         DenseMatrix<Scalar, Device::CPU>& solution, const IterativeSolverOptions& options = {})
+{
+    return detail::hostSolve(matrix, rhs, solution, options, false);
+}
+
+/// Jacobi-preconditioned conjugate gradients on a host system
+/// (iterative_solver.h:101-108); useJacobiPreconditioner=false gives CG.
+template <typename Scalar>
+IterativeSolverReport pcg(const CSRMatrix<Scalar, Device::CPU>& matrix, const DenseMatrix<Scalar, Device::CPU>& rhs,
+                          DenseMatrix<Scalar, Device::CPU>& solution, const IterativeSolverOptions& options = {})
+{
+    return detail::hostSolve(matrix, rhs, solution, options, true);
+}
+
 template <typename Scalar>
 IterativeSolverReport cg(const CSRMatrix<Scalar, Device::GPU>& matrix, const DenseMatrix<Scalar, Device::GPU>& rhs,
                          DenseMatrix<Scalar, Device::GPU>& solution, IterativeSolverWorkspace<Scalar>& workspace,

@@ -11,6 +11,7 @@
 #include "plamatrix/core/status.h"
 #include "plamatrix/dense/dense_matrix.h"
 #include "plamatrix/sparse/csr_matrix.h"
+#include "plamatrix/sparse/host_sparse.h"
 #include "plamatrix_amd.h"
 
 namespace plamatrix
@@ -84,6 +85,111 @@ private:
     void* _statusOutput = nullptr;
     void (*_statusFinalize)(void*, bool) = nullptr;
 };
+
+/// Convert host COO triplets to sorted, duplicate-combined CSR storage
+/// (sparse_ops.h:79-85 of the reference; host matrices only).
+template <typename Scalar>
+CSRMatrix<Scalar, Device::CPU> cooToCsr(Index rows, Index cols, const std::vector<Index>& row_indices,
+                                        const std::vector<Index>& col_indices, const std::vector<Scalar>& values)
+{
+    std::vector<std::size_t> order;
+    std::vector<Index> row_start;
+    detail::host::orderTriplets(rows, cols, row_indices, col_indices, values.size(), order, row_start);
+    CSRMatrix<Scalar, Device::CPU> out;
+    detail::host::foldInto(rows, col_indices, values, order, row_start,
+                           [&](Index nnz, Index*& ro, Index*& ci, Scalar*& v) {
+                               out = CSRMatrix<Scalar, Device::CPU>(rows, cols, nnz);
+                               ro = out.rowOffsets();
+                               ci = out.colIndices();
+                               v = out.values();
+                           });
+    detail::CSRMatrixAccess::markTrusted(out);
+    return out;
+}
+
+/// Convert host COO triplets into an existing host CSR matrix whose shape and
+/// nnz must match the combined triplets (sparse_ops.h:87-96; runtime_error
+/// otherwise, after the triplets themselves were checked).
+template <typename Scalar>
+void cooToCsr(Index rows, Index cols, const std::vector<Index>& row_indices, const std::vector<Index>& col_indices,
+              const std::vector<Scalar>& values, CSRMatrix<Scalar, Device::CPU>& output)
+{
+    std::vector<std::size_t> order;
+    std::vector<Index> row_start;
+    detail::host::orderTriplets(rows, cols, row_indices, col_indices, values.size(), order, row_start);
+    detail::host::foldInto(rows, col_indices, values, order, row_start,
+                           [&](Index nnz, Index*& ro, Index*& ci, Scalar*& v) {
+                               if (output.rows() != rows || output.cols() != cols)
+                                   throw std::runtime_error("cooToCsr output dimension mismatch: output is " +
+                                                            std::to_string(output.rows()) + "x" +
+                                                            std::to_string(output.cols()) + ", expected " +
+                                                            std::to_string(rows) + "x" + std::to_string(cols));
+                               if (output.nnz() != nnz)
+                                   throw std::runtime_error("cooToCsr output nnz mismatch: output has " +
+                                                            std::to_string(output.nnz()) + ", expected " +
+                                                            std::to_string(nnz));
+                               ro = output.rowOffsets();
+                               ci = output.colIndices();
+                               v = output.values();
+                           });
+    detail::CSRMatrixAccess::markTrusted(output);
+}
+
+/// y = A x for host matrices (sparse_ops.h:122-131): each row left to right
+/// in Scalar.
+template <typename Scalar>
+void spmv(const CSRMatrix<Scalar, Device::CPU>& csr, const DenseMatrix<Scalar, Device::CPU>& x,
+          DenseMatrix<Scalar, Device::CPU>& output)
+{
+    if (x.rows() != csr.cols() || x.cols() != 1)
+        throw std::runtime_error("spmv dimension mismatch: CSR is " + std::to_string(csr.rows()) + "x" +
+                                 std::to_string(csr.cols()) + ", x is " + std::to_string(x.rows()) + "x" +
+                                 std::to_string(x.cols()) + ", expected " + std::to_string(csr.cols()) + "x1");
+    if (output.rows() != csr.rows() || output.cols() != 1)
+        throw std::runtime_error("spmv output dimension mismatch: output is " + std::to_string(output.rows()) + "x" +
+                                 std::to_string(output.cols()) + ", expected " + std::to_string(csr.rows()) + "x1");
+    if (x.data() != nullptr && x.data() == output.data())
+        throw std::invalid_argument("spmv input and output data must not alias");
+    detail::host::spmvRows(csr.rows(), csr.rowOffsets(), csr.colIndices(), csr.values(), x.data(), output.data());
+}
+
+template <typename Scalar>
+DenseMatrix<Scalar, Device::CPU> spmv(const CSRMatrix<Scalar, Device::CPU>& csr,
+                                      const DenseMatrix<Scalar, Device::CPU>& x)
+{
+    DenseMatrix<Scalar, Device::CPU> y(csr.rows(), 1);
+    spmv(csr, x, y);
+    return y;
+}
+
+/// Y = A B for host matrices, B and Y column-major (sparse_ops.h:133-146).
+template <typename Scalar>
+void spmm(const CSRMatrix<Scalar, Device::CPU>& csr, const DenseMatrix<Scalar, Device::CPU>& B,
+          DenseMatrix<Scalar, Device::CPU>& output)
+{
+    if (B.rows() != csr.cols())
+        throw std::runtime_error("spmm dimension mismatch: CSR is " + std::to_string(csr.rows()) + "x" +
+                                 std::to_string(csr.cols()) + ", B is " + std::to_string(B.rows()) + "x" +
+                                 std::to_string(B.cols()));
+    if (output.rows() != csr.rows() || output.cols() != B.cols())
+        throw std::runtime_error("spmm output dimension mismatch: output is " + std::to_string(output.rows()) + "x" +
+                                 std::to_string(output.cols()) + ", expected " + std::to_string(csr.rows()) + "x" +
+                                 std::to_string(B.cols()));
+    if (B.data() != nullptr && B.data() == output.data())
+        throw std::invalid_argument("spmm input and output data must not alias");
+    for (Index j = 0; j < B.cols(); ++j)
+        detail::host::spmvRows(csr.rows(), csr.rowOffsets(), csr.colIndices(), csr.values(), B.data() + j * B.rows(),
+                               output.data() + j * output.rows());
+}
+
+template <typename Scalar>
+DenseMatrix<Scalar, Device::CPU> spmm(const CSRMatrix<Scalar, Device::CPU>& csr,
+                                      const DenseMatrix<Scalar, Device::CPU>& B)
+{
+    DenseMatrix<Scalar, Device::CPU> y(csr.rows(), B.cols());
+    spmm(csr, B, y);
+    return y;
+}
 
 /// Synchronously convert device COO triplets to sorted, duplicate-combined
 /// GPU CSR (bit-identical to the CPU conversion).

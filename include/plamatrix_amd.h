@@ -395,6 +395,15 @@ typedef struct pm_residual_batch {
  * triangle, one "iteration", converged when ||r|| <= max(abs + rel ||b||,
  * 100 eps max(1, ||b||)); use_initial_guess and max_iterations are ignored). */
 enum { PM_SCHUR_BLOCK_PCG = 0, PM_SCHUR_DENSE_CHOLESKY = 1 };
+/* flags of pm_schur_options (block-PCG backend):
+ * PM_SCHUR_TOLERANCE_SUM: stop at ||r|| <= abs + rel ||r0||, the reference's
+ *   Cpu backend rule (block_schur_cpu_solver.h:34-35), instead of
+ *   max(abs, rel ||r0||);
+ * PM_SCHUR_MIXED_PRECISION: useMixedPrecision (block_schur.h:43-44,
+ *   block_schur_accelerated.cpp:131-193): an FP32 block-PCG pass on the
+ *   rounded system (at most 200 iterations, relative tolerance at least 1e-3,
+ *   absolute at least 1e-7) seeds the FP64 solve when it converges. */
+enum { PM_SCHUR_TOLERANCE_SUM = 1, PM_SCHUR_MIXED_PRECISION = 2 };
 typedef struct pm_schur_options {
     int32_t max_iterations;
     int32_t use_initial_guess;
@@ -402,7 +411,7 @@ typedef struct pm_schur_options {
     double absolute_tolerance;
     double minimum_diagonal;
     int32_t linear_backend;
-    int32_t pad;
+    int32_t flags;
 } pm_schur_options;
 
 /* SchurComplementSolverReport (block_schur.h:51-80).  status: 0 converged,
@@ -418,7 +427,7 @@ typedef struct pm_schur_report {
     int32_t status;
     int32_t pattern_reused;
     int32_t assembly_on_device;
-    int32_t pad;
+    int32_t mixed_precision_used; /* the FP32 seed was converged and used */
     int64_t pattern_build_count;
     char message[96];
 } pm_schur_report;

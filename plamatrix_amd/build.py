@@ -71,17 +71,23 @@ def build(jobs: int | None = None) -> str:
     return LIBRARY
 
 
-CPP_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_api.cpp")
+CPP_TESTS = {
+    # drop-in API over the device (run by a GPU test)
+    os.path.join(ROOT, "tests", "cpp", "test_api.cpp"): os.path.join(ROOT, "tests", "cpp", "test_api"),
+    # host overloads of the drop-in API (no GPU call; run by a CPU test)
+    os.path.join(ROOT, "tests", "cpp", "test_host_api.cpp"): os.path.join(ROOT, "tests", "cpp", "test_host_api"),
+}
 CPP_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_api")
 
 
 def build_cpp_tests() -> str:
-    """Host-only C++ program over the drop-in headers, run by a GPU test."""
-    cmd = [hipcc(), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), CPP_TEST_SRC,
-           "-L" + HERE, "-lplamatrix_amd", "-Wl,-rpath,$ORIGIN/../../plamatrix_amd", "-o", CPP_TEST_BIN]
-    proc = subprocess.run(cmd, capture_output=True, text=True)
-    if proc.returncode != 0:
-        raise RuntimeError(f"C++ API test build failed:\n{proc.stderr}")
+    """Host C++ programs over the drop-in headers, linked to the library."""
+    for src, binary in CPP_TESTS.items():
+        cmd = [hipcc(), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), src,
+               "-L" + HERE, "-lplamatrix_amd", "-Wl,-rpath,$ORIGIN/../../plamatrix_amd", "-o", binary]
+        proc = subprocess.run(cmd, capture_output=True, text=True)
+        if proc.returncode != 0:
+            raise RuntimeError(f"C++ API test build failed for {src}:\n{proc.stderr}")
     return CPP_TEST_BIN
 
 

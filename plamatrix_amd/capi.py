@@ -85,7 +85,7 @@ PM_SCHUR_DENSE_CHOLESKY = 1
 
 class pm_schur_options(ctypes.Structure):
     _fields_ = [("max_iterations", _i32), ("use_initial_guess", _i32), ("relative_tolerance", _f64),
-                ("absolute_tolerance", _f64), ("minimum_diagonal", _f64), ("linear_backend", _i32), ("pad", _i32)]
+                ("absolute_tolerance", _f64), ("minimum_diagonal", _f64), ("linear_backend", _i32), ("flags", _i32)]
 
 
 class pm_dense_report(ctypes.Structure):
@@ -95,7 +95,7 @@ class pm_dense_report(ctypes.Structure):
 class pm_schur_report(ctypes.Structure):
     _fields_ = [("converged", _i32), ("iterations", _i32), ("initial_residual_norm", _f64),
                 ("final_residual_norm", _f64), ("status", _i32), ("pattern_reused", _i32),
-                ("assembly_on_device", _i32), ("pad", _i32), ("pattern_build_count", _i64),
+                ("assembly_on_device", _i32), ("mixed_precision_used", _i32), ("pattern_build_count", _i64),
                 ("message", ctypes.c_char * 96)]
 
 
@@ -561,22 +561,30 @@ class SchurReport:
     assembly_on_device: bool
     pattern_build_count: int
     message: str
+    mixed_precision_used: bool = False
+
+
+PM_SCHUR_TOLERANCE_SUM = 1
+PM_SCHUR_MIXED_PRECISION = 2
 
 
 def schur_solve(sw: SchurWorkspace, ne: NormalEquations, damping, primary_step, eliminated_step, ws: Workspace,
                 max_iterations=100, relative_tolerance=1e-8, absolute_tolerance=1e-12, minimum_diagonal=1e-12,
-                use_initial_guess=False, dense=False) -> SchurReport:
-    """dense=True selects the dense Cholesky backend (the reference's DenseCpu)."""
+                use_initial_guess=False, dense=False, tolerance_sum=False, mixed_precision=False) -> SchurReport:
+    """dense=True selects the dense Cholesky backend (the reference's DenseCpu);
+    tolerance_sum=True the Cpu backend's stop rule abs + rel ||r0||;
+    mixed_precision=True the FP32-seeded solve (useMixedPrecision)."""
+    flags = (PM_SCHUR_TOLERANCE_SUM if tolerance_sum else 0) | (PM_SCHUR_MIXED_PRECISION if mixed_precision else 0)
     o = pm_schur_options(int(max_iterations), int(bool(use_initial_guess)), float(relative_tolerance),
                          float(absolute_tolerance), float(minimum_diagonal),
-                         PM_SCHUR_DENSE_CHOLESKY if dense else PM_SCHUR_BLOCK_PCG, 0)
+                         PM_SCHUR_DENSE_CHOLESKY if dense else PM_SCHUR_BLOCK_PCG, flags)
     r = pm_schur_report()
     check(lib().pm_schur_solve(sw.handle, ctypes.byref(ne.desc), _f64(damping), ctypes.byref(o),
                                _ptr(primary_step), _ptr(eliminated_step), ctypes.byref(r), ws.handle,
                                current_stream()))
     return SchurReport(bool(r.converged), r.iterations, r.initial_residual_norm, r.final_residual_norm, r.status,
                        bool(r.pattern_reused), bool(r.assembly_on_device), r.pattern_build_count,
-                       r.message.decode())
+                       r.message.decode(), bool(r.mixed_precision_used))
 
 
 def schur_assemble(sw: SchurWorkspace, ne: NormalEquations, damping, minimum_diagonal=1e-12) -> SchurReport:

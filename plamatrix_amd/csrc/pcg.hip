@@ -59,8 +59,10 @@ constexpr int kDirGrid = kMaxPartials;          // 2048 x 4 waves: 8 workgroups 
 constexpr int kPartialsDen = 0;                 // p.q partials (1 value)
 constexpr int kPartialsRes = 2 * kMaxPartials;  // r.r, r.z partials (2 values)
 
-__global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel_tol, double abs_tol)
+__global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel_tol, double abs_tol,
+                             int32_t tol_rule)
 {
+    st->tol_rule = tol_rule;
     st->rho[0] = 0.0;
     st->rho[1] = 0.0;
     st->alpha = 0.0;
@@ -90,7 +92,11 @@ __device__ __forceinline__ int direction_decision(SolverState* st, const double*
     reduce_partials<2>(partials + kPartialsRes, count_res, res);
     const double rr = res[0], rz = res[1];
     const int completed = st->iter[1 - par] + 1;
-    const double tol = completed == 0 ? fmax(st->abs_tol, st->rel_tol * sqrt(rr)) : st->tol;
+    // the accelerated solvers' max(abs, rel ||r0||), or the reference CPU
+    // Schur solver's abs + rel ||r0|| (block_schur_cpu_solver.h:34-35)
+    const double tol = completed != 0   ? st->tol
+                       : st->tol_rule   ? st->abs_tol + st->rel_tol * sqrt(rr)
+                                        : fmax(st->abs_tol, st->rel_tol * sqrt(rr));
     int code = kRunning;
     beta = 0.0;
     if (!isfinite(rr) || rr < 0.0) code = completed == 0 ? kInitNonFinite : kNonFinite;
@@ -1103,7 +1109,8 @@ int finish_report(const SolverState& fin, const pm_solver_options* options, pm_s
 template <typename S>
 int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_blocks, int64_t bsize, int pre,
                    const pm_solver_options* options, const pm_solve_policy* policy_in, pm_solver_report* report,
-                   pm_workspace* ws, hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr)
+                   pm_workspace* ws, hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr,
+                   int tol_rule = 0)
 {
     const Policy policy = resolve_policy(policy_in);
     Problem pb{};
@@ -1145,7 +1152,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         return PM_OK;
     }
     hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, stream, pb.st, options->max_iterations,
-                       options->relative_tolerance, options->absolute_tolerance);
+                       options->relative_tolerance, options->absolute_tolerance, tol_rule);
     PM_HIP(hipGetLastError());
     PM_TRY(get_plan<S>(ws, pb, 0, 0, 0, 0)->run(stream, false));
 
@@ -1189,14 +1196,27 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
 
 int solve_dispatch(const pm_csr* a, const void* rhs, void* x, const void* inv_blocks, int64_t bsize, int pre,
                    const pm_solver_options* options, const pm_solve_policy* policy, pm_solver_report* report,
-                   pm_workspace* ws, hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr)
+                   pm_workspace* ws, hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr,
+                   int tol_rule = 0)
 {
     PM_TRY(validate_options(options));
     PM_TRY(check_system(a, rhs, x, "iterative solver"));
     if (!ws) return fail(PM_INVALID_ARGUMENT, "iterative solver: null workspace");
     if (a->dtype == PM_F32)
-        return adaptive_solve<float>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream, gf, gtf);
-    return adaptive_solve<double>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream, gf, gtf);
+        return adaptive_solve<float>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream, gf, gtf,
+                                     tol_rule);
+    return adaptive_solve<double>(a, rhs, x, inv_blocks, bsize, pre, options, policy, report, ws, stream, gf, gtf,
+                                  tol_rule);
+}
+
+// Block-Jacobi PCG with the stopping rule chosen by the caller (tol_rule 1:
+// abs + rel ||r0||), for the Schur solve's CPU-backend semantics.
+int block_pcg_rule(const pm_csr* a, const void* rhs, void* x, const void* inverse_blocks, int64_t block_size,
+                   const pm_solver_options* options, pm_solver_report* report, pm_workspace* ws, hipStream_t stream,
+                   int tol_rule)
+{
+    return solve_dispatch(a, rhs, x, inverse_blocks, block_size, kPreBlock, options, nullptr, report, ws, stream,
+                          nullptr, nullptr, tol_rule);
 }
 
 // The factors of M^-1 = G^T G: square, same size and dtype as A, with 32-bit
@@ -1246,7 +1266,7 @@ int fixed_solve(const pm_csr* a, const void* rhs, void* x, int32_t iterations, i
     else
     {
         // tolerance 0: every submitted iteration runs unless r becomes exactly 0
-        hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, stream, pb.st, iterations, 0.0, 0.0);
+        hipLaunchKernelGGL(k_state_init, dim3(1), dim3(1), 0, stream, pb.st, iterations, 0.0, 0.0, 0);
         PM_HIP(hipGetLastError());
         std::vector<KernelCall> calls;
         build_calls<S>(pb, 0, 0, 0, 0, calls);

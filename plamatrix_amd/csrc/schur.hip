@@ -44,6 +44,7 @@
 
 #include "dense_cholesky.hpp"
 #include "pm_common.hpp"
+#include "solver_internal.hpp"
 #include "sort.hpp"
 #include "workspace.hpp"
 
@@ -191,6 +192,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_i64(const int64_t* __restrict__
 __global__ __launch_bounds__(kBlock) void k_fill_f64(double* __restrict__ dst, int64_t n, double v)
 {
     for (int64_t i = gtid(); i < n; i += gstride()) dst[i] = v;
+}
+
+// FP64 -> FP32 (rounded to nearest) and FP32 -> FP64 copies for the
+// mixed-precision seed solve.
+__global__ __launch_bounds__(kBlock) void k_f64_to_f32(const double* __restrict__ src, int64_t n,
+                                                       float* __restrict__ dst)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) dst[i] = static_cast<float>(src[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_f32_to_f64(const float* __restrict__ src, int64_t n,
+                                                       double* __restrict__ dst)
+{
+    for (int64_t i = gtid(); i < n; i += gstride()) dst[i] = static_cast<double>(src[i]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_u32_to_i64(const unsigned* __restrict__ src, int64_t n,
@@ -1235,6 +1250,9 @@ struct pm_schur_workspace
     int32_t* ci32 = nullptr;
     int64_t* diag_slot = nullptr;
 
+    // FP32 copies of the reduced system for the mixed-precision seed solve
+    DeviceBuffer seed;
+
     // per-solve numerics
     DeviceBuffer numerics;
     double* Bd = nullptr;
@@ -1927,6 +1945,55 @@ int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, 
     return PM_OK;
 }
 
+namespace
+{
+// Seed of the FP64 block-PCG (block_schur_accelerated.cpp:131-193): the
+// reduced system, right-hand side and block inverses rounded to FP32 and
+// solved by FP32 block-PCG (at most 200 iterations, relative tolerance at
+// least 1e-3, absolute at least 1e-7, tolerance max(abs, rel ||r0||)).  A
+// converged seed becomes the FP64 solve's initial guess; a seed that does not
+// converge or breaks down leaves the FP64 solve's start unchanged.
+int mixed_precision_seed(pm_schur_workspace* w, const pm_csr& a, int ps, const pm_schur_options* o, double* dp,
+                         pm_schur_report* report, pm_workspace* ws, hipStream_t stream)
+{
+    const int64_t pd = a.rows, nnz = a.nnz;
+    auto al = [](int64_t n) { return static_cast<size_t>((n * 4 + 255) & ~int64_t(255)); };
+    PM_TRY(w->seed.reserve(al(nnz) + al(pd) + al(pd * ps) + al(pd)));
+    char* base = w->seed.as<char>();
+    float* values = reinterpret_cast<float*>(base);
+    float* rhs = reinterpret_cast<float*>(base + al(nnz));
+    float* inv = reinterpret_cast<float*>(base + al(nnz) + al(pd));
+    float* x = reinterpret_cast<float*>(base + al(nnz) + al(pd) + al(pd * ps));
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(nnz)), dim3(kBlock), 0, stream, w->values, nnz, values);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(pd)), dim3(kBlock), 0, stream, w->rhs, pd, rhs);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(pd * ps)), dim3(kBlock), 0, stream, w->pinv, pd * ps, inv);
+    if (o->use_initial_guess)
+        hipLaunchKernelGGL(k_f64_to_f32, dim3(grid_for(pd)), dim3(kBlock), 0, stream, dp, pd, x);
+    else PM_HIP(hipMemsetAsync(x, 0, sizeof(float) * pd, stream));
+    PM_HIP(hipGetLastError());
+    pm_csr af = a;
+    af.values = values;
+    af.dtype = PM_F32;
+    pm_solver_options so{};
+    so.max_iterations = std::min(o->max_iterations, 200);
+    so.relative_tolerance = std::max(1.0e-3, static_cast<double>(static_cast<float>(o->relative_tolerance)));
+    so.absolute_tolerance = std::max(static_cast<double>(1.0e-7f), static_cast<double>(static_cast<float>(o->absolute_tolerance)));
+    so.use_jacobi = 1;
+    so.require_convergence = 0;
+    pm_solver_report sr{};
+    const int st = block_pcg_rule(&af, rhs, x, inv, ps, &so, &sr, ws, stream, 0);
+    if (st == PM_RUNTIME_ERROR) return PM_OK;  // a breakdown of the seed: the FP64 solve starts as asked
+    PM_TRY(st);
+    if (sr.converged)
+    {
+        hipLaunchKernelGGL(k_f32_to_f64, dim3(grid_for(pd)), dim3(kBlock), 0, stream, x, pd, dp);
+        PM_HIP(hipGetLastError());
+        report->mixed_precision_used = 1;
+    }
+    return PM_OK;
+}
+} // namespace
+
 int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double damping, const pm_schur_options* o,
                    double* dp, double* de, pm_schur_report* report, pm_workspace* ws, hipStream_t stream)
 {
@@ -1941,7 +2008,8 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     if (!std::isfinite(damping) || damping < 0.0 || o->max_iterations <= 0 || !std::isfinite(o->relative_tolerance) ||
         o->relative_tolerance < 0.0 || !std::isfinite(o->absolute_tolerance) || o->absolute_tolerance < 0.0 ||
         !std::isfinite(o->minimum_diagonal) || o->minimum_diagonal <= 0.0 ||
-        (o->linear_backend != PM_SCHUR_BLOCK_PCG && o->linear_backend != PM_SCHUR_DENSE_CHOLESKY))
+        (o->linear_backend != PM_SCHUR_BLOCK_PCG && o->linear_backend != PM_SCHUR_DENSE_CHOLESKY) ||
+        (o->flags & ~(PM_SCHUR_TOLERANCE_SUM | PM_SCHUR_MIXED_PRECISION)) != 0)
         return fail(PM_INVALID_ARGUMENT, "solveDampedSchurComplement: invalid solver options");
     const bool dense = o->linear_backend == PM_SCHUR_DENSE_CHOLESKY;
     PM_TRY(schur_assemble(w, eq, damping, o->minimum_diagonal, report, stream));
@@ -2001,18 +2069,20 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         }
         else
         {
-        pm_solver_options so{};
-        so.max_iterations = o->max_iterations;
-        so.relative_tolerance = o->relative_tolerance;
-        so.absolute_tolerance = o->absolute_tolerance;
-        so.use_jacobi = 1;
-        so.require_convergence = 0;
-        pm_solver_report sr{};
-        PM_TRY(pm_block_pcg(&a, w->rhs, dp, w->pinv, ps, &so, nullptr, &sr, ws, stream));
-        report->converged = sr.converged;
-        report->iterations = sr.iterations;
-        report->initial_residual_norm = sr.initial_residual;
-        report->final_residual_norm = sr.final_residual;
+            if (o->flags & PM_SCHUR_MIXED_PRECISION) PM_TRY(mixed_precision_seed(w, a, ps, o, dp, report, ws, stream));
+            pm_solver_options so{};
+            so.max_iterations = o->max_iterations;
+            so.relative_tolerance = o->relative_tolerance;
+            so.absolute_tolerance = o->absolute_tolerance;
+            so.use_jacobi = 1;
+            so.require_convergence = 0;
+            pm_solver_report sr{};
+            PM_TRY(block_pcg_rule(&a, w->rhs, dp, w->pinv, ps, &so, &sr, ws, stream,
+                                  (o->flags & PM_SCHUR_TOLERANCE_SUM) ? 1 : 0));
+            report->converged = sr.converged;
+            report->iterations = sr.iterations;
+            report->initial_residual_norm = sr.initial_residual;
+            report->final_residual_norm = sr.final_residual;
         }
     }
     if (!report->converged)

@@ -33,7 +33,7 @@ struct SolverState
     int32_t iteration;  // completed updates (host visible)
     int32_t max_iterations;
     int32_t done;       // see Done below
-    int32_t pad0;
+    int32_t tol_rule;   // 0: tol = max(abs, rel ||r0||); 1: tol = abs + rel ||r0||
     unsigned long long bad_row;  // first row with an unusable Jacobi diagonal (~0 if none)
     unsigned counters[8];
     double pad1[2];

@@ -843,6 +843,79 @@ static void schur()
     EXPECT(std::fabs(dp[0] + 1.345185185185185) <= 1e-11 && std::fabs(dp[1] + 0.512592592592593) <= 1e-11);
     EXPECT(std::fabs(de[0] + 0.493827160493827) <= 1e-11);
 
+    // the Cpu (default) and Hip backends on the synthetic and multi-primary
+    // systems (block_schur_test.cpp:77-158 via expectSyntheticReference /
+    // expectMultiPrimaryReference)
+    for (const auto backend : {SchurComplementLinearBackend::Cpu, SchurComplementLinearBackend::Hip})
+    {
+        auto ob = o;
+        ob.linearBackend = backend;
+        const auto rb = solveDampedSchurComplement(eq, 0.1, ob, &dp, &de);
+        EXPECT(rb.converged && rb.linearBackend == backend && !rb.deviceName.empty() && rb.linearSolveSeconds > 0.0);
+        for (int i = 0; i < 4; ++i) EXPECT(std::fabs(dp[static_cast<std::size_t>(i)] - expected[i]) <= 1e-10);
+        for (int i = 0; i < 2; ++i) EXPECT(std::fabs(de[static_cast<std::size_t>(i)] - expected[4 + i]) <= 1e-10);
+        const auto rm = solveDampedSchurComplement(multi, 0.1, ob, &dp, &de);
+        EXPECT(rm.converged);
+        EXPECT(std::fabs(dp[0] + 1.345185185185185) <= 1e-11 && std::fabs(dp[1] + 0.512592592592593) <= 1e-11);
+        EXPECT(std::fabs(de[0] + 0.493827160493827) <= 1e-11);
+    }
+    EXPECT(SchurComplementSolverOptions<double>{}.linearBackend == SchurComplementLinearBackend::Cpu);
+    {
+        // block_schur_test.cpp:444-466: useMixedPrecision converges to the FP64
+        // answer (the FP32 seed converges on this well-conditioned system and
+        // seeds the FP64 solve)
+        auto om = o;
+        om.linearBackend = SchurComplementLinearBackend::Hip;
+        om.relativeTolerance = 1e-12;
+        om.useMixedPrecision = true;
+        const auto rm = solveDampedSchurComplement(eq, 0.1, om, &dp, &de);
+        EXPECT(rm.converged);
+        EXPECT(std::fabs(dp[0] - expected[0]) <= 1e-10 && std::fabs(de[0] - expected[4]) <= 1e-10);
+        EXPECT(rm.mixedPrecisionUsed);
+    }
+    {
+        // block_schur_test.cpp:531-548: float point-only system
+        BlockNormalEquations<float> fe(0, 1, 6, 1);
+        const float jf = 2.0f, rf = 4.0f;
+        fe.addEliminatedResidualBlock(0, &jf, &rf, 1);
+        SchurComplementSolverOptions<float> of;
+        std::vector<float> fp, fde;
+        const auto rf_rep = solveDampedSchurComplement(fe, 0.5f, of, &fp, &fde);
+        EXPECT(rf_rep.converged && fp.empty() && fde.size() == 1);
+        EXPECT(std::fabs(fde[0] + 4.0f / 3.0f) <= 1e-6f);
+    }
+    {
+        // the synthetic system accumulated and solved with float blocks
+        BlockNormalEquations<float> fe(2, 2, 2, 1);
+        const struct
+        {
+            Index p, e;
+            float pj[2], ej, r;
+        } fres[] = {{0, 0, {1.0f, 2.0f}, 0.5f, -1.0f},
+                    {1, 0, {-0.5f, 1.0f}, 1.5f, 0.25f},
+                    {0, 1, {2.0f, -1.0f}, -0.75f, 2.0f},
+                    {1, 1, {1.25f, 0.5f}, 2.0f, -0.5f}};
+        for (const auto& q : fres) fe.addResidualBlock(q.p, q.e, q.pj, &q.ej, &q.r, 1, 1.0f);
+        const float fpj[2] = {0.4f, -0.2f}, fpr = 0.75f;
+        fe.addPrimaryResidualBlock(0, fpj, &fpr, 1, 2.0f);
+        const float fej = 0.8f, fer = -0.3f;
+        fe.addEliminatedResidualBlock(1, &fej, &fer, 1, 1.5f);
+        SchurComplementSolverOptions<float> of;
+        of.relativeTolerance = 1e-6f;
+        std::vector<float> fp, fde;
+        const auto rf_rep = solveDampedSchurComplement(fe, 0.1f, of, &fp, &fde);
+        EXPECT(rf_rep.converged && fp.size() == 4 && fde.size() == 2);
+        for (int i = 0; i < 4; ++i) EXPECT(std::fabs(fp[static_cast<std::size_t>(i)] - expected[i]) <= 1e-5);
+        for (int i = 0; i < 2; ++i) EXPECT(std::fabs(fde[static_cast<std::size_t>(i)] - expected[4 + i]) <= 1e-5);
+    }
+    {
+        auto bad = o;
+        bad.linearBackend = SchurComplementLinearBackend::Cuda;
+        EXPECT(throws<std::invalid_argument>([&] { solveDampedSchurComplement(eq, 0.1, bad, &dp, &de); }));
+        bad.linearBackend = SchurComplementLinearBackend::OpenCl;
+        EXPECT(throws<std::invalid_argument>([&] { solveDampedSchurComplement(eq, 0.1, bad, &dp, &de); }));
+    }
+
     EXPECT(throws<std::out_of_range>([&] { eq.addResidualBlock(2, 0, ppj, &eej, &er, 1); }));
     const double inf = INFINITY;
     EXPECT(throws<std::invalid_argument>([&] { eq.addResidualBlock(0, 0, ppj, &eej, &er, 1, inf); }));
@@ -880,7 +953,7 @@ static void denseSchur()
         std::vector<double> dp, de;
         const auto rep = solveDampedSchurComplement(eq, 0.1, o, ws, &dp, &de);
         EXPECT(rep.converged && rep.iterations == 1);
-        EXPECT(rep.linearBackend == SchurComplementLinearBackend::DenseHip);
+        EXPECT(rep.linearBackend == SchurComplementLinearBackend::DenseCpu);
         EXPECT(rep.linearSolveSeconds >= rep.choleskyFactorizationSeconds + rep.triangularSolveSeconds +
                                              rep.residualCheckSeconds);
         const double expected[] = {-0.532196868252671, 0.717080672008074, 0.078633780070238, -0.210823022930113,
