@@ -82,7 +82,8 @@ def main() -> int:
                     help="diffusion128 is the headline; the others measure the remaining solve-path workloads")
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--spmv-rounds", type=int, default=50)
-    ap.add_argument("--cpu-iterations", type=int, default=150)
+    ap.add_argument("--cpu-iterations", type=int, default=5000,
+                    help="iteration cap of the CPU baseline's FP64 solve (it stops at relres 1e-6 first)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=0)
@@ -113,6 +114,24 @@ def main() -> int:
     line = run(args, torch, capi, generators)
     print(json.dumps(line))
     return 0
+
+
+def host_cpu() -> str:
+    """CPU model, online CPUs and the CPUs this process may run on."""
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count()
+    return f"{model}, nproc {os.cpu_count()}, {allowed} available to this process, 1 used"
 
 
 def base_line(args, metric, value, unit, higher_is_better, dtype, config, roofline, cpu):
@@ -248,25 +267,29 @@ def run_diffusion(args, torch, capi, generators):
     torch.cuda.synchronize()
     ms_1e8 = (time.perf_counter() - t8) * 1e3
 
-    # ---- CPU baseline: the oracle's FP64 Jacobi-PCG, one core, bounded sample ----
+    # ---- CPU baseline: the reference's FP64 Jacobi-PCG, one core, one full solve ----
     cpu = None
     if not args.no_cpu:
         import oracle
 
         use_ref = oracle.ref_available()
-        ref = oracle.CSR(n, n, h.row_offsets, h.col_indices, h.values)
-        k = args.cpu_iterations
+        ref = oracle.CSR(n, n, h.row_offsets, h.col_indices, h.values.astype(np.float64))
         c0 = time.perf_counter()
-        _, crep = oracle.pcg(ref, b_host.astype(np.float32), relative_tolerance=0.0, max_iterations=k,
-                             use_ref=use_ref)
+        cx, crep = oracle.pcg(ref, b_host, relative_tolerance=1e-6, max_iterations=args.cpu_iterations,
+                              use_ref=use_ref)
         cpu_s = time.perf_counter() - c0
-        per_it = cpu_s / max(crep.iterations, 1)
-        who = ("the reference's own CPU pcg (iterative_solver_cpu.cpp built from source, serial)" if use_ref
-               else "the oracle's restatement of iterative_solver_cpu.cpp (oracle/sparse.c, serial)")
-        cpu = {"value": round(per_it * iterations * 1e3, 1), "unit": "ms", "cores": 1,
+        import scipy.sparse as sp
+
+        A64 = sp.csr_matrix((h.values.astype(np.float64), h.col_indices, h.row_offsets), shape=(n, n))
+        cpu_relres = float(np.linalg.norm(b_host - A64 @ cx) / np.linalg.norm(b_host))
+        who = ("the reference's own CPU pcg (iterative_solver_cpu.cpp built from source, serial, FP64 vectors)"
+               if use_ref else "the oracle's restatement of iterative_solver_cpu.cpp (oracle/sparse.c, serial)")
+        cpu = {"value": round(cpu_s * 1e3, 1), "unit": "ms", "cores": 1,
                "kind": "reference" if use_ref else "port",
-               "sample": f"{crep.iterations} FP64 Jacobi-PCG iterations of {who} on the full 128^3 system "
-                         f"took {cpu_s:.1f} s; scaled to the {iterations} iterations the GPU solve needed"}
+               "sample": f"one full solve of the 128^3 system (FP64 values and rhs, x0 = 0) by {who} to its "
+                         f"relative tolerance 1e-6: {crep.iterations} iterations, converged={crep.converged}, "
+                         f"FP64 true relres {cpu_relres:.3e}; host {host_cpu()}",
+               "iterations": crep.iterations, "true_relres": cpu_relres}
 
     kernel = "k_spmv<float,int>" if lanes == 0 else f"k_spmv_lanes<float,int,{lanes}>"
     # headline: learned preconditioner with its Jacobi fallback (taken when the
@@ -298,6 +321,8 @@ def run_diffusion(args, torch, capi, generators):
         },
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": committed_traffic(kernel),
+                     "traffic_source": "read from the committed rocprofv3 --pmc summary "
+                                       "profiles/k_spmv_diffusion128_traffic.txt, not measured in this run",
                      "kernel": kernel, "bytes_per_launch": spmv_b,
                      "ms_per_launch": round(spmv_ms, 5)},
         "cpu_baseline": cpu,
@@ -576,80 +601,17 @@ def run_schur_dense(args, torch, capi, generators):
     return line
 
 
-def _batch_systems(generators, capi, torch, count=512, seed=4):
-    """The batch: sizes log-uniform in [1e3, 2e5] (seed 4), members cycling 2-D
-    Laplacian / 3-D diffusion / reduced LM Schur system (primaries = n/6,
-    3 secondaries per primary, assembled on the device).  Returns the
-    block-diagonal device CSR (FP32, int32), row ranges and member kinds."""
-    dev = torch.device("cuda:0")
-    sizes = generators.batch_sizes(count, seed=seed)
-    kinds = ["laplace2d", "diffusion3d", "schur"]
-    members = []
-    for i, n in enumerate(sizes):
-        kind = kinds[i % 3]
-        if kind == "laplace2d":
-            h = generators.laplacian2d_csr(*generators.grid2d_dims(int(n)), dtype=np.float32)
-            members.append(("host", h))
-        elif kind == "diffusion3d":
-            h = generators.diffusion3d_box_csr(*generators.grid3d_dims(int(n)), seed=100 + i, dtype=np.float32)
-            members.append(("host", h))
-        else:
-            pc = max(int(n) // 6, 20)
-            inst = generators.lm_schur_instance(primary_count=pc, eliminated_count=3 * pc, seed=5000 + i)
-            t = lambda arr, dt: torch.from_numpy(np.ascontiguousarray(arr, dtype=dt)).to(dev)
-            sw = capi.SchurWorkspace()
-            ne = capi.assemble_normal_equations(sw, pc, 3 * pc, 6, 3, t(inst.primary, np.int64),
-                                                t(inst.eliminated, np.int64), t(inst.primary_jacobians, np.float64),
-                                                t(inst.eliminated_jacobians, np.float64),
-                                                t(inst.residuals, np.float64), weights=None,
-                                                huber_delta=inst.huber_delta)
-            rep = capi.schur_assemble(sw, ne, inst.damping)
-            if rep.status != 0:
-                raise RuntimeError(rep.message)
-            a, _, _ = capi.schur_system(sw)
-            rows = a.rows
-            ro = torch.empty(rows + 1, dtype=torch.int64, device=dev)
-            ci = torch.empty(a.nnz, dtype=torch.int64, device=dev)
-            va = torch.empty(a.nnz, dtype=torch.float64, device=dev)
-            capi.lib_memcpy(ro.data_ptr(), a.row_offsets, ro.numel() * 8)
-            capi.lib_memcpy(ci.data_ptr(), a.col_indices, ci.numel() * 8)
-            capi.lib_memcpy(va.data_ptr(), a.values, va.numel() * 8)
-            members.append(("device", (rows, ro, ci, va.to(torch.float32))))
-            del sw, ne
-    lens = [m[1].rows if m[0] == "host" else m[1][0] for m in members]
-    begin, total = capi.batch_layout(lens)
-    nnzs = [int(m[1].nnz) if m[0] == "host" else int(m[1][2].numel()) for m in members]
-    nnz_total = int(sum(nnzs))
-    ro_all = torch.zeros(total + 1, dtype=torch.int32, device=dev)
-    ci_all = torch.empty(nnz_total, dtype=torch.int32, device=dev)
-    va_all = torch.empty(nnz_total, dtype=torch.float32, device=dev)
-    counts = torch.zeros(total, dtype=torch.int64, device=dev)
-    k = 0
-    for (where, m), b0, ln, z in zip(members, begin, lens, nnzs):
-        if where == "host":
-            ro = torch.from_numpy(m.row_offsets).to(dev)
-            ci = torch.from_numpy(m.col_indices).to(dev)
-            va = torch.from_numpy(m.values.astype(np.float32)).to(dev)
-        else:
-            _, ro, ci, va = m
-        counts[b0:b0 + ln] = ro[1:] - ro[:-1]
-        ci_all[k:k + z] = (ci + int(b0)).to(torch.int32)
-        va_all[k:k + z] = va
-        k += z
-    # rows between systems (64-row alignment padding) are empty
-    ro_all[1:] = torch.cumsum(counts, 0).to(torch.int32)
-    a = capi.DeviceCSR(total, total, None, None, va_all, ro_all, ci_all)
-    end = begin + np.asarray(lens, dtype=np.int64)
-    return a, begin, end, [kinds[i % 3] for i in range(count)], lens, nnzs, members
-
-
 def run_batch(args, torch, capi, generators):
     """512 uneven SPD systems in one batched solve: per-system FP32 SpMV with
     FP64 dots, per-system relative tolerance 1e-6, device-side convergence
     flags and graph-replayed iterations."""
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
-    a, begin, end, kinds, lens, nnzs, members = _batch_systems(generators, capi, torch)
+    from plamatrix_amd import workloads
+
+    batch = workloads.batch_systems(capi, generators, torch)
+    a, begin, end, kinds, lens, nnzs, members = (batch.a, batch.begin, batch.end, batch.kinds, batch.lens,
+                                                 batch.nnzs, batch.members)
     gen_s = time.perf_counter() - t0
     total = a.rows
     rng = np.random.default_rng(7)

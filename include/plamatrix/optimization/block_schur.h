@@ -502,7 +502,13 @@ SchurComplementSolverReport<Scalar> solveDampedSchurComplement(const BlockNormal
     report.message = r.message;
     hipDeviceProp_t prop{};
     int dev = 0;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) report.deviceName = prop.name;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    {
+        // the marketing name can be blank (some virtualised devices report
+        // none); the architecture name always identifies the device
+        report.deviceName = prop.name[0] != '\0' ? std::string(prop.name) : std::string(prop.gcnArchName);
+        if (report.deviceName.empty()) report.deviceName = "HIP device " + std::to_string(dev);
+    }
     return report;
 }
 

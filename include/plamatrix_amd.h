@@ -283,6 +283,29 @@ int pm_batch_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_beg
                         int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
                         pm_workspace* ws, hipStream_t stream);
 
+/* Batched PCG with a factored preconditioner M_s^-1 = G_s^T G_s per system (no
+ * reference counterpart; it replaces one pm_pcg_factored call per system, the
+ * shape of the reference's one-system-per-call solver,
+ * iterative_solver.cu:305-340).  g is lower-triangular and gt its transpose,
+ * both block-diagonal over the same row ranges as `a` (pm_fsai_factors of a
+ * graph built on the batched matrix gives exactly that), with a's dtype and
+ * index widths.  jacobi_systems (device, one byte per system, or null) selects
+ * Jacobi-PCG for the systems flagged nonzero: their z = D^-1 r, as in
+ * pm_batch_pcg with use_jacobi.  Per iteration: q = A p, t = G (r - alpha q)
+ * with x += alpha p, then r -= alpha q and z = G^T t.  Same stop rules,
+ * reports and errors as pm_batch_pcg; options->use_jacobi is ignored. */
+int pm_batch_pcg_factored(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                          const void* rhs, void* x, const pm_csr* g, const pm_csr* gt, const uint8_t* jacobi_systems,
+                          const pm_solver_options* options, const pm_solve_policy* policy, pm_batch_report* reports,
+                          pm_workspace* ws, hipStream_t stream);
+/* pm_batch_pcg_refine with the factored preconditioner (and its per-system
+ * Jacobi flags) in the FP32 inner batched solves. */
+int pm_batch_pcg_refine_factored(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                                 const double* rhs, double* x, const pm_csr* g, const pm_csr* gt,
+                                 const uint8_t* jacobi_systems, const pm_solver_options* inner, double target_relres,
+                                 int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
+                                 pm_workspace* ws, hipStream_t stream);
+
 /* ---- Factored preconditioner M^-1 = G^T G ----------------------------------------
  * PCG with z = G^T (G r) (no reference counterpart).  g is lower-triangular
  * CSR, gt its transpose; both need 32-bit index mirrors and a's dtype.  Same
@@ -337,6 +360,16 @@ int pm_fsai_graph_pool_bytes(int64_t* bytes);
 int pm_fsai_forward(pm_fsai_net* net, pm_fsai_graph* graph, hipStream_t stream);
 /* CSR descriptors of G (lower) and G^T (upper), owned by the graph. */
 int pm_fsai_factors(const pm_fsai_graph* graph, pm_csr* g, pm_csr* gt);
+/* Per-system choice between the learned factor and Jacobi for a graph built on
+ * a batched block-diagonal matrix (no reference counterpart).  With `probes`
+ * (1..8) Gaussian probes z drawn from `seed`, system s keeps G when
+ * sum_p ||(I - G A G^T) z_p||^2 over its rows [row_begin[s], row_end[s]) is
+ * below the same loss of Jacobi's factor D^-1/2, else jacobi_systems[s] = 1
+ * (device array of `systems` bytes, the input of pm_batch_pcg_factored).
+ * losses (host, 2 * systems: learned, Jacobi per system) may be null.  Needs a
+ * forward pass first; synchronises `stream`. */
+int pm_fsai_batch_select(pm_fsai_graph* graph, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                         int32_t probes, uint64_t seed, uint8_t* jacobi_systems, double* losses, hipStream_t stream);
 /* Probe loss and full parameter gradient (host) for `probes` (1..8) probes
  * drawn from `seed`; synchronises `stream`. */
 int pm_fsai_loss_grad(pm_fsai_net* net, pm_fsai_graph* graph, int32_t probes, uint64_t seed, double* loss,

@@ -113,6 +113,7 @@ EXPORTS = [
     "pm_fsai_net_set_parameters", "pm_fsai_net_get_parameters", "pm_fsai_graph_create", "pm_fsai_graph_destroy",
     "pm_fsai_graph_pool_release", "pm_fsai_graph_pool_bytes",
     "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step", "pm_dense_cholesky_solve",
+    "pm_batch_pcg_factored", "pm_batch_pcg_refine_factored", "pm_fsai_batch_select",
 ]
 
 _lib = None
@@ -800,6 +801,69 @@ def pcg_refine_factored(a: "DeviceCSR", b64, x64, graph: FsaiGraph, ws: Workspac
                                        ctypes.byref(rep), ws.handle, current_stream()))
     return RefineReport(bool(rep.converged), rep.outer_iterations, rep.inner_iterations, rep.initial_relres,
                         rep.final_relres)
+
+
+def fsai_batch_select(graph: FsaiGraph, row_begin, row_end, probes: int = 8, seed: int = 0):
+    """Per-system choice between the learned factor and Jacobi on a batched
+    graph: a uint8 device tensor (1 = Jacobi) and the host losses (systems x 2:
+    learned, Jacobi)."""
+    import numpy as np
+    import torch
+
+    row_begin = np.ascontiguousarray(row_begin, dtype=np.int64)
+    row_end = np.ascontiguousarray(row_end, dtype=np.int64)
+    systems = len(row_begin)
+    flags = torch.zeros(max(systems, 1), dtype=torch.uint8, device="cuda")
+    losses = np.zeros((max(systems, 1), 2), dtype=np.float64)
+    check(lib().pm_fsai_batch_select(graph.handle, _i64(systems), row_begin.ctypes.data_as(_p),
+                                     row_end.ctypes.data_as(_p), _i32(probes), ctypes.c_uint64(seed),
+                                     _ptr(flags), losses.ctypes.data_as(_p), current_stream()))
+    return flags[:systems], losses[:systems]
+
+
+def batch_pcg_factored(a: "DeviceCSR", row_begin, row_end, b, x, graph: FsaiGraph, ws: Workspace, jacobi=None,
+                       pol=None, factors=None, **opts):
+    """Batched PCG with the graph's factors (or explicit (G, G^T) descriptors);
+    `jacobi` is an optional uint8 device tensor of per-system Jacobi flags."""
+    import numpy as np
+
+    d = a.descriptor()
+    g, gt = factors if factors is not None else graph.factors()
+    o = options(**opts)
+    p = pol if pol is not None else policy()
+    row_begin = np.ascontiguousarray(row_begin, dtype=np.int64)
+    row_end = np.ascontiguousarray(row_end, dtype=np.int64)
+    systems = len(row_begin)
+    reps = (pm_batch_report * max(systems, 1))()
+    check(lib().pm_batch_pcg_factored(ctypes.byref(d), _i64(systems), row_begin.ctypes.data_as(_p),
+                                      row_end.ctypes.data_as(_p), _ptr(b), _ptr(x), ctypes.byref(g), ctypes.byref(gt),
+                                      _ptr(jacobi) if jacobi is not None else None, ctypes.byref(o), ctypes.byref(p),
+                                      reps, ws.handle, current_stream()))
+    return [BatchReport(bool(r.converged), r.iterations, r.initial_residual, r.final_residual, r.status, r.bad_row)
+            for r in reps[:systems]]
+
+
+def batch_pcg_refine_factored(a: "DeviceCSR", row_begin, row_end, b64, x64, graph: FsaiGraph, ws: Workspace,
+                              jacobi=None, target=1e-6, max_outer=10, pol=None, factors=None, **inner):
+    """Batched FP64 refinement with the factored preconditioner in the FP32
+    inner batched solves; one report per system."""
+    import numpy as np
+
+    d = a.descriptor()
+    g, gt = factors if factors is not None else graph.factors()
+    o = options(**inner)
+    p = pol if pol is not None else policy()
+    row_begin = np.ascontiguousarray(row_begin, dtype=np.int64)
+    row_end = np.ascontiguousarray(row_end, dtype=np.int64)
+    systems = len(row_begin)
+    reps = (pm_batch_refine_report * max(systems, 1))()
+    check(lib().pm_batch_pcg_refine_factored(ctypes.byref(d), _i64(systems), row_begin.ctypes.data_as(_p),
+                                             row_end.ctypes.data_as(_p), _ptr(b64), _ptr(x64), ctypes.byref(g),
+                                             ctypes.byref(gt), _ptr(jacobi) if jacobi is not None else None,
+                                             ctypes.byref(o), _f64(target), _i32(max_outer), ctypes.byref(p), reps,
+                                             ws.handle, current_stream()))
+    return [BatchRefineReport(bool(r.converged), r.outer_iterations, r.inner_iterations, r.initial_relres,
+                              r.final_relres, r.last_inner_status) for r in reps[:systems]]
 
 
 def csr_arrays(d: pm_csr):

@@ -264,6 +264,120 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const i
     }
 }
 
+// ---- factored preconditioner M_s^-1 = G_s^T G_s per system ------------------------
+//
+// G (lower) and G^T are block-diagonal over the same row ranges as A.  A
+// system flagged in sys_jacobi (device, one byte per system; null: none) takes
+// Jacobi instead (z = D^-1 r with the inverse diagonal of k_batch_jacobi_inverse).
+
+// t = G r_new for the chunks of running factored systems.  STEP: alpha_s from
+// the state, x += alpha_s p on the chunk's rows (every system), and
+// r_new = r - alpha_s q formed at every gathered column with the rounding of
+// k_batch_update (r itself is rewritten by k_batch_factor_upper, since other
+// chunks gather it here); otherwise r_new = r.
+template <typename S, typename I, bool STEP>
+__global__ __launch_bounds__(kBlock) void k_batch_factor_lower(const int32_t* __restrict__ chunk_sys,
+                                                               const int64_t* __restrict__ sys_end,
+                                                               const I* __restrict__ gro, const I* __restrict__ gci,
+                                                               const S* __restrict__ gv, S* __restrict__ x,
+                                                               const S* __restrict__ r, const S* __restrict__ p,
+                                                               const S* __restrict__ q, S* __restrict__ t,
+                                                               const BatchState* st,
+                                                               const uint8_t* __restrict__ sys_jacobi,
+                                                               const int64_t* __restrict__ list,
+                                                               const unsigned long long* __restrict__ list_count)
+{
+    __shared__ S lv_all[kBatchWaves][kStreamCap];
+    __shared__ I lc_all[kBatchWaves][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning) continue;
+        const int64_t end = sys_end[s];
+        const int64_t row0 = c * kChunkRows;
+        const int64_t row = row0 + lane;
+        const S alpha = STEP ? static_cast<S>(st[s].alpha) : S(0);
+        if (!(sys_jacobi && sys_jacobi[s]))
+        {
+            int64_t ks, ke;
+            S acc;
+            if (STEP)
+                acc = chunk_row_sums<S>(row0, end, gro, gci, lv_all[wave], lc_all[wave],
+                                        StepSource<S>{gv, r, q, -alpha}, ks, ke);
+            else
+                acc = chunk_row_sums<S>(row0, end, gro, gci, lv_all[wave], lc_all[wave], SpmvSource<S>{gv, r}, ks,
+                                        ke);
+            if (row < end) t[row] = acc;
+        }
+        if (STEP && row < end) x[row] = fma_s(alpha, p[row], x[row]);
+    }
+}
+
+// z = G^T t (Jacobi systems: z = D^-1 r_new) and the per-chunk partials r.r,
+// r.z.  STEP: r_new = r - alpha_s q on the chunk's own rows, written back.
+template <typename S, typename I, bool STEP>
+__global__ __launch_bounds__(kBlock) void k_batch_factor_upper(const int32_t* __restrict__ chunk_sys,
+                                                               const int64_t* __restrict__ sys_end,
+                                                               const I* __restrict__ uro, const I* __restrict__ uci,
+                                                               const S* __restrict__ uv, const S* __restrict__ t,
+                                                               S* __restrict__ r, const S* __restrict__ q,
+                                                               const S* __restrict__ inv, S* __restrict__ z,
+                                                               const BatchState* st,
+                                                               const uint8_t* __restrict__ sys_jacobi,
+                                                               double* __restrict__ prr, double* __restrict__ prz,
+                                                               const int64_t* __restrict__ list,
+                                                               const unsigned long long* __restrict__ list_count)
+{
+    __shared__ S lv_all[kBatchWaves][kStreamCap];
+    __shared__ I lc_all[kBatchWaves][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning)
+        {
+            if (lane == 0) prr[c] = prz[c] = 0.0;
+            continue;
+        }
+        const int64_t end = sys_end[s];
+        const int64_t row0 = c * kChunkRows;
+        const int64_t row = row0 + lane;
+        const bool jac = sys_jacobi && sys_jacobi[s];
+        S acc = S(0);
+        if (!jac)
+        {
+            int64_t ks, ke;
+            acc = chunk_row_sums<S>(row0, end, uro, uci, lv_all[wave], lc_all[wave], SpmvSource<S>{uv, t}, ks, ke);
+        }
+        double rr = 0.0, rz = 0.0;
+        if (row < end)
+        {
+            S ri = r[row];
+            if (STEP)
+            {
+                ri = fma_s(-static_cast<S>(st[s].alpha), q[row], ri);
+                r[row] = ri;
+            }
+            const S zi = jac ? inv[row] * ri : acc;
+            z[row] = zi;
+            rr = static_cast<double>(ri) * static_cast<double>(ri);
+            rz = static_cast<double>(ri) * static_cast<double>(zi);
+        }
+        rr = wave_sum(rr);
+        rz = wave_sum(rz);
+        if (lane == 0)
+        {
+            prr[c] = rr;
+            prz[c] = rz;
+        }
+    }
+}
+
 // Rebuild the list of chunks whose system is still running: per workgroup
 // an ordered compaction in LDS, one atomic per workgroup for its offset.
 __global__ __launch_bounds__(kBlock) void k_batch_list_reset(unsigned long long* list_count) { *list_count = 0ull; }
@@ -308,12 +422,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_list(int64_t chunks, const int
 // (coalesced), the largest magnitude sets the threshold; a row with one
 // diagonal entry takes it as is, several are summed in CSR order.  Rows of a
 // rejected diagonal get NaN and the system records its first such row.
+// only (device, one byte per system; null: every system) limits it to the
+// systems flagged.
 template <typename S, typename I>
 __global__ __launch_bounds__(kBlock) void k_batch_jacobi_inverse(int64_t chunks, const int32_t* __restrict__ chunk_sys,
                                                                  const int64_t* __restrict__ sys_end,
                                                                  const I* __restrict__ ro, const I* __restrict__ ci,
                                                                  const S* __restrict__ v, S* __restrict__ inv,
-                                                                 BatchState* st)
+                                                                 BatchState* st, const uint8_t* __restrict__ only)
 {
     constexpr int L = 8;
     const int lane = threadIdx.x % kWave;
@@ -321,7 +437,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_jacobi_inverse(int64_t chunks,
     PM_FOR_CHUNKS(c, chunks, kBatchWaves)
     {
         const int32_t s = chunk_sys[c];
-        if (s < 0) continue;
+        if (s < 0 || (only && !only[s])) continue;
         const int64_t end = sys_end[s];
         for (int pass = 0; pass < L; ++pass)
         {
@@ -475,12 +591,17 @@ struct BatchProblem
     const void* rhs;
     void* x;
     bool jacobi;
+    // factored preconditioner (G, G^T block-diagonal like a) with optional
+    // per-system Jacobi fallback flags (device), else null
+    const pm_csr* g;
+    const pm_csr* gt;
+    const uint8_t* sys_jacobi;
     int64_t systems;
     int64_t chunks;
     int32_t max_iterations;
     double rel_tol, abs_tol;
     int grid_chunks, grid_systems;
-    void *r, *z, *q, *p0, *p1, *inv;
+    void *r, *z, *q, *p0, *p1, *inv, *t;
     int32_t* chunk_sys;
     int64_t* sys_end;
     int64_t* sys_chunk0;
@@ -525,14 +646,36 @@ struct BatchCalls
                          static_cast<const int64_t*>(b.list), static_cast<const unsigned long long*>(b.list_count));
     }
 
+    // z = M^-1 r for the factored preconditioner: t = G r_new, z = G^T t (or
+    // Jacobi for flagged systems), partials r.r and r.z.
+    template <bool STEP>
+    static void factored(const BatchProblem& b, const void* p, std::vector<KernelCall>& out)
+    {
+        out.push_back(make_call(k_batch_factor_lower<S, I, STEP>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys,
+                                b.sys_end, ro(b.g), ci(b.g), static_cast<const S*>(b.g->values),
+                                static_cast<S*>(b.x), static_cast<const S*>(b.r), static_cast<const S*>(p),
+                                static_cast<const S*>(b.q), static_cast<S*>(b.t),
+                                static_cast<const BatchState*>(b.st), b.sys_jacobi,
+                                static_cast<const int64_t*>(b.list),
+                                static_cast<const unsigned long long*>(b.list_count)));
+        out.push_back(make_call(k_batch_factor_upper<S, I, STEP>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys,
+                                b.sys_end, ro(b.gt), ci(b.gt), static_cast<const S*>(b.gt->values),
+                                static_cast<const S*>(b.t), static_cast<S*>(b.r), static_cast<const S*>(b.q),
+                                static_cast<const S*>(b.inv), static_cast<S*>(b.z),
+                                static_cast<const BatchState*>(b.st), b.sys_jacobi, b.prr, b.prz,
+                                static_cast<const int64_t*>(b.list),
+                                static_cast<const unsigned long long*>(b.list_count)));
+    }
+
     static void init(const BatchProblem& b, std::vector<KernelCall>& out)
     {
-        if (b.jacobi)
+        if (b.jacobi || (b.g && b.sys_jacobi))
             out.push_back(make_call(k_batch_jacobi_inverse<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
                                     b.chunk_sys, b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
-                                    static_cast<S*>(b.inv), b.st));
+                                    static_cast<S*>(b.inv), b.st, b.g ? b.sys_jacobi : nullptr));
         list(b, out);
         out.push_back(residual<true>(b, b.p0));
+        if (b.g) factored<false>(b, nullptr, out);
         out.push_back(reduce<kReduceInit>(b, b.prr, b.prz));
     }
 
@@ -562,7 +705,12 @@ struct BatchCalls
                                 static_cast<const int64_t*>(b.list),
                                 static_cast<const unsigned long long*>(b.list_count)));
         out.push_back(reduce<kReduceAlpha>(b, b.pden, nullptr));
-        if (!refresh)
+        if (b.g && !refresh)
+        {
+            // the step rides in the two factor products
+            factored<true>(b, p_new, out);
+        }
+        else if (!refresh)
         {
             const auto k = b.jacobi ? k_batch_update<S, true, true> : k_batch_update<S, false, true>;
             out.push_back(make_call(k, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys, b.sys_end,
@@ -581,6 +729,7 @@ struct BatchCalls
                                     b.prz, static_cast<const int64_t*>(b.list),
                                     static_cast<const unsigned long long*>(b.list_count)));
             out.push_back(residual<false>(b, nullptr));
+            if (b.g) factored<false>(b, nullptr, out);
         }
         out.push_back(reduce<kReduceBeta>(b, b.prr, b.prz));
     }
@@ -634,14 +783,26 @@ std::string batch_key(const BatchProblem& b, int what, int64_t first, int64_t co
 {
     char buf[512];
     std::snprintf(buf, sizeof(buf),
-                  "batch|%d|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%lld|%lld|%d|%d|%d|%.17g|%.17g|%d", what, b.r,
+                  "batch|%d|%p|%p|%p|%p|%p|%p|%p|%p|%p|%p|%lld|%lld|%d|%lld|%lld|%d|%d|%d|%.17g|%.17g|%d|%p", what, b.r,
                   static_cast<void*>(b.st), static_cast<void*>(b.pden), b.a->values, (const void*)b.a->row_offsets32,
                   (const void*)b.a->row_offsets, (const void*)b.a->col_indices32, (const void*)b.a->col_indices,
                   b.rhs, b.x,
                   static_cast<long long>(b.a->rows), static_cast<long long>(b.systems), b.jacobi ? 1 : 0,
                   static_cast<long long>(first), static_cast<long long>(count), refresh, b.a->dtype,
-                  b.max_iterations, b.rel_tol, b.abs_tol, b.split ? 1 : 0);
-    return buf;
+                  b.max_iterations, b.rel_tol, b.abs_tol, b.split ? 1 : 0, static_cast<const void*>(b.sys_jacobi));
+    std::string key(buf);
+    if (b.g)
+    {
+        // every array of G and G^T the factored kernels capture
+        for (const pm_csr* m : {b.g, b.gt})
+        {
+            std::snprintf(buf, sizeof(buf), "|%p|%p|%p|%p|%p|%lld|%d", m->values, (const void*)m->row_offsets32,
+                          (const void*)m->col_indices32, (const void*)m->row_offsets, (const void*)m->col_indices,
+                          static_cast<long long>(m->nnz), m->dtype);
+            key += buf;
+        }
+    }
+    return key;
 }
 
 template <typename S>
@@ -770,7 +931,7 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     PM_TRY(ws->ensure_events());
     const size_t es = a->dtype == PM_F64 ? 8 : 4;
     const size_t vec = ((es * static_cast<size_t>(n > 0 ? n : 1)) + 255) & ~size_t(255);
-    PM_TRY(ws->vectors.reserve(vec * 6));
+    PM_TRY(ws->vectors.reserve(vec * 7));
     if (n > ws->capacity_rows) ws->capacity_rows = n;
     auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
     const size_t s1 = std::max<int64_t>(systems, 1);
@@ -804,6 +965,7 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     b.p0 = vb + 3 * vec;
     b.p1 = vb + 4 * vec;
     b.inv = vb + 5 * vec;
+    b.t = vb + 6 * vec;
     char* p = ws->batch.as<char>();
     b.chunk_sys = reinterpret_cast<int32_t*>(p);
     p += al(sizeof(int32_t) * c1);
@@ -830,6 +992,31 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     if (chunks > 0)
         PM_HIP(hipMemcpy(b.chunk_sys, chunk_sys.data(), sizeof(int32_t) * chunks, hipMemcpyHostToDevice));
     if (systems > 0) PM_HIP(hipMemcpy(b.sys_end, sys.data(), sizeof(int64_t) * 3 * systems, hipMemcpyHostToDevice));
+    return PM_OK;
+}
+
+// The factored preconditioner of a batch: G lower-triangular and G^T, square
+// like a, with a's dtype and index widths (32-bit mirrors when a uses them).
+int batch_attach_factors(BatchProblem& b, const pm_csr* g, const pm_csr* gt, const uint8_t* sys_jacobi)
+{
+    if (!g || !gt) return fail(PM_INVALID_ARGUMENT, "batched factored pcg: null factor");
+    PM_TRY(check_csr(g, "batched factored pcg (G)"));
+    PM_TRY(check_csr(gt, "batched factored pcg (G^T)"));
+    const pm_csr* a = b.a;
+    for (const pm_csr* m : {g, gt})
+    {
+        if (m->rows != a->rows || m->cols != a->cols)
+            return fail(PM_INVALID_ARGUMENT, "batched factored pcg: factor shape %lld x %lld does not match A",
+                        static_cast<long long>(m->rows), static_cast<long long>(m->cols));
+        if (m->dtype != a->dtype) return fail(PM_INVALID_ARGUMENT, "batched factored pcg: factor dtype differs from A");
+        if (use_compact(a) ? (!m->row_offsets32 || !m->col_indices32) : (!m->row_offsets || !m->col_indices))
+            return fail(PM_INVALID_ARGUMENT, "batched factored pcg: factor lacks the index arrays of A's width");
+    }
+    if (g->nnz != gt->nnz) return fail(PM_INVALID_ARGUMENT, "batched factored pcg: G and G^T differ in nnz");
+    b.g = g;
+    b.gt = gt;
+    b.sys_jacobi = sys_jacobi;
+    b.jacobi = false;
     return PM_OK;
 }
 
@@ -1099,10 +1286,11 @@ int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, con
     return batch_solve<double>(b, pol, reports, ws, stream);
 }
 
-int pm_batch_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
-                        const double* rhs, double* x, const pm_solver_options* inner, double target_relres,
-                        int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
-                        pm_workspace* ws, hipStream_t stream)
+static int batch_refine_entry(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                              const double* rhs, double* x, const pm_csr* g, const pm_csr* gt,
+                              const uint8_t* sys_jacobi, const pm_solver_options* inner, double target_relres,
+                              int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
+                              pm_workspace* ws, hipStream_t stream)
 {
     if (!(target_relres >= 0.0) || max_outer < 0)
         return fail(PM_INVALID_ARGUMENT, "batched refinement: invalid target or outer limit");
@@ -1116,8 +1304,43 @@ int pm_batch_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_beg
     // only checked for presence
     BatchProblem b;
     PM_TRY(batch_setup(a, systems, row_begin, row_end, rhs, x, &in, reports != nullptr, ws, stream, b));
+    if (g || gt) PM_TRY(batch_attach_factors(b, g, gt, sys_jacobi));
     if (systems == 0) return PM_OK;
     return batch_refine(b, rhs, x, target_relres, max_outer, resolve_batch_policy(policy), reports, ws, stream);
+}
+
+int pm_batch_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                        const double* rhs, double* x, const pm_solver_options* inner, double target_relres,
+                        int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
+                        pm_workspace* ws, hipStream_t stream)
+{
+    return batch_refine_entry(a, systems, row_begin, row_end, rhs, x, nullptr, nullptr, nullptr, inner, target_relres,
+                              max_outer, policy, reports, ws, stream);
+}
+
+int pm_batch_pcg_factored(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                          const void* rhs, void* x, const pm_csr* g, const pm_csr* gt, const uint8_t* jacobi_systems,
+                          const pm_solver_options* options, const pm_solve_policy* policy, pm_batch_report* reports,
+                          pm_workspace* ws, hipStream_t stream)
+{
+    BatchProblem b;
+    PM_TRY(batch_setup(a, systems, row_begin, row_end, rhs, x, options, reports != nullptr, ws, stream, b));
+    PM_TRY(batch_attach_factors(b, g, gt, jacobi_systems));
+    if (systems == 0) return PM_OK;
+    const BatchPolicy pol = resolve_batch_policy(policy);
+    if (a->dtype == PM_F32) return batch_solve<float>(b, pol, reports, ws, stream);
+    return batch_solve<double>(b, pol, reports, ws, stream);
+}
+
+int pm_batch_pcg_refine_factored(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                                 const double* rhs, double* x, const pm_csr* g, const pm_csr* gt,
+                                 const uint8_t* jacobi_systems, const pm_solver_options* inner, double target_relres,
+                                 int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
+                                 pm_workspace* ws, hipStream_t stream)
+{
+    if (!g || !gt) return fail(PM_INVALID_ARGUMENT, "batched factored refinement: null factor");
+    return batch_refine_entry(a, systems, row_begin, row_end, rhs, x, g, gt, jacobi_systems, inner, target_relres,
+                              max_outer, policy, reports, ws, stream);
 }
 
 } // extern "C"

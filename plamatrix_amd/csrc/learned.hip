@@ -713,6 +713,63 @@ __global__ __launch_bounds__(kBlock) void k_residual_probes(int64_t n, int probe
     }
 }
 
+// Per-system choice between the learned factor and Jacobi.  Per row, over the
+// probes: e[2i] = sum_p (z - G A G^T z)_i^2 (w holds G A G^T z) and, once the
+// Jacobi product is in v (v = A D^-1/2 z), e[2i+1] = sum_p (z - D^-1/2 v)_i^2.
+template <bool JACOBI>
+__global__ __launch_bounds__(kBlock) void k_select_rows(int64_t n, int probes, const float* __restrict__ z,
+                                                        const float* __restrict__ w, const float* __restrict__ dinv,
+                                                        double* __restrict__ e)
+{
+    for (int64_t i = gid(); i < n; i += gstr())
+    {
+        double acc = 0.0;
+        const float d = JACOBI ? dinv[i] : 1.f;
+        for (int p = 0; p < probes; ++p)
+        {
+            const float wi = JACOBI ? d * w[i * probes + p] : w[i * probes + p];
+            const double ri = static_cast<double>(z[i * probes + p] - wi);
+            acc = __builtin_fma(ri, ri, acc);
+        }
+        e[2 * i + (JACOBI ? 1 : 0)] = acc;
+    }
+}
+
+// y[i][p] = dinv[i] x[i][p]
+__global__ __launch_bounds__(kBlock) void k_scale_probe_rows(int64_t n, int probes, const float* __restrict__ dinv,
+                                                             const float* __restrict__ x, float* __restrict__ y)
+{
+    for (int64_t t = gid(); t < n * probes; t += gstr()) y[t] = dinv[t / probes] * x[t];
+}
+
+// One wavefront per system: the two probe losses summed over its rows in a
+// fixed order; Jacobi is chosen when the learned factor does not lower it.
+__global__ __launch_bounds__(kBlock) void k_select_systems(int64_t systems, const int64_t* __restrict__ range,
+                                                           const double* __restrict__ e, double* __restrict__ loss,
+                                                           uint8_t* __restrict__ jacobi)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / kWave);
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; s < systems;
+         s += waves)
+    {
+        double l = 0.0, j = 0.0;
+        for (int64_t i = range[s] + lane; i < range[systems + s]; i += kWave)
+        {
+            l += e[2 * i];
+            j += e[2 * i + 1];
+        }
+        l = wave_sum(l);
+        j = wave_sum(j);
+        if (lane == 0)
+        {
+            loss[2 * s] = l;
+            loss[2 * s + 1] = j;
+            jacobi[s] = l < j ? 0 : 1;
+        }
+    }
+}
+
 // loss and per-probe scale c_p = 1 / (P ||z_p||^2); stat[0] = loss.
 __global__ void k_loss(const double* __restrict__ part, int blocks, int probes, double* __restrict__ stat)
 {
@@ -934,6 +991,7 @@ struct pm_fsai_graph
     DeviceBuffer act;      // activations and gradients
     DeviceBuffer parts;    // per-workgroup partials
     DeviceBuffer tmp;      // graph-build counts and scan scratch
+    DeviceBuffer select;   // per-system ranges and losses of pm_fsai_batch_select
     int device = 0;
     float* dinv = nullptr;
     float* s = nullptr;
@@ -1506,6 +1564,69 @@ int pm_fsai_factors(const pm_fsai_graph* gr, pm_csr* g, pm_csr* gt)
     if (!gr || !g || !gt) return fail(PM_INVALID_ARGUMENT, "pm_fsai_factors: null argument");
     *g = pm_csr{gr->n, gr->n, gr->nl, gr->lro64, gr->lci64, gr->g, gr->dtype, gr->lro, gr->lci};
     *gt = pm_csr{gr->n, gr->n, gr->nl, gr->uro64, gr->uci64, gr->gt, gr->dtype, gr->uro, gr->uci};
+    return PM_OK;
+}
+
+int pm_fsai_batch_select(pm_fsai_graph* gr, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                         int32_t probes, uint64_t seed, uint8_t* jacobi_systems, double* losses, hipStream_t stream)
+{
+    if (!gr || systems < 0 || (systems > 0 && (!row_begin || !row_end || !jacobi_systems)))
+        return fail(PM_INVALID_ARGUMENT, "pm_fsai_batch_select: null argument");
+    if (probes < 1 || probes > kMaxProbes) return fail(PM_INVALID_ARGUMENT, "probes must be in [1, %d]", kMaxProbes);
+    if (systems == 0) return PM_OK;
+    const int64_t n = gr->n;
+    std::vector<int64_t> range(static_cast<size_t>(2 * systems));
+    for (int64_t s = 0; s < systems; ++s)
+    {
+        if (row_begin[s] < 0 || row_end[s] < row_begin[s] || row_end[s] > n)
+            return fail(PM_INVALID_ARGUMENT, "pm_fsai_batch_select: system %lld row range [%lld, %lld) is outside "
+                        "the %lld-row graph", static_cast<long long>(s), static_cast<long long>(row_begin[s]),
+                        static_cast<long long>(row_end[s]), static_cast<long long>(n));
+        range[static_cast<size_t>(s)] = row_begin[s];
+        range[static_cast<size_t>(systems + s)] = row_end[s];
+    }
+    PM_TRY(gr->select.reserve(sizeof(int64_t) * 2 * systems + sizeof(double) * 2 * systems + 256));
+    int64_t* drange = gr->select.as<int64_t>();
+    double* dloss = reinterpret_cast<double*>(drange + 2 * systems);
+    PM_HIP(hipMemcpyAsync(drange, range.data(), sizeof(int64_t) * 2 * systems, hipMemcpyHostToDevice, stream));
+    size_t bytes = 0;
+    const Act a = carve_act(gr, &bytes);
+    double* e = reinterpret_cast<double*>(a.q);  // n x 8 floats hold the n x 2 doubles
+    const int gp = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBlock / kMaxProbes), 16384)));
+    const int gn = grid_of(n * probes);
+    auto spmm = [&](const int32_t* ro, const int32_t* ci, const void* val, const float* x, float* y) {
+        if (gr->dtype == PM_F32)
+            hipLaunchKernelGGL(k_spmm8<float>, dim3(gp), dim3(kBlock), 0, stream, n, probes, ro, ci,
+                               static_cast<const float*>(val), x, y);
+        else
+            hipLaunchKernelGGL(k_spmm8<double>, dim3(gp), dim3(kBlock), 0, stream, n, probes, ro, ci,
+                               static_cast<const double*>(val), x, y);
+    };
+    hipLaunchKernelGGL(k_probes, dim3(gn), dim3(kBlock), 0, stream, n, probes, static_cast<unsigned long long>(seed),
+                       a.z);
+    spmm(gr->uro, gr->uci, gr->gt, a.z, a.u);     // u = G^T z
+    spmm(gr->ro, gr->ci, gr->values, a.u, a.v);   // v = A u
+    spmm(gr->lro, gr->lci, gr->g, a.v, a.w);      // w = G v
+    hipLaunchKernelGGL(k_select_rows<false>, dim3(grid_of(n)), dim3(kBlock), 0, stream, n, probes, a.z, a.w,
+                       gr->dinv, e);
+    hipLaunchKernelGGL(k_scale_probe_rows, dim3(gn), dim3(kBlock), 0, stream, n, probes, gr->dinv, a.z, a.u);
+    spmm(gr->ro, gr->ci, gr->values, a.u, a.v);   // v = A D^-1/2 z
+    hipLaunchKernelGGL(k_select_rows<true>, dim3(grid_of(n)), dim3(kBlock), 0, stream, n, probes, a.z, a.v,
+                       gr->dinv, e);
+    const int gs = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(systems, kBlock / kWave), 1024)));
+    hipLaunchKernelGGL(k_select_systems, dim3(gs), dim3(kBlock), 0, stream, systems, drange, e, dloss,
+                       jacobi_systems);
+    PM_HIP(hipGetLastError());
+    if (losses)
+    {
+        PM_HIP(hipMemcpyAsync(losses, dloss, sizeof(double) * 2 * systems, hipMemcpyDeviceToHost, stream));
+        PM_HIP(hipStreamSynchronize(stream));
+    }
+    else
+    {
+        // the host range array goes out of scope
+        PM_HIP(hipStreamSynchronize(stream));
+    }
     return PM_OK;
 }
 

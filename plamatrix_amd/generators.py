@@ -141,10 +141,12 @@ def grid3d_dims(n: int) -> tuple:
     return nx, ny, max(1, int(round(n / (nx * ny))))
 
 
-def block_diagonal(systems: list, dtype=np.float32):
+def block_diagonal(systems: list, dtype=np.float32, pad_identity: bool = False):
     """Pack CSR systems as the diagonal blocks of one matrix, each starting at
-    a multiple of 64 rows (padding rows are empty).  Returns the CSR and the
-    row_begin / row_end arrays."""
+    a multiple of 64 rows.  Padding rows are empty, or carry a unit diagonal
+    with pad_identity (so the whole matrix is SPD and a learned graph can be
+    built over it in one go).  Returns the CSR and the row_begin / row_end
+    arrays."""
     sizes = np.array([s.rows for s in systems], dtype=np.int64)
     begin = np.zeros(len(systems), dtype=np.int64)
     cursor = 0
@@ -154,13 +156,25 @@ def block_diagonal(systems: list, dtype=np.float32):
     total = cursor
     end = begin + sizes
     counts = np.zeros(total, dtype=np.int64)
+    pad = np.ones(total, dtype=bool)
     for s, b0 in zip(systems, begin):
         counts[b0:b0 + s.rows] = np.diff(s.row_offsets)
+        pad[b0:b0 + s.rows] = False
+    if pad_identity:
+        counts[pad] = 1
     ro = np.zeros(total + 1, dtype=np.int64)
     np.cumsum(counts, out=ro[1:])
-    ci = np.concatenate([s.col_indices + b0 for s, b0 in zip(systems, begin)]) if systems else np.zeros(0, np.int64)
-    vals = np.concatenate([s.values.astype(dtype) for s in systems]) if systems else np.zeros(0, dtype)
-    return CSRHost(total, total, ro, ci.astype(np.int64), vals), begin, end
+    ci = np.zeros(int(ro[-1]), dtype=np.int64)
+    vals = np.zeros(int(ro[-1]), dtype=dtype)
+    for s, b0 in zip(systems, begin):
+        k0, k1 = ro[b0], ro[b0 + s.rows]
+        ci[k0:k1] = s.col_indices + b0
+        vals[k0:k1] = s.values.astype(dtype)
+    if pad_identity:
+        rows = np.nonzero(pad)[0]
+        ci[ro[rows]] = rows
+        vals[ro[rows]] = 1.0
+    return CSRHost(total, total, ro, ci, vals), begin, end
 
 
 def rhs_uniform(n: int, seed: int = 1, dtype=np.float64) -> np.ndarray:

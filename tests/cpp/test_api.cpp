@@ -851,7 +851,11 @@ static void schur()
         auto ob = o;
         ob.linearBackend = backend;
         const auto rb = solveDampedSchurComplement(eq, 0.1, ob, &dp, &de);
-        EXPECT(rb.converged && rb.linearBackend == backend && !rb.deviceName.empty() && rb.linearSolveSeconds > 0.0);
+        if (!rb.converged) std::printf("  backend %d: %s\n", static_cast<int>(backend), rb.message.c_str());
+        EXPECT(rb.converged);
+        EXPECT(rb.linearBackend == backend);
+        EXPECT(!rb.deviceName.empty());
+        EXPECT(rb.linearSolveSeconds > 0.0);
         for (int i = 0; i < 4; ++i) EXPECT(std::fabs(dp[static_cast<std::size_t>(i)] - expected[i]) <= 1e-10);
         for (int i = 0; i < 2; ++i) EXPECT(std::fabs(de[static_cast<std::size_t>(i)] - expected[4 + i]) <= 1e-10);
         const auto rm = solveDampedSchurComplement(multi, 0.1, ob, &dp, &de);
