@@ -583,6 +583,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(int64_t systems, const 
     }
 }
 
+// Stopped-system count into the host's fine-grained status slot.
+__global__ void k_batch_publish(const unsigned long long* __restrict__ done_count, unsigned long long* host)
+{
+    if (threadIdx.x == 0) *host = *done_count;
+}
+
 // ---- host side -----------------------------------------------------------------------
 
 struct BatchProblem
@@ -837,7 +843,11 @@ int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* rep
         const int64_t phase = policy.refresh > 0 ? launched % policy.refresh : 0;
         PM_TRY(batch_plan<S>(ws, b, 1, phase, K, policy.refresh)->run(stream, policy.graph));
         const int slot = static_cast<int>(chunk_id % 2);
-        PM_HIP(hipMemcpyAsync(hs + slot, b.done_count, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+        // the count goes straight into fine-grained host memory (a one-wave
+        // kernel instead of a copy command per chunk)
+        hipLaunchKernelGGL(k_batch_publish, dim3(1), dim3(kWave), 0, stream,
+                           static_cast<const unsigned long long*>(b.done_count), hs + slot);
+        PM_HIP(hipGetLastError());
         PM_HIP(hipEventRecord(ws->events[slot], stream));
         inflight.push_back(chunk_id);
         ++chunk_id;
@@ -941,7 +951,7 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     PM_TRY(ws->batch.reserve(bytes));
     if (!ws->host_batch)
         PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_batch), 2 * sizeof(unsigned long long),
-                             hipHostMallocDefault));
+                             hipHostMallocCoherent));
     b = BatchProblem{};
     b.a = a;
     b.rhs = rhs;

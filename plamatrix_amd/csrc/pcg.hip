@@ -678,6 +678,277 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaAr
     publish_partials<2>(part, partials + kPartialsRes);
 }
 
+// ---- single-reduction iteration (Chronopoulos & Gear, 1989) ---------------------
+//
+// CG / Jacobi-PCG with one reduction phase per iteration, two kernels:
+//   k_cg_step  reduce r.r, gamma = r.u, delta = w.u of the previous SpMV;
+//              stop test on ||r||; beta = gamma / gamma_prev (betaKernel's
+//              rules), alpha = gamma / (delta - beta gamma / alpha_prev), which
+//              equals p.Ap in exact arithmetic (alphaKernel's rules applied to
+//              it; a non-positive value restarts the step from u); p = u + beta p,
+//              s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r
+//   k_cg_spmv  w = A u and the partials r.r, (r.u,) w.u
+// The recurrence s = A p replaces the SpMV of the direction, so the
+// iteration streams the same bytes as the three-kernel form with one kernel
+// boundary and one cross-workgroup reduction fewer.  u is r itself without a
+// preconditioner (the partial r.u is then r.r), M^-1 r with Jacobi.  State
+// slots alternate with the iteration parity like rho / iter.
+
+template <int PRE>
+constexpr int cg_values()
+{
+    return PRE == kPreJacobi ? 3 : 2;  // r.r, [r.u], w.u
+}
+
+// The decision at the head of single-reduction iteration t (slot par).
+template <typename S, int PRE>
+__device__ __forceinline__ int cg_decision(SolverState* st, const double* partials, int par, int count,
+                                           double& alpha, double& beta)
+{
+    constexpr int NV = cg_values<PRE>();
+    double red[NV];
+    reduce_partials<NV>(partials + kPartialsRes, count, red);
+    const double rr = red[0];
+    const double gamma = PRE == kPreJacobi ? red[1] : red[0];
+    const double delta = red[NV - 1];
+    const int completed = st->iter[1 - par] + 1;
+    const double tol = completed != 0   ? st->tol
+                       : st->tol_rule   ? st->abs_tol + st->rel_tol * sqrt(rr)
+                                        : fmax(st->abs_tol, st->rel_tol * sqrt(rr));
+    int code = kRunning;
+    beta = 0.0;
+    alpha = 0.0;
+    if (!isfinite(rr) || rr < 0.0) code = completed == 0 ? kInitNonFinite : kNonFinite;
+    else if (sqrt(rr) <= tol) code = kConverged;
+    else if (completed == 0 && st->bad_row != ~0ull) code = kBadDiagonal;
+    else if (completed >= st->max_iterations) code = kIterationLimit;
+    else
+    {
+        double den = delta;
+        if (completed > 0)
+        {
+            const double gamma_prev = st->rho[1 - par];
+            if (gamma_prev == 0.0 && gamma == 0.0) beta = 0.0;
+            else if (isfinite(gamma_prev) && isfinite(gamma) && gamma_prev > 0.0 && gamma > 0.0)
+            {
+                beta = gamma / gamma_prev;
+                if (!isfinite(beta) || !isfinite(static_cast<S>(beta))) code = kBreakdown;
+            }
+            else code = kBreakdown;
+            const double alpha_prev = st->alpha_par[1 - par];
+            if (code == kRunning) den = alpha_prev != 0.0 ? delta - beta * gamma / alpha_prev : delta;
+            // Rounding (FP32 vectors, a residual refresh that replaced r) can
+            // cost the recurrence its positivity; the step then restarts from
+            // u (beta = 0, p.Ap = u.Au = delta computed directly), and only a
+            // non-positive delta is a breakdown of A itself.
+            if (code == kRunning && !(den > 0.0 && isfinite(den)))
+            {
+                beta = 0.0;
+                den = delta;
+            }
+        }
+        if (code == kRunning && !step_length<S>(gamma, den, alpha)) code = kBreakdown;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+    {
+        st->rr = rr;
+        st->iteration = completed;
+        st->iter[par] = completed;
+        st->rho[par] = gamma;
+        st->alpha_par[par] = alpha;
+        st->alpha = alpha;
+        if (completed == 0)
+        {
+            st->rr0 = rr;
+            st->tol = tol;
+        }
+        if (code != kRunning) st->done = code;
+    }
+    return code;
+}
+
+// The vector half of single-reduction iteration t.  STEP_R = false (a residual
+// refresh follows) leaves r and u to the refresh.
+template <typename S, int PRE, bool STEP_R>
+__global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, S* __restrict__ x, S* __restrict__ r,
+                                                    S* __restrict__ u, S* __restrict__ p, S* __restrict__ s,
+                                                    const S* __restrict__ w, const S* __restrict__ inv,
+                                                    SolverState* st, double* partials, int par, int count)
+{
+    if (st->done != kRunning) return;
+    constexpr int U = 4;
+    int64_t begin, end;
+    block_range(rows, U * kBlock, begin, end);
+    const int64_t first = begin + threadIdx.x;
+    // the first step's operands are loaded before the cross-workgroup
+    // reduction so their latency overlaps it
+    S fr[U], fu[U], fp[U], fs[U], fw[U], fx[U], fi[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+    {
+        const int64_t i = first + j * kBlock;
+        const bool in = i < end;
+        fr[j] = in ? r[i] : S(0);
+        fu[j] = PRE == kPreJacobi ? (in ? u[i] : S(0)) : fr[j];
+        fp[j] = in ? p[i] : S(0);
+        fs[j] = in ? s[i] : S(0);
+        fw[j] = in ? w[i] : S(0);
+        fx[j] = in ? x[i] : S(0);
+        fi[j] = PRE == kPreJacobi && in ? inv[i] : S(0);
+    }
+    double alpha_d = 0.0, beta_d = 0.0;
+    if (cg_decision<S, PRE>(st, partials, par, count, alpha_d, beta_d) != kRunning) return;
+    const S a = static_cast<S>(alpha_d), b = static_cast<S>(beta_d);
+    for (int64_t base = first; base < end; base += U * kBlock)
+    {
+        const bool pre = base == first;
+#pragma unroll
+        for (int j = 0; j < U; ++j)
+        {
+            const int64_t i = base + j * kBlock;
+            if (i >= end) continue;
+            const S ri = pre ? fr[j] : r[i];
+            const S ui = PRE == kPreJacobi ? (pre ? fu[j] : u[i]) : ri;
+            const S pn = fma_s(b, pre ? fp[j] : p[i], ui);
+            const S sn = fma_s(b, pre ? fs[j] : s[i], pre ? fw[j] : w[i]);
+            p[i] = pn;
+            s[i] = sn;
+            x[i] = fma_s(a, pn, pre ? fx[j] : x[i]);
+            if (STEP_R)
+            {
+                const S rn = fma_s(-a, sn, ri);
+                r[i] = rn;
+                if (PRE == kPreJacobi) u[i] = (pre ? fi[j] : inv[i]) * rn;
+            }
+        }
+    }
+}
+
+// u = M^-1 r after a residual refresh or at initialisation (Jacobi only).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_cg_precondition(int64_t rows, const S* __restrict__ r,
+                                                            const S* __restrict__ inv, S* __restrict__ u,
+                                                            const SolverState* st)
+{
+    if (st->done != kRunning) return;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < rows;
+         i += static_cast<int64_t>(gridDim.x) * kBlock)
+        u[i] = inv[i] * r[i];
+}
+
+// Initialisation after the residual: s = 0 (its first update is s = w), and
+// with Jacobi u = M^-1 r.
+template <typename S, int PRE>
+__global__ __launch_bounds__(kBlock) void k_cg_init(int64_t rows, const S* __restrict__ r, const S* __restrict__ inv,
+                                                    S* __restrict__ u, S* __restrict__ s, const SolverState* st)
+{
+    if (st->done != kRunning) return;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < rows;
+         i += static_cast<int64_t>(gridDim.x) * kBlock)
+    {
+        s[i] = S(0);
+        if (PRE == kPreJacobi) u[i] = inv[i] * r[i];
+    }
+}
+
+// w = A u (CSR) and the partials of the next decision.
+template <typename S, typename I, int PRE>
+__global__ __launch_bounds__(kDirBlock) void k_cg_spmv(int64_t rows, const I* __restrict__ ro,
+                                                       const I* __restrict__ ci, const S* __restrict__ v,
+                                                       const S* __restrict__ r, const S* __restrict__ u,
+                                                       S* __restrict__ w, const SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
+    __shared__ S lv_all[kDirWaves][kStreamCap];
+    __shared__ I lc_all[kDirWaves][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const S* uu = PRE == kPreJacobi ? u : r;
+    double rr = 0.0, ru = 0.0, wu = 0.0;
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    for (int64_t c = cbegin + wave; c < cend; c += kDirWaves)
+    {
+        const int64_t row0 = c * kChunkRows;
+        int64_t ks, ke;
+        const S acc = chunk_row_sums<S, I, SpmvSource<S>, false, false>(row0, rows, ro, ci, lv_all[wave],
+                                                                        lc_all[wave], SpmvSource<S>{v, uu}, ks, ke);
+        const int64_t row = row0 + lane;
+        if (row < rows)
+        {
+            w[row] = acc;
+            const double rv = static_cast<double>(r[row]);
+            const double uv = static_cast<double>(uu[row]);
+            rr = __builtin_fma(rv, rv, rr);
+            if (PRE == kPreJacobi) ru = __builtin_fma(rv, uv, ru);
+            wu = __builtin_fma(static_cast<double>(acc), uv, wu);
+        }
+    }
+    if constexpr (PRE == kPreJacobi)
+    {
+        double part[3] = {rr, ru, wu};
+        publish_partials<3>(part, partials + kPartialsRes);
+    }
+    else
+    {
+        double part[2] = {rr, wu};
+        publish_partials<2>(part, partials + kPartialsRes);
+    }
+}
+
+// k_cg_spmv on the diagonal layout.
+template <typename S, int ND, int PRE>
+__global__ __launch_bounds__(kDirBlock) void k_cg_spmv_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                           const uint16_t* __restrict__ mask,
+                                                           const S* __restrict__ r, const S* __restrict__ u,
+                                                           S* __restrict__ w, const SolverState* st,
+                                                           double* partials)
+{
+    if (st->done != kRunning) return;
+    const int lane = threadIdx.x % kWave;
+    const S* uu = PRE == kPreJacobi ? u : r;
+    double rr = 0.0, ru = 0.0, wu = 0.0;
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    for (int64_t c = cbegin + threadIdx.x / kWave; c < cend; c += kDirWaves)
+    {
+        const int64_t row = c * kChunkRows + lane;
+        if (row < rows)
+        {
+            const S acc = dia_row<S, ND>(row, rows, dia, dv, mask[row], uu);
+            w[row] = acc;
+            const double rv = static_cast<double>(r[row]);
+            const double uv = static_cast<double>(uu[row]);
+            rr = __builtin_fma(rv, rv, rr);
+            if (PRE == kPreJacobi) ru = __builtin_fma(rv, uv, ru);
+            wu = __builtin_fma(static_cast<double>(acc), uv, wu);
+        }
+    }
+    if constexpr (PRE == kPreJacobi)
+    {
+        double part[3] = {rr, ru, wu};
+        publish_partials<3>(part, partials + kPartialsRes);
+    }
+    else
+    {
+        double part[2] = {rr, wu};
+        publish_partials<2>(part, partials + kPartialsRes);
+    }
+}
+
+// The solver state into the host's fine-grained status slot, 8 bytes per lane.
+__global__ void k_publish_state(const SolverState* __restrict__ st, SolverState* host)
+{
+    static_assert(sizeof(SolverState) % 8 == 0 && sizeof(SolverState) / 8 <= kWave, "state record size");
+    const auto* src = reinterpret_cast<const unsigned long long*>(st);
+    auto* dst = reinterpret_cast<unsigned long long*>(host);
+    if (threadIdx.x < sizeof(SolverState) / 8) dst[threadIdx.x] = src[threadIdx.x];
+}
+
 // Squared residuals of a fixed-iteration solve (toDoubleKernel /
 // toDoubleCheckedKernel, iterative_solver_kernels.cuh:240-252).
 __global__ __launch_bounds__(kBlock) void k_fixed_out(const SolverState* st, const double* partials, int count_res,
@@ -715,6 +986,7 @@ struct Problem
     bool fuse_direction;     // on the diagonal layout: p_new formed inside the SpMV
     bool fuse_step;          // factored: the step folded into G r and G^T t
     int fuse_step_env;       // PLAMATRIX_AMD_FUSE_STEP: 0 off, 1 on, -1 unset
+    bool single_reduction;   // CG / Jacobi-PCG as the two-kernel Chronopoulos-Gear iteration
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     SolverState* st;
@@ -772,8 +1044,67 @@ struct Calls
                                     ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
                                     static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(nullptr),
                                     static_cast<S*>(pb.p0), pb.st));
+        if (pb.single_reduction)
+        {
+            const auto k = pb.pre == kPreJacobi ? k_cg_init<S, kPreJacobi> : k_cg_init<S, kPreNone>;
+            out.push_back(make_call(k, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<const S*>(pb.r),
+                                    static_cast<const S*>(pb.inv), static_cast<S*>(pb.z), static_cast<S*>(pb.p1),
+                                    static_cast<const SolverState*>(pb.st)));
+            cg_spmv(pb, out);
+            return;
+        }
         if (pb.pre == kPreFactored) factored_apply(pb, out);
         else out.push_back(update_any<kModeInit>(pb, pb.p0, 0));
+    }
+
+    // Single-reduction iteration: w = A u and the partials (grid_rows
+    // workgroups).  Vectors: u = z (Jacobi) or r, w = q, p = p0, s = p1.
+    static void cg_spmv(const Problem& pb, std::vector<KernelCall>& out)
+    {
+        const bool jac = pb.pre == kPreJacobi;
+        const int nd = pb.dia.args.ndiag;
+        if (pb.dia.active)
+        {
+            const auto k = jac ? (nd == 7 ? k_cg_spmv_dia<S, 7, kPreJacobi> : nd == 5 ? k_cg_spmv_dia<S, 5, kPreJacobi>
+                                                                                  : k_cg_spmv_dia<S, 0, kPreJacobi>)
+                               : (nd == 7 ? k_cg_spmv_dia<S, 7, kPreNone> : nd == 5 ? k_cg_spmv_dia<S, 5, kPreNone>
+                                                                                : k_cg_spmv_dia<S, 0, kPreNone>);
+            out.push_back(make_call(k, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, pb.dia.args,
+                                    static_cast<const S*>(pb.dia.values), pb.dia.mask, static_cast<const S*>(pb.r),
+                                    static_cast<const S*>(pb.z), static_cast<S*>(pb.q),
+                                    static_cast<const SolverState*>(pb.st), pb.partials));
+            return;
+        }
+        const auto k = jac ? k_cg_spmv<S, I, kPreJacobi> : k_cg_spmv<S, I, kPreNone>;
+        out.push_back(make_call(k, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a),
+                                static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.r),
+                                static_cast<const S*>(pb.z), static_cast<S*>(pb.q),
+                                static_cast<const SolverState*>(pb.st), pb.partials));
+    }
+
+    static void cg_iteration(const Problem& pb, int64_t t, bool refresh, std::vector<KernelCall>& out)
+    {
+        const int par = static_cast<int>(t % 2);
+        const bool jac = pb.pre == kPreJacobi;
+        const auto step = refresh ? (jac ? k_cg_step<S, kPreJacobi, false> : k_cg_step<S, kPreNone, false>)
+                                  : (jac ? k_cg_step<S, kPreJacobi, true> : k_cg_step<S, kPreNone, true>);
+        out.push_back(make_call(step, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<S*>(pb.x),
+                                static_cast<S*>(pb.r), static_cast<S*>(pb.z), static_cast<S*>(pb.p0),
+                                static_cast<S*>(pb.p1), static_cast<const S*>(pb.q), static_cast<const S*>(pb.inv),
+                                pb.st, pb.partials, par, pb.grid_rows));
+        if (refresh)
+        {
+            out.push_back(make_call(k_residual<S, I, false, false>, dim3(full_grid(pb.a->rows)), dim3(kBlock),
+                                    pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
+                                    static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x),
+                                    static_cast<S*>(pb.r), static_cast<S*>(nullptr), static_cast<S*>(nullptr),
+                                    pb.st));
+            if (jac)
+                out.push_back(make_call(k_cg_precondition<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows,
+                                        static_cast<const S*>(pb.r), static_cast<const S*>(pb.inv),
+                                        static_cast<S*>(pb.z), static_cast<const SolverState*>(pb.st)));
+        }
+        cg_spmv(pb, out);
     }
 
     // z = G^T (G r) and the r.r, r.z partials (grid_rows workgroups).  With
@@ -837,6 +1168,11 @@ struct Calls
     // Iteration t: t % 2 picks the p buffers and the state slots.
     static void iteration(const Problem& pb, int64_t t, bool refresh, std::vector<KernelCall>& out)
     {
+        if (pb.single_reduction)
+        {
+            cg_iteration(pb, t, refresh, out);
+            return;
+        }
         const int par = static_cast<int>(t % 2);
         const void* p_old = par == 0 ? pb.p0 : pb.p1;
         void* p_new = par == 0 ? pb.p1 : pb.p0;
@@ -953,6 +1289,16 @@ bool fuse_step_wanted(const Problem& pb)
 
 size_t elem_size(int32_t dtype) { return dtype == PM_F64 ? 8 : 4; }
 
+// CG and Jacobi-PCG run as the two-kernel single-reduction iteration in the
+// adaptive solves (PLAMATRIX_AMD_SINGLE_REDUCTION=0 keeps the three-kernel
+// form, which the fused-direction option also needs).
+bool single_reduction_wanted(const Problem& pb)
+{
+    const char* env = std::getenv("PLAMATRIX_AMD_SINGLE_REDUCTION");
+    if (env != nullptr && env[0] == '0') return false;
+    return (pb.pre == kPreNone || pb.pre == kPreJacobi) && !pb.fuse_direction;
+}
+
 // Reserve workspace storage and fill the workspace part of a Problem.
 int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
 {
@@ -1031,7 +1377,8 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
                   static_cast<void*>(pb.st), static_cast<void*>(pb.partials), pb.rhs, pb.x, pb.inv_blocks, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
                   static_cast<long long>(pb.bsize),
-                  (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0),
+                  (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0) |
+                      (pb.single_reduction ? 16 : 0),
                   pb.dia.active ? 1 : 0, pb.dia.args.ndiag, dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask));
     std::string key(buf);
     append_matrix_key(key, pb.a);
@@ -1142,6 +1489,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         if (!pb.dia_g.active || !pb.dia_gt.active) pb.dia_g.active = pb.dia_gt.active = false;
     }
     pb.fuse_step = fuse_step_wanted(pb);
+    pb.single_reduction = single_reduction_wanted(pb);
     ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
     ws->last_preconditioner = pb.pre;
     if (a->rows == 0)
@@ -1170,7 +1518,11 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         Plan* plan = get_plan<S>(ws, pb, 1, phase, K, policy.refresh);
         PM_TRY(plan->run(stream, policy.graph));
         const int slot = static_cast<int>(chunk_id % 2);
-        PM_HIP(hipMemcpyAsync(hs + slot, pb.st, sizeof(SolverState), hipMemcpyDeviceToHost, stream));
+        // the state record goes straight into fine-grained host memory (a
+        // one-wave kernel instead of a copy command per chunk)
+        hipLaunchKernelGGL(k_publish_state, dim3(1), dim3(kWave), 0, stream, static_cast<const SolverState*>(pb.st),
+                           hs + slot);
+        PM_HIP(hipGetLastError());
         PM_HIP(hipEventRecord(ws->events[slot], stream));
         inflight.push_back(chunk_id);
         ++chunk_id;

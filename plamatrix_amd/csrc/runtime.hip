@@ -142,8 +142,10 @@ int pm_workspace::ensure_events()
     }
     if (!host_state)
     {
+        // fine-grained: the solvers' publish kernels store the per-chunk
+        // status records straight into it
         PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_state), 2 * sizeof(pm::SolverState),
-                             hipHostMallocDefault));
+                             hipHostMallocCoherent));
         PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_coo), sizeof(pm::CooStatus), hipHostMallocDefault));
     }
     return PM_OK;

@@ -96,7 +96,16 @@ def test_forward_matches_oracle(dev):
     rows, cols, ref = fsai.factors(h.row_offsets, h.col_indices, h.values, p)
     assert np.array_equal(np.repeat(np.arange(h.rows), np.diff(ro)), rows) and np.array_equal(ci, cols)
     err = np.abs(vals - ref) / (np.abs(ref) + 1e-3)
-    assert err.max() < 2e-2, err.max()
+    norm_err = float(np.linalg.norm(vals - ref) / np.linalg.norm(ref))
+    print(f"forward: max elementwise {err.max():.3e}, 99th percentile {np.percentile(err, 99):.3e}, "
+          f"median {np.median(err):.3e}, norm-wise {norm_err:.3e}")
+    # the oracle rounds to BF16 at the same points; the FP32 accumulation
+    # order inside the MFMA differs, so an activation sitting on a BF16
+    # rounding boundary can land one BF16 step (2^-8 relative) away and carry
+    # that step through the later layers.  Measured on this system: max
+    # 4.8e-4, 99th percentile 2.3e-5, norm-wise 6.8e-6.
+    assert err.max() < 2e-3, err.max()
+    assert np.percentile(err, 99) < 2e-4 and norm_err < 1e-4, (np.percentile(err, 99), norm_err)
     # G^T is the transpose of G
     tro, tci, tvals = capi.csr_arrays(gt)
     import scipy.sparse as sp
@@ -187,13 +196,18 @@ def test_loss_and_gradient_match_oracle(dev):
     graph = capi.FsaiGraph(a)
     loss, grad = capi.fsai_loss_grad(net, graph, probes=8, seed=21)
     ref_loss, ref_grad = fsai.loss_and_grad(h.row_offsets, h.col_indices, h.values, p, 8, 21)
-    assert abs(loss - ref_loss) <= 2e-3 * ref_loss, (loss, ref_loss)
+    # measured: loss 2.3e-8 relative, gradient cosine 1 - 8.9e-9, norm-wise
+    # 1.3e-4 (BF16 rounding points shared with the oracle, FP32 accumulation
+    # orders not)
+    assert abs(loss - ref_loss) <= 1e-6 * ref_loss, (loss, ref_loss)
     cos = float(grad @ ref_grad / (np.linalg.norm(grad) * np.linalg.norm(ref_grad)))
-    assert cos > 0.995, cos
-    assert np.linalg.norm(grad - ref_grad) <= 0.1 * np.linalg.norm(ref_grad)
+    print(f"loss rel. diff {abs(loss - ref_loss) / ref_loss:.3e}, gradient cosine 1-{1 - cos:.3e}, "
+          f"norm-wise {np.linalg.norm(grad - ref_grad) / np.linalg.norm(ref_grad):.3e}")
+    assert cos > 1.0 - 1e-6, cos
+    assert np.linalg.norm(grad - ref_grad) <= 2e-3 * np.linalg.norm(ref_grad)
     # the decoder scalars individually
     for k in (fsai.OFF_ALPHA, fsai.OFF_BOFF, fsai.OFF_BDIAG):
-        assert abs(grad[k] - ref_grad[k]) <= 0.05 * abs(ref_grad[k]) + 1e-6, (k, grad[k], ref_grad[k])
+        assert abs(grad[k] - ref_grad[k]) <= 1e-2 * abs(ref_grad[k]) + 1e-6, (k, grad[k], ref_grad[k])
 
 
 def test_factored_pcg_matches_oracle(dev, ws):
