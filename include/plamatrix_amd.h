@@ -143,6 +143,16 @@ int pm_spmv(const pm_csr* a, const void* x, void* y, pm_workspace* ws, hipStream
  * zeros.  ws is required (it holds a packed copy of 8 columns of B at a
  * time).  Stream-ordered, no host synchronisation. */
 int pm_spmm(const pm_csr* a, const void* b, int64_t k, void* y, pm_workspace* ws, hipStream_t stream);
+/* pm_spmm without a workspace and without any allocation: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The packed tile of B lives in a 64 MiB device array of the library that each
+ * GPU holds from the library's load until the process ends, so a->cols * 8 *
+ * sizeof(value) must not exceed 64 MiB (2,097,152 columns in FP32, 1,048,576
+ * in FP64); larger matrices get PM_OVERFLOW.  Same results as pm_spmm, bit for
+ * bit.  Every call on one GPU shares that array: calls that overlap on one GPU
+ * (other streams, other threads) race, so serialise them. */
+int pm_spmm_noalloc(const pm_csr* a, const void* b, int64_t k, void* y, hipStream_t stream);
 /* Same with an explicit lanes-per-row choice (1..64, power of two); used by
  * the autotuner. */
 int pm_spmv_lanes(const pm_csr* a, const void* x, void* y, int32_t lanes_per_row, pm_workspace* ws,

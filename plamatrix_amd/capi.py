@@ -114,6 +114,7 @@ EXPORTS = [
     "pm_fsai_graph_pool_release", "pm_fsai_graph_pool_bytes",
     "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step", "pm_dense_cholesky_solve",
     "pm_batch_pcg_factored", "pm_batch_pcg_refine_factored", "pm_fsai_batch_select",
+    "pm_spmm_noalloc",
 ]
 
 _lib = None

@@ -1,6 +1,8 @@
 // SpMV, CSR validation, index compaction and the FP64 true residual.
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "spmv_kernels.hpp"
 #include "tuning.hpp"
@@ -64,11 +66,9 @@ int spmv_stream(const pm_csr* a, const void* x, void* y, hipStream_t stream)
 // Y = A B in tiles of kSpmmTile columns through a packed row-major copy of
 // the tile's columns of B in the workspace scratch.
 template <typename S>
-int spmm_tiles(const pm_csr* a, const S* b, int64_t k, S* y, pm_workspace* ws, hipStream_t stream)
+int spmm_tiles_into(const pm_csr* a, const S* b, int64_t k, S* y, S* bt, hipStream_t stream)
 {
     const int64_t cols = a->cols > 0 ? a->cols : 1;
-    PM_TRY(ws->scratch.reserve(sizeof(S) * static_cast<size_t>(cols) * kSpmmTile + 256));
-    S* bt = ws->scratch.as<S>();
     const int grid = static_cast<int>(std::min<int64_t>(ceil_div(chunk_count(a->rows), kWavesPerBlock), kSpmvGrid));
     const int pgrid = static_cast<int>(std::min<int64_t>(ceil_div(cols, kBlock), 8192));
     for (int64_t j0 = 0; j0 < k; j0 += kSpmmTile)
@@ -85,6 +85,47 @@ int spmm_tiles(const pm_csr* a, const S* b, int64_t k, S* y, pm_workspace* ws, h
                                y + j0 * a->rows);
         PM_HIP(hipGetLastError());
     }
+    return PM_OK;
+}
+
+template <typename S>
+int spmm_tiles(const pm_csr* a, const S* b, int64_t k, S* y, pm_workspace* ws, hipStream_t stream)
+{
+    const int64_t cols = a->cols > 0 ? a->cols : 1;
+    PM_TRY(ws->scratch.reserve(sizeof(S) * static_cast<size_t>(cols) * kSpmmTile + 256));
+    return spmm_tiles_into<S>(a, b, k, y, ws->scratch.as<S>(), stream);
+}
+
+// ---- allocation-free SpMM --------------------------------------------------
+//
+// The packed-B tile of the SpMM lives in `scratch`, a file-scope device
+// array: each GPU the library runs on holds its own copy (kNoAllocScratchBytes)
+// from the first load of the library until the process ends.  Host code never
+// uses the array's name as an address (on the host it is only a stand-in
+// symbol): its device address is looked up once per device with
+// hipGetSymbolAddress and kept in g_scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mutex;
+std::vector<void*> g_scratch_ptr;  // per device, null until looked up
+
+int scratch_address(void** out)
+{
+    int dev = 0;
+    PM_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (static_cast<size_t>(dev) >= g_scratch_ptr.size()) g_scratch_ptr.resize(static_cast<size_t>(dev) + 1, nullptr);
+    if (!g_scratch_ptr[static_cast<size_t>(dev)])
+    {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(PM_DEVICE_ERROR, "spmm (allocation-free): no device address for the scratch array on "
+                        "device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[static_cast<size_t>(dev)] = ptr;
+    }
+    *out = g_scratch_ptr[static_cast<size_t>(dev)];
     return PM_OK;
 }
 
@@ -207,6 +248,30 @@ int pm_spmm(const pm_csr* a, const void* b, int64_t k, void* y, pm_workspace* ws
     return a->dtype == PM_F32 ? spmm_tiles<float>(a, static_cast<const float*>(b), k, static_cast<float*>(y), ws, stream)
                               : spmm_tiles<double>(a, static_cast<const double*>(b), k, static_cast<double*>(y), ws,
                                                    stream);
+}
+
+int pm_spmm_noalloc(const pm_csr* a, const void* b, int64_t k, void* y, hipStream_t stream)
+{
+    PM_TRY(check_csr(a, "spmm (allocation-free)"));
+    if (k < 0) return fail(PM_INVALID_ARGUMENT, "spmm (allocation-free): negative column count");
+    if (a->rows > 0 && k > 0 && !y) return fail(PM_INVALID_ARGUMENT, "spmm (allocation-free): null output");
+    if (a->cols > 0 && k > 0 && !b) return fail(PM_INVALID_ARGUMENT, "spmm (allocation-free): null input");
+    if (b != nullptr && b == y)
+        return fail(PM_INVALID_ARGUMENT, "spmm (allocation-free): input and output data must not alias");
+    const size_t es = a->dtype == PM_F64 ? sizeof(double) : sizeof(float);
+    const size_t need = es * static_cast<size_t>(a->cols > 0 ? a->cols : 1) * kSpmmTile;
+    if (need > kNoAllocScratchBytes)
+        return fail(PM_OVERFLOW, "spmm (allocation-free): %lld columns need %zu bytes of packed tile, above the "
+                    "%zu-byte scratch (at most %zu columns for this dtype)", static_cast<long long>(a->cols), need,
+                    kNoAllocScratchBytes, kNoAllocScratchBytes / (es * kSpmmTile));
+    if (a->rows == 0 || k == 0) return PM_OK;
+    void* bt = nullptr;
+    PM_TRY(scratch_address(&bt));
+    return a->dtype == PM_F32
+               ? spmm_tiles_into<float>(a, static_cast<const float*>(b), k, static_cast<float*>(y),
+                                        static_cast<float*>(bt), stream)
+               : spmm_tiles_into<double>(a, static_cast<const double*>(b), k, static_cast<double*>(y),
+                                         static_cast<double*>(bt), stream);
 }
 
 int pm_csr_validate(const pm_csr* a, pm_workspace* ws, hipStream_t stream)
