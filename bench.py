@@ -643,8 +643,10 @@ def run_batch(args, torch, capi, generators):
         if where == "host":
             A = sp.csr_matrix((m.values.astype(np.float64), m.col_indices, m.row_offsets), shape=(ln, ln))
         else:
+            # the FP32-rounded values the batch solves
             _, ro, ci, va = m
-            A = sp.csr_matrix((va.double().cpu().numpy(), ci.cpu().numpy(), ro.cpu().numpy()), shape=(ln, ln))
+            A = sp.csr_matrix((va.float().double().cpu().numpy(), ci.cpu().numpy(), ro.cpu().numpy()),
+                              shape=(ln, ln))
         bb = b_host[b0:b0 + ln]
         worst = max(worst, float(np.linalg.norm(bb - A @ xh[b0:b0 + ln]) / np.linalg.norm(bb)))
     # algorithmic bytes actually streamed: each system's iteration traffic times its iterations
@@ -664,7 +666,8 @@ def run_batch(args, torch, capi, generators):
                 csr = oracle.CSR(lens[i], lens[i], m.row_offsets, m.col_indices, m.values.astype(np.float64))
             else:
                 _, ro, ci, va = m
-                csr = oracle.CSR(lens[i], lens[i], ro.cpu().numpy(), ci.cpu().numpy(), va.double().cpu().numpy())
+                csr = oracle.CSR(lens[i], lens[i], ro.cpu().numpy(), ci.cpu().numpy(),
+                                 va.float().double().cpu().numpy())
             rhs = b_host[begin[i]:begin[i] + lens[i]]
             c0 = time.perf_counter()
             _, crep = oracle.pcg(csr, rhs, relative_tolerance=1e-6, max_iterations=20000, use_ref=use_ref)

@@ -28,6 +28,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -56,7 +58,11 @@ struct BatchState
 
 // Chunks of the systems still running, as a device list with its length in
 // device memory (so replayed graphs follow every rebuild): a superset of the
-// running systems' chunks, since systems only stop between rebuilds.
+// running systems' chunks, since systems only stop between rebuilds.  The
+// list is dealt round-robin over the workgroups: chunk costs differ by more
+// than ten times between members (a reduced Schur row holds ~84 entries, a
+// Laplacian row 5), and contiguous per-workgroup ranges measured 897 ms
+// against 538 ms on the 512-system batch.
 #define PM_FOR_LISTED(i, list_count, W)                                                               \
     for (int64_t i = static_cast<int64_t>(blockIdx.x) * (W) + threadIdx.x / kWave,                    \
                  i##_n = static_cast<int64_t>(*(list_count));                                         \
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_pupdate(const int32_t* __restr
 
 // p_new = z + beta p_old, q = A p_new, per-chunk FP64 partial p_new.q.
 // SPLIT: p_new was formed by k_batch_pupdate and is gathered directly.
-template <typename S, typename I, bool SPLIT>
+template <typename S, typename I, bool SPLIT, int LL = 8>
 __global__ __launch_bounds__(kBlock) void k_batch_direction(int64_t chunks, const int32_t* __restrict__ chunk_sys,
                                                             const int64_t* __restrict__ sys_end,
                                                             const I* __restrict__ ro, const I* __restrict__ ci,
@@ -188,10 +194,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_direction(int64_t chunks, cons
         int64_t ks, ke;
         S acc;
         if (SPLIT)
-            acc = chunk_row_sums<S>(row0, end, ro, ci, lv_all[wave], lc_all[wave], SpmvSource<S>{v, p_new}, ks, ke);
+            acc = chunk_row_sums<S, I, SpmvSource<S>, false, false, LL>(row0, end, ro, ci, lv_all[wave], lc_all[wave],
+                                                                        SpmvSource<S>{v, p_new}, ks, ke);
         else
-            acc = chunk_row_sums<S>(row0, end, ro, ci, lv_all[wave], lc_all[wave],
-                                    DirectionSource<S>{v, z, p_old, beta}, ks, ke);
+            acc = chunk_row_sums<S, I, DirectionSource<S>, false, false, LL>(
+                row0, end, ro, ci, lv_all[wave], lc_all[wave], DirectionSource<S>{v, z, p_old, beta}, ks, ke);
         const int64_t row = row0 + lane;
         double den = 0.0;
         if (row < end)
@@ -275,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const i
 // r_new = r - alpha_s q formed at every gathered column with the rounding of
 // k_batch_update (r itself is rewritten by k_batch_factor_upper, since other
 // chunks gather it here); otherwise r_new = r.
-template <typename S, typename I, bool STEP>
+template <typename S, typename I, bool STEP, int LL = 8>
 __global__ __launch_bounds__(kBlock) void k_batch_factor_lower(const int32_t* __restrict__ chunk_sys,
                                                                const int64_t* __restrict__ sys_end,
                                                                const I* __restrict__ gro, const I* __restrict__ gci,
@@ -305,20 +312,47 @@ __global__ __launch_bounds__(kBlock) void k_batch_factor_lower(const int32_t* __
             int64_t ks, ke;
             S acc;
             if (STEP)
-                acc = chunk_row_sums<S>(row0, end, gro, gci, lv_all[wave], lc_all[wave],
-                                        StepSource<S>{gv, r, q, -alpha}, ks, ke);
+                acc = chunk_row_sums<S, I, StepSource<S>, false, false, LL>(row0, end, gro, gci, lv_all[wave],
+                                                                            lc_all[wave],
+                                                                            StepSource<S>{gv, r, q, -alpha}, ks, ke);
             else
-                acc = chunk_row_sums<S>(row0, end, gro, gci, lv_all[wave], lc_all[wave], SpmvSource<S>{gv, r}, ks,
-                                        ke);
+                acc = chunk_row_sums<S, I, SpmvSource<S>, false, false, LL>(row0, end, gro, gci, lv_all[wave],
+                                                                            lc_all[wave], SpmvSource<S>{gv, r}, ks,
+                                                                            ke);
             if (row < end) t[row] = acc;
         }
         if (STEP && row < end) x[row] = fma_s(alpha, p[row], x[row]);
     }
 }
 
+// The step of the factored iteration on its own (systems with long factor
+// rows): x += alpha_s p, r -= alpha_s q, with the rounding of k_batch_update,
+// so the factor products gather r once per entry instead of r and q.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_step_xr(const int32_t* __restrict__ chunk_sys,
+                                                          const int64_t* __restrict__ sys_end, S* __restrict__ x,
+                                                          S* __restrict__ r, const S* __restrict__ p,
+                                                          const S* __restrict__ q, const BatchState* st,
+                                                          const int64_t* __restrict__ list,
+                                                          const unsigned long long* __restrict__ list_count)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning) continue;
+        const int64_t row = c * kChunkRows + lane;
+        if (row >= sys_end[s]) continue;
+        const S alpha = static_cast<S>(st[s].alpha);
+        x[row] = fma_s(alpha, p[row], x[row]);
+        r[row] = fma_s(-alpha, q[row], r[row]);
+    }
+}
+
 // z = G^T t (Jacobi systems: z = D^-1 r_new) and the per-chunk partials r.r,
 // r.z.  STEP: r_new = r - alpha_s q on the chunk's own rows, written back.
-template <typename S, typename I, bool STEP>
+template <typename S, typename I, bool STEP, int LL = 8>
 __global__ __launch_bounds__(kBlock) void k_batch_factor_upper(const int32_t* __restrict__ chunk_sys,
                                                                const int64_t* __restrict__ sys_end,
                                                                const I* __restrict__ uro, const I* __restrict__ uci,
@@ -352,7 +386,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_factor_upper(const int32_t* __
         if (!jac)
         {
             int64_t ks, ke;
-            acc = chunk_row_sums<S>(row0, end, uro, uci, lv_all[wave], lc_all[wave], SpmvSource<S>{uv, t}, ks, ke);
+            acc = chunk_row_sums<S, I, SpmvSource<S>, false, false, LL>(row0, end, uro, uci, lv_all[wave],
+                                                                        lc_all[wave], SpmvSource<S>{uv, t}, ks, ke);
         }
         double rr = 0.0, rz = 0.0;
         if (row < end)
@@ -618,6 +653,8 @@ struct BatchProblem
     int64_t* list;                    // chunks of running systems
     unsigned long long* list_count;
     bool split;                       // direction update as its own kernel (SpMV gathers p once)
+    int long_lanes;                   // lanes per row of chunks with long rows (8, 16, 32 or 64)
+    bool fuse_step;                   // factored: the step folded into the factor products
 };
 
 template <typename S, typename I>
@@ -654,17 +691,56 @@ struct BatchCalls
 
     // z = M^-1 r for the factored preconditioner: t = G r_new, z = G^T t (or
     // Jacobi for flagged systems), partials r.r and r.z.
+    template <template <int> class K>
+    static auto by_lanes(int lanes)
+    {
+        switch (lanes)
+        {
+        case 16: return K<16>::f;
+        case 32: return K<32>::f;
+        case 64: return K<64>::f;
+        default: return K<8>::f;
+        }
+    }
+    template <bool SPLIT>
+    struct Dir
+    {
+        template <int LL>
+        struct K
+        {
+            static constexpr auto f = k_batch_direction<S, I, SPLIT, LL>;
+        };
+    };
+    template <bool STEP>
+    struct Lower
+    {
+        template <int LL>
+        struct K
+        {
+            static constexpr auto f = k_batch_factor_lower<S, I, STEP, LL>;
+        };
+    };
+    template <bool STEP>
+    struct Upper
+    {
+        template <int LL>
+        struct K
+        {
+            static constexpr auto f = k_batch_factor_upper<S, I, STEP, LL>;
+        };
+    };
+
     template <bool STEP>
     static void factored(const BatchProblem& b, const void* p, std::vector<KernelCall>& out)
     {
-        out.push_back(make_call(k_batch_factor_lower<S, I, STEP>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys,
+        out.push_back(make_call(by_lanes<Lower<STEP>::template K>(b.long_lanes), dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys,
                                 b.sys_end, ro(b.g), ci(b.g), static_cast<const S*>(b.g->values),
                                 static_cast<S*>(b.x), static_cast<const S*>(b.r), static_cast<const S*>(p),
                                 static_cast<const S*>(b.q), static_cast<S*>(b.t),
                                 static_cast<const BatchState*>(b.st), b.sys_jacobi,
                                 static_cast<const int64_t*>(b.list),
                                 static_cast<const unsigned long long*>(b.list_count)));
-        out.push_back(make_call(k_batch_factor_upper<S, I, STEP>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys,
+        out.push_back(make_call(by_lanes<Upper<STEP>::template K>(b.long_lanes), dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys,
                                 b.sys_end, ro(b.gt), ci(b.gt), static_cast<const S*>(b.gt->values),
                                 static_cast<const S*>(b.t), static_cast<S*>(b.r), static_cast<const S*>(b.q),
                                 static_cast<const S*>(b.inv), static_cast<S*>(b.z),
@@ -703,7 +779,8 @@ struct BatchCalls
                                     static_cast<const S*>(b.z), static_cast<const S*>(p_old), static_cast<S*>(p_new),
                                     static_cast<const BatchState*>(b.st), static_cast<const int64_t*>(b.list),
                                     static_cast<const unsigned long long*>(b.list_count)));
-        out.push_back(make_call(b.split ? k_batch_direction<S, I, true> : k_batch_direction<S, I, false>,
+        out.push_back(make_call(b.split ? by_lanes<Dir<true>::template K>(b.long_lanes)
+                                        : by_lanes<Dir<false>::template K>(b.long_lanes),
                                 dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys,
                                 b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
                                 static_cast<const S*>(b.z), static_cast<const S*>(p_old), static_cast<S*>(p_new),
@@ -711,10 +788,19 @@ struct BatchCalls
                                 static_cast<const int64_t*>(b.list),
                                 static_cast<const unsigned long long*>(b.list_count)));
         out.push_back(reduce<kReduceAlpha>(b, b.pden, nullptr));
-        if (b.g && !refresh)
+        if (b.g && !refresh && b.fuse_step)
         {
             // the step rides in the two factor products
             factored<true>(b, p_new, out);
+        }
+        else if (b.g && !refresh)
+        {
+            out.push_back(make_call(k_batch_step_xr<S>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys, b.sys_end,
+                                    static_cast<S*>(b.x), static_cast<S*>(b.r), static_cast<const S*>(p_new),
+                                    static_cast<const S*>(b.q), static_cast<const BatchState*>(b.st),
+                                    static_cast<const int64_t*>(b.list),
+                                    static_cast<const unsigned long long*>(b.list_count)));
+            factored<false>(b, nullptr, out);
         }
         else if (!refresh)
         {
@@ -738,6 +824,30 @@ struct BatchCalls
             if (b.g) factored<false>(b, nullptr, out);
         }
         out.push_back(reduce<kReduceBeta>(b, b.prr, b.prz));
+    }
+
+    // Device time (ms) of `reps` direction launches of the first iteration
+    // with `lanes` lanes per long row (idempotent: the first iteration writes
+    // the same p, q and partials again).
+    static float time_direction(const BatchProblem& b, int lanes, int reps, hipStream_t stream, hipEvent_t e0,
+                                hipEvent_t e1)
+    {
+        const auto k = by_lanes<Dir<false>::template K>(lanes);
+        auto launch = [&]() {
+            hipLaunchKernelGGL(k, dim3(b.grid_chunks), dim3(kBlock), 0, stream, b.chunks, b.chunk_sys, b.sys_end,
+                               ro(b.a), ci(b.a), static_cast<const S*>(b.a->values), static_cast<const S*>(b.z),
+                               static_cast<const S*>(b.p0), static_cast<S*>(b.p1), static_cast<S*>(b.q),
+                               static_cast<const BatchState*>(b.st), b.pden, static_cast<const int64_t*>(b.list),
+                               static_cast<const unsigned long long*>(b.list_count));
+        };
+        launch();
+        (void)hipEventRecord(e0, stream);
+        for (int i = 0; i < reps; ++i) launch();
+        (void)hipEventRecord(e1, stream);
+        if (hipEventSynchronize(e1) != hipSuccess) return -1.f;
+        float ms = -1.f;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.f;
+        return ms;
     }
 
     static void build(const BatchProblem& b, int what, int64_t first, int64_t count, int refresh_interval,
@@ -795,7 +905,9 @@ std::string batch_key(const BatchProblem& b, int what, int64_t first, int64_t co
                   b.rhs, b.x,
                   static_cast<long long>(b.a->rows), static_cast<long long>(b.systems), b.jacobi ? 1 : 0,
                   static_cast<long long>(first), static_cast<long long>(count), refresh, b.a->dtype,
-                  b.max_iterations, b.rel_tol, b.abs_tol, b.split ? 1 : 0, static_cast<const void*>(b.sys_jacobi));
+                  b.max_iterations, b.rel_tol, b.abs_tol,
+                  (b.split ? 1 : 0) | (b.fuse_step ? 2 : 0) | (b.long_lanes << 2),
+                  static_cast<const void*>(b.sys_jacobi));
     std::string key(buf);
     if (b.g)
     {
@@ -826,6 +938,66 @@ Plan* batch_plan(pm_workspace* ws, const BatchProblem& b, int what, int64_t firs
     return plan;
 }
 
+// Rows-per-wavefront autotuner of the batched iteration: the lanes given to
+// each row of a chunk too long for the LDS windows (8, 16, 32 or 64 lanes per
+// row: 8 to 1 rows per wavefront step).  Timed once per matrix (arrays, size,
+// systems) and process on the batch's own direction kernel, then kept.
+std::mutex g_lanes_mutex;
+std::map<std::string, int> g_lanes;
+
+std::string lanes_key(const BatchProblem& b)
+{
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%p|%p|%p|%lld|%lld|%lld|%d", b.a->values, (const void*)b.a->col_indices32,
+                  (const void*)b.a->col_indices, static_cast<long long>(b.a->rows),
+                  static_cast<long long>(b.a->nnz), static_cast<long long>(b.systems), b.a->dtype);
+    return buf;
+}
+
+template <typename S>
+int tune_long_lanes(BatchProblem& b, hipStream_t stream)
+{
+    // chunks of fewer than kMaxWindows * kStreamCap entries never take the
+    // long-row path: rows averaging under 24 entries keep the default
+    if (b.a->rows == 0 || b.a->nnz < 24 * b.a->rows)
+    {
+        b.long_lanes = 8;
+        return PM_OK;
+    }
+    const std::string key = lanes_key(b);
+    {
+        std::lock_guard<std::mutex> lock(g_lanes_mutex);
+        auto it = g_lanes.find(key);
+        if (it != g_lanes.end())
+        {
+            b.long_lanes = it->second;
+            return PM_OK;
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PM_HIP(hipEventCreate(&e0));
+    PM_HIP(hipEventCreate(&e1));
+    int best = 8;
+    float best_ms = -1.f;
+    for (int lanes : {8, 16, 32, 64})
+    {
+        const float ms = use_compact(b.a) ? BatchCalls<S, int32_t>::time_direction(b, lanes, 3, stream, e0, e1)
+                                          : BatchCalls<S, int64_t>::time_direction(b, lanes, 3, stream, e0, e1);
+        if (ms >= 0.f && (best_ms < 0.f || ms < best_ms))
+        {
+            best_ms = ms;
+            best = lanes;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    PM_HIP(hipGetLastError());
+    b.long_lanes = best;
+    std::lock_guard<std::mutex> lock(g_lanes_mutex);
+    g_lanes[key] = best;
+    return PM_OK;
+}
+
 template <typename S>
 int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* reports, pm_workspace* ws,
                 hipStream_t stream)
@@ -834,6 +1006,7 @@ int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* rep
                        dim3(kBlock), 0, stream, b.systems, b.st, b.done_count);
     PM_HIP(hipGetLastError());
     PM_TRY(batch_plan<S>(ws, b, 0, 0, 0, 0)->run(stream, false));
+    if (b.long_lanes == 0) PM_TRY(tune_long_lanes<S>(b, stream));
     const int64_t K = policy.chunk;
     const int64_t needed = b.max_iterations;
     unsigned long long* hs = ws->host_batch;
@@ -966,6 +1139,11 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     // 640 ms); PLAMATRIX_AMD_BATCH_SPLIT=1 selects the split form
     const char* split = std::getenv("PLAMATRIX_AMD_BATCH_SPLIT");
     b.split = split != nullptr && split[0] == '1';
+    // lanes per long row: PLAMATRIX_AMD_BATCH_LANES (8, 16, 32, 64), else tuned
+    // on the first solve of this matrix
+    const char* lanes = std::getenv("PLAMATRIX_AMD_BATCH_LANES");
+    const int lv = lanes ? std::atoi(lanes) : 0;
+    b.long_lanes = (lv == 8 || lv == 16 || lv == 32 || lv == 64) ? lv : 0;
     b.grid_chunks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(chunks, kBatchWaves), kBatchGrid)));
     b.grid_systems = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(systems, kBatchWaves), 1024)));
     char* vb = ws->vectors.as<char>();
@@ -1005,6 +1183,8 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     return PM_OK;
 }
 
+constexpr int64_t kFuseStepEntriesPerRow = 32;
+
 // The factored preconditioner of a batch: G lower-triangular and G^T, square
 // like a, with a's dtype and index widths (32-bit mirrors when a uses them).
 int batch_attach_factors(BatchProblem& b, const pm_csr* g, const pm_csr* gt, const uint8_t* sys_jacobi)
@@ -1027,6 +1207,15 @@ int batch_attach_factors(BatchProblem& b, const pm_csr* g, const pm_csr* gt, con
     b.gt = gt;
     b.sys_jacobi = sys_jacobi;
     b.jacobi = false;
+    // Folding the step into the products costs a second gather (of q) per
+    // factor entry and saves one pass over x, p, q, r and a launch.  On the
+    // held-out batch (256 diffusion boxes with 4-entry factor rows, 256 reduced
+    // Schur systems with about 70) both forms measured the same (70.5 vs
+    // 71.0 ms), so the fold is kept unless factor rows average 32 entries or
+    // more.  PLAMATRIX_AMD_FUSE_STEP=0 / 1 forces either form; both give the
+    // same bits.
+    const char* fuse = std::getenv("PLAMATRIX_AMD_FUSE_STEP");
+    b.fuse_step = fuse != nullptr ? fuse[0] != '0' : g->nnz < kFuseStepEntriesPerRow * g->rows;
     return PM_OK;
 }
 

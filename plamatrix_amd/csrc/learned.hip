@@ -587,37 +587,53 @@ __global__ __launch_bounds__(kBlock) void k_decode(int64_t n, const int32_t* __r
 #pragma unroll
         for (int q = 0; q < 8; ++q) hi[q] = bf(hv[q]);
         const int32_t k1 = lro[i + 1];
-        for (int32_t k = lro[i]; k < k1; ++k)
+        // batches of U entries: every index and neighbour load of a batch is
+        // in flight at once (long rows, such as a reduced Schur system's ~70
+        // lower entries, would otherwise be one dependent chain per entry)
+        constexpr int U = 4;
+        for (int32_t k0 = lro[i]; k0 < k1; k0 += U)
         {
-            const int32_t j = lci[k];
-            float part = 0.f;
-            if (j == i)
-            {
+            int32_t jj[U];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) part = fmaf(w_diag[q], hi[q], part);
-            }
-            else
-            {
-                const bf16x8 hj = *reinterpret_cast<const bf16x8*>(h + static_cast<int64_t>(j) * kHid + sub * 8);
+            for (int u = 0; u < U; ++u) jj[u] = k0 + u < k1 ? lci[k0 + u] : static_cast<int32_t>(i);
+            bf16x8 hj[U];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) part = fmaf(w_off[q] * hi[q], bf(hj[q]), part);
-            }
+            for (int u = 0; u < U; ++u)
+                hj[u] = *reinterpret_cast<const bf16x8*>(h + static_cast<int64_t>(jj[u]) * kHid + sub * 8);
 #pragma unroll
-            for (int off = kLanesPerRow / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, kLanesPerRow);
-            if (sub == 0)
+            for (int u = 0; u < U; ++u)
             {
-                float o;
+                const int32_t k = k0 + u;
+                if (k >= k1) break;
+                const int32_t j = jj[u];
+                float part = 0.f;
                 if (j == i)
                 {
-                    o = part + b_diag;
-                    g[k] = static_cast<G>(softplus(o) * dinv[i]);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) part = fmaf(w_diag[q], hi[q], part);
                 }
                 else
                 {
-                    o = part + b_off + alpha * s[lsrc[k]];
-                    g[k] = static_cast<G>(o * dinv[j]);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) part = fmaf(w_off[q] * hi[q], bf(hj[u][q]), part);
                 }
-                out[k] = o;
+#pragma unroll
+                for (int off = kLanesPerRow / 2; off > 0; off >>= 1) part += __shfl_xor(part, off, kLanesPerRow);
+                if (sub == 0)
+                {
+                    float o;
+                    if (j == i)
+                    {
+                        o = part + b_diag;
+                        g[k] = static_cast<G>(softplus(o) * dinv[i]);
+                    }
+                    else
+                    {
+                        o = part + b_off + alpha * s[lsrc[k]];
+                        g[k] = static_cast<G>(o * dinv[j]);
+                    }
+                    out[k] = o;
+                }
             }
         }
     }

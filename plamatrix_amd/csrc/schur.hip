@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <cstdlib>
 #include <cstring>
 
 #include "dense_cholesky.hpp"
@@ -862,18 +863,19 @@ __global__ __launch_bounds__(kBlock) void k_transform_cross(int64_t nc, int ps, 
                                                             const double* __restrict__ W, double* __restrict__ TW)
 {
 #pragma clang fp contract(off)
-    for (int64_t t = gtid(); t < nc * ps; t += gstride())
+    // one thread per output entry (c, row, i): consecutive threads store
+    // consecutive entries, and the threads of one row read the same W row
+    const int64_t bv = static_cast<int64_t>(ps) * es;
+    for (int64_t t = gtid(); t < nc * bv; t += gstride())
     {
-        const int64_t c = t / ps;
-        const int row = static_cast<int>(t % ps);
-        const double* inv = Cinv + cross_eliminated[c] * es * es;
-        const double* w = W + c * ps * es + row * es;
-        for (int i = 0; i < es; ++i)
-        {
-            double value = 0.0;
-            for (int j = 0; j < es; ++j) value += inv[i * es + j] * w[j];
-            TW[c * ps * es + row * es + i] = value;
-        }
+        const int64_t c = t / bv;
+        const int rem = static_cast<int>(t % bv);
+        const int row = rem / es, i = rem % es;
+        const double* inv = Cinv + cross_eliminated[c] * es * es + i * es;
+        const double* w = W + c * bv + row * es;
+        double value = 0.0;
+        for (int j = 0; j < es; ++j) value += inv[j] * w[j];
+        TW[t] = value;
     }
 }
 
@@ -1003,6 +1005,184 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
         if (active) values[ro[row * ps + lr] + (s - bro[row]) * ps + lc] = acc;
         }
     }
+}
+
+// ---- segmented assembly ------------------------------------------------------------
+//
+// Each slot's terms are cut into segments of at most kSegTerms (in the slot's
+// eliminated order).  k_assemble_segments gives every segment one wavefront:
+// the segment's W C^-1 and W blocks are staged into LDS with all loads in
+// flight, and each lane (one entry of the 6 x 6 block) sums -(W_l C^-1) W_r^T
+// over the segment's terms in order.  k_assemble_reduce then forms each value
+// as damped B, the primary cross markers and the segment sums in segment
+// order.  The summation order is fixed (deterministic run to run) but groups
+// the terms by segment, so values differ from the CPU accumulation
+// (block_schur_sparse_assembly.h:817-903) by rounding only; the slot kernel
+// above keeps the CPU order (PLAMATRIX_AMD_SCHUR_ASSEMBLY=slots).  Long slots
+// (the diagonal ones hold every secondary of their primary) no longer run as
+// one serial chain, and a launch has about 30 times more wavefronts.
+constexpr int kSegTerms = 16;
+
+// Per slot: the number of terms (the run minus its trailing markers) and of
+// segments; seg_count[slots] = 0 closes the scan.
+__global__ __launch_bounds__(kBlock) void k_slot_segments(int64_t slots, const int64_t* __restrict__ slot_start,
+                                                          int64_t entries, const unsigned* __restrict__ term_e,
+                                                          unsigned* __restrict__ slot_terms,
+                                                          unsigned* __restrict__ seg_count)
+{
+    for (int64_t s = gtid(); s <= slots; s += gstride())
+    {
+        if (s == slots)
+        {
+            seg_count[s] = 0u;
+            continue;
+        }
+        const int64_t begin = slot_start[s];
+        const int64_t end = s + 1 < slots ? slot_start[s + 1] : entries;
+        int64_t te = end;
+        while (te > begin && term_e[te - 1] >= ~1u) --te;
+        slot_terms[s] = static_cast<unsigned>(te - begin);
+        seg_count[s] = static_cast<unsigned>((te - begin + kSegTerms - 1) / kSegTerms);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_segment_slots(int64_t slots, const unsigned* __restrict__ seg_offs,
+                                                          unsigned* __restrict__ seg_slot)
+{
+    for (int64_t s = gtid(); s < slots; s += gstride())
+        for (unsigned g = seg_offs[s]; g < seg_offs[s + 1]; ++g) seg_slot[g] = static_cast<unsigned>(s);
+}
+
+template <int ES, int PS>
+__global__ __launch_bounds__(kBlock) void k_assemble_segments(
+    int64_t nseg, const unsigned* __restrict__ seg_slot, const unsigned* __restrict__ seg_offs,
+    const int64_t* __restrict__ slot_start, const unsigned* __restrict__ slot_terms,
+    const unsigned* __restrict__ term_left, const unsigned* __restrict__ term_right, const double* __restrict__ TW,
+    const double* __restrict__ W, double* __restrict__ partial)
+{
+#pragma clang fp contract(off)
+    constexpr int BV = PS * ES;              // doubles of one cross block
+    constexpr int PER = 2 * BV;              // W C^-1 block, then W block
+    constexpr int STAGE = kSegTerms * PER;
+    constexpr int LOADS = (STAGE + kWave - 1) / kWave;
+    constexpr int kWaves = kBlock / kWave;
+    __shared__ double stage_all[kWaves][STAGE];
+    const int lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
+    double* st = stage_all[wave];
+    const bool active = lane < PS * PS;
+    const int lr = active ? lane / PS : 0, lc = active ? lane % PS : 0;
+    // each workgroup owns a contiguous, XCD-local range of segments:
+    // neighbouring segments share slots and blocks, which then stay in one L2
+    const int64_t per = ceil_div(ceil_div(nseg, static_cast<int64_t>(gridDim.x)), int64_t(kWaves)) * kWaves;
+    const int64_t gbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t gend = gbegin + per < nseg ? gbegin + per : nseg;
+    for (int64_t g = gbegin + wave; g < gend; g += kWaves)
+    {
+        const unsigned s = seg_slot[g];
+        const int64_t j = g - static_cast<int64_t>(seg_offs[s]);
+        const int64_t begin = slot_start[s] + j * kSegTerms;
+        const int64_t left = static_cast<int64_t>(slot_terms[s]) - j * kSegTerms;
+        const int nt = static_cast<int>(left < kSegTerms ? left : kSegTerms);
+        unsigned idl = 0u, idr = 0u;
+        if (lane < nt)
+        {
+            idl = term_left[begin + lane];
+            idr = term_right[begin + lane];
+        }
+        double val[LOADS];
+#pragma unroll
+        for (int m = 0; m < LOADS; ++m)
+        {
+            const int f = lane + m * kWave;
+            const int u = f / PER, within = f % PER;
+            const unsigned a = __shfl(idl, u < kWave ? u : 0, kWave);
+            const unsigned b = __shfl(idr, u < kWave ? u : 0, kWave);
+            val[m] = 0.0;
+            if (f < STAGE && u < nt)
+                val[m] = within < BV ? TW[static_cast<int64_t>(a) * BV + within]
+                                     : W[static_cast<int64_t>(b) * BV + (within - BV)];
+        }
+#pragma unroll
+        for (int m = 0; m < LOADS; ++m)
+        {
+            const int f = lane + m * kWave;
+            if (f < STAGE) st[f] = val[m];
+        }
+        wave_lds_sync();
+        if (active)
+        {
+            double acc = 0.0;
+            for (int u = 0; u < nt; ++u)
+            {
+                const double* t = st + u * PER + lr * ES;
+                const double* w = st + u * PER + BV + lc * ES;
+                double value = 0.0;
+#pragma unroll
+                for (int i = 0; i < ES; ++i) value += t[i] * w[i];
+                acc += -value;
+            }
+            partial[g * (PS * PS) + lane] = acc;
+        }
+        wave_lds_sync();
+    }
+}
+
+// One wavefront per slot: damped B on the diagonal, the primary cross
+// markers, then the segment sums in segment order.
+template <int PS>
+__global__ __launch_bounds__(kBlock) void k_assemble_reduce(
+    int64_t slots, int pb, const unsigned long long* __restrict__ slot_key, const int64_t* __restrict__ slot_start,
+    int64_t entries, const unsigned* __restrict__ term_e, const unsigned* __restrict__ term_left,
+    const unsigned* __restrict__ term_right, const unsigned* __restrict__ seg_offs,
+    const int64_t* __restrict__ bro, const int64_t* __restrict__ ro, const double* __restrict__ Bd,
+    const double* __restrict__ pcv, const double* __restrict__ partial, double* __restrict__ values)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x % kWave;
+    const bool active = lane < PS * PS;
+    const int lr = active ? lane / PS : 0, lc = active ? lane % PS : 0;
+    const unsigned long long mask = (1ull << pb) - 1ull;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / kWave);
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave; s < slots;
+         s += waves)
+    {
+        if (!active) continue;
+        const int64_t row = static_cast<int64_t>(slot_key[s] >> pb);
+        const int64_t col = static_cast<int64_t>(slot_key[s] & mask);
+        const int64_t end = s + 1 < slots ? slot_start[s + 1] : entries;
+        double acc = 0.0;
+        if (row == col) acc += Bd[row * PS * PS + lr * PS + lc];
+        int64_t te = end;
+        while (te > slot_start[s] && term_e[te - 1] >= ~1u) --te;
+        for (int64_t k = te; k < end; ++k)
+        {
+            if (term_e[k] != ~1u) continue;
+            const double* m = pcv + static_cast<int64_t>(term_left[k]) * PS * PS;
+            acc += term_right[k] == 0 ? m[lr * PS + lc] : m[lc * PS + lr];
+        }
+        const int64_t g1 = seg_offs[s + 1];
+        constexpr int U = 8;
+        for (int64_t g = seg_offs[s]; g < g1; g += U)
+        {
+            double v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = g + u < g1 ? partial[(g + u) * (PS * PS) + lane] : 0.0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (g + u < g1) acc += v[u];
+        }
+        values[ro[row * PS + lr] + (s - bro[row]) * PS + lc] = acc;
+    }
+}
+
+// The segmented kernels serve 6 x 6 primaries with 3 x 3 eliminated blocks;
+// PLAMATRIX_AMD_SCHUR_ASSEMBLY=slots keeps the slot kernel and with it the
+// CPU accumulation order bit for bit.
+bool segmented_assembly_wanted()
+{
+    const char* env = std::getenv("PLAMATRIX_AMD_SCHUR_ASSEMBLY");
+    return env == nullptr || std::strcmp(env, "slots") != 0;
 }
 
 // Preconditioner: inverse of each assembled diagonal block.
@@ -1243,6 +1423,10 @@ struct pm_schur_workspace
     unsigned* term_e = nullptr;
     unsigned* term_left = nullptr;
     unsigned* term_right = nullptr;
+    unsigned* seg_offs = nullptr;    // segmented assembly: first segment of each slot (slots + 1)
+    unsigned* slot_terms = nullptr;  // terms of each slot (markers excluded)
+    DeviceBuffer segment_slot;       // slot of each segment
+    int64_t segments = 0;
     int64_t* bro = nullptr;
     int64_t* ro = nullptr;
     int32_t* ro32 = nullptr;
@@ -1304,6 +1488,7 @@ struct pm_schur_workspace
         ne_sort.release();
         pattern.release();
         pattern_sort.release();
+        segment_slot.release();
         numerics.release();
     }
 };
@@ -1498,6 +1683,9 @@ int build_pattern(pm_schur_workspace* w, const pm_normal_equations* eq, hipStrea
     fz.take<int64_t>(values);
     fz.take<int32_t>(values);
     fz.take<int64_t>(pc);
+    fz.take<unsigned>(slots + 1);
+    fz.take<unsigned>(slots + 1);
+    fz.take<unsigned>(slots);
     PM_TRY(w->pattern.reserve(fz.off));
     Carve f;
     f.base = w->pattern.as<char>();
@@ -1520,6 +1708,9 @@ int build_pattern(pm_schur_workspace* w, const pm_normal_equations* eq, hipStrea
     w->ci = f.take<int64_t>(values);
     w->ci32 = f.take<int32_t>(values);
     w->diag_slot = f.take<int64_t>(pc);
+    unsigned* seg_count = f.take<unsigned>(slots + 1);
+    w->seg_offs = f.take<unsigned>(slots + 1);
+    w->slot_terms = f.take<unsigned>(slots);
     // the head block was carved in the same order and sizes as the front of
     // the pattern buffer, so one copy moves cp, ce, pr, pcl, adjacency and
     // the primary lists
@@ -1530,6 +1721,17 @@ int build_pattern(pm_schur_workspace* w, const pm_normal_equations* eq, hipStrea
                        adj_offs, adj, w->term_e, w->term_left, w->term_right);
     hipLaunchKernelGGL(k_group_offsets, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, w->slot_key, slots, pb, pc,
                        w->bro);
+    // segments of the slots' term runs (segmented assembly)
+    hipLaunchKernelGGL(k_slot_segments, dim3(grid_for(slots + 1)), dim3(kBlock), 0, stream, slots, w->slot_start,
+                       entries, w->term_e, w->slot_terms, seg_count);
+    PM_TRY(scan_u32(seg_count, w->seg_offs, slots, sp2.scratch, w->seg_offs + slots, stream));
+    unsigned nseg32 = 0;
+    PM_HIP(hipMemcpyAsync(&nseg32, w->seg_offs + slots, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    PM_HIP(hipStreamSynchronize(stream));
+    w->segments = nseg32;
+    PM_TRY(w->segment_slot.reserve(sizeof(unsigned) * std::max<int64_t>(w->segments, 1)));
+    hipLaunchKernelGGL(k_segment_slots, dim3(grid_for(slots)), dim3(kBlock), 0, stream, slots, w->seg_offs,
+                       w->segment_slot.as<unsigned>());
     hipLaunchKernelGGL(k_scalar_rows, dim3(grid_for(dim + 1)), dim3(kBlock), 0, stream, pc, ps, w->bro, w->ro,
                        w->ro32);
     hipLaunchKernelGGL(k_scalar_cols, dim3(grid_for(values)), dim3(kBlock), 0, stream, slots, ps, pb, w->slot_key,
@@ -1750,6 +1952,7 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     nz.take<double>(pd > 0 ? w->values_count : 0);
     nz.take<double>(pc * ps * ps);
     nz.take<double>(nc * (ps > es ? ps : es));
+    nz.take<double>(w->segments * ps * ps);
     PM_TRY(w->numerics.reserve(nz.off));
     Carve n;
     n.base = w->numerics.as<char>();
@@ -1762,6 +1965,7 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     w->values = n.take<double>(pd > 0 ? w->values_count : 0);
     w->pinv = n.take<double>(pc * ps * ps);
     w->crv = n.take<double>(nc * (ps > es ? ps : es));
+    double* seg_partial = n.take<double>(w->segments * ps * ps);
 
     hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, st);
     if (pc > 0)
@@ -1784,17 +1988,42 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
                                w->prim_cross, eq->cross_eliminated, eq->cross_values, w->y, w->crv);
         hipLaunchKernelGGL(k_reduced_rhs, dim3(static_cast<unsigned>(ceil_div(pd, kWave))), dim3(kWave), 0, stream, pc,
                            ps, eq->primary_gradient, w->prim_offs, static_cast<const double*>(w->crv), w->rhs);
-        if (nc > 0)
-            hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
-                               eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
-        const int64_t waves_per_block = kBlock / kWave;
-        const int slot_grid =
-            static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
-        const auto assemble = es == 3 && ps == 6 ? k_assemble_slots<3, 6> : k_assemble_slots<0, 0>;
-        hipLaunchKernelGGL(assemble, dim3(slot_grid), dim3(kBlock), 0,
-                           stream, w->slots, ps, es, w->pb, w->slot_key, w->slot_start, w->entries, w->term_e,
-                           w->term_left, w->term_right, w->bro, w->ro, w->Bd, eq->primary_cross_values, w->TW,
-                           eq->cross_values, w->values);
+        if (es == 3 && ps == 6 && segmented_assembly_wanted())
+        {
+            if (nc > 0)
+                hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
+                                   es, eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
+            const int64_t waves_per_block = kBlock / kWave;
+            if (w->segments > 0)
+            {
+                const int seg_grid = static_cast<int>(
+                    std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->segments, waves_per_block), 65536)));
+                hipLaunchKernelGGL((k_assemble_segments<3, 6>), dim3(seg_grid), dim3(kBlock), 0, stream, w->segments,
+                                   w->segment_slot.as<unsigned>(), static_cast<const unsigned*>(w->seg_offs),
+                                   w->slot_start, static_cast<const unsigned*>(w->slot_terms), w->term_left,
+                                   w->term_right, static_cast<const double*>(w->TW), eq->cross_values, seg_partial);
+            }
+            const int slot_grid =
+                static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
+            hipLaunchKernelGGL(k_assemble_reduce<6>, dim3(slot_grid), dim3(kBlock), 0, stream, w->slots, w->pb,
+                               w->slot_key, w->slot_start, w->entries, w->term_e, w->term_left, w->term_right,
+                               static_cast<const unsigned*>(w->seg_offs), w->bro, w->ro, w->Bd,
+                               eq->primary_cross_values, static_cast<const double*>(seg_partial), w->values);
+        }
+        else
+        {
+            if (nc > 0)
+                hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
+                                   es, eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
+            const int64_t waves_per_block = kBlock / kWave;
+            const int slot_grid =
+                static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
+            const auto assemble = es == 3 && ps == 6 ? k_assemble_slots<3, 6> : k_assemble_slots<0, 0>;
+            hipLaunchKernelGGL(assemble, dim3(slot_grid), dim3(kBlock), 0,
+                               stream, w->slots, ps, es, w->pb, w->slot_key, w->slot_start, w->entries, w->term_e,
+                               w->term_left, w->term_right, w->bro, w->ro, w->Bd, eq->primary_cross_values, w->TW,
+                               eq->cross_values, w->values);
+        }
         const auto preconditioner_kernel = ps == 6                  ? k_preconditioner<6, 6>
                                            : ps <= kSmallSchurBlock ? k_preconditioner<kSmallSchurBlock>
                                                                     : k_preconditioner<kMaxSchurBlock>;

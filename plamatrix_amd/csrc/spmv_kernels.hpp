@@ -202,11 +202,12 @@ __device__ __forceinline__ T lane_value(T x, int lane_id)
 }
 
 // Chunks whose entries do not fit one LDS window: up to kMaxWindows windows
-// walked like the single-window path, else 8 lanes per row with a fixed
-// butterfly.  Inlined on purpose: an out-of-line call gives the kernel a
-// stack (scratch) and ~70 VGPRs, which halves the throughput of a wavefront
-// that loops over many chunks.
-template <typename Acc, typename I, typename Src, bool NT>
+// walked like the single-window path, else L lanes per row (L = 8 by
+// default; 16, 32 or 64 give each lane group longer contiguous runs of a
+// long row) with a fixed butterfly.  Inlined on purpose: an out-of-line call
+// gives the kernel a stack (scratch) and ~70 VGPRs, which halves the
+// throughput of a wavefront that loops over many chunks.
+template <typename Acc, typename I, typename Src, bool NT, int L = 8>
 __device__ __forceinline__ Acc chunk_row_sums_long(int64_t base, int64_t total, int64_t my_s, int64_t my_e,
                                                 const I* __restrict__ ci, typename Src::V* lv, I* lc,
                                                 const Src src)
@@ -240,18 +241,20 @@ __device__ __forceinline__ Acc chunk_row_sums_long(int64_t base, int64_t total, 
         }
         return acc;
     }
-    const int group = lane / 8;
-    const int sub = lane % 8;
-    for (int pass = 0; pass < 8; ++pass)
+    static_assert(L >= 2 && L <= kWave && (L & (L - 1)) == 0, "lanes per row");
+    constexpr int G = kWave / L;  // rows per pass
+    const int group = lane / L;
+    const int sub = lane % L;
+    for (int pass = 0; pass < L; ++pass)
     {
-        const int row_lane = pass * 8 + group;
+        const int row_lane = pass * G + group;
         const int64_t s = __shfl(my_s, row_lane, kWave);
         const int64_t e = __shfl(my_e, row_lane, kWave);
         Acc part = Acc(0);
-        for (int64_t k = s + sub; k < e; k += 8) part = src.madd(stream_load(src.v + k, NT), src.gather(ci[k]), part);
-        part = group_sum<8>(part);
-        const Acc got = __shfl(part, (lane % 8) * 8, kWave);
-        if (lane / 8 == pass) acc = got;
+        for (int64_t k = s + sub; k < e; k += L) part = src.madd(stream_load(src.v + k, NT), src.gather(ci[k]), part);
+        part = group_sum<L>(part);
+        const Acc got = __shfl(part, (lane % G) * L, kWave);
+        if (lane / G == pass) acc = got;
     }
     return acc;
 }
@@ -260,7 +263,7 @@ __device__ __forceinline__ Acc chunk_row_sums_long(int64_t base, int64_t total, 
 // lv / lc are this wavefront's kStreamCap-entry LDS staging arrays.  s_out /
 // e_out receive the lane's entry range; rows at or beyond `rows` get an empty
 // range.  Each row is accumulated left to right.
-template <typename Acc, typename I, typename Src, bool NT = false, bool DMA = false>
+template <typename Acc, typename I, typename Src, bool NT = false, bool DMA = false, int L = 8>
 __device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const I* __restrict__ ro,
                                               const I* __restrict__ ci, typename Src::V* lv, I* lc,
                                               const Src& src, int64_t& s_out, int64_t& e_out)
@@ -274,7 +277,7 @@ __device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const 
     const int64_t total = static_cast<int64_t>(lane_value(e_i, kWave - 1)) - base;
     s_out = static_cast<int64_t>(s_i);
     e_out = static_cast<int64_t>(e_i);
-    if (total > kStreamCap) return chunk_row_sums_long<Acc, I, Src, NT>(base, total, s_out, e_out, ci, lv, lc, src);
+    if (total > kStreamCap) return chunk_row_sums_long<Acc, I, Src, NT, L>(base, total, s_out, e_out, ci, lv, lc, src);
     const int n = static_cast<int>(total);
     const int s = static_cast<int>(s_out - base);
     const int e = static_cast<int>(e_out - base);

@@ -56,12 +56,13 @@ class Batch:
     members: list = field(repr=False)  # ("host", CSRHost) or ("device", (rows, ro, ci, values))
 
     def host_csr(self, i, dtype=np.float64):
-        """(row_offsets, col_indices, values) of system i on the host."""
+        """(row_offsets, col_indices, values) of system i on the host, with the
+        values the batch solves (rounded to FP32), widened to `dtype`."""
         where, m = self.members[i]
         if where == "host":
-            return m.row_offsets, m.col_indices, m.values.astype(dtype)
+            return m.row_offsets, m.col_indices, m.values.astype(np.float32).astype(dtype)
         _, ro, ci, va = m
-        return ro.cpu().numpy(), ci.cpu().numpy(), va.cpu().numpy().astype(dtype)
+        return ro.cpu().numpy(), ci.cpu().numpy(), va.cpu().numpy().astype(np.float32).astype(dtype)
 
 
 def pack_members(capi, torch, members, dev, pad_identity=False):

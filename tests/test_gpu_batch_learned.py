@@ -221,3 +221,32 @@ def test_batch_refine_factored_reaches_target(dev, ws, net):
         res = relres(s, b[b0:e0], xs[b0:e0])
         assert rep.converged and rep.final_relres <= 1e-6, rep
         assert res <= 1.05e-6, res
+
+
+def test_split_and_fused_step_are_bitwise(dev, ws, net, monkeypatch):
+    # the step x += alpha p, r -= alpha q folded into the factor products
+    # (short factor rows) or run as its own pass (long rows): same bits,
+    # on a batch that holds reduced Schur systems
+    from plamatrix_amd import train_fsai, workloads
+
+    hosts = [train_fsai.make_system("diffusion", 2_000_100 + i) for i in range(2)]
+    hosts += [train_fsai.make_system("schur", 2_050_100 + i) for i in range(2)]
+    a, begin, end, lens, _ = workloads.pack_members(capi, torch, [("host", h) for h in hosts], dev,
+                                                    pad_identity=True)
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    rng = np.random.default_rng(3)
+    b = np.zeros(a.rows, dtype=np.float32)
+    for b0, ln in zip(begin, lens):
+        b[b0:b0 + ln] = rng.standard_normal(ln)
+    bt = torch.from_numpy(b).to(dev)
+    out = {}
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_FUSE_STEP", fuse)
+        x = torch.zeros_like(bt)
+        reps = capi.batch_pcg_factored(a, begin, end, bt, x, graph, ws, pol=capi.policy(refresh_interval=10),
+                                       relative_tolerance=1e-5, max_iterations=2000)
+        out[fuse] = (x.cpu().numpy().view(np.uint32), [r.iterations for r in reps])
+        assert all(r.converged for r in reps)
+    assert out["0"][1] == out["1"][1]
+    assert np.array_equal(out["0"][0], out["1"][0])

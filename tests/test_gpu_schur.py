@@ -161,7 +161,12 @@ def test_normal_equations_errors(dev, sw):
 
 # ---- Schur pattern, values and solve ------------------------------------------------
 
-def test_pattern_and_values_match_oracle(dev, sw, ws, oracle):
+@pytest.mark.parametrize("kernel", ["slots", "segments"])
+def test_pattern_and_values_match_oracle(dev, sw, ws, oracle, monkeypatch, kernel):
+    # slots: the CPU accumulation order, bit for bit; segments (the default
+    # for 6 x 6 / 3 x 3 blocks): the terms summed in segments of 16, a fixed
+    # order that equals the CPU's to rounding
+    monkeypatch.setenv("PLAMATRIX_AMD_SCHUR_ASSEMBLY", kernel)
     inst = shuffled_instance(24, 400, (3, 9), seed=21)
     b = device_batch(inst, dev)
     ne = capi.assemble_normal_equations(sw, 24, 400, 6, 3, **b, weights=t(inst.weights, dev, np.float64))
@@ -180,8 +185,11 @@ def test_pattern_and_values_match_oracle(dev, sw, ws, oracle):
     assert np.array_equal(capi.device_array(a.row_offsets32, n + 1, "int32"), pat["row_offsets"])
     ref_values = eq.csr_values(inst.damping).values
     got_values = capi.device_array(a.values, a.nnz)
-    # same terms in the same (eliminated, left, right) order and association
-    assert np.array_equal(got_values.view(np.uint64), ref_values.view(np.uint64))
+    if kernel == "slots":
+        # same terms in the same (eliminated, left, right) order and association
+        assert np.array_equal(got_values.view(np.uint64), ref_values.view(np.uint64))
+    else:
+        assert np.max(np.abs(got_values - ref_values)) <= 1e-13 * np.max(np.abs(ref_values))
     B, Cinv, rhs, Pinv = eq.prologue(inst.damping)
     assert np.array_equal(capi.device_array(rhs_ptr, n).view(np.uint64), rhs.view(np.uint64))
     # the preconditioner comes from the assembled diagonal block (terms in
@@ -333,6 +341,37 @@ def test_full_size_lm_system(dev, sw, ws):
     assert values == slots * 36 and S.nnz == values
     # symmetry of the assembled reduced system
     assert abs(S - S.T).max() <= 1e-9 * abs(S).max()
+
+
+@pytest.mark.parametrize("size", ["small", "full"])
+def test_segmented_assembly_matches_slot_assembly(dev, ws, monkeypatch, size):
+    # the segmented kernels (terms in segments of 16, one wavefront each,
+    # then the segment sums in order) against the slot kernel (the CPU order):
+    # equal to rounding, and bitwise the same from run to run, at a ring with
+    # long diagonal slots and at the full LM size
+    if size == "small":
+        inst = shuffled_instance(40, 3000, (3, 12), seed=5)
+        pc, ec = 40, 3000
+    else:
+        inst = generators.lm_schur_instance()
+        pc, ec = 2000, 200000
+    b = device_batch(inst, dev)
+    got = {}
+    for kernel in ("segments", "slots", "segments-again"):
+        monkeypatch.setenv("PLAMATRIX_AMD_SCHUR_ASSEMBLY", kernel.split("-")[0])
+        sw = capi.SchurWorkspace()
+        ne = capi.assemble_normal_equations(sw, pc, ec, 6, 3, **b, weights=None, huber_delta=inst.huber_delta)
+        rep = capi.schur_assemble(sw, ne, inst.damping)
+        assert rep.status == 0, rep.message
+        a, rhs_ptr, pre_ptr = capi.schur_system(sw)
+        got[kernel] = (capi.device_array(a.values, a.nnz), capi.device_array(pre_ptr, pc * 36))
+        del sw, ne
+    seg, slot = got["segments"], got["slots"]
+    assert np.max(np.abs(seg[0] - slot[0])) <= 1e-13 * np.max(np.abs(slot[0]))
+    np.testing.assert_allclose(seg[1], slot[1], rtol=1e-9, atol=1e-12 * np.max(np.abs(slot[1])))
+    again = got["segments-again"]
+    assert np.array_equal(seg[0].view(np.uint64), again[0].view(np.uint64))
+    assert np.array_equal(seg[1].view(np.uint64), again[1].view(np.uint64))
 
 
 @pytest.mark.parametrize("ps,es", [(9, 3), (12, 10), (16, 16)])
