@@ -681,36 +681,67 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaAr
 // ---- single-reduction iteration (Chronopoulos & Gear, 1989) ---------------------
 //
 // CG / Jacobi-PCG with one reduction phase per iteration, two kernels:
-//   k_cg_step  reduce r.r, gamma = r.u, delta = w.u of the previous SpMV;
-//              stop test on ||r||; beta = gamma / gamma_prev (betaKernel's
-//              rules), alpha = gamma / (delta - beta gamma / alpha_prev), which
-//              equals p.Ap in exact arithmetic (alphaKernel's rules applied to
-//              it; a non-positive value restarts the step from u); p = u + beta p,
-//              s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r
+//   k_cg_step  reduce r.r, gamma = r.u, delta = w.u of the previous SpMV and
+//              u.s of the previous step; stop test on ||r||; beta = gamma /
+//              gamma_prev (betaKernel's rules), alpha = gamma / den with den =
+//              p.Ap of the new direction (alphaKernel's rules applied to it; a
+//              non-positive den restarts the step from u); p = u + beta p,
+//              s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r, and the
+//              partial u.s of the new vectors
 //   k_cg_spmv  w = A u and the partials r.r, (r.u,) w.u
 // The recurrence s = A p replaces the SpMV of the direction, so the
 // iteration streams the same bytes as the three-kernel form with one kernel
 // boundary and one cross-workgroup reduction fewer.  u is r itself without a
-// preconditioner (the partial r.u is then r.r), M^-1 r with Jacobi.  State
-// slots alternate with the iteration parity like rho / iter.
+// preconditioner (r.u is then r.r and is not published twice), M^-1 r with
+// Jacobi.
+//
+// den expands p.Ap for p = u + beta p_prev, s_prev = A p_prev and A
+// symmetric: den = w.u + 2 beta u.s_prev + beta^2 p_prev.s_prev, with
+// p_prev.s_prev taken as the previous den (the same expansion one step
+// earlier).  The shorter textbook form delta - beta gamma / alpha_prev also
+// assumes u.r_prev = 0, which FP32 rounding and every residual refresh break;
+// on ill-conditioned systems it then stalls convergence (relres 0.07 after
+// 20,000 iterations of a refreshed FP32 solve that the three-kernel form ends
+// in 1,660), while the expanded form stays on the three-kernel iteration's
+// count.  u.s comes from vectors the step holds in registers.  State slots
+// alternate with the iteration parity like rho / iter, and so do the step's
+// partials.
 
-template <int PRE>
-constexpr int cg_values()
-{
-    return PRE == kPreJacobi ? 3 : 2;  // r.r, [r.u], w.u
-}
+constexpr int kCounterRestart = 0;  // SolverState::counters slot: a single-reduction restart happened
 
-// The decision at the head of single-reduction iteration t (slot par).
+// The step's partial u.s of iteration t sits at cg_step_slot(t % 2): the
+// three-kernel form's p.q area (unused by this form) and the area after the
+// r.r, r.u, w.u partials.
+__host__ __device__ constexpr int cg_step_slot(int par) { return par == 0 ? kPartialsDen : kPartialsRes + 3 * kMaxPartials; }
+
+// The decision at the head of single-reduction iteration t (slot par).  The
+// SpMV's partials r.r, r.u (r.r again without a preconditioner), w.u: the
+// first two sit where the three-kernel update publishes r.r and r.z, so a
+// solve can continue in that form after any iteration.
 template <typename S, int PRE>
 __device__ __forceinline__ int cg_decision(SolverState* st, const double* partials, int par, int count,
                                            double& alpha, double& beta)
 {
-    constexpr int NV = cg_values<PRE>();
+    // count is the solver grid (at most kSolverGrid workgroups), which halves
+    // the registers the loads hold next to the step's prefetched operands
+    constexpr int NV = PRE == kPreJacobi ? 4 : 3;
+    const double* res = partials + kPartialsRes;
+    const double* ex = partials + cg_step_slot(1 - par);
     double red[NV];
-    reduce_partials<NV>(partials + kPartialsRes, count, red);
+    if constexpr (PRE == kPreJacobi)
+    {
+        const double* base[NV] = {res, res + kMaxPartials, res + 2 * kMaxPartials, ex};
+        reduce_partials_at<NV, kSolverGrid>(base, count, red);
+    }
+    else
+    {
+        const double* base[NV] = {res, res + 2 * kMaxPartials, ex};
+        reduce_partials_at<NV, kSolverGrid>(base, count, red);
+    }
     const double rr = red[0];
-    const double gamma = PRE == kPreJacobi ? red[1] : red[0];
-    const double delta = red[NV - 1];
+    const double gamma = red[PRE == kPreJacobi ? 1 : 0];
+    const double delta = red[NV - 2];
+    const double us = red[NV - 1];
     const int completed = st->iter[1 - par] + 1;
     const double tol = completed != 0   ? st->tol
                        : st->tol_rule   ? st->abs_tol + st->rel_tol * sqrt(rr)
@@ -718,13 +749,13 @@ __device__ __forceinline__ int cg_decision(SolverState* st, const double* partia
     int code = kRunning;
     beta = 0.0;
     alpha = 0.0;
+    double den = delta;
     if (!isfinite(rr) || rr < 0.0) code = completed == 0 ? kInitNonFinite : kNonFinite;
     else if (sqrt(rr) <= tol) code = kConverged;
     else if (completed == 0 && st->bad_row != ~0ull) code = kBadDiagonal;
     else if (completed >= st->max_iterations) code = kIterationLimit;
     else
     {
-        double den = delta;
         if (completed > 0)
         {
             const double gamma_prev = st->rho[1 - par];
@@ -735,16 +766,18 @@ __device__ __forceinline__ int cg_decision(SolverState* st, const double* partia
                 if (!isfinite(beta) || !isfinite(static_cast<S>(beta))) code = kBreakdown;
             }
             else code = kBreakdown;
-            const double alpha_prev = st->alpha_par[1 - par];
-            if (code == kRunning) den = alpha_prev != 0.0 ? delta - beta * gamma / alpha_prev : delta;
-            // Rounding (FP32 vectors, a residual refresh that replaced r) can
-            // cost the recurrence its positivity; the step then restarts from
-            // u (beta = 0, p.Ap = u.Au = delta computed directly), and only a
-            // non-positive delta is a breakdown of A itself.
+            // u.s exists only after a completed step
+            if (code == kRunning && beta != 0.0) den = delta + beta * (2.0 * us + beta * st->den[1 - par]);
+            // a non-positive p.Ap of the new direction restarts the step from
+            // u (beta = 0, p.Ap = u.Au = delta); only a non-positive delta is
+            // a breakdown of A itself
             if (code == kRunning && !(den > 0.0 && isfinite(den)))
             {
                 beta = 0.0;
                 den = delta;
+                // tells the host to continue in the three-kernel form, which
+                // forms p.Ap from a fresh product every iteration
+                if (blockIdx.x == 0 && threadIdx.x == 0) st->counters[kCounterRestart] = 1u;
             }
         }
         if (code == kRunning && !step_length<S>(gamma, den, alpha)) code = kBreakdown;
@@ -755,7 +788,7 @@ __device__ __forceinline__ int cg_decision(SolverState* st, const double* partia
         st->iteration = completed;
         st->iter[par] = completed;
         st->rho[par] = gamma;
-        st->alpha_par[par] = alpha;
+        st->den[par] = den;
         st->alpha = alpha;
         if (completed == 0)
         {
@@ -768,7 +801,7 @@ __device__ __forceinline__ int cg_decision(SolverState* st, const double* partia
 }
 
 // The vector half of single-reduction iteration t.  STEP_R = false (a residual
-// refresh follows) leaves r and u to the refresh.
+// refresh follows) leaves r, u and the partial u.s to k_cg_resync.
 template <typename S, int PRE, bool STEP_R>
 __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, S* __restrict__ x, S* __restrict__ r,
                                                     S* __restrict__ u, S* __restrict__ p, S* __restrict__ s,
@@ -799,6 +832,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, S* __restrict_
     double alpha_d = 0.0, beta_d = 0.0;
     if (cg_decision<S, PRE>(st, partials, par, count, alpha_d, beta_d) != kRunning) return;
     const S a = static_cast<S>(alpha_d), b = static_cast<S>(beta_d);
+    double us = 0.0;
     for (int64_t base = first; base < end; base += U * kBlock)
     {
         const bool pre = base == first;
@@ -818,22 +852,47 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, S* __restrict_
             {
                 const S rn = fma_s(-a, sn, ri);
                 r[i] = rn;
-                if (PRE == kPreJacobi) u[i] = (pre ? fi[j] : inv[i]) * rn;
+                S un = rn;
+                if (PRE == kPreJacobi)
+                {
+                    un = (pre ? fi[j] : inv[i]) * rn;
+                    u[i] = un;
+                }
+                us = __builtin_fma(static_cast<double>(un), static_cast<double>(sn), us);
             }
         }
     }
+    if (STEP_R)
+    {
+        double part[1] = {us};
+        publish_partials<1>(part, partials + cg_step_slot(par));
+    }
 }
 
-// u = M^-1 r after a residual refresh or at initialisation (Jacobi only).
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_cg_precondition(int64_t rows, const S* __restrict__ r,
-                                                            const S* __restrict__ inv, S* __restrict__ u,
-                                                            const SolverState* st)
+// After the residual refresh of iteration t: u = M^-1 r (Jacobi) and the
+// partial u.s (grid_units workgroups, like the step).
+template <typename S, int PRE>
+__global__ __launch_bounds__(kBlock) void k_cg_resync(int64_t rows, const S* __restrict__ r,
+                                                      const S* __restrict__ inv, const S* __restrict__ s,
+                                                      S* __restrict__ u, const SolverState* st, double* partials,
+                                                      int par)
 {
     if (st->done != kRunning) return;
-    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < rows;
-         i += static_cast<int64_t>(gridDim.x) * kBlock)
-        u[i] = inv[i] * r[i];
+    int64_t begin, end;
+    block_range(rows, kBlock, begin, end);
+    double us = 0.0;
+    for (int64_t i = begin + threadIdx.x; i < end; i += kBlock)
+    {
+        S uv = r[i];
+        if (PRE == kPreJacobi)
+        {
+            uv = inv[i] * uv;
+            u[i] = uv;
+        }
+        us = __builtin_fma(static_cast<double>(uv), static_cast<double>(s[i]), us);
+    }
+    double part[1] = {us};
+    publish_partials<1>(part, partials + cg_step_slot(par));
 }
 
 // Initialisation after the residual: s = 0 (its first update is s = w), and
@@ -886,16 +945,8 @@ __global__ __launch_bounds__(kDirBlock) void k_cg_spmv(int64_t rows, const I* __
             wu = __builtin_fma(static_cast<double>(acc), uv, wu);
         }
     }
-    if constexpr (PRE == kPreJacobi)
-    {
-        double part[3] = {rr, ru, wu};
-        publish_partials<3>(part, partials + kPartialsRes);
-    }
-    else
-    {
-        double part[2] = {rr, wu};
-        publish_partials<2>(part, partials + kPartialsRes);
-    }
+    double part[3] = {rr, PRE == kPreJacobi ? ru : rr, wu};
+    publish_partials<3>(part, partials + kPartialsRes);
 }
 
 // k_cg_spmv on the diagonal layout.
@@ -928,16 +979,8 @@ __global__ __launch_bounds__(kDirBlock) void k_cg_spmv_dia(int64_t rows, DiaArgs
             wu = __builtin_fma(static_cast<double>(acc), uv, wu);
         }
     }
-    if constexpr (PRE == kPreJacobi)
-    {
-        double part[3] = {rr, ru, wu};
-        publish_partials<3>(part, partials + kPartialsRes);
-    }
-    else
-    {
-        double part[2] = {rr, wu};
-        publish_partials<2>(part, partials + kPartialsRes);
-    }
+    double part[3] = {rr, PRE == kPreJacobi ? ru : rr, wu};
+    publish_partials<3>(part, partials + kPartialsRes);
 }
 
 // The solver state into the host's fine-grained status slot, 8 bytes per lane.
@@ -1057,8 +1100,10 @@ struct Calls
         else out.push_back(update_any<kModeInit>(pb, pb.p0, 0));
     }
 
-    // Single-reduction iteration: w = A u and the partials (grid_rows
-    // workgroups).  Vectors: u = z (Jacobi) or r, w = q, p = p0, s = p1.
+    // Single-reduction iteration: w = A u and the partials (grid_units
+    // workgroups, as many as the three-kernel update publishes, so the solve
+    // can switch forms between chunks).  Vectors: u = z (Jacobi) or r, w = q,
+    // p = p0, s = p1.
     static void cg_spmv(const Problem& pb, std::vector<KernelCall>& out)
     {
         const bool jac = pb.pre == kPreJacobi;
@@ -1069,14 +1114,14 @@ struct Calls
                                                                                   : k_cg_spmv_dia<S, 0, kPreJacobi>)
                                : (nd == 7 ? k_cg_spmv_dia<S, 7, kPreNone> : nd == 5 ? k_cg_spmv_dia<S, 5, kPreNone>
                                                                                 : k_cg_spmv_dia<S, 0, kPreNone>);
-            out.push_back(make_call(k, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, pb.dia.args,
+            out.push_back(make_call(k, dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows, pb.dia.args,
                                     static_cast<const S*>(pb.dia.values), pb.dia.mask, static_cast<const S*>(pb.r),
                                     static_cast<const S*>(pb.z), static_cast<S*>(pb.q),
                                     static_cast<const SolverState*>(pb.st), pb.partials));
             return;
         }
         const auto k = jac ? k_cg_spmv<S, I, kPreJacobi> : k_cg_spmv<S, I, kPreNone>;
-        out.push_back(make_call(k, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a),
+        out.push_back(make_call(k, dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a),
                                 static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.r),
                                 static_cast<const S*>(pb.z), static_cast<S*>(pb.q),
                                 static_cast<const SolverState*>(pb.st), pb.partials));
@@ -1091,7 +1136,7 @@ struct Calls
         out.push_back(make_call(step, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<S*>(pb.x),
                                 static_cast<S*>(pb.r), static_cast<S*>(pb.z), static_cast<S*>(pb.p0),
                                 static_cast<S*>(pb.p1), static_cast<const S*>(pb.q), static_cast<const S*>(pb.inv),
-                                pb.st, pb.partials, par, pb.grid_rows));
+                                pb.st, pb.partials, par, pb.grid_units));
         if (refresh)
         {
             out.push_back(make_call(k_residual<S, I, false, false>, dim3(full_grid(pb.a->rows)), dim3(kBlock),
@@ -1099,12 +1144,18 @@ struct Calls
                                     static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x),
                                     static_cast<S*>(pb.r), static_cast<S*>(nullptr), static_cast<S*>(nullptr),
                                     pb.st));
-            if (jac)
-                out.push_back(make_call(k_cg_precondition<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows,
-                                        static_cast<const S*>(pb.r), static_cast<const S*>(pb.inv),
-                                        static_cast<S*>(pb.z), static_cast<const SolverState*>(pb.st)));
+            cg_resync(pb, par, out);
         }
         cg_spmv(pb, out);
+    }
+
+    // u = M^-1 r and the partial u.s after a residual refresh.
+    static void cg_resync(const Problem& pb, int par, std::vector<KernelCall>& out)
+    {
+        const auto k = pb.pre == kPreJacobi ? k_cg_resync<S, kPreJacobi> : k_cg_resync<S, kPreNone>;
+        out.push_back(make_call(k, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<const S*>(pb.r),
+                                static_cast<const S*>(pb.inv), static_cast<const S*>(pb.p1), static_cast<S*>(pb.z),
+                                static_cast<const SolverState*>(pb.st), pb.partials, par));
     }
 
     // z = G^T (G r) and the r.r, r.z partials (grid_rows workgroups).  With
@@ -1539,6 +1590,9 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         PM_HIP(hipEventSynchronize(ws->events[slot]));
         fin = hs[slot];
         if (fin.done != kRunning) break;
+        // a single-reduction restart: the chunks still to be issued take the
+        // three-kernel form (same state, parity slots, partials and vectors)
+        if (pb.single_reduction && fin.counters[kCounterRestart] != 0u) pb.single_reduction = false;
         if (inflight.empty() && launched >= needed)
             return fail(PM_RUNTIME_ERROR, "iterative solver stopped without a final state");
     }

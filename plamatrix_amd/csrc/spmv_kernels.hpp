@@ -337,16 +337,18 @@ __device__ __forceinline__ Acc chunk_row_sums(int64_t row0, int64_t rows, const 
     return acc;
 }
 
-// Every workgroup sums `count` published partials (NV values each) in the same
-// order; all threads receive the totals.
-template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* __restrict__ partials, int count, double (&out)[NV])
+// Every workgroup sums `count` (at most CAP) published partials (NV values
+// each) in the same order; all threads receive the totals.  Value k's slots
+// start at base[k].
+template <int NV, int CAP = kMaxPartials>
+__device__ __forceinline__ void reduce_partials_at(const double* const (&base)[NV], int count, double (&out)[NV])
 {
+    static_assert(CAP % kBlock == 0 && CAP <= kMaxPartials, "partial capacity");
     __shared__ double lds[NV * 16];
     __shared__ double result[NV];
     // every load is issued before the first add (kBlock threads, at most
-    // kMaxPartials / kBlock values each); the adds run in slot order
-    constexpr int kPer = kMaxPartials / kBlock;
+    // CAP / kBlock values each); the adds run in slot order
+    constexpr int kPer = CAP / kBlock;
     double vals[NV][kPer];
 #pragma unroll
     for (int k = 0; k < NV; ++k)
@@ -355,7 +357,7 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
         for (int j = 0; j < kPer; ++j)
         {
             const int b = threadIdx.x + j * kBlock;
-            vals[k][j] = b < count ? partials[k * kMaxPartials + b] : 0.0;
+            vals[k][j] = b < count ? base[k][b] : 0.0;
         }
     }
     double acc[NV];
@@ -377,6 +379,17 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
 #pragma unroll
     for (int k = 0; k < NV; ++k) out[k] = result[k];
     __syncthreads();
+}
+
+// reduce_partials_at over the layout publish_partials writes: value k's slots
+// at partials + k * kMaxPartials.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ partials, int count, double (&out)[NV])
+{
+    const double* base[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) base[k] = partials + k * kMaxPartials;
+    reduce_partials_at<NV>(base, count, out);
 }
 
 // Publish this workgroup's NV partial sums at slot blockIdx.x.

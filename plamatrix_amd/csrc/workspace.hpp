@@ -36,7 +36,7 @@ struct SolverState
     int32_t tol_rule;   // 0: tol = max(abs, rel ||r0||); 1: tol = abs + rel ||r0||
     unsigned long long bad_row;  // first row with an unusable Jacobi diagonal (~0 if none)
     unsigned counters[8];
-    double alpha_par[2];  // single-reduction iteration: step length of iteration t, slot t % 2
+    double den[2];      // single-reduction iteration: p.Ap of iteration t's direction, slot t % 2
 };
 
 enum Done : int32_t

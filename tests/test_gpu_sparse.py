@@ -552,3 +552,29 @@ def test_spmm_diffusion3d_full_size_eight_columns(dev, ws):
     for j in range(8):
         capi.spmv(a, b[j].contiguous(), yj, ws)
         assert torch.equal(y[j], yj), j
+
+
+@pytest.mark.parametrize("dims", [(40, 40, 40), (48, 48, 48)])
+def test_single_reduction_keeps_iteration_count_on_ill_conditioned_refresh(dev, monkeypatch, dims):
+    # FP32 Jacobi-PCG with a residual refresh every 50 iterations on a badly
+    # conditioned diffusion box (log-normal sigma 3, z-couplings x 0.01).  The
+    # textbook single-reduction denominator (delta - beta gamma / alpha_prev)
+    # stalls here: a CPU restatement reaches only relres 0.07 after 20,000
+    # iterations, where the three-kernel form converges in about 1,660.  The
+    # expanded denominator must converge in the three-kernel form's count.
+    # (48^3 = 110,592 rows takes the diagonal layout, 40^3 the CSR kernels.)
+    h = generators.diffusion3d_box_csr(*dims, seed=3, sigma=3.0, z_scale=0.01, dtype=np.float32)
+    a = capi.to_device_csr(h, compact=True)
+    b = torch.from_numpy(np.random.default_rng(1).standard_normal(h.rows).astype(np.float32)).to(dev)
+    pol = capi.policy(refresh_interval=50)
+    its = {}
+    for form in ("1", "0"):
+        monkeypatch.setenv("PLAMATRIX_AMD_SINGLE_REDUCTION", form)
+        ws = capi.Workspace()
+        x = torch.zeros_like(b)
+        rep = capi.pcg(a, b, x, ws, pol, relative_tolerance=1e-3, max_iterations=20000)
+        assert rep.converged, (form, rep)
+        its[form] = rep.iterations
+        res = capi.true_relres(a, b.double(), x.double(), ws)
+        assert res <= 2e-3, (form, res)
+    assert its["1"] <= 1.05 * its["0"], its
