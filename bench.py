@@ -85,7 +85,8 @@ def main() -> int:
     ap.add_argument("--cpu-iterations", type=int, default=5000,
                     help="iteration cap of the CPU baseline's FP64 solve (it stops at relres 1e-6 first)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--graph", type=int, default=-1, help="1: chunks as HIP graphs, 0: direct launches, "
+                    "-1: the library's default (graphs below 262,144 rows)")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--max-inner", type=int, default=100000)
     ap.add_argument("--inner-rel", type=float, default=1e-3,
@@ -317,7 +318,7 @@ def run_diffusion(args, torch, capi, generators):
                         "the timed solve) with Jacobi-PCG fallback; FP32 SpMV/vectors + FP64 dots, residual "
                         "refresh every 50, FP64 refinement",
             "headline": "jacobi (fallback)" if learned is None or learned["fallback_to_jacobi"] else "learned",
-            "rows": n, "nnz": nnz, "index_bits": 32, "lanes_per_row": lanes, "graph": bool(args.graph),
+            "rows": n, "nnz": nnz, "index_bits": 32, "lanes_per_row": lanes, "graph": {-1: "library default (direct launches at this size)", 0: False, 1: True}[args.graph],
         },
         "roofline": {"bound": "hbm", "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": committed_traffic(kernel),

@@ -84,6 +84,13 @@ int64_t pm_workspace_capacity_rows(const pm_workspace* ws);
  * start of each solve (PLAMATRIX_AMD_DIA=0 disables it); results are bitwise
  * those of the CSR kernels. */
 int32_t pm_workspace_solver_diagonals(const pm_workspace* ws);
+/* No reference counterpart: the value arrays the last adaptive solve's
+ * diagonal-layout row sums read per row (0: the CSR kernels).  A matrix that is
+ * symmetric bit for bit reads its lower diagonals from their upper mirrors,
+ * and a main diagonal of exact ones from no array, so a unit-diagonal 7-point
+ * stencil reads 3 arrays instead of 7 (PLAMATRIX_AMD_DIA_MIRROR=0 reads every
+ * diagonal); results are bitwise those of the CSR kernels. */
+int32_t pm_workspace_solver_value_streams(const pm_workspace* ws);
 /* No reference counterpart: the preconditioner the last adaptive solve ran
  * (0: none, 1: Jacobi, 2: block-Jacobi, 3: factored G^T G).  A Jacobi solve of
  * a matrix of at least 65,536 rows whose every Jacobi inverse is exactly 1 (a
@@ -196,7 +203,8 @@ typedef struct pm_solver_report {
  * the defaults. */
 typedef struct pm_solve_policy {
     int32_t chunk_iterations;  /* iterations launched per device-side chunk (0: 16) */
-    int32_t use_graph;         /* 1: replay chunks as HIP graphs, 0: direct launches, -1: default */
+    int32_t use_graph;         /* 1: replay chunks as HIP graphs, 0: direct launches, -1: default
+                                  (graphs below 262,144 rows, direct launches above) */
     int32_t refresh_interval;  /* recompute r = b - A x (FP64 accumulation) every k iterations; 0: never */
     int32_t lanes_per_row;     /* SpMV lanes per row; 0: autotuned / heuristic */
 } pm_solve_policy;

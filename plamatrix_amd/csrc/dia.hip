@@ -124,6 +124,62 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(int64_t rows, const int32_t
     }
 }
 
+// Mirror check of a filled layout: flag 8 when some lower entry (row,
+// row + off[d]) is present without its mirror (row + off[d], row) on diagonal
+// mirror.of[d], or the other way round, or their values differ in any bit;
+// flag 16 when some row's main-diagonal entry is absent or not exactly 1.
+struct DiaMirror
+{
+    int32_t of[kMaxDia];  // index of diagonal -off[d], for every lower diagonal d
+};
+
+template <typename S>
+__device__ __forceinline__ bool same_bits(S a, S b)
+{
+    if constexpr (sizeof(S) == 4) return __float_as_uint(a) == __float_as_uint(b);
+    else return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_dia_mirror(int64_t rows, DiaArgs dia, DiaMirror mirror,
+                                                      const S* __restrict__ dv, const uint16_t* __restrict__ mask,
+                                                      int main_d, int* table)
+{
+    __shared__ int flags;
+    if (threadIdx.x == 0) flags = 0;
+    __syncthreads();
+    int my_flags = 0;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += stride)
+    {
+        const unsigned m = mask[r];
+        for (int d = 0; d < dia.ndiag && dia.off[d] < 0; ++d)
+        {
+            const bool here = (m >> d) & 1u;
+            const int64_t src = r + dia.off[d];
+            if (src < 0)
+            {
+                if (here) my_flags |= 8;
+                continue;
+            }
+            const int e = mirror.of[d];
+            const bool there = (mask[src] >> e) & 1u;
+            if (here != there) my_flags |= 8;
+            else if (here)
+            {
+                const S a = dv[static_cast<int64_t>(d) * rows + r];
+                const S b = dv[static_cast<int64_t>(e) * rows + src];
+                if (!same_bits(a, b)) my_flags |= 8;
+            }
+        }
+        if (main_d < 0 || !((m >> main_d) & 1u) || dv[static_cast<int64_t>(main_d) * rows + r] != S(1))
+            my_flags |= 16;
+    }
+    if (my_flags) atomicOr(&flags, my_flags);
+    __syncthreads();
+    if (threadIdx.x == 0 && flags) atomicOr(&table[kMaxDia], flags);
+}
+
 } // namespace
 
 unsigned long long dia_signature(const DiaLayout& d)
@@ -132,13 +188,19 @@ unsigned long long dia_signature(const DiaLayout& d)
     auto mix = [&](unsigned long long v) { sig = (sig ^ v) * 1099511628211ull; };
     mix(d.active ? 1 : 0);
     if (!d.active) return sig;
-    for (int i = 0; i < d.args.ndiag; ++i) mix(static_cast<unsigned>(d.args.off[i]));
+    for (int i = 0; i < d.args.ndiag; ++i)
+    {
+        mix(static_cast<unsigned>(d.args.off[i]));
+        mix(static_cast<unsigned>(d.args.slot[i]));
+        mix(static_cast<unsigned>(d.args.shift[i]));
+    }
     mix(reinterpret_cast<unsigned long long>(d.values));
     mix(reinterpret_cast<unsigned long long>(d.mask));
     return sig;
 }
 
-int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream, bool* unit_diagonal)
+int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_t stream, bool* unit_diagonal,
+              bool mirror)
 {
     *out = DiaLayout{};
     if (unit_diagonal) *unit_diagonal = false;
@@ -172,6 +234,11 @@ int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_
     for (int s = 0; s < kMaxDia; ++s)
         if (host[s] != kEmpty) args.off[args.ndiag++] = host[s];
     std::sort(args.off, args.off + args.ndiag);
+    for (int d = 0; d < args.ndiag; ++d)
+    {
+        args.slot[d] = d;
+        args.shift[d] = 0;
+    }
     if (args.ndiag == 0 || static_cast<double>(args.ndiag) * static_cast<double>(n) >
                                kDiaMaxFill * static_cast<double>(a->nnz))
         return PM_OK;
@@ -189,6 +256,55 @@ int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_
         hipLaunchKernelGGL(k_dia_fill<float>, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32,
                            a->col_indices32, static_cast<const float*>(a->values), args, static_cast<float*>(dv), mask);
     PM_HIP(hipGetLastError());
+    out->streams = args.ndiag;
+    if (mirror)
+    {
+        // the mirror of every lower diagonal must be a diagonal of the layout
+        DiaMirror mirrors{};
+        int main_d = -1, lower = 0;
+        bool paired = true;
+        for (int d = 0; d < args.ndiag; ++d)
+        {
+            if (args.off[d] == 0) main_d = d;
+            if (args.off[d] >= 0) continue;
+            ++lower;
+            int e = -1;
+            for (int k = 0; k < args.ndiag; ++k)
+                if (args.off[k] == -args.off[d]) e = k;
+            if (e < 0) paired = false;
+            mirrors.of[d] = e;
+        }
+        if (paired)
+        {
+            table = reinterpret_cast<int*>(base);  // the storage may have moved
+            hipLaunchKernelGGL(k_dia_reset, dim3(1), dim3(64), 0, stream, table);
+            if (a->dtype == PM_F64)
+                hipLaunchKernelGGL(k_dia_mirror<double>, dim3(grid), dim3(kBlock), 0, stream, n, args, mirrors,
+                                   static_cast<const double*>(dv), mask, main_d, table);
+            else
+                hipLaunchKernelGGL(k_dia_mirror<float>, dim3(grid), dim3(kBlock), 0, stream, n, args, mirrors,
+                                   static_cast<const float*>(dv), mask, main_d, table);
+            PM_HIP(hipGetLastError());
+            int flags = 0;
+            PM_HIP(hipMemcpyAsync(&flags, table + kMaxDia, sizeof(flags), hipMemcpyDeviceToHost, stream));
+            PM_HIP(hipStreamSynchronize(stream));
+            if ((flags & 8) == 0)
+            {
+                for (int d = 0; d < args.ndiag; ++d)
+                    if (args.off[d] < 0)
+                    {
+                        args.slot[d] = mirrors.of[d];
+                        args.shift[d] = args.off[d];
+                    }
+                out->streams -= lower;
+            }
+            if ((flags & 16) == 0 && main_d >= 0)
+            {
+                args.slot[main_d] = -1;
+                --out->streams;
+            }
+        }
+    }
     out->active = true;
     out->args = args;
     out->values = dv;

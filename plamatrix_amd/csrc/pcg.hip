@@ -58,6 +58,7 @@ constexpr int kDirWaves = kDirBlock / kWave;
 constexpr int kDirGrid = kMaxPartials;          // 2048 x 4 waves: 8 workgroups per CU, chunks dealt round-robin
 constexpr int kPartialsDen = 0;                 // p.q partials (1 value)
 constexpr int kPartialsRes = 2 * kMaxPartials;  // r.r, r.z partials (2 values)
+constexpr int kPartialsDoubles = 10 * kMaxPartials;  // with the single-reduction regions at 2K..6K, 6K..10K
 
 __global__ void k_state_init(SolverState* st, int32_t max_iterations, double rel_tol, double abs_tol,
                              int32_t tol_rule)
@@ -207,6 +208,11 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den(int64_t rows, const I* _
     publish_partials<1>(part, partials + kPartialsDen);
 }
 
+// Chunks a wave of a diagonal-layout SpMV keeps in flight: one chunk's loads
+// (mask, value arrays, gathers) do not cover the latency of the 256 MB
+// last-level cache the iteration's working set lives in.
+constexpr int kDiaInFlight = 2;
+
 // k_spmv_den on the diagonal layout: a lane per row adds the row's present
 // diagonals in increasing column order with the CSR row walk's fused
 // multiply-adds, so q and p.q are bitwise those of k_spmv_den.
@@ -218,42 +224,34 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den_dia(int64_t rows, DiaArg
 {
     if (st->done != kRunning) return;
     const int lane = threadIdx.x % kWave;
-    const int nd = ND > 0 ? ND : dia.ndiag;
     double den = 0.0;
-    constexpr int kSlots = ND > 0 ? ND : kMaxDia;
     // each XCD walks one contiguous range of rows, so the gathered p entries
     // (the same rows plus the diagonals' reach) are fetched into its L2 once
     const int64_t chunks = chunk_count(rows);
     const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
     const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
     const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
-    for (int64_t c = cbegin + threadIdx.x / kWave; c < cend; c += kDirWaves)
+    for (int64_t c0 = cbegin + threadIdx.x / kWave; c0 < cend; c0 += kDiaInFlight * kDirWaves)
     {
-        const int64_t row = c * kChunkRows + lane;
-        if (row < rows)
+        int64_t rw[kDiaInFlight];
+        bool ok[kDiaInFlight];
+        S acc[kDiaInFlight], ps[kDiaInFlight];
+#pragma unroll
+        for (int k = 0; k < kDiaInFlight; ++k)
         {
-            // every load is issued before the mask is known (absent slots read
-            // an in-range value and a clamped gather that the mask discards),
-            // so the row's loads are all in flight together
-            const unsigned m = mask[row];
-            S vals[kSlots], xs[kSlots];
+            const int64_t c = c0 + k * kDirWaves;
+            const int64_t row = c * kChunkRows + lane;
+            ok[k] = c < cend && row < rows;
+            rw[k] = ok[k] ? row : rows - 1;
+            acc[k] = dia_row<S, ND>(rw[k], rows, dia, dv, mask[rw[k]], p);
+            ps[k] = p[rw[k]];
+        }
 #pragma unroll
-            for (int d = 0; d < kSlots; ++d)
-            {
-                if (d < nd)
-                {
-                    int64_t col = row + dia.off[d];
-                    col = col < 0 ? 0 : (col >= rows ? rows - 1 : col);
-                    vals[d] = dv[d * rows + row];
-                    xs[d] = p[col];
-                }
-            }
-            S acc = S(0);
-#pragma unroll
-            for (int d = 0; d < kSlots; ++d)
-                if (d < nd && ((m >> d) & 1u)) acc = fma_s(vals[d], xs[d], acc);
-            q[row] = acc;
-            den = __builtin_fma(static_cast<double>(p[row]), static_cast<double>(acc), den);
+        for (int k = 0; k < kDiaInFlight; ++k)
+        {
+            if (!ok[k]) continue;
+            q[rw[k]] = acc[k];
+            den = __builtin_fma(static_cast<double>(ps[k]), static_cast<double>(acc[k]), den);
         }
     }
     double part[1] = {den};
@@ -277,9 +275,7 @@ __global__ __launch_bounds__(kDirBlock) void k_dir_spmv_dia(int64_t rows, DiaArg
     if (direction_decision<S>(st, partials, par, count_res, beta) != kRunning) return;
     const S b = static_cast<S>(beta);
     const int lane = threadIdx.x % kWave;
-    const int nd = ND > 0 ? ND : dia.ndiag;
     double den = 0.0;
-    constexpr int kSlots = ND > 0 ? ND : kMaxDia;
     const int64_t chunks = chunk_count(rows);
     const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
     const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
@@ -289,23 +285,8 @@ __global__ __launch_bounds__(kDirBlock) void k_dir_spmv_dia(int64_t rows, DiaArg
         const int64_t row = c * kChunkRows + lane;
         if (row < rows)
         {
-            const unsigned m = mask[row];
-            S vals[kSlots], xs[kSlots];
-#pragma unroll
-            for (int d = 0; d < kSlots; ++d)
-            {
-                if (d < nd)
-                {
-                    int64_t col = row + dia.off[d];
-                    col = col < 0 ? 0 : (col >= rows ? rows - 1 : col);
-                    vals[d] = dv[d * rows + row];
-                    xs[d] = fma_s(b, p_old[col], z[col]);
-                }
-            }
-            S acc = S(0);
-#pragma unroll
-            for (int d = 0; d < kSlots; ++d)
-                if (d < nd && ((m >> d) & 1u)) acc = fma_s(vals[d], xs[d], acc);
+            const S acc = dia_row_gather<S, ND>(row, rows, dia, dv, mask[row],
+                                                [=](int64_t c) { return fma_s(b, p_old[c], z[c]); });
             const S pn = fma_s(b, p_old[row], z[row]);
             p_new[row] = pn;
             q[row] = acc;
@@ -680,20 +661,28 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaAr
 
 // ---- single-reduction iteration (Chronopoulos & Gear, 1989) ---------------------
 //
-// CG / Jacobi-PCG with one reduction phase per iteration, two kernels:
-//   k_cg_step  reduce r.r, gamma = r.u, delta = w.u of the previous SpMV and
-//              u.s of the previous step; stop test on ||r||; beta = gamma /
-//              gamma_prev (betaKernel's rules), alpha = gamma / den with den =
-//              p.Ap of the new direction (alphaKernel's rules applied to it; a
+// CG / Jacobi-PCG with one reduction phase per iteration.  Two kernels on CSR:
+//   k_cg_step  reduce r.r, gamma = r.u, delta = w.u and u.s of the previous
+//              iteration; stop test on ||r||; beta = gamma / gamma_prev
+//              (betaKernel's rules), alpha = gamma / den with den = p.Ap of the
+//              new direction (alphaKernel's rules applied to it; a
 //              non-positive den restarts the step from u); p = u + beta p,
-//              s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r, and the
-//              partial u.s of the new vectors
-//   k_cg_spmv  w = A u and the partials r.r, (r.u,) w.u
+//              s = w + beta s, x += alpha p, r -= alpha s, u = M^-1 r
+//   k_cg_spmv  w = A u and the partials r.r, (r.u,) w.u, u.s
 // The recurrence s = A p replaces the SpMV of the direction, so the
 // iteration streams the same bytes as the three-kernel form with one kernel
 // boundary and one cross-workgroup reduction fewer.  u is r itself without a
 // preconditioner (r.u is then r.r and is not published twice), M^-1 r with
 // Jacobi.
+//
+// On the diagonal layout one kernel does both (k_cg_fused_dia): after the
+// decision every workgroup forms the new r, s, u of its own rows and, for the
+// product w = A u of the new u, recomputes the new u of each neighbouring row
+// from that row's r, w and s with the same fused multiply-adds its owner
+// uses.  r, w, s (and u) alternate between two buffers with the iteration
+// parity, so the old values a neighbour reads are never overwritten in the
+// launch; p and x are only read at their own rows and stay in place.  A
+// residual-refresh iteration runs the two-kernel form on the same buffers.
 //
 // den expands p.Ap for p = u + beta p_prev, s_prev = A p_prev and A
 // symmetric: den = w.u + 2 beta u.s_prev + beta^2 p_prev.s_prev, with
@@ -703,39 +692,42 @@ __global__ __launch_bounds__(kBlock) void k_factor_upper_dia(int64_t rows, DiaAr
 // on ill-conditioned systems it then stalls convergence (relres 0.07 after
 // 20,000 iterations of a refreshed FP32 solve that the three-kernel form ends
 // in 1,660), while the expanded form stays on the three-kernel iteration's
-// count.  u.s comes from vectors the step holds in registers.  State slots
-// alternate with the iteration parity like rho / iter, and so do the step's
-// partials.
+// count.  The SpMV kernels sum u.s with the other partials, over the same
+// chunks in the same order as the one-kernel iteration, so the two forms
+// (and the CSR and diagonal layouts) stay bitwise equal.
+//
+// Partials: iteration t publishes r.r, r.u, w.u, u.s (kCgValues slots of
+// kMaxPartials each) into region t % 2 and its decision reads region
+// (t - 1) % 2.  Region 1 is the three-kernel form's r.r / r.z area, so after
+// an odd iteration (every chunk of iterations ends on one) a solve can
+// continue in that form.
 
 constexpr int kCounterRestart = 0;  // SolverState::counters slot: a single-reduction restart happened
+constexpr int kCgValues = 4;        // r.r, r.u, w.u, u.s
 
-// The step's partial u.s of iteration t sits at cg_step_slot(t % 2): the
-// three-kernel form's p.q area (unused by this form) and the area after the
-// r.r, r.u, w.u partials.
-__host__ __device__ constexpr int cg_step_slot(int par) { return par == 0 ? kPartialsDen : kPartialsRes + 3 * kMaxPartials; }
+__host__ __device__ inline double* cg_region(double* partials, int par)
+{
+    return partials + kPartialsRes + (par == 1 ? 0 : kCgValues * kMaxPartials);
+}
 
-// The decision at the head of single-reduction iteration t (slot par).  The
-// SpMV's partials r.r, r.u (r.r again without a preconditioner), w.u: the
-// first two sit where the three-kernel update publishes r.r and r.z, so a
-// solve can continue in that form after any iteration.
+// The decision at the head of single-reduction iteration t (slot par), from
+// the partials `in` of iteration t - 1.
 template <typename S, int PRE>
-__device__ __forceinline__ int cg_decision(SolverState* st, const double* partials, int par, int count,
-                                           double& alpha, double& beta)
+__device__ __forceinline__ int cg_decision(SolverState* st, const double* in, int par, int count, double& alpha,
+                                           double& beta)
 {
     // count is the solver grid (at most kSolverGrid workgroups), which halves
     // the registers the loads hold next to the step's prefetched operands
     constexpr int NV = PRE == kPreJacobi ? 4 : 3;
-    const double* res = partials + kPartialsRes;
-    const double* ex = partials + cg_step_slot(1 - par);
     double red[NV];
     if constexpr (PRE == kPreJacobi)
     {
-        const double* base[NV] = {res, res + kMaxPartials, res + 2 * kMaxPartials, ex};
+        const double* base[NV] = {in, in + kMaxPartials, in + 2 * kMaxPartials, in + 3 * kMaxPartials};
         reduce_partials_at<NV, kSolverGrid>(base, count, red);
     }
     else
     {
-        const double* base[NV] = {res, res + 2 * kMaxPartials, ex};
+        const double* base[NV] = {in, in + 2 * kMaxPartials, in + 3 * kMaxPartials};
         reduce_partials_at<NV, kSolverGrid>(base, count, red);
     }
     const double rr = red[0];
@@ -800,18 +792,36 @@ __device__ __forceinline__ int cg_decision(SolverState* st, const double* partia
     return code;
 }
 
+// Vectors of single-reduction iteration t: iteration t's r, u, w and s_{t-1}
+// (`_in`) and the new r, u, s (`_out`), which are the same arrays when the
+// iteration works in place.  p and x are always updated in place.  Without a
+// preconditioner u is r.
+template <typename S>
+struct CgVectors
+{
+    S* x;
+    S* p;
+    const S* r_in;
+    const S* u_in;
+    const S* w_in;
+    const S* s_in;
+    S* r_out;
+    S* u_out;
+    S* w_out;
+    S* s_out;
+    const S* inv;
+};
+
 // The vector half of single-reduction iteration t.  STEP_R = false (a residual
-// refresh follows) leaves r, u and the partial u.s to k_cg_resync.
+// refresh follows) leaves r and u to the refresh.
 template <typename S, int PRE, bool STEP_R>
-__global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, S* __restrict__ x, S* __restrict__ r,
-                                                    S* __restrict__ u, S* __restrict__ p, S* __restrict__ s,
-                                                    const S* __restrict__ w, const S* __restrict__ inv,
-                                                    SolverState* st, double* partials, int par, int count)
+__global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, CgVectors<S> v, SolverState* st,
+                                                    const double* part_in, int par, int count)
 {
     if (st->done != kRunning) return;
-    constexpr int U = 4;
+    constexpr int U = 8;
     int64_t begin, end;
-    block_range(rows, U * kBlock, begin, end);
+    block_range(rows, kBlock, begin, end);
     const int64_t first = begin + threadIdx.x;
     // the first step's operands are loaded before the cross-workgroup
     // reduction so their latency overlaps it
@@ -821,18 +831,17 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, S* __restrict_
     {
         const int64_t i = first + j * kBlock;
         const bool in = i < end;
-        fr[j] = in ? r[i] : S(0);
-        fu[j] = PRE == kPreJacobi ? (in ? u[i] : S(0)) : fr[j];
-        fp[j] = in ? p[i] : S(0);
-        fs[j] = in ? s[i] : S(0);
-        fw[j] = in ? w[i] : S(0);
-        fx[j] = in ? x[i] : S(0);
-        fi[j] = PRE == kPreJacobi && in ? inv[i] : S(0);
+        fr[j] = in ? v.r_in[i] : S(0);
+        fu[j] = PRE == kPreJacobi ? (in ? v.u_in[i] : S(0)) : fr[j];
+        fp[j] = in ? v.p[i] : S(0);
+        fs[j] = in ? v.s_in[i] : S(0);
+        fw[j] = in ? v.w_in[i] : S(0);
+        fx[j] = in ? v.x[i] : S(0);
+        fi[j] = PRE == kPreJacobi && in ? v.inv[i] : S(0);
     }
     double alpha_d = 0.0, beta_d = 0.0;
-    if (cg_decision<S, PRE>(st, partials, par, count, alpha_d, beta_d) != kRunning) return;
+    if (cg_decision<S, PRE>(st, part_in, par, count, alpha_d, beta_d) != kRunning) return;
     const S a = static_cast<S>(alpha_d), b = static_cast<S>(beta_d);
-    double us = 0.0;
     for (int64_t base = first; base < end; base += U * kBlock)
     {
         const bool pre = base == first;
@@ -841,58 +850,31 @@ __global__ __launch_bounds__(kBlock) void k_cg_step(int64_t rows, S* __restrict_
         {
             const int64_t i = base + j * kBlock;
             if (i >= end) continue;
-            const S ri = pre ? fr[j] : r[i];
-            const S ui = PRE == kPreJacobi ? (pre ? fu[j] : u[i]) : ri;
-            const S pn = fma_s(b, pre ? fp[j] : p[i], ui);
-            const S sn = fma_s(b, pre ? fs[j] : s[i], pre ? fw[j] : w[i]);
-            p[i] = pn;
-            s[i] = sn;
-            x[i] = fma_s(a, pn, pre ? fx[j] : x[i]);
+            const S ri = pre ? fr[j] : v.r_in[i];
+            const S ui = PRE == kPreJacobi ? (pre ? fu[j] : v.u_in[i]) : ri;
+            const S pn = fma_s(b, pre ? fp[j] : v.p[i], ui);
+            const S sn = fma_s(b, pre ? fs[j] : v.s_in[i], pre ? fw[j] : v.w_in[i]);
+            v.p[i] = pn;
+            v.s_out[i] = sn;
+            v.x[i] = fma_s(a, pn, pre ? fx[j] : v.x[i]);
             if (STEP_R)
             {
                 const S rn = fma_s(-a, sn, ri);
-                r[i] = rn;
-                S un = rn;
-                if (PRE == kPreJacobi)
-                {
-                    un = (pre ? fi[j] : inv[i]) * rn;
-                    u[i] = un;
-                }
-                us = __builtin_fma(static_cast<double>(un), static_cast<double>(sn), us);
+                v.r_out[i] = rn;
+                if (PRE == kPreJacobi) v.u_out[i] = (pre ? fi[j] : v.inv[i]) * rn;
             }
         }
     }
-    if (STEP_R)
-    {
-        double part[1] = {us};
-        publish_partials<1>(part, partials + cg_step_slot(par));
-    }
 }
 
-// After the residual refresh of iteration t: u = M^-1 r (Jacobi) and the
-// partial u.s (grid_units workgroups, like the step).
-template <typename S, int PRE>
-__global__ __launch_bounds__(kBlock) void k_cg_resync(int64_t rows, const S* __restrict__ r,
-                                                      const S* __restrict__ inv, const S* __restrict__ s,
-                                                      S* __restrict__ u, const SolverState* st, double* partials,
-                                                      int par)
+// u = M^-1 r after the residual refresh of iteration t (Jacobi only).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_cg_precondition(int64_t rows, CgVectors<S> v, const SolverState* st)
 {
     if (st->done != kRunning) return;
-    int64_t begin, end;
-    block_range(rows, kBlock, begin, end);
-    double us = 0.0;
-    for (int64_t i = begin + threadIdx.x; i < end; i += kBlock)
-    {
-        S uv = r[i];
-        if (PRE == kPreJacobi)
-        {
-            uv = inv[i] * uv;
-            u[i] = uv;
-        }
-        us = __builtin_fma(static_cast<double>(uv), static_cast<double>(s[i]), us);
-    }
-    double part[1] = {us};
-    publish_partials<1>(part, partials + cg_step_slot(par));
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < rows;
+         i += static_cast<int64_t>(gridDim.x) * kBlock)
+        v.u_out[i] = v.inv[i] * v.r_out[i];
 }
 
 // Initialisation after the residual: s = 0 (its first update is s = w), and
@@ -910,20 +892,21 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(int64_t rows, const S* __res
     }
 }
 
-// w = A u (CSR) and the partials of the next decision.
+// w = A u (CSR) of the new u and the partials r.r, (r.u,) w.u, u.s of the
+// next decision.
 template <typename S, typename I, int PRE>
 __global__ __launch_bounds__(kDirBlock) void k_cg_spmv(int64_t rows, const I* __restrict__ ro,
-                                                       const I* __restrict__ ci, const S* __restrict__ v,
-                                                       const S* __restrict__ r, const S* __restrict__ u,
-                                                       S* __restrict__ w, const SolverState* st, double* partials)
+                                                       const I* __restrict__ ci, const S* __restrict__ vals,
+                                                       CgVectors<S> v, const SolverState* st, double* part_out)
 {
     if (st->done != kRunning) return;
     __shared__ S lv_all[kDirWaves][kStreamCap];
     __shared__ I lc_all[kDirWaves][kStreamCap];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    const S* uu = PRE == kPreJacobi ? u : r;
-    double rr = 0.0, ru = 0.0, wu = 0.0;
+    const S* r = v.r_out;
+    const S* uu = PRE == kPreJacobi ? v.u_out : r;
+    double rr = 0.0, ru = 0.0, wu = 0.0, us = 0.0;
     const int64_t chunks = chunk_count(rows);
     const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
     const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
@@ -933,54 +916,208 @@ __global__ __launch_bounds__(kDirBlock) void k_cg_spmv(int64_t rows, const I* __
         const int64_t row0 = c * kChunkRows;
         int64_t ks, ke;
         const S acc = chunk_row_sums<S, I, SpmvSource<S>, false, false>(row0, rows, ro, ci, lv_all[wave],
-                                                                        lc_all[wave], SpmvSource<S>{v, uu}, ks, ke);
+                                                                        lc_all[wave], SpmvSource<S>{vals, uu}, ks, ke);
         const int64_t row = row0 + lane;
         if (row < rows)
         {
-            w[row] = acc;
+            v.w_out[row] = acc;
             const double rv = static_cast<double>(r[row]);
             const double uv = static_cast<double>(uu[row]);
             rr = __builtin_fma(rv, rv, rr);
             if (PRE == kPreJacobi) ru = __builtin_fma(rv, uv, ru);
             wu = __builtin_fma(static_cast<double>(acc), uv, wu);
+            us = __builtin_fma(uv, static_cast<double>(v.s_out[row]), us);
         }
     }
-    double part[3] = {rr, PRE == kPreJacobi ? ru : rr, wu};
-    publish_partials<3>(part, partials + kPartialsRes);
+    double part[4] = {rr, PRE == kPreJacobi ? ru : rr, wu, us};
+    publish_partials<4>(part, part_out);
 }
 
 // k_cg_spmv on the diagonal layout.
 template <typename S, int ND, int PRE>
 __global__ __launch_bounds__(kDirBlock) void k_cg_spmv_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
-                                                           const uint16_t* __restrict__ mask,
-                                                           const S* __restrict__ r, const S* __restrict__ u,
-                                                           S* __restrict__ w, const SolverState* st,
-                                                           double* partials)
+                                                           const uint16_t* __restrict__ mask, CgVectors<S> v,
+                                                           const SolverState* st, double* part_out)
 {
     if (st->done != kRunning) return;
     const int lane = threadIdx.x % kWave;
-    const S* uu = PRE == kPreJacobi ? u : r;
-    double rr = 0.0, ru = 0.0, wu = 0.0;
+    const S* r = v.r_out;
+    const S* uu = PRE == kPreJacobi ? v.u_out : r;
+    double rr = 0.0, ru = 0.0, wu = 0.0, us = 0.0;
     const int64_t chunks = chunk_count(rows);
     const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
     const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
     const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
-    for (int64_t c = cbegin + threadIdx.x / kWave; c < cend; c += kDirWaves)
+    // kDiaInFlight consecutive chunks of the wave per pass, every load issued
+    // before the first sum (rows past the end load row rows - 1 and are
+    // dropped); the partials still add the chunks in the wave's chunk order
+    for (int64_t c0 = cbegin + threadIdx.x / kWave; c0 < cend; c0 += kDiaInFlight * kDirWaves)
     {
-        const int64_t row = c * kChunkRows + lane;
-        if (row < rows)
+        int64_t rw[kDiaInFlight];
+        bool ok[kDiaInFlight];
+        S acc[kDiaInFlight], rs[kDiaInFlight], uk[kDiaInFlight], sk[kDiaInFlight];
+#pragma unroll
+        for (int k = 0; k < kDiaInFlight; ++k)
         {
-            const S acc = dia_row<S, ND>(row, rows, dia, dv, mask[row], uu);
-            w[row] = acc;
-            const double rv = static_cast<double>(r[row]);
-            const double uv = static_cast<double>(uu[row]);
+            const int64_t c = c0 + k * kDirWaves;
+            const int64_t row = c * kChunkRows + lane;
+            ok[k] = c < cend && row < rows;
+            rw[k] = ok[k] ? row : rows - 1;
+            acc[k] = dia_row<S, ND>(rw[k], rows, dia, dv, mask[rw[k]], uu);
+            rs[k] = r[rw[k]];
+            uk[k] = uu[rw[k]];
+            sk[k] = v.s_out[rw[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kDiaInFlight; ++k)
+        {
+            if (!ok[k]) continue;
+            v.w_out[rw[k]] = acc[k];
+            const double rv = static_cast<double>(rs[k]);
+            const double uv = static_cast<double>(uk[k]);
+            rr = __builtin_fma(rv, rv, rr);
+            if (PRE == kPreJacobi) ru = __builtin_fma(rv, uv, ru);
+            wu = __builtin_fma(static_cast<double>(acc[k]), uv, wu);
+            us = __builtin_fma(uv, static_cast<double>(sk[k]), us);
+        }
+    }
+    double part[4] = {rr, PRE == kPreJacobi ? ru : rr, wu, us};
+    publish_partials<4>(part, part_out);
+}
+
+// The operands one lane of k_cg_fused_dia loads for one row: the row's
+// diagonal values and, per diagonal, the old r, w, s (and inverse diagonal)
+// of the neighbouring row, plus the row's own vectors.  None of the addresses
+// depend on alpha or beta, so a pass's loads can be in flight during the
+// decision.
+template <typename S, int ND, int PRE>
+struct CgFusedRow
+{
+    static constexpr int kSlots = ND > 0 ? ND : kMaxDia;
+    S val[kSlots], nr[kSlots], nw[kSlots], ns[kSlots], ni[kSlots];
+    S r, s, w, p, x, u, iv;
+    unsigned m;
+
+    __device__ __forceinline__ void load(int64_t i, int64_t rows, const DiaArgs& dia, const S* __restrict__ dv,
+                                         const uint16_t* __restrict__ mask, const CgVectors<S>& v)
+    {
+        const int nd = ND > 0 ? ND : dia.ndiag;
+        m = mask[i];
+#pragma unroll
+        for (int d = 0; d < kSlots; ++d)
+        {
+            if (d < nd)
+            {
+                int64_t c = i + dia.off[d];
+                c = c < 0 ? 0 : (c >= rows ? rows - 1 : c);
+                val[d] = dia_value(i, rows, dia, dv, d);
+                nr[d] = v.r_in[c];
+                nw[d] = v.w_in[c];
+                ns[d] = v.s_in[c];
+                ni[d] = PRE == kPreJacobi ? v.inv[c] : S(0);
+            }
+        }
+        r = v.r_in[i];
+        s = v.s_in[i];
+        w = v.w_in[i];
+        p = v.p[i];
+        x = v.x[i];
+        u = PRE == kPreJacobi ? v.u_in[i] : r;
+        iv = PRE == kPreJacobi ? v.inv[i] : S(0);
+    }
+
+    // The row of w = A u of the new u: each present neighbour's new u formed
+    // with its owner's fused multiply-adds, summed in increasing column order.
+    __device__ __forceinline__ S product(S a, S b, const DiaArgs& dia) const
+    {
+        const int nd = ND > 0 ? ND : dia.ndiag;
+        S acc = S(0);
+#pragma unroll
+        for (int d = 0; d < kSlots; ++d)
+        {
+            if (d < nd && ((m >> d) & 1u))
+            {
+                const S rn = fma_s(-a, fma_s(b, ns[d], nw[d]), nr[d]);
+                acc = fma_s(val[d], PRE == kPreJacobi ? ni[d] * rn : rn, acc);
+            }
+        }
+        return acc;
+    }
+};
+
+// One whole single-reduction iteration on the diagonal layout: the decision,
+// then for every own row the step (p, s, x, r, u) and the row of w = A u of
+// the new u, whose neighbouring entries are recomputed from the old r, w, s
+// of those rows; partials r.r, (r.u,) w.u, u.s.  Needs distinct _in and _out
+// arrays for r, w, s (and u).  The first pass's loads are issued before the
+// decision's cross-workgroup reduction.
+template <typename S, int ND, int PRE>
+__global__ __launch_bounds__(kDirBlock) void k_cg_fused_dia(int64_t rows, DiaArgs dia, const S* __restrict__ dv,
+                                                            const uint16_t* __restrict__ mask, CgVectors<S> v,
+                                                            SolverState* st, const double* part_in, double* part_out,
+                                                            int par, int count)
+{
+    if (st->done != kRunning) return;
+    // rows in flight per lane and pass (the generic diagonal count holds 16
+    // slots of operands per row)
+    constexpr int F = ND > 0 ? kDiaInFlight : 1;
+    const int lane = threadIdx.x % kWave;
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    CgFusedRow<S, ND, PRE> row[F];
+    int64_t rw[F];
+    bool ok[F];
+    // rows past the end load row rows - 1 and are dropped
+    auto load_pass = [&](int64_t c0) {
+#pragma unroll
+        for (int k = 0; k < F; ++k)
+        {
+            const int64_t c = c0 + k * kDirWaves;
+            const int64_t i = c * kChunkRows + lane;
+            ok[k] = c < cend && i < rows;
+            rw[k] = ok[k] ? i : rows - 1;
+            row[k].load(rw[k], rows, dia, dv, mask, v);
+        }
+    };
+    int64_t c0 = cbegin + threadIdx.x / kWave;
+    load_pass(c0);
+    double alpha_d = 0.0, beta_d = 0.0;
+    if (cg_decision<S, PRE>(st, part_in, par, count, alpha_d, beta_d) != kRunning) return;
+    const S a = static_cast<S>(alpha_d), b = static_cast<S>(beta_d);
+    double rr = 0.0, ru = 0.0, wu = 0.0, us = 0.0;
+    while (c0 < cend)
+    {
+#pragma unroll
+        for (int k = 0; k < F; ++k)
+        {
+            if (!ok[k]) continue;
+            const CgFusedRow<S, ND, PRE>& o = row[k];
+            const int64_t i = rw[k];
+            const S acc = o.product(a, b, dia);
+            const S pn = fma_s(b, o.p, o.u);
+            const S sn = fma_s(b, o.s, o.w);
+            const S rn = fma_s(-a, sn, o.r);
+            const S un = PRE == kPreJacobi ? o.iv * rn : rn;
+            v.p[i] = pn;
+            v.x[i] = fma_s(a, pn, o.x);
+            v.s_out[i] = sn;
+            v.r_out[i] = rn;
+            if (PRE == kPreJacobi) v.u_out[i] = un;
+            v.w_out[i] = acc;
+            const double rv = static_cast<double>(rn);
+            const double uv = static_cast<double>(un);
             rr = __builtin_fma(rv, rv, rr);
             if (PRE == kPreJacobi) ru = __builtin_fma(rv, uv, ru);
             wu = __builtin_fma(static_cast<double>(acc), uv, wu);
+            us = __builtin_fma(uv, static_cast<double>(sn), us);
         }
+        c0 += F * kDirWaves;
+        if (c0 < cend) load_pass(c0);
     }
-    double part[3] = {rr, PRE == kPreJacobi ? ru : rr, wu};
-    publish_partials<3>(part, partials + kPartialsRes);
+    double part[4] = {rr, PRE == kPreJacobi ? ru : rr, wu, us};
+    publish_partials<4>(part, part_out);
 }
 
 // The solver state into the host's fine-grained status slot, 8 bytes per lane.
@@ -1030,8 +1167,10 @@ struct Problem
     bool fuse_step;          // factored: the step folded into G r and G^T t
     int fuse_step_env;       // PLAMATRIX_AMD_FUSE_STEP: 0 off, 1 on, -1 unset
     bool single_reduction;   // CG / Jacobi-PCG as the two-kernel Chronopoulos-Gear iteration
+    bool cg_fused;           // ... as one kernel per iteration on the diagonal layout
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
+    void *r2, *z2, *q2, *s2; // second buffers of r, u, w, s for the one-kernel iteration (or null)
     SolverState* st;
     double* partials;
 };
@@ -1093,21 +1232,56 @@ struct Calls
             out.push_back(make_call(k, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<const S*>(pb.r),
                                     static_cast<const S*>(pb.inv), static_cast<S*>(pb.z), static_cast<S*>(pb.p1),
                                     static_cast<const SolverState*>(pb.st)));
-            cg_spmv(pb, out);
+            // the initialisation acts as iteration -1: w = A u into the first
+            // buffers, partials into region 1
+            cg_spmv(pb, cg_vectors(pb, 1), 1, out);
             return;
         }
         if (pb.pre == kPreFactored) factored_apply(pb, out);
         else out.push_back(update_any<kModeInit>(pb, pb.p0, 0));
     }
 
-    // Single-reduction iteration: w = A u and the partials (grid_units
+    // Vectors of single-reduction iteration t (parity par): u = z (Jacobi) or
+    // r, w = q, p = p0, s = p1.  The one-kernel iteration reads them from the
+    // first buffers and writes the second ones at even t, the other way round
+    // at odd t; every chunk of iterations ends on an odd t, so between chunks
+    // the vectors sit in the first buffers, where the three-kernel form finds
+    // them.  The two-kernel iteration works in place.
+    static CgVectors<S> cg_vectors(const Problem& pb, int par)
+    {
+        const bool jac = pb.pre == kPreJacobi;
+        auto* r1 = static_cast<S*>(pb.r);
+        auto* u1 = jac ? static_cast<S*>(pb.z) : r1;
+        auto* w1 = static_cast<S*>(pb.q);
+        auto* s1 = static_cast<S*>(pb.p1);
+        auto* r2 = pb.cg_fused ? static_cast<S*>(pb.r2) : r1;
+        auto* u2 = pb.cg_fused ? (jac ? static_cast<S*>(pb.z2) : r2) : u1;
+        auto* w2 = pb.cg_fused ? static_cast<S*>(pb.q2) : w1;
+        auto* s2 = pb.cg_fused ? static_cast<S*>(pb.s2) : s1;
+        CgVectors<S> v{};
+        v.x = static_cast<S*>(pb.x);
+        v.p = static_cast<S*>(pb.p0);
+        v.inv = static_cast<const S*>(pb.inv);
+        const bool even = par == 0;
+        v.r_in = even ? r1 : r2;
+        v.u_in = even ? u1 : u2;
+        v.w_in = even ? w1 : w2;
+        v.s_in = even ? s1 : s2;
+        v.r_out = even ? r2 : r1;
+        v.u_out = even ? u2 : u1;
+        v.w_out = even ? w2 : w1;
+        v.s_out = even ? s2 : s1;
+        return v;
+    }
+
+    // w = A u of the new u and the partials r.r, (r.u,) w.u, u.s (grid_units
     // workgroups, as many as the three-kernel update publishes, so the solve
-    // can switch forms between chunks).  Vectors: u = z (Jacobi) or r, w = q,
-    // p = p0, s = p1.
-    static void cg_spmv(const Problem& pb, std::vector<KernelCall>& out)
+    // can switch forms between chunks) into region par.
+    static void cg_spmv(const Problem& pb, const CgVectors<S>& v, int par, std::vector<KernelCall>& out)
     {
         const bool jac = pb.pre == kPreJacobi;
         const int nd = pb.dia.args.ndiag;
+        double* part_out = cg_region(pb.partials, par);
         if (pb.dia.active)
         {
             const auto k = jac ? (nd == 7 ? k_cg_spmv_dia<S, 7, kPreJacobi> : nd == 5 ? k_cg_spmv_dia<S, 5, kPreJacobi>
@@ -1115,47 +1289,52 @@ struct Calls
                                : (nd == 7 ? k_cg_spmv_dia<S, 7, kPreNone> : nd == 5 ? k_cg_spmv_dia<S, 5, kPreNone>
                                                                                 : k_cg_spmv_dia<S, 0, kPreNone>);
             out.push_back(make_call(k, dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows, pb.dia.args,
-                                    static_cast<const S*>(pb.dia.values), pb.dia.mask, static_cast<const S*>(pb.r),
-                                    static_cast<const S*>(pb.z), static_cast<S*>(pb.q),
-                                    static_cast<const SolverState*>(pb.st), pb.partials));
+                                    static_cast<const S*>(pb.dia.values), pb.dia.mask, v,
+                                    static_cast<const SolverState*>(pb.st), part_out));
             return;
         }
         const auto k = jac ? k_cg_spmv<S, I, kPreJacobi> : k_cg_spmv<S, I, kPreNone>;
         out.push_back(make_call(k, dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows, ro(pb.a), ci(pb.a),
-                                static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.r),
-                                static_cast<const S*>(pb.z), static_cast<S*>(pb.q),
-                                static_cast<const SolverState*>(pb.st), pb.partials));
+                                static_cast<const S*>(pb.a->values), v, static_cast<const SolverState*>(pb.st),
+                                part_out));
     }
 
     static void cg_iteration(const Problem& pb, int64_t t, bool refresh, std::vector<KernelCall>& out)
     {
         const int par = static_cast<int>(t % 2);
         const bool jac = pb.pre == kPreJacobi;
+        const CgVectors<S> v = cg_vectors(pb, par);
+        const double* part_in = cg_region(pb.partials, 1 - par);
+        double* part_out = cg_region(pb.partials, par);
+        if (pb.cg_fused && !refresh)
+        {
+            const int nd = pb.dia.args.ndiag;
+            const auto k = jac ? (nd == 7   ? k_cg_fused_dia<S, 7, kPreJacobi>
+                                  : nd == 5 ? k_cg_fused_dia<S, 5, kPreJacobi>
+                                            : k_cg_fused_dia<S, 0, kPreJacobi>)
+                               : (nd == 7   ? k_cg_fused_dia<S, 7, kPreNone>
+                                  : nd == 5 ? k_cg_fused_dia<S, 5, kPreNone>
+                                            : k_cg_fused_dia<S, 0, kPreNone>);
+            out.push_back(make_call(k, dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows, pb.dia.args,
+                                    static_cast<const S*>(pb.dia.values), pb.dia.mask, v, pb.st, part_in, part_out,
+                                    par, pb.grid_units));
+            return;
+        }
         const auto step = refresh ? (jac ? k_cg_step<S, kPreJacobi, false> : k_cg_step<S, kPreNone, false>)
                                   : (jac ? k_cg_step<S, kPreJacobi, true> : k_cg_step<S, kPreNone, true>);
-        out.push_back(make_call(step, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<S*>(pb.x),
-                                static_cast<S*>(pb.r), static_cast<S*>(pb.z), static_cast<S*>(pb.p0),
-                                static_cast<S*>(pb.p1), static_cast<const S*>(pb.q), static_cast<const S*>(pb.inv),
-                                pb.st, pb.partials, par, pb.grid_units));
+        out.push_back(make_call(step, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, v, pb.st, part_in, par,
+                                pb.grid_units));
         if (refresh)
         {
             out.push_back(make_call(k_residual<S, I, false, false>, dim3(full_grid(pb.a->rows)), dim3(kBlock),
                                     pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
-                                    static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x),
-                                    static_cast<S*>(pb.r), static_cast<S*>(nullptr), static_cast<S*>(nullptr),
-                                    pb.st));
-            cg_resync(pb, par, out);
+                                    static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x), v.r_out,
+                                    static_cast<S*>(nullptr), static_cast<S*>(nullptr), pb.st));
+            if (jac)
+                out.push_back(make_call(k_cg_precondition<S>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, v,
+                                        static_cast<const SolverState*>(pb.st)));
         }
-        cg_spmv(pb, out);
-    }
-
-    // u = M^-1 r and the partial u.s after a residual refresh.
-    static void cg_resync(const Problem& pb, int par, std::vector<KernelCall>& out)
-    {
-        const auto k = pb.pre == kPreJacobi ? k_cg_resync<S, kPreJacobi> : k_cg_resync<S, kPreNone>;
-        out.push_back(make_call(k, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<const S*>(pb.r),
-                                static_cast<const S*>(pb.inv), static_cast<const S*>(pb.p1), static_cast<S*>(pb.z),
-                                static_cast<const SolverState*>(pb.st), pb.partials, par));
+        cg_spmv(pb, v, par, out);
     }
 
     // z = G^T (G r) and the r.r, r.z partials (grid_rows workgroups).  With
@@ -1350,6 +1529,8 @@ bool single_reduction_wanted(const Problem& pb)
     return (pb.pre == kPreNone || pb.pre == kPreJacobi) && !pb.fuse_direction;
 }
 
+bool dia_wanted(const pm_csr* a);
+
 // Reserve workspace storage and fill the workspace part of a Problem.
 int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
 {
@@ -1357,9 +1538,12 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     PM_TRY(ws->ensure_events());
     const int64_t n = pb.a->rows;
     const size_t vec = ((elem_size(pb.a->dtype) * static_cast<size_t>(n > 0 ? n : 1)) + 255) & ~size_t(255);
-    PM_TRY(ws->vectors.reserve(vec * 6));
+    // a matrix the diagonal layout may take also gets the second buffers of
+    // the one-kernel single-reduction iteration
+    const bool second = dia_wanted(pb.a);
+    PM_TRY(ws->vectors.reserve(vec * (second ? 10 : 6)));
     if (n > ws->capacity_rows) ws->capacity_rows = n;
-    PM_TRY(ws->partials.reserve(sizeof(double) * kMaxPartials * 8));
+    PM_TRY(ws->partials.reserve(sizeof(double) * kPartialsDoubles));
     PM_TRY(ws->state.reserve(4096));
     char* base = ws->vectors.as<char>();
     pb.r = base;
@@ -1368,6 +1552,11 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     pb.p0 = base + 3 * vec;
     pb.p1 = base + 4 * vec;
     pb.inv = base + 5 * vec;
+    pb.r2 = second ? base + 6 * vec : nullptr;
+    pb.z2 = second ? base + 7 * vec : nullptr;
+    pb.q2 = second ? base + 8 * vec : nullptr;
+    pb.s2 = second ? base + 9 * vec : nullptr;
+    pb.cg_fused = false;
     pb.st = ws->state.as<SolverState>();
     pb.partials = ws->partials.as<double>();
     const int64_t units = pb.pre == kPreBlock ? n / pb.bsize : n;
@@ -1429,7 +1618,7 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
                   static_cast<long long>(pb.bsize),
                   (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0) |
-                      (pb.single_reduction ? 16 : 0),
+                      (pb.single_reduction ? 16 : 0) | (pb.cg_fused ? 32 : 0),
                   pb.dia.active ? 1 : 0, pb.dia.args.ndiag, dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask));
     std::string key(buf);
     append_matrix_key(key, pb.a);
@@ -1459,9 +1648,17 @@ struct Policy
     int refresh = 0;
 };
 
-Policy resolve_policy(const pm_solve_policy* p)
+// By default a chunk of iterations is replayed as a HIP graph below
+// kDirectLaunchRows rows, where the kernels are short enough for the launch
+// path to matter (65,536-row Poisson: 12.8 vs 13.5 us per iteration), and
+// launched directly above, where the graph's own completion and launch cost
+// more than the host's launches (2.1M-row diffusion: 21.6 vs 22.2 ms).
+constexpr int64_t kDirectLaunchRows = 262144;
+
+Policy resolve_policy(const pm_solve_policy* p, int64_t rows)
 {
     Policy out;
+    out.graph = rows < kDirectLaunchRows;
     if (p)
     {
         if (p->chunk_iterations > 0) out.chunk = p->chunk_iterations;
@@ -1510,7 +1707,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
                    pm_workspace* ws, hipStream_t stream, const pm_csr* gf = nullptr, const pm_csr* gtf = nullptr,
                    int tol_rule = 0)
 {
-    const Policy policy = resolve_policy(policy_in);
+    const Policy policy = resolve_policy(policy_in, a->rows);
     Problem pb{};
     pb.a = a;
     pb.rhs = rhs;
@@ -1530,7 +1727,11 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         const char* unit_env = std::getenv("PLAMATRIX_AMD_UNIT_DIAGONAL");
         const bool check_unit = pb.pre == kPreJacobi && (unit_env == nullptr || unit_env[0] != '0');
         bool unit = false;
-        PM_TRY(dia_build(ws->dia, a, &pb.dia, stream, check_unit ? &unit : nullptr));
+        // A symmetric bit for bit streams only its upper diagonals
+        // (PLAMATRIX_AMD_DIA_MIRROR=0 streams every diagonal)
+        const char* mirror_env = std::getenv("PLAMATRIX_AMD_DIA_MIRROR");
+        const bool mirror = mirror_env == nullptr || mirror_env[0] != '0';
+        PM_TRY(dia_build(ws->dia, a, &pb.dia, stream, check_unit ? &unit : nullptr, mirror));
         if (check_unit && unit) pb.pre = kPreNone;
     }
     if (pb.pre == kPreFactored && dia_wanted(gf) && dia_wanted(gtf))
@@ -1541,7 +1742,18 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     }
     pb.fuse_step = fuse_step_wanted(pb);
     pb.single_reduction = single_reduction_wanted(pb);
+    // the single-reduction kernels walk 64-row chunks, one per wavefront
+    // where the solver grid allows (65,536 rows: 256 workgroups instead of
+    // 64); the three-kernel form a restart hands over to uses the same grid
+    if (pb.single_reduction) pb.grid_units = solver_grid_chunks(a->rows);
+    {
+        // on the diagonal layout, one kernel per iteration
+        // (PLAMATRIX_AMD_CG_FUSED=0 keeps two)
+        const char* env = std::getenv("PLAMATRIX_AMD_CG_FUSED");
+        pb.cg_fused = pb.single_reduction && pb.dia.active && pb.r2 != nullptr && (env == nullptr || env[0] != '0');
+    }
     ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
+    ws->last_value_streams = pb.dia.active ? pb.dia.streams : 0;
     ws->last_preconditioner = pb.pre;
     if (a->rows == 0)
     {
@@ -1856,7 +2068,7 @@ int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options*
     const size_t r_bytes = (sizeof(double) * static_cast<size_t>(n) + 255) & ~size_t(255);
     const size_t f_bytes = (sizeof(float) * static_cast<size_t>(n) + 255) & ~size_t(255);
     PM_TRY(ws->refine.reserve(r_bytes + 2 * f_bytes + 4096));
-    PM_TRY(ws->partials.reserve(sizeof(double) * kMaxPartials * 8));
+    PM_TRY(ws->partials.reserve(sizeof(double) * kPartialsDoubles));
     char* base = ws->refine.as<char>();
     double* r64 = reinterpret_cast<double*>(base);
     float* rf = reinterpret_cast<float*>(base + r_bytes);

@@ -125,6 +125,7 @@ struct pm_workspace
     pm::DeviceBuffer dia_g;     // ... of the factored preconditioner's G
     pm::DeviceBuffer dia_gt;    // ... and of G^T
     int32_t last_diagonals = 0; // diagonals of the last solve's layout (0: CSR kernels)
+    int32_t last_value_streams = 0; // value arrays its row sums read (fewer than the diagonals when mirrored)
     int32_t last_preconditioner = 0; // preconditioner the last solve ran (pm::Pre)
     int64_t capacity_rows = 0;
 

@@ -58,6 +58,7 @@ def solve_both(dev, h, dtype, monkeypatch, rel, refresh=50):
                        max_iterations=20000)
         torch.cuda.synchronize()
         grow[flag] = ws.capacity_bytes - before
+        grow["diagonals" + flag] = ws.solver_diagonals
         out[flag] = (rep, host(x).copy())
     return out, grow
 
@@ -104,7 +105,7 @@ def test_unsorted_rows_fall_back(dev, monkeypatch):
         v[s:e] = v[s:e][::-1]
     rev = generators.CSRHost(h.rows, h.cols, ro, ci, v)
     out, grow = solve_both(dev, rev, np.float64, monkeypatch, 1e-10)
-    assert grow["1"] - grow["0"] < h.rows * 8  # no diagonal values
+    assert grow["diagonals1"] == 0  # the CSR kernels ran
     assert out["1"][0].converged and np.array_equal(out["0"][1], out["1"][1])
 
 
@@ -199,3 +200,103 @@ def test_unit_diagonal_check_is_exact(dev, monkeypatch, change):
     assert ref == got
     if change == "huge_off_diagonal":
         assert ref[0] == "error" and "diagonal" in ref[2]
+
+
+def layout_outcome(dev, h, monkeypatch, dia, mirror, rel=1e-12, iterations=40, unit="1"):
+    # a fixed number of iterations (the tolerance is out of reach), so
+    # matrices that are not symmetric compare too
+    monkeypatch.setenv("PLAMATRIX_AMD_DIA", dia)
+    monkeypatch.setenv("PLAMATRIX_AMD_DIA_MIRROR", mirror)
+    monkeypatch.setenv("PLAMATRIX_AMD_UNIT_DIAGONAL", unit)
+    a = capi.to_device_csr(h, compact=True)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=41).astype(h.values.dtype)).to(dev)
+    ws = capi.Workspace()
+    x = torch.zeros_like(b)
+    rep = capi.pcg(a, b, x, ws, capi.policy(refresh_interval=50), relative_tolerance=rel, max_iterations=iterations)
+    torch.cuda.synchronize()
+    return (rep.iterations, rep.final_residual, host(x).tobytes()), ws.solver_diagonals, ws.solver_value_streams
+
+
+@pytest.mark.parametrize("kind,diagonals,streams", [
+    ("diffusion3d", 7, 3),   # unit diagonal: 3 upper diagonals only
+    ("laplacian2d", 5, 3),   # 2 upper diagonals and the main diagonal (4)
+    ("nine_point", 9, 5),    # 4 upper diagonals and the main diagonal (8)
+])
+def test_mirrored_layout_is_bitwise(dev, monkeypatch, kind, diagonals, streams):
+    # a symmetric matrix reads its lower diagonals from their upper mirrors
+    # and an all-ones main diagonal from no array; the products keep their bits
+    if kind == "diffusion3d":
+        h = generators.diffusion3d_csr(48, seed=43, dtype=np.float32)
+    elif kind == "laplacian2d":
+        h = generators.laplacian2d_csr(400, dtype=np.float32)
+    else:
+        h = from_scipy(nine_point(320), np.float64)
+    csr, d0, s0 = layout_outcome(dev, h, monkeypatch, "0", "1")
+    full, d1, s1 = layout_outcome(dev, h, monkeypatch, "1", "0")
+    mirrored, d2, s2 = layout_outcome(dev, h, monkeypatch, "1", "1")
+    assert (d0, s0) == (0, 0)
+    assert (d1, s1) == (diagonals, diagonals)
+    assert (d2, s2) == (diagonals, streams)
+    assert csr == full == mirrored
+
+
+@pytest.mark.parametrize("change", ["one_ulp", "missing_mirror", "diagonal_not_one"])
+def test_mirror_check_is_exact(dev, monkeypatch, change):
+    # one entry breaks the property: an off-diagonal value one ulp away from
+    # its mirror, a lower entry whose mirror is absent, or a main-diagonal
+    # entry that is not exactly 1; the layout must then read that array
+    h = generators.diffusion3d_csr(48, seed=45, dtype=np.float32)
+    ro, ci, v = h.row_offsets.copy(), h.col_indices.copy(), h.values.copy()
+    row = h.rows // 2 + 7
+    s, e = ro[row], ro[row + 1]
+    cols = ci[s:e]
+    if change == "one_ulp":
+        k = s + int(np.nonzero(cols == row - 1)[0][0])
+        v[k] = np.nextafter(v[k], np.float32(0))
+        expect = 6  # every diagonal but the main one
+    elif change == "missing_mirror":
+        k = s + int(np.nonzero(cols == row - 1)[0][0])
+        ci = np.delete(ci, k)
+        v = np.delete(v, k)
+        ro[row + 1:] -= 1
+        expect = 6
+    else:
+        k = s + int(np.nonzero(cols == row)[0][0])
+        v[k] = np.float32(1.0) + np.float32(2.0 ** -23)
+        expect = 4  # the mirrors hold, the main diagonal is read
+    m = generators.CSRHost(h.rows, h.cols, ro, ci, v)
+    # unit="0": the Jacobi kernels run throughout, whatever the diagonal holds
+    csr, _, _ = layout_outcome(dev, m, monkeypatch, "0", "1", unit="0")
+    got, d, streams = layout_outcome(dev, m, monkeypatch, "1", "1", unit="0")
+    assert d == 7 and streams == expect
+    assert csr == got
+
+
+@pytest.mark.parametrize("kind", ["diffusion3d_unit", "laplacian2d_jacobi", "nine_point_fp64"])
+def test_one_kernel_iteration_is_bitwise(dev, monkeypatch, kind):
+    # on the diagonal layout the single-reduction iteration runs as one kernel
+    # (neighbours' new u recomputed from the old r, w, s of those rows, the
+    # vectors alternating between two buffers); it must keep the two-kernel
+    # form's bits, residual refreshes every 50 iterations included
+    if kind == "diffusion3d_unit":
+        h = generators.diffusion3d_csr(48, seed=47, dtype=np.float32)  # Jacobi runs as z = r
+        rel = 1e-6
+    elif kind == "laplacian2d_jacobi":
+        h = generators.laplacian2d_csr(400, dtype=np.float32)  # diagonal 4: the Jacobi kernels
+        rel = 1e-5
+    else:
+        h = from_scipy(nine_point(320), np.float64)
+        rel = 1e-10
+    a = capi.to_device_csr(h, compact=True)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=48).astype(h.values.dtype)).to(dev)
+    out = {}
+    for fused in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_CG_FUSED", fused)
+        ws = capi.Workspace()
+        x = torch.zeros_like(b)
+        rep = capi.pcg(a, b, x, ws, capi.policy(refresh_interval=50), relative_tolerance=rel, max_iterations=20000)
+        torch.cuda.synchronize()
+        assert ws.solver_diagonals > 0
+        out[fused] = (rep.converged, rep.iterations, rep.final_residual, host(x).tobytes())
+    assert out["0"][0] and out["0"][1] > 50
+    assert out["0"] == out["1"]
