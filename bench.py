@@ -40,16 +40,23 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4) -> 
 
 
 def pcg_iteration_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int = 4, diagonals: int = 0,
-                        unit_diagonal: bool = False) -> int:
-    """Algorithmic bytes of one Jacobi-PCG iteration (three kernels): the
-    direction kernel reads z and p and writes p; the SpMV streams values,
-    columns and offsets (or, on the diagonal layout, `diagonals` value slots
-    and a 2-byte mask per row), reads p once and writes q; the vector update
-    reads x, p, r, q, M^-1 and writes x, r, z.  With a unit diagonal (every
-    Jacobi inverse exactly 1) z is r: no M^-1 read and no z write."""
-    vectors = 11 if unit_diagonal else 13
+                        unit_diagonal: bool = False, form: int = 3, value_streams: int = 0) -> int:
+    """Algorithmic bytes of one CG / Jacobi-PCG iteration.  The matrix: CSR
+    values, columns and offsets, or on the diagonal layout `value_streams`
+    value arrays (default: one per diagonal) and a 2-byte mask per row.  The
+    vectors, per row, by form (kernels per iteration):
+    3  direction reads z, p, writes p; SpMV reads p, writes q; update reads
+       x, p, r, q, M^-1, writes x, r, z                        (13 vectors)
+    2  step reads r, u, p, s, w, x, M^-1, writes p, s, x, r, u; SpMV reads r,
+       u, s, writes w                                          (16 vectors)
+    1  one kernel reads r, u, w, s, p, x, M^-1, writes p, x, s, r, u, w
+                                                               (13 vectors)
+    With a unit diagonal (every Jacobi inverse exactly 1) u is r: no M^-1
+    read and no u write or second read (11, 12, 10 vectors).  Gathers of
+    neighbouring rows are counted once, with the row's own read."""
+    vectors = {3: (13, 11), 2: (16, 12), 1: (13, 10)}[form][1 if unit_diagonal else 0]
     if diagonals:
-        return diagonals * n * value_bytes + 2 * n + vectors * n * value_bytes
+        return (value_streams or diagonals) * n * value_bytes + 2 * n + vectors * n * value_bytes
     return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + vectors * n * value_bytes
 
 
@@ -218,7 +225,9 @@ def run_diffusion(args, torch, capi, generators):
     # the matrix is scaled to a unit diagonal: its Jacobi solve runs as the
     # identical unpreconditioned iteration (no M^-1 read, no z write)
     unit_diagonal = ws.solver_preconditioner == "none"
-    it_bytes = pcg_iteration_bytes(n, nnz, diagonals=diagonals, unit_diagonal=unit_diagonal)
+    form, streams = ws.solver_form, ws.solver_value_streams
+    it_bytes = pcg_iteration_bytes(n, nnz, diagonals=diagonals, unit_diagonal=unit_diagonal, form=form,
+                                   value_streams=streams)
     it_ms = ms_per_solve / max(iterations, 1)
 
     # ---- learned preconditioner: G inferred on this matrix inside the timed solve ----
@@ -330,7 +339,9 @@ def run_diffusion(args, torch, capi, generators):
         "learned": learned,
         "solve": {"preconditioner": "jacobi", "jacobi_kernels": "unit diagonal: z = r" if unit_diagonal else "z = M^-1 r", "ms": round(ms_per_solve, 3), "converged": rep.converged, "outer": rep.outer_iterations, "inner_iterations": iterations,
                   "true_relres": relres, "ms_per_iteration": round(it_ms, 4),
-                  "matrix_layout": f"diagonal ({diagonals} diagonals)" if diagonals else "csr",
+                  "matrix_layout": (f"diagonal ({diagonals} diagonals, {streams} value arrays read)" if diagonals
+                                    else "csr"),
+                  "kernels_per_iteration": form,
                   "iteration_bytes": it_bytes, "iteration_gbs": round(it_bytes / (it_ms * 1e-3) / 1e9, 1),
                   "to_1e8": {"ms": round(ms_1e8, 2), "outer": rep8.outer_iterations,
                              "inner_iterations": rep8.inner_iterations, "relres": rep8.final_relres}},
@@ -415,8 +426,10 @@ def run_poisson(args, torch, capi, generators):
     for _ in range(args.warmup):
         solve()
     solve_ms, rep = _median_ms(solve, max(args.steps, 3))
+    form = ws.solver_form
     relres = capi.true_relres(a, b, x, ws)
-    it_bytes = pcg_iteration_bytes(n, nnz, value_bytes=8, index_bytes=4 if a.row_offsets32 is not None else 8)
+    it_bytes = pcg_iteration_bytes(n, nnz, value_bytes=8, index_bytes=4 if a.row_offsets32 is not None else 8,
+                                   form=form)
     it_ms = solve_ms / max(rep.iterations, 1)
     cpu = None
     if not args.no_cpu:
@@ -438,9 +451,10 @@ def run_poisson(args, torch, capi, generators):
                                   "to relres 1e-8, max 2000", "rows": n, "nnz": nnz, "triplets": t_count},
                      {"bound": "hbm", "achieved": round(it_bytes / (it_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": round(it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                      "traffic": None, "kernel": "PCG iteration (k_direction + k_spmv_den + k_update)",
+                      "traffic": None, "kernel": {1: "PCG iteration (k_cg_fused_dia)", 2: "PCG iteration (k_cg_step + k_cg_spmv)",
+                                 3: "PCG iteration (k_direction + k_spmv_den + k_update)"}[form],
                       "bytes_per_launch": it_bytes, "ms_per_launch": round(it_ms, 5),
-                      "note": "latency-bound: 5.2 MB per iteration"},
+                      "note": f"latency-bound: {it_bytes / 1e6:.1f} MB per iteration"},
                      cpu)
     line["solve"] = {"converged": rep.converged, "iterations": rep.iterations, "true_relres": relres}
     line["coo_to_csr"] = {"ms_steady": round(conv_ms, 4), "ms_synchronous": round(sync_ms, 4),

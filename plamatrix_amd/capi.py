@@ -102,7 +102,7 @@ class pm_schur_report(ctypes.Structure):
 # Every symbol declared in include/plamatrix_amd.h (checked by the CPU tests).
 EXPORTS = [
     "pm_last_error", "pm_version", "pm_workspace_create", "pm_workspace_destroy", "pm_workspace_close",
-    "pm_workspace_capacity_bytes", "pm_workspace_capacity_rows", "pm_workspace_solver_diagonals", "pm_workspace_solver_value_streams", "pm_workspace_solver_preconditioner", "pm_csr_validate", "pm_spmm", "pm_csr_compact_indices",
+    "pm_workspace_capacity_bytes", "pm_workspace_capacity_rows", "pm_workspace_solver_diagonals", "pm_workspace_solver_value_streams", "pm_workspace_solver_form", "pm_workspace_solver_preconditioner", "pm_csr_validate", "pm_spmm", "pm_csr_compact_indices",
     "pm_coo_to_csr_prepare", "pm_coo_to_csr_nnz", "pm_coo_to_csr_finish", "pm_coo_to_csr_async",
     "pm_workspace_check_status", "pm_spmv", "pm_spmv_lanes", "pm_autotune_spmv", "pm_tuned_lanes", "pm_tune_record", "pm_true_relres", "pm_cg", "pm_pcg",
     "pm_block_pcg", "pm_solve_fixed_async", "pm_fixed_finalize", "pm_fixed_state_destroy", "pm_pcg_refine",
@@ -140,6 +140,8 @@ def lib():
         _lib.pm_workspace_solver_diagonals.argtypes = [_p]
         _lib.pm_workspace_solver_value_streams.restype = _i32
         _lib.pm_workspace_solver_value_streams.argtypes = [_p]
+        _lib.pm_workspace_solver_form.restype = _i32
+        _lib.pm_workspace_solver_form.argtypes = [_p]
         _lib.pm_workspace_solver_preconditioner.restype = _i32
         _lib.pm_workspace_solver_preconditioner.argtypes = [_p]
         _lib.pm_workspace_destroy.argtypes = [_p]
@@ -198,6 +200,11 @@ class Workspace:
         """Value arrays the last solve's diagonal-layout row sums read per row
         (fewer than the diagonals when A is mirrored; 0: CSR)."""
         return int(lib().pm_workspace_solver_value_streams(self.handle))
+
+    @property
+    def solver_form(self) -> int:
+        """Kernels per iteration of the form the last solve started in (1, 2, 3)."""
+        return int(lib().pm_workspace_solver_form(self.handle))
 
     @property
     def solver_preconditioner(self) -> str:

@@ -1754,6 +1754,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     }
     ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
     ws->last_value_streams = pb.dia.active ? pb.dia.streams : 0;
+    ws->last_form = pb.cg_fused ? 1 : pb.single_reduction ? 2 : 3;
     ws->last_preconditioner = pb.pre;
     if (a->rows == 0)
     {

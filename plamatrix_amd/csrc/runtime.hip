@@ -256,6 +256,7 @@ int64_t pm_workspace_capacity_rows(const pm_workspace* ws) { return ws ? ws->cap
 
 int32_t pm_workspace_solver_diagonals(const pm_workspace* ws) { return ws ? ws->last_diagonals : 0; }
 int32_t pm_workspace_solver_value_streams(const pm_workspace* ws) { return ws ? ws->last_value_streams : 0; }
+int32_t pm_workspace_solver_form(const pm_workspace* ws) { return ws ? ws->last_form : 0; }
 int32_t pm_workspace_solver_preconditioner(const pm_workspace* ws) { return ws ? ws->last_preconditioner : 0; }
 
 } // extern "C"

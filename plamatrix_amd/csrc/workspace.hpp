@@ -126,6 +126,7 @@ struct pm_workspace
     pm::DeviceBuffer dia_gt;    // ... and of G^T
     int32_t last_diagonals = 0; // diagonals of the last solve's layout (0: CSR kernels)
     int32_t last_value_streams = 0; // value arrays its row sums read (fewer than the diagonals when mirrored)
+    int32_t last_form = 0;      // kernels per iteration of the last solve's form (1, 2 or 3)
     int32_t last_preconditioner = 0; // preconditioner the last solve ran (pm::Pre)
     int64_t capacity_rows = 0;
 
