@@ -332,6 +332,26 @@ int pm_batch_pcg_refine_factored(const pm_csr* a, int64_t systems, const int64_t
                                  int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
                                  pm_workspace* ws, hipStream_t stream);
 
+/* Batched block-Jacobi PCG: blockPcg (iterative_solver.h:130-141) per system
+ * in one batch.  block_sizes (host, one per system, 1 to 8) gives each
+ * system's block size; its rows [row_begin + k bs, row_begin + (k + 1) bs)
+ * form diagonal block k, whose inverse the device computes from A (FP64
+ * Gauss-Jordan, rounded to A's dtype) at every solve; block size 1 is
+ * Jacobi.  A system's size must be a multiple of its block size
+ * (PM_INVALID_ARGUMENT otherwise).  A block with a pivot that is not finite
+ * or not above 16 eps |block|max stops its system as a bad diagonal (status
+ * and bad_row as pm_batch_pcg's Jacobi rules).  Same stop rules, reports and
+ * errors as pm_batch_pcg; options->use_jacobi is ignored. */
+int pm_batch_block_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                       const int32_t* block_sizes, const void* rhs, void* x, const pm_solver_options* options,
+                       const pm_solve_policy* policy, pm_batch_report* reports, pm_workspace* ws, hipStream_t stream);
+/* pm_batch_pcg_refine with per-system block-Jacobi in the FP32 inner solves. */
+int pm_batch_block_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                              const int32_t* block_sizes, const double* rhs, double* x,
+                              const pm_solver_options* inner, double target_relres, int32_t max_outer,
+                              const pm_solve_policy* policy, pm_batch_refine_report* reports, pm_workspace* ws,
+                              hipStream_t stream);
+
 /* ---- Factored preconditioner M^-1 = G^T G ----------------------------------------
  * PCG with z = G^T (G r) (no reference counterpart).  g is lower-triangular
  * CSR, gt its transpose; both need 32-bit index mirrors and a's dtype.  Same

@@ -114,6 +114,7 @@ EXPORTS = [
     "pm_fsai_graph_pool_release", "pm_fsai_graph_pool_bytes",
     "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step", "pm_dense_cholesky_solve",
     "pm_batch_pcg_factored", "pm_batch_pcg_refine_factored", "pm_fsai_batch_select",
+    "pm_batch_block_pcg", "pm_batch_block_pcg_refine",
     "pm_spmm_noalloc",
 ]
 
@@ -702,6 +703,46 @@ def batch_pcg_refine(a: "DeviceCSR", row_begin, row_end, b64, x64, ws: Workspace
     check(lib().pm_batch_pcg_refine(ctypes.byref(d), _i64(systems), row_begin.ctypes.data_as(_p),
                                     row_end.ctypes.data_as(_p), _ptr(b64), _ptr(x64), ctypes.byref(o), _f64(target),
                                     _i32(max_outer), ctypes.byref(p), reps, ws.handle, current_stream()))
+    return [BatchRefineReport(bool(r.converged), r.outer_iterations, r.inner_iterations, r.initial_relres,
+                              r.final_relres, r.last_inner_status) for r in reps[:systems]]
+
+
+def batch_block_pcg(a: "DeviceCSR", row_begin, row_end, block_sizes, b, x, ws: Workspace, pol=None, **opts):
+    """Batched block-Jacobi PCG: per-system block sizes (1 = Jacobi)."""
+    import numpy as np
+
+    d = a.descriptor()
+    o = options(**opts)
+    p = pol if pol is not None else policy()
+    row_begin = np.ascontiguousarray(row_begin, dtype=np.int64)
+    row_end = np.ascontiguousarray(row_end, dtype=np.int64)
+    bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+    systems = len(row_begin)
+    reps = (pm_batch_report * max(systems, 1))()
+    check(lib().pm_batch_block_pcg(ctypes.byref(d), _i64(systems), row_begin.ctypes.data_as(_p),
+                                   row_end.ctypes.data_as(_p), bs.ctypes.data_as(_p), _ptr(b), _ptr(x),
+                                   ctypes.byref(o), ctypes.byref(p), reps, ws.handle, current_stream()))
+    return [BatchReport(bool(r.converged), r.iterations, r.initial_residual, r.final_residual, r.status, r.bad_row)
+            for r in reps[:systems]]
+
+
+def batch_block_pcg_refine(a: "DeviceCSR", row_begin, row_end, block_sizes, b64, x64, ws: Workspace, target=1e-6,
+                           max_outer=10, pol=None, **inner):
+    """Batched FP64 refinement with per-system block-Jacobi inner solves."""
+    import numpy as np
+
+    d = a.descriptor()
+    o = options(**inner)
+    p = pol if pol is not None else policy()
+    row_begin = np.ascontiguousarray(row_begin, dtype=np.int64)
+    row_end = np.ascontiguousarray(row_end, dtype=np.int64)
+    bs = np.ascontiguousarray(block_sizes, dtype=np.int32)
+    systems = len(row_begin)
+    reps = (pm_batch_refine_report * max(systems, 1))()
+    check(lib().pm_batch_block_pcg_refine(ctypes.byref(d), _i64(systems), row_begin.ctypes.data_as(_p),
+                                          row_end.ctypes.data_as(_p), bs.ctypes.data_as(_p), _ptr(b64), _ptr(x64),
+                                          ctypes.byref(o), _f64(target), _i32(max_outer), ctypes.byref(p), reps,
+                                          ws.handle, current_stream()))
     return [BatchRefineReport(bool(r.converged), r.outer_iterations, r.inner_iterations, r.initial_relres,
                               r.final_relres, r.last_inner_status) for r in reps[:systems]]
 

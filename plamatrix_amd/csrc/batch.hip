@@ -222,6 +222,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_direction(int64_t chunks, cons
 }
 
 // x += alpha p; STEP: r -= alpha q, z = M^-1 r, partial r.r and r.z.
+// Systems with a block size above 1 (sys_bs, null: none) leave z and r.z to
+// k_batch_block_apply, which runs next.
 template <typename S, bool JACOBI, bool STEP>
 __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const int32_t* __restrict__ chunk_sys,
                                                          const int64_t* __restrict__ sys_end, S* __restrict__ x,
@@ -230,7 +232,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const i
                                                          const S* __restrict__ inv, const BatchState* st,
                                                          double* __restrict__ prr, double* __restrict__ prz,
                                                          const int64_t* __restrict__ list,
-                                                         const unsigned long long* __restrict__ list_count)
+                                                         const unsigned long long* __restrict__ list_count,
+                                                         const int32_t* __restrict__ sys_bs)
 {
     const int lane = threadIdx.x % kWave;
     PM_FOR_LISTED(li, list_count, kBatchWaves)
@@ -243,6 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const i
             continue;
         }
         const S alpha = static_cast<S>(st[s].alpha);
+        const bool blocks = sys_bs != nullptr && sys_bs[s] > 1;
         const int64_t row = c * kChunkRows + lane;
         double rr = 0.0, rz = 0.0;
         if (row < sys_end[s])
@@ -252,10 +256,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_update(int64_t chunks, const i
             {
                 const S ri = fma_s(-alpha, q[row], r[row]);
                 r[row] = ri;
-                const S zi = JACOBI ? inv[row] * ri : ri;
-                z[row] = zi;
                 rr = static_cast<double>(ri) * static_cast<double>(ri);
-                rz = static_cast<double>(ri) * static_cast<double>(zi);
+                if (!blocks)
+                {
+                    const S zi = JACOBI ? inv[row] * ri : ri;
+                    z[row] = zi;
+                    rz = static_cast<double>(ri) * static_cast<double>(zi);
+                }
             }
         }
         if (STEP)
@@ -458,13 +465,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_list(int64_t chunks, const int
 // diagonal entry takes it as is, several are summed in CSR order.  Rows of a
 // rejected diagonal get NaN and the system records its first such row.
 // only (device, one byte per system; null: every system) limits it to the
-// systems flagged.
+// systems flagged; block-Jacobi systems (sys_bs above 1) are skipped.
 template <typename S, typename I>
 __global__ __launch_bounds__(kBlock) void k_batch_jacobi_inverse(int64_t chunks, const int32_t* __restrict__ chunk_sys,
                                                                  const int64_t* __restrict__ sys_end,
                                                                  const I* __restrict__ ro, const I* __restrict__ ci,
                                                                  const S* __restrict__ v, S* __restrict__ inv,
-                                                                 BatchState* st, const uint8_t* __restrict__ only)
+                                                                 BatchState* st, const uint8_t* __restrict__ only,
+                                                                 const int32_t* __restrict__ sys_bs)
 {
     constexpr int L = 8;
     const int lane = threadIdx.x % kWave;
@@ -472,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_jacobi_inverse(int64_t chunks,
     PM_FOR_CHUNKS(c, chunks, kBatchWaves)
     {
         const int32_t s = chunk_sys[c];
-        if (s < 0 || (only && !only[s])) continue;
+        if (s < 0 || (only && !only[s]) || (sys_bs && sys_bs[s] > 1)) continue;
         const int64_t end = sys_end[s];
         for (int pass = 0; pass < L; ++pass)
         {
@@ -518,6 +526,135 @@ __global__ __launch_bounds__(kBlock) void k_batch_jacobi_inverse(int64_t chunks,
                 if (!ok) atomicMin(&st[s].bad_row, static_cast<unsigned long long>(row));
             }
         }
+    }
+}
+
+// ---- block-Jacobi systems -------------------------------------------------------------
+//
+// A system with block size bs = sys_bs[s] > 1 is preconditioned by the
+// inverses of its bs x bs diagonal blocks (blockPcg's M^-1, iterative_solver.h
+// :130-141), computed here from A: rows [begin + k bs, begin + (k + 1) bs)
+// form block k.  Row i of the system keeps row (i - block start) of its
+// block's inverse at binv[i * bmax ...], so the apply reads one short
+// contiguous row per lane.
+
+constexpr int kMaxBatchBlock = 8;
+
+// One lane per block (the lane of its first row): the block gathered from its
+// rows' entries (duplicates summed in CSR order), inverted in FP64 by
+// Gauss-Jordan elimination without pivoting (an SPD block has positive
+// pivots), rounded to S.  A pivot that is not finite or not above 16 eps(S)
+// times the block's largest magnitude rejects the block: its rows get NaN and
+// the system records its first row, as a rejected Jacobi diagonal does.
+template <typename S, typename I>
+__global__ __launch_bounds__(kBlock) void k_batch_block_inverse(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                                const int64_t* __restrict__ sys_end,
+                                                                const int64_t* __restrict__ sys_chunk0,
+                                                                const int32_t* __restrict__ sys_bs,
+                                                                const I* __restrict__ ro, const I* __restrict__ ci,
+                                                                const S* __restrict__ v, S* __restrict__ binv,
+                                                                int bmax, BatchState* st)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        if (s < 0) continue;
+        const int bs = sys_bs[s];
+        const int64_t row = c * kChunkRows + lane;
+        const int64_t begin = sys_chunk0[s] * kChunkRows;
+        if (bs <= 1 || row >= sys_end[s] || (row - begin) % bs != 0) continue;
+        double m[kMaxBatchBlock][kMaxBatchBlock], w[kMaxBatchBlock][kMaxBatchBlock];
+        double scale = 0.0;
+        for (int i = 0; i < bs; ++i)
+        {
+            for (int j = 0; j < bs; ++j)
+            {
+                m[i][j] = 0.0;
+                w[i][j] = i == j ? 1.0 : 0.0;
+            }
+            for (int64_t k = static_cast<int64_t>(ro[row + i]); k < static_cast<int64_t>(ro[row + i + 1]); ++k)
+            {
+                const int64_t col = static_cast<int64_t>(ci[k]);
+                if (col >= row && col < row + bs) m[i][col - row] += static_cast<double>(v[k]);
+            }
+            for (int j = 0; j < bs; ++j) scale = fmax(scale, fabs(m[i][j]));
+        }
+        const double threshold = static_cast<double>(scalar_epsilon<S>()) * scale * 16.0;
+        bool ok = true;
+        for (int k = 0; k < bs && ok; ++k)
+        {
+            const double piv = m[k][k];
+            if (!(isfinite(piv) && piv > threshold))
+            {
+                ok = false;
+                break;
+            }
+            const double rp = 1.0 / piv;
+            for (int j = 0; j < bs; ++j)
+            {
+                m[k][j] *= rp;
+                w[k][j] *= rp;
+            }
+            for (int i = 0; i < bs; ++i)
+            {
+                if (i == k) continue;
+                const double f = m[i][k];
+                for (int j = 0; j < bs; ++j)
+                {
+                    m[i][j] = __builtin_fma(-f, m[k][j], m[i][j]);
+                    w[i][j] = __builtin_fma(-f, w[k][j], w[i][j]);
+                }
+            }
+        }
+        for (int i = 0; i < bs; ++i)
+            for (int j = 0; j < bs; ++j)
+            {
+                const bool fin = ok && isfinite(w[i][j]);
+                if (!fin) ok = false;
+                binv[(row + i) * bmax + j] = fin ? static_cast<S>(w[i][j]) : static_cast<S>(NAN);
+            }
+        if (!ok) atomicMin(&st[s].bad_row, static_cast<unsigned long long>(row));
+    }
+}
+
+// z = M^-1 r on the block systems' chunks: each row takes the dot of its
+// inverse row with its block's r (the block may reach into a neighbouring
+// chunk, whose r the preceding kernel has finished), and the chunk's partial
+// r.z replaces the one the update left.
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_block_apply(const int32_t* __restrict__ chunk_sys,
+                                                              const int64_t* __restrict__ sys_end,
+                                                              const int64_t* __restrict__ sys_chunk0,
+                                                              const int32_t* __restrict__ sys_bs,
+                                                              const S* __restrict__ r, S* __restrict__ z,
+                                                              const S* __restrict__ binv, int bmax,
+                                                              const BatchState* st, double* __restrict__ prz,
+                                                              const int64_t* __restrict__ list,
+                                                              const unsigned long long* __restrict__ list_count)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning) continue;
+        const int bs = sys_bs[s];
+        if (bs <= 1) continue;
+        const int64_t row = c * kChunkRows + lane;
+        const int64_t begin = sys_chunk0[s] * kChunkRows;
+        double rz = 0.0;
+        if (row < sys_end[s])
+        {
+            const int64_t b0 = row - (row - begin) % bs;
+            const S* wrow = binv + row * bmax;
+            S acc = S(0);
+            for (int j = 0; j < bs; ++j) acc = fma_s(wrow[j], r[b0 + j], acc);
+            z[row] = acc;
+            rz = static_cast<double>(r[row]) * static_cast<double>(acc);
+        }
+        rz = wave_sum(rz);
+        if (lane == 0) prz[c] = rz;
     }
 }
 
@@ -655,6 +792,11 @@ struct BatchProblem
     bool split;                       // direction update as its own kernel (SpMV gathers p once)
     int long_lanes;                   // lanes per row of chunks with long rows (8, 16, 32 or 64)
     bool fuse_step;                   // factored: the step folded into the factor products
+    // block-Jacobi: per-system block size (device; null: none), the largest
+    // one and the inverse rows (bmax values per row)
+    const int32_t* sys_bs;
+    int bmax;
+    void* binv;
 };
 
 template <typename S, typename I>
@@ -754,11 +896,27 @@ struct BatchCalls
         if (b.jacobi || (b.g && b.sys_jacobi))
             out.push_back(make_call(k_batch_jacobi_inverse<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
                                     b.chunk_sys, b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
-                                    static_cast<S*>(b.inv), b.st, b.g ? b.sys_jacobi : nullptr));
+                                    static_cast<S*>(b.inv), b.st, b.g ? b.sys_jacobi : nullptr, b.sys_bs));
+        if (b.sys_bs)
+            out.push_back(make_call(k_batch_block_inverse<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
+                                    b.chunk_sys, b.sys_end, b.sys_chunk0, b.sys_bs, ro(b.a), ci(b.a),
+                                    static_cast<const S*>(b.a->values), static_cast<S*>(b.binv), b.bmax, b.st));
         list(b, out);
         out.push_back(residual<true>(b, b.p0));
+        block_apply(b, out);
         if (b.g) factored<false>(b, nullptr, out);
         out.push_back(reduce<kReduceInit>(b, b.prr, b.prz));
+    }
+
+    // z = M^-1 r and the partial r.z of the block-Jacobi systems.
+    static void block_apply(const BatchProblem& b, std::vector<KernelCall>& out)
+    {
+        if (!b.sys_bs) return;
+        out.push_back(make_call(k_batch_block_apply<S>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys, b.sys_end,
+                                b.sys_chunk0, b.sys_bs, static_cast<const S*>(b.r), static_cast<S*>(b.z),
+                                static_cast<const S*>(b.binv), b.bmax, static_cast<const BatchState*>(b.st), b.prz,
+                                static_cast<const int64_t*>(b.list),
+                                static_cast<const unsigned long long*>(b.list_count)));
     }
 
     static void list(const BatchProblem& b, std::vector<KernelCall>& out)
@@ -810,7 +968,8 @@ struct BatchCalls
                                     static_cast<const S*>(p_new), static_cast<const S*>(b.q),
                                     static_cast<const S*>(b.inv), static_cast<const BatchState*>(b.st), b.prr,
                                     b.prz, static_cast<const int64_t*>(b.list),
-                                    static_cast<const unsigned long long*>(b.list_count)));
+                                    static_cast<const unsigned long long*>(b.list_count), b.sys_bs));
+            block_apply(b, out);
         }
         else
         {
@@ -819,9 +978,10 @@ struct BatchCalls
                                     static_cast<S*>(b.z), static_cast<const S*>(p_new), static_cast<const S*>(b.q),
                                     static_cast<const S*>(b.inv), static_cast<const BatchState*>(b.st), b.prr,
                                     b.prz, static_cast<const int64_t*>(b.list),
-                                    static_cast<const unsigned long long*>(b.list_count)));
+                                    static_cast<const unsigned long long*>(b.list_count), b.sys_bs));
             out.push_back(residual<false>(b, nullptr));
             if (b.g) factored<false>(b, nullptr, out);
+            block_apply(b, out);
         }
         out.push_back(reduce<kReduceBeta>(b, b.prr, b.prz));
     }
@@ -909,6 +1069,8 @@ std::string batch_key(const BatchProblem& b, int what, int64_t first, int64_t co
                   (b.split ? 1 : 0) | (b.fuse_step ? 2 : 0) | (b.long_lanes << 2),
                   static_cast<const void*>(b.sys_jacobi));
     std::string key(buf);
+    std::snprintf(buf, sizeof(buf), "|blocks|%p|%d|%p", static_cast<const void*>(b.sys_bs), b.bmax, b.binv);
+    key += buf;
     if (b.g)
     {
         // every array of G and G^T the factored kernels capture
@@ -1180,6 +1342,42 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     if (chunks > 0)
         PM_HIP(hipMemcpy(b.chunk_sys, chunk_sys.data(), sizeof(int32_t) * chunks, hipMemcpyHostToDevice));
     if (systems > 0) PM_HIP(hipMemcpy(b.sys_end, sys.data(), sizeof(int64_t) * 3 * systems, hipMemcpyHostToDevice));
+    return PM_OK;
+}
+
+// Block-Jacobi sizes of a batch (host, one per system, 1 to kMaxBatchBlock;
+// 1 is Jacobi): every block-Jacobi system's size must be a multiple of its
+// block size.  A batch whose sizes are all 1 is a Jacobi batch.
+int batch_attach_blocks(BatchProblem& b, const int64_t* row_begin, const int64_t* row_end,
+                        const int32_t* block_sizes, pm_workspace* ws, hipStream_t stream)
+{
+    if (b.systems > 0 && !block_sizes) return fail(PM_INVALID_ARGUMENT, "batched block pcg: null block sizes");
+    int bmax = 1;
+    for (int64_t s = 0; s < b.systems; ++s)
+    {
+        const int bs = block_sizes[s];
+        if (bs < 1 || bs > kMaxBatchBlock)
+            return fail(PM_INVALID_ARGUMENT, "batched block pcg: system %lld block size %d is outside [1, %d]",
+                        static_cast<long long>(s), bs, kMaxBatchBlock);
+        if ((row_end[s] - row_begin[s]) % bs != 0)
+            return fail(PM_INVALID_ARGUMENT, "batched block pcg: system %lld has %lld rows, not a multiple of %d",
+                        static_cast<long long>(s), static_cast<long long>(row_end[s] - row_begin[s]), bs);
+        bmax = std::max(bmax, bs);
+    }
+    b.jacobi = true;
+    if (bmax == 1) return PM_OK;
+    auto al = [](size_t v) { return (v + 255) & ~size_t(255); };
+    const size_t es = b.a->dtype == PM_F64 ? 8 : 4;
+    const size_t rows = static_cast<size_t>(std::max<int64_t>(b.chunks, 1)) * kChunkRows;
+    const size_t sizes_bytes = al(sizeof(int32_t) * static_cast<size_t>(b.systems));
+    PM_TRY(ws->blocks.reserve(sizes_bytes + al(es * rows * static_cast<size_t>(bmax))));
+    char* base = ws->blocks.as<char>();
+    b.sys_bs = reinterpret_cast<const int32_t*>(base);
+    b.binv = base + sizes_bytes;
+    b.bmax = bmax;
+    // the host array is consumed by the first kernels: copy it synchronously
+    PM_HIP(hipStreamSynchronize(stream));
+    PM_HIP(hipMemcpy(base, block_sizes, sizeof(int32_t) * static_cast<size_t>(b.systems), hipMemcpyHostToDevice));
     return PM_OK;
 }
 
@@ -1489,7 +1687,7 @@ static int batch_refine_entry(const pm_csr* a, int64_t systems, const int64_t* r
                               const double* rhs, double* x, const pm_csr* g, const pm_csr* gt,
                               const uint8_t* sys_jacobi, const pm_solver_options* inner, double target_relres,
                               int32_t max_outer, const pm_solve_policy* policy, pm_batch_refine_report* reports,
-                              pm_workspace* ws, hipStream_t stream)
+                              pm_workspace* ws, hipStream_t stream, const int32_t* block_sizes = nullptr)
 {
     if (!(target_relres >= 0.0) || max_outer < 0)
         return fail(PM_INVALID_ARGUMENT, "batched refinement: invalid target or outer limit");
@@ -1504,6 +1702,7 @@ static int batch_refine_entry(const pm_csr* a, int64_t systems, const int64_t* r
     BatchProblem b;
     PM_TRY(batch_setup(a, systems, row_begin, row_end, rhs, x, &in, reports != nullptr, ws, stream, b));
     if (g || gt) PM_TRY(batch_attach_factors(b, g, gt, sys_jacobi));
+    if (block_sizes) PM_TRY(batch_attach_blocks(b, row_begin, row_end, block_sizes, ws, stream));
     if (systems == 0) return PM_OK;
     return batch_refine(b, rhs, x, target_relres, max_outer, resolve_batch_policy(policy), reports, ws, stream);
 }
@@ -1540,6 +1739,30 @@ int pm_batch_pcg_refine_factored(const pm_csr* a, int64_t systems, const int64_t
     if (!g || !gt) return fail(PM_INVALID_ARGUMENT, "batched factored refinement: null factor");
     return batch_refine_entry(a, systems, row_begin, row_end, rhs, x, g, gt, jacobi_systems, inner, target_relres,
                               max_outer, policy, reports, ws, stream);
+}
+
+int pm_batch_block_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                       const int32_t* block_sizes, const void* rhs, void* x, const pm_solver_options* options,
+                       const pm_solve_policy* policy, pm_batch_report* reports, pm_workspace* ws, hipStream_t stream)
+{
+    BatchProblem b;
+    PM_TRY(batch_setup(a, systems, row_begin, row_end, rhs, x, options, reports != nullptr, ws, stream, b));
+    PM_TRY(batch_attach_blocks(b, row_begin, row_end, block_sizes, ws, stream));
+    if (systems == 0) return PM_OK;
+    const BatchPolicy pol = resolve_batch_policy(policy);
+    if (a->dtype == PM_F32) return batch_solve<float>(b, pol, reports, ws, stream);
+    return batch_solve<double>(b, pol, reports, ws, stream);
+}
+
+int pm_batch_block_pcg_refine(const pm_csr* a, int64_t systems, const int64_t* row_begin, const int64_t* row_end,
+                              const int32_t* block_sizes, const double* rhs, double* x,
+                              const pm_solver_options* inner, double target_relres, int32_t max_outer,
+                              const pm_solve_policy* policy, pm_batch_refine_report* reports, pm_workspace* ws,
+                              hipStream_t stream)
+{
+    if (systems > 0 && !block_sizes) return fail(PM_INVALID_ARGUMENT, "batched block refinement: null block sizes");
+    return batch_refine_entry(a, systems, row_begin, row_end, rhs, x, nullptr, nullptr, nullptr, inner, target_relres,
+                              max_outer, policy, reports, ws, stream, block_sizes);
 }
 
 } // extern "C"

@@ -120,6 +120,7 @@ struct pm_workspace
     pm::DeviceBuffer sort;      // COO conversion buffers
     pm::DeviceBuffer refine;    // FP64 residual / FP32 correction of the refinement loop
     pm::DeviceBuffer batch;     // batched solver: layout, per-system state, per-chunk partials
+    pm::DeviceBuffer blocks;    // batched solver: per-system block sizes and block-Jacobi inverse rows
     pm::DeviceBuffer dense;     // dense Cholesky: n x n lower triangle, diagonal-block inverses
     pm::DeviceBuffer dia;       // diagonal layout of the solver's matrix
     pm::DeviceBuffer dia_g;     // ... of the factored preconditioner's G

@@ -196,3 +196,122 @@ def test_batch_refinement_outer_limit(dev, ws):
     with pytest.raises(capi.PlamatrixError) as e:
         capi.batch_pcg_refine(a, begin, end, bt, x, ws, target=-1.0)
     assert e.value.kind == "invalid_argument"
+
+
+# ---- block-Jacobi systems in a batch -------------------------------------------------
+
+
+def schur_members(dev, counts, seed0=7000):
+    """Reduced LM Schur systems (6x6 block CSR, device-assembled), as host CSR."""
+    from plamatrix_amd import workloads
+
+    out = []
+    for i, pc in enumerate(counts):
+        inst = generators.lm_schur_instance(primary_count=pc, eliminated_count=3 * pc, seed=seed0 + i)
+        rows, ro, ci, va = workloads.reduced_schur(capi, torch, inst, dev)
+        out.append(generators.CSRHost(rows, rows, ro.cpu().numpy(), ci.cpu().numpy(), va.cpu().numpy()))
+    return out
+
+
+def host_block_inverses(sys, bs, dtype):
+    import scipy.sparse as sp
+
+    m = sp.csr_matrix((sys.values.astype(np.float64), sys.col_indices, sys.row_offsets), shape=(sys.rows, sys.rows))
+    nb = sys.rows // bs
+    inv = np.empty((nb, bs, bs))
+    for k in range(nb):
+        inv[k] = np.linalg.inv(m[k * bs:(k + 1) * bs, k * bs:(k + 1) * bs].toarray())
+    return inv.astype(dtype).reshape(-1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_block_jacobi_matches_single_solves(dev, ws, dtype):
+    # Schur members with 6x6 block-Jacobi beside Jacobi Laplacian / diffusion
+    # members: each system behaves as the single-system solve of its kind
+    # (blockPcg with the same block inverses, or Jacobi-PCG)
+    schur = schur_members(dev, [40, 90])
+    systems = [schur[0], generators.laplacian2d_csr(17, 19, dtype=np.float64), schur[1],
+               generators.diffusion3d_box_csr(9, 8, 7, seed=5, dtype=np.float64)]
+    systems = [generators.CSRHost(s.rows, s.cols, s.row_offsets, s.col_indices, s.values.astype(dtype))
+               for s in systems]
+    blocks = [6, 1, 6, 1]
+    a, begin, end, big, b = pack(systems, dev, dtype)
+    bt = torch.from_numpy(b).to(dev)
+    x = torch.zeros(big.rows, dtype=bt.dtype, device=dev)
+    rel = 1e-10 if dtype == np.float64 else 1e-5
+    reps = capi.batch_block_pcg(a, begin, end, blocks, bt, x, ws, relative_tolerance=rel, max_iterations=5000)
+    for s, bs, b0, e0, rep in zip(systems, blocks, begin, end, reps):
+        assert rep.converged, rep
+        single = capi.to_device_csr(s, compact=True)
+        xs = torch.zeros(s.rows, dtype=bt.dtype, device=dev)
+        rhs = bt[b0:e0].contiguous()
+        if bs > 1:
+            inv = torch.from_numpy(host_block_inverses(s, bs, dtype)).to(dev)
+            r1 = capi.block_pcg(single, rhs, xs, inv, bs, ws, relative_tolerance=rel, max_iterations=5000)
+        else:
+            r1 = capi.pcg(single, rhs, xs, ws, relative_tolerance=rel, max_iterations=5000)
+        assert r1.converged
+        # same recurrence; the inverses (device Gauss-Jordan vs host LAPACK)
+        # and the dot-product partition differ in rounding only
+        assert abs(rep.iterations - r1.iterations) <= 2, (bs, rep.iterations, r1.iterations)
+        xb, x1 = x[b0:e0].cpu().numpy().astype(np.float64), xs.cpu().numpy().astype(np.float64)
+        tol = 1e-7 if dtype == np.float64 else 1e-3
+        assert np.linalg.norm(xb - x1) <= tol * np.linalg.norm(x1)
+        assert relres(s, b[b0:e0], xb) <= (2e-10 if dtype == np.float64 else 5e-5)
+    # block-Jacobi needs fewer iterations than Jacobi on the Schur members
+    xj = torch.zeros_like(x)
+    repj = capi.batch_pcg(a, begin, end, bt, xj, ws, relative_tolerance=rel, max_iterations=5000)
+    assert reps[0].iterations < repj[0].iterations and reps[2].iterations < repj[2].iterations
+    # all block sizes 1 is the Jacobi batch, bit for bit
+    x1 = torch.zeros_like(x)
+    reps1 = capi.batch_block_pcg(a, begin, end, [1, 1, 1, 1], bt, x1, ws, relative_tolerance=rel,
+                                 max_iterations=5000)
+    assert [r.iterations for r in reps1] == [r.iterations for r in repj]
+    assert torch.equal(x1, xj)
+
+
+def test_batch_block_jacobi_rejects_bad_block_and_layout(dev, ws):
+    lap = generators.laplacian2d_csr(12, 12)
+    bad = generators.laplacian2d_csr(10, 10)
+    bad.values = bad.values.copy()
+    # rows 4, 5 form block 2 of size 2; a negative diagonal breaks its SPD pivots
+    k = bad.row_offsets[4] + int(np.nonzero(bad.col_indices[bad.row_offsets[4]:bad.row_offsets[5]] == 4)[0][0])
+    bad.values[k] = -4.0
+    systems = [lap, bad]
+    a, begin, end, big, b = pack(systems, dev, np.float64)
+    bt = torch.from_numpy(b).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    reps = capi.batch_block_pcg(a, begin, end, [2, 2], bt, x, ws, relative_tolerance=1e-10, max_iterations=2000)
+    assert reps[0].converged
+    assert reps[1].status == 6 and reps[1].bad_row == begin[1] + 4
+    assert torch.count_nonzero(x[begin[1]:end[1]]).item() == 0
+    with pytest.raises(capi.PlamatrixError) as e:  # 144 rows are not a multiple of 5
+        capi.batch_block_pcg(a, begin, end, [5, 2], bt, x, ws)
+    assert e.value.kind == "invalid_argument"
+    with pytest.raises(capi.PlamatrixError) as e:
+        capi.batch_block_pcg(a, begin, end, [2, 9], bt, x, ws)
+    assert e.value.kind == "invalid_argument"
+
+
+def test_batch_block_jacobi_refinement_reaches_target(dev, ws, oracle):
+    # FP32 inner solves with 6x6 block-Jacobi on the Schur members, FP64 x:
+    # every system reaches the FP64 target and the oracle's FP64 solution
+    schur = schur_members(dev, [50, 120], seed0=7100)
+    systems = [schur[0], generators.laplacian2d_csr(20, 20, dtype=np.float64), schur[1]]
+    systems = [generators.CSRHost(s.rows, s.cols, s.row_offsets, s.col_indices, s.values.astype(np.float32))
+               for s in systems]
+    a, begin, end, big, b = pack(systems, dev, np.float32)
+    b64 = b.astype(np.float64)
+    bt = torch.from_numpy(b64).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    reps = capi.batch_block_pcg_refine(a, begin, end, [6, 1, 6], bt, x, ws, target=1e-9, max_outer=20,
+                                       relative_tolerance=1e-3, max_iterations=5000)
+    xs = x.cpu().numpy()
+    for s, b0, e0, rep in zip(systems, begin, end, reps):
+        assert rep.converged and rep.final_relres <= 1e-9, rep
+        got = xs[b0:e0]
+        assert relres(s, b64[b0:e0], got) <= 1.01e-9
+        ref_sys = oracle.CSR(s.rows, s.rows, s.row_offsets, s.col_indices, s.values.astype(np.float64))
+        xr, rr = oracle.pcg(ref_sys, b64[b0:e0], relative_tolerance=1e-12, max_iterations=20000)
+        assert rr.converged
+        assert np.linalg.norm(got - xr) <= 1e-6 * np.linalg.norm(xr)
