@@ -530,6 +530,37 @@ int pm_normal_equations_assemble(pm_schur_workspace* ws, int64_t primary_count, 
                                  int64_t primary_size, int64_t eliminated_size, const pm_residual_batch* batch,
                                  pm_normal_equations* out, hipStream_t stream);
 
+/* Residuals that each link k distinct primary blocks and no eliminated block
+ * (addPrimaryTerms, block_normal_equations.cpp:135-222: pose priors,
+ * relative-pose or other primary-only constraints).  primary: count x k ids;
+ * jacobians: count x k row-major residual_size x primary_size blocks (block
+ * a of residual i at (i k + a) residual_size primary_size); weights == NULL
+ * takes the Huber IRLS weight of ||r||^2 as pm_residual_batch does. */
+typedef struct pm_primary_residual_batch {
+    int64_t count;
+    int64_t residual_size;
+    int64_t blocks_per_residual;
+    const int64_t* primary;
+    const double* jacobians;
+    const double* residuals;
+    const double* weights;
+    double huber_delta;
+} pm_primary_residual_batch;
+
+/* Device assembly of a primary-only residual batch on top of `in` (for
+ * example the output of pm_normal_equations_assemble): out holds in's arrays
+ * with the primary diagonal blocks, primary gradient and primary cross blocks
+ * replaced by workspace arrays that add the batch's terms, bit for bit as
+ * addPrimaryTerms called for every residual in order after the equations in
+ * `in` were accumulated (primary cross blocks in their order of creation,
+ * in's first).  in is not modified and may equal out; out's new arrays live
+ * in the workspace until the next call.  Invalid ids or values fail like the
+ * reference's exceptions (the lowest offending residual decides) and leave
+ * out unchanged.  Synchronises `stream`. */
+int pm_normal_equations_add_primary(pm_schur_workspace* ws, const pm_normal_equations* in,
+                                    const pm_primary_residual_batch* batch, pm_normal_equations* out,
+                                    hipStream_t stream);
+
 /* solveDampedSchurComplement(equations, damping, options, workspace,
  * primary_step, eliminated_step) with the device backend
  * (block_schur.cpp:53-459): damping, eliminated inverses, reduced rhs, Schur

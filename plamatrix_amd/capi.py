@@ -114,7 +114,7 @@ EXPORTS = [
     "pm_fsai_graph_pool_release", "pm_fsai_graph_pool_bytes",
     "pm_fsai_forward", "pm_fsai_factors", "pm_fsai_loss_grad", "pm_fsai_train_step", "pm_dense_cholesky_solve",
     "pm_batch_pcg_factored", "pm_batch_pcg_refine_factored", "pm_fsai_batch_select",
-    "pm_batch_block_pcg", "pm_batch_block_pcg_refine",
+    "pm_batch_block_pcg", "pm_batch_block_pcg_refine", "pm_normal_equations_add_primary",
     "pm_spmm_noalloc",
 ]
 
@@ -518,6 +518,26 @@ def assemble_normal_equations(sw: SchurWorkspace, primary_count, eliminated_coun
     return NormalEquations(out, (primary, eliminated, primary_jacobians, eliminated_jacobians, residuals, weights))
 
 
+class pm_primary_residual_batch(ctypes.Structure):
+    _fields_ = [("count", _i64), ("residual_size", _i64), ("blocks_per_residual", _i64), ("primary", _p),
+                ("jacobians", _p), ("residuals", _p), ("weights", _p), ("huber_delta", _f64)]
+
+
+def add_primary_terms(sw: SchurWorkspace, ne: NormalEquations, primary, jacobians, residuals, weights=None,
+                      huber_delta=1.0) -> NormalEquations:
+    """Device addPrimaryTerms over a batch: primary (count x k int64 ids),
+    jacobians (count x k x rs x ps float64), residuals (count x rs float64)
+    device tensors; returns new equations (the input is unchanged)."""
+    count, k = int(primary.shape[0]), int(primary.shape[1]) if primary.dim() == 2 else 1
+    rs = int(residuals.shape[1]) if residuals.dim() == 2 else 1
+    b = pm_primary_residual_batch(count, rs, k, _ptr(primary), _ptr(jacobians), _ptr(residuals),
+                                  _ptr(weights) if weights is not None else None, float(huber_delta))
+    out = pm_normal_equations()
+    check(lib().pm_normal_equations_add_primary(sw.handle, ctypes.byref(ne.desc), ctypes.byref(b),
+                                                ctypes.byref(out), current_stream()))
+    return NormalEquations(out, (ne.keep, primary, jacobians, residuals, weights))
+
+
 def upload_normal_equations(pc, ec, ps, es, exported: dict, device="cuda") -> NormalEquations:
     """Normal equations from host arrays (the layout of BlockNormalEquations'
     members, e.g. oracle.NormalEquations.export())."""
@@ -551,7 +571,10 @@ def normal_equations_arrays(ne: NormalEquations) -> dict:
                 eliminated_gradient=device_array(d.eliminated_gradient, ec * es),
                 cross_primary=device_array(d.cross_primary, nc, "int64"),
                 cross_eliminated=device_array(d.cross_eliminated, nc, "int64"),
-                cross_values=device_array(d.cross_values, nc * ps * es))
+                cross_values=device_array(d.cross_values, nc * ps * es),
+                primary_cross_rows=device_array(d.primary_cross_rows, d.primary_cross_count, "int64"),
+                primary_cross_cols=device_array(d.primary_cross_cols, d.primary_cross_count, "int64"),
+                primary_cross_values=device_array(d.primary_cross_values, d.primary_cross_count * ps * ps))
 
 
 def lib_memcpy(dst: int, src: int, nbytes: int) -> int:

@@ -41,6 +41,7 @@
 #include <cmath>
 #include <limits>
 #include <cstdlib>
+#include <unordered_map>
 #include <cstring>
 
 #include "dense_cholesky.hpp"
@@ -1372,6 +1373,116 @@ SortSpace carve_sort(Carve& c, int64_t n)
     return s;
 }
 
+// ---- primary-only residuals (addPrimaryTerms, block_normal_equations.cpp:135-222) ----
+//
+// A batch of residuals, each linking k distinct primary blocks (k x residual
+// size x primary size Jacobians).  The groupings (which terms land in which
+// block, and the primary cross blocks in their order of creation) are built on
+// the host from the ids; the sums run on the device, per output entry over
+// its terms in residual order, starting from the entry's current value, with
+// the reference's operation order ((w * a) * b, then +=, no contraction): the
+// result is bit for bit that of calling addPrimaryTerms for every residual in
+// order after the equations already held.
+
+// Values check and weights: flags the lowest residual with a non-finite
+// Jacobian or residual entry (bad_value) or an invalid caller weight
+// (bad_weight); weights == null takes the Huber weight of ||r||^2.
+__global__ __launch_bounds__(kBlock) void k_pt_check(int64_t count, int k, int rs, int ps,
+                                                     const double* __restrict__ jac, const double* __restrict__ r,
+                                                     const double* __restrict__ weights, double delta,
+                                                     double* __restrict__ w_out, SchurStatus* st)
+{
+#pragma clang fp contract(off)
+    for (int64_t i = gtid(); i < count; i += gstride())
+    {
+        bool finite = true;
+        double s = 0.0;
+        for (int q = 0; q < rs; ++q)
+        {
+            const double v = r[i * rs + q];
+            finite &= isfinite(v);
+            s += v * v;
+        }
+        const int64_t per = static_cast<int64_t>(k) * rs * ps;
+        for (int64_t q = 0; q < per; ++q) finite &= isfinite(jac[i * per + q]);
+        if (!finite) atomicMin(&st->bad_value, static_cast<unsigned long long>(i));
+        double w;
+        if (weights)
+        {
+            w = weights[i];
+            if (!isfinite(w) || w < 0.0) atomicMin(&st->bad_weight, static_cast<unsigned long long>(i));
+        }
+        else w = huber_weight(s, delta);
+        w_out[i] = w;
+    }
+}
+
+// Diagonal blocks and gradients: thread per (block, entry) over the blocks
+// with terms; terms[offs[b] .. offs[b+1]) are i * k + slot in residual order.
+__global__ __launch_bounds__(kBlock) void k_pt_diag(int64_t touched, const int64_t* __restrict__ blocks,
+                                                    const int64_t* __restrict__ offs,
+                                                    const int64_t* __restrict__ terms, int k, int rs, int ps,
+                                                    const double* __restrict__ jac, const double* __restrict__ r,
+                                                    const double* __restrict__ w, double* __restrict__ B,
+                                                    double* __restrict__ g)
+{
+#pragma clang fp contract(off)
+    const int entries = ps * ps + ps;
+    for (int64_t t = gtid(); t < touched * entries; t += gstride())
+    {
+        const int64_t x = t / entries;
+        const int en = static_cast<int>(t % entries);
+        const bool is_b = en < ps * ps;
+        const int column = is_b ? en / ps : en - ps * ps;
+        const int other = is_b ? en % ps : 0;
+        const int64_t blk = blocks[x];
+        double* out = is_b ? B + blk * ps * ps + column * ps + other : g + blk * ps + column;
+        double acc = *out;
+        for (int64_t q = offs[x]; q < offs[x + 1]; ++q)
+        {
+            const int64_t term = terms[q];
+            const int64_t i = term / k;
+            const double* J = jac + term * rs * ps;
+            const double wi = w[i];
+            for (int row = 0; row < rs; ++row)
+                acc += wi * J[row * ps + column] * (is_b ? J[row * ps + other] : r[i * rs + row]);
+        }
+        *out = acc;
+    }
+}
+
+// Primary cross blocks: thread per (cross block, entry) over the blocks with
+// terms; a term is (i * k + row slot) * k + column slot.
+__global__ __launch_bounds__(kBlock) void k_pt_cross(int64_t touched, const int64_t* __restrict__ index,
+                                                     const int64_t* __restrict__ offs,
+                                                     const int64_t* __restrict__ terms, int k, int rs, int ps,
+                                                     const double* __restrict__ jac, const double* __restrict__ w,
+                                                     double* __restrict__ values)
+{
+#pragma clang fp contract(off)
+    const int entries = ps * ps;
+    for (int64_t t = gtid(); t < touched * entries; t += gstride())
+    {
+        const int64_t x = t / entries;
+        const int en = static_cast<int>(t % entries);
+        const int row = en / ps, column = en % ps;
+        double* out = values + index[x] * entries + en;
+        double acc = *out;
+        for (int64_t q = offs[x]; q < offs[x + 1]; ++q)
+        {
+            const int64_t term = terms[q];
+            const int64_t cs = term % k;
+            const int64_t rsl = (term / k) % k;
+            const int64_t i = term / k / k;
+            const double* Jr = jac + (i * k + rsl) * rs * ps;
+            const double* Jc = jac + (i * k + cs) * rs * ps;
+            const double wi = w[i];
+            for (int rr = 0; rr < rs; ++rr) acc += wi * Jr[rr * ps + row] * Jc[rr * ps + column];
+        }
+        *out = acc;
+    }
+}
+
 } // namespace pm
 
 using namespace pm;
@@ -1402,6 +1513,11 @@ struct pm_schur_workspace
     unsigned* ne_run_cross = nullptr;
     int64_t* ne_cross_primary = nullptr;
     int64_t* ne_cross_eliminated = nullptr;
+
+    // primary-only residuals (addPrimaryTerms): groupings, weights, primary
+    // cross blocks
+    DeviceBuffer pt_meta;
+    DeviceBuffer pt_cross;
 
     // Schur pattern (reused while the cross-block topology is unchanged)
     DeviceBuffer pattern;
@@ -1490,6 +1606,8 @@ struct pm_schur_workspace
         pattern_sort.release();
         segment_slot.release();
         numerics.release();
+        pt_meta.release();
+        pt_cross.release();
     }
 };
 
@@ -2074,6 +2192,221 @@ int pm_schur_pattern_size(const pm_schur_workspace* w, int64_t* slots, int64_t* 
     if (slots) *slots = w->slots;
     if (values) *values = w->values_count;
     if (terms) *terms = w->terms;
+    return PM_OK;
+}
+
+int pm_normal_equations_add_primary(pm_schur_workspace* w, const pm_normal_equations* in,
+                                    const pm_primary_residual_batch* pb, pm_normal_equations* out,
+                                    hipStream_t stream)
+{
+    if (!w || !in || !pb || !out) return fail(PM_INVALID_ARGUMENT, "pm_normal_equations_add_primary: null argument");
+    PM_TRY(check_normal_equations(in));
+    const int64_t pc = in->primary_count, ps = in->primary_size;
+    const int64_t n = pb->count, k = pb->blocks_per_residual, rs = pb->residual_size;
+    if (n < 0 || k <= 0 || rs <= 0)
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations: primary block and Jacobian lists must have equal non-zero size");
+    if (n > 0 && (!pb->primary || !pb->jacobians || !pb->residuals))
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations::addPrimaryTerms: pointers must be non-null and residual size positive");
+    if (n > 0 && pc * ps == 0)
+        return fail(PM_OUT_OF_RANGE, "BlockNormalEquations: primary block index out of range");
+    if (!pb->weights && (!std::isfinite(pb->huber_delta) || pb->huber_delta < 0.0))
+        return fail(PM_INVALID_ARGUMENT, "evaluateHuberLoss: delta must be finite and non-negative");
+    if (n * k >= (int64_t(1) << 40)) return fail(PM_OVERFLOW, "primary residual batch is too large");
+    PM_TRY(w->bind(stream));
+    // ids and the existing primary cross blocks on the host
+    std::vector<int64_t> ids(static_cast<size_t>(n * k));
+    const int64_t n0 = in->primary_cross_count;
+    std::vector<int64_t> rows0(static_cast<size_t>(n0)), cols0(static_cast<size_t>(n0));
+    if (n > 0) PM_HIP(hipMemcpyAsync(ids.data(), pb->primary, sizeof(int64_t) * n * k, hipMemcpyDeviceToHost, stream));
+    if (n0 > 0)
+    {
+        PM_HIP(hipMemcpyAsync(rows0.data(), in->primary_cross_rows, sizeof(int64_t) * n0, hipMemcpyDeviceToHost,
+                              stream));
+        PM_HIP(hipMemcpyAsync(cols0.data(), in->primary_cross_cols, sizeof(int64_t) * n0, hipMemcpyDeviceToHost,
+                              stream));
+    }
+    PM_HIP(hipStreamSynchronize(stream));
+    int64_t bad_id = -1;
+    bool bad_range = false;
+    for (int64_t i = 0; i < n && bad_id < 0; ++i)
+        for (int64_t a = 0; a < k; ++a)
+        {
+            const int64_t b = ids[static_cast<size_t>(i * k + a)];
+            if (b < 0 || b >= pc)
+            {
+                bad_id = i;
+                bad_range = true;
+                break;
+            }
+            for (int64_t prior = 0; prior < a; ++prior)
+                if (ids[static_cast<size_t>(i * k + prior)] == b) bad_id = i;
+            if (bad_id >= 0) break;
+        }
+    // values and weights
+    Carve mz;
+    mz.take<double>(n);
+    PM_TRY(w->pt_meta.reserve(mz.off));
+    double* weights = w->pt_meta.as<double>();
+    hipLaunchKernelGGL(k_schur_status_init, dim3(1), dim3(1), 0, stream, w->dstatus());
+    if (n > 0)
+        hipLaunchKernelGGL(k_pt_check, dim3(grid_for(n)), dim3(kBlock), 0, stream, n, static_cast<int>(k),
+                           static_cast<int>(rs), static_cast<int>(ps), pb->jacobians, pb->residuals, pb->weights,
+                           pb->huber_delta, weights, w->dstatus());
+    PM_HIP(hipGetLastError());
+    PM_TRY(w->read_status(stream));
+    const SchurStatus& hs = *w->host_status;
+    const unsigned long long bad_val = std::min(hs.bad_value, hs.bad_weight);
+    // the lowest offending residual decides the error, as a sequence of
+    // addPrimaryTerms calls would stop at it (ids are checked first within it)
+    if (bad_id >= 0 && static_cast<unsigned long long>(bad_id) <= bad_val)
+    {
+        if (bad_range)
+            return fail(PM_OUT_OF_RANGE, "BlockNormalEquations: primary block index out of range (residual %lld)",
+                        static_cast<long long>(bad_id));
+        return fail(PM_INVALID_ARGUMENT, "BlockNormalEquations: primary block indices must be unique (residual %lld)",
+                    static_cast<long long>(bad_id));
+    }
+    if (hs.bad_value != ~0ull && hs.bad_value <= hs.bad_weight)
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations::addPrimaryTerms: Jacobian and residual values must be finite (residual "
+                    "%llu)",
+                    hs.bad_value);
+    if (hs.bad_weight != ~0ull)
+        return fail(PM_INVALID_ARGUMENT,
+                    "BlockNormalEquations::addPrimaryTerms: weight must be finite and non-negative (residual %llu)",
+                    hs.bad_weight);
+
+    // groupings: diagonal terms by block (terms in residual order), primary
+    // cross blocks found or created in the reference's order
+    std::vector<int64_t> order(static_cast<size_t>(n * k));
+    for (int64_t t = 0; t < n * k; ++t) order[static_cast<size_t>(t)] = t;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t x, int64_t y) { return ids[static_cast<size_t>(x)] < ids[static_cast<size_t>(y)]; });
+    std::vector<int64_t> dblocks, doffs{0};
+    for (int64_t q = 0; q < n * k; ++q)
+    {
+        const int64_t b = ids[static_cast<size_t>(order[static_cast<size_t>(q)])];
+        if (dblocks.empty() || dblocks.back() != b)
+        {
+            if (!dblocks.empty()) doffs.push_back(q);
+            dblocks.push_back(b);
+        }
+    }
+    if (!dblocks.empty()) doffs.push_back(n * k);
+    std::unordered_map<int64_t, int64_t> where;
+    std::vector<int64_t> crows(rows0), ccols(cols0);
+    for (int64_t x = 0; x < n0; ++x) where.emplace(rows0[static_cast<size_t>(x)] * pc + cols0[static_cast<size_t>(x)], x);
+    std::vector<std::vector<int64_t>> cterms(static_cast<size_t>(n0));
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t l = 0; l < k; ++l)
+            for (int64_t r = l + 1; r < k; ++r)
+            {
+                const int64_t bl = ids[static_cast<size_t>(i * k + l)], br = ids[static_cast<size_t>(i * k + r)];
+                const bool ordered = bl < br;
+                const int64_t row = ordered ? bl : br, col = ordered ? br : bl;
+                const int64_t rslot = ordered ? l : r, cslot = ordered ? r : l;
+                auto it = where.find(row * pc + col);
+                int64_t x;
+                if (it == where.end())
+                {
+                    x = static_cast<int64_t>(crows.size());
+                    where.emplace(row * pc + col, x);
+                    crows.push_back(row);
+                    ccols.push_back(col);
+                    cterms.emplace_back();
+                }
+                else x = it->second;
+                cterms[static_cast<size_t>(x)].push_back((i * k + rslot) * k + cslot);
+            }
+    const int64_t ncross = static_cast<int64_t>(crows.size());
+    std::vector<int64_t> cindex, coffs{0}, cflat;
+    for (int64_t x = 0; x < ncross; ++x)
+    {
+        const auto& v = cterms[static_cast<size_t>(x)];
+        if (v.empty()) continue;
+        cindex.push_back(x);
+        cflat.insert(cflat.end(), v.begin(), v.end());
+        coffs.push_back(static_cast<int64_t>(cflat.size()));
+    }
+    const int64_t dtouched = static_cast<int64_t>(dblocks.size()), ctouched = static_cast<int64_t>(cindex.size());
+
+    // outputs: B and g copied from the input, then summed into; the cross
+    // blocks' ids, with the existing values copied and the new ones zero
+    Carve oz;
+    oz.take<double>(pc * ps * ps);
+    oz.take<double>(pc * ps);
+    oz.take<int64_t>(ncross);
+    oz.take<int64_t>(ncross);
+    oz.take<double>(ncross * ps * ps);
+    oz.take<int64_t>(dtouched);
+    oz.take<int64_t>(dtouched + 1);
+    oz.take<int64_t>(n * k);
+    oz.take<int64_t>(ctouched);
+    oz.take<int64_t>(ctouched + 1);
+    oz.take<int64_t>(static_cast<int64_t>(cflat.size()));
+    DeviceBuffer fresh;
+    PM_TRY(fresh.reserve(oz.off));
+    Carve o;
+    o.base = fresh.as<char>();
+    double* B = o.take<double>(pc * ps * ps);
+    double* g = o.take<double>(pc * ps);
+    int64_t* prow = o.take<int64_t>(ncross);
+    int64_t* pcol = o.take<int64_t>(ncross);
+    double* pval = o.take<double>(ncross * ps * ps);
+    int64_t* d_blocks = o.take<int64_t>(dtouched);
+    int64_t* d_offs = o.take<int64_t>(dtouched + 1);
+    int64_t* d_terms = o.take<int64_t>(n * k);
+    int64_t* c_index = o.take<int64_t>(ctouched);
+    int64_t* c_offs = o.take<int64_t>(ctouched + 1);
+    int64_t* c_terms = o.take<int64_t>(static_cast<int64_t>(cflat.size()));
+    const size_t block_bytes = sizeof(double) * static_cast<size_t>(ps * ps);
+    if (pc * ps > 0)
+    {
+        PM_HIP(hipMemcpyAsync(B, in->primary_diagonal, block_bytes * pc, hipMemcpyDeviceToDevice, stream));
+        PM_HIP(hipMemcpyAsync(g, in->primary_gradient, sizeof(double) * pc * ps, hipMemcpyDeviceToDevice, stream));
+    }
+    if (n0 > 0)
+        PM_HIP(hipMemcpyAsync(pval, in->primary_cross_values, block_bytes * n0, hipMemcpyDeviceToDevice, stream));
+    if (ncross > n0)
+        hipLaunchKernelGGL(k_fill_f64, dim3(grid_for((ncross - n0) * ps * ps)), dim3(kBlock), 0, stream,
+                           pval + n0 * ps * ps, (ncross - n0) * ps * ps, 0.0);
+    // the host arrays are consumed by the kernels below: copy them synchronously
+    PM_HIP(hipStreamSynchronize(stream));
+    auto up = [&](int64_t* dst, const std::vector<int64_t>& v) -> int {
+        if (!v.empty()) PM_HIP(hipMemcpy(dst, v.data(), sizeof(int64_t) * v.size(), hipMemcpyHostToDevice));
+        return PM_OK;
+    };
+    PM_TRY(up(prow, crows));
+    PM_TRY(up(pcol, ccols));
+    PM_TRY(up(d_blocks, dblocks));
+    PM_TRY(up(d_offs, doffs));
+    PM_TRY(up(d_terms, order));
+    PM_TRY(up(c_index, cindex));
+    PM_TRY(up(c_offs, coffs));
+    PM_TRY(up(c_terms, cflat));
+    if (dtouched > 0)
+        hipLaunchKernelGGL(k_pt_diag, dim3(grid_for(dtouched * (ps * ps + ps))), dim3(kBlock), 0, stream, dtouched,
+                           d_blocks, d_offs, d_terms, static_cast<int>(k), static_cast<int>(rs),
+                           static_cast<int>(ps), pb->jacobians, pb->residuals, weights, B, g);
+    if (ctouched > 0)
+        hipLaunchKernelGGL(k_pt_cross, dim3(grid_for(ctouched * ps * ps)), dim3(kBlock), 0, stream, ctouched,
+                           c_index, c_offs, c_terms, static_cast<int>(k), static_cast<int>(rs), static_cast<int>(ps),
+                           pb->jacobians, weights, pval);
+    PM_HIP(hipGetLastError());
+    // the previous outputs (possibly this call's input) are no longer read
+    PM_HIP(hipStreamSynchronize(stream));
+    w->pt_cross.release();
+    w->pt_cross = fresh;
+    pm_normal_equations res = *in;
+    res.primary_diagonal = B;
+    res.primary_gradient = g;
+    res.primary_cross_count = ncross;
+    res.primary_cross_rows = prow;
+    res.primary_cross_cols = pcol;
+    res.primary_cross_values = pval;
+    *out = res;
     return PM_OK;
 }
 

@@ -406,3 +406,117 @@ def test_large_blocks_match_oracle_and_lapack(dev, sw, ws, oracle, ps, es):
         assert relres <= 1e-11
         x = np.linalg.solve(S.toarray(), rhs)
         assert np.linalg.norm(dp.cpu().numpy() - x) <= 1e-9 * np.linalg.norm(x)
+
+
+# ---- primary-only residuals (addPrimaryTerms) on the device -------------------------
+
+
+def primary_batch(pc, count, k, rs, ps, seed):
+    """count residuals over k distinct primaries each (consecutive ring
+    neighbours, some in reversed order, some repeated), N(0,1) Jacobians."""
+    rng = np.random.default_rng(seed)
+    ids = np.empty((count, k), np.int64)
+    for i in range(count):
+        start = int(rng.integers(0, pc))
+        blocks = [(start + 2 * j) % pc for j in range(k)]
+        if i % 3 == 1:
+            blocks = blocks[::-1]
+        ids[i] = blocks
+    jac = rng.standard_normal((count, k, rs, ps))
+    res = rng.standard_normal((count, rs)) * 0.5
+    res[::7] *= 20.0  # outliers for the Huber weight
+    weights = rng.uniform(0.2, 2.0, count)
+    return ids, jac, res, weights
+
+
+@pytest.mark.parametrize("huber", [False, True])
+def test_primary_terms_bitwise(dev, sw, oracle, huber):
+    inst = shuffled_instance(16, 300, (3, 8), seed=12)
+    ne = capi.assemble_normal_equations(sw, 16, 300, 6, 3, **device_batch(inst, dev),
+                                        weights=t(inst.weights, dev, np.float64))
+    eq = oracle_equations(oracle, inst, inst.weights)
+    for k, count, seed in ((2, 40, 3), (3, 12, 4)):
+        ids, jac, res, weights = primary_batch(16, count, k, 2, 6, seed)
+        if huber:
+            w_ref = np.array([oracle.huber(squared_norm(r), 1.0)[1] for r in res])
+            ne = capi.add_primary_terms(sw, ne, t(ids, dev), t(jac, dev), t(res, dev), weights=None, huber_delta=1.0)
+        else:
+            w_ref = weights
+            ne = capi.add_primary_terms(sw, ne, t(ids, dev), t(jac, dev), t(res, dev), weights=t(weights, dev))
+        for i in range(count):
+            eq.add_primary_residual_blocks(ids[i], [jac[i, a] for a in range(k)], res[i], w_ref[i])
+    got = capi.normal_equations_arrays(ne)
+    ref = eq.export()
+    assert len(ref["primary_cross_rows"]) > 0
+    for key in ("primary_diagonal", "eliminated_diagonal", "primary_gradient", "eliminated_gradient",
+                "cross_primary", "cross_eliminated", "cross_values", "primary_cross_rows", "primary_cross_cols",
+                "primary_cross_values"):
+        assert got[key].shape == ref[key].shape, key
+        assert np.array_equal(got[key].view(np.uint64), ref[key].view(np.uint64)), key
+
+
+def test_primary_terms_solve_matches_oracle(dev, sw, ws, oracle):
+    # the primary cross blocks enter the reduced system: the device solve of
+    # the augmented equations agrees with the oracle's CPU solve
+    inst = shuffled_instance(20, 400, (3, 8), seed=13)
+    ne = capi.assemble_normal_equations(sw, 20, 400, 6, 3, **device_batch(inst, dev),
+                                        weights=t(inst.weights, dev, np.float64))
+    ids, jac, res, weights = primary_batch(20, 30, 2, 6, 6, seed=5)
+    ne = capi.add_primary_terms(sw, ne, t(ids, dev), t(jac, dev), t(res, dev), weights=t(weights, dev))
+    eq = oracle_equations(oracle, inst, inst.weights)
+    for i in range(len(ids)):
+        eq.add_primary_residual_blocks(ids[i], [jac[i, 0], jac[i, 1]], res[i], weights[i])
+    dp = torch.zeros(20 * 6, dtype=torch.float64, device=dev)
+    de = torch.zeros(400 * 3, dtype=torch.float64, device=dev)
+    rep = capi.schur_solve(sw, ne, inst.damping, dp, de, ws, max_iterations=1000, relative_tolerance=1e-10,
+                           absolute_tolerance=0.0)
+    assert rep.converged
+    rdp, rde, rrep = eq.solve(inst.damping, max_iterations=1000, relative_tolerance=1e-10, absolute_tolerance=0.0)
+    assert rrep.converged
+    got = np.concatenate([dp.cpu().numpy(), de.cpu().numpy()])
+    refv = np.concatenate([rdp, rde])
+    assert np.linalg.norm(got - refv) <= 1e-6 * np.linalg.norm(refv)
+    relres, _, _ = reduced_relres(sw, dp)
+    assert relres <= 1e-10 * 1.01
+
+
+def test_primary_terms_errors(dev, sw):
+    inst = shuffled_instance(6, 20, (3, 4), seed=14)
+    ne = capi.assemble_normal_equations(sw, 6, 20, 6, 3, **device_batch(inst, dev),
+                                        weights=t(inst.weights, dev, np.float64))
+    before = capi.normal_equations_arrays(ne)
+    ids, jac, res, weights = primary_batch(6, 5, 2, 2, 6, seed=6)
+
+    def call(i, j, r, w):
+        return capi.add_primary_terms(sw, ne, t(i, dev), t(j, dev), t(r, dev), weights=t(w, dev))
+
+    bad = ids.copy()
+    bad[3, 1] = 6
+    with pytest.raises(capi.PlamatrixError) as e:
+        call(bad, jac, res, weights)
+    assert e.value.kind == "out_of_range" and "residual 3" in str(e.value)
+    bad = ids.copy()
+    bad[2, 1] = bad[2, 0]
+    with pytest.raises(capi.PlamatrixError) as e:
+        call(bad, jac, res, weights)
+    assert e.value.kind == "invalid_argument" and "unique" in str(e.value)
+    badj = jac.copy()
+    badj[1, 0, 1, 2] = np.nan
+    with pytest.raises(capi.PlamatrixError) as e:
+        call(ids, badj, res, weights)
+    assert e.value.kind == "invalid_argument" and "finite" in str(e.value) and "residual 1" in str(e.value)
+    badw = weights.copy()
+    badw[4] = -1.0
+    with pytest.raises(capi.PlamatrixError) as e:
+        call(ids, jac, res, badw)
+    assert e.value.kind == "invalid_argument" and "weight" in str(e.value)
+    # the lowest offending residual decides: a NaN in residual 1 before a bad id in residual 3
+    bad = ids.copy()
+    bad[3, 1] = 6
+    with pytest.raises(capi.PlamatrixError) as e:
+        call(bad, badj, res, weights)
+    assert "finite" in str(e.value)
+    # the input equations are untouched
+    after = capi.normal_equations_arrays(ne)
+    for key in before:
+        assert np.array_equal(before[key], after[key]), key
