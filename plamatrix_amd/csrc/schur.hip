@@ -1023,6 +1023,12 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
 // (the diagonal ones hold every secondary of their primary) no longer run as
 // one serial chain, and a launch has about 30 times more wavefronts.
 constexpr int kSegTerms = 16;
+// Segments a wavefront of k_assemble_segments sums.  One per wavefront, over
+// a grid of every segment, measured fastest on the 2,000-primary LM system:
+// 0.88 ms for the assembly against 0.90 with two and 0.94 with four (their
+// loads in flight together cost occupancy: 134 VGPRs at four), and 1.01 ms
+// for two segments per wavefront in sequence under a 65,536-workgroup grid.
+constexpr int kSegsPerWave = 1;
 
 // Per slot: the number of terms (the run minus its trailing markers) and of
 // segments; seg_count[slots] = 0 closes the scan.
@@ -1067,54 +1073,71 @@ __global__ __launch_bounds__(kBlock) void k_assemble_segments(
     constexpr int STAGE = kSegTerms * PER;
     constexpr int LOADS = (STAGE + kWave - 1) / kWave;
     constexpr int kWaves = kBlock / kWave;
+    constexpr int SEGS = kSegsPerWave;
     __shared__ double stage_all[kWaves][STAGE];
     const int lane = threadIdx.x % kWave;
     const int wave = threadIdx.x / kWave;
     double* st = stage_all[wave];
     const bool active = lane < PS * PS;
     const int lr = active ? lane / PS : 0, lc = active ? lane % PS : 0;
-    // each workgroup owns a contiguous, XCD-local range of segments:
-    // neighbouring segments share slots and blocks, which then stay in one L2
-    const int64_t per = ceil_div(ceil_div(nseg, static_cast<int64_t>(gridDim.x)), int64_t(kWaves)) * kWaves;
-    const int64_t gbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
-    const int64_t gend = gbegin + per < nseg ? gbegin + per : nseg;
-    for (int64_t g = gbegin + wave; g < gend; g += kWaves)
+    // each wavefront owns SEGS consecutive segments and each workgroup a
+    // contiguous, XCD-local range: neighbouring segments share slots and
+    // blocks, which then stay in one L2.  The index chain (segment -> slot
+    // -> terms -> blocks) of all SEGS segments is issued before the first
+    // sum.
+    const int64_t g0 = (xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave) * SEGS;
+    unsigned sl[SEGS];
+#pragma unroll
+    for (int q = 0; q < SEGS; ++q) sl[q] = g0 + q < nseg ? seg_slot[g0 + q] : 0u;
+    int nt[SEGS];
+    int64_t begin[SEGS];
+#pragma unroll
+    for (int q = 0; q < SEGS; ++q)
     {
-        const unsigned s = seg_slot[g];
-        const int64_t j = g - static_cast<int64_t>(seg_offs[s]);
-        const int64_t begin = slot_start[s] + j * kSegTerms;
-        const int64_t left = static_cast<int64_t>(slot_terms[s]) - j * kSegTerms;
-        const int nt = static_cast<int>(left < kSegTerms ? left : kSegTerms);
-        unsigned idl = 0u, idr = 0u;
-        if (lane < nt)
-        {
-            idl = term_left[begin + lane];
-            idr = term_right[begin + lane];
-        }
-        double val[LOADS];
+        const int64_t j = g0 + q - static_cast<int64_t>(seg_offs[sl[q]]);
+        begin[q] = slot_start[sl[q]] + j * kSegTerms;
+        const int64_t left = static_cast<int64_t>(slot_terms[sl[q]]) - j * kSegTerms;
+        nt[q] = g0 + q < nseg ? static_cast<int>(left < kSegTerms ? left : kSegTerms) : 0;
+    }
+    unsigned idl[SEGS], idr[SEGS];
+#pragma unroll
+    for (int q = 0; q < SEGS; ++q)
+    {
+        idl[q] = lane < nt[q] ? term_left[begin[q] + lane] : 0u;
+        idr[q] = lane < nt[q] ? term_right[begin[q] + lane] : 0u;
+    }
+    double val[SEGS][LOADS];
+#pragma unroll
+    for (int q = 0; q < SEGS; ++q)
+    {
 #pragma unroll
         for (int m = 0; m < LOADS; ++m)
         {
             const int f = lane + m * kWave;
             const int u = f / PER, within = f % PER;
-            const unsigned a = __shfl(idl, u < kWave ? u : 0, kWave);
-            const unsigned b = __shfl(idr, u < kWave ? u : 0, kWave);
-            val[m] = 0.0;
-            if (f < STAGE && u < nt)
-                val[m] = within < BV ? TW[static_cast<int64_t>(a) * BV + within]
-                                     : W[static_cast<int64_t>(b) * BV + (within - BV)];
+            const unsigned a = __shfl(idl[q], u < kWave ? u : 0, kWave);
+            const unsigned b = __shfl(idr[q], u < kWave ? u : 0, kWave);
+            val[q][m] = 0.0;
+            if (f < STAGE && u < nt[q])
+                val[q][m] = within < BV ? TW[static_cast<int64_t>(a) * BV + within]
+                                        : W[static_cast<int64_t>(b) * BV + (within - BV)];
         }
+    }
+#pragma unroll
+    for (int q = 0; q < SEGS; ++q)
+    {
+        if (nt[q] == 0) break;  // wave-uniform: later segments are past the end too
 #pragma unroll
         for (int m = 0; m < LOADS; ++m)
         {
             const int f = lane + m * kWave;
-            if (f < STAGE) st[f] = val[m];
+            if (f < STAGE) st[f] = val[q][m];
         }
         wave_lds_sync();
         if (active)
         {
             double acc = 0.0;
-            for (int u = 0; u < nt; ++u)
+            for (int u = 0; u < nt[q]; ++u)
             {
                 const double* t = st + u * PER + lr * ES;
                 const double* w = st + u * PER + BV + lc * ES;
@@ -1123,7 +1146,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble_segments(
                 for (int i = 0; i < ES; ++i) value += t[i] * w[i];
                 acc += -value;
             }
-            partial[g * (PS * PS) + lane] = acc;
+            partial[(g0 + q) * (PS * PS) + lane] = acc;
         }
         wave_lds_sync();
     }
@@ -2114,8 +2137,9 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
             const int64_t waves_per_block = kBlock / kWave;
             if (w->segments > 0)
             {
-                const int seg_grid = static_cast<int>(
-                    std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->segments, waves_per_block), 65536)));
+                const int64_t seg_blocks = ceil_div(w->segments, waves_per_block * kSegsPerWave);
+                if (seg_blocks > INT32_MAX) return fail(PM_OVERFLOW, "Schur assembly: too many segments");
+                const int seg_grid = static_cast<int>(std::max<int64_t>(1, seg_blocks));
                 hipLaunchKernelGGL((k_assemble_segments<3, 6>), dim3(seg_grid), dim3(kBlock), 0, stream, w->segments,
                                    w->segment_slot.as<unsigned>(), static_cast<const unsigned*>(w->seg_offs),
                                    w->slot_start, static_cast<const unsigned*>(w->slot_terms), w->term_left,
