@@ -68,14 +68,16 @@ struct BatchState
                  i##_n = static_cast<int64_t>(*(list_count));                                         \
          i < i##_n; i += static_cast<int64_t>(gridDim.x) * (W))
 
+// keep_bad_row: the preconditioner's inverses are reused from the previous
+// solve of the same matrix, so are the rows they rejected.
 __global__ __launch_bounds__(kBlock) void k_batch_state_init(int64_t systems, BatchState* st,
-                                                             unsigned long long* done_count)
+                                                             unsigned long long* done_count, int keep_bad_row)
 {
     for (int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; s < systems;
          s += static_cast<int64_t>(gridDim.x) * blockDim.x)
     {
         BatchState b{};
-        b.bad_row = ~0ull;
+        b.bad_row = keep_bad_row ? st[s].bad_row : ~0ull;
         b.done = kRunning;
         st[s] = b;
     }
@@ -539,77 +541,177 @@ __global__ __launch_bounds__(kBlock) void k_batch_jacobi_inverse(int64_t chunks,
 // contiguous row per lane.
 
 constexpr int kMaxBatchBlock = 8;
+constexpr int kRegisterBlock = 6;  // the block size inverted with its block in registers (6x6 pose blocks)
 
-// One lane per block (the lane of its first row): the block gathered from its
-// rows' entries (duplicates summed in CSR order), inverted in FP64 by
-// Gauss-Jordan elimination without pivoting (an SPD block has positive
-// pivots), rounded to S.  A pivot that is not finite or not above 16 eps(S)
-// times the block's largest magnitude rejects the block: its rows get NaN and
-// the system records its first row, as a rejected Jacobi diagonal does.
+// Rows of the diagonal blocks of the block-Jacobi systems, FP64, into
+// tmp[row * bmax + j] (duplicates summed in CSR order): eight lanes per row
+// scan its entries (coalesced); a row with a repeated block column is summed
+// again serially in CSR order.
 template <typename S, typename I>
-__global__ __launch_bounds__(kBlock) void k_batch_block_inverse(int64_t chunks, const int32_t* __restrict__ chunk_sys,
-                                                                const int64_t* __restrict__ sys_end,
-                                                                const int64_t* __restrict__ sys_chunk0,
-                                                                const int32_t* __restrict__ sys_bs,
-                                                                const I* __restrict__ ro, const I* __restrict__ ci,
-                                                                const S* __restrict__ v, S* __restrict__ binv,
-                                                                int bmax, BatchState* st)
+__global__ __launch_bounds__(kBlock) void k_batch_block_gather(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                               const int64_t* __restrict__ sys_end,
+                                                               const int64_t* __restrict__ sys_chunk0,
+                                                               const int32_t* __restrict__ sys_bs,
+                                                               const I* __restrict__ ro, const I* __restrict__ ci,
+                                                               const S* __restrict__ v, double* __restrict__ tmp,
+                                                               int bmax)
 {
+    constexpr int L = 8;
+    const int lane = threadIdx.x % kWave;
+    const int sub = lane % L;
+    PM_FOR_CHUNKS(c, chunks, kBatchWaves)
+    {
+        const int32_t s = chunk_sys[c];
+        if (s < 0) continue;
+        const int bs = sys_bs[s];
+        if (bs <= 1) continue;
+        const int64_t end = sys_end[s];
+        const int64_t begin = sys_chunk0[s] * kChunkRows;
+        for (int pass = 0; pass < L; ++pass)
+        {
+            const int64_t row = c * kChunkRows + pass * L + lane / L;
+            const bool valid = row < end;
+            const int64_t b0 = valid ? row - (row - begin) % bs : 0;
+            double m[kMaxBatchBlock];
+            int cnt[kMaxBatchBlock];
+#pragma unroll
+            for (int jj = 0; jj < kMaxBatchBlock; ++jj)
+            {
+                m[jj] = 0.0;
+                cnt[jj] = 0;
+            }
+            int64_t k0 = 0, k1 = 0;
+            if (valid)
+            {
+                k0 = static_cast<int64_t>(ro[row]);
+                k1 = static_cast<int64_t>(ro[row + 1]);
+                for (int64_t k = k0 + sub; k < k1; k += L)
+                {
+                    const int64_t d = static_cast<int64_t>(ci[k]) - b0;
+                    const double a = static_cast<double>(v[k]);
+#pragma unroll
+                    for (int jj = 0; jj < kMaxBatchBlock; ++jj)
+                        if (d == jj && jj < bs)
+                        {
+                            m[jj] += a;
+                            ++cnt[jj];
+                        }
+                }
+            }
+#pragma unroll
+            for (int off = L / 2; off > 0; off >>= 1)
+            {
+#pragma unroll
+                for (int jj = 0; jj < kMaxBatchBlock; ++jj)
+                {
+                    m[jj] += __shfl_xor(m[jj], off, L);
+                    cnt[jj] += __shfl_xor(cnt[jj], off, L);
+                }
+            }
+            if (valid && sub == 0)
+            {
+                bool dup = false;
+#pragma unroll
+                for (int jj = 0; jj < kMaxBatchBlock; ++jj) dup |= cnt[jj] > 1;
+                if (dup)
+                {
+#pragma unroll
+                    for (int jj = 0; jj < kMaxBatchBlock; ++jj) m[jj] = 0.0;
+                    for (int64_t k = k0; k < k1; ++k)
+                    {
+                        const int64_t d = static_cast<int64_t>(ci[k]) - b0;
+#pragma unroll
+                        for (int jj = 0; jj < kMaxBatchBlock; ++jj)
+                            if (d == jj && jj < bs) m[jj] += static_cast<double>(v[k]);
+                    }
+                }
+#pragma unroll
+                for (int jj = 0; jj < kMaxBatchBlock; ++jj)
+                    if (jj < bs) tmp[row * bmax + jj] = m[jj];
+            }
+        }
+    }
+}
+
+// Inverse of an n x n (n <= N) block by Gauss-Jordan elimination without
+// pivoting (an SPD block has positive pivots); false when a pivot is not
+// finite or not above `threshold`.
+template <int N>
+__device__ __forceinline__ bool gauss_jordan_inverse(double (&m)[N][N], double (&w)[N][N], int n, double threshold)
+{
+    for (int k = 0; k < N; ++k)
+    {
+        if (k >= n) break;
+        const double piv = m[k][k];
+        if (!(isfinite(piv) && piv > threshold)) return false;
+        const double rp = 1.0 / piv;
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+        {
+            m[k][j] *= rp;
+            w[k][j] *= rp;
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+        {
+            if (i == k || i >= n) continue;
+            const double f = m[i][k];
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+            {
+                m[i][j] = __builtin_fma(-f, m[k][j], m[i][j]);
+                w[i][j] = __builtin_fma(-f, w[k][j], w[i][j]);
+            }
+        }
+    }
+    return true;
+}
+
+// One lane per block (the lane of its first row): the gathered block,
+// inverted in FP64, rounded to S into binv.  A pivot that is not finite or not
+// above 16 eps(S) times the block's largest magnitude rejects the block: its
+// rows get NaN and the system records its first row, as a rejected Jacobi
+// diagonal does.  BS > 0 handles the systems of that block size with the
+// block in registers; BS = 0 every other size above 1.
+template <typename S, int BS>
+__global__ __launch_bounds__(kBlock) void k_batch_block_invert(int64_t chunks, const int32_t* __restrict__ chunk_sys,
+                                                               const int64_t* __restrict__ sys_end,
+                                                               const int64_t* __restrict__ sys_chunk0,
+                                                               const int32_t* __restrict__ sys_bs,
+                                                               const double* __restrict__ tmp, S* __restrict__ binv,
+                                                               int bmax, BatchState* st)
+{
+    constexpr int N = BS > 0 ? BS : kMaxBatchBlock;
     const int lane = threadIdx.x % kWave;
     PM_FOR_CHUNKS(c, chunks, kBatchWaves)
     {
         const int32_t s = chunk_sys[c];
         if (s < 0) continue;
         const int bs = sys_bs[s];
+        if (bs <= 1 || (BS > 0 ? bs != BS : bs == kRegisterBlock)) continue;
         const int64_t row = c * kChunkRows + lane;
         const int64_t begin = sys_chunk0[s] * kChunkRows;
-        if (bs <= 1 || row >= sys_end[s] || (row - begin) % bs != 0) continue;
-        double m[kMaxBatchBlock][kMaxBatchBlock], w[kMaxBatchBlock][kMaxBatchBlock];
+        if (row >= sys_end[s] || (row - begin) % bs != 0) continue;
+        const int n = BS > 0 ? BS : bs;
+        double m[N][N], w[N][N];
         double scale = 0.0;
-        for (int i = 0; i < bs; ++i)
-        {
-            for (int j = 0; j < bs; ++j)
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j)
             {
-                m[i][j] = 0.0;
+                m[i][j] = i < n && j < n ? tmp[(row + i) * bmax + j] : 0.0;
                 w[i][j] = i == j ? 1.0 : 0.0;
+                scale = fmax(scale, fabs(m[i][j]));
             }
-            for (int64_t k = static_cast<int64_t>(ro[row + i]); k < static_cast<int64_t>(ro[row + i + 1]); ++k)
-            {
-                const int64_t col = static_cast<int64_t>(ci[k]);
-                if (col >= row && col < row + bs) m[i][col - row] += static_cast<double>(v[k]);
-            }
-            for (int j = 0; j < bs; ++j) scale = fmax(scale, fabs(m[i][j]));
-        }
         const double threshold = static_cast<double>(scalar_epsilon<S>()) * scale * 16.0;
-        bool ok = true;
-        for (int k = 0; k < bs && ok; ++k)
-        {
-            const double piv = m[k][k];
-            if (!(isfinite(piv) && piv > threshold))
+        bool ok = gauss_jordan_inverse<N>(m, w, n, threshold);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+#pragma unroll
+            for (int j = 0; j < N; ++j)
             {
-                ok = false;
-                break;
-            }
-            const double rp = 1.0 / piv;
-            for (int j = 0; j < bs; ++j)
-            {
-                m[k][j] *= rp;
-                w[k][j] *= rp;
-            }
-            for (int i = 0; i < bs; ++i)
-            {
-                if (i == k) continue;
-                const double f = m[i][k];
-                for (int j = 0; j < bs; ++j)
-                {
-                    m[i][j] = __builtin_fma(-f, m[k][j], m[i][j]);
-                    w[i][j] = __builtin_fma(-f, w[k][j], w[i][j]);
-                }
-            }
-        }
-        for (int i = 0; i < bs; ++i)
-            for (int j = 0; j < bs; ++j)
-            {
+                if (i >= n || j >= n) continue;
                 const bool fin = ok && isfinite(w[i][j]);
                 if (!fin) ok = false;
                 binv[(row + i) * bmax + j] = fin ? static_cast<S>(w[i][j]) : static_cast<S>(NAN);
@@ -797,6 +899,10 @@ struct BatchProblem
     const int32_t* sys_bs;
     int bmax;
     void* binv;
+    double* block_rows;               // the gathered diagonal-block rows (FP64, bmax per row)
+    // the Jacobi and block inverses of this matrix are in place from an
+    // earlier solve (the refinement loop's later inner solves)
+    bool inverses_ready;
 };
 
 template <typename S, typename I>
@@ -893,14 +999,20 @@ struct BatchCalls
 
     static void init(const BatchProblem& b, std::vector<KernelCall>& out)
     {
-        if (b.jacobi || (b.g && b.sys_jacobi))
+        if ((b.jacobi || (b.g && b.sys_jacobi)) && !b.inverses_ready)
             out.push_back(make_call(k_batch_jacobi_inverse<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
                                     b.chunk_sys, b.sys_end, ro(b.a), ci(b.a), static_cast<const S*>(b.a->values),
                                     static_cast<S*>(b.inv), b.st, b.g ? b.sys_jacobi : nullptr, b.sys_bs));
-        if (b.sys_bs)
-            out.push_back(make_call(k_batch_block_inverse<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
+        if (b.sys_bs && !b.inverses_ready)
+        {
+            out.push_back(make_call(k_batch_block_gather<S, I>, dim3(b.grid_chunks), dim3(kBlock), b.chunks,
                                     b.chunk_sys, b.sys_end, b.sys_chunk0, b.sys_bs, ro(b.a), ci(b.a),
-                                    static_cast<const S*>(b.a->values), static_cast<S*>(b.binv), b.bmax, b.st));
+                                    static_cast<const S*>(b.a->values), b.block_rows, b.bmax));
+            for (const auto k : {k_batch_block_invert<S, kRegisterBlock>, k_batch_block_invert<S, 0>})
+                out.push_back(make_call(k, dim3(b.grid_chunks), dim3(kBlock), b.chunks, b.chunk_sys, b.sys_end,
+                                        b.sys_chunk0, b.sys_bs, static_cast<const double*>(b.block_rows),
+                                        static_cast<S*>(b.binv), b.bmax, b.st));
+        }
         list(b, out);
         out.push_back(residual<true>(b, b.p0));
         block_apply(b, out);
@@ -1069,7 +1181,8 @@ std::string batch_key(const BatchProblem& b, int what, int64_t first, int64_t co
                   (b.split ? 1 : 0) | (b.fuse_step ? 2 : 0) | (b.long_lanes << 2),
                   static_cast<const void*>(b.sys_jacobi));
     std::string key(buf);
-    std::snprintf(buf, sizeof(buf), "|blocks|%p|%d|%p", static_cast<const void*>(b.sys_bs), b.bmax, b.binv);
+    std::snprintf(buf, sizeof(buf), "|blocks|%p|%d|%p|%p|%d", static_cast<const void*>(b.sys_bs), b.bmax, b.binv,
+                  static_cast<void*>(b.block_rows), b.inverses_ready ? 1 : 0);
     key += buf;
     if (b.g)
     {
@@ -1165,7 +1278,7 @@ int batch_solve(BatchProblem& b, const BatchPolicy& policy, pm_batch_report* rep
                 hipStream_t stream)
 {
     hipLaunchKernelGGL(k_batch_state_init, dim3(static_cast<int>(std::min<int64_t>(ceil_div(b.systems, kBlock), 1024))),
-                       dim3(kBlock), 0, stream, b.systems, b.st, b.done_count);
+                       dim3(kBlock), 0, stream, b.systems, b.st, b.done_count, b.inverses_ready ? 1 : 0);
     PM_HIP(hipGetLastError());
     PM_TRY(batch_plan<S>(ws, b, 0, 0, 0, 0)->run(stream, false));
     if (b.long_lanes == 0) PM_TRY(tune_long_lanes<S>(b, stream));
@@ -1370,10 +1483,12 @@ int batch_attach_blocks(BatchProblem& b, const int64_t* row_begin, const int64_t
     const size_t es = b.a->dtype == PM_F64 ? 8 : 4;
     const size_t rows = static_cast<size_t>(std::max<int64_t>(b.chunks, 1)) * kChunkRows;
     const size_t sizes_bytes = al(sizeof(int32_t) * static_cast<size_t>(b.systems));
-    PM_TRY(ws->blocks.reserve(sizes_bytes + al(es * rows * static_cast<size_t>(bmax))));
+    const size_t inv_bytes = al(es * rows * static_cast<size_t>(bmax));
+    PM_TRY(ws->blocks.reserve(sizes_bytes + inv_bytes + al(sizeof(double) * rows * static_cast<size_t>(bmax))));
     char* base = ws->blocks.as<char>();
     b.sys_bs = reinterpret_cast<const int32_t*>(base);
     b.binv = base + sizes_bytes;
+    b.block_rows = reinterpret_cast<double*>(base + sizes_bytes + inv_bytes);
     b.bmax = bmax;
     // the host array is consumed by the first kernels: copy it synchronously
     PM_HIP(hipStreamSynchronize(stream));
@@ -1624,6 +1739,8 @@ int batch_refine(BatchProblem& b, const double* rhs, double* x, double target, i
                            b.sys_end, r64, rs, rf, d);
         PM_HIP(hipGetLastError());
         PM_TRY(batch_solve<float>(b, pol, inner.data(), ws, stream));
+        // later inner solves reuse this matrix's Jacobi and block inverses
+        b.inverses_ready = true;
         for (int64_t s = 0; s < systems; ++s)
             if (host[s].active)
             {
