@@ -5,6 +5,7 @@
 #include <climits>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 
 #include "dia.hpp"
 
@@ -48,9 +49,9 @@ __device__ __forceinline__ bool dia_insert(int* slots, int off, bool shared_mem)
 // the diagonal entries, rejected when missing, non-finite or not above
 // 16 eps(S) max|a_row|, reciprocal in FP64 rounded to S): UNIT asks whether it
 // is exactly 1 in every row, i.e. whether z = D^-1 r is r itself.
-template <typename S, bool UNIT>
-__global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t rows, const int32_t* __restrict__ ro,
-                                                      const int32_t* __restrict__ ci, const S* __restrict__ v,
+template <typename S, bool UNIT, typename I>
+__global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t rows, const I* __restrict__ ro,
+                                                      const I* __restrict__ ci, const S* __restrict__ v,
                                                       int* table)
 {
     __shared__ int local[kMaxDia];
@@ -62,13 +63,13 @@ __global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t rows, const int32
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
     for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += stride)
     {
-        int prev = -1;
-        const int end = ro[r + 1];
+        int64_t prev = -1;
+        const int64_t end = static_cast<int64_t>(ro[r + 1]);
         double diag = 0.0, scale = 0.0;
         bool found = false;
-        for (int k = ro[r]; k < end; ++k)
+        for (int64_t k = static_cast<int64_t>(ro[r]); k < end; ++k)
         {
-            const int c = ci[k];
+            const int64_t c = static_cast<int64_t>(ci[k]);
             if (c <= prev) my_flags |= 1;
             prev = c;
             if (!dia_insert(local, static_cast<int>(c - r), true)) my_flags |= 2;
@@ -101,9 +102,9 @@ __global__ __launch_bounds__(kBlock) void k_dia_detect(int64_t rows, const int32
     if (threadIdx.x == 0 && flags) atomicOr(&table[kMaxDia], flags);
 }
 
-template <typename S>
-__global__ __launch_bounds__(kBlock) void k_dia_fill(int64_t rows, const int32_t* __restrict__ ro,
-                                                    const int32_t* __restrict__ ci, const S* __restrict__ v,
+template <typename S, typename I>
+__global__ __launch_bounds__(kBlock) void k_dia_fill(int64_t rows, const I* __restrict__ ro,
+                                                    const I* __restrict__ ci, const S* __restrict__ v,
                                                     DiaArgs dia, S* __restrict__ dv, uint16_t* __restrict__ mask)
 {
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -111,10 +112,10 @@ __global__ __launch_bounds__(kBlock) void k_dia_fill(int64_t rows, const int32_t
     {
         unsigned m = 0;
         int d = 0;
-        const int end = ro[r + 1];
-        for (int k = ro[r]; k < end; ++k)
+        const int64_t end = static_cast<int64_t>(ro[r + 1]);
+        for (int64_t k = static_cast<int64_t>(ro[r]); k < end; ++k)
         {
-            const int off = static_cast<int>(ci[k] - r);
+            const int off = static_cast<int>(static_cast<int64_t>(ci[k]) - r);
             while (d < dia.ndiag && dia.off[d] < off) ++d;  // columns increase, so d only moves forward
             if (d >= dia.ndiag || dia.off[d] != off) continue;  // cannot happen after detection
             dv[static_cast<int64_t>(d) * rows + r] = v[k];
@@ -205,25 +206,34 @@ int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_
     *out = DiaLayout{};
     if (unit_diagonal) *unit_diagonal = false;
     const int64_t n = a->rows;
-    if (n <= 0 || a->nnz <= 0 || !a->row_offsets32 || !a->col_indices32 || a->rows != a->cols) return PM_OK;
+    const bool compact = a->row_offsets32 && a->col_indices32;
+    if (n <= 0 || a->nnz <= 0 || a->rows != a->cols || n >= (int64_t(1) << 31)) return PM_OK;
+    if (!compact && (!a->row_offsets || !a->col_indices)) return PM_OK;
     const size_t es = a->dtype == PM_F64 ? 8 : 4;
     const size_t table_bytes = 256;
     PM_TRY(storage.reserve(table_bytes));
     int* table = storage.as<int>();
     hipLaunchKernelGGL(k_dia_reset, dim3(1), dim3(64), 0, stream, table);
     const int grid = static_cast<int>(std::min<int64_t>(ceil_div(n, kBlock), 2048));
-    if (a->dtype == PM_F64)
-    {
-        const auto detect = unit_diagonal ? k_dia_detect<double, true> : k_dia_detect<double, false>;
-        hipLaunchKernelGGL(detect, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32, a->col_indices32,
-                           static_cast<const double*>(a->values), table);
-    }
-    else
-    {
-        const auto detect = unit_diagonal ? k_dia_detect<float, true> : k_dia_detect<float, false>;
-        hipLaunchKernelGGL(detect, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32, a->col_indices32,
-                           static_cast<const float*>(a->values), table);
-    }
+    // the layout reads A through its 32-bit mirrors when it has them, else
+    // through the 64-bit arrays
+    auto detect_with = [&](auto ro, auto ci) {
+        using I = std::remove_cv_t<std::remove_pointer_t<decltype(ro)>>;
+        if (a->dtype == PM_F64)
+        {
+            const auto detect = unit_diagonal ? k_dia_detect<double, true, I> : k_dia_detect<double, false, I>;
+            hipLaunchKernelGGL(detect, dim3(grid), dim3(kBlock), 0, stream, n, ro, ci,
+                               static_cast<const double*>(a->values), table);
+        }
+        else
+        {
+            const auto detect = unit_diagonal ? k_dia_detect<float, true, I> : k_dia_detect<float, false, I>;
+            hipLaunchKernelGGL(detect, dim3(grid), dim3(kBlock), 0, stream, n, ro, ci,
+                               static_cast<const float*>(a->values), table);
+        }
+    };
+    if (compact) detect_with(static_cast<const int32_t*>(a->row_offsets32), static_cast<const int32_t*>(a->col_indices32));
+    else detect_with(static_cast<const int64_t*>(a->row_offsets), static_cast<const int64_t*>(a->col_indices));
     PM_HIP(hipGetLastError());
     int host[kMaxDia + 1];
     PM_HIP(hipMemcpyAsync(host, table, sizeof(host), hipMemcpyDeviceToHost, stream));
@@ -248,13 +258,17 @@ int dia_build(DeviceBuffer& storage, const pm_csr* a, DiaLayout* out, hipStream_
     char* base = storage.as<char>();
     void* dv = base + table_bytes;
     uint16_t* mask = reinterpret_cast<uint16_t*>(base + table_bytes + values_bytes);
-    if (a->dtype == PM_F64)
-        hipLaunchKernelGGL(k_dia_fill<double>, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32,
-                           a->col_indices32, static_cast<const double*>(a->values), args, static_cast<double*>(dv),
-                           mask);
-    else
-        hipLaunchKernelGGL(k_dia_fill<float>, dim3(grid), dim3(kBlock), 0, stream, n, a->row_offsets32,
-                           a->col_indices32, static_cast<const float*>(a->values), args, static_cast<float*>(dv), mask);
+    auto fill_with = [&](auto ro, auto ci) {
+        using I = std::remove_cv_t<std::remove_pointer_t<decltype(ro)>>;
+        if (a->dtype == PM_F64)
+            hipLaunchKernelGGL((k_dia_fill<double, I>), dim3(grid), dim3(kBlock), 0, stream, n, ro, ci,
+                               static_cast<const double*>(a->values), args, static_cast<double*>(dv), mask);
+        else
+            hipLaunchKernelGGL((k_dia_fill<float, I>), dim3(grid), dim3(kBlock), 0, stream, n, ro, ci,
+                               static_cast<const float*>(a->values), args, static_cast<float*>(dv), mask);
+    };
+    if (compact) fill_with(static_cast<const int32_t*>(a->row_offsets32), static_cast<const int32_t*>(a->col_indices32));
+    else fill_with(static_cast<const int64_t*>(a->row_offsets), static_cast<const int64_t*>(a->col_indices));
     PM_HIP(hipGetLastError());
     out->streams = args.ndiag;
     if (mirror)

@@ -48,8 +48,9 @@ struct DiaLayout
     int streams = 0;                // value arrays the row sums read
 };
 
-// Builds the layout into `storage` when the square CSR `a` (int32
-// mirrors present) has at most kMaxDia distinct diagonals, strictly
+// Builds the layout into `storage` when the square CSR `a` (read through its
+// int32 mirrors when present, else its int64 arrays; fewer than 2^31 rows)
+// has at most kMaxDia distinct diagonals, strictly
 // increasing columns in every row, and the diagonals' slots are at least
 // 1/kDiaMaxFill full; otherwise out->active stays false.  Synchronises
 // `stream` once to read the detected diagonals.  With `unit_diagonal`, the

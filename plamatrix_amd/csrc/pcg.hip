@@ -1586,7 +1586,7 @@ bool dia_wanted(const pm_csr* a)
 {
     const char* env = std::getenv("PLAMATRIX_AMD_DIA");
     if (env != nullptr && env[0] == '0') return false;
-    return use_compact(a) && a->rows >= 65536 && a->nnz <= static_cast<int64_t>(kMaxDia) * a->rows;
+    return a->rows >= 65536 && a->rows < (int64_t(1) << 31) && a->nnz <= static_cast<int64_t>(kMaxDia) * a->rows;
 }
 
 // Every operand address and size a cached plan's kernel arguments capture.

@@ -300,3 +300,23 @@ def test_one_kernel_iteration_is_bitwise(dev, monkeypatch, kind):
         out[fused] = (rep.converged, rep.iterations, rep.final_residual, host(x).tobytes())
     assert out["0"][0] and out["0"][1] > 50
     assert out["0"] == out["1"]
+
+
+def test_layout_from_64bit_indices(dev, monkeypatch):
+    # a CSR with only 64-bit index arrays (the output of cooToCsr) takes the
+    # layout and the one-kernel iteration too, with the CSR kernels' bits
+    h = generators.laplacian2d_csr(300, dtype=np.float64)  # 90,000 rows, diagonal 4: Jacobi kernels
+    a = capi.to_device_csr(h, compact=False)
+    assert a.row_offsets32 is None
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=51).astype(np.float64)).to(dev)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_DIA", flag)
+        ws = capi.Workspace()
+        x = torch.zeros_like(b)
+        rep = capi.pcg(a, b, x, ws, relative_tolerance=1e-8, max_iterations=4000)
+        torch.cuda.synchronize()
+        out[flag] = (rep.converged, rep.iterations, rep.final_residual, host(x).tobytes())
+        if flag == "1":
+            assert ws.solver_diagonals == 5 and ws.solver_value_streams == 3 and ws.solver_form == 1
+    assert out["0"][0] and out["0"] == out["1"]
