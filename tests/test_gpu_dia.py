@@ -320,3 +320,42 @@ def test_layout_from_64bit_indices(dev, monkeypatch):
         if flag == "1":
             assert ws.solver_diagonals == 5 and ws.solver_value_streams == 3 and ws.solver_form == 1
     assert out["0"][0] and out["0"] == out["1"]
+
+
+def outcome(fn):
+    try:
+        rep, x = fn()
+        return ("ok", rep.converged, rep.iterations, rep.final_residual, host(x).tobytes())
+    except capi.PlamatrixError as e:
+        torch.cuda.synchronize()
+        return ("error", e.kind, str(e))
+
+
+@pytest.mark.parametrize("shift", [3.9, 3.0])
+def test_restart_hands_over_identically(dev, monkeypatch, shift):
+    # an indefinite matrix (2-D Laplacian minus shift * I) makes p.Ap
+    # non-positive: the single-reduction iteration restarts at iteration 1 and
+    # after its chunks in flight the solve continues in the three-kernel form
+    # from the vectors the one-kernel iteration left in its first buffers,
+    # which reports the breakdown (at iteration 21; the three-kernel form alone
+    # stops at 2); one and two kernels per iteration must end identically
+    import scipy.sparse as sp
+
+    h = generators.laplacian2d_csr(300, dtype=np.float64)
+    m = sp.csr_matrix((h.values, h.col_indices, h.row_offsets), shape=(h.rows, h.rows)) - shift * sp.identity(h.rows)
+    a = capi.to_device_csr(from_scipy(m, np.float64), compact=True)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=53).astype(np.float64)).to(dev)
+    got = {}
+    for fused in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_CG_FUSED", fused)
+
+        def run():
+            ws = capi.Workspace()
+            x = torch.zeros_like(b)
+            rep = capi.pcg(a, b, x, ws, capi.policy(refresh_interval=50), relative_tolerance=1e-8,
+                           max_iterations=3000)
+            torch.cuda.synchronize()
+            return rep, x
+
+        got[fused] = outcome(run)
+    assert got["0"] == got["1"]
