@@ -247,6 +247,7 @@ def run_diffusion(args, torch, capi, generators):
             return capi.pcg_refine_factored(a, b, x, fg, ws, target=1e-6, max_outer=10, pol=pol, **inner)
 
         l_ms, l_rep = timed(learned_solve)
+        l_form = ws.solver_form
         l_relres = capi.true_relres(a, b, x, ws)
         torch.cuda.synchronize()
         f0 = time.perf_counter()
@@ -263,6 +264,7 @@ def run_diffusion(args, torch, capi, generators):
                    "inner_iterations": l_rep.inner_iterations, "true_relres": l_relres,
                    "forward_ms": round(fwd_ms, 3), "graph_build_ms": round(graph_ms, 3),
                    "ms_per_iteration": round((l_ms - fwd_ms) / max(l_rep.inner_iterations, 1), 4),
+                   "kernels_per_iteration": {5: 2, 4: 3, 3: 4}.get(l_form),
                    "time_ratio_vs_jacobi": round(total_ms / ms_per_solve, 4),
                    "iteration_ratio_vs_jacobi": round(l_rep.inner_iterations / max(iterations, 1), 4),
                    "fewer_iterations_than_jacobi": l_rep.inner_iterations < iterations,

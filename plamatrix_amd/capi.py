@@ -361,7 +361,19 @@ def _report(r: pm_solver_report) -> Report:
     return Report(bool(r.converged), r.iterations, r.initial_residual, r.final_residual)
 
 
+def _system_vectors(a, *vs):
+    """The solve entry points read and write vectors of A's dtype and row
+    count through raw pointers: refuse any other before the call."""
+    for v in vs:
+        if v is None:
+            continue
+        if v.dtype != a.values.dtype or v.numel() < a.rows or not v.is_contiguous():
+            raise ValueError(f"solve vector must be a contiguous {a.values.dtype} vector of {a.rows} entries, "
+                             f"got {v.dtype} with shape {tuple(v.shape)}")
+
+
 def _solve(fn, a, b, x, ws, pol, extra=(), **opts):
+    _system_vectors(a, b, x)
     d = a.descriptor()
     o = options(**opts)
     rep = pm_solver_report()
@@ -859,6 +871,7 @@ def fsai_train_step(net: FsaiNet, graph: FsaiGraph, probes: int = 8, seed: int =
 
 
 def pcg_factored(a: "DeviceCSR", b, x, graph: FsaiGraph, ws: Workspace, pol=None, **opts) -> "Report":
+    _system_vectors(a, b, x)
     d = a.descriptor()
     g, gt = graph.factors()
     o = options(**opts)

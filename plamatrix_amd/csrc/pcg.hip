@@ -1120,6 +1120,266 @@ __global__ __launch_bounds__(kDirBlock) void k_cg_fused_dia(int64_t rows, DiaArg
     publish_partials<4>(part, part_out);
 }
 
+// ---- single-reduction iteration with a factored preconditioner ---------------------
+//
+// PCG with M^-1 = G^T G (G lower, both factors and A on the diagonal layout)
+// as the single-reduction recurrence of k_cg_step / k_cg_spmv with u = G^T G r,
+// three kernels per iteration:
+//   k_cgf_step_lower  decision; p, s, x, r of the own rows; t = G r_new, each
+//                     entry's r_new formed from that row's old r, w, s with the
+//                     fused multiply-adds of its owner
+//   k_cgf_upper       u = G^T t
+//   k_cg_spmv_dia     w = A u and the partials r.r, r.u, w.u, u.s
+// r, w, s and u alternate between two buffers with the iteration parity as in
+// k_cg_fused_dia.
+
+// The operands one lane of k_cgf_step_lower loads for one row: G's row and,
+// per diagonal, the old r, w, s of the column's row, plus the row's own
+// vectors (none of the addresses depend on alpha or beta).
+template <typename S, int ND>
+struct CgfRow
+{
+    static constexpr int kSlots = ND > 0 ? ND : kMaxDia;
+    S val[kSlots], nr[kSlots], nw[kSlots], ns[kSlots];
+    S r, s, w, p, x, u;
+    unsigned m;
+
+    __device__ __forceinline__ void load(int64_t i, int64_t rows, const DiaArgs& g, const S* __restrict__ gv,
+                                         const uint16_t* __restrict__ gmask, const CgVectors<S>& v)
+    {
+        const int nd = ND > 0 ? ND : g.ndiag;
+        m = gmask[i];
+#pragma unroll
+        for (int d = 0; d < kSlots; ++d)
+        {
+            if (d < nd)
+            {
+                int64_t c = i + g.off[d];
+                c = c < 0 ? 0 : (c >= rows ? rows - 1 : c);
+                val[d] = dia_value(i, rows, g, gv, d);
+                nr[d] = v.r_in[c];
+                nw[d] = v.w_in[c];
+                ns[d] = v.s_in[c];
+            }
+        }
+        r = v.r_in[i];
+        s = v.s_in[i];
+        w = v.w_in[i];
+        p = v.p[i];
+        x = v.x[i];
+        u = v.u_in[i];
+    }
+
+    // The row of t = G r_new, its present entries in increasing column order.
+    __device__ __forceinline__ S product(S a, S b, const DiaArgs& g) const
+    {
+        const int nd = ND > 0 ? ND : g.ndiag;
+        S acc = S(0);
+#pragma unroll
+        for (int d = 0; d < kSlots; ++d)
+            if (d < nd && ((m >> d) & 1u)) acc = fma_s(val[d], fma_s(-a, fma_s(b, ns[d], nw[d]), nr[d]), acc);
+        return acc;
+    }
+};
+
+template <typename S, int ND>
+__global__ __launch_bounds__(kDirBlock) void k_cgf_step_lower(int64_t rows, DiaArgs g, const S* __restrict__ gv,
+                                                              const uint16_t* __restrict__ gmask, CgVectors<S> v,
+                                                              S* __restrict__ t, SolverState* st,
+                                                              const double* part_in, int par, int count)
+{
+    if (st->done != kRunning) return;
+    constexpr int F = ND > 0 ? kDiaInFlight : 1;
+    const int lane = threadIdx.x % kWave;
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    CgfRow<S, ND> row[F];
+    int64_t rw[F];
+    bool ok[F];
+    auto load_pass = [&](int64_t c0) {
+#pragma unroll
+        for (int k = 0; k < F; ++k)
+        {
+            const int64_t c = c0 + k * kDirWaves;
+            const int64_t i = c * kChunkRows + lane;
+            ok[k] = c < cend && i < rows;
+            rw[k] = ok[k] ? i : rows - 1;
+            row[k].load(rw[k], rows, g, gv, gmask, v);
+        }
+    };
+    int64_t c0 = cbegin + threadIdx.x / kWave;
+    load_pass(c0);
+    double alpha_d = 0.0, beta_d = 0.0;
+    if (cg_decision<S, kPreJacobi>(st, part_in, par, count, alpha_d, beta_d) != kRunning) return;
+    const S a = static_cast<S>(alpha_d), b = static_cast<S>(beta_d);
+    while (c0 < cend)
+    {
+#pragma unroll
+        for (int k = 0; k < F; ++k)
+        {
+            if (!ok[k]) continue;
+            const CgfRow<S, ND>& o = row[k];
+            const int64_t i = rw[k];
+            const S tr = o.product(a, b, g);
+            const S pn = fma_s(b, o.p, o.u);
+            const S sn = fma_s(b, o.s, o.w);
+            v.p[i] = pn;
+            v.x[i] = fma_s(a, pn, o.x);
+            v.s_out[i] = sn;
+            v.r_out[i] = fma_s(-a, sn, o.r);
+            t[i] = tr;
+        }
+        c0 += F * kDirWaves;
+        if (c0 < cend) load_pass(c0);
+    }
+}
+
+// u = G^T t (the transpose factor on its diagonal layout).
+template <typename S, int ND>
+__global__ __launch_bounds__(kDirBlock) void k_cgf_upper(int64_t rows, DiaArgs gt, const S* __restrict__ gtv,
+                                                         const uint16_t* __restrict__ gtmask,
+                                                         const S* __restrict__ t, S* __restrict__ u,
+                                                         const SolverState* st)
+{
+    if (st->done != kRunning) return;
+    const int lane = threadIdx.x % kWave;
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    for (int64_t c0 = cbegin + threadIdx.x / kWave; c0 < cend; c0 += kDiaInFlight * kDirWaves)
+    {
+        int64_t rw[kDiaInFlight];
+        bool ok[kDiaInFlight];
+        S acc[kDiaInFlight];
+#pragma unroll
+        for (int k = 0; k < kDiaInFlight; ++k)
+        {
+            const int64_t c = c0 + k * kDirWaves;
+            const int64_t i = c * kChunkRows + lane;
+            ok[k] = c < cend && i < rows;
+            rw[k] = ok[k] ? i : rows - 1;
+            acc[k] = dia_row<S, ND>(rw[k], rows, gt, gtv, gtmask[rw[k]], t);
+        }
+#pragma unroll
+        for (int k = 0; k < kDiaInFlight; ++k)
+            if (ok[k]) u[rw[k]] = acc[k];
+    }
+}
+
+// k_cgf_step_lower and k_cgf_upper as one kernel for a four-diagonal G:
+// u_i = sum_d G[k_d, i] t_{k_d} over the rows k_d = i - off_d that hold an
+// entry in column i, each t_k = G r_new of row k recomputed from G's row k and
+// the old r, w, s of its columns.  t never reaches memory and G^T is not read;
+// the sums run in the order of k_factor_lower_dia on G (t) and of
+// k_cgf_upper on G^T (u, increasing column k), so the bits are those of the
+// two-kernel pair.
+template <typename S>
+struct CgfPairRow
+{
+    static constexpr int D = 4;
+    S gv[D][D];                      // G[k_d, k_d + off_e]
+    S nr[D][D], nw[D][D], ns[D][D];  // old r, w, s of row k_d + off_e (e != d)
+    unsigned km[D];                  // presence bits of row k_d (none past the last row)
+    S r, s, w, p, x, u;
+
+    __device__ __forceinline__ void load(int64_t i, int64_t rows, const DiaArgs& g, const S* __restrict__ gvals,
+                                         const uint16_t* __restrict__ gmask, const CgVectors<S>& v)
+    {
+#pragma unroll
+        for (int d = 0; d < D; ++d)
+        {
+            const int64_t k = i - g.off[d];
+            const bool in = k >= 0 && k < rows;
+            const int64_t kc = in ? k : (k < 0 ? 0 : rows - 1);
+            km[d] = in ? gmask[kc] : 0u;
+#pragma unroll
+            for (int e = 0; e < D; ++e)
+            {
+                gv[d][e] = dia_value(kc, rows, g, gvals, e);
+                if (e == d) continue;  // column i: the row's own r, w, s
+                int64_t c = kc + g.off[e];
+                c = c < 0 ? 0 : (c >= rows ? rows - 1 : c);
+                nr[d][e] = v.r_in[c];
+                nw[d][e] = v.w_in[c];
+                ns[d][e] = v.s_in[c];
+            }
+        }
+        r = v.r_in[i];
+        s = v.s_in[i];
+        w = v.w_in[i];
+        p = v.p[i];
+        x = v.x[i];
+        u = v.u_in[i];
+    }
+
+    __device__ __forceinline__ S product(S a, S b) const
+    {
+        const S rn_own = fma_s(-a, fma_s(b, s, w), r);
+        S acc = S(0);
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d)
+        {
+            if (!((km[d] >> d) & 1u)) continue;  // no G[k_d, i]
+            S t = S(0);
+#pragma unroll
+            for (int e = 0; e < D; ++e)
+            {
+                if (!((km[d] >> e) & 1u)) continue;
+                const S rn = e == d ? rn_own : fma_s(-a, fma_s(b, ns[d][e], nw[d][e]), nr[d][e]);
+                t = fma_s(gv[d][e], rn, t);
+            }
+            acc = fma_s(gv[d][d], t, acc);
+        }
+        return acc;
+    }
+};
+
+template <typename S>
+__global__ __launch_bounds__(kDirBlock) void k_cgf_step_pair(int64_t rows, DiaArgs g, const S* __restrict__ gv,
+                                                             const uint16_t* __restrict__ gmask, CgVectors<S> v,
+                                                             SolverState* st, const double* part_in, int par,
+                                                             int count)
+{
+    if (st->done != kRunning) return;
+    const int lane = threadIdx.x % kWave;
+    const int64_t chunks = chunk_count(rows);
+    const int64_t per = ceil_div(chunks, static_cast<int64_t>(gridDim.x));
+    const int64_t cbegin = xcd_logical_block(blockIdx.x, gridDim.x) * per;
+    const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
+    CgfPairRow<S> row;
+    int64_t c0 = cbegin + threadIdx.x / kWave;
+    auto load_row = [&](int64_t c, int64_t& i) {
+        i = c * kChunkRows + lane;
+        const bool ok = c < cend && i < rows;
+        row.load(ok ? i : rows - 1, rows, g, gv, gmask, v);
+        return ok;
+    };
+    int64_t i;
+    bool ok = load_row(c0, i);
+    double alpha_d = 0.0, beta_d = 0.0;
+    if (cg_decision<S, kPreJacobi>(st, part_in, par, count, alpha_d, beta_d) != kRunning) return;
+    const S a = static_cast<S>(alpha_d), b = static_cast<S>(beta_d);
+    while (c0 < cend)
+    {
+        if (ok)
+        {
+            const S un = row.product(a, b);
+            const S pn = fma_s(b, row.p, row.u);
+            const S sn = fma_s(b, row.s, row.w);
+            v.p[i] = pn;
+            v.x[i] = fma_s(a, pn, row.x);
+            v.s_out[i] = sn;
+            v.r_out[i] = fma_s(-a, sn, row.r);
+            v.u_out[i] = un;
+        }
+        c0 += kDirWaves;
+        if (c0 < cend) ok = load_row(c0, i);
+    }
+}
+
 // The solver state into the host's fine-grained status slot, 8 bytes per lane.
 __global__ void k_publish_state(const SolverState* __restrict__ st, SolverState* host)
 {
@@ -1167,7 +1427,8 @@ struct Problem
     bool fuse_step;          // factored: the step folded into G r and G^T t
     int fuse_step_env;       // PLAMATRIX_AMD_FUSE_STEP: 0 off, 1 on, -1 unset
     bool single_reduction;   // CG / Jacobi-PCG as the two-kernel Chronopoulos-Gear iteration
-    bool cg_fused;           // ... as one kernel per iteration on the diagonal layout
+    bool cg_fused;           // ... as one kernel per iteration on the diagonal layout (factored: G r and G^T t in one)
+    bool cg_factored;        // factored PCG as the three-kernel single-reduction iteration (diagonal layouts)
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     void *r2, *z2, *q2, *s2; // second buffers of r, u, w, s for the one-kernel iteration (or null)
@@ -1232,6 +1493,8 @@ struct Calls
             out.push_back(make_call(k, dim3(pb.grid_units), dim3(kBlock), pb.a->rows, static_cast<const S*>(pb.r),
                                     static_cast<const S*>(pb.inv), static_cast<S*>(pb.z), static_cast<S*>(pb.p1),
                                     static_cast<const SolverState*>(pb.st)));
+            // factored: u = G^T (G r)
+            if (pb.cg_factored) cg_factor_apply(pb, static_cast<const S*>(pb.r), static_cast<S*>(pb.z), out);
             // the initialisation acts as iteration -1: w = A u into the first
             // buffers, partials into region 1
             cg_spmv(pb, cg_vectors(pb, 1), 1, out);
@@ -1249,15 +1512,16 @@ struct Calls
     // them.  The two-kernel iteration works in place.
     static CgVectors<S> cg_vectors(const Problem& pb, int par)
     {
-        const bool jac = pb.pre == kPreJacobi;
+        const bool jac = pb.pre != kPreNone;
+        const bool alt = pb.cg_fused || pb.cg_factored;
         auto* r1 = static_cast<S*>(pb.r);
         auto* u1 = jac ? static_cast<S*>(pb.z) : r1;
         auto* w1 = static_cast<S*>(pb.q);
         auto* s1 = static_cast<S*>(pb.p1);
-        auto* r2 = pb.cg_fused ? static_cast<S*>(pb.r2) : r1;
-        auto* u2 = pb.cg_fused ? (jac ? static_cast<S*>(pb.z2) : r2) : u1;
-        auto* w2 = pb.cg_fused ? static_cast<S*>(pb.q2) : w1;
-        auto* s2 = pb.cg_fused ? static_cast<S*>(pb.s2) : s1;
+        auto* r2 = alt ? static_cast<S*>(pb.r2) : r1;
+        auto* u2 = alt ? (jac ? static_cast<S*>(pb.z2) : r2) : u1;
+        auto* w2 = alt ? static_cast<S*>(pb.q2) : w1;
+        auto* s2 = alt ? static_cast<S*>(pb.s2) : s1;
         CgVectors<S> v{};
         v.x = static_cast<S*>(pb.x);
         v.p = static_cast<S*>(pb.p0);
@@ -1279,7 +1543,8 @@ struct Calls
     // can switch forms between chunks) into region par.
     static void cg_spmv(const Problem& pb, const CgVectors<S>& v, int par, std::vector<KernelCall>& out)
     {
-        const bool jac = pb.pre == kPreJacobi;
+        // a preconditioned u (Jacobi or factored) is published next to r
+        const bool jac = pb.pre != kPreNone;
         const int nd = pb.dia.args.ndiag;
         double* part_out = cg_region(pb.partials, par);
         if (pb.dia.active)
@@ -1299,6 +1564,26 @@ struct Calls
                                 part_out));
     }
 
+    // u = G^T (G r) on the factors' diagonal layouts, t = G r in pb.inv.
+    static void cg_factor_apply(const Problem& pb, const S* r, S* u, std::vector<KernelCall>& out)
+    {
+        const int64_t rows = pb.a->rows;
+        auto* t = static_cast<S*>(pb.inv);
+        const int grid = static_cast<int>(std::min<int64_t>(full_grid(rows), kSpmvGrid));
+        out.push_back(make_call(pb.dia_g.args.ndiag == 4 ? k_factor_lower_dia<S, 4> : k_factor_lower_dia<S, 0>,
+                                dim3(grid), dim3(kBlock), rows, pb.dia_g.args, static_cast<const S*>(pb.dia_g.values),
+                                pb.dia_g.mask, r, t, static_cast<const SolverState*>(pb.st)));
+        cg_factor_upper(pb, u, out);
+    }
+
+    static void cg_factor_upper(const Problem& pb, S* u, std::vector<KernelCall>& out)
+    {
+        out.push_back(make_call(pb.dia_gt.args.ndiag == 4 ? k_cgf_upper<S, 4> : k_cgf_upper<S, 0>,
+                                dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows, pb.dia_gt.args,
+                                static_cast<const S*>(pb.dia_gt.values), pb.dia_gt.mask,
+                                static_cast<const S*>(pb.inv), u, static_cast<const SolverState*>(pb.st)));
+    }
+
     static void cg_iteration(const Problem& pb, int64_t t, bool refresh, std::vector<KernelCall>& out)
     {
         const int par = static_cast<int>(t % 2);
@@ -1306,6 +1591,37 @@ struct Calls
         const CgVectors<S> v = cg_vectors(pb, par);
         const double* part_in = cg_region(pb.partials, 1 - par);
         double* part_out = cg_region(pb.partials, par);
+        if (pb.cg_factored)
+        {
+            if (refresh)
+            {
+                // p, s, x of the step; r = b - A x; u = G^T (G r)
+                out.push_back(make_call(k_cg_step<S, kPreJacobi, false>, dim3(pb.grid_units), dim3(kBlock),
+                                        pb.a->rows, v, pb.st, part_in, par, pb.grid_units));
+                out.push_back(make_call(k_residual<S, I, false, false>, dim3(full_grid(pb.a->rows)), dim3(kBlock),
+                                        pb.a->rows, ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
+                                        static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x), v.r_out,
+                                        static_cast<S*>(nullptr), static_cast<S*>(nullptr), pb.st));
+                cg_factor_apply(pb, v.r_out, v.u_out, out);
+            }
+            else if (pb.cg_fused)
+            {
+                // G r and G^T t in one kernel
+                out.push_back(make_call(k_cgf_step_pair<S>, dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows,
+                                        pb.dia_g.args, static_cast<const S*>(pb.dia_g.values), pb.dia_g.mask, v,
+                                        pb.st, part_in, par, pb.grid_units));
+            }
+            else
+            {
+                const auto k = pb.dia_g.args.ndiag == 4 ? k_cgf_step_lower<S, 4> : k_cgf_step_lower<S, 0>;
+                out.push_back(make_call(k, dim3(pb.grid_units), dim3(kDirBlock), pb.a->rows, pb.dia_g.args,
+                                        static_cast<const S*>(pb.dia_g.values), pb.dia_g.mask, v,
+                                        static_cast<S*>(pb.inv), pb.st, part_in, par, pb.grid_units));
+                cg_factor_upper(pb, v.u_out, out);
+            }
+            cg_spmv(pb, v, par, out);
+            return;
+        }
         if (pb.cg_fused && !refresh)
         {
             const int nd = pb.dia.args.ndiag;
@@ -1520,13 +1836,16 @@ bool fuse_step_wanted(const Problem& pb)
 size_t elem_size(int32_t dtype) { return dtype == PM_F64 ? 8 : 4; }
 
 // CG and Jacobi-PCG run as the two-kernel single-reduction iteration in the
-// adaptive solves (PLAMATRIX_AMD_SINGLE_REDUCTION=0 keeps the three-kernel
-// form, which the fused-direction option also needs).
+// adaptive solves, factored PCG as the three-kernel one when A, G and G^T are
+// all on the diagonal layout (PLAMATRIX_AMD_SINGLE_REDUCTION=0 keeps the
+// classic forms, which the fused-direction option also needs).
 bool single_reduction_wanted(const Problem& pb)
 {
     const char* env = std::getenv("PLAMATRIX_AMD_SINGLE_REDUCTION");
     if (env != nullptr && env[0] == '0') return false;
-    return (pb.pre == kPreNone || pb.pre == kPreJacobi) && !pb.fuse_direction;
+    if (pb.fuse_direction) return false;
+    if (pb.pre == kPreFactored) return pb.dia.active && pb.dia_g.active && pb.dia_gt.active && pb.r2 != nullptr;
+    return pb.pre == kPreNone || pb.pre == kPreJacobi;
 }
 
 bool dia_wanted(const pm_csr* a);
@@ -1557,6 +1876,7 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     pb.q2 = second ? base + 8 * vec : nullptr;
     pb.s2 = second ? base + 9 * vec : nullptr;
     pb.cg_fused = false;
+    pb.cg_factored = false;
     pb.st = ws->state.as<SolverState>();
     pb.partials = ws->partials.as<double>();
     const int64_t units = pb.pre == kPreBlock ? n / pb.bsize : n;
@@ -1613,13 +1933,14 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
     char buf[512];
     const unsigned long long dia_sig =
         dia_signature(pb.dia) ^ (dia_signature(pb.dia_g) * 31ull) ^ (dia_signature(pb.dia_gt) * 131ull);
-    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%d|%lld|%lld|%d|%lld|%d|%d|%d|%llx|%p|%p", kind, pb.r,
+    std::snprintf(buf, sizeof(buf), "%s|%p|%p|%p|%p|%p|%p|%d|%lld|%lld|%d|%lld|%d|%d|%d|%llx|%p|%p|%d|%d", kind, pb.r,
                   static_cast<void*>(pb.st), static_cast<void*>(pb.partials), pb.rhs, pb.x, pb.inv_blocks, pb.pre,
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
                   static_cast<long long>(pb.bsize),
                   (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0) |
-                      (pb.single_reduction ? 16 : 0) | (pb.cg_fused ? 32 : 0),
-                  pb.dia.active ? 1 : 0, pb.dia.args.ndiag, dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask));
+                      (pb.single_reduction ? 16 : 0) | (pb.cg_fused ? 32 : 0) | (pb.cg_factored ? 64 : 0),
+                  pb.dia.active ? 1 : 0, pb.dia.args.ndiag, dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask),
+                  pb.grid_rows, pb.grid_units);
     std::string key(buf);
     append_matrix_key(key, pb.a);
     append_matrix_key(key, pb.gf);
@@ -1746,15 +2067,24 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     // where the solver grid allows (65,536 rows: 256 workgroups instead of
     // 64); the three-kernel form a restart hands over to uses the same grid
     if (pb.single_reduction) pb.grid_units = solver_grid_chunks(a->rows);
+    // the factored three-kernel form a restart hands over to reads the r.r,
+    // r.z partials with grid_rows workgroups
+    pb.cg_factored = pb.single_reduction && pb.pre == kPreFactored;
+    if (pb.cg_factored) pb.grid_rows = pb.grid_units;
     {
-        // on the diagonal layout, one kernel per iteration
-        // (PLAMATRIX_AMD_CG_FUSED=0 keeps two)
+        // on the diagonal layout, one kernel per iteration; factored with a
+        // four-diagonal G, G r and G^T t in one kernel (PLAMATRIX_AMD_CG_FUSED=0
+        // keeps two, and three)
         const char* env = std::getenv("PLAMATRIX_AMD_CG_FUSED");
-        pb.cg_fused = pb.single_reduction && pb.dia.active && pb.r2 != nullptr && (env == nullptr || env[0] != '0');
+        bool pair = pb.dia_g.active && pb.dia_g.args.ndiag == 4 && pb.dia_gt.args.ndiag == 4;
+        for (int d = 0; pair && d < 4; ++d)
+            pair = pb.dia_g.args.off[d] <= 0 && pb.dia_gt.args.off[3 - d] == -pb.dia_g.args.off[d];
+        pb.cg_fused = pb.single_reduction && (!pb.cg_factored || pair) && pb.dia.active && pb.r2 != nullptr &&
+                      (env == nullptr || env[0] != '0');
     }
     ws->last_diagonals = pb.dia.active ? pb.dia.args.ndiag : 0;
     ws->last_value_streams = pb.dia.active ? pb.dia.streams : 0;
-    ws->last_form = pb.cg_fused ? 1 : pb.single_reduction ? 2 : 3;
+    ws->last_form = pb.cg_factored ? (pb.cg_fused ? 5 : 4) : pb.cg_fused ? 1 : pb.single_reduction ? 2 : 3;
     ws->last_preconditioner = pb.pre;
     if (a->rows == 0)
     {
@@ -1805,7 +2135,8 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         if (fin.done != kRunning) break;
         // a single-reduction restart: the chunks still to be issued take the
         // three-kernel form (same state, parity slots, partials and vectors)
-        if (pb.single_reduction && fin.counters[kCounterRestart] != 0u) pb.single_reduction = false;
+        if (pb.single_reduction && fin.counters[kCounterRestart] != 0u)
+            pb.single_reduction = pb.cg_fused = pb.cg_factored = false;
         if (inflight.empty() && launched >= needed)
             return fail(PM_RUNTIME_ERROR, "iterative solver stopped without a final state");
     }

@@ -128,7 +128,10 @@ def test_refinement_with_dia_reaches_target(dev, monkeypatch):
 
 def test_learned_factors_on_the_diagonal_layout(dev, monkeypatch):
     # the factored preconditioner's G and G^T (4 diagonals each on a 3-D
-    # 7-point pattern) take the diagonal layout too; the solve stays bitwise
+    # 7-point pattern) take the diagonal layout too; the classic factored
+    # iteration stays bitwise (the single-reduction form needs all three
+    # matrices on the layout and has its own tests)
+    monkeypatch.setenv("PLAMATRIX_AMD_SINGLE_REDUCTION", "0")
     h = generators.diffusion3d_box_csr(48, 48, 48, seed=13, dtype=np.float32)
     a = capi.to_device_csr(h, compact=True)
     net = capi.FsaiNet(seed=0)
@@ -144,6 +147,7 @@ def test_learned_factors_on_the_diagonal_layout(dev, monkeypatch):
         torch.cuda.synchronize()
         grow[flag] = ws.capacity_bytes
         out[flag] = (rep, host(x).copy())
+        assert ws.solver_form == 3
     # A (7 diagonals) plus G and G^T (4 each)
     assert grow["1"] - grow["0"] >= (7 + 4 + 4) * h.rows * 4
     assert out["0"][0].converged and out["0"][0].iterations == out["1"][0].iterations

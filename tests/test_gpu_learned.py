@@ -237,7 +237,10 @@ def test_factored_pcg_matches_oracle(dev, ws):
 @pytest.mark.parametrize("dims", [(14, 12, 10), (48, 40, 40)])  # CSR factors; diagonal-layout factors
 def test_fused_step_is_bitwise(dev, monkeypatch, dtype, dims):
     # x += alpha p and r -= alpha q folded into the two factor products give
-    # the bits of the separate step kernel (PLAMATRIX_AMD_FUSE_STEP=0)
+    # the bits of the separate step kernel (PLAMATRIX_AMD_FUSE_STEP=0); the
+    # classic factored iteration, which the larger system would otherwise
+    # leave for the single-reduction one
+    monkeypatch.setenv("PLAMATRIX_AMD_SINGLE_REDUCTION", "0")
     h = system(*dims, seed=19, dtype=dtype)
     a = capi.to_device_csr(h, compact=True)
     net = capi.FsaiNet(seed=0)
@@ -258,6 +261,128 @@ def test_fused_step_is_bitwise(dev, monkeypatch, dtype, dims):
     assert r0.iterations == r1.iterations and r0.final_residual == r1.final_residual, (r0, r1)
     assert np.array_equal(x0, x1)
     assert r0.converged and r1.converged, (r0, r1)
+
+
+def trained_factors(h):
+    a = capi.to_device_csr(h, compact=True)
+    net = capi.FsaiNet(seed=0)
+    net.parameters = np.load(WEIGHTS)
+    graph = capi.FsaiGraph(a)
+    capi.fsai_forward(net, graph)
+    return a, graph
+
+
+@pytest.mark.parametrize("refresh", [0, 20])
+def test_factored_single_reduction_matches_oracle(dev, refresh):
+    # A, G and G^T on the diagonal layout (76,800 rows): the factored PCG runs
+    # as the single-reduction iteration, G r and G^T t in one kernel (solver
+    # form 5); FP64 against the oracle's classic factored PCG with the same
+    # factors
+    h = system(48, 40, 40, seed=31, dtype=np.float64)
+    a, graph = trained_factors(h)
+    g, _ = graph.factors()
+    gro, gci, gvals = capi.csr_arrays(g)
+    rows = np.repeat(np.arange(h.rows), np.diff(gro))
+    rhs = generators.rhs_normal(h.rows, seed=32).astype(np.float64)
+    x_ref, it_ref = fsai.factored_pcg(h.row_offsets, h.col_indices, h.values, rows, gci, gvals, rhs,
+                                      relative_tolerance=1e-10, max_iterations=3000)
+    b = torch.from_numpy(rhs).to(dev)
+    w = capi.Workspace()
+    x = torch.zeros_like(b)
+    rep = capi.pcg_factored(a, b, x, graph, w, capi.policy(refresh_interval=refresh),
+                            relative_tolerance=1e-10, max_iterations=3000)
+    torch.cuda.synchronize()
+    assert w.solver_form == 5, w.solver_form
+    # the recursive residual follows s = A p by recurrence, not the product
+    # the oracle forms each iteration: the stop test may fire a few
+    # iterations apart (247 vs 250 here); solution and true residual bind
+    assert rep.converged and abs(rep.iterations - it_ref) <= max(3, 0.02 * it_ref), (rep.iterations, it_ref)
+    got = x.cpu().numpy()
+    assert np.linalg.norm(got - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    assert capi.true_relres(a, b, x, w) <= 1e-9
+
+
+def test_factored_single_reduction_keeps_fp32_iterations(dev, monkeypatch):
+    # FP32 with residual refreshes: the single-reduction factored iteration
+    # ends within 3 % of the classic form's iteration count at the same
+    # true residual
+    h = system(48, 40, 40, seed=33, dtype=np.float32)
+    a, graph = trained_factors(h)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=34).astype(np.float32)).to(dev)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_SINGLE_REDUCTION", flag)
+        w = capi.Workspace()
+        x = torch.zeros_like(b)
+        rep = capi.pcg_factored(a, b, x, graph, w, capi.policy(refresh_interval=50), relative_tolerance=1e-5,
+                                max_iterations=4000)
+        torch.cuda.synchronize()
+        out[flag] = (rep, w.solver_form, capi.true_relres(a, b, x, w))
+    (r0, f0, t0), (r1, f1, t1) = out["0"], out["1"]
+    assert (f0, f1) == (3, 5)
+    assert r0.converged and r1.converged, (r0, r1)
+    assert abs(r1.iterations - r0.iterations) <= max(3, 0.03 * r0.iterations), (r0.iterations, r1.iterations)
+    assert t1 <= 5e-5 and t0 <= 5e-5, (t0, t1)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_factored_pair_kernel_is_bitwise(dev, monkeypatch, dtype):
+    # G r and G^T t in one kernel (each t of the four rows holding column i
+    # recomputed, G^T not read) against the kernel pair that writes t and
+    # reads G^T: same sums in the same order, so the same bits
+    h = system(48, 40, 40, seed=37, dtype=dtype)
+    a, graph = trained_factors(h)
+    b = torch.from_numpy(generators.rhs_normal(h.rows, seed=38).astype(dtype)).to(dev)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_CG_FUSED", flag)
+        w = capi.Workspace()
+        x = torch.zeros_like(b)
+        rep = capi.pcg_factored(a, b, x, graph, w, capi.policy(refresh_interval=20),
+                                relative_tolerance=1e-5 if dtype == np.float32 else 1e-10, max_iterations=4000)
+        torch.cuda.synchronize()
+        out[flag] = (w.solver_form, rep, x.cpu().numpy())
+    (f0, r0, x0), (f1, r1, x1) = out["0"], out["1"]
+    assert (f0, f1) == (4, 5)
+    assert r0.converged and r0.iterations == r1.iterations and r0.final_residual == r1.final_residual, (r0, r1)
+    assert np.array_equal(x0, x1)
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_factored_single_reduction_restart_hands_over(dev, monkeypatch, dims):
+    # an indefinite matrix (Laplacian minus a shift that leaves a diagonal of
+    # 0.1; 2-D: 90,000 rows, G with three diagonals, G r and G^T t in two
+    # kernels; 3-D: 110,592 rows, four diagonals, one kernel) makes p.Ap
+    # non-positive: the single-reduction factored iteration restarts and the
+    # solve continues in the classic factored form from the vectors and
+    # partials it left, which reports the breakdown like the classic form does
+    import scipy.sparse as sp
+
+    n = 300 if dims == 2 else 48
+    t = sp.diags([-1.0, 2.0, -1.0], [-1, 0, 1], shape=(n, n))
+    eye = sp.identity(n)
+    lap = sp.kron(t, eye) + sp.kron(eye, t) if dims == 2 else (
+        sp.kron(sp.kron(t, eye), eye) + sp.kron(sp.kron(eye, t), eye) + sp.kron(sp.kron(eye, eye), t))
+    m = (lap - (2.0 * dims - 0.1) * sp.identity(n ** dims)).tocsr()
+    m.sort_indices()
+    hm = generators.CSRHost(m.shape[0], m.shape[0], m.indptr.astype(np.int64), m.indices.astype(np.int64),
+                            m.data.astype(np.float64))
+    a, graph = trained_factors(hm)
+    b = torch.from_numpy(generators.rhs_normal(hm.rows, seed=35).astype(np.float64)).to(dev)
+    got = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("PLAMATRIX_AMD_SINGLE_REDUCTION", flag)
+        w = capi.Workspace()
+        x = torch.zeros_like(b)
+        try:
+            capi.pcg_factored(a, b, x, graph, w, capi.policy(refresh_interval=50), relative_tolerance=1e-8,
+                              max_iterations=3000)
+            got[flag] = ("ok",)
+        except capi.PlamatrixError as e:
+            got[flag] = ("error", e.kind, "not numerically SPD" in str(e))
+        torch.cuda.synchronize()
+        assert w.solver_form == ((4 if dims == 2 else 5) if flag == "1" else 3), w.solver_form
+    assert got["0"] == got["1"] == ("error", got["0"][1], True), got
 
 
 def test_training_improves_on_jacobi(dev, ws):
