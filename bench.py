@@ -60,6 +60,23 @@ def pcg_iteration_bytes(n: int, nnz: int, value_bytes: int = 4, index_bytes: int
     return nnz * (value_bytes + index_bytes) + (n + 1) * index_bytes + vectors * n * value_bytes
 
 
+def factored_iteration_bytes(n: int, form: int, value_streams: int, g_diagonals: int = 4, value_bytes: int = 4):
+    """Algorithmic bytes of one single-reduction factored iteration with A, G
+    and G^T on the diagonal layout (2-byte masks per row, neighbours counted
+    once), or None for another form.
+    4  step + G r reads G, r, w, s, p, x, u, writes p, x, s, r, t; G^T t reads
+       G^T, t, writes u; SpMV reads A's value arrays, u, r, s, writes w
+    5  step + G r + G^T t reads G, r, w, s, p, x, u, writes p, x, s, r, u
+       (t is recomputed, G^T not read); SpMV as in 4"""
+    g = g_diagonals * value_bytes + 2
+    spmv = value_streams * value_bytes + 2 + 4 * value_bytes
+    if form == 5:
+        return n * (g + 11 * value_bytes + spmv)
+    if form == 4:
+        return n * (g + 11 * value_bytes + g + 2 * value_bytes + spmv)
+    return None
+
+
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_spmv_diffusion128_traffic.txt")
 
 
@@ -248,6 +265,7 @@ def run_diffusion(args, torch, capi, generators):
 
         l_ms, l_rep = timed(learned_solve)
         l_form = ws.solver_form
+        l_bytes = factored_iteration_bytes(n, l_form, ws.solver_value_streams)
         l_relres = capi.true_relres(a, b, x, ws)
         torch.cuda.synchronize()
         f0 = time.perf_counter()
@@ -265,6 +283,7 @@ def run_diffusion(args, torch, capi, generators):
                    "forward_ms": round(fwd_ms, 3), "graph_build_ms": round(graph_ms, 3),
                    "ms_per_iteration": round((l_ms - fwd_ms) / max(l_rep.inner_iterations, 1), 4),
                    "kernels_per_iteration": {5: 2, 4: 3, 3: 4}.get(l_form),
+                   "iteration_bytes": l_bytes,
                    "time_ratio_vs_jacobi": round(total_ms / ms_per_solve, 4),
                    "iteration_ratio_vs_jacobi": round(l_rep.inner_iterations / max(iterations, 1), 4),
                    "fewer_iterations_than_jacobi": l_rep.inner_iterations < iterations,

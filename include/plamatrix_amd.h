@@ -91,13 +91,17 @@ int32_t pm_workspace_solver_diagonals(const pm_workspace* ws);
  * stencil reads 3 arrays instead of 7 (PLAMATRIX_AMD_DIA_MIRROR=0 reads every
  * diagonal); results are bitwise those of the CSR kernels. */
 int32_t pm_workspace_solver_value_streams(const pm_workspace* ws);
-/* No reference counterpart: kernels per iteration of the form the last
- * adaptive solve started in: 3 (direction, SpMV, update: block-Jacobi and the
- * factored preconditioner, or PLAMATRIX_AMD_SINGLE_REDUCTION=0), 2 (the
- * single-reduction iteration of CG and Jacobi-PCG: step, SpMV) or 1 (the
- * single-reduction iteration as one kernel on the diagonal layout;
- * PLAMATRIX_AMD_CG_FUSED=0 keeps two).  A solve whose single-reduction
- * denominator loses positivity continues in the three-kernel form. */
+/* No reference counterpart: the iteration form the last adaptive solve
+ * started in.  3: the classic form (direction, SpMV, update; block-Jacobi,
+ * factored factors off the diagonal layout, or
+ * PLAMATRIX_AMD_SINGLE_REDUCTION=0).  2: the single-reduction iteration of CG
+ * and Jacobi-PCG (step, SpMV).  1: that iteration as one kernel on the
+ * diagonal layout (PLAMATRIX_AMD_CG_FUSED=0 keeps 2).  4: the factored
+ * single-reduction iteration with A, G and G^T on the diagonal layout (step
+ * and G r, G^T t, SpMV).  5: the same with a four-diagonal lower G, G r and
+ * G^T t in one kernel (PLAMATRIX_AMD_CG_FUSED=0 keeps 4).  A solve whose
+ * single-reduction denominator loses positivity continues in the classic
+ * form. */
 int32_t pm_workspace_solver_form(const pm_workspace* ws);
 /* No reference counterpart: the preconditioner the last adaptive solve ran
  * (0: none, 1: Jacobi, 2: block-Jacobi, 3: factored G^T G).  A Jacobi solve of

@@ -204,7 +204,9 @@ class Workspace:
 
     @property
     def solver_form(self) -> int:
-        """Kernels per iteration of the form the last solve started in (1, 2, 3)."""
+        """Iteration form the last solve started in: 3 classic, 2 / 1 single-reduction
+        CG or Jacobi-PCG in two kernels / one, 4 / 5 single-reduction factored PCG in
+        three kernels / two."""
         return int(lib().pm_workspace_solver_form(self.handle))
 
     @property

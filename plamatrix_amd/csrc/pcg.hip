@@ -1337,6 +1337,10 @@ struct CgfPairRow
     }
 };
 
+// One row per lane and pass; the first pass's loads are issued before the
+// decision.  (Two rows in flight need 170 VGPRs and ran slower: 63.4 against
+// 55.9 us per 128^3 learned iteration; so did a variant that shares the
+// neighbouring rows' r and t across lanes by wavefront shuffles, 62.6 us.)
 template <typename S>
 __global__ __launch_bounds__(kDirBlock) void k_cgf_step_pair(int64_t rows, DiaArgs g, const S* __restrict__ gv,
                                                              const uint16_t* __restrict__ gmask, CgVectors<S> v,
@@ -1351,6 +1355,7 @@ __global__ __launch_bounds__(kDirBlock) void k_cgf_step_pair(int64_t rows, DiaAr
     const int64_t cend = cbegin + per < chunks ? cbegin + per : chunks;
     CgfPairRow<S> row;
     int64_t c0 = cbegin + threadIdx.x / kWave;
+    // rows past the end load row rows - 1 and are dropped
     auto load_row = [&](int64_t c, int64_t& i) {
         i = c * kChunkRows + lane;
         const bool ok = c < cend && i < rows;
