@@ -251,7 +251,10 @@ int pm_fixed_state_destroy(pm_fixed_state* state);
 /* Mixed precision: FP32 matrix and inner Jacobi-PCG (FP64 dots, periodic
  * FP64-accumulated residual refresh), FP64 solution and outer iterative
  * refinement until the FP64 true relative residual reaches target_relres.
- * rhs and x are FP64 device vectors. */
+ * rhs and x are FP64 device vectors.  Each inner solve runs to `inner`'s
+ * relative tolerance, except that a step whose relres times that tolerance is
+ * within twice the target runs to 0.9 * target / relres, so that it ends the
+ * refinement although the inner solve stops on its recursive residual. */
 typedef struct pm_refine_report {
     int32_t converged;
     int32_t outer_iterations;
@@ -295,8 +298,9 @@ int pm_batch_pcg(const pm_csr* a, int64_t systems, const int64_t* row_begin, con
 /* Batched FP64 iterative refinement (no reference counterpart): for every
  * system, r = b - A x with FP64 accumulation over FP64 x; systems above
  * target_relres (||r|| / ||b||) solve A d = r / ||r|| with the FP32 batched
- * PCG under `inner` (its relative tolerance and iteration limit; absolute
- * tolerance ignored) and take x += ||r|| d, for at most max_outer steps.
+ * PCG under `inner` (its relative tolerance, tightened per system as in
+ * pm_pcg_refine, and iteration limit; absolute tolerance ignored) and take
+ * x += ||r|| d, for at most max_outer steps.
  * a is FP32; rhs and x are FP64 and x is the initial guess.  Synchronises
  * `stream` once per outer step. */
 typedef struct pm_batch_refine_report {

@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "plan_cache.hpp"
+#include "solver_internal.hpp"
 #include "spmv_kernels.hpp"
 #include "tuning.hpp"
 
@@ -54,6 +55,7 @@ struct BatchState
     int32_t iteration;
     int32_t done;
     unsigned long long bad_row;
+    double den;  // single-reduction form: p.Ap of the last direction
 };
 
 // Chunks of the systems still running, as a device list with its length in
@@ -772,7 +774,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(int64_t systems, const 
                                                          const double* __restrict__ pa,
                                                          const double* __restrict__ pb, int32_t max_iterations,
                                                          double rel_tol, double abs_tol,
-                                                         unsigned long long* done_count)
+                                                         unsigned long long* done_count,
+                                                         const double* __restrict__ sys_rel)
 {
     const int lane = threadIdx.x % kWave;
     const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
@@ -812,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(int64_t systems, const 
             const double rr = a, rz = b;
             w.rr = rr;
             w.rr0 = rr;
-            w.tol = fmax(abs_tol, rel_tol * sqrt(rr));
+            w.tol = fmax(abs_tol, (sys_rel ? sys_rel[s] : rel_tol) * sqrt(rr));
             w.rho = rz;
             w.beta = 0.0;
             w.iteration = 0;
@@ -857,6 +860,230 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(int64_t systems, const 
     }
 }
 
+// ---- single-reduction (Chronopoulos & Gear) form of the batch --------------------
+//
+// Jacobi, CG and block-Jacobi batches run the recurrence of the single-system
+// k_cg_step / k_cg_spmv per system, three kernels per iteration (four with
+// block systems):
+//   step     p = u + beta_s p, s = w + beta_s s (s tracks A p), x += alpha_s p,
+//            r -= alpha_s s, u = M^-1 r (block systems: k_batch_block_apply next)
+//   SpMV     w = A u (one gather per entry, where the direction kernel gathers
+//            z and p_old), per-chunk partials r.r, r.u, w.u, u.s
+//   reduce   per system, in a fixed order: stop test, beta = gamma / gamma_prev,
+//            den = w.u + 2 beta u.s + beta^2 den_prev (p.Ap of the new
+//            direction), alpha = gamma / den; a non-positive den restarts the
+//            system's step from u (beta = 0, den = w.u)
+// Vectors in place: u in z, w in q, p in p0, s in p1.  A refresh iteration
+// forms p, s, x, then r = b - A x in FP64 and u = M^-1 r, then the SpMV.
+
+template <typename S, bool JACOBI, bool STEP_R>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_step(const int32_t* __restrict__ chunk_sys,
+                                                          const int64_t* __restrict__ sys_end, S* __restrict__ x,
+                                                          S* __restrict__ p, S* __restrict__ r, S* __restrict__ u,
+                                                          const S* __restrict__ w, S* __restrict__ sv,
+                                                          const S* __restrict__ inv, const BatchState* st,
+                                                          const int64_t* __restrict__ list,
+                                                          const unsigned long long* __restrict__ list_count,
+                                                          const int32_t* __restrict__ sys_bs)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning) continue;
+        const int64_t row = c * kChunkRows + lane;
+        if (row >= sys_end[s]) continue;
+        const S a = static_cast<S>(st[s].alpha), b = static_cast<S>(st[s].beta);
+        const S pn = fma_s(b, p[row], u[row]);
+        const S sn = fma_s(b, sv[row], w[row]);
+        p[row] = pn;
+        sv[row] = sn;
+        x[row] = fma_s(a, pn, x[row]);
+        if (STEP_R)
+        {
+            const S rn = fma_s(-a, sn, r[row]);
+            r[row] = rn;
+            if (!(sys_bs != nullptr && sys_bs[s] > 1)) u[row] = JACOBI ? inv[row] * rn : rn;
+        }
+    }
+}
+
+// s = 0 on the rows of running systems (its first update is s = w).
+template <typename S>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_clear(const int32_t* __restrict__ chunk_sys,
+                                                           const int64_t* __restrict__ sys_end, S* __restrict__ sv,
+                                                           const BatchState* st, const int64_t* __restrict__ list,
+                                                           const unsigned long long* __restrict__ list_count)
+{
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning) continue;
+        const int64_t row = c * kChunkRows + lane;
+        if (row < sys_end[s]) sv[row] = S(0);
+    }
+}
+
+template <typename S, typename I, int LL = 8>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_spmv(const int32_t* __restrict__ chunk_sys,
+                                                          const int64_t* __restrict__ sys_end,
+                                                          const I* __restrict__ ro, const I* __restrict__ ci,
+                                                          const S* __restrict__ v, const S* __restrict__ r,
+                                                          const S* __restrict__ u, S* __restrict__ w,
+                                                          const S* __restrict__ sv, const BatchState* st,
+                                                          double* __restrict__ prr, double* __restrict__ pru,
+                                                          double* __restrict__ pwu, double* __restrict__ pus,
+                                                          const int64_t* __restrict__ list,
+                                                          const unsigned long long* __restrict__ list_count)
+{
+    __shared__ S lv_all[kBatchWaves][kStreamCap];
+    __shared__ I lc_all[kBatchWaves][kStreamCap];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    PM_FOR_LISTED(li, list_count, kBatchWaves)
+    {
+        const int64_t c = list[li];
+        const int32_t s = chunk_sys[c];
+        if (s < 0 || st[s].done != kRunning)
+        {
+            if (lane == 0) prr[c] = pru[c] = pwu[c] = pus[c] = 0.0;
+            continue;
+        }
+        const int64_t end = sys_end[s];
+        const int64_t row0 = c * kChunkRows;
+        int64_t ks, ke;
+        const S acc = chunk_row_sums<S, I, SpmvSource<S>, false, false, LL>(row0, end, ro, ci, lv_all[wave],
+                                                                            lc_all[wave], SpmvSource<S>{v, u}, ks, ke);
+        const int64_t row = row0 + lane;
+        double rr = 0.0, ru = 0.0, wu = 0.0, us = 0.0;
+        if (row < end)
+        {
+            w[row] = acc;
+            const double rv = static_cast<double>(r[row]);
+            const double uv = static_cast<double>(u[row]);
+            rr = rv * rv;
+            ru = rv * uv;
+            wu = static_cast<double>(acc) * uv;
+            us = uv * static_cast<double>(sv[row]);
+        }
+        rr = wave_sum(rr);
+        ru = wave_sum(ru);
+        wu = wave_sum(wu);
+        us = wave_sum(us);
+        if (lane == 0)
+        {
+            prr[c] = rr;
+            pru[c] = ru;
+            pwu[c] = wu;
+            pus[c] = us;
+        }
+    }
+}
+
+// One wavefront per system: fixed-order sums of its chunk partials, then the
+// system's decision.  INIT: the decision at iteration 0 (tolerance from
+// ||r0||, the first step from p = u, den = u.Au).
+template <typename S, bool INIT>
+__global__ __launch_bounds__(kBlock) void k_batch_cg_reduce(int64_t systems, const int64_t* __restrict__ sys_chunk0,
+                                                            const int64_t* __restrict__ sys_chunk1, BatchState* st,
+                                                            const double* __restrict__ prr,
+                                                            const double* __restrict__ pru,
+                                                            const double* __restrict__ pwu,
+                                                            const double* __restrict__ pus, int32_t max_iterations,
+                                                            double rel_tol, double abs_tol,
+                                                            unsigned long long* done_count,
+                                                            const double* __restrict__ sys_rel)
+{
+    const int lane = threadIdx.x % kWave;
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t s = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; s < systems;
+         s += waves)
+    {
+        if (st[s].done != kRunning) continue;
+        double sum[4] = {0.0, 0.0, 0.0, 0.0};
+        const double* const src[4] = {prr, pru, pwu, pus};
+        const int64_t c1 = sys_chunk1[s];
+        constexpr int U = 4;
+        for (int64_t c = sys_chunk0[s] + lane; c < c1; c += U * kWave)
+        {
+            // all loads of U strided partials first, then the adds in order
+            double va[4][U];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int j = 0; j < U; ++j)
+                {
+                    const int64_t cc = c + j * kWave;
+                    va[k][j] = cc < c1 ? src[k][cc] : 0.0;
+                }
+#pragma unroll
+            for (int j = 0; j < U; ++j)
+            {
+                if (c + j * kWave >= c1) break;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sum[k] += va[k][j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sum[k] = wave_sum(sum[k]);
+        if (lane != 0) continue;
+        const double rr = sum[0], gamma = sum[1], delta = sum[2], us = sum[3];
+        BatchState w = st[s];
+        int code = kRunning;
+        double beta = 0.0, den = delta, alpha = 0.0;
+        if (INIT)
+        {
+            w.rr = rr;
+            w.rr0 = rr;
+            w.tol = fmax(abs_tol, (sys_rel ? sys_rel[s] : rel_tol) * sqrt(rr));
+            w.iteration = 0;
+            if (!isfinite(rr) || rr < 0.0) code = kInitNonFinite;
+            else if (sqrt(rr) <= w.tol) code = kConverged;
+            else if (w.bad_row != ~0ull) code = kBadDiagonal;
+            else if (max_iterations <= 0) code = kIterationLimit;
+        }
+        else
+        {
+            w.iteration += 1;
+            w.rr = rr;
+            if (!isfinite(rr) || rr < 0.0) code = kNonFinite;
+            else if (sqrt(rr) <= w.tol) code = kConverged;
+            else if (w.iteration >= max_iterations) code = kIterationLimit;
+            else
+            {
+                const double rho_prev = w.rho;
+                if (rho_prev == 0.0 && gamma == 0.0) beta = 0.0;
+                else if (isfinite(rho_prev) && isfinite(gamma) && rho_prev > 0.0 && gamma > 0.0)
+                {
+                    beta = gamma / rho_prev;
+                    if (!isfinite(beta) || !isfinite(static_cast<S>(beta))) code = kBreakdown;
+                }
+                else code = kBreakdown;
+                if (code == kRunning && beta != 0.0) den = delta + beta * (2.0 * us + beta * w.den);
+                if (code == kRunning && !(den > 0.0 && isfinite(den)))
+                {
+                    beta = 0.0;
+                    den = delta;
+                }
+            }
+        }
+        if (code == kRunning && !step_length<S>(gamma, den, alpha)) code = kBreakdown;
+        w.beta = beta;
+        w.alpha = alpha;
+        w.rho = gamma;
+        w.den = den;
+        if (code != kRunning)
+        {
+            w.done = code;
+            atomicAdd(done_count, 1ull);
+        }
+        st[s] = w;
+    }
+}
+
 // Stopped-system count into the host's fine-grained status slot.
 __global__ void k_batch_publish(const unsigned long long* __restrict__ done_count, unsigned long long* host)
 {
@@ -887,11 +1114,13 @@ struct BatchProblem
     int64_t* sys_chunk0;
     int64_t* sys_chunk1;
     BatchState* st;
-    double *pden, *prr, *prz;
+    double *pden, *prr, *prz, *pus;
     unsigned long long* done_count;
     int64_t* list;                    // chunks of running systems
     unsigned long long* list_count;
     bool split;                       // direction update as its own kernel (SpMV gathers p once)
+    bool single_reduction;            // Jacobi / CG / block-Jacobi batches as the single-reduction form
+    const double* sys_rel;            // per-system relative tolerance (device; null: rel_tol for all)
     int long_lanes;                   // lanes per row of chunks with long rows (8, 16, 32 or 64)
     bool fuse_step;                   // factored: the step folded into the factor products
     // block-Jacobi: per-system block size (device; null: none), the largest
@@ -923,7 +1152,7 @@ struct BatchCalls
     static KernelCall reduce(const BatchProblem& b, const double* pa, const double* pb)
     {
         return make_call(k_batch_reduce<S, MODE>, dim3(b.grid_systems), dim3(kBlock), b.systems, b.sys_chunk0,
-                         b.sys_chunk1, b.st, pa, pb, b.max_iterations, b.rel_tol, b.abs_tol, b.done_count);
+                         b.sys_chunk1, b.st, pa, pb, b.max_iterations, b.rel_tol, b.abs_tol, b.done_count, b.sys_rel);
     }
 
     template <bool INIT>
@@ -958,6 +1187,11 @@ struct BatchCalls
         {
             static constexpr auto f = k_batch_direction<S, I, SPLIT, LL>;
         };
+    };
+    template <int LL>
+    struct CgSpmv
+    {
+        static constexpr auto f = k_batch_cg_spmv<S, I, LL>;
     };
     template <bool STEP>
     struct Lower
@@ -997,6 +1231,38 @@ struct BatchCalls
                                 static_cast<const unsigned long long*>(b.list_count)));
     }
 
+    static bool single(const BatchProblem& b) { return b.single_reduction && !b.g; }
+
+    // w = A u and the partials r.r, r.u, w.u, u.s (single-reduction form)
+    static KernelCall cg_spmv(const BatchProblem& b, int lanes)
+    {
+        return make_call(by_lanes<CgSpmv>(lanes), dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys, b.sys_end, ro(b.a),
+                         ci(b.a), static_cast<const S*>(b.a->values), static_cast<const S*>(b.r),
+                         static_cast<const S*>(b.z), static_cast<S*>(b.q), static_cast<const S*>(b.p1),
+                         static_cast<const BatchState*>(b.st), b.prr, b.prz, b.pden, b.pus,
+                         static_cast<const int64_t*>(b.list), static_cast<const unsigned long long*>(b.list_count));
+    }
+
+    template <bool INIT>
+    static KernelCall cg_reduce(const BatchProblem& b)
+    {
+        return make_call(k_batch_cg_reduce<S, INIT>, dim3(b.grid_systems), dim3(kBlock), b.systems, b.sys_chunk0,
+                         b.sys_chunk1, b.st, static_cast<const double*>(b.prr), static_cast<const double*>(b.prz),
+                         static_cast<const double*>(b.pden), static_cast<const double*>(b.pus), b.max_iterations,
+                         b.rel_tol, b.abs_tol, b.done_count, b.sys_rel);
+    }
+
+    template <bool STEP_R>
+    static KernelCall cg_step(const BatchProblem& b)
+    {
+        const auto k = b.jacobi ? k_batch_cg_step<S, true, STEP_R> : k_batch_cg_step<S, false, STEP_R>;
+        return make_call(k, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys, b.sys_end, static_cast<S*>(b.x),
+                         static_cast<S*>(b.p0), static_cast<S*>(b.r), static_cast<S*>(b.z),
+                         static_cast<const S*>(b.q), static_cast<S*>(b.p1), static_cast<const S*>(b.inv),
+                         static_cast<const BatchState*>(b.st), static_cast<const int64_t*>(b.list),
+                         static_cast<const unsigned long long*>(b.list_count), b.sys_bs);
+    }
+
     static void init(const BatchProblem& b, std::vector<KernelCall>& out)
     {
         if ((b.jacobi || (b.g && b.sys_jacobi)) && !b.inverses_ready)
@@ -1016,6 +1282,16 @@ struct BatchCalls
         list(b, out);
         out.push_back(residual<true>(b, b.p0));
         block_apply(b, out);
+        if (single(b))
+        {
+            out.push_back(make_call(k_batch_cg_clear<S>, dim3(b.grid_chunks), dim3(kBlock), b.chunk_sys, b.sys_end,
+                                    static_cast<S*>(b.p1), static_cast<const BatchState*>(b.st),
+                                    static_cast<const int64_t*>(b.list),
+                                    static_cast<const unsigned long long*>(b.list_count)));
+            out.push_back(cg_spmv(b, b.long_lanes));
+            out.push_back(cg_reduce<true>(b));
+            return;
+        }
         if (b.g) factored<false>(b, nullptr, out);
         out.push_back(reduce<kReduceInit>(b, b.prr, b.prz));
     }
@@ -1041,6 +1317,19 @@ struct BatchCalls
 
     static void iteration(const BatchProblem& b, int64_t t, bool refresh, std::vector<KernelCall>& out)
     {
+        if (single(b))
+        {
+            if (refresh)
+            {
+                out.push_back(cg_step<false>(b));
+                out.push_back(residual<false>(b, nullptr));
+            }
+            else out.push_back(cg_step<true>(b));
+            block_apply(b, out);
+            out.push_back(cg_spmv(b, b.long_lanes));
+            out.push_back(cg_reduce<false>(b));
+            return;
+        }
         const bool even = t % 2 == 0;
         const void* p_old = even ? b.p0 : b.p1;
         void* p_new = even ? b.p1 : b.p0;
@@ -1105,7 +1394,15 @@ struct BatchCalls
                                 hipEvent_t e1)
     {
         const auto k = by_lanes<Dir<false>::template K>(lanes);
+        const KernelCall spmv = cg_spmv(b, lanes);
+        // the single-reduction form times its SpMV, which rewrites the same w
+        // and partials from the same r, u, s
         auto launch = [&]() {
+            if (single(b))
+            {
+                (void)pm::launch(spmv, stream);
+                return;
+            }
             hipLaunchKernelGGL(k, dim3(b.grid_chunks), dim3(kBlock), 0, stream, b.chunks, b.chunk_sys, b.sys_end,
                                ro(b.a), ci(b.a), static_cast<const S*>(b.a->values), static_cast<const S*>(b.z),
                                static_cast<const S*>(b.p0), static_cast<S*>(b.p1), static_cast<S*>(b.q),
@@ -1178,11 +1475,11 @@ std::string batch_key(const BatchProblem& b, int what, int64_t first, int64_t co
                   static_cast<long long>(b.a->rows), static_cast<long long>(b.systems), b.jacobi ? 1 : 0,
                   static_cast<long long>(first), static_cast<long long>(count), refresh, b.a->dtype,
                   b.max_iterations, b.rel_tol, b.abs_tol,
-                  (b.split ? 1 : 0) | (b.fuse_step ? 2 : 0) | (b.long_lanes << 2),
+                  (b.split ? 1 : 0) | (b.fuse_step ? 2 : 0) | (b.single_reduction ? 4 : 0) | (b.long_lanes << 3),
                   static_cast<const void*>(b.sys_jacobi));
     std::string key(buf);
-    std::snprintf(buf, sizeof(buf), "|blocks|%p|%d|%p|%p|%d", static_cast<const void*>(b.sys_bs), b.bmax, b.binv,
-                  static_cast<void*>(b.block_rows), b.inverses_ready ? 1 : 0);
+    std::snprintf(buf, sizeof(buf), "|blocks|%p|%d|%p|%p|%d|%p", static_cast<const void*>(b.sys_bs), b.bmax, b.binv,
+                  static_cast<void*>(b.block_rows), b.inverses_ready ? 1 : 0, static_cast<const void*>(b.sys_rel));
     key += buf;
     if (b.g)
     {
@@ -1395,7 +1692,7 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     const size_t s1 = std::max<int64_t>(systems, 1);
     const size_t c1 = std::max<int64_t>(chunks, 1);
     const size_t bytes = al(sizeof(int32_t) * c1) + 3 * al(sizeof(int64_t) * s1) + al(sizeof(BatchState) * s1) +
-                         3 * al(sizeof(double) * c1) + 256 + al(sizeof(int64_t) * c1) + 256;
+                         4 * al(sizeof(double) * c1) + 256 + al(sizeof(int64_t) * c1) + 256;
     PM_TRY(ws->batch.reserve(bytes));
     if (!ws->host_batch)
         PM_HIP(hipHostMalloc(reinterpret_cast<void**>(&ws->host_batch), 2 * sizeof(unsigned long long),
@@ -1414,6 +1711,10 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     // 640 ms); PLAMATRIX_AMD_BATCH_SPLIT=1 selects the split form
     const char* split = std::getenv("PLAMATRIX_AMD_BATCH_SPLIT");
     b.split = split != nullptr && split[0] == '1';
+    // Jacobi / CG / block-Jacobi batches as the single-reduction form
+    // (PLAMATRIX_AMD_BATCH_SINGLE_REDUCTION=0 keeps direction / update)
+    const char* single = std::getenv("PLAMATRIX_AMD_BATCH_SINGLE_REDUCTION");
+    b.single_reduction = single == nullptr || single[0] != '0';
     // lanes per long row: PLAMATRIX_AMD_BATCH_LANES (8, 16, 32, 64), else tuned
     // on the first solve of this matrix
     const char* lanes = std::getenv("PLAMATRIX_AMD_BATCH_LANES");
@@ -1443,6 +1744,8 @@ int batch_setup(const pm_csr* a, int64_t systems, const int64_t* row_begin, cons
     b.prr = reinterpret_cast<double*>(p);
     p += al(sizeof(double) * c1);
     b.prz = reinterpret_cast<double*>(p);
+    p += al(sizeof(double) * c1);
+    b.pus = reinterpret_cast<double*>(p);
     p += al(sizeof(double) * c1);
     b.done_count = reinterpret_cast<unsigned long long*>(p);
     p += 256;
@@ -1695,7 +1998,7 @@ int batch_refine(BatchProblem& b, const double* rhs, double* x, double target, i
     const size_t c1 = static_cast<size_t>(std::max<int64_t>(chunks, 1));
     const size_t s1 = static_cast<size_t>(std::max<int64_t>(systems, 1));
     const size_t bytes = al(sizeof(double) * rows_alloc) + 2 * al(sizeof(float) * rows_alloc) +
-                         2 * al(sizeof(double) * c1) + al(sizeof(BatchRefineState) * s1);
+                         2 * al(sizeof(double) * c1) + al(sizeof(BatchRefineState) * s1) + al(sizeof(double) * s1);
     PM_TRY(ws->refine.reserve(bytes));
     char* p = ws->refine.as<char>();
     double* r64 = reinterpret_cast<double*>(p);
@@ -1709,8 +2012,12 @@ int batch_refine(BatchProblem& b, const double* rhs, double* x, double target, i
     double* pbb = reinterpret_cast<double*>(p);
     p += al(sizeof(double) * c1);
     BatchRefineState* rs = reinterpret_cast<BatchRefineState*>(p);
+    p += al(sizeof(BatchRefineState) * s1);
+    double* sys_rel = reinterpret_cast<double*>(p);
+    std::vector<double> host_rel(s1, b.rel_tol);
     b.rhs = rf;
     b.x = d;
+    b.sys_rel = sys_rel;
     const int gs = static_cast<int>(std::min<int64_t>(ceil_div(systems, kBlock), 1024));
     hipLaunchKernelGGL(k_batch_refine_init, dim3(gs), dim3(kBlock), 0, stream, systems, rs);
     std::vector<BatchRefineState> host(s1);
@@ -1738,7 +2045,14 @@ int batch_refine(BatchProblem& b, const double* rhs, double* x, double target, i
         hipLaunchKernelGGL(k_batch_refine_scale, dim3(b.grid_chunks), dim3(kBlock), 0, stream, chunks, b.chunk_sys,
                            b.sys_end, r64, rs, rf, d);
         PM_HIP(hipGetLastError());
+        // each system's inner tolerance for this pass (the host copy of the
+        // states above is current)
+        for (int64_t s = 0; s < systems; ++s)
+            host_rel[s] = refine_pass_tolerance(b.rel_tol, host[s].relres, target);
+        PM_HIP(hipMemcpyAsync(sys_rel, host_rel.data(), sizeof(double) * systems, hipMemcpyHostToDevice, stream));
         PM_TRY(batch_solve<float>(b, pol, inner.data(), ws, stream));
+        // the host array stays unchanged until the stream has read it: the
+        // next pass first synchronises on its state copy
         // later inner solves reuse this matrix's Jacobi and block inverses
         b.inverses_ready = true;
         for (int64_t s = 0; s < systems; ++s)

@@ -31,6 +31,7 @@
 
 #include "dia.hpp"
 #include "plan_cache.hpp"
+#include "solver_internal.hpp"
 #include "spmv_kernels.hpp"
 #include "tuning.hpp"
 
@@ -2452,6 +2453,7 @@ int refine(const pm_csr* a, const double* b, double* x, const pm_solver_options*
         hipLaunchKernelGGL(k_scale_to_f32, dim3(grid), dim3(kBlock), 0, stream, n, r64, 1.0 / norm, rf, d);
         PM_HIP(hipGetLastError());
         pm_solver_report inner_rep{};
+        in.relative_tolerance = refine_pass_tolerance(inner->relative_tolerance, relres, target);
         const int pre = gf ? kPreFactored : (in.use_jacobi ? kPreJacobi : kPreNone);
         PM_TRY(adaptive_solve<float>(a, rf, d, nullptr, 1, pre, &in, policy, &inner_rep, ws, stream, gf, gtf));
         rep.inner_iterations += inner_rep.iterations;

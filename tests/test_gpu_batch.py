@@ -315,3 +315,42 @@ def test_batch_block_jacobi_refinement_reaches_target(dev, ws, oracle):
         xr, rr = oracle.pcg(ref_sys, b64[b0:e0], relative_tolerance=1e-12, max_iterations=20000)
         assert rr.converged
         assert np.linalg.norm(got - xr) <= 1e-6 * np.linalg.norm(xr)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_batch_single_reduction_matches_classic(dev, monkeypatch, dtype):
+    # Jacobi, CG and block-Jacobi batches run the single-reduction form
+    # (step, SpMV of u with four partials, per-system reduce); against the
+    # direction / update form on the same batch, with residual refreshes:
+    # every system ends in the same state, within 2 % (at least 2) of its
+    # iterations, at a true residual within the solve's tolerance
+    schur = schur_members(dev, [40, 90])
+    systems = mixed_systems(6, seed=31, dtype=np.float64) + schur
+    systems = [generators.CSRHost(s.rows, s.cols, s.row_offsets, s.col_indices, s.values.astype(dtype))
+               for s in systems]
+    blocks = [1] * 6 + [6, 6]
+    a, begin, end, big, b = pack(systems, dev, dtype)
+    bt = torch.from_numpy(b).to(dev)
+    tol = 1e-9 if dtype == np.float64 else 1e-5
+    pol = capi.policy(refresh_interval=20)
+    for kind in ("cg", "jacobi", "block"):
+        out = {}
+        for flag in ("0", "1"):
+            monkeypatch.setenv("PLAMATRIX_AMD_BATCH_SINGLE_REDUCTION", flag)
+            w = capi.Workspace()
+            x = torch.zeros(big.rows, dtype=bt.dtype, device=dev)
+            if kind == "block":
+                reps = capi.batch_block_pcg(a, begin, end, blocks, bt, x, w, pol=pol, relative_tolerance=tol,
+                                            max_iterations=4000)
+            else:
+                reps = capi.batch_pcg(a, begin, end, bt, x, w, pol=pol, use_jacobi=kind == "jacobi",
+                                      relative_tolerance=tol, max_iterations=4000)
+            torch.cuda.synchronize()
+            out[flag] = (reps, x.cpu().numpy())
+        (r0, x0), (r1, x1) = out["0"], out["1"]
+        for s, b0, e0, c, d in zip(systems, begin, end, r0, r1):
+            assert c.converged and d.converged and c.status == d.status, (kind, c, d)
+            assert abs(c.iterations - d.iterations) <= max(2, 0.02 * c.iterations), (kind, s.rows, c, d)
+            for xs in (x0, x1):
+                assert relres(s, b[b0:e0], xs[b0:e0]) <= (10 if dtype == np.float64 else 40) * tol, (kind, s.rows)
+

@@ -137,7 +137,10 @@ def test_batch_factored_matches_oracle(dev, ws, net, dtype, rtol):
         print(f"n={s.rows} iterations gpu={rep.iterations} oracle={it} true relres={res:.2e}")
 
 
-def test_jacobi_flags_run_batched_jacobi_bitwise(dev, ws, net):
+def test_jacobi_flags_run_batched_jacobi_bitwise(dev, ws, net, monkeypatch):
+    # the factored batch keeps the direction / update form: its Jacobi-flagged
+    # systems match the Jacobi batch in that form bit for bit
+    monkeypatch.setenv("PLAMATRIX_AMD_BATCH_SINGLE_REDUCTION", "0")
     dtype = np.float32
     systems = systems_mix(8, seed=21, dtype=dtype)
     a, begin, end, big, b = pack(systems, dev, dtype)
