@@ -1022,6 +1022,14 @@ __global__ __launch_bounds__(kBlock) void k_assemble_slots(
 // above keeps the CPU order (PLAMATRIX_AMD_SCHUR_ASSEMBLY=slots).  Long slots
 // (the diagonal ones hold every secondary of their primary) no longer run as
 // one serial chain, and a launch has about 30 times more wavefronts.
+//
+// Only slots on and above the block diagonal get segments: S is symmetric
+// (C^-1 is), so the value of slot (l, r) below the diagonal takes the segment
+// sums of its mirror (r, l) transposed, which halves the gathered terms of the
+// off-diagonal slots, and the lower triangle of a diagonal block takes its
+// upper triangle's value (damped B included, which the normal equations leave
+// symmetric only to rounding).  S is then symmetric bit for bit, and each
+// value still differs from the CPU accumulation by rounding only.
 constexpr int kSegTerms = 16;
 // Segments a wavefront of k_assemble_segments sums.  One per wavefront, over
 // a grid of every segment, measured fastest on the 2,000-primary LM system:
@@ -1031,12 +1039,16 @@ constexpr int kSegTerms = 16;
 constexpr int kSegsPerWave = 1;
 
 // Per slot: the number of terms (the run minus its trailing markers) and of
-// segments; seg_count[slots] = 0 closes the scan.
-__global__ __launch_bounds__(kBlock) void k_slot_segments(int64_t slots, const int64_t* __restrict__ slot_start,
-                                                          int64_t entries, const unsigned* __restrict__ term_e,
+// segments (none below the block diagonal); seg_count[slots] = 0 closes the
+// scan.
+__global__ __launch_bounds__(kBlock) void k_slot_segments(int64_t slots, int pb,
+                                                          const unsigned long long* __restrict__ slot_key,
+                                                          const int64_t* __restrict__ slot_start, int64_t entries,
+                                                          const unsigned* __restrict__ term_e,
                                                           unsigned* __restrict__ slot_terms,
                                                           unsigned* __restrict__ seg_count)
 {
+    const unsigned long long mask = (1ull << pb) - 1ull;
     for (int64_t s = gtid(); s <= slots; s += gstride())
     {
         if (s == slots)
@@ -1049,7 +1061,29 @@ __global__ __launch_bounds__(kBlock) void k_slot_segments(int64_t slots, const i
         int64_t te = end;
         while (te > begin && term_e[te - 1] >= ~1u) --te;
         slot_terms[s] = static_cast<unsigned>(te - begin);
-        seg_count[s] = static_cast<unsigned>((te - begin + kSegTerms - 1) / kSegTerms);
+        const bool lower = (slot_key[s] >> pb) > (slot_key[s] & mask);
+        seg_count[s] = lower ? 0u : static_cast<unsigned>((te - begin + kSegTerms - 1) / kSegTerms);
+    }
+}
+
+// The slot of the mirrored block (c, r) of every slot (r, c) (the pattern is
+// symmetric: every term and primary cross block lands in both).
+__global__ __launch_bounds__(kBlock) void k_slot_mirror(int64_t slots, int pb,
+                                                        const unsigned long long* __restrict__ slot_key,
+                                                        unsigned* __restrict__ mirror)
+{
+    const unsigned long long mask = (1ull << pb) - 1ull;
+    for (int64_t s = gtid(); s < slots; s += gstride())
+    {
+        const unsigned long long key = ((slot_key[s] & mask) << pb) | (slot_key[s] >> pb);
+        int64_t lo = 0, hi = slots;
+        while (lo < hi)
+        {
+            const int64_t mid = (lo + hi) / 2;
+            if (slot_key[mid] < key) lo = mid + 1;
+            else hi = mid;
+        }
+        mirror[s] = static_cast<unsigned>(lo < slots && slot_key[lo] == key ? lo : s);
     }
 }
 
@@ -1153,14 +1187,16 @@ __global__ __launch_bounds__(kBlock) void k_assemble_segments(
 }
 
 // One wavefront per slot: damped B on the diagonal, the primary cross
-// markers, then the segment sums in segment order.
+// markers, then the segment sums in segment order (below the block diagonal:
+// the mirrored slot's, transposed).
 template <int PS>
 __global__ __launch_bounds__(kBlock) void k_assemble_reduce(
     int64_t slots, int pb, const unsigned long long* __restrict__ slot_key, const int64_t* __restrict__ slot_start,
     int64_t entries, const unsigned* __restrict__ term_e, const unsigned* __restrict__ term_left,
     const unsigned* __restrict__ term_right, const unsigned* __restrict__ seg_offs,
-    const int64_t* __restrict__ bro, const int64_t* __restrict__ ro, const double* __restrict__ Bd,
-    const double* __restrict__ pcv, const double* __restrict__ partial, double* __restrict__ values)
+    const unsigned* __restrict__ mirror, const int64_t* __restrict__ bro, const int64_t* __restrict__ ro,
+    const double* __restrict__ Bd, const double* __restrict__ pcv, const double* __restrict__ partial,
+    double* __restrict__ values)
 {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x % kWave;
@@ -1175,23 +1211,30 @@ __global__ __launch_bounds__(kBlock) void k_assemble_reduce(
         const int64_t row = static_cast<int64_t>(slot_key[s] >> pb);
         const int64_t col = static_cast<int64_t>(slot_key[s] & mask);
         const int64_t end = s + 1 < slots ? slot_start[s + 1] : entries;
+        // the lower triangle of a diagonal block is its upper triangle's
+        // value: (er, ec) is the entry computed
+        const bool diag_lower = row == col && lr > lc;
+        const int er = diag_lower ? lc : lr, ec = diag_lower ? lr : lc;
         double acc = 0.0;
-        if (row == col) acc += Bd[row * PS * PS + lr * PS + lc];
+        if (row == col) acc += Bd[row * PS * PS + er * PS + ec];
         int64_t te = end;
         while (te > slot_start[s] && term_e[te - 1] >= ~1u) --te;
         for (int64_t k = te; k < end; ++k)
         {
             if (term_e[k] != ~1u) continue;
             const double* m = pcv + static_cast<int64_t>(term_left[k]) * PS * PS;
-            acc += term_right[k] == 0 ? m[lr * PS + lc] : m[lc * PS + lr];
+            acc += term_right[k] == 0 ? m[er * PS + ec] : m[ec * PS + er];
         }
-        const int64_t g1 = seg_offs[s + 1];
+        const bool lower = row > col;
+        const int64_t src = lower ? static_cast<int64_t>(mirror[s]) : s;
+        const int entry = lower ? ec * PS + er : er * PS + ec;
+        const int64_t g1 = seg_offs[src + 1];
         constexpr int U = 8;
-        for (int64_t g = seg_offs[s]; g < g1; g += U)
+        for (int64_t g = seg_offs[src]; g < g1; g += U)
         {
             double v[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) v[u] = g + u < g1 ? partial[(g + u) * (PS * PS) + lane] : 0.0;
+            for (int u = 0; u < U; ++u) v[u] = g + u < g1 ? partial[(g + u) * (PS * PS) + entry] : 0.0;
 #pragma unroll
             for (int u = 0; u < U; ++u)
                 if (g + u < g1) acc += v[u];
@@ -1565,6 +1608,7 @@ struct pm_schur_workspace
     unsigned* seg_offs = nullptr;    // segmented assembly: first segment of each slot (slots + 1)
     unsigned* slot_terms = nullptr;  // terms of each slot (markers excluded)
     DeviceBuffer segment_slot;       // slot of each segment
+    DeviceBuffer slot_mirror;        // slot of the mirrored block of each slot
     int64_t segments = 0;
     int64_t* bro = nullptr;
     int64_t* ro = nullptr;
@@ -1628,6 +1672,7 @@ struct pm_schur_workspace
         pattern.release();
         pattern_sort.release();
         segment_slot.release();
+        slot_mirror.release();
         numerics.release();
         pt_meta.release();
         pt_cross.release();
@@ -1863,8 +1908,11 @@ int build_pattern(pm_schur_workspace* w, const pm_normal_equations* eq, hipStrea
     hipLaunchKernelGGL(k_group_offsets, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, w->slot_key, slots, pb, pc,
                        w->bro);
     // segments of the slots' term runs (segmented assembly)
-    hipLaunchKernelGGL(k_slot_segments, dim3(grid_for(slots + 1)), dim3(kBlock), 0, stream, slots, w->slot_start,
-                       entries, w->term_e, w->slot_terms, seg_count);
+    hipLaunchKernelGGL(k_slot_segments, dim3(grid_for(slots + 1)), dim3(kBlock), 0, stream, slots, pb, w->slot_key,
+                       w->slot_start, entries, w->term_e, w->slot_terms, seg_count);
+    PM_TRY(w->slot_mirror.reserve(sizeof(unsigned) * std::max<int64_t>(slots, 1)));
+    hipLaunchKernelGGL(k_slot_mirror, dim3(grid_for(slots)), dim3(kBlock), 0, stream, slots, pb, w->slot_key,
+                       w->slot_mirror.as<unsigned>());
     PM_TRY(scan_u32(seg_count, w->seg_offs, slots, sp2.scratch, w->seg_offs + slots, stream));
     unsigned nseg32 = 0;
     PM_HIP(hipMemcpyAsync(&nseg32, w->seg_offs + slots, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
@@ -2149,8 +2197,9 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
                 static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(ceil_div(w->slots, waves_per_block), 65536)));
             hipLaunchKernelGGL(k_assemble_reduce<6>, dim3(slot_grid), dim3(kBlock), 0, stream, w->slots, w->pb,
                                w->slot_key, w->slot_start, w->entries, w->term_e, w->term_left, w->term_right,
-                               static_cast<const unsigned*>(w->seg_offs), w->bro, w->ro, w->Bd,
-                               eq->primary_cross_values, static_cast<const double*>(seg_partial), w->values);
+                               static_cast<const unsigned*>(w->seg_offs), w->slot_mirror.as<unsigned>(), w->bro,
+                               w->ro, w->Bd, eq->primary_cross_values, static_cast<const double*>(seg_partial),
+                               w->values);
         }
         else
         {

@@ -346,9 +346,10 @@ def test_full_size_lm_system(dev, sw, ws):
 @pytest.mark.parametrize("size", ["small", "full"])
 def test_segmented_assembly_matches_slot_assembly(dev, ws, monkeypatch, size):
     # the segmented kernels (terms in segments of 16, one wavefront each,
-    # then the segment sums in order) against the slot kernel (the CPU order):
-    # equal to rounding, and bitwise the same from run to run, at a ring with
-    # long diagonal slots and at the full LM size
+    # then the segment sums in order; slots below the block diagonal take
+    # their mirror's sums transposed) against the slot kernel (the CPU order):
+    # equal to rounding, symmetric bit for bit, and bitwise the same from run
+    # to run, at a ring with long diagonal slots and at the full LM size
     if size == "small":
         inst = shuffled_instance(40, 3000, (3, 12), seed=5)
         pc, ec = 40, 3000
@@ -365,6 +366,13 @@ def test_segmented_assembly_matches_slot_assembly(dev, ws, monkeypatch, size):
         assert rep.status == 0, rep.message
         a, rhs_ptr, pre_ptr = capi.schur_system(sw)
         got[kernel] = (capi.device_array(a.values, a.nnz), capi.device_array(pre_ptr, pc * 36))
+        if kernel == "segments":
+            import scipy.sparse as sp
+
+            ro = capi.device_array(a.row_offsets, a.rows + 1, "int64")
+            ci = capi.device_array(a.col_indices, a.nnz, "int64")
+            S = sp.csr_matrix((got[kernel][0], ci, ro), shape=(a.rows, a.rows))
+            assert (S != S.T).nnz == 0
         del sw, ne
     seg, slot = got["segments"], got["slots"]
     assert np.max(np.abs(seg[0] - slot[0])) <= 1e-13 * np.max(np.abs(slot[0]))
