@@ -219,8 +219,11 @@ __global__ __launch_bounds__(kBlock) void k_u32_to_i64(const unsigned* __restric
 // ---- normal equations from residuals -------------------------------------------
 
 // Validation, weights and the three grouping keys of every residual.
+// RS, PS, ES > 0: the residual and block sizes at compile time (unrolled
+// loops, same arithmetic).
+template <int RS = 0, int PS = 0, int ES = 0>
 __global__ __launch_bounds__(kBlock) void k_residual_keys(
-    int64_t count, int64_t pc, int64_t ec, int rs, int ps, int es, int eb, const int64_t* __restrict__ primary,
+    int64_t count, int64_t pc, int64_t ec, int rs_rt, int ps_rt, int es_rt, int eb, const int64_t* __restrict__ primary,
     const int64_t* __restrict__ eliminated, const double* __restrict__ jp, const double* __restrict__ je,
     const double* __restrict__ r, const double* __restrict__ weights, double delta, double* __restrict__ w_out,
     unsigned long long* __restrict__ key_p, unsigned long long* __restrict__ key_e,
@@ -228,6 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_keys(
     unsigned* __restrict__ val_pe, SchurStatus* st)
 {
 #pragma clang fp contract(off)
+    const int rs = RS > 0 ? RS : rs_rt, ps = PS > 0 ? PS : ps_rt, es = ES > 0 ? ES : es_rt;
     for (int64_t i = gtid(); i < count; i += gstride())
     {
         const int64_t p = primary[i], e = eliminated[i];
@@ -506,7 +510,8 @@ __global__ __launch_bounds__(kBlock) void k_ne_eliminated(int64_t ec, int rs, in
 
 // W blocks (ps x es, row-major), thread per (run, entry), stored at the run's
 // cross index.
-__global__ __launch_bounds__(kBlock) void k_ne_cross(int64_t runs, int64_t n, int rs, int ps, int es,
+template <int RS = 0, int PS = 0, int ES = 0>
+__global__ __launch_bounds__(kBlock) void k_ne_cross(int64_t runs, int64_t n, int rs_rt, int ps_rt, int es_rt,
                                                      const int64_t* __restrict__ run_start,
                                                      const unsigned* __restrict__ run_cross,
                                                      const unsigned* __restrict__ perm,
@@ -514,6 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_ne_cross(int64_t runs, int64_t n, in
                                                      const double* __restrict__ w, double* __restrict__ W)
 {
 #pragma clang fp contract(off)
+    const int rs = RS > 0 ? RS : rs_rt, ps = PS > 0 ? PS : ps_rt, es = ES > 0 ? ES : es_rt;
     const int entries = ps * es;
     for (int64_t t = gtid(); t < runs * entries; t += gstride())
     {
@@ -809,13 +815,17 @@ __global__ __launch_bounds__(kBlock) void k_eliminated_inverse(int64_t ec, int e
 // Per cross block c and row: v_c[row] = W_c[row] . y_e (one thread each),
 // stored in the primaries' summation order (position k of prim_cross), so
 // that k_reduced_rhs streams each primary's products contiguously.
-__global__ __launch_bounds__(kBlock) void k_cross_rhs(int64_t nc, int ps, int es,
+// PS, ES > 0: the block sizes at compile time (the divisions by ps become
+// multiplications), same arithmetic.
+template <int PS = 0, int ES = 0>
+__global__ __launch_bounds__(kBlock) void k_cross_rhs(int64_t nc, int ps_rt, int es_rt,
                                                       const unsigned* __restrict__ prim_cross,
                                                       const int64_t* __restrict__ cross_eliminated,
                                                       const double* __restrict__ W, const double* __restrict__ y,
                                                       double* __restrict__ crv)
 {
 #pragma clang fp contract(off)
+    const int ps = PS > 0 ? PS : ps_rt, es = ES > 0 ? ES : es_rt;
     for (int64_t t = gtid(); t < nc * ps; t += gstride())
     {
         const int64_t c = prim_cross[t / ps];
@@ -858,12 +868,14 @@ __global__ __launch_bounds__(kBlock) void k_reduced_rhs(int64_t pc, int ps, cons
 }
 
 // TW_c[row][i] = sum_j Cinv_e[i][j] W_c[row][j] (multiplyMatrixVector per row).
-__global__ __launch_bounds__(kBlock) void k_transform_cross(int64_t nc, int ps, int es,
+template <int PS = 0, int ES = 0>
+__global__ __launch_bounds__(kBlock) void k_transform_cross(int64_t nc, int ps_rt, int es_rt,
                                                             const int64_t* __restrict__ cross_eliminated,
                                                             const double* __restrict__ Cinv,
                                                             const double* __restrict__ W, double* __restrict__ TW)
 {
 #pragma clang fp contract(off)
+    const int ps = PS > 0 ? PS : ps_rt, es = ES > 0 ? ES : es_rt;
     // one thread per output entry (c, row, i): consecutive threads store
     // consecutive entries, and the threads of one row read the same W row
     const int64_t bv = static_cast<int64_t>(ps) * es;
@@ -1972,7 +1984,9 @@ int residual_pass(pm_schur_workspace* w, const pm_residual_batch* rb, int64_t pc
     const int g = grid_for(n);
     if (n > 0)
     {
-        hipLaunchKernelGGL(k_residual_keys, dim3(g), dim3(kBlock), 0, stream, n, pc, ec,
+        const bool lm = rb->residual_size == 2 && ps == 6 && es == 3;
+        hipLaunchKernelGGL((lm ? k_residual_keys<2, 6, 3> : k_residual_keys<>), dim3(g), dim3(kBlock), 0, stream, n,
+                           pc, ec,
                            static_cast<int>(rb->residual_size), ps, es, ebits, rb->primary, rb->eliminated,
                            rb->primary_jacobians, rb->eliminated_jacobians, rb->residuals, rb->weights,
                            rb->huber_delta, weights_out, sp.keys_a, key_e, key_pe, sp.vals_a, val_e, val_pe, st);
@@ -2173,14 +2187,15 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     if (pd > 0)
     {
         if (nc > 0)
-            hipLaunchKernelGGL(k_cross_rhs, dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
+            hipLaunchKernelGGL((ps == 6 && es == 3 ? k_cross_rhs<6, 3> : k_cross_rhs<>), dim3(grid_for(nc * ps)),
+                               dim3(kBlock), 0, stream, nc, ps, es,
                                w->prim_cross, eq->cross_eliminated, eq->cross_values, w->y, w->crv);
         hipLaunchKernelGGL(k_reduced_rhs, dim3(static_cast<unsigned>(ceil_div(pd, kWave))), dim3(kWave), 0, stream, pc,
                            ps, eq->primary_gradient, w->prim_offs, static_cast<const double*>(w->crv), w->rhs);
         if (es == 3 && ps == 6 && segmented_assembly_wanted())
         {
             if (nc > 0)
-                hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
+                hipLaunchKernelGGL((k_transform_cross<6, 3>), dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
                                    es, eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
             const int64_t waves_per_block = kBlock / kWave;
             if (w->segments > 0)
@@ -2204,7 +2219,8 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         else
         {
             if (nc > 0)
-                hipLaunchKernelGGL(k_transform_cross, dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
+                hipLaunchKernelGGL((ps == 6 && es == 3 ? k_transform_cross<6, 3> : k_transform_cross<>),
+                                   dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
                                    es, eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
             const int64_t waves_per_block = kBlock / kWave;
             const int slot_grid =
@@ -2557,7 +2573,8 @@ int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, 
                            static_cast<int>(es), w->ne_offs_e, w->ne_perm_e, rb->eliminated_jacobians,
                            rb->residuals, weights, C, ge);
     if (runs > 0)
-        hipLaunchKernelGGL(k_ne_cross, dim3(grid_for(runs * ps * es)), dim3(kBlock), 0, stream, runs, n, rs,
+        hipLaunchKernelGGL((rs == 2 && ps == 6 && es == 3 ? k_ne_cross<2, 6, 3> : k_ne_cross<>),
+                           dim3(grid_for(runs * ps * es)), dim3(kBlock), 0, stream, runs, n, rs,
                            static_cast<int>(ps), static_cast<int>(es), w->ne_run_start, w->ne_run_cross,
                            w->ne_perm_pe, rb->primary_jacobians, rb->eliminated_jacobians, weights, W);
     PM_HIP(hipGetLastError());
