@@ -539,6 +539,48 @@ __global__ __launch_bounds__(kBlock) void k_ne_cross(int64_t runs, int64_t n, in
     }
 }
 
+// k_ne_cross with the sizes at compile time and one thread per run: the
+// run's residuals are loaded once for all PS * ES entries of its block (each
+// entry summed in the same order as above, so the same bits).
+template <int RS, int PS, int ES>
+__global__ __launch_bounds__(kBlock) void k_ne_cross_runs(int64_t runs, int64_t n,
+                                                          const int64_t* __restrict__ run_start,
+                                                          const unsigned* __restrict__ run_cross,
+                                                          const unsigned* __restrict__ perm,
+                                                          const double* __restrict__ jp,
+                                                          const double* __restrict__ je,
+                                                          const double* __restrict__ w, double* __restrict__ W)
+{
+#pragma clang fp contract(off)
+    for (int64_t run = gtid(); run < runs; run += gstride())
+    {
+        const int64_t end = run + 1 < runs ? run_start[run + 1] : n;
+        double acc[PS * ES];
+#pragma unroll
+        for (int en = 0; en < PS * ES; ++en) acc[en] = 0.0;
+        for (int64_t k = run_start[run]; k < end; ++k)
+        {
+            const int64_t i = perm[k];
+            const double wi = w[i];
+            double a[RS * PS], b[RS * ES];
+#pragma unroll
+            for (int j = 0; j < RS * PS; ++j) a[j] = jp[i * RS * PS + j];
+#pragma unroll
+            for (int j = 0; j < RS * ES; ++j) b[j] = je[i * RS * ES + j];
+#pragma unroll
+            for (int pcol = 0; pcol < PS; ++pcol)
+#pragma unroll
+                for (int ecol = 0; ecol < ES; ++ecol)
+#pragma unroll
+                    for (int row = 0; row < RS; ++row)
+                        acc[pcol * ES + ecol] += wi * a[row * PS + pcol] * b[row * ES + ecol];
+        }
+        double* out = W + static_cast<int64_t>(run_cross[run]) * PS * ES;
+#pragma unroll
+        for (int en = 0; en < PS * ES; ++en) out[en] = acc[en];
+    }
+}
+
 // ---- Schur pattern ---------------------------------------------------------------
 
 // Range checks of caller-supplied topology: cross ids in range, primary
@@ -2572,9 +2614,12 @@ int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, 
         hipLaunchKernelGGL(k_ne_eliminated, dim3(grid_for(ec * (es * es + es))), dim3(kBlock), 0, stream, ec, rs,
                            static_cast<int>(es), w->ne_offs_e, w->ne_perm_e, rb->eliminated_jacobians,
                            rb->residuals, weights, C, ge);
-    if (runs > 0)
-        hipLaunchKernelGGL((rs == 2 && ps == 6 && es == 3 ? k_ne_cross<2, 6, 3> : k_ne_cross<>),
-                           dim3(grid_for(runs * ps * es)), dim3(kBlock), 0, stream, runs, n, rs,
+    if (runs > 0 && rs == 2 && ps == 6 && es == 3)
+        hipLaunchKernelGGL((k_ne_cross_runs<2, 6, 3>), dim3(grid_for(runs)), dim3(kBlock), 0, stream, runs, n,
+                           w->ne_run_start, w->ne_run_cross, w->ne_perm_pe, rb->primary_jacobians,
+                           rb->eliminated_jacobians, weights, W);
+    else if (runs > 0)
+        hipLaunchKernelGGL(k_ne_cross<>, dim3(grid_for(runs * ps * es)), dim3(kBlock), 0, stream, runs, n, rs,
                            static_cast<int>(ps), static_cast<int>(es), w->ne_run_start, w->ne_run_cross,
                            w->ne_perm_pe, rb->primary_jacobians, rb->eliminated_jacobians, weights, W);
     PM_HIP(hipGetLastError());
