@@ -442,7 +442,8 @@ def run_poisson(args, torch, capi, generators):
 
     def solve():
         x.zero_()
-        return capi.pcg(a, b, x, ws, relative_tolerance=1e-8, max_iterations=2000)
+        return capi.pcg(a, b, x, ws, capi.policy(chunk_iterations=args.chunk, use_graph=args.graph),
+                        relative_tolerance=1e-8, max_iterations=2000)
 
     for _ in range(args.warmup):
         solve()

@@ -1973,7 +1973,15 @@ struct Policy
     int chunk = 16;
     bool graph = true;
     int refresh = 0;
+    bool grow = false;  // chunks double up to kMaxGrownChunk (no refresh interval, chunk not set)
 };
+
+// Without a refresh interval to divide, a solve's chunks grow 16, 16, 32, 64
+// iterations: short solves keep their early stop (the 5-iteration Schur
+// block-PCG of an LM step lost 0.09 ms when its second chunk was 32), long
+// launch-bound ones cross the host less often (65,536-row FP64 Poisson, 767
+// iterations: 7.91 ms with chunks of 16, 7.54 ms with 64).
+constexpr int kMaxGrownChunk = 64;
 
 // By default a chunk of iterations is replayed as a HIP graph below
 // kDirectLaunchRows rows, where the kernels are short enough for the launch
@@ -1992,6 +2000,7 @@ Policy resolve_policy(const pm_solve_policy* p, int64_t rows)
         if (p->use_graph >= 0) out.graph = p->use_graph != 0;
         out.refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
     }
+    out.grow = out.refresh == 0 && !(p && p->chunk_iterations > 0);
     if (out.chunk % 2) out.chunk += 1;
     if (out.refresh > 0)
     {
@@ -2106,7 +2115,7 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
 
     // iteration t = max_iterations must run its first kernel to record the
     // final decision, so chunks cover iterations [0, max_iterations].
-    const int64_t K = policy.chunk;
+    int64_t K = policy.chunk;
     const int64_t needed = static_cast<int64_t>(options->max_iterations) + 1;
     SolverState* hs = ws->host_state;
     SolverState fin{};
@@ -2127,6 +2136,9 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
         inflight.push_back(chunk_id);
         ++chunk_id;
         launched += K;
+        // the first two chunks go out together; later ones follow a
+        // completed chunk that did not stop the solve
+        if (policy.grow && chunk_id >= 2 && K < kMaxGrownChunk) K *= 2;
         return PM_OK;
     };
     PM_TRY(issue());
