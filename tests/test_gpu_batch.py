@@ -354,3 +354,36 @@ def test_batch_single_reduction_matches_classic(dev, monkeypatch, dtype):
             for xs in (x0, x1):
                 assert relres(s, b[b0:e0], xs[b0:e0]) <= (10 if dtype == np.float64 else 40) * tol, (kind, s.rows)
 
+
+
+def test_batch_refinement_final_pass_aims_below_target(dev, ws):
+    # inner tolerance 1e-3 and target 1e-6: a second pass would land a hair
+    # either side of the target, so a pass that can end the refinement aims at
+    # 0.9 x target / relres; every system then ends in two passes, at most
+    # 0.95 of the target (0.9 and the inner solve's small recursive-residual
+    # gap), with the true FP64 residual
+    systems = mixed_systems(16, seed=41, dtype=np.float32)
+    a, begin, end, big, b = pack(systems, dev, np.float32)
+    b64 = b.astype(np.float64)
+    bt = torch.from_numpy(b64).to(dev)
+    x = torch.zeros(big.rows, dtype=torch.float64, device=dev)
+    reps = capi.batch_pcg_refine(a, begin, end, bt, x, ws, target=1e-6, max_outer=10, relative_tolerance=1e-3,
+                                 max_iterations=5000)
+    xs = x.cpu().numpy()
+    for s, b0, e0, rep in zip(systems, begin, end, reps):
+        assert rep.converged and rep.outer_iterations == 2, (s.rows, rep)
+        assert rep.final_relres <= 0.95e-6, (s.rows, rep)
+        assert relres(s, b64[b0:e0], xs[b0:e0]) <= 0.96e-6
+
+
+def test_refinement_final_pass_aims_below_target(dev, ws):
+    # the single-system refinement follows the same rule
+    h = generators.laplacian2d_csr(300, 300, dtype=np.float32)
+    a = capi.to_device_csr(h, compact=True)
+    rng = np.random.default_rng(43)
+    b64 = rng.standard_normal(h.rows)
+    bt = torch.from_numpy(b64).to(dev)
+    x = torch.zeros(h.rows, dtype=torch.float64, device=dev)
+    rep = capi.pcg_refine(a, bt, x, ws, target=1e-6, max_outer=10, relative_tolerance=1e-3, max_iterations=20000)
+    assert rep.converged and rep.outer_iterations == 2 and rep.final_relres <= 0.95e-6, rep
+    assert relres(h, b64, x.cpu().numpy()) <= 0.96e-6
