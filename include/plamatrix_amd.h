@@ -214,8 +214,8 @@ typedef struct pm_solver_report {
 /* Execution policy (no counterpart in the reference). Zero-initialised means
  * the defaults. */
 typedef struct pm_solve_policy {
-    int32_t chunk_iterations;  /* iterations launched per device-side chunk (0: 16; without a refresh
-                                  interval the third and later chunks grow to 32, then 64) */
+    int32_t chunk_iterations;  /* iterations launched per device-side chunk (0: 16 with a refresh
+                                  interval; without one 4, 4, then doubling up to 64) */
     int32_t use_graph;         /* 1: replay chunks as HIP graphs, 0: direct launches, -1: default
                                   (graphs below 262,144 rows, direct launches above) */
     int32_t refresh_interval;  /* recompute r = b - A x (FP64 accumulation) every k iterations; 0: never */

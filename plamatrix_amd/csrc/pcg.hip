@@ -1976,11 +1976,12 @@ struct Policy
     bool grow = false;  // chunks double up to kMaxGrownChunk (no refresh interval, chunk not set)
 };
 
-// Without a refresh interval to divide, a solve's chunks grow 16, 16, 32, 64
-// iterations: short solves keep their early stop (the 5-iteration Schur
-// block-PCG of an LM step lost 0.09 ms when its second chunk was 32), long
-// launch-bound ones cross the host less often (65,536-row FP64 Poisson, 767
-// iterations: 7.91 ms with chunks of 16, 7.54 ms with 64).
+// Without a refresh interval to divide, a solve's chunks grow 4, 4, 8, 16,
+// 32, 64 iterations: short solves stop early (the 5-iteration Schur block-PCG
+// of an LM step ran 27 iterations of early-exit kernels with two chunks of 16
+// in flight), long launch-bound ones cross the host less often (65,536-row
+// FP64 Poisson, 767 iterations: 7.91 ms with chunks of 16, 7.54 ms with 64).
+constexpr int kFirstGrownChunk = 4;
 constexpr int kMaxGrownChunk = 64;
 
 // By default a chunk of iterations is replayed as a HIP graph below
@@ -2001,6 +2002,7 @@ Policy resolve_policy(const pm_solve_policy* p, int64_t rows)
         out.refresh = p->refresh_interval > 0 ? p->refresh_interval : 0;
     }
     out.grow = out.refresh == 0 && !(p && p->chunk_iterations > 0);
+    if (out.grow) out.chunk = kFirstGrownChunk;
     if (out.chunk % 2) out.chunk += 1;
     if (out.refresh > 0)
     {
