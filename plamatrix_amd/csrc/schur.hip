@@ -882,6 +882,54 @@ __global__ __launch_bounds__(kBlock) void k_cross_rhs(int64_t nc, int ps_rt, int
     }
 }
 
+// k_transform_cross and k_cross_rhs in one pass over W for PS x ES blocks:
+// thread (c, row) reads row `row` of cross block c once and forms its W C^-1
+// row and its W C^-1 g product, the latter stored at the block's position in
+// the primaries' order (cross_pos, the inverse of prim_cross).  Same sums in
+// the same order as the two kernels.
+template <int PS, int ES>
+__global__ __launch_bounds__(kBlock) void k_transform_rhs(int64_t nc, const int64_t* __restrict__ cross_eliminated,
+                                                          const unsigned* __restrict__ cross_pos,
+                                                          const double* __restrict__ Cinv,
+                                                          const double* __restrict__ y, const double* __restrict__ W,
+                                                          double* __restrict__ TW, double* __restrict__ crv)
+{
+#pragma clang fp contract(off)
+    for (int64_t t = gtid(); t < nc * PS; t += gstride())
+    {
+        const int64_t c = t / PS;
+        const int row = static_cast<int>(t % PS);
+        const int64_t e = cross_eliminated[c];
+        double w[ES], v[ES];
+#pragma unroll
+        for (int j = 0; j < ES; ++j)
+        {
+            w[j] = W[c * PS * ES + row * ES + j];
+            v[j] = y[e * ES + j];
+        }
+#pragma unroll
+        for (int i = 0; i < ES; ++i)
+        {
+            const double* inv = Cinv + e * ES * ES + i * ES;
+            double value = 0.0;
+#pragma unroll
+            for (int j = 0; j < ES; ++j) value += inv[j] * w[j];
+            TW[c * PS * ES + row * ES + i] = value;
+        }
+        double value = 0.0;
+#pragma unroll
+        for (int col = 0; col < ES; ++col) value += w[col] * v[col];
+        crv[static_cast<int64_t>(cross_pos[c]) * PS + row] = value;
+    }
+}
+
+// cross_pos[prim_cross[k]] = k.
+__global__ __launch_bounds__(kBlock) void k_invert_order(int64_t n, const unsigned* __restrict__ order,
+                                                         unsigned* __restrict__ pos)
+{
+    for (int64_t k = gtid(); k < n; k += gstride()) pos[order[k]] = static_cast<unsigned>(k);
+}
+
 // rhs_p = -g_p + sum over the primary's cross blocks in (primary, eliminated)
 // order of W C^-1 g_e (block_schur.cpp:158-181), the products precomputed by
 // k_cross_rhs; the ordered sum loads kRhsBatch products ahead of the adds.
@@ -1656,6 +1704,7 @@ struct pm_schur_workspace
     unsigned* adj = nullptr;
     int64_t* prim_offs = nullptr;
     unsigned* prim_cross = nullptr;
+    unsigned* cross_pos = nullptr;   // position of each cross block in prim_cross
     unsigned long long* slot_key = nullptr;
     int64_t* slot_start = nullptr;
     unsigned* term_e = nullptr;
@@ -1928,6 +1977,7 @@ int build_pattern(pm_schur_workspace* w, const pm_normal_equations* eq, hipStrea
     fz.take<unsigned>(slots + 1);
     fz.take<unsigned>(slots + 1);
     fz.take<unsigned>(slots);
+    fz.take<unsigned>(nc);
     PM_TRY(w->pattern.reserve(fz.off));
     Carve f;
     f.base = w->pattern.as<char>();
@@ -1953,10 +2003,14 @@ int build_pattern(pm_schur_workspace* w, const pm_normal_equations* eq, hipStrea
     unsigned* seg_count = f.take<unsigned>(slots + 1);
     w->seg_offs = f.take<unsigned>(slots + 1);
     w->slot_terms = f.take<unsigned>(slots);
+    w->cross_pos = f.take<unsigned>(nc);
     // the head block was carved in the same order and sizes as the front of
     // the pattern buffer, so one copy moves cp, ce, pr, pcl, adjacency and
     // the primary lists
     PM_HIP(hipMemcpyAsync(w->cp_copy, cp, prefix_bytes, hipMemcpyDeviceToDevice, stream));
+    if (nc > 0)
+        hipLaunchKernelGGL(k_invert_order, dim3(grid_for(nc)), dim3(kBlock), 0, stream, nc,
+                           static_cast<const unsigned*>(w->prim_cross), w->cross_pos);
     hipLaunchKernelGGL(k_slots, dim3(ge), dim3(kBlock), 0, stream, sk, heads, slot_id, entries, w->slot_key,
                        w->slot_start);
     hipLaunchKernelGGL(k_decode_terms, dim3(ge), dim3(kBlock), 0, stream, sv, entries, terms, pc, ec, pair_offs,
@@ -2230,17 +2284,20 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     }
     if (pd > 0)
     {
-        if (nc > 0)
-            hipLaunchKernelGGL((ps == 6 && es == 3 ? k_cross_rhs<6, 3> : k_cross_rhs<>), dim3(grid_for(nc * ps)),
-                               dim3(kBlock), 0, stream, nc, ps, es,
+        // 6 x 3 blocks: W C^-1 and W C^-1 g in one pass over W
+        const bool fused = ps == 6 && es == 3;
+        if (nc > 0 && fused)
+            hipLaunchKernelGGL((k_transform_rhs<6, 3>), dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc,
+                               eq->cross_eliminated, static_cast<const unsigned*>(w->cross_pos), w->Cinv, w->y,
+                               eq->cross_values, w->TW, w->crv);
+        else if (nc > 0)
+            hipLaunchKernelGGL((k_cross_rhs<>), dim3(grid_for(nc * ps)), dim3(kBlock), 0, stream, nc, ps, es,
                                w->prim_cross, eq->cross_eliminated, eq->cross_values, w->y, w->crv);
         hipLaunchKernelGGL(k_reduced_rhs, dim3(static_cast<unsigned>(ceil_div(pd, kWave))), dim3(kWave), 0, stream, pc,
                            ps, eq->primary_gradient, w->prim_offs, static_cast<const double*>(w->crv), w->rhs);
         if (es == 3 && ps == 6 && segmented_assembly_wanted())
         {
-            if (nc > 0)
-                hipLaunchKernelGGL((k_transform_cross<6, 3>), dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
-                                   es, eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
+            // W C^-1 came with the rhs products
             const int64_t waves_per_block = kBlock / kWave;
             if (w->segments > 0)
             {
@@ -2262,9 +2319,8 @@ int schur_assemble(pm_schur_workspace* w, const pm_normal_equations* eq, double 
         }
         else
         {
-            if (nc > 0)
-                hipLaunchKernelGGL((ps == 6 && es == 3 ? k_transform_cross<6, 3> : k_transform_cross<>),
-                                   dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
+            if (nc > 0 && !fused)
+                hipLaunchKernelGGL((k_transform_cross<>), dim3(grid_for(nc * ps * es)), dim3(kBlock), 0, stream, nc, ps,
                                    es, eq->cross_eliminated, w->Cinv, eq->cross_values, w->TW);
             const int64_t waves_per_block = kBlock / kWave;
             const int slot_grid =
