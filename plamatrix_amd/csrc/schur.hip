@@ -510,6 +510,51 @@ __global__ __launch_bounds__(kBlock) void k_ne_eliminated(int64_t ec, int rs, in
     }
 }
 
+// k_ne_eliminated with the sizes at compile time and one thread per
+// eliminated block: its residuals are loaded once for all ES * ES + ES
+// entries (each summed in the same order as above, so the same bits).
+template <int RS, int ES>
+__global__ __launch_bounds__(kBlock) void k_ne_eliminated_points(int64_t ec, const int64_t* __restrict__ offs,
+                                                                 const unsigned* __restrict__ perm,
+                                                                 const double* __restrict__ je,
+                                                                 const double* __restrict__ r,
+                                                                 const double* __restrict__ w,
+                                                                 double* __restrict__ C, double* __restrict__ g)
+{
+#pragma clang fp contract(off)
+    for (int64_t e = gtid(); e < ec; e += gstride())
+    {
+        double acc[ES * ES + ES];
+#pragma unroll
+        for (int en = 0; en < ES * ES + ES; ++en) acc[en] = 0.0;
+        for (int64_t k = offs[e]; k < offs[e + 1]; ++k)
+        {
+            const int64_t i = perm[k];
+            const double wi = w[i];
+            double J[RS * ES], res[RS];
+#pragma unroll
+            for (int j = 0; j < RS * ES; ++j) J[j] = je[i * RS * ES + j];
+#pragma unroll
+            for (int j = 0; j < RS; ++j) res[j] = r[i * RS + j];
+#pragma unroll
+            for (int column = 0; column < ES; ++column)
+            {
+#pragma unroll
+                for (int other = 0; other < ES; ++other)
+#pragma unroll
+                    for (int row = 0; row < RS; ++row)
+                        acc[column * ES + other] += wi * J[row * ES + column] * J[row * ES + other];
+#pragma unroll
+                for (int row = 0; row < RS; ++row) acc[ES * ES + column] += wi * J[row * ES + column] * res[row];
+            }
+        }
+#pragma unroll
+        for (int en = 0; en < ES * ES; ++en) C[e * ES * ES + en] = acc[en];
+#pragma unroll
+        for (int column = 0; column < ES; ++column) g[e * ES + column] = acc[ES * ES + column];
+    }
+}
+
 // W blocks (ps x es, row-major), thread per (run, entry), stored at the run's
 // cross index.
 template <int RS = 0, int PS = 0, int ES = 0>
@@ -1405,12 +1450,15 @@ __global__ __launch_bounds__(kBlock) void k_diag_slots(int64_t pc, int pb, int64
 
 // Back-substitution, first half: t_c[col] = sum_row W_c[row][col] dp_p[row]
 // for every cross block c (a thread per (c, col): W is read in order).
-__global__ __launch_bounds__(kBlock) void k_cross_point(int64_t nc, int ps, int es,
+// PS, ES > 0: block sizes at compile time (same arithmetic).
+template <int PS = 0, int ES = 0>
+__global__ __launch_bounds__(kBlock) void k_cross_point(int64_t nc, int ps_rt, int es_rt,
                                                         const int64_t* __restrict__ cross_primary,
                                                         const double* __restrict__ W, const double* __restrict__ dp,
                                                         double* __restrict__ t)
 {
 #pragma clang fp contract(off)
+    const int ps = PS > 0 ? PS : ps_rt, es = ES > 0 ? ES : es_rt;
     for (int64_t i = gtid(); i < nc * es; i += gstride())
     {
         const int64_t c = i / es;
@@ -2668,7 +2716,10 @@ int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, 
                                dim3(kBlock), 0, stream, pc, rs, static_cast<int>(ps), w->ne_offs_p, w->ne_perm_p,
                                rb->primary_jacobians, rb->residuals, weights, B, gp);
     }
-    if (ec * es > 0)
+    if (ec * es > 0 && rs == 2 && es == 3)
+        hipLaunchKernelGGL((k_ne_eliminated_points<2, 3>), dim3(grid_for(ec)), dim3(kBlock), 0, stream, ec,
+                           w->ne_offs_e, w->ne_perm_e, rb->eliminated_jacobians, rb->residuals, weights, C, ge);
+    else if (ec * es > 0)
         hipLaunchKernelGGL(k_ne_eliminated, dim3(grid_for(ec * (es * es + es))), dim3(kBlock), 0, stream, ec, rs,
                            static_cast<int>(es), w->ne_offs_e, w->ne_perm_e, rb->eliminated_jacobians,
                            rb->residuals, weights, C, ge);
@@ -2849,7 +2900,8 @@ int pm_schur_solve(pm_schur_workspace* w, const pm_normal_equations* eq, double 
     if (ec > 0 && pd > 0 && eq->cross_count > 0)
     {
         const int64_t nc = eq->cross_count;
-        hipLaunchKernelGGL(k_cross_point, dim3(grid_for(nc * es)), dim3(kBlock), 0, stream, nc, ps, es,
+        hipLaunchKernelGGL((ps == 6 && es == 3 ? k_cross_point<6, 3> : k_cross_point<>), dim3(grid_for(nc * es)),
+                           dim3(kBlock), 0, stream, nc, ps, es,
                            eq->cross_primary, eq->cross_values, dp, w->crv);
         const auto point_kernel = es == 3                  ? k_back_substitute_point<3, 3>
                                   : es <= kSmallSchurBlock ? k_back_substitute_point<kSmallSchurBlock>
