@@ -209,6 +209,43 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den(int64_t rows, const I* _
     publish_partials<1>(part, partials + kPartialsDen);
 }
 
+// k_spmv_den with kLongLanes lanes per row, for matrices whose rows are long
+// and few (a 12,000-row reduced Schur system with ~110 entries per row is 188
+// chunks: a wavefront per chunk leaves most of the GPU idle); the lanes of a
+// row sum strided entries and then combine them, the p.q partial is that of
+// the workgroup's rows.
+constexpr int kLongLanes = 8;
+constexpr int64_t kLongRowEntries = 32;  // average entries per row from which the lanes kernel serves
+
+template <typename S, typename I>
+__global__ __launch_bounds__(kDirBlock) void k_spmv_den_lanes(int64_t rows, const I* __restrict__ ro,
+                                                              const I* __restrict__ ci, const S* __restrict__ v,
+                                                              const S* __restrict__ p, S* __restrict__ q,
+                                                              const SolverState* st, double* partials)
+{
+    if (st->done != kRunning) return;
+    constexpr int64_t gpb = kDirBlock / kLongLanes;
+    const int group = threadIdx.x / kLongLanes;
+    const int lane = threadIdx.x % kLongLanes;
+    int64_t begin, end;
+    block_range(rows, gpb, begin, end);
+    double den = 0.0;
+    for (int64_t row = begin + group; row < end; row += gpb)
+    {
+        const int64_t s0 = ro[row], e0 = ro[row + 1];
+        S acc = S(0);
+        for (int64_t k = s0 + lane; k < e0; k += kLongLanes) acc = fma_s(v[k], p[ci[k]], acc);
+        acc = group_sum<kLongLanes>(acc);
+        if (lane == 0)
+        {
+            q[row] = acc;
+            den = __builtin_fma(static_cast<double>(p[row]), static_cast<double>(acc), den);
+        }
+    }
+    double part[1] = {den};
+    publish_partials<1>(part, partials + kPartialsDen);
+}
+
 // Chunks a wave of a diagonal-layout SpMV keeps in flight: one chunk's loads
 // (mask, value arrays, gathers) do not cover the latency of the 256 MB
 // last-level cache the iteration's working set lives in.
@@ -296,6 +333,74 @@ __global__ __launch_bounds__(kDirBlock) void k_dir_spmv_dia(int64_t rows, DiaArg
     }
     double part[1] = {den};
     publish_partials<1>(part, partials + kPartialsDen);
+}
+
+// The block-Jacobi update with a lane per row: a wavefront holds
+// kWave / bsize whole diagonal blocks (its last kWave % bsize lanes idle), so
+// every lane reads its block's r (and q) before any lane of the wavefront
+// writes r back; row j of the block's inverse is read by its own lane
+// (coalesced).  Each row's x, r and z take the arithmetic of k_update's block
+// path; the r.r, r.z partials group the rows per workgroup.
+template <typename S, int MODE>
+__global__ __launch_bounds__(kBlock) void k_update_blocks(int64_t blocks, int64_t bsize, S* __restrict__ x,
+                                                          S* __restrict__ r, S* __restrict__ z,
+                                                          const S* __restrict__ p, const S* __restrict__ q,
+                                                          const S* __restrict__ inv, SolverState* st,
+                                                          double* partials, int par, int count_den)
+{
+    if (st->done != kRunning) return;
+    S alpha = S(0);
+    if (MODE == kModeStep)
+    {
+        double den[1];
+        reduce_partials<1>(partials + kPartialsDen, count_den, den);
+        double a = 0.0;
+        const bool ok = step_length<S>(st->rho[par], den[0], a);
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+        {
+            st->alpha = a;
+            if (!ok) st->done = kBreakdown;
+        }
+        if (!ok) return;
+        alpha = static_cast<S>(a);
+    }
+    const int lane = threadIdx.x % kWave;
+    const int64_t bpw = kWave / bsize;
+    const int64_t local = lane / bsize;  // block of the wavefront (bpw for an idle lane)
+    const int j = static_cast<int>(lane % bsize);
+    const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / kWave);
+    double rr = 0.0, rz = 0.0;
+    for (int64_t w = static_cast<int64_t>(blockIdx.x) * (kBlock / kWave) + threadIdx.x / kWave;
+         w * bpw < blocks; w += waves)
+    {
+        const int64_t u = w * bpw + local;
+        const bool active = local < bpw && u < blocks;
+        const int64_t base = (active ? u : 0) * bsize;
+        S ri[kMaxBlock];
+        for (int c = 0; c < bsize; ++c)
+        {
+            const int64_t i = base + c;
+            ri[c] = MODE == kModeStep ? fma_s(-alpha, q[i], r[i]) : r[i];
+        }
+        const S* m = inv + (active ? u : 0) * bsize * bsize + j * bsize;
+        S zj = S(0);
+        for (int c = 0; c < bsize; ++c) zj = fma_s(m[c], ri[c], zj);
+        // every lane of the wavefront has read its block's r above
+        if (active)
+        {
+            const int64_t i = base + j;
+            if (MODE == kModeStep)
+            {
+                x[i] = fma_s(alpha, p[i], x[i]);
+                r[i] = ri[j];
+            }
+            z[i] = zj;
+            rr = __builtin_fma(static_cast<double>(ri[j]), static_cast<double>(ri[j]), rr);
+            rz = __builtin_fma(static_cast<double>(ri[j]), static_cast<double>(zj), rz);
+        }
+    }
+    double part[2] = {rr, rz};
+    publish_partials<2>(part, partials + kPartialsRes);
 }
 
 // Second kernel of iteration t (STEP), or the preconditioning half of the
@@ -1435,6 +1540,7 @@ struct Problem
     bool single_reduction;   // CG / Jacobi-PCG as the two-kernel Chronopoulos-Gear iteration
     bool cg_fused;           // ... as one kernel per iteration on the diagonal layout (factored: G r and G^T t in one)
     bool cg_factored;        // factored PCG as the three-kernel single-reduction iteration (diagonal layouts)
+    bool long_rows;          // classic CSR SpMV with kLongLanes lanes per row (few long rows)
     // workspace vectors
     void *r, *z, *q, *p0, *p1, *inv;
     void *r2, *z2, *q2, *s2; // second buffers of r, u, w, s for the one-kernel iteration (or null)
@@ -1461,6 +1567,11 @@ struct Calls
     template <int PRE, int MODE>
     static KernelCall update_p(const Problem& pb, const void* p, int par)
     {
+        if constexpr (PRE == kPreBlock)
+            return make_call(k_update_blocks<S, MODE>, dim3(pb.grid_units), dim3(kBlock), pb.a->rows / pb.bsize,
+                             pb.bsize, static_cast<S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(pb.z),
+                             static_cast<const S*>(p), static_cast<const S*>(pb.q),
+                             static_cast<const S*>(pb.inv_blocks), pb.st, pb.partials, par, pb.grid_rows);
         const int64_t units = PRE == kPreBlock ? pb.a->rows / pb.bsize : pb.a->rows;
         S* zz = PRE == kPreNone ? static_cast<S*>(pb.r) : static_cast<S*>(pb.z);
         const S* inv = PRE == kPreBlock ? static_cast<const S*>(pb.inv_blocks) : static_cast<const S*>(pb.inv);
@@ -1756,6 +1867,13 @@ struct Calls
                                     static_cast<const S*>(pb.dia.values), pb.dia.mask, static_cast<const S*>(p_new),
                                     static_cast<S*>(pb.q), static_cast<const SolverState*>(pb.st), pb.partials));
         }
+        else if (pb.long_rows)
+        {
+            out.push_back(make_call(k_spmv_den_lanes<S, I>, dim3(pb.grid_rows), dim3(kDirBlock), pb.a->rows,
+                                    ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
+                                    static_cast<const S*>(p_new), static_cast<S*>(pb.q),
+                                    static_cast<const SolverState*>(pb.st), pb.partials));
+        }
         else
         {
             const auto spmv_den = pb.nt ? (pb.dma ? k_spmv_den<S, I, true, true> : k_spmv_den<S, I, true, false>)
@@ -1883,11 +2001,16 @@ int prepare(pm_workspace* ws, Problem& pb, hipStream_t stream)
     pb.s2 = second ? base + 9 * vec : nullptr;
     pb.cg_fused = false;
     pb.cg_factored = false;
+    pb.long_rows = false;
     pb.st = ws->state.as<SolverState>();
     pb.partials = ws->partials.as<double>();
     const int64_t units = pb.pre == kPreBlock ? n / pb.bsize : n;
     pb.grid_rows = static_cast<int>(std::min<int64_t>(ceil_div(chunk_count(n), kDirWaves), kDirGrid));
     pb.grid_units = solver_grid_units(units);
+    // block-Jacobi: a wavefront per kWave / bsize blocks in the update
+    if (pb.pre == kPreBlock)
+        pb.grid_units = static_cast<int>(std::max<int64_t>(
+            1, std::min<int64_t>(ceil_div(ceil_div(units, kWave / pb.bsize), kBlock / kWave), kSolverGrid)));
     const char* nt = std::getenv("PLAMATRIX_AMD_SOLVER_NT");
     pb.nt = nt != nullptr && nt[0] == '1';
     // LDS-DMA staging speeds up the HBM-streamed SpMV but slows the solver's
@@ -1944,7 +2067,8 @@ std::string plan_key(const Problem& pb, const char* kind, int64_t first, int64_t
                   static_cast<long long>(first), static_cast<long long>(count), refresh,
                   static_cast<long long>(pb.bsize),
                   (pb.nt ? 1 : 0) | (pb.dma ? 2 : 0) | (pb.fuse_direction ? 4 : 0) | (pb.fuse_step ? 8 : 0) |
-                      (pb.single_reduction ? 16 : 0) | (pb.cg_fused ? 32 : 0) | (pb.cg_factored ? 64 : 0),
+                      (pb.single_reduction ? 16 : 0) | (pb.cg_fused ? 32 : 0) | (pb.cg_factored ? 64 : 0) |
+                      (pb.long_rows ? 128 : 0),
                   pb.dia.active ? 1 : 0, pb.dia.args.ndiag, dia_sig, pb.dia.values, static_cast<const void*>(pb.dia.mask),
                   pb.grid_rows, pb.grid_units);
     std::string key(buf);
@@ -2080,6 +2204,18 @@ int adaptive_solve(const pm_csr* a, const void* rhs, void* x, const void* inv_bl
     }
     pb.fuse_step = fuse_step_wanted(pb);
     pb.single_reduction = single_reduction_wanted(pb);
+    {
+        // few long rows (under kDirectLaunchRows rows, at least kLongRowEntries
+        // entries per row on average): the classic CSR SpMV takes kLongLanes
+        // lanes per row and a grid of its own (PLAMATRIX_AMD_LONG_ROWS=0 keeps
+        // a wavefront per 64-row chunk)
+        const char* env = std::getenv("PLAMATRIX_AMD_LONG_ROWS");
+        pb.long_rows = !pb.dia.active && a->rows > 0 && a->rows < kDirectLaunchRows &&
+                       a->nnz >= kLongRowEntries * a->rows && (env == nullptr || env[0] != '0');
+        if (pb.long_rows)
+            pb.grid_rows = static_cast<int>(
+                std::min<int64_t>(ceil_div(a->rows, static_cast<int64_t>(kDirBlock / kLongLanes)), kDirGrid));
+    }
     // the single-reduction kernels walk 64-row chunks, one per wavefront
     // where the solver grid allows (65,536 rows: 256 workgroups instead of
     // 64); the three-kernel form a restart hands over to uses the same grid
