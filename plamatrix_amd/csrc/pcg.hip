@@ -246,6 +246,36 @@ __global__ __launch_bounds__(kDirBlock) void k_spmv_den_lanes(int64_t rows, cons
     publish_partials<1>(part, partials + kPartialsDen);
 }
 
+// k_residual without the Jacobi inverse, kLongLanes lanes per row (the
+// long-row systems of k_spmv_den_lanes): each lane accumulates its strided
+// entries in FP64, then the lanes combine.
+template <typename S, typename I, bool ZERO_P>
+__global__ __launch_bounds__(kBlock) void k_residual_lanes(int64_t rows, const I* __restrict__ ro,
+                                                           const I* __restrict__ ci, const S* __restrict__ v,
+                                                           const S* __restrict__ b, const S* __restrict__ x,
+                                                           S* __restrict__ r, S* __restrict__ p0, SolverState* st)
+{
+    if (st->done != kRunning) return;
+    constexpr int64_t gpb = kBlock / kLongLanes;
+    const int group = threadIdx.x / kLongLanes;
+    const int lane = threadIdx.x % kLongLanes;
+    int64_t begin, end;
+    block_range(rows, gpb, begin, end);
+    for (int64_t row = begin + group; row < end; row += gpb)
+    {
+        const int64_t s0 = ro[row], e0 = ro[row + 1];
+        double acc = 0.0;
+        for (int64_t k = s0 + lane; k < e0; k += kLongLanes)
+            acc = __builtin_fma(static_cast<double>(v[k]), static_cast<double>(x[ci[k]]), acc);
+        acc = group_sum<kLongLanes>(acc);
+        if (lane == 0)
+        {
+            r[row] = static_cast<S>(static_cast<double>(b[row]) - acc);
+            if (ZERO_P) p0[row] = S(0);
+        }
+    }
+}
+
 // Chunks a wave of a diagonal-layout SpMV keeps in flight: one chunk's loads
 // (mask, value arrays, gathers) do not cover the latency of the 256 MB
 // last-level cache the iteration's working set lives in.
@@ -1599,6 +1629,11 @@ struct Calls
                                     ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
                                     static_cast<const S*>(pb.x), static_cast<S*>(pb.r), static_cast<S*>(pb.inv),
                                     static_cast<S*>(pb.p0), pb.st));
+        else if (pb.long_rows)
+            out.push_back(make_call(k_residual_lanes<S, I, true>, dim3(pb.grid_rows), dim3(kBlock), pb.a->rows,
+                                    ro(pb.a), ci(pb.a), static_cast<const S*>(pb.a->values),
+                                    static_cast<const S*>(pb.rhs), static_cast<const S*>(pb.x),
+                                    static_cast<S*>(pb.r), static_cast<S*>(pb.p0), pb.st));
         else
             out.push_back(make_call(k_residual<S, I, false, true>, dim3(grid), dim3(kBlock), pb.a->rows, ro(pb.a),
                                     ci(pb.a), static_cast<const S*>(pb.a->values), static_cast<const S*>(pb.rhs),
