@@ -586,6 +586,57 @@ __global__ __launch_bounds__(kBlock) void k_ne_cross(int64_t runs, int64_t n, in
     }
 }
 
+// Per residual: its cross block when it is the only residual of its
+// (primary, eliminated) pair, ~0u otherwise.
+__global__ __launch_bounds__(kBlock) void k_residual_cross(int64_t runs, int64_t n,
+                                                           const int64_t* __restrict__ run_start,
+                                                           const unsigned* __restrict__ perm,
+                                                           const unsigned* __restrict__ run_cross,
+                                                           unsigned* __restrict__ res_cross)
+{
+    for (int64_t run = gtid(); run < runs; run += gstride())
+    {
+        const int64_t b = run_start[run], e = run + 1 < runs ? run_start[run + 1] : n;
+        const unsigned c = e - b == 1 ? run_cross[run] : ~0u;
+        for (int64_t k = b; k < e; ++k) res_cross[perm[k]] = c;
+    }
+}
+
+// The cross blocks of single-residual pairs, a thread per residual in
+// residual order: the Jacobian rows are read in order, and cross blocks are
+// numbered in the order residuals first reach them, so the stores follow
+// too.  k_ne_cross_runs' arithmetic for a run of one residual.
+template <int RS, int PS, int ES>
+__global__ __launch_bounds__(kBlock) void k_ne_cross_single(int64_t n, const unsigned* __restrict__ res_cross,
+                                                            const double* __restrict__ jp,
+                                                            const double* __restrict__ je,
+                                                            const double* __restrict__ w, double* __restrict__ W)
+{
+#pragma clang fp contract(off)
+    for (int64_t i = gtid(); i < n; i += gstride())
+    {
+        const unsigned c = res_cross[i];
+        if (c == ~0u) continue;
+        const double wi = w[i];
+        double a[RS * PS], b[RS * ES];
+#pragma unroll
+        for (int j = 0; j < RS * PS; ++j) a[j] = jp[i * RS * PS + j];
+#pragma unroll
+        for (int j = 0; j < RS * ES; ++j) b[j] = je[i * RS * ES + j];
+        double* out = W + static_cast<int64_t>(c) * PS * ES;
+#pragma unroll
+        for (int pcol = 0; pcol < PS; ++pcol)
+#pragma unroll
+            for (int ecol = 0; ecol < ES; ++ecol)
+            {
+                double acc = 0.0;
+#pragma unroll
+                for (int row = 0; row < RS; ++row) acc += wi * a[row * PS + pcol] * b[row * ES + ecol];
+                out[pcol * ES + ecol] = acc;
+            }
+    }
+}
+
 // k_ne_cross with the sizes at compile time and one thread per run: the
 // run's residuals are loaded once for all PS * ES entries of its block (each
 // entry summed in the same order as above, so the same bits).
@@ -596,12 +647,14 @@ __global__ __launch_bounds__(kBlock) void k_ne_cross_runs(int64_t runs, int64_t 
                                                           const unsigned* __restrict__ perm,
                                                           const double* __restrict__ jp,
                                                           const double* __restrict__ je,
-                                                          const double* __restrict__ w, double* __restrict__ W)
+                                                          const double* __restrict__ w, double* __restrict__ W,
+                                                          bool skip_single)
 {
 #pragma clang fp contract(off)
     for (int64_t run = gtid(); run < runs; run += gstride())
     {
         const int64_t end = run + 1 < runs ? run_start[run + 1] : n;
+        if (skip_single && end - run_start[run] == 1) continue;  // k_ne_cross_single's
         double acc[PS * ES];
 #pragma unroll
         for (int en = 0; en < PS * ES; ++en) acc[en] = 0.0;
@@ -1731,6 +1784,7 @@ struct pm_schur_workspace
     unsigned* ne_run_cross = nullptr;
     int64_t* ne_cross_primary = nullptr;
     int64_t* ne_cross_eliminated = nullptr;
+    unsigned* ne_res_cross = nullptr;  // per residual: its cross block if the only residual of its pair, else ~0u
 
     // primary-only residuals (addPrimaryTerms): groupings, weights, primary
     // cross blocks
@@ -2171,6 +2225,7 @@ int residual_pass(pm_schur_workspace* w, const pm_residual_batch* rb, int64_t pc
     tz.take<unsigned>(n);
     tz.take<int64_t>(n);
     tz.take<int64_t>(n);
+    tz.take<unsigned>(n);
     PM_TRY(w->ne_topology.reserve(tz.off));
     Carve t;
     t.base = w->ne_topology.as<char>();
@@ -2185,6 +2240,7 @@ int residual_pass(pm_schur_workspace* w, const pm_residual_batch* rb, int64_t pc
     w->ne_run_cross = t.take<unsigned>(n);
     w->ne_cross_primary = t.take<int64_t>(n);
     w->ne_cross_eliminated = t.take<int64_t>(n);
+    w->ne_res_cross = t.take<unsigned>(n);
     if (n == 0)
     {
         hipLaunchKernelGGL(k_fill_i64, dim3(grid_for(pc + 1)), dim3(kBlock), 0, stream, w->ne_offs_p, pc + 1, 0);
@@ -2223,6 +2279,12 @@ int residual_pass(pm_schur_workspace* w, const pm_residual_batch* rb, int64_t pc
     unsigned runs = 0;
     PM_HIP(hipMemcpyAsync(&runs, sp.total, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
     PM_HIP(hipStreamSynchronize(stream));
+    if (runs > 0)
+        hipLaunchKernelGGL(k_residual_cross, dim3(grid_for(runs)), dim3(kBlock), 0, stream,
+                           static_cast<int64_t>(runs), n, static_cast<const int64_t*>(w->ne_run_start),
+                           static_cast<const unsigned*>(w->ne_perm_pe), static_cast<const unsigned*>(w->ne_run_cross),
+                           w->ne_res_cross);
+    PM_HIP(hipGetLastError());
     w->ne_runs = runs;
     ++w->ne_builds;
     return PM_OK;
@@ -2724,9 +2786,15 @@ int pm_normal_equations_assemble(pm_schur_workspace* w, int64_t pc, int64_t ec, 
                            static_cast<int>(es), w->ne_offs_e, w->ne_perm_e, rb->eliminated_jacobians,
                            rb->residuals, weights, C, ge);
     if (runs > 0 && rs == 2 && ps == 6 && es == 3)
+    {
+        // single-residual pairs in residual order, the others per run
+        hipLaunchKernelGGL((k_ne_cross_single<2, 6, 3>), dim3(grid_for(n)), dim3(kBlock), 0, stream, n,
+                           static_cast<const unsigned*>(w->ne_res_cross), rb->primary_jacobians,
+                           rb->eliminated_jacobians, weights, W);
         hipLaunchKernelGGL((k_ne_cross_runs<2, 6, 3>), dim3(grid_for(runs)), dim3(kBlock), 0, stream, runs, n,
                            w->ne_run_start, w->ne_run_cross, w->ne_perm_pe, rb->primary_jacobians,
-                           rb->eliminated_jacobians, weights, W);
+                           rb->eliminated_jacobians, weights, W, true);
+    }
     else if (runs > 0)
         hipLaunchKernelGGL(k_ne_cross<>, dim3(grid_for(runs * ps * es)), dim3(kBlock), 0, stream, runs, n, rs,
                            static_cast<int>(ps), static_cast<int>(es), w->ne_run_start, w->ne_run_cross,
