@@ -5,7 +5,7 @@
  * CSRMatrix, cooToCsr, spmv, cg / pcg / blockPcg, the fixed-iteration async
  * solves and the damped Schur-complement solve).  The C++ headers in
  * include/plamatrix/ implement those templates on top of these entry points;
- * any other host language binds the same symbols.
+ * any other host language (ctypes, cgo, JNI) binds the same symbols.
  *
  * Conventions
  *   - All array arguments are device pointers unless stated otherwise.
